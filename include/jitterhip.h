@@ -1,0 +1,220 @@
+/*
+ * jitterhip — C ABI of the MI355X (gfx950) implementation of Jitter2's per-step hot path:
+ * broadphase pair finding -> MPR/EPA narrowphase with one-shot manifolds -> contact cache ->
+ * substepped projected Gauss-Seidel (sequential impulse) solver -> integration.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns a jh_status
+ * (0 = ok, negative = error; jh_last_error() gives a thread-local message).  No exceptions
+ * cross this boundary.  A world handle is not thread-safe; distinct handles may be stepped
+ * from distinct host threads (each handle owns one HIP stream).  Device memory is owned by the
+ * handle and released by jh_world_destroy.  All arrays passed in are copied before return;
+ * all arrays passed out are caller-allocated.
+ *
+ * Reference interfaces replaced (paths relative to the Jitter2 source tree):
+ *   World.Step(dt, multiThread)              src/Jitter2/World.Step.cs:141-151, StepInternal :153-296
+ *   World.CreateRigidBody / RigidBody.AddShape src/Jitter2/World.cs:943-956, Dynamics/RigidBody.cs:1163-1209
+ *   RigidBodyData (body state)               src/Jitter2/Dynamics/RigidBody.cs:107-250
+ *   ContactData / Contact (contact cache)    src/Jitter2/Dynamics/Contact.cs:30-118, 617-661
+ *   World.RawData spans                      src/Jitter2/World.cs:95-141
+ *   DynamicTree.EnumerateOverlaps (pairs)    src/Jitter2/Collision/DynamicTree/DynamicTree.cs:266-324
+ *   ISupportMappable shapes                  src/Jitter2/Collision/NarrowPhase/ISupportMappable.cs:19-33
+ */
+#ifndef JITTERHIP_H
+#define JITTERHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t jh_status;
+
+enum {
+    JH_OK = 0,
+    JH_ERR_INVALID_ARG = -1,  /* ArgumentException in World.Step.cs:155-165 and property setters   */
+    JH_ERR_NON_FINITE = -2,   /* ArgumentCheck.Finite(dt) in World.Step.cs:157                       */
+    JH_ERR_CAPACITY = -3,     /* MaximumSizeException, Unmanaged/PartitionedBuffer.cs:134-139         */
+    JH_ERR_HIP = -4,          /* a HIP runtime call failed                                            */
+    JH_ERR_UNSUPPORTED = -5,  /* a feature outside the GPU path (motion locks, gyroscopic forces ...)  */
+    JH_ERR_STATE = -6,        /* ObjectDisposedException / call order errors                          */
+    JH_ERR_NO_DEVICE = -7     /* no HIP device visible                                                */
+};
+
+/* MotionType, Dynamics/RigidBody.cs:75-95 (bits 0-1 of RigidBodyData.Flags). */
+enum { JH_MOTION_DYNAMIC = 0, JH_MOTION_KINEMATIC = 1, JH_MOTION_STATIC = 2 };
+
+/* Shape kinds of the hot path (files under Collision/Shapes). */
+enum {
+    JH_SHAPE_BOX = 0,        /* BoxShape.cs: params[0..2] = half extents                                  */
+    JH_SHAPE_SPHERE = 1,     /* SphereShape.cs: params[0] = radius                                         */
+    JH_SHAPE_CAPSULE = 2,    /* CapsuleShape.cs: params[0] = radius, params[1] = half length              */
+    JH_SHAPE_POINTCLOUD = 3, /* PointCloudShape.cs / VertexSupportMap.cs: params[0..2] = center (GetCenter),
+                                params[3..5] = shift, params[6..8] = cached local box min,
+                                params[9..11] = cached local box max; vertices[vertex_offset ..]           */
+    JH_SHAPE_TRIANGLE = 4    /* TriangleShape.cs (static meshes): vertices[vertex_offset .. +3]          */
+};
+
+/* ContactData.SolveMode bits, Dynamics/Contact.cs:36-58. */
+enum { JH_SOLVE_LINEAR1 = 1, JH_SOLVE_ANGULAR1 = 2, JH_SOLVE_LINEAR2 = 4, JH_SOLVE_ANGULAR2 = 8 };
+
+/* World-level settings; names and defaults follow World.cs:306-412 and World.Detect.cs:88-122. */
+typedef struct jh_config {
+    float gravity[3];                    /* World.Gravity, default (0, -9.81, 0)                 */
+    int32_t substep_count;               /* World.SubstepCount, default 1                        */
+    int32_t solver_iterations;           /* World.SolverIterations.solver, default 6             */
+    int32_t relaxation_iterations;       /* World.SolverIterations.relaxation, default 4         */
+    float speculative_relaxation_factor; /* World.SpeculativeRelaxationFactor, default 0.9       */
+    int32_t enable_auxiliary_contact_points; /* World.EnableAuxiliaryContactPoints, default 1    */
+    int32_t persistent_contact_manifold; /* World.PersistentContactManifold, default 1           */
+    int32_t pair_capacity;               /* broadphase pair buffer size; 0 = 24 per shape         */
+    int32_t reserved[6];
+} jh_config;
+
+/* One rigid body.  Position/orientation/velocities and the delta velocities are the solver state
+ * of RigidBodyData (RigidBody.cs:107-250); the rest are the managed RigidBody fields that
+ * RigidBody.Update (RigidBody.cs:716-764) reads.  InverseInertiaWorld and InverseMassVector are
+ * derived on upload from orientation, inverse_inertia and inverse_mass (UpdateWorldInertia,
+ * RigidBody.cs:903-923). */
+typedef struct jh_body {
+    float position[3];
+    float orientation[4];            /* quaternion x, y, z, w                                      */
+    float velocity[3];
+    float angular_velocity[3];
+    float delta_velocity[3];         /* RigidBodyData.DeltaVelocity, applied each substep          */
+    float delta_angular_velocity[3];
+    float inverse_inertia[6];        /* body frame M11 M12 M13 M22 M23 M33                         */
+    float inverse_mass;
+    float friction;                  /* RigidBody.Friction, default 0.2                            */
+    float restitution;               /* RigidBody.Restitution, default 0                           */
+    float linear_damping_multiplier; /* 1 - RigidBody.Damping.linear, default 0.998                */
+    float angular_damping_multiplier;/* 1 - RigidBody.Damping.angular, default 0.995               */
+    float force[3];                  /* RigidBody.Force, cleared by each step                      */
+    float torque[3];
+    int32_t motion_type;             /* JH_MOTION_*                                                 */
+    int32_t affected_by_gravity;     /* RigidBody.AffectedByGravity, default 1                     */
+    int32_t world_id;                /* independent sub-world (batched worlds); 0 for one world    */
+    int32_t reserved;
+} jh_body;
+
+/* One shape attached to a body.  shape_id is Shape.ShapeId (Collision/Shapes/Shape.cs:28); shapes
+ * must be passed in strictly ascending shape_id order so that array order equals key order. */
+typedef struct jh_shape {
+    uint64_t shape_id;
+    int32_t body;            /* index into the body array                                          */
+    int32_t type;            /* JH_SHAPE_*                                                          */
+    float params[12];
+    int32_t vertex_offset;   /* point clouds / triangles: first vertex in the vertex array         */
+    int32_t vertex_count;
+} jh_shape;
+
+/* One contact slot, field for field the Contact struct (Contact.cs:617-661). */
+typedef struct jh_contact {
+    uint32_t flag;                 /* Contact.Flags: 2 = NewContact, 4 = SkipWarmStart   */
+    float bias;
+    float penalty_bias;
+    float normal[3];
+    float tangent1[3];
+    float tangent2[3];
+    float mass_normal_tangent[3];
+    float accumulated[3];          /* normal, tangent1, tangent2 accumulated impulse    */
+    float position1[3];            /* body-1 local anchor                               */
+    float position2[3];            /* body-2 local anchor                               */
+    float relative_position1[3];   /* world offset from body 1                          */
+    float relative_position2[3];   /* world offset from body 2                          */
+} jh_contact;
+
+/* One arbiter (ContactData, Contact.cs:30-118) keyed by the ordered shape pair (Arbiter.cs:67-103). */
+typedef struct jh_arbiter {
+    uint32_t shape1;       /* index of the shape with the smaller shape_id (ArbiterKey.Key1) */
+    uint32_t shape2;       /* ArbiterKey.Key2                                                */
+    int32_t body1;
+    int32_t body2;
+    uint32_t usage_mask;   /* low nibble = intact slots                                      */
+    uint32_t mode;         /* JH_SOLVE_* bits                                                */
+    float friction;        /* max of the two bodies (Contact.cs:276)                         */
+    float restitution;     /* max of the two bodies (Contact.cs:277)                         */
+    float speculative_relaxation_factor;
+    uint32_t reserved;
+    jh_contact contacts[4];
+} jh_arbiter;
+
+/* Narrowphase result of one colliding broadphase pair in the last step (World.Detect.cs:165-217). */
+typedef struct jh_manifold {
+    uint32_t shape1, shape2;
+    int32_t count;          /* manifold points (CollisionManifold.Count), 1..4             */
+    int32_t flags;          /* bit 0: EPA refined the MPR answer                           */
+    float normal[3];
+    float penetration;
+    float point_a[3];       /* MPR/EPA deepest point on shape 1                            */
+    float point_b[3];
+    float manifold_a[4][3];
+    float manifold_b[4][3];
+} jh_manifold;
+
+/* Per-step counters and stage timings (World.DebugTimings buckets, World.Step.cs:44-81). */
+typedef struct jh_stats {
+    int64_t bodies, shapes, pairs, colliding_pairs, arbiters, contacts, colours, epa_calls;
+    int64_t steps;
+    int64_t pair_overflow;
+    double ms_broadphase, ms_narrowphase, ms_contact_cache, ms_colouring, ms_solve, ms_update;
+    double ms_step;
+} jh_stats;
+
+typedef struct jh_world jh_world;
+
+/* Error text of the last failing call on this thread (never NULL). */
+const char* jh_last_error(void);
+/* ABI version: 1. */
+int32_t jh_version(void);
+
+/* World lifetime (World ctor World.cs:423-441, Dispose World.cs:977-990).  config may be NULL for
+ * the reference defaults. */
+jh_status jh_world_create(const jh_config* config, jh_world** out_world);
+jh_status jh_world_destroy(jh_world* world);
+jh_status jh_world_set_config(jh_world* world, const jh_config* config);
+jh_status jh_config_defaults(jh_config* out_config);
+
+/* Scene upload/download.  set_bodies replaces all bodies; set_shapes replaces all shapes and their
+ * vertex data (vertices = 3 floats each) and clears the contact cache when the shape set changes. */
+jh_status jh_world_set_bodies(jh_world* world, const jh_body* bodies, int32_t count);
+jh_status jh_world_get_bodies(jh_world* world, jh_body* bodies, int32_t count);
+jh_status jh_world_set_shapes(jh_world* world, const jh_shape* shapes, int32_t count,
+                              const float* vertices, int32_t vertex_count);
+
+/* The reference's 128-byte RigidBodyData layout (FP32), for zero-copy exchange with World.RawData
+ * (World.cs:95-141).  stride = bytes between records (>= 128). */
+jh_status jh_world_get_rigidbodydata(jh_world* world, void* records, int32_t count, int32_t stride);
+
+/* World.Step(dt): one full step (narrowphase over the pair set, contact cache, substepped
+ * solve, contact refresh, body update, broadphase).  Returns after the step finished on the GPU. */
+jh_status jh_world_step(jh_world* world, float dt);
+
+/* Broadphase only, on the current state: the pair set {(Key1, Key2)} that
+ * DynamicTree.EnumerateOverlaps + World.Detect gating would visit, sorted by key.
+ * pairs holds 2 * capacity uint32 (shape indices).  *out_count is the full count even when it
+ * exceeds capacity (then JH_ERR_CAPACITY is returned). */
+jh_status jh_world_compute_pairs(jh_world* world, uint32_t* pairs, int32_t capacity, int32_t* out_count);
+
+/* Narrowphase results of the last step, sorted by key. */
+jh_status jh_world_get_manifolds(jh_world* world, jh_manifold* manifolds, int32_t capacity, int32_t* out_count);
+
+/* Contact cache (arbiters sorted by key) after the last step, and restore (checkpoint/resume). */
+jh_status jh_world_get_arbiters(jh_world* world, jh_arbiter* arbiters, int32_t capacity, int32_t* out_count);
+jh_status jh_world_set_arbiters(jh_world* world, const jh_arbiter* arbiters, int32_t count);
+
+/* Solve order of the last step: arbiter keys (2 uint32 each) in the order the solver visited them,
+ * grouped into batches of body-disjoint arbiters (graph colours); batch_offsets has batch_count+1
+ * entries.  Replaying the keys sequentially on a CPU reproduces the GPU's Gauss-Seidel order. */
+jh_status jh_world_get_solve_order(jh_world* world, uint32_t* keys, int32_t key_capacity,
+                                   int32_t* batch_offsets, int32_t batch_capacity,
+                                   int32_t* out_key_count, int32_t* out_batch_count);
+
+jh_status jh_world_get_stats(jh_world* world, jh_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JITTERHIP_H */

@@ -1,0 +1,292 @@
+"""Seeded scene builders for the workloads of this path, producing jh_body / jh_shape arrays.
+
+Mass properties follow the reference's shapes and RigidBody.SetMassInertia in FP32:
+BoxShape.CalculateMassInertia (src/Jitter2/Collision/Shapes/BoxShape.cs:188-198),
+SphereShape (SphereShape.cs:98-111), CapsuleShape (CapsuleShape.cs:109-129),
+JSymmetricMatrix.Inverse (src/Jitter2/LinearMath/JSymmetricMatrix.cs:119-178),
+RigidBody.SetMassInertia (src/Jitter2/Dynamics/RigidBody.cs:1575-1608).
+Body defaults are those of a new RigidBody (RigidBody.cs:265-266, 398-409).
+
+Random streams use SplitMix64 so that every scene is reproducible from its seed.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._abi import (BODY_DTYPE, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE,
+                   SHAPE_POINTCLOUD, SHAPE_SPHERE, default_config)
+
+f32 = np.float32
+
+
+class SplitMix64:
+    """SplitMix64 generator; uniform() returns doubles in [0, 1) from the top 53 bits."""
+
+    def __init__(self, seed: int):
+        self.state = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, a: float = 0.0, b: float = 1.0) -> float:
+        return a + (b - a) * ((self.next_u64() >> 11) * (1.0 / 9007199254740992.0))
+
+    def normal(self) -> float:
+        u1 = max(self.uniform(), 1e-300)
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+    def quaternion(self) -> tuple[float, float, float, float]:
+        """Uniform unit quaternion (Shoemake), as x, y, z, w."""
+        u1, u2, u3 = self.uniform(), self.uniform(), self.uniform()
+        a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+        return (a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2),
+                b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3))
+
+
+def _sym_inverse(m11, m12, m13, m22, m23, m33):
+    """JSymmetricMatrix.TryInverseUnscaled in FP32 (JSymmetricMatrix.cs:157-178)."""
+    m11, m12, m13, m22, m23, m33 = (f32(v) for v in (m11, m12, m13, m22, m23, m33))
+    det = m11 * (m22 * m33 - m23 * m23) + m12 * (m23 * m13 - m12 * m33) + m13 * (m12 * m23 - m22 * m13)
+    idet = f32(1.0) / f32(det)
+    a11 = m22 * m33 - m23 * m23
+    a12 = m13 * m23 - m12 * m33
+    a13 = m12 * m23 - m22 * m13
+    a22 = m11 * m33 - m13 * m13
+    a23 = m13 * m12 - m23 * m11
+    a33 = m11 * m22 - m12 * m12
+    return [f32(a11 * idet), f32(a12 * idet), f32(a13 * idet), f32(a22 * idet), f32(a23 * idet), f32(a33 * idet)]
+
+
+def box_mass(size):
+    """BoxShape.CalculateMassInertia (BoxShape.cs:188-198) for full extents `size`."""
+    sx, sy, sz = (f32(v) for v in size)
+    mass = sx * sy * sz
+    k = f32(1.0 / 12.0)
+    i11 = k * mass * (sy * sy + sz * sz)
+    i22 = k * mass * (sx * sx + sz * sz)
+    i33 = k * mass * (sx * sx + sy * sy)
+    return f32(mass), (i11, f32(0), f32(0), i22, f32(0), i33)
+
+
+def sphere_mass(r):
+    """SphereShape.CalculateMassInertia."""
+    r = f32(r)
+    mass = f32(4.0 / 3.0) * f32(math.pi) * r * r * r
+    i = f32(2.0 / 5.0) * mass * r * r
+    return f32(mass), (i, f32(0), f32(0), i, f32(0), i)
+
+
+def capsule_mass(r, length):
+    """CapsuleShape.CalculateMassInertia."""
+    r, length = f32(r), f32(length)
+    pi = f32(math.pi)
+    ms = f32(4.0 / 3.0) * pi * r * r * r
+    mc = pi * r * r * length
+    i11 = mc * (f32(1.0 / 12.0) * length * length + f32(1.0 / 4.0) * r * r) + ms * (
+        f32(2.0 / 5.0) * r * r + f32(1.0 / 4.0) * length * length + f32(3.0 / 8.0) * length * r)
+    i22 = f32(1.0 / 2.0) * mc * r * r + f32(2.0 / 5.0) * ms * r * r
+    return f32(mc + ms), (i11, f32(0), f32(0), i22, f32(0), i11)
+
+
+class SceneBuilder:
+    """Accumulates bodies and shapes in creation order (shape ids ascend with creation)."""
+
+    def __init__(self):
+        self.bodies: list[np.void] = []
+        self.shapes: list[np.void] = []
+        self.vertices: list[float] = []
+        self._next_id = 1
+
+    def body(self, position, orientation=(0.0, 0.0, 0.0, 1.0), velocity=(0, 0, 0), angular_velocity=(0, 0, 0),
+             motion=MOTION_DYNAMIC, friction=0.2, restitution=0.0, world=0) -> int:
+        b = np.zeros(1, BODY_DTYPE)[0]
+        b["position"] = position
+        b["orientation"] = orientation
+        b["velocity"] = velocity
+        b["angular_velocity"] = angular_velocity
+        b["inverse_inertia"] = (1, 0, 0, 1, 0, 1)
+        b["inverse_mass"] = 1.0
+        b["friction"] = friction
+        b["restitution"] = restitution
+        b["linear_damping_multiplier"] = f32(0.998)
+        b["angular_damping_multiplier"] = f32(0.995)
+        b["motion_type"] = motion
+        b["affected_by_gravity"] = 1
+        b["world_id"] = world
+        self.bodies.append(b)
+        return len(self.bodies) - 1
+
+    def _shape(self, body: int, kind: int, params, mass_inertia, verts=None) -> int:
+        s = np.zeros(1, SHAPE_DTYPE)[0]
+        s["shape_id"] = self._next_id
+        self._next_id += 1
+        s["body"] = body
+        s["type"] = kind
+        p = np.zeros(12, np.float32)
+        p[:len(params)] = params
+        s["params"] = p
+        if verts is not None:
+            s["vertex_offset"] = len(self.vertices) // 3
+            s["vertex_count"] = len(verts)
+            self.vertices.extend(np.asarray(verts, np.float32).ravel().tolist())
+        self.shapes.append(s)
+        # RigidBody.SetMassInertia with a single shape (RigidBody.cs:1575-1608)
+        mass, inertia = mass_inertia
+        b = self.bodies[body]
+        b["inverse_inertia"] = _sym_inverse(*inertia)
+        b["inverse_mass"] = f32(1.0) / f32(mass)
+        return len(self.shapes) - 1
+
+    def box(self, body: int, size) -> int:
+        half = [f32(0.5) * f32(v) for v in size]
+        return self._shape(body, SHAPE_BOX, half, box_mass(size))
+
+    def sphere(self, body: int, radius) -> int:
+        return self._shape(body, SHAPE_SPHERE, [radius], sphere_mass(radius))
+
+    def capsule(self, body: int, radius, length) -> int:
+        return self._shape(body, SHAPE_CAPSULE, [radius, f32(0.5) * f32(length)], capsule_mass(radius, length))
+
+    def arrays(self):
+        bodies = np.array(self.bodies, dtype=BODY_DTYPE) if self.bodies else np.zeros(0, BODY_DTYPE)
+        shapes = np.array(self.shapes, dtype=SHAPE_DTYPE) if self.shapes else np.zeros(0, SHAPE_DTYPE)
+        verts = np.array(self.vertices, dtype=np.float32) if self.vertices else None
+        return bodies, shapes, verts
+
+
+def pyramid(rows: int = 20, seed: int = 0x4A2D0001, jitter: float = 0.01):
+    """20-row pyramid of unit boxes on a static 200x1x200 ground box (mass 1, friction 0.6,
+    restitution 0); layout of Helper.BuildPyramidBox (src/JitterTests/Helper.cs:63-79), all rows
+    dynamic, lateral jitter U(-jitter, jitter).  Returns (bodies, shapes, verts, config, dt)."""
+    rng = SplitMix64(seed)
+    sb = SceneBuilder()
+    g = sb.body((0.0, -0.5, 0.0), motion=MOTION_STATIC, friction=0.6)
+    sb.box(g, (200.0, 1.0, 200.0))
+    sb.bodies[g]["inverse_mass"] = 0.0
+    sb.bodies[g]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    for i in range(rows):
+        for e in range(i, rows):
+            x = (e - i * 0.5) * 1.01 + rng.uniform(-jitter, jitter)
+            z = rng.uniform(-jitter, jitter)
+            b = sb.body((x, 0.5 + i, z), friction=0.6)
+            sb.box(b, (1.0, 1.0, 1.0))
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    cfg["solver_iterations"] = 8
+    cfg["relaxation_iterations"] = 4
+    return (*sb.arrays(), cfg, 1.0 / 100.0)
+
+
+def _static_box(sb: SceneBuilder, pos, size, world=0, friction=0.2):
+    b = sb.body(pos, motion=MOTION_STATIC, friction=friction, world=world)
+    sb.box(b, size)
+    sb.bodies[b]["inverse_mass"] = 0.0
+    sb.bodies[b]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    return b
+
+
+def _random_shape(sb: SceneBuilder, rng: SplitMix64, body: int, kinds):
+    k = kinds[int(rng.uniform() * len(kinds)) % len(kinds)]
+    if k == SHAPE_SPHERE:
+        sb.sphere(body, rng.uniform(0.3, 0.7))
+    elif k == SHAPE_BOX:
+        sb.box(body, [2.0 * rng.uniform(0.2, 0.8) for _ in range(3)])
+    else:
+        sb.capsule(body, rng.uniform(0.2, 0.4), rng.uniform(0.5, 1.5))
+
+
+def heap(n_bodies: int = 32768, seed: int = 0x4A2D0002, interior: float = 60.0, height: float = 200.0,
+         pitch: float = 2.7, kinds=(SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE)):
+    """Dynamic bodies dropped into a static box container (floor + 4 walls, interior
+    `interior` x `interior`, height `height`, wall thickness 1): shape uniform over sphere
+    r~U(0.3,0.7), box half extents ~U(0.2,0.8), capsule r~U(0.2,0.4) length~U(0.5,1.5); positions on a
+    jittered grid (pitch 2.7 m, jitter U(-0.05,0.05), bottom layer at y=1.5), uniform random
+    orientations, velocities N(0,1).  dt = 1/60, 4 substeps, 10 iterations.  The container widens
+    with n so the grid keeps at most 68 layers for larger heaps."""
+    rng = SplitMix64(seed)
+    sb = SceneBuilder()
+    per_row = max(22, int(math.ceil(math.sqrt(n_bodies / 68.0))))
+    interior = max(interior, per_row * pitch + 0.6)
+    h = interior / 2.0
+    _static_box(sb, (0.0, -0.5, 0.0), (interior + 2.0, 1.0, interior + 2.0))
+    _static_box(sb, (-(h + 0.5), height / 2.0, 0.0), (1.0, height, interior + 2.0))
+    _static_box(sb, ((h + 0.5), height / 2.0, 0.0), (1.0, height, interior + 2.0))
+    _static_box(sb, (0.0, height / 2.0, -(h + 0.5)), (interior, height, 1.0))
+    _static_box(sb, (0.0, height / 2.0, (h + 0.5)), (interior, height, 1.0))
+    x0 = -(per_row - 1) * pitch / 2.0
+    count = 0
+    layer = 0
+    while count < n_bodies:
+        for iz in range(per_row):
+            for ix in range(per_row):
+                if count >= n_bodies:
+                    break
+                pos = (x0 + ix * pitch + rng.uniform(-0.05, 0.05), 1.5 + layer * pitch + rng.uniform(-0.05, 0.05),
+                       x0 + iz * pitch + rng.uniform(-0.05, 0.05))
+                q = rng.quaternion()
+                v = (rng.normal(), rng.normal(), rng.normal())
+                b = sb.body(pos, orientation=q, velocity=v)
+                _random_shape(sb, rng, b, kinds)
+                count += 1
+        layer += 1
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    cfg["solver_iterations"] = 10
+    cfg["relaxation_iterations"] = 4
+    return (*sb.arrays(), cfg, 1.0 / 60.0)
+
+
+def batched_worlds(n_worlds: int = 4096, seed: int = 0x4A2D0003, min_bodies: int = 16, max_bodies: int = 2048,
+                   spacing: float = 0.0):
+    """Independent worlds with log-uniform body counts in [min_bodies, max_bodies]; each world is a
+    random heap of boxes and spheres over its own static ground box.  Worlds share coordinates
+    (they never interact: bodies carry a world id).  dt = 1/60, 4 substeps, 10 iterations."""
+    rng = SplitMix64(seed)
+    sb = SceneBuilder()
+    pitch = 1.9
+    for w in range(n_worlds):
+        n = int(math.exp(rng.uniform(math.log(min_bodies), math.log(max_bodies))))
+        side = max(4, int(math.ceil(math.sqrt(n / 8.0))))
+        ox = w * spacing
+        _static_box(sb, (ox, -0.5, 0.0), (side * pitch + 20.0, 1.0, side * pitch + 20.0), world=w)
+        x0 = ox - (side - 1) * pitch / 2.0
+        for i in range(n):
+            layer, rem = divmod(i, side * side)
+            iz, ix = divmod(rem, side)
+            pos = (x0 + ix * pitch + rng.uniform(-0.05, 0.05), 1.0 + layer * pitch,
+                   -(side - 1) * pitch / 2.0 + iz * pitch + rng.uniform(-0.05, 0.05))
+            b = sb.body(pos, orientation=rng.quaternion(), velocity=(rng.normal(), rng.normal(), rng.normal()),
+                        world=w)
+            if rng.uniform() < 0.5:
+                sb.sphere(b, rng.uniform(0.3, 0.7))
+            else:
+                sb.box(b, [2.0 * rng.uniform(0.2, 0.8) for _ in range(3)])
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    cfg["solver_iterations"] = 10
+    cfg["relaxation_iterations"] = 4
+    return (*sb.arrays(), cfg, 1.0 / 60.0)
+
+
+def two_bodies(shape_a, shape_b, pos_a, pos_b, q_a=(0, 0, 0, 1), q_b=(0, 0, 0, 1), static_a=False):
+    """Tiny two-body scene for unit tests; shape_x = ("box", size) | ("sphere", r) | ("capsule", r, len)."""
+    sb = SceneBuilder()
+    for spec, pos, q, static in ((shape_a, pos_a, q_a, static_a), (shape_b, pos_b, q_b, False)):
+        b = sb.body(pos, orientation=q, motion=MOTION_STATIC if static else MOTION_DYNAMIC)
+        if spec[0] == "box":
+            sb.box(b, spec[1])
+        elif spec[0] == "sphere":
+            sb.sphere(b, spec[1])
+        else:
+            sb.capsule(b, spec[1], spec[2])
+        if static:
+            sb.bodies[b]["inverse_mass"] = 0.0
+            sb.bodies[b]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    return sb.arrays()

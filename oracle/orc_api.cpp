@@ -1,0 +1,407 @@
+// ORACLE — test infrastructure only.  C entry points of the CPU restatement of Jitter2's step
+// (liborc.so), called from tests/ through ctypes, from __graft_entry__.smoke() as the checker,
+// and from bench.py's cpu_baseline leg.  fp64 != 0 selects the double instantiation
+// (≙ Jitter2.Double).  Never part of the product path.
+#include <chrono>
+#include <cstdio>
+#include <memory>
+
+#include "orc_world.hpp"
+
+using namespace orc;
+
+namespace {
+struct Handle {
+    bool fp64;
+    World<float>* wf = nullptr;
+    World<double>* wd = nullptr;
+    std::vector<float> verts;
+};
+
+template <class R> void set_config(World<R>& w, const jh_config* c) {
+    w.gravity = {R(c->gravity[0]), R(c->gravity[1]), R(c->gravity[2])};
+    w.substeps = c->substep_count;
+    w.iterations = c->solver_iterations;
+    w.relaxations = c->relaxation_iterations;
+    w.specrel = R(c->speculative_relaxation_factor);
+    w.aux_contacts = c->enable_auxiliary_contact_points != 0;
+    w.persistent = c->persistent_contact_manifold != 0;
+}
+
+template <class R> V3<R> vin(const float* f) { return {R(f[0]), R(f[1]), R(f[2])}; }
+template <class R> void vout(V3<R> v, float* f) { f[0] = float(v.x); f[1] = float(v.y); f[2] = float(v.z); }
+
+template <class R> void set_bodies(World<R>& w, const jh_body* b, int n) {
+    w.bodies.resize(n);
+    for (int i = 0; i < n; ++i) {
+        Body<R>& o = w.bodies[i];
+        o.pos = vin<R>(b[i].position);
+        o.q = {R(b[i].orientation[0]), R(b[i].orientation[1]), R(b[i].orientation[2]), R(b[i].orientation[3])};
+        o.vel = vin<R>(b[i].velocity);
+        o.angvel = vin<R>(b[i].angular_velocity);
+        o.dvel = vin<R>(b[i].delta_velocity);
+        o.dangvel = vin<R>(b[i].delta_angular_velocity);
+        const float* I = b[i].inverse_inertia;
+        o.inv_inertia = {R(I[0]), R(I[1]), R(I[2]), R(I[3]), R(I[4]), R(I[5])};
+        o.inv_mass = R(b[i].inverse_mass);
+        o.friction = R(b[i].friction);
+        o.restitution = R(b[i].restitution);
+        o.lin_damp = R(b[i].linear_damping_multiplier);
+        o.ang_damp = R(b[i].angular_damping_multiplier);
+        o.force = vin<R>(b[i].force);
+        o.torque = vin<R>(b[i].torque);
+        o.motion = b[i].motion_type;
+        o.gravity = b[i].affected_by_gravity != 0;
+        o.world = b[i].world_id;
+        o.update_world_inertia();
+    }
+}
+
+template <class R> void get_bodies(const World<R>& w, jh_body* b, int n) {
+    for (int i = 0; i < n && i < (int)w.bodies.size(); ++i) {
+        const Body<R>& o = w.bodies[i];
+        vout(o.pos, b[i].position);
+        b[i].orientation[0] = float(o.q.x);
+        b[i].orientation[1] = float(o.q.y);
+        b[i].orientation[2] = float(o.q.z);
+        b[i].orientation[3] = float(o.q.w);
+        vout(o.vel, b[i].velocity);
+        vout(o.angvel, b[i].angular_velocity);
+        vout(o.dvel, b[i].delta_velocity);
+        vout(o.dangvel, b[i].delta_angular_velocity);
+        vout(o.force, b[i].force);
+        vout(o.torque, b[i].torque);
+    }
+}
+
+template <class R> void get_state64(const World<R>& w, double* out, int n) {
+    for (int i = 0; i < n && i < (int)w.bodies.size(); ++i) {
+        const Body<R>& o = w.bodies[i];
+        double* d = out + 13 * i;
+        d[0] = o.pos.x; d[1] = o.pos.y; d[2] = o.pos.z;
+        d[3] = o.q.x; d[4] = o.q.y; d[5] = o.q.z; d[6] = o.q.w;
+        d[7] = o.vel.x; d[8] = o.vel.y; d[9] = o.vel.z;
+        d[10] = o.angvel.x; d[11] = o.angvel.y; d[12] = o.angvel.z;
+    }
+}
+
+template <class R> void set_shapes(World<R>& w, const jh_shape* s, int n, const float* verts) {
+    w.shapes.resize(n);
+    for (int i = 0; i < n; ++i) {
+        Shape<R>& o = w.shapes[i];
+        o.type = s[i].type;
+        o.body = s[i].body;
+        o.id = s[i].shape_id;
+        for (int k = 0; k < 12; ++k) o.p[k] = R(s[i].params[k]);
+        o.verts = verts ? verts + 3 * s[i].vertex_offset : nullptr;
+        o.nverts = s[i].vertex_count;
+    }
+    w.arbiters.clear();
+}
+
+template <class R> void contact_out(const Contact<R>& c, jh_contact& o) {
+    o.flag = c.flag;
+    o.bias = float(c.bias);
+    o.penalty_bias = float(c.penalty);
+    vout(c.n, o.normal);
+    vout(c.t1, o.tangent1);
+    vout(c.t2, o.tangent2);
+    for (int k = 0; k < 3; ++k) { o.mass_normal_tangent[k] = float(c.mass[k]); o.accumulated[k] = float(c.acc[k]); }
+    vout(c.pos1, o.position1);
+    vout(c.pos2, o.position2);
+    vout(c.rel1, o.relative_position1);
+    vout(c.rel2, o.relative_position2);
+}
+template <class R> void contact_in(Contact<R>& c, const jh_contact& o) {
+    c.flag = o.flag;
+    c.bias = R(o.bias);
+    c.penalty = R(o.penalty_bias);
+    c.n = vin<R>(o.normal);
+    c.t1 = vin<R>(o.tangent1);
+    c.t2 = vin<R>(o.tangent2);
+    for (int k = 0; k < 3; ++k) { c.mass[k] = R(o.mass_normal_tangent[k]); c.acc[k] = R(o.accumulated[k]); }
+    c.pos1 = vin<R>(o.position1);
+    c.pos2 = vin<R>(o.position2);
+    c.rel1 = vin<R>(o.relative_position1);
+    c.rel2 = vin<R>(o.relative_position2);
+}
+
+template <class R> int get_arbiters(const World<R>& w, jh_arbiter* out, int cap) {
+    int n = 0;
+    for (auto& kv : w.arbiters) {
+        if (n < cap && out) {
+            const Arbiter<R>& a = kv.second;
+            jh_arbiter& o = out[n];
+            o.shape1 = a.key1;
+            o.shape2 = a.key2;
+            o.body1 = a.b1;
+            o.body2 = a.b2;
+            o.usage_mask = a.usage;
+            o.mode = a.mode;
+            o.friction = float(a.friction);
+            o.restitution = float(a.restitution);
+            o.speculative_relaxation_factor = float(a.specrel);
+            o.reserved = 0;
+            for (int i = 0; i < 4; ++i) contact_out(a.c[i], o.contacts[i]);
+        }
+        n++;
+    }
+    return n;
+}
+template <class R> void set_arbiters(World<R>& w, const jh_arbiter* in, int n) {
+    w.arbiters.clear();
+    for (int i = 0; i < n; ++i) {
+        Arbiter<R> a;
+        a.key1 = in[i].shape1;
+        a.key2 = in[i].shape2;
+        a.b1 = in[i].body1;
+        a.b2 = in[i].body2;
+        a.usage = in[i].usage_mask;
+        a.mode = in[i].mode;
+        a.friction = R(in[i].friction);
+        a.restitution = R(in[i].restitution);
+        a.specrel = R(in[i].speculative_relaxation_factor);
+        for (int k = 0; k < 4; ++k) contact_in(a.c[k], in[i].contacts[k]);
+        w.arbiters[std::make_pair(a.key1, a.key2)] = a;
+    }
+}
+
+template <class R> int get_manifolds(const World<R>& w, jh_manifold* out, int cap) {
+    int n = 0;
+    for (auto& m : w.last_manifolds) {
+        if (n < cap && out) {
+            jh_manifold& o = out[n];
+            o.shape1 = m.k1;
+            o.shape2 = m.k2;
+            o.count = m.count;
+            o.flags = m.epa ? 1 : 0;
+            vout(m.normal, o.normal);
+            o.penetration = float(m.pen);
+            vout(m.pa, o.point_a);
+            vout(m.pb, o.point_b);
+            for (int i = 0; i < 4; ++i) {
+                if (i < m.count) { vout(m.mA[i], o.manifold_a[i]); vout(m.mB[i], o.manifold_b[i]); }
+                else { for (int k = 0; k < 3; ++k) { o.manifold_a[i][k] = 0; o.manifold_b[i][k] = 0; } }
+            }
+        }
+        n++;
+    }
+    return n;
+}
+
+Shape<float> shape_f(const jh_shape& s, const float* verts) {
+    Shape<float> o;
+    o.type = s.type;
+    o.body = s.body;
+    for (int k = 0; k < 12; ++k) o.p[k] = s.params[k];
+    o.verts = verts ? verts + 3 * s.vertex_offset : nullptr;
+    o.nverts = s.vertex_count;
+    return o;
+}
+Shape<double> shape_d(const jh_shape& s, const float* verts) {
+    Shape<double> o;
+    o.type = s.type;
+    o.body = s.body;
+    for (int k = 0; k < 12; ++k) o.p[k] = s.params[k];
+    o.verts = verts ? verts + 3 * s.vertex_offset : nullptr;
+    o.nverts = s.vertex_count;
+    return o;
+}
+}  // namespace
+
+#define DISPATCH(h, expr) ((h)->fp64 ? ([&](World<double>& W) { return expr; })(*(h)->wd) \
+                                     : ([&](World<float>& W) { return expr; })(*(h)->wf))
+
+namespace {
+template <class R>
+int mpr_manifold_t(const jh_shape* a, const jh_shape* b, const float* verts, const float* qa, const float* pa,
+                   const float* qb, const float* pb, float epa_threshold, float* out,
+                   Shape<R> (*mk)(const jh_shape&, const float*)) {
+    Shape<R> sa = mk(*a, verts), sb = mk(*b, verts);
+    std::unique_ptr<Polytope<R>> poly(new Polytope<R>());
+    Q4<R> QA{R(qa[0]), R(qa[1]), R(qa[2]), R(qa[3])}, QB{R(qb[0]), R(qb[1]), R(qb[2]), R(qb[3])};
+    MprResult<R> r = mpr_epa(sa, sb, QA, QB, vin<R>(pa), vin<R>(pb), poly.get(), R(epa_threshold));
+    vout(r.pa, out);
+    vout(r.pb, out + 3);
+    vout(r.normal, out + 6);
+    out[9] = float(r.penetration);
+    out[10] = r.epa ? 1.f : 0.f;
+    out[11] = 0;
+    if (r.hit) {
+        Manifold<R> m;
+        m.build(sa, sb, QA, QB, vin<R>(pa), vin<R>(pb), r.pa, r.pb, r.normal);
+        out[11] = float(m.count);
+        for (int i = 0; i < m.count && i < 4; ++i) { vout(m.mA[i], out + 12 + 3 * i); vout(m.mB[i], out + 24 + 3 * i); }
+    }
+    return r.hit ? 1 : 0;
+}
+}  // namespace
+
+extern "C" {
+
+void* orc_create(int fp64, const jh_config* cfg) {
+    Handle* h = new Handle();
+    h->fp64 = fp64 != 0;
+    if (h->fp64) h->wd = new World<double>();
+    else h->wf = new World<float>();
+    if (cfg) DISPATCH(h, (set_config(W, cfg), 0));
+    return h;
+}
+void orc_destroy(void* p) {
+    Handle* h = (Handle*)p;
+    delete h->wf;
+    delete h->wd;
+    delete h;
+}
+int orc_set_config(void* p, const jh_config* cfg) {
+    Handle* h = (Handle*)p;
+    DISPATCH(h, (set_config(W, cfg), 0));
+    return 0;
+}
+int orc_set_bodies(void* p, const jh_body* b, int n) {
+    Handle* h = (Handle*)p;
+    DISPATCH(h, (set_bodies(W, b, n), 0));
+    return 0;
+}
+int orc_get_bodies(void* p, jh_body* b, int n) {
+    Handle* h = (Handle*)p;
+    DISPATCH(h, (get_bodies(W, b, n), 0));
+    return 0;
+}
+// 13 doubles per body: position, orientation (x,y,z,w), velocity, angular velocity.
+int orc_get_state64(void* p, double* out, int n) {
+    Handle* h = (Handle*)p;
+    DISPATCH(h, (get_state64(W, out, n), 0));
+    return 0;
+}
+int orc_set_shapes(void* p, const jh_shape* s, int n, const float* verts, int nverts) {
+    Handle* h = (Handle*)p;
+    h->verts.assign(verts ? verts : (const float*)nullptr, verts ? verts + 3 * (size_t)nverts : nullptr);
+    const float* v = h->verts.empty() ? nullptr : h->verts.data();
+    DISPATCH(h, (set_shapes(W, s, n, v), 0));
+    return 0;
+}
+int orc_set_arbiters(void* p, const jh_arbiter* a, int n) {
+    Handle* h = (Handle*)p;
+    DISPATCH(h, (set_arbiters(W, a, n), 0));
+    return 0;
+}
+int orc_get_arbiters(void* p, jh_arbiter* a, int cap) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, get_arbiters(W, a, cap));
+}
+int orc_get_manifolds(void* p, jh_manifold* m, int cap) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, get_manifolds(W, m, cap));
+}
+// Pair set on the current state (sorted); brute != 0 uses the O(n^2) all-pairs scan.
+int orc_compute_pairs(void* p, uint32_t* out, int cap, int brute) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, ([&]() {
+        auto pairs = W.compute_pairs(brute != 0);
+        int n = (int)pairs.size();
+        for (int i = 0; i < n && i < cap; ++i) { out[2 * i] = pairs[i].first; out[2 * i + 1] = pairs[i].second; }
+        return n;
+    })());
+}
+// One World.Step; order = nkeys (key1, key2) pairs to replay, or NULL for key order.
+int orc_step(void* p, float dt, const uint32_t* order, int nkeys) {
+    Handle* h = (Handle*)p;
+    std::vector<std::pair<uint32_t, uint32_t>> ord;
+    if (order) {
+        ord.resize(nkeys);
+        for (int i = 0; i < nkeys; ++i) ord[i] = {order[2 * i], order[2 * i + 1]};
+    }
+    DISPATCH(h, (W.step(decltype(W.gravity.x)(dt), order ? &ord : nullptr), 0));
+    return 0;
+}
+long orc_epa_calls(void* p) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, W.epa_calls);
+}
+int orc_arbiter_count(void* p) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, (int)W.arbiters.size());
+}
+
+// ---- pure functions for known-answer tests ------------------------------------------------
+// Shape support map in the local frame (FP32 or FP64 evaluation, result rounded to float).
+int orc_support(int fp64, const jh_shape* s, const float* verts, const float* dir, float* out) {
+    if (fp64) {
+        Shape<double> sh = shape_d(*s, verts);
+        vout(sh.support(vin<double>(dir)), out);
+    } else {
+        Shape<float> sh = shape_f(*s, verts);
+        vout(sh.support(vin<float>(dir)), out);
+    }
+    return 0;
+}
+// World AABB of a shape at pose (q, p): out = min xyz, max xyz.
+int orc_aabb(int fp64, const jh_shape* s, const float* verts, const float* q, const float* p, float* out) {
+    if (fp64) {
+        Shape<double> sh = shape_d(*s, verts);
+        V3<double> mn, mx;
+        sh.aabb({q[0], q[1], q[2], q[3]}, vin<double>(p), mn, mx);
+        vout(mn, out);
+        vout(mx, out + 3);
+    } else {
+        Shape<float> sh = shape_f(*s, verts);
+        V3<float> mn, mx;
+        sh.aabb({q[0], q[1], q[2], q[3]}, vin<float>(p), mn, mx);
+        vout(mn, out);
+        vout(mx, out + 3);
+    }
+    return 0;
+}
+// NarrowPhase.MprEpa + optional BuildManifold for two posed shapes.  out: [0..2] pA, [3..5] pB,
+// [6..8] normal, [9] penetration, [10] epa flag, [11] manifold count, [12..23] manifold A,
+// [24..35] manifold B.  Returns 1 on hit.
+int orc_mpr_manifold(int fp64, const jh_shape* a, const jh_shape* b, const float* verts, const float* qa,
+                     const float* pa, const float* qb, const float* pb, float epa_threshold, float* out) {
+    if (fp64) return mpr_manifold_t<double>(a, b, verts, qa, pa, qb, pb, epa_threshold, out, shape_d);
+    return mpr_manifold_t<float>(a, b, verts, qa, pa, qb, pb, epa_threshold, out, shape_f);
+}
+// BuildManifold alone (CollisionManifold.cs:393-566) from a given contact.  out as above from [11].
+int orc_build_manifold(int fp64, const jh_shape* a, const jh_shape* b, const float* verts, const float* qa,
+                       const float* pa, const float* qb, const float* pb, const float* pA, const float* pB,
+                       const float* n, float* out) {
+    if (fp64) {
+        Shape<double> sa = shape_d(*a, verts), sb = shape_d(*b, verts);
+        Manifold<double> m;
+        m.build(sa, sb, {qa[0], qa[1], qa[2], qa[3]}, {qb[0], qb[1], qb[2], qb[3]}, vin<double>(pa), vin<double>(pb),
+                vin<double>(pA), vin<double>(pB), vin<double>(n));
+        out[0] = float(m.count);
+        for (int i = 0; i < m.count; ++i) { vout(m.mA[i], out + 1 + 3 * i); vout(m.mB[i], out + 13 + 3 * i); }
+        return m.count;
+    }
+    Shape<float> sa = shape_f(*a, verts), sb = shape_f(*b, verts);
+    Manifold<float> m;
+    m.build(sa, sb, {qa[0], qa[1], qa[2], qa[3]}, {qb[0], qb[1], qb[2], qb[3]}, vin<float>(pa), vin<float>(pb),
+            vin<float>(pA), vin<float>(pB), vin<float>(n));
+    out[0] = float(m.count);
+    for (int i = 0; i < m.count; ++i) { vout(m.mA[i], out + 1 + 3 * i); vout(m.mB[i], out + 13 + 3 * i); }
+    return m.count;
+}
+// StableMath.SinCos / Atan2 in FP32 (float) for the trigonometry checks.
+void orc_sincos_f(float a, float* s, float* c) { StableMath<float>::sincos(a, *s, *c); }
+float orc_atan2_f(float y, float x) { return StableMath<float>::atan2(y, x); }
+double orc_atan2_d(double y, double x) { return StableMath<double>::atan2(y, x); }
+void orc_sincos_d(double a, double* s, double* c) { StableMath<double>::sincos(a, *s, *c); }
+// MathHelper.RotationQuaternion (FP32).
+void orc_rotation_quaternion_f(const float* w, float dt, float* q) {
+    Q4<float> r = rotation_quaternion(vin<float>(w), dt);
+    q[0] = r.x; q[1] = r.y; q[2] = r.z; q[3] = r.w;
+}
+// ExpansionArithmetic.Side on FP32 coordinates (exact orientation sign).
+int orc_exact_side(const float* a, const float* b, const float* c, const float* p) {
+    return exact::side(vin<float>(a), vin<float>(b), vin<float>(c), vin<float>(p));
+}
+// Wall-clock time of k steps in key order (CPU baseline).
+double orc_time_steps(void* p, float dt, int k) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < k; ++i) orc_step(p, dt, nullptr, 0);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,499 @@
+// ORACLE — test infrastructure only (see orc_math.hpp).  Contact cache (ContactData/Contact),
+// the projected Gauss-Seidel solver (PrepareForIterationAccelerated / IterateAccelerated),
+// integration, end-of-step body update, and the step orchestration of World.StepInternal,
+// restated from Jitter2.  The solve order is either given (a replay of an order produced
+// elsewhere) or the key order (ArbiterKey ascending), which is the deterministic order of
+// World.Deterministic.cs:142-203 for a world with a single island.
+#pragma once
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "orc_collision.hpp"
+
+namespace orc {
+
+template <class R> struct Body {
+    // RigidBodyData (RigidBody.cs:107-250)
+    V3<R> pos, vel, angvel, dvel, dangvel;
+    Q4<R> q;
+    Sym<R> inv_inertia_world;
+    V3<R> inv_mass_vec;
+    int motion = JH_MOTION_DYNAMIC;
+    // managed RigidBody fields read by RigidBody.Update (RigidBody.cs:398-409, 716-764)
+    Sym<R> inv_inertia;
+    R inv_mass = R(1);
+    R friction = R(0.2), restitution = R(0);
+    R lin_damp = R(0.998), ang_damp = R(0.995);
+    V3<R> force{}, torque{};
+    bool gravity = true;
+    int world = 0;
+
+    // RigidBody.UpdateWorldInertia (RigidBody.cs:903-923), no motion locks.
+    void update_world_inertia() {
+        if (motion == JH_MOTION_DYNAMIC) {
+            M3<R> m = mat_from_quat(q);
+            inv_inertia_world = sym_transform(inv_inertia, m);
+            inv_mass_vec = {inv_mass, inv_mass, inv_mass};
+        } else {
+            inv_inertia_world = Sym<R>{R(0), R(0), R(0), R(0), R(0), R(0)};
+            inv_mass_vec = {R(0), R(0), R(0)};
+        }
+    }
+};
+
+// Contact (Contact.cs:617-661); the Vector128 rows NormalTangentX/Y/Z hold (n, t1, t2) lanes.
+template <class R> struct Contact {
+    static constexpr R MaximumBias = R(100.0), BiasFactor = R(0.2), AllowedPenetration = R(0.01),
+                       BreakThreshold = R(0.02);
+    enum { NewContact = 2, SkipWarmStart = 4 };
+    uint32_t flag = 0;
+    R bias = 0, penalty = 0;
+    V3<R> n{}, t1{}, t2{};
+    R mass[3] = {0, 0, 0};
+    R acc[3] = {0, 0, 0};
+    V3<R> pos1{}, pos2{}, rel1{}, rel2{};
+};
+
+template <class R> struct Arbiter {
+    uint32_t key1 = 0, key2 = 0;  // shape indices (ascending shape id)
+    int b1 = 0, b2 = 0;
+    uint32_t usage = 0;
+    uint32_t mode = 0;
+    R friction = 0, restitution = 0, specrel = R(0.9);
+    Contact<R> c[4];
+};
+
+template <class R> struct StepTiming {
+    R dt, substep_dt, inv_dt, inv_substep_dt;
+    int substeps;
+    // TimeStep(dt, substepCount) (TimeStep.cs:36-46)
+    StepTiming(R d, int n) : dt(d), substeps(n) {
+        substep_dt = d / R(n);
+        inv_dt = d > R(0) ? R(1.0) / d : R(0);
+        inv_substep_dt = substep_dt > R(0) ? R(1.0) / substep_dt : R(0);
+    }
+};
+
+template <class R> struct ManifoldOut {
+    uint32_t k1, k2;
+    bool epa;
+    V3<R> normal, pa, pb;
+    R pen;
+    int count;
+    V3<R> mA[4], mB[4];
+};
+
+template <class R> struct World {
+    std::vector<Body<R>> bodies;
+    std::vector<Shape<R>> shapes;
+    std::vector<float> verts;
+    std::map<std::pair<uint32_t, uint32_t>, Arbiter<R>> arbiters;  // ArbiterKey -> ContactData
+    V3<R> gravity{R(0), R(-9.81), R(0)};
+    int substeps = 1, iterations = 6, relaxations = 4;
+    R specrel = R(0.9);
+    bool aux_contacts = true, persistent = true;
+    std::vector<ManifoldOut<R>> last_manifolds;
+    std::vector<std::pair<uint32_t, uint32_t>> last_pairs;
+    long epa_calls = 0;
+    Polytope<R>* poly = nullptr;
+
+    World() { poly = new Polytope<R>(); }
+    ~World() { delete poly; }
+
+    void world_aabb(const Shape<R>& s, V3<R>& mn, V3<R>& mx) const {
+        const Body<R>& b = bodies[s.body];
+        s.aabb(b.q, b.pos, mn, mx);
+    }
+
+    // The pair set visited by DynamicTree.EnumerateOverlaps (DynamicTree.cs:266-286) after the
+    // World.DefaultDynamicTreeFilter (World.cs:447-455) and the Detect gating (World.Detect.cs:158-160):
+    // shapes of different bodies in the same world, at least one body dynamic, world AABBs not
+    // disjoint.  Sort-and-sweep along x; returns pairs (lower index, higher index) sorted.
+    std::vector<std::pair<uint32_t, uint32_t>> compute_pairs(bool brute = false) const {
+        size_t n = shapes.size();
+        std::vector<V3<R>> mn(n), mx(n);
+        for (size_t i = 0; i < n; ++i) world_aabb(shapes[i], mn[i], mx[i]);
+        std::vector<std::pair<uint32_t, uint32_t>> out;
+        auto accept = [&](size_t i, size_t j) {
+            const Shape<R>& a = shapes[i];
+            const Shape<R>& b = shapes[j];
+            if (a.body == b.body) return;
+            const Body<R>& ba = bodies[a.body];
+            const Body<R>& bb = bodies[b.body];
+            if (ba.world != bb.world) return;
+            if (ba.motion != JH_MOTION_DYNAMIC && bb.motion != JH_MOTION_DYNAMIC) return;
+            if (disjoint(mn[i], mx[i], mn[j], mx[j])) return;
+            out.emplace_back(uint32_t(std::min(i, j)), uint32_t(std::max(i, j)));
+        };
+        if (brute) {
+            for (size_t i = 0; i < n; ++i)
+                for (size_t j = i + 1; j < n; ++j) accept(i, j);
+        } else {
+            std::vector<uint32_t> order(n);
+            for (size_t i = 0; i < n; ++i) order[i] = uint32_t(i);
+            std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                return mn[a].x < mn[b].x || (mn[a].x == mn[b].x && a < b);
+            });
+            for (size_t oi = 0; oi < n; ++oi) {
+                uint32_t i = order[oi];
+                for (size_t oj = oi + 1; oj < n; ++oj) {
+                    uint32_t j = order[oj];
+                    if (mn[j].x > mx[i].x) break;
+                    accept(i, j);
+                }
+            }
+        }
+        std::sort(out.begin(), out.end());
+        return out;
+    }
+
+    // ContactData.Init (Contact.cs:271-284).
+    void init_arbiter(Arbiter<R>& a, uint32_t k1, uint32_t k2) {
+        a = Arbiter<R>{};
+        a.key1 = k1;
+        a.key2 = k2;
+        a.b1 = shapes[k1].body;
+        a.b2 = shapes[k2].body;
+        a.friction = rmax(bodies[a.b1].friction, bodies[a.b2].friction);
+        a.restitution = rmax(bodies[a.b1].restitution, bodies[a.b2].restitution);
+        a.specrel = specrel;
+        a.mode = 0;
+        a.usage = 0;
+    }
+    // ContactData.ResetMode (Contact.cs:291-296).
+    void reset_mode(Arbiter<R>& a) {
+        a.mode = (bodies[a.b1].motion == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1) : 0;
+        a.mode |= (bodies[a.b2].motion == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2) : 0;
+    }
+
+    // Contact.Initialize (Contact.cs:703-759).
+    void contact_initialize(Arbiter<R>& cd, Contact<R>& c, V3<R> p1, V3<R> p2, V3<R> n, bool is_new) {
+        const Body<R>& b1 = bodies[cd.b1];
+        const Body<R>& b2 = bodies[cd.b2];
+        c.rel1 = p1 - b1.pos;
+        c.rel2 = p2 - b2.pos;
+        c.pos1 = rotate_inv(c.rel1, b1.q);
+        c.pos2 = rotate_inv(c.rel2, b2.q);
+        if (!is_new) return;
+        c.flag = Contact<R>::NewContact;
+        c.acc[0] = c.acc[1] = c.acc[2] = R(0.0);
+        V3<R> dv = b2.vel - b1.vel;
+        if (cd.mode == 0 || (cd.mode & (JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2)) != 0) {
+            dv = b2.vel + cross(b2.angvel, c.rel2);
+            dv = dv - (b1.vel + cross(b1.angvel, c.rel1));
+        }
+        R rel_nv = dot(dv, n);
+        c.bias = 0;
+        if (rel_nv < R(-1.0)) c.bias = -cd.restitution * rel_nv;
+        V3<R> t1 = dv - n * rel_nv;
+        R num = len_sq(t1);
+        if (num > R(1e-12)) {
+            num = R(1.0) / std::sqrt(num);
+            t1 = t1 * num;
+        } else {
+            t1 = create_orthonormal(n);
+        }
+        V3<R> t2 = cross(t1, n);
+        c.n = n;
+        c.t1 = t1;
+        c.t2 = t2;
+    }
+
+    // CalcArea4Points (Contact.cs:418-432) as evaluated lane-wise by the vectorized selector.
+    static R area4(V3<R> p0, V3<R> p1, V3<R> p2, V3<R> p3) {
+        V3<R> a0 = p0 - p1, a1 = p0 - p2, a2 = p0 - p3, b0 = p2 - p3, b1 = p1 - p3, b2 = p1 - p2;
+        R c0 = len_sq(cross(a0, b0)), c1 = len_sq(cross(a1, b1)), c2 = len_sq(cross(a2, b2));
+        return rmax(rmax(c0, c1), c2);
+    }
+    // SortCachedPoints (Contact.cs:529-553) with SelectCachedPointReplacementMask (:434-466).
+    void sort_cached_points(Arbiter<R>& cd, V3<R> p1, V3<R> p2, V3<R> n) {
+        V3<R> rp1 = p1 - bodies[cd.b1].pos;
+        R a0 = area4(rp1, cd.c[1].rel1, cd.c[2].rel1, cd.c[3].rel1);
+        R a1 = area4(rp1, cd.c[0].rel1, cd.c[2].rel1, cd.c[3].rel1);
+        R a2 = area4(rp1, cd.c[0].rel1, cd.c[1].rel1, cd.c[3].rel1);
+        R a3 = area4(rp1, cd.c[0].rel1, cd.c[1].rel1, cd.c[2].rel1);
+        const R eps = -R(0.0001);
+        R big = 0;
+        uint32_t mask = 0;
+        if (a0 > big + eps) { big = a0; mask = 1; }
+        if (a1 > big + eps) { big = a1; mask = 2; }
+        if (a2 > big + eps) { big = a2; mask = 4; }
+        if (a3 > big + eps) { mask = 8; }
+        int slot = mask == 2 ? 1 : mask == 4 ? 2 : mask == 8 ? 3 : 0;
+        contact_initialize(cd, cd.c[slot], p1, p2, n, false);
+        cd.usage |= mask;
+    }
+    // ContactData.AddContact (Contact.cs:329-416).
+    void add_contact(Arbiter<R>& cd, V3<R> p1, V3<R> p2, V3<R> n) {
+        if ((cd.usage & 15u) == 15u) { sort_cached_points(cd, p1, p2, n); return; }
+        int closest = -1;
+        R dsq = std::numeric_limits<R>::max();
+        V3<R> rel = p1 - bodies[cd.b1].pos;
+        for (int i = 0; i < 4; ++i) {
+            if ((cd.usage & (1u << i)) == 0) continue;
+            R d = len_sq(cd.c[i].rel1 - rel);
+            if (d < dsq) { dsq = d; closest = i; }
+        }
+        if (dsq < Contact<R>::BreakThreshold * Contact<R>::BreakThreshold) {
+            contact_initialize(cd, cd.c[closest], p1, p2, n, false);
+            return;
+        }
+        for (int i = 0; i < 4; ++i) {
+            if ((cd.usage & (1u << i)) == 0) {
+                contact_initialize(cd, cd.c[i], p1, p2, n, true);
+                cd.usage |= (1u << i);
+                return;
+            }
+        }
+    }
+
+    // World.Detect (World.Detect.cs:125-218) for one broadphase pair (k1 < k2), default filters,
+    // no speculative contacts; the default TriangleEdgeCollisionFilter only acts on TriangleShape.
+    void detect(uint32_t k1, uint32_t k2) {
+        const Shape<R>& sA = shapes[k1];
+        const Shape<R>& sB = shapes[k2];
+        const Body<R>& b1 = bodies[sA.body];
+        const Body<R>& b2 = bodies[sB.body];
+        if (b1.motion != JH_MOTION_DYNAMIC && b2.motion != JH_MOTION_DYNAMIC) return;
+        MprResult<R> r = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, poly);
+        if (r.epa) epa_calls++;
+        if (!r.hit) return;
+        ManifoldOut<R> mo{};
+        mo.k1 = k1;
+        mo.k2 = k2;
+        mo.epa = r.epa;
+        mo.normal = r.normal;
+        mo.pa = r.pa;
+        mo.pb = r.pb;
+        mo.pen = r.penetration;
+        Manifold<R> m;
+        if (aux_contacts) {
+            m.build(sA, sB, b1.q, b2.q, b1.pos, b2.pos, r.pa, r.pb, r.normal);
+        } else {
+            m.count = 1;
+            m.mA[0] = r.pa;
+            m.mB[0] = r.pb;
+        }
+        mo.count = m.count;
+        for (int i = 0; i < m.count; ++i) { mo.mA[i] = m.mA[i]; mo.mB[i] = m.mB[i]; }
+        last_manifolds.push_back(mo);
+        // RegisterContact (World.Detect.cs:294-324)
+        auto key = std::make_pair(k1, k2);
+        auto it = arbiters.find(key);
+        if (it == arbiters.end()) {
+            Arbiter<R> a;
+            init_arbiter(a, k1, k2);
+            it = arbiters.emplace(key, a).first;
+        }
+        Arbiter<R>& cd = it->second;
+        if (!persistent) cd.usage = 0;
+        reset_mode(cd);
+        for (int e = 0; e < m.count; ++e) add_contact(cd, m.mA[e], m.mB[e], r.normal);
+    }
+
+    static R sum3(const R* w, R a, R b, R c) { return w[0] * a + w[1] * b + w[2] * c; }
+
+    // Contact.PrepareForIterationAccelerated (Contact.cs:1038-1180), no linear locks.
+    void prepare_contact(Arbiter<R>& cd, Contact<R>& c, const StepTiming<R>& ts) {
+        Body<R>& b1 = bodies[cd.b1];
+        Body<R>& b2 = bodies[cd.b2];
+        bool skip = (c.flag & Contact<R>::SkipWarmStart) != 0;
+        c.flag &= ~uint32_t(Contact<R>::SkipWarmStart);
+        R ws[3] = {skip ? R(0) : c.acc[0], skip ? R(0) : c.acc[1], skip ? R(0) : c.acc[2]};
+        V3<R> cr1 = rotate(c.pos1, b1.q);
+        V3<R> cr2 = rotate(c.pos2, b2.q);
+        V3<R> imv{R(0), R(0), R(0)};
+        if (cd.mode & JH_SOLVE_LINEAR1) imv = b1.inv_mass_vec;
+        if (cd.mode & JH_SOLVE_LINEAR2) imv = imv + b2.inv_mass_vec;
+        R k[3] = {imv.x, imv.x, imv.x};
+        if ((c.flag & Contact<R>::NewContact) == 0) c.bias = 0;
+        V3<R> r1 = cr1 + b1.pos, r2 = cr2 + b2.pos;
+        V3<R> dist = r1 - r2;
+        R pen = dot(dist, c.n);
+        if (pen < -Contact<R>::BreakThreshold) c.bias = pen * ts.inv_dt * cd.specrel;
+        c.flag &= ~uint32_t(Contact<R>::NewContact);
+        // Rows: X = (n.x, t1.x, t2.x), Y, Z.
+        R NX[3] = {c.n.x, c.t1.x, c.t2.x}, NY[3] = {c.n.y, c.t1.y, c.t2.y}, NZ[3] = {c.n.z, c.t1.z, c.t2.z};
+        R ktx[3] = {0, 0, 0}, kty[3] = {0, 0, 0}, ktz[3] = {0, 0, 0};
+        V3<R> lin{sum3(ws, NX[0], NX[1], NX[2]), sum3(ws, NY[0], NY[1], NY[2]), sum3(ws, NZ[0], NZ[1], NZ[2])};
+        if (cd.mode & JH_SOLVE_LINEAR1) b1.vel = b1.vel - cmul(lin, b1.inv_mass_vec);
+        if (cd.mode & JH_SOLVE_ANGULAR1) {
+            V3<R> rp = c.rel1;
+            const Sym<R>& I = b1.inv_inertia_world;
+            R e1[3], e2[3], e3[3];
+            for (int l = 0; l < 3; ++l) {
+                R rrx = rp.y * NZ[l] - rp.z * NY[l];
+                R rry = rp.z * NX[l] - rp.x * NZ[l];
+                R rrz = rp.x * NY[l] - rp.y * NX[l];
+                e1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+                e2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+                e3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+            }
+            V3<R> ai{sum3(ws, e1[0], e1[1], e1[2]), sum3(ws, e2[0], e2[1], e2[2]), sum3(ws, e3[0], e3[1], e3[2])};
+            b1.angvel = b1.angvel - ai;
+            for (int l = 0; l < 3; ++l) {
+                ktx[l] = e2[l] * rp.z - e3[l] * rp.y;
+                kty[l] = e3[l] * rp.x - e1[l] * rp.z;
+                ktz[l] = e1[l] * rp.y - e2[l] * rp.x;
+            }
+        }
+        if (cd.mode & JH_SOLVE_LINEAR2) b2.vel = b2.vel + cmul(lin, b2.inv_mass_vec);
+        if (cd.mode & JH_SOLVE_ANGULAR2) {
+            V3<R> rp = c.rel2;
+            const Sym<R>& I = b2.inv_inertia_world;
+            R f1[3], f2[3], f3[3];
+            for (int l = 0; l < 3; ++l) {
+                R rrx = rp.y * NZ[l] - rp.z * NY[l];
+                R rry = rp.z * NX[l] - rp.x * NZ[l];
+                R rrz = rp.x * NY[l] - rp.y * NX[l];
+                f1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+                f2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+                f3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+            }
+            V3<R> ai{sum3(ws, f1[0], f1[1], f1[2]), sum3(ws, f2[0], f2[1], f2[2]), sum3(ws, f3[0], f3[1], f3[2])};
+            b2.angvel = b2.angvel + ai;
+            for (int l = 0; l < 3; ++l) {
+                ktx[l] = ktx[l] + (f2[l] * rp.z - f3[l] * rp.y);
+                kty[l] = kty[l] + (f3[l] * rp.x - f1[l] * rp.z);
+                ktz[l] = ktz[l] + (f1[l] * rp.y - f2[l] * rp.x);
+            }
+        }
+        for (int l = 0; l < 3; ++l) {
+            R kres = NX[l] * ktx[l] + NY[l] * kty[l] + NZ[l] * ktz[l];
+            R kk = k[l] + kres;
+            c.mass[l] = kk > R(0.0) ? R(1.0) / kk : R(0.0);
+        }
+        c.penalty = Contact<R>::BiasFactor * ts.inv_dt * rmax(R(0.0), pen - Contact<R>::AllowedPenetration);
+        c.penalty = rmin(c.penalty, Contact<R>::MaximumBias);
+    }
+
+    // Contact.IterateAccelerated (Contact.cs:1183-1248).
+    void iterate_contact(Arbiter<R>& cd, Contact<R>& c, bool apply_bias) {
+        Body<R>& b1 = bodies[cd.b1];
+        Body<R>& b2 = bodies[cd.b2];
+        V3<R> dv = b2.vel - b1.vel;
+        if (cd.mode & (JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2)) {
+            dv = b2.vel + cross(b2.angvel, c.rel2);
+            dv = dv - (b1.vel + cross(b1.angvel, c.rel1));
+        }
+        R bias = c.bias;
+        if (apply_bias && c.penalty > 0) bias = rmax(c.penalty, c.bias);
+        R NX[3] = {c.n.x, c.t1.x, c.t2.x}, NY[3] = {c.n.y, c.t1.y, c.t2.y}, NZ[3] = {c.n.z, c.t1.z, c.t2.z};
+        R imp[3], old[3] = {c.acc[0], c.acc[1], c.acc[2]};
+        R bv[3] = {bias, R(0), R(0)};
+        for (int l = 0; l < 3; ++l) {
+            R vd = NX[l] * dv.x + NY[l] * dv.y + NZ[l] * dv.z;
+            imp[l] = c.mass[l] * (bv[l] - vd);
+        }
+        R maxT = cd.friction * c.acc[0];
+        R lo[3] = {R(0), -maxT, -maxT}, hi[3] = {std::numeric_limits<R>::max(), maxT, maxT};
+        for (int l = 0; l < 3; ++l) {
+            R a = old[l] + imp[l];
+            a = rmin(rmax(a, lo[l]), hi[l]);
+            c.acc[l] = a;
+            imp[l] = a - old[l];
+        }
+        V3<R> lin{sum3(imp, NX[0], NX[1], NX[2]), sum3(imp, NY[0], NY[1], NY[2]), sum3(imp, NZ[0], NZ[1], NZ[2])};
+        if (cd.mode & JH_SOLVE_LINEAR1) b1.vel = b1.vel - cmul(lin, b1.inv_mass_vec);
+        if (cd.mode & JH_SOLVE_ANGULAR1) b1.angvel = b1.angvel - sym_mul_contact(cross(c.rel1, lin), b1.inv_inertia_world);
+        if (cd.mode & JH_SOLVE_LINEAR2) b2.vel = b2.vel + cmul(lin, b2.inv_mass_vec);
+        if (cd.mode & JH_SOLVE_ANGULAR2) b2.angvel = b2.angvel + sym_mul_contact(cross(c.rel2, lin), b2.inv_inertia_world);
+    }
+
+    // Contact.UpdatePosition (Contact.cs:762-797).
+    bool contact_update_position(Arbiter<R>& cd, Contact<R>& c) {
+        const Body<R>& b1 = bodies[cd.b1];
+        const Body<R>& b2 = bodies[cd.b2];
+        c.rel1 = rotate(c.pos1, b1.q);
+        V3<R> p1 = c.rel1 + b1.pos;
+        c.rel2 = rotate(c.pos2, b2.q);
+        V3<R> p2 = c.rel2 + b2.pos;
+        V3<R> dist = p1 - p2;
+        R pen = dot(dist, c.n);
+        if (pen < -Contact<R>::BreakThreshold * R(0.1)) return false;
+        dist = dist - pen * c.n;
+        R tsq = len_sq(dist);
+        if (tsq > Contact<R>::BreakThreshold * Contact<R>::BreakThreshold) return false;
+        return true;
+    }
+    // ContactData.UpdatePosition (Contact.cs:238-264).
+    void arbiter_update_position(Arbiter<R>& cd) {
+        cd.usage &= 15u;
+        cd.usage |= cd.usage << 4;
+        for (int i = 0; i < 4; ++i)
+            if ((cd.usage & (1u << i)) != 0 && !contact_update_position(cd, cd.c[i])) cd.usage &= ~(1u << i);
+    }
+
+    // World.Step (World.Step.cs:141-296) with a fixed solve order.
+    void step(R dt, const std::vector<std::pair<uint32_t, uint32_t>>* order) {
+        if (!(dt >= R(0)) || !std::isfinite(dt)) return;
+        if (dt < std::numeric_limits<R>::denorm_min()) return;
+        StepTiming<R> ts(dt, substeps);
+        // Narrowphase over the broadphase pair set.
+        last_manifolds.clear();
+        epa_calls = 0;
+        last_pairs = compute_pairs();
+        for (auto& p : last_pairs) detect(p.first, p.second);
+        // Solve order: given, else key order.
+        std::vector<Arbiter<R>*> list;
+        if (order) {
+            for (auto& k : *order) {
+                auto it = arbiters.find(k);
+                if (it != arbiters.end()) list.push_back(&it->second);
+            }
+        } else {
+            for (auto& kv : arbiters) list.push_back(&kv.second);
+        }
+        for (int s = 0; s < ts.substeps; ++s) {
+            // IntegrateForces (World.Step.cs:1060-1072)
+            for (auto& b : bodies) {
+                if (b.motion != JH_MOTION_DYNAMIC) continue;
+                b.angvel = b.angvel + b.dangvel;
+                b.vel = b.vel + b.dvel;
+            }
+            // SolveVelocities (World.Step.cs:1155-1191): prepare, then iterations
+            for (auto* a : list)
+                for (int i = 0; i < 4; ++i)
+                    if (a->usage & (1u << i)) prepare_contact(*a, a->c[i], ts);
+            for (int it = 0; it < iterations; ++it)
+                for (auto* a : list)
+                    for (int i = 0; i < 4; ++i)
+                        if (a->usage & (1u << i)) iterate_contact(*a, a->c[i], true);
+            // IntegrateVelocities (World.Step.cs:1106-1133), no gyroscopic term
+            for (auto& b : bodies) {
+                if (b.motion == JH_MOTION_STATIC) continue;
+                b.pos = b.pos + b.vel * ts.substep_dt;
+                Q4<R> rq = rotation_quaternion(b.angvel, ts.substep_dt);
+                b.q = qnormalize(qmul(rq, b.q));
+            }
+            // RelaxVelocities (World.Step.cs:1135-1153)
+            for (int it = 0; it < relaxations; ++it)
+                for (auto* a : list)
+                    for (int i = 0; i < 4; ++i)
+                        if (a->usage & (1u << i)) iterate_contact(*a, a->c[i], false);
+        }
+        // UpdateContacts (World.Step.cs:862-878).  Arbiters left without intact contacts carry no
+        // solver work; the reference frees them one step later (RemoveBrokenArbiters runs before
+        // UpdateContacts, World.Step.cs:259-265), which leaves body state unchanged.
+        for (auto it = arbiters.begin(); it != arbiters.end();) {
+            arbiter_update_position(it->second);
+            if ((it->second.usage & 15u) == 0) it = arbiters.erase(it);
+            else ++it;
+        }
+        // ForeachActiveBody -> RigidBody.Update (RigidBody.cs:716-764)
+        for (auto& b : bodies) {
+            if (b.motion != JH_MOTION_DYNAMIC) continue;
+            b.update_world_inertia();
+            b.angvel = b.angvel * b.ang_damp;
+            b.vel = b.vel * b.lin_damp;
+            b.dvel = cmul(b.force, b.inv_mass_vec) * ts.substep_dt;
+            b.dangvel = sym_mul(b.torque, b.inv_inertia_world) * ts.substep_dt;
+            if (b.gravity) b.dvel = b.dvel + gravity * ts.substep_dt;
+            b.force = {R(0), R(0), R(0)};
+            b.torque = {R(0), R(0), R(0)};
+        }
+    }
+};
+
+}  // namespace orc

@@ -1,0 +1,124 @@
+"""ctypes wrapper of the oracle (oracle/liborc.so): the CPU restatement of the reference step used as
+the checker by the tests, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg."""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from jitterphysics2_amd._abi import ARBITER_DTYPE, BODY_DTYPE, MANIFOLD_DTYPE, ptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORC_DIR = os.path.join(ROOT, "oracle")
+ORC_LIB = os.path.join(ORC_DIR, "liborc.so")
+
+_lib = None
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", ORC_DIR], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORC_LIB):
+            build_oracle()
+        L = ctypes.CDLL(ORC_LIB)
+        vp, i32, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        L.orc_create.argtypes = [i32, vp]
+        L.orc_create.restype = vp
+        L.orc_destroy.argtypes = [vp]
+        for n, a in {
+            "orc_set_config": [vp, vp], "orc_set_bodies": [vp, vp, i32], "orc_get_bodies": [vp, vp, i32],
+            "orc_get_state64": [vp, vp, i32], "orc_set_shapes": [vp, vp, i32, vp, i32],
+            "orc_set_arbiters": [vp, vp, i32], "orc_get_arbiters": [vp, vp, i32],
+            "orc_get_manifolds": [vp, vp, i32], "orc_compute_pairs": [vp, vp, i32, i32],
+            "orc_step": [vp, f, vp, i32], "orc_arbiter_count": [vp],
+            "orc_support": [i32, vp, vp, vp, vp], "orc_aabb": [i32, vp, vp, vp, vp, vp],
+            "orc_mpr_manifold": [i32, vp, vp, vp, vp, vp, vp, vp, f, vp],
+            "orc_build_manifold": [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            "orc_exact_side": [vp, vp, vp, vp],
+        }.items():
+            getattr(L, n).argtypes = a
+            getattr(L, n).restype = i32
+        L.orc_epa_calls.argtypes = [vp]
+        L.orc_epa_calls.restype = ctypes.c_long
+        L.orc_sincos_f.argtypes = [f, ctypes.POINTER(f), ctypes.POINTER(f)]
+        L.orc_sincos_d.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.orc_atan2_f.argtypes = [f, f]
+        L.orc_atan2_f.restype = f
+        L.orc_atan2_d.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_atan2_d.restype = ctypes.c_double
+        L.orc_rotation_quaternion_f.argtypes = [vp, f, vp]
+        L.orc_time_steps.argtypes = [vp, f, i32]
+        L.orc_time_steps.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+class OracleWorld:
+    """The CPU restatement of World.Step on jh_body/jh_shape arrays (fp64=True ≙ Jitter2.Double)."""
+
+    def __init__(self, bodies, shapes, verts=None, config=None, fp64=False):
+        self.L = lib()
+        self.h = self.L.orc_create(1 if fp64 else 0, ptr(config) if config is not None else None)
+        self.n = len(bodies)
+        self.fp64 = fp64
+        self.verts = None if verts is None else np.ascontiguousarray(verts, np.float32)
+        self.L.orc_set_bodies(self.h, ptr(np.ascontiguousarray(bodies)), len(bodies))
+        nv = 0 if self.verts is None else len(self.verts) // 3
+        self.L.orc_set_shapes(self.h, ptr(np.ascontiguousarray(shapes)), len(shapes), ptr(self.verts), nv)
+
+    def __del__(self):
+        try:
+            self.L.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def step(self, dt: float, order: np.ndarray | None = None) -> None:
+        if order is None:
+            self.L.orc_step(self.h, dt, None, 0)
+        else:
+            o = np.ascontiguousarray(order, np.uint32).reshape(-1, 2)
+            self.L.orc_step(self.h, dt, ptr(o), len(o))
+
+    def bodies(self) -> np.ndarray:
+        out = np.zeros(self.n, BODY_DTYPE)
+        self.L.orc_get_bodies(self.h, ptr(out), self.n)
+        return out
+
+    def state64(self) -> np.ndarray:
+        out = np.zeros((self.n, 13), np.float64)
+        self.L.orc_get_state64(self.h, ptr(out), self.n)
+        return out
+
+    def pairs(self, brute: bool = False) -> np.ndarray:
+        n = self.L.orc_compute_pairs(self.h, None, 0, 1 if brute else 0)
+        out = np.zeros((max(n, 1), 2), np.uint32)
+        self.L.orc_compute_pairs(self.h, ptr(out), n, 1 if brute else 0)
+        return out[:n]
+
+    def arbiters(self) -> np.ndarray:
+        n = self.L.orc_get_arbiters(self.h, None, 0)
+        out = np.zeros(n, ARBITER_DTYPE)
+        self.L.orc_get_arbiters(self.h, ptr(out), n)
+        return out
+
+    def set_arbiters(self, arbs: np.ndarray) -> None:
+        a = np.ascontiguousarray(arbs)
+        self.L.orc_set_arbiters(self.h, ptr(a), len(a))
+
+    def manifolds(self) -> np.ndarray:
+        n = self.L.orc_get_manifolds(self.h, None, 0)
+        out = np.zeros(n, MANIFOLD_DTYPE)
+        self.L.orc_get_manifolds(self.h, ptr(out), n)
+        return out
+
+    def epa_calls(self) -> int:
+        return int(self.L.orc_epa_calls(self.h))
+
+    def time_steps(self, dt: float, k: int) -> float:
+        return float(self.L.orc_time_steps(self.h, dt, k))
