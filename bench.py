@@ -1,0 +1,141 @@
+"""Benchmark: body-steps/s of World.Step on one MI355X for a 131,072-body heap.
+
+Workload: `scenes.heap(131072)` — dynamic spheres/boxes/capsules dropped into a static container,
+dt = 1/60, 4 substeps, 10 solver iterations, 4 relaxation iterations (the 32,768-body heap
+configuration scaled to 131,072 bodies, the body count the metric is quoted at).  W warmup steps let
+the heap form contacts; then K steps are timed with inputs resident in HBM.
+
+Besides the JSON line's contract fields it reports:
+  roofline     — the persistent solver kernel (k_solve), whose duration is measured with HIP events
+                 on the library's stream (jh_stats.ms_solve); algorithmic bytes per launch from the
+                 per-step counts (see solver_bytes()).
+  cpu_baseline — the oracle (CPU restatement, 1 thread) timed on a bounded sample: a few steps of the
+                 same heap continued from the GPU state at the start of the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: int, relax: int) -> float:
+    """Algorithmic HBM bytes of one k_solve launch (one step, all substeps).
+    Per arbiter per pass (prepare, each iteration, each relaxation): 152 B of body traffic
+    (2 x (v, w read + write 48 B + inverse mass/inertia 28 B)); per contact point per pass 116 B
+    (row data 92 B read, accumulated impulse 12 B read + 12 B write).  Per substep:
+    IntegrateForces 72 B per dynamic body, IntegrateVelocities 80 B per non-static body."""
+    m, k = stats["arbiters_solved"], stats["contacts"]
+    passes = 1 + iters + relax
+    per_substep = passes * (152.0 * m + 116.0 * k) + 72.0 * n_dyn + 80.0 * n_moving
+    return substeps * per_substep
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=240)
+    ap.add_argument("--bodies", type=int, default=131072)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from jitterphysics2_amd import scenes
+    from jitterphysics2_amd.gpu import GpuWorld
+
+    t0 = time.time()
+    bodies, shapes, verts, cfg, dt = scenes.heap(args.bodies)
+    print(f"# scene built: {len(bodies)} bodies, {len(shapes)} shapes in {time.time() - t0:.1f}s", flush=True)
+    n_dyn = int(np.sum(bodies["motion_type"] == 0))
+    n_moving = int(np.sum(bodies["motion_type"] != 2))
+    substeps = int(cfg["substep_count"][0])
+    iters = int(cfg["solver_iterations"][0])
+    relax = int(cfg["relaxation_iterations"][0])
+
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    t0 = time.time()
+    for i in range(args.warmup):
+        gw.step(dt)
+        if (i + 1) % 60 == 0:
+            s = gw.stats()
+            print(f"# warmup {i + 1}/{args.warmup}: pairs {s['pairs']} arbiters {s['arbiters']} "
+                  f"colours {s['colours']} ms/step {s['ms_step']:.2f}", flush=True)
+    print(f"# warmup done in {time.time() - t0:.1f}s", flush=True)
+
+    start_bodies = gw.bodies()
+    start_arbiters = gw.arbiters()
+
+    solve_ms, step_ms, bytes_per_launch, contacts = [], [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        gw.step(dt)  # returns after the step finished on the GPU
+        s = gw.stats()
+        solve_ms.append(s["ms_solve"])
+        step_ms.append(s["ms_step"])
+        contacts.append(s["contacts"])
+        bytes_per_launch.append(solver_bytes(s, n_dyn, n_moving, substeps, iters, relax))
+    t_end = time.perf_counter()
+    wall = t_end - t_start
+    value = n_dyn * args.steps / wall
+    ms_per_step = 1000.0 * wall / args.steps
+    avg_solve_s = float(np.mean(solve_ms)) / 1000.0
+    achieved = float(np.mean(bytes_per_launch)) / avg_solve_s / 1e9
+    final = gw.stats()
+
+    cpu = None
+    if not args.no_cpu:
+        from tests.orc import OracleWorld
+
+        ow = OracleWorld(start_bodies, shapes, verts, cfg)
+        ow.set_arbiters(start_arbiters)
+        secs = ow.time_steps(dt, args.cpu_steps)
+        cpu = {"value": n_dyn * args.cpu_steps / secs, "unit": "body-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body heap continued from the GPU state at "
+                         f"the start of the timed region (oracle FP32, key-order solve, 1 thread)"}
+
+    line = {
+        "metric": "body-steps/sec at 100k bodies",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"heap of {n_dyn} dynamic spheres/boxes/capsules in a static container, dt=1/60, "
+                               f"{substeps} substeps, {iters} iterations, {relax} relaxations (plain warm start)",
+                   "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],
+                   "contacts": final["contacts"], "colours": final["colours"],
+                   "contacts_per_s": float(np.mean(contacts)) * args.steps / wall,
+                   "stage_ms": {k: final[k] for k in ("ms_broadphase", "ms_narrowphase", "ms_contact_cache",
+                                                      "ms_colouring", "ms_solve", "ms_update")}},
+        "roofline": {"bound": "hbm", "kernel": "k_solve", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": 1000.0 * avg_solve_s,
+                     "algorithmic_bytes_per_launch": float(np.mean(bytes_per_launch))},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

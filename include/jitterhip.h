@@ -158,6 +158,7 @@ typedef struct jh_stats {
     int64_t bodies, shapes, pairs, colliding_pairs, arbiters, contacts, colours, epa_calls;
     int64_t steps;
     int64_t pair_overflow;
+    int64_t arbiters_solved;  /* arbiters in the last solve (before contact refresh removed broken ones) */
     double ms_broadphase, ms_narrowphase, ms_contact_cache, ms_colouring, ms_solve, ms_update;
     double ms_step;
 } jh_stats;
@@ -168,6 +169,9 @@ typedef struct jh_world jh_world;
 const char* jh_last_error(void);
 /* ABI version: 1. */
 int32_t jh_version(void);
+/* sizeof of the ABI structs, for binding checks: 0 jh_config, 1 jh_body, 2 jh_shape, 3 jh_contact,
+ * 4 jh_arbiter, 5 jh_manifold, 6 jh_stats; -1 otherwise. */
+int32_t jh_struct_size(int32_t which);
 
 /* World lifetime (World ctor World.cs:423-441, Dispose World.cs:977-990).  config may be NULL for
  * the reference defaults. */

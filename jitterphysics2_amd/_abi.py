@@ -113,7 +113,7 @@ MANIFOLD_DTYPE = np.dtype([
 STATS_DTYPE = np.dtype([
     ("bodies", "<i8"), ("shapes", "<i8"), ("pairs", "<i8"), ("colliding_pairs", "<i8"),
     ("arbiters", "<i8"), ("contacts", "<i8"), ("colours", "<i8"), ("epa_calls", "<i8"),
-    ("steps", "<i8"), ("pair_overflow", "<i8"),
+    ("steps", "<i8"), ("pair_overflow", "<i8"), ("arbiters_solved", "<i8"),
     ("ms_broadphase", "<f8"), ("ms_narrowphase", "<f8"), ("ms_contact_cache", "<f8"),
     ("ms_colouring", "<f8"), ("ms_solve", "<f8"), ("ms_update", "<f8"), ("ms_step", "<f8"),
 ])
@@ -143,11 +143,12 @@ def default_config() -> np.ndarray:
 
 
 def ptr(a: np.ndarray | None):
-    """ctypes void pointer to a contiguous numpy array (or NULL)."""
+    """ctypes void pointer to a contiguous numpy array (or NULL).  The pointer object holds a
+    reference to the array, so temporaries stay alive for the duration of the call."""
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
-    return ctypes.c_void_p(a.ctypes.data)
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 _lib = None

@@ -1,0 +1,1058 @@
+// Per-step kernels of the gfx950 path: body pack/unpack, broadphase (AABB, Morton, LBVH build,
+// refit, traversal), narrowphase (MPR + manifold, EPA on the deep queue), contact cache
+// (merge-join against the key-sorted arbiter array, AddContact), graph colouring, the persistent
+// projected Gauss-Seidel solver, contact refresh and the end-of-step body update.
+#pragma once
+#include <hip/hip_cooperative_groups.h>
+
+#include "jh_collide.h"
+#include "jh_epa.h"
+
+namespace cg = cooperative_groups;
+
+namespace jh {
+
+// ---- Device state ---------------------------------------------------------------------------
+struct Bodies {
+    float4 *pos, *quat, *vel, *angvel, *dvel, *dangvel;
+    float4* imv;          // InverseMassVector (xyz)
+    float4* iw0;          // InverseInertiaWorld M11 M12 M13 M22
+    float2* iw1;          // InverseInertiaWorld M23 M33
+    float4* il0;          // body-frame inverse inertia M11 M12 M13 M22
+    float2* il1;          // M23 M33
+    float4* mat;          // inverse mass, friction, restitution, linear damping multiplier
+    float4* mat2;         // angular damping multiplier, -, -, -
+    float4 *force, *torque;
+    int* motion;          // MotionType
+    int* gravity;         // AffectedByGravity
+    int* world;           // sub-world id
+};
+
+struct StepTime {
+    float dt, substep_dt, inv_dt, inv_substep_dt;
+    int substeps;
+};
+
+__device__ __forceinline__ sym3 ld_iw(const Bodies& B, int b) {
+    float4 a = B.iw0[b];
+    float2 c = B.iw1[b];
+    return {a.x, a.y, a.z, a.w, c.x, c.y};
+}
+
+// ---- Bodies ---------------------------------------------------------------------------------
+// RigidBody.UpdateWorldInertia (RigidBody.cs:903-923), no motion locks.
+__device__ __forceinline__ void update_world_inertia(const Bodies& B, int b) {
+    if (B.motion[b] == JH_MOTION_DYNAMIC) {
+        float4 l0 = B.il0[b];
+        float2 l1 = B.il1[b];
+        sym3 il{l0.x, l0.y, l0.z, l0.w, l1.x, l1.y};
+        sym3 w = sym_rotate(il, mat_from_quat(ldq(B.quat[b])));
+        B.iw0[b] = make_float4(w.m11, w.m12, w.m13, w.m22);
+        B.iw1[b] = make_float2(w.m23, w.m33);
+        float im = B.mat[b].x;
+        B.imv[b] = make_float4(im, im, im, 0.0f);
+    } else {
+        B.iw0[b] = make_float4(0, 0, 0, 0);
+        B.iw1[b] = make_float2(0, 0);
+        B.imv[b] = make_float4(0, 0, 0, 0);
+    }
+}
+
+__global__ void k_unpack_bodies(const jh_body* __restrict__ in, int n, Bodies B) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const jh_body& s = in[b];
+    B.pos[b] = make_float4(s.position[0], s.position[1], s.position[2], 0);
+    B.quat[b] = make_float4(s.orientation[0], s.orientation[1], s.orientation[2], s.orientation[3]);
+    B.vel[b] = make_float4(s.velocity[0], s.velocity[1], s.velocity[2], 0);
+    B.angvel[b] = make_float4(s.angular_velocity[0], s.angular_velocity[1], s.angular_velocity[2], 0);
+    B.dvel[b] = make_float4(s.delta_velocity[0], s.delta_velocity[1], s.delta_velocity[2], 0);
+    B.dangvel[b] = make_float4(s.delta_angular_velocity[0], s.delta_angular_velocity[1], s.delta_angular_velocity[2], 0);
+    B.il0[b] = make_float4(s.inverse_inertia[0], s.inverse_inertia[1], s.inverse_inertia[2], s.inverse_inertia[3]);
+    B.il1[b] = make_float2(s.inverse_inertia[4], s.inverse_inertia[5]);
+    B.mat[b] = make_float4(s.inverse_mass, s.friction, s.restitution, s.linear_damping_multiplier);
+    B.mat2[b] = make_float4(s.angular_damping_multiplier, 0, 0, 0);
+    B.force[b] = make_float4(s.force[0], s.force[1], s.force[2], 0);
+    B.torque[b] = make_float4(s.torque[0], s.torque[1], s.torque[2], 0);
+    B.motion[b] = s.motion_type;
+    B.gravity[b] = s.affected_by_gravity;
+    B.world[b] = s.world_id;
+    update_world_inertia(B, b);
+}
+
+__global__ void k_pack_bodies(Bodies B, int n, jh_body* __restrict__ out) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    jh_body& o = out[b];
+    float4 v;
+    v = B.pos[b]; o.position[0] = v.x; o.position[1] = v.y; o.position[2] = v.z;
+    v = B.quat[b]; o.orientation[0] = v.x; o.orientation[1] = v.y; o.orientation[2] = v.z; o.orientation[3] = v.w;
+    v = B.vel[b]; o.velocity[0] = v.x; o.velocity[1] = v.y; o.velocity[2] = v.z;
+    v = B.angvel[b]; o.angular_velocity[0] = v.x; o.angular_velocity[1] = v.y; o.angular_velocity[2] = v.z;
+    v = B.dvel[b]; o.delta_velocity[0] = v.x; o.delta_velocity[1] = v.y; o.delta_velocity[2] = v.z;
+    v = B.dangvel[b]; o.delta_angular_velocity[0] = v.x; o.delta_angular_velocity[1] = v.y; o.delta_angular_velocity[2] = v.z;
+    v = B.il0[b]; o.inverse_inertia[0] = v.x; o.inverse_inertia[1] = v.y; o.inverse_inertia[2] = v.z; o.inverse_inertia[3] = v.w;
+    float2 w = B.il1[b]; o.inverse_inertia[4] = w.x; o.inverse_inertia[5] = w.y;
+    v = B.mat[b]; o.inverse_mass = v.x; o.friction = v.y; o.restitution = v.z; o.linear_damping_multiplier = v.w;
+    o.angular_damping_multiplier = B.mat2[b].x;
+    v = B.force[b]; o.force[0] = v.x; o.force[1] = v.y; o.force[2] = v.z;
+    v = B.torque[b]; o.torque[0] = v.x; o.torque[1] = v.y; o.torque[2] = v.z;
+    o.motion_type = B.motion[b];
+    o.affected_by_gravity = B.gravity[b];
+    o.world_id = B.world[b];
+    o.reserved = 0;
+}
+
+// RigidBodyData in the reference's 128-byte FP32 layout (RigidBody.cs:107-160).
+__global__ void k_pack_rigidbodydata(Bodies B, int n, char* __restrict__ out, int stride) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    char* r = out + size_t(b) * stride;
+    int* ri = (int*)r;
+    float* rf = (float*)(r + 8);
+    ri[0] = b;
+    ri[1] = 0;
+    float4 v;
+    v = B.pos[b]; rf[0] = v.x; rf[1] = v.y; rf[2] = v.z;
+    v = B.vel[b]; rf[3] = v.x; rf[4] = v.y; rf[5] = v.z;
+    v = B.angvel[b]; rf[6] = v.x; rf[7] = v.y; rf[8] = v.z;
+    v = B.dvel[b]; rf[9] = v.x; rf[10] = v.y; rf[11] = v.z;
+    v = B.dangvel[b]; rf[12] = v.x; rf[13] = v.y; rf[14] = v.z;
+    v = B.quat[b]; rf[15] = v.x; rf[16] = v.y; rf[17] = v.z; rf[18] = v.w;
+    v = B.iw0[b]; rf[19] = v.x; rf[20] = v.y; rf[21] = v.z; rf[22] = v.w;
+    float2 w = B.iw1[b]; rf[23] = w.x; rf[24] = w.y;
+    v = B.imv[b]; rf[25] = v.x; rf[26] = v.y; rf[27] = v.z;
+    ((int*)(rf + 28))[0] = (B.motion[b] & 3) | 4;  // MotionType | IsActive
+}
+
+// ---- Broadphase -----------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// World AABB per shape; bounds[0..2] = min centre (ordered), bounds[3..5] = max centre.
+__global__ void k_aabb(const ShapeRec* __restrict__ shapes, int ns, const float* __restrict__ verts, Bodies B,
+                       float4* __restrict__ amin, float4* __restrict__ amax, uint32_t* __restrict__ bounds) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    vec3 c{0, 0, 0};
+    bool valid = i < ns;
+    if (valid) {
+        ShapeRec s = shapes[i];
+        vec3 mn, mx;
+        shape_aabb(s, verts, ldq(B.quat[s.body]), ld3(B.pos[s.body]), mn, mx);
+        amin[i] = st3(mn);
+        amax[i] = st3(mx);
+        c = 0.5f * (mn + mx);
+    }
+    // block reduction of the centre bounds
+    __shared__ uint32_t red[6];
+    if (threadIdx.x < 6) red[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
+    __syncthreads();
+    if (valid) {
+        atomicMin(&red[0], f2ord(c.x));
+        atomicMin(&red[1], f2ord(c.y));
+        atomicMin(&red[2], f2ord(c.z));
+        atomicMax(&red[3], f2ord(c.x));
+        atomicMax(&red[4], f2ord(c.y));
+        atomicMax(&red[5], f2ord(c.z));
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) atomicMin(&bounds[threadIdx.x], red[threadIdx.x]);
+    else if (threadIdx.x < 6) atomicMax(&bounds[threadIdx.x], red[threadIdx.x]);
+}
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// 64-bit sort key: world id in the high word, 30-bit Morton code of the AABB centre in the low word.
+__global__ void k_morton(const ShapeRec* __restrict__ shapes, int ns, const int* __restrict__ body_world,
+                         const float4* __restrict__ amin, const float4* __restrict__ amax,
+                         const uint32_t* __restrict__ bounds, uint64_t* __restrict__ keys,
+                         uint32_t* __restrict__ vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    float4 a = amin[i], b = amax[i];
+    float lo[3] = {ord2f(bounds[0]), ord2f(bounds[1]), ord2f(bounds[2])};
+    float hi[3] = {ord2f(bounds[3]), ord2f(bounds[4]), ord2f(bounds[5])};
+    float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float ext = hi[k] - lo[k];
+        float t = ext > 0.0f ? (c[k] - lo[k]) / ext : 0.0f;
+        t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
+        q[k] = uint32_t(t);
+    }
+    uint32_t m = (expand_bits10(q[0]) << 2) | (expand_bits10(q[1]) << 1) | expand_bits10(q[2]);
+    keys[i] = (uint64_t(uint32_t(body_world[shapes[i].body])) << 32) | m;
+    vals[i] = uint32_t(i);
+}
+
+// Karras (2012) LBVH over sorted keys; equal keys are split by index.  Children < 0 are leaves (~j).
+__device__ __forceinline__ int lbvh_delta(const uint64_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a == b) return 64 + __clz(uint32_t(i ^ j));
+    return __clzll(a ^ b);
+}
+
+__global__ void k_lbvh_build(const uint64_t* __restrict__ keys, int n, int* __restrict__ left, int* __restrict__ right,
+                             int2* __restrict__ range, int* __restrict__ parent, int* __restrict__ leaf_parent,
+                             int* __restrict__ flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (lbvh_delta(keys, n, i, i + 1) - lbvh_delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = lbvh_delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (lbvh_delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (lbvh_delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = lbvh_delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (lbvh_delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int lo = min(i, j), hi = max(i, j);
+    int lc = (lo == gamma) ? ~gamma : gamma;
+    int rc = (hi == gamma + 1) ? ~(gamma + 1) : (gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    range[i] = make_int2(lo, hi);
+    flags[i] = 0;
+    if (lc < 0) leaf_parent[~lc] = i; else parent[lc] = i;
+    if (rc < 0) leaf_parent[~rc] = i; else parent[rc] = i;
+    if (i == 0) parent[0] = -1;
+}
+
+// Leaf boxes in sorted order, then bottom-up union of child boxes (second arrival continues).
+__global__ void k_lbvh_refit(int n, const uint32_t* __restrict__ order, const float4* __restrict__ amin,
+                             const float4* __restrict__ amax, float4* __restrict__ lmin, float4* __restrict__ lmax,
+                             const int* __restrict__ left, const int* __restrict__ right, const int* __restrict__ parent,
+                             const int* __restrict__ leaf_parent, float4* __restrict__ nmin, float4* __restrict__ nmax,
+                             int* __restrict__ flags) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t s = order[j];
+    lmin[j] = amin[s];
+    lmax[j] = amax[s];
+    if (n == 1) return;
+    __threadfence();
+    int node = leaf_parent[j];
+    while (node >= 0) {
+        // the first child to arrive stops; the second sees both child boxes after the acquire fence
+        if (atomicAdd(&flags[node], 1) == 0) return;
+        __threadfence();
+        int l = left[node], r = right[node];
+        float4 a0 = l < 0 ? lmin[~l] : nmin[l];
+        float4 a1 = l < 0 ? lmax[~l] : nmax[l];
+        float4 b0 = r < 0 ? lmin[~r] : nmin[r];
+        float4 b1 = r < 0 ? lmax[~r] : nmax[r];
+        nmin[node] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0);
+        nmax[node] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0);
+        __threadfence();
+        node = parent[node];
+    }
+}
+
+__device__ __forceinline__ bool box_disjoint(float4 amn, float4 amx, float4 bmn, float4 bmx) {
+    // JBoundingBox.Disjoint (JBoundingBox.cs:314-318)
+    return amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z || amn.z > bmx.z;
+}
+
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* __restrict__ a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+constexpr int kTraverseStack = 96;
+
+// One thread per sorted leaf whose body is dynamic: walks the LBVH and emits each pair of the
+// reference's pair set once (DynamicTree.cs:266-286 + World.cs:447-455 + World.Detect.cs:158-160):
+// different bodies of the same world, at least one dynamic, world AABBs not disjoint.
+__global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restrict__ keys,
+                                                  const uint32_t* __restrict__ order,
+                                                  const ShapeRec* __restrict__ shapes, const int* __restrict__ motion,
+                                                  const float4* __restrict__ lmin, const float4* __restrict__ lmax,
+                                                  const int* __restrict__ left, const int* __restrict__ right,
+                                                  const int2* __restrict__ range, const float4* __restrict__ nmin,
+                                                  const float4* __restrict__ nmax, uint64_t* __restrict__ pairs,
+                                                  int* __restrict__ pair_count, int cap, int* __restrict__ err) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || n < 2) return;
+    uint32_t si = order[i];
+    int bi = shapes[si].body;
+    if (motion[bi] != JH_MOTION_DYNAMIC) return;
+    uint64_t w = keys[i] >> 32;
+    int lo = lower_bound_u64(keys, n, w << 32);
+    int hi = lower_bound_u64(keys, n, (w + 1) << 32);
+    float4 qmn = lmin[i], qmx = lmax[i];
+    int stack[kTraverseStack];
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        int ch[2] = {left[node], right[node]};
+        for (int k = 0; k < 2; ++k) {
+            int c = ch[k];
+            if (c < 0) {
+                int j = ~c;
+                if (j < lo || j >= hi || j == i) continue;
+                if (box_disjoint(qmn, qmx, lmin[j], lmax[j])) continue;
+                uint32_t sj = order[j];
+                int bj = shapes[sj].body;
+                if (bj == bi) continue;
+                if (motion[bj] == JH_MOTION_DYNAMIC && j < i) continue;
+                uint32_t a = min(si, sj), b = max(si, sj);
+                int slot = atomicAdd(pair_count, 1);
+                if (slot < cap) pairs[slot] = (uint64_t(a) << 32) | b;
+            } else {
+                int2 r = range[c];
+                if (r.y < lo || r.x >= hi) continue;
+                if (box_disjoint(qmn, qmx, nmin[c], nmax[c])) continue;
+                if (sp < kTraverseStack) stack[sp++] = c;
+                else atomicOr(err, 1);
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+}
+
+// ---- Narrowphase ----------------------------------------------------------------------------
+struct DeepRec {
+    int pair;
+    int pad;
+    MVertex portal[4];
+};
+
+__device__ __forceinline__ void pair_frame(const ShapeRec* shapes, const float* verts, const Bodies& B, uint64_t key,
+                                           MinkowskiPair& mk, quat& qa, quat& qb, vec3& pa, vec3& pb) {
+    uint32_t s1 = uint32_t(key >> 32), s2 = uint32_t(key & 0xffffffffu);
+    mk.a = ShapeView{&shapes[s1], verts};
+    mk.b = ShapeView{&shapes[s2], verts};
+    int b1 = shapes[s1].body, b2 = shapes[s2].body;
+    qa = ldq(B.quat[b1]);
+    qb = ldq(B.quat[b2]);
+    pa = ld3(B.pos[b1]);
+    pb = ld3(B.pos[b2]);
+    // NarrowPhase.MprEpa (NarrowPhase.cs:910-912): B in A's frame
+    mk.qb = qconjmul(qa, qb);
+    vec3 pos = pb - pa;
+    mk.pb = rot_inv(pos, qa);
+}
+
+__device__ __forceinline__ void write_manifold(jh_manifold& o, uint64_t key, const MprOut& r, quat qa, vec3 posA,
+                                               const ManifoldBuf* m, int epa) {
+    o.shape1 = uint32_t(key >> 32);
+    o.shape2 = uint32_t(key & 0xffffffffu);
+    o.flags = epa;
+    // back to world space (NarrowPhase.cs:920-924)
+    vec3 pa = rot(r.pa, qa) + posA;
+    vec3 pb = rot(r.pb, qa) + posA;
+    vec3 n = rot(r.normal, qa);
+    o.normal[0] = n.x; o.normal[1] = n.y; o.normal[2] = n.z;
+    o.penetration = r.pen;
+    o.point_a[0] = pa.x; o.point_a[1] = pa.y; o.point_a[2] = pa.z;
+    o.point_b[0] = pb.x; o.point_b[1] = pb.y; o.point_b[2] = pb.z;
+    if (!m) {
+        o.count = 0;
+        return;
+    }
+    o.count = m->count;
+    for (int i = 0; i < 4; ++i) {
+        vec3 a = i < m->count ? m->mA[i] : vec3{0, 0, 0};
+        vec3 b = i < m->count ? m->mB[i] : vec3{0, 0, 0};
+        o.manifold_a[i][0] = a.x; o.manifold_a[i][1] = a.y; o.manifold_a[i][2] = a.z;
+        o.manifold_b[i][0] = b.x; o.manifold_b[i][1] = b.y; o.manifold_b[i][2] = b.z;
+    }
+}
+
+__device__ __forceinline__ void manifold_for(const MinkowskiPair& mk, quat qa, quat qb, vec3 posA, vec3 posB,
+                                             const MprOut& r, int aux, ManifoldBuf& m) {
+    vec3 pa = rot(r.pa, qa) + posA;
+    vec3 pb = rot(r.pb, qa) + posA;
+    vec3 n = rot(r.normal, qa);
+    if (aux) {
+        build_manifold(mk.a, mk.b, qa, qb, posA, posB, pa, pb, n, m);
+    } else {
+        m.count = 1;
+        m.mA[0] = pa;
+        m.mB[0] = pb;
+    }
+}
+
+// World.Detect (World.Detect.cs:125-218) per broadphase pair: MPR, then the manifold; pairs deeper
+// than EpaPenetrationThreshold (NarrowPhase.cs:877) go to the EPA queue.
+__global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
+                                                const ShapeRec* __restrict__ shapes, const float* __restrict__ verts,
+                                                Bodies B, int aux, jh_manifold* __restrict__ out,
+                                                DeepRec* __restrict__ deep, int* __restrict__ deep_count) {
+    int n = *n_ptr;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint64_t key = pairs[p];
+        MinkowskiPair mk;
+        quat qa, qb;
+        vec3 pa, pb;
+        pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
+        MprOut r;
+        solve_mpr(mk, 0.02f, r);
+        if (!r.hit) {
+            write_manifold(out[p], key, r, qa, pa, nullptr, 0);
+            continue;
+        }
+        if (r.deep) {
+            int slot = atomicAdd(deep_count, 1);
+            DeepRec& d = deep[slot];
+            d.pair = p;
+            for (int k = 0; k < 4; ++k) d.portal[k] = r.portal[k];
+        }
+        ManifoldBuf m;
+        manifold_for(mk, qa, qb, pa, pb, r, aux, m);
+        write_manifold(out[p], key, r, qa, pa, &m, 0);
+    }
+}
+
+// EPA over the deep queue: a successful EPA replaces the MPR answer and its manifold.
+__global__ void __launch_bounds__(64) k_epa(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
+                                            const float* __restrict__ verts, Bodies B, int aux,
+                                            jh_manifold* __restrict__ out, const DeepRec* __restrict__ deep,
+                                            const int* __restrict__ deep_count, epa::Work* __restrict__ work) {
+    int n = *deep_count;
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    epa::Work& w = work[tid];
+    for (int q = tid; q < n; q += gridDim.x * blockDim.x) {
+        const DeepRec& d = deep[q];
+        uint64_t key = pairs[d.pair];
+        MinkowskiPair mk;
+        quat qa, qb;
+        vec3 pa, pb;
+        pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
+        for (int k = 0; k < 4; ++k) w.v[k] = d.portal[k];
+        MprOut r;
+        r.hit = 1;
+        if (!epa::solve(w, mk, r.pa, r.pb, r.normal, r.pen)) continue;
+        ManifoldBuf m;
+        manifold_for(mk, qa, qb, pa, pb, r, aux, m);
+        write_manifold(out[d.pair], key, r, qa, pa, &m, 1);
+    }
+}
+
+// ---- Contact cache --------------------------------------------------------------------------
+__global__ void k_collide_flags(const jh_manifold* __restrict__ man, const int* __restrict__ n_ptr,
+                                uint32_t* __restrict__ flags) {
+    int n = *n_ptr;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        flags[p] = man[p].count > 0 ? 1u : 0u;
+}
+
+// Colliding pairs (in key order) and whether their arbiter exists (binary search of the sorted cache).
+__global__ void k_collide_compact(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
+                                  const uint32_t* __restrict__ flags, const uint32_t* __restrict__ offs,
+                                  const uint64_t* __restrict__ akey, const int* __restrict__ a_count,
+                                  uint64_t* __restrict__ ckey, int* __restrict__ cpair, int* __restrict__ cfound,
+                                  uint32_t* __restrict__ cnew) {
+    int n = *n_ptr;
+    int A = *a_count;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        if (!flags[p]) continue;
+        int c = offs[p];
+        uint64_t k = pairs[p];
+        ckey[c] = k;
+        cpair[c] = p;
+        int f = lower_bound_u64(akey, A, k);
+        bool found = f < A && akey[f] == k;
+        cfound[c] = found ? f : -1;
+        cnew[c] = found ? 0u : 1u;
+    }
+}
+
+__global__ void k_new_compact(const uint64_t* __restrict__ ckey, const int* __restrict__ c_count,
+                              const uint32_t* __restrict__ cnew, const uint32_t* __restrict__ noffs,
+                              uint64_t* __restrict__ nkey) {
+    int n = *c_count;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+        if (cnew[c]) nkey[noffs[c]] = ckey[c];
+}
+
+// Merge the key-sorted cache with the key-sorted new keys into the other buffer.
+__global__ void k_merge_existing(const jh_arbiter* __restrict__ ain, const uint64_t* __restrict__ akey,
+                                 const int* __restrict__ a_count, const uint64_t* __restrict__ nkey,
+                                 const int* __restrict__ n_new, jh_arbiter* __restrict__ aout,
+                                 uint64_t* __restrict__ okey) {
+    int A = *a_count, N = *n_new;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) {
+        int pos = i + lower_bound_u64(nkey, N, akey[i]);
+        aout[pos] = ain[i];
+        okey[pos] = akey[i];
+    }
+}
+
+// ContactData.Init (Contact.cs:271-284) for each new arbiter; memContacts.Allocate(clear: true).
+__global__ void k_merge_new(const uint64_t* __restrict__ akey, const int* __restrict__ a_count,
+                            const uint64_t* __restrict__ nkey, const int* __restrict__ n_new,
+                            const ShapeRec* __restrict__ shapes, Bodies B, float specrel,
+                            jh_arbiter* __restrict__ aout, uint64_t* __restrict__ okey, int* __restrict__ out_count) {
+    int A = *a_count, N = *n_new;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+        uint64_t k = nkey[j];
+        int pos = j + lower_bound_u64(akey, A, k);
+        jh_arbiter a;
+        memset(&a, 0, sizeof(a));
+        a.shape1 = uint32_t(k >> 32);
+        a.shape2 = uint32_t(k & 0xffffffffu);
+        a.body1 = shapes[a.shape1].body;
+        a.body2 = shapes[a.shape2].body;
+        float4 m1 = B.mat[a.body1], m2 = B.mat[a.body2];
+        a.friction = fmaxr(m1.y, m2.y);
+        a.restitution = fmaxr(m1.z, m2.z);
+        a.speculative_relaxation_factor = specrel;
+        aout[pos] = a;
+        okey[pos] = k;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = A + N;
+}
+
+// Contact.Initialize (Contact.cs:703-759)
+__device__ __forceinline__ void contact_initialize(jh_arbiter& cd, jh_contact& c, const Bodies& B, vec3 p1, vec3 p2,
+                                                   vec3 n, bool is_new) {
+    vec3 x1 = ld3(B.pos[cd.body1]), x2 = ld3(B.pos[cd.body2]);
+    quat q1 = ldq(B.quat[cd.body1]), q2 = ldq(B.quat[cd.body2]);
+    vec3 rel1 = p1 - x1, rel2 = p2 - x2;
+    vec3 pos1 = rot_inv(rel1, q1), pos2 = rot_inv(rel2, q2);
+    c.relative_position1[0] = rel1.x; c.relative_position1[1] = rel1.y; c.relative_position1[2] = rel1.z;
+    c.relative_position2[0] = rel2.x; c.relative_position2[1] = rel2.y; c.relative_position2[2] = rel2.z;
+    c.position1[0] = pos1.x; c.position1[1] = pos1.y; c.position1[2] = pos1.z;
+    c.position2[0] = pos2.x; c.position2[1] = pos2.y; c.position2[2] = pos2.z;
+    if (!is_new) return;
+    c.flag = 2u;  // NewContact
+    c.accumulated[0] = c.accumulated[1] = c.accumulated[2] = 0.0f;
+    vec3 v1 = ld3(B.vel[cd.body1]), v2 = ld3(B.vel[cd.body2]);
+    vec3 dv = v2 - v1;
+    if (cd.mode == 0 || (cd.mode & (JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2)) != 0) {
+        vec3 w1 = ld3(B.angvel[cd.body1]), w2 = ld3(B.angvel[cd.body2]);
+        dv = v2 + cross(w2, rel2);
+        dv = dv - (v1 + cross(w1, rel1));
+    }
+    float rel_nv = dot(dv, n);
+    c.bias = 0.0f;
+    if (rel_nv < -1.0f) c.bias = -cd.restitution * rel_nv;
+    vec3 t1 = dv - n * rel_nv;
+    float num = len_sq(t1);
+    if (num > 1e-12f) {
+        num = 1.0f / sqrtf(num);
+        t1 = t1 * num;
+    } else {
+        t1 = create_orthonormal(n);
+    }
+    vec3 t2 = cross(t1, n);
+    c.normal[0] = n.x; c.normal[1] = n.y; c.normal[2] = n.z;
+    c.tangent1[0] = t1.x; c.tangent1[1] = t1.y; c.tangent1[2] = t1.z;
+    c.tangent2[0] = t2.x; c.tangent2[1] = t2.y; c.tangent2[2] = t2.z;
+}
+
+__device__ __forceinline__ vec3 rel1_of(const jh_contact& c) {
+    return {c.relative_position1[0], c.relative_position1[1], c.relative_position1[2]};
+}
+
+// CalcArea4Points (Contact.cs:418-432) as evaluated lane-wise by the vectorized selector (:469-527).
+__device__ __forceinline__ float area4(vec3 p0, vec3 p1, vec3 p2, vec3 p3) {
+    vec3 a0 = p0 - p1, a1 = p0 - p2, a2 = p0 - p3, b0 = p2 - p3, b1 = p1 - p3, b2 = p1 - p2;
+    float c0 = len_sq(cross(a0, b0)), c1 = len_sq(cross(a1, b1)), c2 = len_sq(cross(a2, b2));
+    return fmaxr(fmaxr(c0, c1), c2);
+}
+
+// ContactData.AddContact (Contact.cs:329-416) with SortCachedPoints (:529-553).
+__device__ __forceinline__ void add_contact(jh_arbiter& cd, const Bodies& B, vec3 p1, vec3 p2, vec3 n) {
+    if ((cd.usage_mask & 15u) == 15u) {
+        vec3 rp1 = p1 - ld3(B.pos[cd.body1]);
+        vec3 r0 = rel1_of(cd.contacts[0]), r1 = rel1_of(cd.contacts[1]), r2 = rel1_of(cd.contacts[2]),
+             r3 = rel1_of(cd.contacts[3]);
+        float a0 = area4(rp1, r1, r2, r3), a1 = area4(rp1, r0, r2, r3), a2 = area4(rp1, r0, r1, r3),
+              a3 = area4(rp1, r0, r1, r2);
+        const float eps = -0.0001f;
+        float big = 0.0f;
+        uint32_t mask = 0;
+        if (a0 > big + eps) { big = a0; mask = 1; }
+        if (a1 > big + eps) { big = a1; mask = 2; }
+        if (a2 > big + eps) { big = a2; mask = 4; }
+        if (a3 > big + eps) { mask = 8; }
+        int slot = mask == 2 ? 1 : mask == 4 ? 2 : mask == 8 ? 3 : 0;
+        contact_initialize(cd, cd.contacts[slot], B, p1, p2, n, false);
+        cd.usage_mask |= mask;
+        return;
+    }
+    int closest = -1;
+    float dsq = 3.402823466e+38f;
+    vec3 rel = p1 - ld3(B.pos[cd.body1]);
+    for (int i = 0; i < 4; ++i) {
+        if ((cd.usage_mask & (1u << i)) == 0) continue;
+        float d = len_sq(rel1_of(cd.contacts[i]) - rel);
+        if (d < dsq) {
+            dsq = d;
+            closest = i;
+        }
+    }
+    if (dsq < 0.02f * 0.02f) {
+        contact_initialize(cd, cd.contacts[closest], B, p1, p2, n, false);
+        return;
+    }
+    for (int i = 0; i < 4; ++i) {
+        if ((cd.usage_mask & (1u << i)) == 0) {
+            contact_initialize(cd, cd.contacts[i], B, p1, p2, n, true);
+            cd.usage_mask |= (1u << i);
+            return;
+        }
+    }
+}
+
+// World.RegisterContact (World.Detect.cs:294-324) for each colliding pair.
+__global__ void k_register(const uint64_t* __restrict__ ckey, const int* __restrict__ cpair,
+                           const int* __restrict__ cfound, const uint32_t* __restrict__ noffs,
+                           const int* __restrict__ c_count, const uint64_t* __restrict__ akey_old,
+                           const int* __restrict__ a_count_old, const uint64_t* __restrict__ nkey,
+                           const int* __restrict__ n_new, const jh_manifold* __restrict__ man, Bodies B,
+                           int persistent, jh_arbiter* __restrict__ arb) {
+    int n = *c_count, A = *a_count_old, N = *n_new;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        uint64_t k = ckey[c];
+        int f = cfound[c];
+        int pos = f >= 0 ? f + lower_bound_u64(nkey, N, k) : int(noffs[c]) + lower_bound_u64(akey_old, A, k);
+        jh_arbiter cd = arb[pos];
+        const jh_manifold& m = man[cpair[c]];
+        if (!persistent) cd.usage_mask = 0;
+        // ContactData.ResetMode (Contact.cs:291-296)
+        cd.mode = (B.motion[cd.body1] == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1) : 0;
+        cd.mode |= (B.motion[cd.body2] == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2) : 0;
+        vec3 n3{m.normal[0], m.normal[1], m.normal[2]};
+        for (int e = 0; e < m.count; ++e) {
+            vec3 a{m.manifold_a[e][0], m.manifold_a[e][1], m.manifold_a[e][2]};
+            vec3 b{m.manifold_b[e][0], m.manifold_b[e][1], m.manifold_b[e][2]};
+            add_contact(cd, B, a, b, n3);
+        }
+        arb[pos] = cd;
+    }
+}
+
+// ---- Graph colouring (Jones-Plassmann order, first-fit colours) -----------------------------
+constexpr int kMaxColours = 127;  // colour 127 = serial overflow batch
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Jones-Plassmann priority: hashed key in the high 40 bits, arbiter index in the low 24 (unique).
+__device__ __forceinline__ unsigned long long colour_prio(const jh_arbiter& r, int a) {
+    uint64_t h = mix64((uint64_t(r.shape1) << 32) | r.shape2);
+    return (h & 0xffffffffff000000ull) | (uint64_t(a) & 0xffffffull);
+}
+
+struct ColourArgs {
+    const jh_arbiter* arb;
+    const int* a_count;
+    const int* motion;
+    int* colour;
+    unsigned long long* body_max;
+    unsigned long long* mask_lo;
+    unsigned long long* mask_hi;
+    int* remaining;  // [2]
+    int* ncolours;
+    int* hist;       // [128]
+};
+
+__global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
+    cg::grid_group grid = cg::this_grid();
+    int A = *P.a_count;
+    int stride = gridDim.x * blockDim.x;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int a = t0; a < A; a += stride) {
+        const jh_arbiter& r = P.arb[a];
+        bool active = (r.usage_mask & 15u) != 0;
+        P.colour[a] = active ? -1 : -2;
+        P.body_max[r.body1] = 0;
+        P.body_max[r.body2] = 0;
+        P.mask_lo[r.body1] = 0;
+        P.mask_lo[r.body2] = 0;
+        P.mask_hi[r.body1] = 0;
+        P.mask_hi[r.body2] = 0;
+    }
+    if (t0 < 128) P.hist[t0] = 0;
+    if (t0 == 0) { P.remaining[0] = 0; P.remaining[1] = 0; *P.ncolours = 0; }
+    grid.sync();
+    for (int round = 0;; ++round) {
+        for (int a = t0; a < A; a += stride) {
+            if (P.colour[a] != -1) continue;
+            const jh_arbiter& r = P.arb[a];
+            unsigned long long prio = colour_prio(r, a);
+            if (P.motion[r.body1] == JH_MOTION_DYNAMIC) atomicMax(&P.body_max[r.body1], prio);
+            if (P.motion[r.body2] == JH_MOTION_DYNAMIC) atomicMax(&P.body_max[r.body2], prio);
+        }
+        __threadfence();
+        grid.sync();
+        int* rem = &P.remaining[round & 1];
+        for (int a = t0; a < A; a += stride) {
+            if (P.colour[a] != -1) continue;
+            const jh_arbiter& r = P.arb[a];
+            unsigned long long prio = colour_prio(r, a);
+            bool d1 = P.motion[r.body1] == JH_MOTION_DYNAMIC, d2 = P.motion[r.body2] == JH_MOTION_DYNAMIC;
+            bool win = (!d1 || P.body_max[r.body1] == prio) && (!d2 || P.body_max[r.body2] == prio);
+            if (!win) {
+                atomicAdd(rem, 1);
+                continue;
+            }
+            unsigned long long lo = 0, hi = 0;
+            if (d1) { lo |= P.mask_lo[r.body1]; hi |= P.mask_hi[r.body1]; }
+            if (d2) { lo |= P.mask_lo[r.body2]; hi |= P.mask_hi[r.body2]; }
+            int c;
+            if (~lo) c = __ffsll((long long)~lo) - 1;
+            else if (~hi) c = 64 + __ffsll((long long)~hi) - 1;
+            else c = kMaxColours;
+            if (c >= kMaxColours) c = kMaxColours;
+            else if (c < 64) {
+                if (d1) P.mask_lo[r.body1] |= (1ull << c);
+                if (d2) P.mask_lo[r.body2] |= (1ull << c);
+            } else {
+                if (d1) P.mask_hi[r.body1] |= (1ull << (c - 64));
+                if (d2) P.mask_hi[r.body2] |= (1ull << (c - 64));
+            }
+            P.colour[a] = c;
+            atomicAdd(&P.hist[c], 1);
+        }
+        __threadfence();
+        grid.sync();
+        int left = atomicAdd(rem, 0);
+        if (t0 == 0) P.remaining[(round + 1) & 1] = 0;
+        for (int a = t0; a < A; a += stride) {
+            if (P.colour[a] != -1) continue;
+            const jh_arbiter& r = P.arb[a];
+            P.body_max[r.body1] = 0;
+            P.body_max[r.body2] = 0;
+        }
+        grid.sync();
+        if (left == 0) break;
+    }
+    if (t0 == 0) {
+        int nc = 0;
+        for (int c = 0; c < kMaxColours; ++c)
+            if (P.hist[c] > 0) nc = c + 1;
+        *P.ncolours = nc;
+    }
+}
+
+// Sort keys for the colour batches: (colour << 32) | arbiter index; inactive arbiters sort last.
+// Also counts the contact points that the solve will visit.
+__global__ void k_colour_keys(const int* __restrict__ colour, const jh_arbiter* __restrict__ arb,
+                              const int* __restrict__ a_count, uint64_t* __restrict__ keys, int* __restrict__ contacts) {
+    int A = *a_count;
+    int local = 0;
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < A; a += gridDim.x * blockDim.x) {
+        int c = colour[a];
+        uint32_t cc = c < 0 ? 255u : uint32_t(c);
+        keys[a] = (uint64_t(cc) << 32) | uint32_t(a);
+        if (c >= 0) local += __popc(arb[a].usage_mask & 15u);
+    }
+    if (local) atomicAdd(contacts, local);
+}
+
+__global__ void k_colour_offsets(const int* __restrict__ hist, int* __restrict__ offs) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int s = 0;
+        for (int c = 0; c < 128; ++c) {
+            offs[c] = s;
+            s += hist[c];
+        }
+        offs[128] = s;
+    }
+}
+
+// ---- Solver ---------------------------------------------------------------------------------
+// Contact.PrepareForIterationAccelerated (Contact.cs:1038-1180), no linear locks.
+__device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact& c, const Bodies& B,
+                                                const StepTime& ts) {
+    int b1 = cd.body1, b2 = cd.body2;
+    uint32_t mode = cd.mode;
+    bool skip = (c.flag & 4u) != 0;
+    c.flag &= ~4u;
+    float ws[3] = {skip ? 0.0f : c.accumulated[0], skip ? 0.0f : c.accumulated[1], skip ? 0.0f : c.accumulated[2]};
+    vec3 pos1{c.position1[0], c.position1[1], c.position1[2]};
+    vec3 pos2{c.position2[0], c.position2[1], c.position2[2]};
+    vec3 x1 = ld3(B.pos[b1]), x2 = ld3(B.pos[b2]);
+    vec3 cr1 = rot(pos1, ldq(B.quat[b1]));
+    vec3 cr2 = rot(pos2, ldq(B.quat[b2]));
+    vec3 imv{0.0f, 0.0f, 0.0f};
+    if (mode & JH_SOLVE_LINEAR1) imv = ld3(B.imv[b1]);
+    if (mode & JH_SOLVE_LINEAR2) imv = imv + ld3(B.imv[b2]);
+    float k[3] = {imv.x, imv.x, imv.x};
+    if ((c.flag & 2u) == 0) c.bias = 0.0f;
+    vec3 r1 = cr1 + x1, r2 = cr2 + x2;
+    vec3 dist = r1 - r2;
+    float NX[3] = {c.normal[0], c.tangent1[0], c.tangent2[0]};
+    float NY[3] = {c.normal[1], c.tangent1[1], c.tangent2[1]};
+    float NZ[3] = {c.normal[2], c.tangent1[2], c.tangent2[2]};
+    vec3 n{NX[0], NY[0], NZ[0]};
+    float pen = dot(dist, n);
+    if (pen < -0.02f) c.bias = pen * ts.inv_dt * cd.speculative_relaxation_factor;
+    c.flag &= ~2u;
+    float ktx[3] = {0, 0, 0}, kty[3] = {0, 0, 0}, ktz[3] = {0, 0, 0};
+    vec3 lin{ws[0] * NX[0] + ws[1] * NX[1] + ws[2] * NX[2], ws[0] * NY[0] + ws[1] * NY[1] + ws[2] * NY[2],
+             ws[0] * NZ[0] + ws[1] * NZ[1] + ws[2] * NZ[2]};
+    if (mode & JH_SOLVE_LINEAR1) {
+        vec3 v = ld3(B.vel[b1]) - cmul(lin, ld3(B.imv[b1]));
+        B.vel[b1] = st3(v);
+    }
+    if (mode & JH_SOLVE_ANGULAR1) {
+        vec3 rp{c.relative_position1[0], c.relative_position1[1], c.relative_position1[2]};
+        sym3 I = ld_iw(B, b1);
+        float e1[3], e2[3], e3[3];
+        for (int l = 0; l < 3; ++l) {
+            float rrx = rp.y * NZ[l] - rp.z * NY[l];
+            float rry = rp.z * NX[l] - rp.x * NZ[l];
+            float rrz = rp.x * NY[l] - rp.y * NX[l];
+            e1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+            e2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+            e3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+        }
+        vec3 ai{ws[0] * e1[0] + ws[1] * e1[1] + ws[2] * e1[2], ws[0] * e2[0] + ws[1] * e2[1] + ws[2] * e2[2],
+                ws[0] * e3[0] + ws[1] * e3[1] + ws[2] * e3[2]};
+        B.angvel[b1] = st3(ld3(B.angvel[b1]) - ai);
+        for (int l = 0; l < 3; ++l) {
+            ktx[l] = e2[l] * rp.z - e3[l] * rp.y;
+            kty[l] = e3[l] * rp.x - e1[l] * rp.z;
+            ktz[l] = e1[l] * rp.y - e2[l] * rp.x;
+        }
+    }
+    if (mode & JH_SOLVE_LINEAR2) {
+        vec3 v = ld3(B.vel[b2]) + cmul(lin, ld3(B.imv[b2]));
+        B.vel[b2] = st3(v);
+    }
+    if (mode & JH_SOLVE_ANGULAR2) {
+        vec3 rp{c.relative_position2[0], c.relative_position2[1], c.relative_position2[2]};
+        sym3 I = ld_iw(B, b2);
+        float f1[3], f2[3], f3[3];
+        for (int l = 0; l < 3; ++l) {
+            float rrx = rp.y * NZ[l] - rp.z * NY[l];
+            float rry = rp.z * NX[l] - rp.x * NZ[l];
+            float rrz = rp.x * NY[l] - rp.y * NX[l];
+            f1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+            f2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+            f3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+        }
+        vec3 ai{ws[0] * f1[0] + ws[1] * f1[1] + ws[2] * f1[2], ws[0] * f2[0] + ws[1] * f2[1] + ws[2] * f2[2],
+                ws[0] * f3[0] + ws[1] * f3[1] + ws[2] * f3[2]};
+        B.angvel[b2] = st3(ld3(B.angvel[b2]) + ai);
+        for (int l = 0; l < 3; ++l) {
+            ktx[l] = ktx[l] + (f2[l] * rp.z - f3[l] * rp.y);
+            kty[l] = kty[l] + (f3[l] * rp.x - f1[l] * rp.z);
+            ktz[l] = ktz[l] + (f1[l] * rp.y - f2[l] * rp.x);
+        }
+    }
+    for (int l = 0; l < 3; ++l) {
+        float kres = NX[l] * ktx[l] + NY[l] * kty[l] + NZ[l] * ktz[l];
+        float kk = k[l] + kres;
+        c.mass_normal_tangent[l] = kk > 0.0f ? 1.0f / kk : 0.0f;
+    }
+    float pb = 0.2f * ts.inv_dt * fmaxr(0.0f, pen - 0.01f);
+    c.penalty_bias = fminr(pb, 100.0f);
+}
+
+// Contact.IterateAccelerated (Contact.cs:1183-1248)
+__device__ __forceinline__ void iterate_contact(const jh_arbiter& cd, jh_contact& c, const Bodies& B, bool apply_bias) {
+    int b1 = cd.body1, b2 = cd.body2;
+    uint32_t mode = cd.mode;
+    vec3 v1 = ld3(B.vel[b1]), v2 = ld3(B.vel[b2]);
+    vec3 rel1{c.relative_position1[0], c.relative_position1[1], c.relative_position1[2]};
+    vec3 rel2{c.relative_position2[0], c.relative_position2[1], c.relative_position2[2]};
+    vec3 dv = v2 - v1;
+    if (mode & (JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2)) {
+        vec3 w1 = ld3(B.angvel[b1]), w2 = ld3(B.angvel[b2]);
+        dv = v2 + cross(w2, rel2);
+        dv = dv - (v1 + cross(w1, rel1));
+    }
+    float bias = c.bias;
+    if (apply_bias && c.penalty_bias > 0) bias = fmaxr(c.penalty_bias, c.bias);
+    float NX[3] = {c.normal[0], c.tangent1[0], c.tangent2[0]};
+    float NY[3] = {c.normal[1], c.tangent1[1], c.tangent2[1]};
+    float NZ[3] = {c.normal[2], c.tangent1[2], c.tangent2[2]};
+    float old[3] = {c.accumulated[0], c.accumulated[1], c.accumulated[2]};
+    float bv[3] = {bias, 0.0f, 0.0f};
+    float imp[3];
+    for (int l = 0; l < 3; ++l) {
+        float vd = NX[l] * dv.x + NY[l] * dv.y + NZ[l] * dv.z;
+        imp[l] = c.mass_normal_tangent[l] * (bv[l] - vd);
+    }
+    float maxT = cd.friction * old[0];
+    float lo[3] = {0.0f, -maxT, -maxT}, hi[3] = {3.402823466e+38f, maxT, maxT};
+    for (int l = 0; l < 3; ++l) {
+        float a = old[l] + imp[l];
+        a = fminr(fmaxr(a, lo[l]), hi[l]);
+        c.accumulated[l] = a;
+        imp[l] = a - old[l];
+    }
+    vec3 lin{imp[0] * NX[0] + imp[1] * NX[1] + imp[2] * NX[2], imp[0] * NY[0] + imp[1] * NY[1] + imp[2] * NY[2],
+             imp[0] * NZ[0] + imp[1] * NZ[1] + imp[2] * NZ[2]};
+    if (mode & JH_SOLVE_LINEAR1) B.vel[b1] = st3(v1 - cmul(lin, ld3(B.imv[b1])));
+    if (mode & JH_SOLVE_ANGULAR1) B.angvel[b1] = st3(ld3(B.angvel[b1]) - sym_mul(cross(rel1, lin), ld_iw(B, b1)));
+    if (mode & JH_SOLVE_LINEAR2) B.vel[b2] = st3(v2 + cmul(lin, ld3(B.imv[b2])));
+    if (mode & JH_SOLVE_ANGULAR2) B.angvel[b2] = st3(ld3(B.angvel[b2]) + sym_mul(cross(rel2, lin), ld_iw(B, b2)));
+}
+
+struct SolveArgs {
+    jh_arbiter* arb;
+    const uint64_t* order;   // (colour << 32) | arbiter, sorted
+    const int* offs;         // 129 colour offsets
+    const int* ncolours;
+    Bodies B;
+    int nb;
+    StepTime ts;
+    int iterations, relaxations;
+};
+
+__device__ __forceinline__ void solve_batches(const SolveArgs& P, cg::grid_group& grid, int phase, int nc) {
+    int stride = gridDim.x * blockDim.x;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int c = 0; c <= nc; ++c) {
+        int col = c < nc ? c : kMaxColours;
+        int beg = P.offs[col], end = P.offs[col + 1];
+        if (beg == end) continue;
+        if (col == kMaxColours) {
+            if (t0 == 0)
+                for (int k = beg; k < end; ++k) {
+                    jh_arbiter& a = P.arb[uint32_t(P.order[k])];
+                    for (int s = 0; s < 4; ++s)
+                        if (a.usage_mask & (1u << s)) {
+                            if (phase == 0) prepare_contact(a, a.contacts[s], P.B, P.ts);
+                            else iterate_contact(a, a.contacts[s], P.B, phase == 1);
+                        }
+                }
+        } else {
+            for (int k = beg + t0; k < end; k += stride) {
+                jh_arbiter& a = P.arb[uint32_t(P.order[k])];
+                for (int s = 0; s < 4; ++s)
+                    if (a.usage_mask & (1u << s)) {
+                        if (phase == 0) prepare_contact(a, a.contacts[s], P.B, P.ts);
+                        else iterate_contact(a, a.contacts[s], P.B, phase == 1);
+                    }
+            }
+        }
+        __threadfence();
+        grid.sync();
+    }
+}
+
+// One launch per step: for each substep IntegrateForces, SolveVelocities (prepare + iterations),
+// IntegrateVelocities, RelaxVelocities (World.Step.cs:234-253), colour batch by colour batch.
+__global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
+    cg::grid_group grid = cg::this_grid();
+    int stride = gridDim.x * blockDim.x;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    int nc = *P.ncolours;
+    const Bodies& B = P.B;
+    for (int s = 0; s < P.ts.substeps; ++s) {
+        // IntegrateForces (World.Step.cs:1060-1072)
+        for (int b = t0; b < P.nb; b += stride) {
+            if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
+            B.angvel[b] = st3(ld3(B.angvel[b]) + ld3(B.dangvel[b]));
+            B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
+        }
+        __threadfence();
+        grid.sync();
+        solve_batches(P, grid, 0, nc);
+        for (int it = 0; it < P.iterations; ++it) solve_batches(P, grid, 1, nc);
+        // IntegrateVelocities (World.Step.cs:1106-1133)
+        for (int b = t0; b < P.nb; b += stride) {
+            if (B.motion[b] == JH_MOTION_STATIC) continue;
+            vec3 p = ld3(B.pos[b]) + ld3(B.vel[b]) * P.ts.substep_dt;
+            B.pos[b] = st3(p);
+            quat rq = rotation_quaternion(ld3(B.angvel[b]), P.ts.substep_dt);
+            B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+        }
+        __threadfence();
+        grid.sync();
+        for (int it = 0; it < P.relaxations; ++it) solve_batches(P, grid, 2, nc);
+    }
+}
+
+// ---- End of step ----------------------------------------------------------------------------
+// ContactData.UpdatePosition (Contact.cs:238-264, 762-797); flags arbiters that keep contacts.
+__global__ void k_update_contacts(jh_arbiter* __restrict__ arb, const int* __restrict__ a_count, Bodies B,
+                                  uint32_t* __restrict__ keep) {
+    int A = *a_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) {
+        jh_arbiter& cd = arb[i];
+        uint32_t usage = cd.usage_mask & 15u;
+        usage |= usage << 4;
+        vec3 x1 = ld3(B.pos[cd.body1]), x2 = ld3(B.pos[cd.body2]);
+        quat q1 = ldq(B.quat[cd.body1]), q2 = ldq(B.quat[cd.body2]);
+        for (int s = 0; s < 4; ++s) {
+            if ((usage & (1u << s)) == 0) continue;
+            jh_contact& c = cd.contacts[s];
+            vec3 rel1 = rot(vec3{c.position1[0], c.position1[1], c.position1[2]}, q1);
+            vec3 p1 = rel1 + x1;
+            vec3 rel2 = rot(vec3{c.position2[0], c.position2[1], c.position2[2]}, q2);
+            vec3 p2 = rel2 + x2;
+            c.relative_position1[0] = rel1.x; c.relative_position1[1] = rel1.y; c.relative_position1[2] = rel1.z;
+            c.relative_position2[0] = rel2.x; c.relative_position2[1] = rel2.y; c.relative_position2[2] = rel2.z;
+            vec3 dist = p1 - p2;
+            vec3 n{c.normal[0], c.normal[1], c.normal[2]};
+            float pen = dot(dist, n);
+            bool ok = true;
+            if (pen < -0.02f * 0.1f) ok = false;
+            else {
+                dist = dist - pen * n;
+                if (len_sq(dist) > 0.02f * 0.02f) ok = false;
+            }
+            if (!ok) usage &= ~(1u << s);
+        }
+        cd.usage_mask = usage;
+        keep[i] = (usage & 15u) ? 1u : 0u;
+    }
+}
+
+__global__ void k_compact_arbiters(const jh_arbiter* __restrict__ ain, const uint64_t* __restrict__ akey,
+                                   const int* __restrict__ a_count, const uint32_t* __restrict__ keep,
+                                   const uint32_t* __restrict__ offs, jh_arbiter* __restrict__ aout,
+                                   uint64_t* __restrict__ okey) {
+    int A = *a_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) {
+        if (!keep[i]) continue;
+        aout[offs[i]] = ain[i];
+        okey[offs[i]] = akey[i];
+    }
+}
+
+// RigidBody.Update (RigidBody.cs:716-764) for dynamic bodies, no motion locks or sleeping.
+__global__ void k_update_bodies(Bodies B, int nb, StepTime ts, float gx, float gy, float gz) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb || B.motion[b] != JH_MOTION_DYNAMIC) return;
+    update_world_inertia(B, b);
+    float4 m = B.mat[b];
+    B.angvel[b] = st3(ld3(B.angvel[b]) * B.mat2[b].x);
+    B.vel[b] = st3(ld3(B.vel[b]) * m.w);
+    vec3 dv = cmul(ld3(B.force[b]), ld3(B.imv[b])) * ts.substep_dt;
+    vec3 dw = sym_mul(ld3(B.torque[b]), ld_iw(B, b)) * ts.substep_dt;
+    if (B.gravity[b]) dv = dv + vec3{gx, gy, gz} * ts.substep_dt;
+    B.dvel[b] = st3(dv);
+    B.dangvel[b] = st3(dw);
+    B.force[b] = make_float4(0, 0, 0, 0);
+    B.torque[b] = make_float4(0, 0, 0, 0);
+}
+
+}  // namespace jh

@@ -1,0 +1,788 @@
+// libjitterhip: C-ABI (include/jitterhip.h) over the gfx950 kernels.  One jh_world owns its device
+// buffers and one HIP stream; jh_world_step enqueues the whole step and synchronises once in the
+// middle (pair count, to size the following buffers) and once at the end (counters, timings).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "jh_kernels.h"
+#include "jh_scan.h"
+#include "jh_sort.h"
+
+using namespace jh;
+
+namespace {
+
+thread_local std::string g_err = "";
+
+jh_status fail(jh_status s, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(JH_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));         \
+    } while (0)
+
+template <class T> struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n, bool keep = false) {
+        if (n <= cap && p) return hipSuccess;
+        size_t nc = std::max<size_t>(n + n / 4, 64);
+        T* q = nullptr;
+        hipError_t e = hipMalloc((void**)&q, nc * sizeof(T));
+        if (e != hipSuccess) return e;
+        if (keep && p && cap) {
+            e = hipMemcpy(q, p, cap * sizeof(T), hipMemcpyDeviceToDevice);
+            if (e != hipSuccess) return e;
+        }
+        if (p) (void)hipFree(p);
+        p = q;
+        cap = nc;
+        return hipSuccess;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Device counters (ints).
+enum {
+    C_PAIRS = 0,
+    C_DEEP = 1,
+    C_COLL = 2,
+    C_NEW = 3,
+    C_A0 = 4,  // arbiter count of buffer 0
+    C_A1 = 5,  // arbiter count of buffer 1
+    C_NCOL = 6,
+    C_REM = 7,   // 2 entries
+    C_TERR = 9,  // traversal stack overflow flag
+    C_NS = 10,   // shape count
+    C_NB = 11,   // body count
+    C_CONTACTS = 12,  // contact points solved
+    C_COUNT = 16
+};
+
+constexpr int kEpaThreads = 2048;
+
+int bits_for(uint32_t v) {
+    int b = 0;
+    while (v) { b++; v >>= 1; }
+    return b;
+}
+int grid_for(int n, int block = 256, int cap = 4096) {
+    int g = (n + block - 1) / block;
+    return std::max(1, std::min(g, cap));
+}
+
+}  // namespace
+
+struct jh_world {
+    jh_config cfg;
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t st = nullptr;
+    int nb = 0, ns = 0, nv = 0;
+    int max_world = 0;
+    std::vector<jh_body> host_bodies;
+    // bodies
+    DBuf<float4> pos, quat, vel, angvel, dvel, dangvel, imv, iw0, il0, mat, mat2, force, torque;
+    DBuf<float2> iw1, il1;
+    DBuf<int> motion, gravity, world;
+    DBuf<jh_body> body_io;
+    DBuf<char> rbd_io;
+    // shapes
+    DBuf<ShapeRec> shapes;
+    DBuf<float> verts;
+    // broadphase
+    DBuf<float4> amin, amax, lmin, lmax, nmin, nmax;
+    DBuf<uint64_t> mk0, mk1;
+    DBuf<uint32_t> mv0, mv1;
+    DBuf<int> left, right, parent, leaf_parent, nflags;
+    DBuf<int2> range;
+    DBuf<uint32_t> bounds, hist;
+    DBuf<uint64_t> pk0, pk1;
+    int pair_cap = 0;
+    int pairs_in_k1 = 0;  // sorted pairs ended in pk1
+    int n_pairs = 0;
+    // counters
+    DBuf<int> cnt;
+    int* hcnt = nullptr;  // pinned host mirror
+    // narrowphase
+    DBuf<jh_manifold> man;
+    DBuf<DeepRec> deep;
+    DBuf<epa::Work> epa_work;
+    // contact cache
+    DBuf<uint32_t> flags, offs, sums, cnew, noffs;
+    DBuf<uint64_t> ckey, nkey;
+    DBuf<int> cpair, cfound;
+    DBuf<jh_arbiter> arb[2];
+    DBuf<uint64_t> akey[2];
+    int cur = 0;
+    int a_host = 0;  // arbiters in arb[cur] (after the last step / upload)
+    // colouring + order
+    DBuf<int> colour, chist, coffs;
+    DBuf<unsigned long long> bmax, mlo, mhi;
+    DBuf<uint64_t> ok0, ok1, solve_keys;
+    int solve_count = 0, solve_batches = 0;
+    std::vector<int> solve_offsets;
+    // cooperative occupancy
+    int coop_colour = 0, coop_solve = 0;
+    hipEvent_t ev[8];
+    jh_stats stats;
+    bool have_step = false;
+
+    Bodies bodies() {
+        Bodies B;
+        B.pos = pos.p; B.quat = quat.p; B.vel = vel.p; B.angvel = angvel.p; B.dvel = dvel.p; B.dangvel = dangvel.p;
+        B.imv = imv.p; B.iw0 = iw0.p; B.iw1 = iw1.p; B.il0 = il0.p; B.il1 = il1.p; B.mat = mat.p; B.mat2 = mat2.p;
+        B.force = force.p; B.torque = torque.p; B.motion = motion.p; B.gravity = gravity.p; B.world = world.p;
+        return B;
+    }
+};
+
+namespace {
+
+__global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
+    int t = threadIdx.x;
+    if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS) cnt[t] = 0;
+    if (t < 3) bounds[t] = 0xffffffffu;
+    else if (t < 6) bounds[t] = 0u;
+}
+__global__ void k_set_int(int* p, int v) { *p = v; }
+__global__ void k_solve_keys(const uint64_t* __restrict__ order, const uint64_t* __restrict__ akey,
+                             const int* __restrict__ a_count, uint64_t* __restrict__ out) {
+    int A = *a_count;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x)
+        out[k] = akey[uint32_t(order[k])];
+}
+
+jh_status check_config(const jh_config* c) {
+    if (!c) return fail(JH_ERR_INVALID_ARG, "config is NULL");
+    if (c->substep_count < 1) return fail(JH_ERR_INVALID_ARG, "substep_count must be >= 1 (World.SubstepCount)");
+    if (c->solver_iterations < 1) return fail(JH_ERR_INVALID_ARG, "solver_iterations must be >= 1");
+    if (c->relaxation_iterations < 0) return fail(JH_ERR_INVALID_ARG, "relaxation_iterations must be >= 0");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(c->gravity[k])) return fail(JH_ERR_NON_FINITE, "gravity must be finite");
+    return JH_OK;
+}
+
+jh_status ensure_pair_buffers(jh_world* w, int n) {
+    HIPCHK(w->pk0.ensure(n));
+    HIPCHK(w->pk1.ensure(n));
+    w->pair_cap = int(std::min(w->pk0.cap, w->pk1.cap));
+    return JH_OK;
+}
+
+// Broadphase: AABBs, Morton keys, sort, LBVH, refit, traversal, pair sort.  Syncs once to read the
+// pair count (growing the pair buffer and re-traversing on overflow).
+jh_status broadphase(jh_world* w) {
+    hipStream_t st = w->st;
+    Bodies B = w->bodies();
+    int ns = w->ns;
+    int* cnt = w->cnt.p;
+    k_reset_counters<<<1, 64, 0, st>>>(cnt, w->bounds.p);
+    w->n_pairs = 0;
+    if (ns < 2) {
+        HIPCHK(hipGetLastError());
+        return JH_OK;
+    }
+    int g = grid_for(ns, 256, 1 << 20);
+    k_aabb<<<g, 256, 0, st>>>(w->shapes.p, ns, w->verts.p, B, w->amin.p, w->amax.p, w->bounds.p);
+    k_morton<<<g, 256, 0, st>>>(w->shapes.p, ns, w->world.p, w->amin.p, w->amax.p, w->bounds.p, w->mk0.p, w->mv0.p);
+    int shifts[8];
+    int nsh = 0;
+    for (int s = 0; s < 32; s += 8) shifts[nsh++] = s;
+    int wb = bits_for(uint32_t(w->max_world));
+    for (int s = 32; s < 32 + wb; s += 8) shifts[nsh++] = s;
+    bool in1 = rsort::sort_pairs(w->mk0.p, w->mk1.p, w->mv0.p, w->mv1.p, cnt + C_NS, ns, shifts, nsh, w->hist.p, st);
+    uint64_t* keys = in1 ? w->mk1.p : w->mk0.p;
+    uint32_t* order = in1 ? w->mv1.p : w->mv0.p;
+    k_lbvh_build<<<grid_for(ns - 1, 256, 1 << 20), 256, 0, st>>>(keys, ns, w->left.p, w->right.p, w->range.p,
+                                                                  w->parent.p, w->leaf_parent.p, w->nflags.p);
+    k_lbvh_refit<<<g, 256, 0, st>>>(ns, order, w->amin.p, w->amax.p, w->lmin.p, w->lmax.p, w->left.p, w->right.p,
+                                    w->parent.p, w->leaf_parent.p, w->nmin.p, w->nmax.p, w->nflags.p);
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        k_set_int<<<1, 1, 0, st>>>(cnt + C_PAIRS, 0);
+        k_traverse<<<g, 256, 0, st>>>(ns, keys, order, w->shapes.p, w->motion.p, w->lmin.p, w->lmax.p, w->left.p,
+                                      w->right.p, w->range.p, w->nmin.p, w->nmax.p, w->pk0.p, cnt + C_PAIRS,
+                                      w->pair_cap, cnt + C_TERR);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (w->hcnt[C_TERR]) return fail(JH_ERR_CAPACITY, "LBVH traversal stack overflow");
+        int np = w->hcnt[C_PAIRS];
+        if (np <= w->pair_cap) {
+            w->n_pairs = np;
+            break;
+        }
+        w->stats.pair_overflow++;
+        jh_status s = ensure_pair_buffers(w, np + np / 2);
+        if (s != JH_OK) return s;
+    }
+    int np = w->n_pairs;
+    if (np > 1) {
+        int pshift[8];
+        int npsh = 0;
+        int sb = bits_for(uint32_t(ns - 1));
+        for (int s = 0; s < sb; s += 8) pshift[npsh++] = s;
+        for (int s = 32; s < 32 + sb; s += 8) pshift[npsh++] = s;
+        HIPCHK(w->hist.ensure(size_t((w->pair_cap + rsort::kTile - 1) / rsort::kTile + 1) * 256));
+        w->pairs_in_k1 = rsort::sort_pairs(w->pk0.p, w->pk1.p, nullptr, nullptr, cnt + C_PAIRS, np, pshift, npsh,
+                                           w->hist.p, st);
+    } else {
+        w->pairs_in_k1 = 0;
+    }
+    HIPCHK(hipGetLastError());
+    return JH_OK;
+}
+
+jh_status ensure_step_buffers(jh_world* w) {
+    size_t np = std::max(w->n_pairs, 1);
+    HIPCHK(w->man.ensure(np));
+    HIPCHK(w->deep.ensure(np));
+    HIPCHK(w->flags.ensure(np));
+    HIPCHK(w->offs.ensure(np));
+    HIPCHK(w->sums.ensure(np / 256 + 2));
+    HIPCHK(w->cnew.ensure(np));
+    HIPCHK(w->noffs.ensure(np));
+    HIPCHK(w->ckey.ensure(np));
+    HIPCHK(w->nkey.ensure(np));
+    HIPCHK(w->cpair.ensure(np));
+    HIPCHK(w->cfound.ensure(np));
+    size_t na = size_t(w->a_host) + np;
+    HIPCHK(w->arb[w->cur].ensure(na, true));
+    HIPCHK(w->akey[w->cur].ensure(na, true));
+    HIPCHK(w->arb[1 - w->cur].ensure(na));
+    HIPCHK(w->akey[1 - w->cur].ensure(na));
+    HIPCHK(w->colour.ensure(na));
+    HIPCHK(w->ok0.ensure(na));
+    HIPCHK(w->ok1.ensure(na));
+    HIPCHK(w->solve_keys.ensure(na));
+    HIPCHK(w->flags.ensure(na));
+    HIPCHK(w->offs.ensure(na));
+    HIPCHK(w->sums.ensure(na / 256 + 2));
+    HIPCHK(w->hist.ensure(size_t((na + rsort::kTile - 1) / rsort::kTile + 1) * 256));
+    return JH_OK;
+}
+
+jh_status launch_coop(const void* fn, int grid, int block, void** args, hipStream_t st) {
+    HIPCHK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, st));
+    return JH_OK;
+}
+
+jh_status do_step(jh_world* w, float dt) {
+    hipStream_t st = w->st;
+    Bodies B = w->bodies();
+    int* cnt = w->cnt.p;
+    const jh_config& cfg = w->cfg;
+    // TimeStep (TimeStep.cs:36-46)
+    StepTime ts;
+    ts.dt = dt;
+    ts.substeps = cfg.substep_count;
+    ts.substep_dt = dt / float(cfg.substep_count);
+    ts.inv_dt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    ts.inv_substep_dt = ts.substep_dt > 0.0f ? 1.0f / ts.substep_dt : 0.0f;
+
+    HIPCHK(hipEventRecord(w->ev[0], st));
+    jh_status s = broadphase(w);
+    if (s != JH_OK) return s;
+    HIPCHK(hipEventRecord(w->ev[1], st));
+    s = ensure_step_buffers(w);
+    if (s != JH_OK) return s;
+    int np = w->n_pairs;
+    const uint64_t* pairs = w->pairs_in_k1 ? w->pk1.p : w->pk0.p;
+    int aux = cfg.enable_auxiliary_contact_points;
+    // narrowphase
+    if (np > 0) {
+        k_narrow<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->verts.p, B, aux,
+                                                             w->man.p, w->deep.p, cnt + C_DEEP);
+        k_epa<<<kEpaThreads / 64, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p, w->deep.p,
+                                               cnt + C_DEEP, w->epa_work.p);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(w->ev[2], st));
+    // contact cache: colliding pairs -> existing/new arbiters -> merged key-sorted cache -> AddContact
+    int cur = w->cur, nxt = 1 - w->cur;
+    int* a_cur = cnt + (cur == 0 ? C_A0 : C_A1);
+    int* a_nxt = cnt + (nxt == 0 ? C_A0 : C_A1);
+    if (np > 0) {
+        int g = grid_for(np);
+        k_collide_flags<<<g, 256, 0, st>>>(w->man.p, cnt + C_PAIRS, w->flags.p);
+        scan::exclusive_scan(w->flags.p, w->offs.p, cnt + C_PAIRS, np, w->sums.p, cnt + C_COLL, st);
+        k_collide_compact<<<g, 256, 0, st>>>(pairs, cnt + C_PAIRS, w->flags.p, w->offs.p, w->akey[cur].p, a_cur,
+                                             w->ckey.p, w->cpair.p, w->cfound.p, w->cnew.p);
+        scan::exclusive_scan(w->cnew.p, w->noffs.p, cnt + C_COLL, np, w->sums.p, cnt + C_NEW, st);
+        k_new_compact<<<g, 256, 0, st>>>(w->ckey.p, cnt + C_COLL, w->cnew.p, w->noffs.p, w->nkey.p);
+    }
+    int amax_ = w->a_host + np;
+    k_merge_existing<<<grid_for(std::max(w->a_host, 1)), 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->nkey.p,
+                                                                       cnt + C_NEW, w->arb[nxt].p, w->akey[nxt].p);
+    k_merge_new<<<grid_for(std::max(np, 1)), 256, 0, st>>>(w->akey[cur].p, a_cur, w->nkey.p, cnt + C_NEW, w->shapes.p, B,
+                                                           cfg.speculative_relaxation_factor, w->arb[nxt].p,
+                                                           w->akey[nxt].p, a_nxt);
+    if (np > 0)
+        k_register<<<grid_for(np), 256, 0, st>>>(w->ckey.p, w->cpair.p, w->cfound.p, w->noffs.p, cnt + C_COLL,
+                                                 w->akey[cur].p, a_cur, w->nkey.p, cnt + C_NEW, w->man.p, B,
+                                                 cfg.persistent_contact_manifold, w->arb[nxt].p);
+    HIPCHK(hipGetLastError());
+    cur = nxt;
+    w->cur = cur;
+    a_cur = a_nxt;
+    HIPCHK(hipEventRecord(w->ev[3], st));
+    // colouring of the arbiters (graph edges between dynamic bodies)
+    {
+        ColourArgs P;
+        P.arb = w->arb[cur].p;
+        P.a_count = a_cur;
+        P.motion = w->motion.p;
+        P.colour = w->colour.p;
+        P.body_max = w->bmax.p;
+        P.mask_lo = w->mlo.p;
+        P.mask_hi = w->mhi.p;
+        P.remaining = cnt + C_REM;
+        P.ncolours = cnt + C_NCOL;
+        P.hist = w->chist.p;
+        int grid = std::min(w->coop_colour, grid_for(std::max(amax_, 128), 256, 1 << 20));
+        void* args[] = {&P};
+        s = launch_coop((const void*)k_colour, grid, 256, args, st);
+        if (s != JH_OK) return s;
+        k_colour_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(w->colour.p, w->arb[cur].p, a_cur, w->ok0.p,
+                                                                     cnt + C_CONTACTS);
+        int csh[1] = {32};
+        bool in1 = rsort::sort_pairs(w->ok0.p, w->ok1.p, nullptr, nullptr, a_cur, std::max(amax_, 1), csh, 1,
+                                     w->hist.p, st);
+        k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p);
+        const uint64_t* order = in1 ? w->ok1.p : w->ok0.p;
+        k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
+        HIPCHK(hipEventRecord(w->ev[4], st));
+        // persistent solver over all substeps
+        SolveArgs S;
+        S.arb = w->arb[cur].p;
+        S.order = order;
+        S.offs = w->coffs.p;
+        S.ncolours = cnt + C_NCOL;
+        S.B = B;
+        S.nb = w->nb;
+        S.ts = ts;
+        S.iterations = cfg.solver_iterations;
+        S.relaxations = cfg.relaxation_iterations;
+        int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(amax_, w->nb), 256), 256, 1 << 20));
+        void* sargs[] = {&S};
+        s = launch_coop((const void*)k_solve, sgrid, 256, sargs, st);
+        if (s != JH_OK) return s;
+    }
+    HIPCHK(hipEventRecord(w->ev[5], st));
+    // UpdateContacts + removal of arbiters without contacts, then RigidBody.Update
+    {
+        int g = grid_for(std::max(amax_, 1));
+        k_update_contacts<<<g, 256, 0, st>>>(w->arb[cur].p, a_cur, B, w->flags.p);
+        scan::exclusive_scan(w->flags.p, w->offs.p, a_cur, std::max(amax_, 1), w->sums.p,
+                             cnt + (cur == 0 ? C_A1 : C_A0), st);
+        k_compact_arbiters<<<g, 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->flags.p, w->offs.p,
+                                              w->arb[1 - cur].p, w->akey[1 - cur].p);
+        w->cur = 1 - cur;
+        if (w->nb > 0)
+            k_update_bodies<<<grid_for(w->nb, 256, 1 << 20), 256, 0, st>>>(B, w->nb, ts, cfg.gravity[0], cfg.gravity[1],
+                                                                         cfg.gravity[2]);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(w->ev[6], st));
+    HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // solve-order bookkeeping and stats
+    w->a_host = w->hcnt[w->cur == 0 ? C_A0 : C_A1];
+    w->solve_count = w->hcnt[cur == 0 ? C_A0 : C_A1];
+    w->solve_batches = w->hcnt[C_NCOL];
+    jh_stats& S = w->stats;
+    S.bodies = w->nb;
+    S.shapes = w->ns;
+    S.pairs = np;
+    S.colliding_pairs = w->hcnt[C_COLL];
+    S.arbiters = w->a_host;
+    S.arbiters_solved = w->solve_count;
+    S.contacts = w->hcnt[C_CONTACTS];
+    S.colours = w->hcnt[C_NCOL];
+    S.epa_calls = w->hcnt[C_DEEP];
+    S.steps++;
+    float ms[7];
+    for (int k = 0; k < 6; ++k) HIPCHK(hipEventElapsedTime(&ms[k], w->ev[k], w->ev[k + 1]));
+    S.ms_broadphase = ms[0];
+    S.ms_narrowphase = ms[1];
+    S.ms_contact_cache = ms[2];
+    S.ms_colouring = ms[3];
+    S.ms_solve = ms[4];
+    S.ms_update = ms[5];
+    float tot;
+    HIPCHK(hipEventElapsedTime(&tot, w->ev[0], w->ev[6]));
+    S.ms_step = tot;
+    w->have_step = true;
+    return JH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* jh_last_error(void) { return g_err.c_str(); }
+int32_t jh_version(void) { return 1; }
+
+// Sizes of the ABI structs (checked by bindings): 0 config, 1 body, 2 shape, 3 contact, 4 arbiter,
+// 5 manifold, 6 stats.
+int32_t jh_struct_size(int32_t which) {
+    switch (which) {
+        case 0: return sizeof(jh_config);
+        case 1: return sizeof(jh_body);
+        case 2: return sizeof(jh_shape);
+        case 3: return sizeof(jh_contact);
+        case 4: return sizeof(jh_arbiter);
+        case 5: return sizeof(jh_manifold);
+        case 6: return sizeof(jh_stats);
+        default: return -1;
+    }
+}
+
+jh_status jh_config_defaults(jh_config* c) {
+    if (!c) return fail(JH_ERR_INVALID_ARG, "config is NULL");
+    memset(c, 0, sizeof(*c));
+    c->gravity[1] = -9.81f;
+    c->substep_count = 1;
+    c->solver_iterations = 6;
+    c->relaxation_iterations = 4;
+    c->speculative_relaxation_factor = 0.9f;
+    c->enable_auxiliary_contact_points = 1;
+    c->persistent_contact_manifold = 1;
+    return JH_OK;
+}
+
+jh_status jh_world_create(const jh_config* config, jh_world** out) {
+    if (!out) return fail(JH_ERR_INVALID_ARG, "out_world is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(JH_ERR_NO_DEVICE, "no HIP device visible");
+    jh_config c;
+    if (config) c = *config;
+    else jh_config_defaults(&c);
+    jh_status s = check_config(&c);
+    if (s != JH_OK) return s;
+    jh_world* w = new jh_world();
+    w->cfg = c;
+    memset(&w->stats, 0, sizeof(w->stats));
+    HIPCHK(hipGetDevice(&w->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, w->device));
+    w->cu_count = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking));
+    for (auto& e : w->ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipHostMalloc((void**)&w->hcnt, C_COUNT * sizeof(int), hipHostMallocDefault));
+    HIPCHK(w->cnt.ensure(C_COUNT));
+    HIPCHK(hipMemset(w->cnt.p, 0, C_COUNT * sizeof(int)));
+    HIPCHK(w->bounds.ensure(8));
+    HIPCHK(w->chist.ensure(128));
+    HIPCHK(w->coffs.ensure(129));
+    HIPCHK(w->epa_work.ensure(kEpaThreads));
+    int occ = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_colour, 256, 0));
+    w->coop_colour = std::max(1, occ) * w->cu_count;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve, 256, 0));
+    w->coop_solve = std::max(1, occ) * w->cu_count;
+    *out = w;
+    return JH_OK;
+}
+
+jh_status jh_world_destroy(jh_world* w) {
+    if (!w) return fail(JH_ERR_INVALID_ARG, "world is NULL");
+    (void)hipStreamSynchronize(w->st);
+    DBuf<float4>* f4[] = {&w->pos, &w->quat, &w->vel, &w->angvel, &w->dvel, &w->dangvel, &w->imv, &w->iw0,
+                          &w->il0, &w->mat, &w->mat2, &w->force, &w->torque, &w->amin, &w->amax, &w->lmin,
+                          &w->lmax, &w->nmin, &w->nmax};
+    for (auto* b : f4) b->release();
+    w->iw1.release(); w->il1.release();
+    w->motion.release(); w->gravity.release(); w->world.release();
+    w->body_io.release(); w->rbd_io.release(); w->shapes.release(); w->verts.release();
+    w->mk0.release(); w->mk1.release(); w->mv0.release(); w->mv1.release();
+    w->left.release(); w->right.release(); w->parent.release(); w->leaf_parent.release(); w->nflags.release();
+    w->range.release(); w->bounds.release(); w->hist.release(); w->pk0.release(); w->pk1.release();
+    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release();
+    w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
+    w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
+    for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
+    w->colour.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
+    w->ok0.release(); w->ok1.release(); w->solve_keys.release();
+    for (auto& e : w->ev) (void)hipEventDestroy(e);
+    if (w->hcnt) (void)hipHostFree(w->hcnt);
+    (void)hipStreamDestroy(w->st);
+    delete w;
+    return JH_OK;
+}
+
+jh_status jh_world_set_config(jh_world* w, const jh_config* c) {
+    if (!w) return fail(JH_ERR_INVALID_ARG, "world is NULL");
+    jh_status s = check_config(c);
+    if (s != JH_OK) return s;
+    w->cfg = *c;
+    return JH_OK;
+}
+
+jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
+    if (!w || (n > 0 && !bodies) || n < 0) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    int maxw = 0;
+    for (int i = 0; i < n; ++i) {
+        const jh_body& b = bodies[i];
+        if (b.motion_type < 0 || b.motion_type > 2) return fail(JH_ERR_INVALID_ARG, "body %d: bad motion_type", i);
+        if (b.world_id < 0) return fail(JH_ERR_INVALID_ARG, "body %d: negative world_id", i);
+        const float* f = (const float*)&b;
+        for (int k = 0; k < 36; ++k)
+            if (!std::isfinite(f[k])) return fail(JH_ERR_NON_FINITE, "body %d: non-finite field", i);
+        maxw = std::max(maxw, b.world_id);
+    }
+    w->nb = n;
+    w->max_world = maxw;
+    w->host_bodies.assign(bodies, bodies + n);
+    size_t nn = std::max(n, 1);
+    DBuf<float4>* f4[] = {&w->pos, &w->quat, &w->vel, &w->angvel, &w->dvel, &w->dangvel, &w->imv,
+                          &w->iw0, &w->il0, &w->mat, &w->mat2, &w->force, &w->torque};
+    for (auto* b : f4) HIPCHK(b->ensure(nn));
+    HIPCHK(w->iw1.ensure(nn));
+    HIPCHK(w->il1.ensure(nn));
+    HIPCHK(w->motion.ensure(nn));
+    HIPCHK(w->gravity.ensure(nn));
+    HIPCHK(w->world.ensure(nn));
+    HIPCHK(w->body_io.ensure(nn));
+    HIPCHK(w->bmax.ensure(nn));
+    HIPCHK(w->mlo.ensure(nn));
+    HIPCHK(w->mhi.ensure(nn));
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(w->body_io.p, bodies, n * sizeof(jh_body), hipMemcpyHostToDevice, w->st));
+        k_unpack_bodies<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(w->body_io.p, n, w->bodies());
+        HIPCHK(hipGetLastError());
+    }
+    k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_NB, n);
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_get_bodies(jh_world* w, jh_body* bodies, int32_t n) {
+    if (!w || !bodies || n < 0 || n > w->nb) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (n == 0) return JH_OK;
+    k_pack_bodies<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(w->bodies(), n, w->body_io.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(bodies, w->body_io.p, n * sizeof(jh_body), hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_get_rigidbodydata(jh_world* w, void* records, int32_t n, int32_t stride) {
+    if (!w || !records || n < 0 || n > w->nb || stride < 128) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (n == 0) return JH_OK;
+    HIPCHK(w->rbd_io.ensure(size_t(n) * stride));
+    HIPCHK(hipMemsetAsync(w->rbd_io.p, 0, size_t(n) * stride, w->st));
+    k_pack_rigidbodydata<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(w->bodies(), n, w->rbd_io.p, stride);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(records, w->rbd_io.p, size_t(n) * stride, hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, const float* vertices, int32_t nv) {
+    if (!w || n < 0 || (n > 0 && !shapes) || nv < 0 || (nv > 0 && !vertices))
+        return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    std::vector<ShapeRec> recs(n);
+    for (int i = 0; i < n; ++i) {
+        const jh_shape& s = shapes[i];
+        if (i > 0 && !(shapes[i - 1].shape_id < s.shape_id))
+            return fail(JH_ERR_INVALID_ARG, "shape %d: shape ids must be strictly ascending", i);
+        if (s.body < 0 || s.body >= w->nb) return fail(JH_ERR_INVALID_ARG, "shape %d: body index out of range", i);
+        if (s.type < JH_SHAPE_BOX || s.type > JH_SHAPE_TRIANGLE)
+            return fail(JH_ERR_INVALID_ARG, "shape %d: unknown type", i);
+        if ((s.type == JH_SHAPE_POINTCLOUD && (s.vertex_count < 1)) || (s.type == JH_SHAPE_TRIANGLE && s.vertex_count != 3))
+            return fail(JH_ERR_INVALID_ARG, "shape %d: bad vertex count", i);
+        if ((s.type == JH_SHAPE_POINTCLOUD || s.type == JH_SHAPE_TRIANGLE) &&
+            (s.vertex_offset < 0 || s.vertex_offset + s.vertex_count > nv))
+            return fail(JH_ERR_INVALID_ARG, "shape %d: vertex range out of bounds", i);
+        for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(s.params[k])) return fail(JH_ERR_NON_FINITE, "shape %d: non-finite params", i);
+        ShapeRec& r = recs[i];
+        r.type = s.type;
+        r.body = s.body;
+        for (int k = 0; k < 12; ++k) r.p[k] = s.params[k];
+        r.voff = s.vertex_offset;
+        r.vcnt = s.vertex_count;
+    }
+    w->ns = n;
+    w->nv = nv;
+    size_t nn = std::max(n, 2);
+    HIPCHK(w->shapes.ensure(nn));
+    HIPCHK(w->verts.ensure(std::max<size_t>(3 * size_t(nv), 3)));
+    DBuf<float4>* f4[] = {&w->amin, &w->amax, &w->lmin, &w->lmax, &w->nmin, &w->nmax};
+    for (auto* b : f4) HIPCHK(b->ensure(nn));
+    HIPCHK(w->mk0.ensure(nn)); HIPCHK(w->mk1.ensure(nn)); HIPCHK(w->mv0.ensure(nn)); HIPCHK(w->mv1.ensure(nn));
+    HIPCHK(w->left.ensure(nn)); HIPCHK(w->right.ensure(nn)); HIPCHK(w->parent.ensure(nn));
+    HIPCHK(w->leaf_parent.ensure(nn)); HIPCHK(w->nflags.ensure(nn)); HIPCHK(w->range.ensure(nn));
+    HIPCHK(w->hist.ensure(size_t((nn + rsort::kTile - 1) / rsort::kTile + 1) * 256));
+    int want = w->cfg.pair_capacity > 0 ? w->cfg.pair_capacity : 24 * n + 1024;
+    if (w->pair_cap < want) {
+        jh_status s = ensure_pair_buffers(w, want);
+        if (s != JH_OK) return s;
+    }
+    if (n > 0) HIPCHK(hipMemcpyAsync(w->shapes.p, recs.data(), n * sizeof(ShapeRec), hipMemcpyHostToDevice, w->st));
+    if (nv > 0) HIPCHK(hipMemcpyAsync(w->verts.p, vertices, 3 * size_t(nv) * sizeof(float), hipMemcpyHostToDevice, w->st));
+    k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_NS, n);
+    // a new shape set invalidates the contact cache (RigidBody.ClearContactCache on shape changes)
+    k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_A0, 0);
+    k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_A1, 0);
+    w->a_host = 0;
+    w->cur = 0;
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_step(jh_world* w, float dt) {
+    if (!w) return fail(JH_ERR_STATE, "world is NULL");
+    // World.StepInternal argument checks (World.Step.cs:155-165)
+    if (!std::isfinite(dt)) return fail(JH_ERR_NON_FINITE, "dt must be finite");
+    if (dt < 0.0f) return fail(JH_ERR_INVALID_ARG, "Time step cannot be negative.");
+    if (dt < 1.401298464e-45f) return JH_OK;
+    if (w->nb == 0) return JH_OK;
+    return do_step(w, dt);
+}
+
+jh_status jh_world_compute_pairs(jh_world* w, uint32_t* pairs, int32_t capacity, int32_t* out_count) {
+    if (!w || !out_count || capacity < 0 || (capacity > 0 && !pairs)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    jh_status s = broadphase(w);
+    if (s != JH_OK) return s;
+    int np = w->n_pairs;
+    *out_count = np;
+    if (np > 0 && capacity > 0) {
+        std::vector<uint64_t> tmp(std::min(np, capacity));
+        HIPCHK(hipMemcpyAsync(tmp.data(), w->pairs_in_k1 ? w->pk1.p : w->pk0.p, tmp.size() * 8, hipMemcpyDeviceToHost,
+                              w->st));
+        HIPCHK(hipStreamSynchronize(w->st));
+        for (size_t i = 0; i < tmp.size(); ++i) {
+            pairs[2 * i] = uint32_t(tmp[i] >> 32);
+            pairs[2 * i + 1] = uint32_t(tmp[i] & 0xffffffffu);
+        }
+    }
+    if (np > capacity) return fail(JH_ERR_CAPACITY, "pair count %d exceeds capacity %d", np, capacity);
+    return JH_OK;
+}
+
+jh_status jh_world_get_manifolds(jh_world* w, jh_manifold* out, int32_t capacity, int32_t* out_count) {
+    if (!w || !out_count || capacity < 0 || (capacity > 0 && !out)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    *out_count = 0;
+    if (!w->have_step || w->n_pairs == 0) return JH_OK;
+    std::vector<jh_manifold> all(w->n_pairs);
+    HIPCHK(hipMemcpyAsync(all.data(), w->man.p, all.size() * sizeof(jh_manifold), hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    int n = 0;
+    for (auto& m : all) {
+        if (m.count <= 0) continue;
+        if (n < capacity) out[n] = m;
+        n++;
+    }
+    *out_count = n;
+    if (n > capacity) return fail(JH_ERR_CAPACITY, "manifold count %d exceeds capacity %d", n, capacity);
+    return JH_OK;
+}
+
+jh_status jh_world_get_arbiters(jh_world* w, jh_arbiter* out, int32_t capacity, int32_t* out_count) {
+    if (!w || !out_count || capacity < 0 || (capacity > 0 && !out)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    int n = w->a_host;
+    *out_count = n;
+    int k = std::min(n, capacity);
+    if (k > 0) {
+        HIPCHK(hipMemcpyAsync(out, w->arb[w->cur].p, k * sizeof(jh_arbiter), hipMemcpyDeviceToHost, w->st));
+        HIPCHK(hipStreamSynchronize(w->st));
+    }
+    if (n > capacity) return fail(JH_ERR_CAPACITY, "arbiter count %d exceeds capacity %d", n, capacity);
+    return JH_OK;
+}
+
+jh_status jh_world_set_arbiters(jh_world* w, const jh_arbiter* in, int32_t n) {
+    if (!w || n < 0 || (n > 0 && !in)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    std::vector<jh_arbiter> a(in, in + n);
+    for (auto& r : a) {
+        if (r.shape1 >= uint32_t(w->ns) || r.shape2 >= uint32_t(w->ns) || !(r.shape1 < r.shape2))
+            return fail(JH_ERR_INVALID_ARG, "arbiter keys must be shape indices with shape1 < shape2");
+        if (r.body1 < 0 || r.body1 >= w->nb || r.body2 < 0 || r.body2 >= w->nb)
+            return fail(JH_ERR_INVALID_ARG, "arbiter body index out of range");
+    }
+    std::sort(a.begin(), a.end(), [](const jh_arbiter& x, const jh_arbiter& y) {
+        return x.shape1 < y.shape1 || (x.shape1 == y.shape1 && x.shape2 < y.shape2);
+    });
+    std::vector<uint64_t> keys(n);
+    for (int i = 0; i < n; ++i) {
+        keys[i] = (uint64_t(a[i].shape1) << 32) | a[i].shape2;
+        if (i > 0 && keys[i] == keys[i - 1]) return fail(JH_ERR_INVALID_ARG, "duplicate arbiter key");
+    }
+    size_t nn = std::max(n, 1);
+    HIPCHK(w->arb[w->cur].ensure(nn));
+    HIPCHK(w->akey[w->cur].ensure(nn));
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(w->arb[w->cur].p, a.data(), n * sizeof(jh_arbiter), hipMemcpyHostToDevice, w->st));
+        HIPCHK(hipMemcpyAsync(w->akey[w->cur].p, keys.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, w->st));
+    }
+    k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + (w->cur == 0 ? C_A0 : C_A1), n);
+    HIPCHK(hipStreamSynchronize(w->st));
+    w->a_host = n;
+    return JH_OK;
+}
+
+jh_status jh_world_get_solve_order(jh_world* w, uint32_t* keys, int32_t key_capacity, int32_t* batch_offsets,
+                                   int32_t batch_capacity, int32_t* out_key_count, int32_t* out_batch_count) {
+    if (!w || !out_key_count || !out_batch_count) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    *out_key_count = 0;
+    *out_batch_count = 0;
+    if (!w->have_step) return JH_OK;
+    std::vector<int> offs(129);
+    HIPCHK(hipMemcpyAsync(offs.data(), w->coffs.p, 129 * sizeof(int), hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    int nc = w->solve_batches;
+    // batches: colours 0..nc-1, then the serial overflow batch if it is not empty
+    std::vector<int> bo;
+    for (int c = 0; c <= nc; ++c) bo.push_back(offs[c]);
+    int serial = offs[kMaxColours + 1] - offs[kMaxColours];
+    if (serial > 0) bo.push_back(offs[nc] + serial);
+    int nkeys = bo.back();
+    int nbatch = int(bo.size()) - 1;
+    *out_key_count = nkeys;
+    *out_batch_count = nbatch;
+    if (keys && key_capacity > 0 && nkeys > 0) {
+        std::vector<uint64_t> k(nkeys);
+        HIPCHK(hipMemcpyAsync(k.data(), w->solve_keys.p, nkeys * 8, hipMemcpyDeviceToHost, w->st));
+        HIPCHK(hipStreamSynchronize(w->st));
+        // colour-sorted keys: colours 0..nc-1 are contiguous, the serial batch (colour 127) follows them
+        for (int i = 0; i < std::min(nkeys, key_capacity); ++i) {
+            keys[2 * i] = uint32_t(k[i] >> 32);
+            keys[2 * i + 1] = uint32_t(k[i] & 0xffffffffu);
+        }
+    }
+    if (batch_offsets && batch_capacity > 0)
+        for (int i = 0; i < std::min(nbatch + 1, batch_capacity); ++i) batch_offsets[i] = bo[i];
+    if (nkeys > key_capacity || nbatch + 1 > batch_capacity)
+        return fail(JH_ERR_CAPACITY, "solve order needs %d keys and %d offsets", nkeys, nbatch + 1);
+    return JH_OK;
+}
+
+jh_status jh_world_get_stats(jh_world* w, jh_stats* s) {
+    if (!w || !s) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    *s = w->stats;
+    return JH_OK;
+}
+
+}  // extern "C"

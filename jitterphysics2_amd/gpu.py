@@ -1,0 +1,111 @@
+"""GpuWorld: thin Python handle over the C-ABI of libjitterhip.so (include/jitterhip.h).
+
+All compute runs in the HIP library; this module only moves arrays across the boundary.  It
+raises if the library is missing — there is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._abi import (ARBITER_DTYPE, BODY_DTYPE, CONFIG_DTYPE, MANIFOLD_DTYPE, STATS_DTYPE, check,
+                   default_config, load_library, ptr)
+
+
+class GpuWorld:
+    """One jh_world: device-resident bodies, shapes and contact cache, stepped by World.Step
+    semantics (src/Jitter2/World.Step.cs:141-296)."""
+
+    def __init__(self, bodies: np.ndarray, shapes: np.ndarray, verts: np.ndarray | None = None,
+                 config: np.ndarray | None = None):
+        self.lib = load_library()
+        self.handle = ctypes.c_void_p()
+        cfg = default_config() if config is None else np.ascontiguousarray(config, CONFIG_DTYPE)
+        check(self.lib, self.lib.jh_world_create(ptr(cfg), ctypes.byref(self.handle)), "jh_world_create")
+        self.n_bodies = 0
+        self.set_bodies(bodies)
+        self.set_shapes(shapes, verts)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) and self.handle.value:
+            self.lib.jh_world_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_config(self, config: np.ndarray) -> None:
+        c = np.ascontiguousarray(config, CONFIG_DTYPE)
+        check(self.lib, self.lib.jh_world_set_config(self.handle, ptr(c)), "jh_world_set_config")
+
+    def set_bodies(self, bodies: np.ndarray) -> None:
+        b = np.ascontiguousarray(bodies, BODY_DTYPE)
+        check(self.lib, self.lib.jh_world_set_bodies(self.handle, ptr(b), len(b)), "jh_world_set_bodies")
+        self.n_bodies = len(b)
+
+    def set_shapes(self, shapes: np.ndarray, verts: np.ndarray | None = None) -> None:
+        s = np.ascontiguousarray(shapes)
+        v = None if verts is None else np.ascontiguousarray(verts, np.float32).reshape(-1)
+        nv = 0 if v is None else len(v) // 3
+        check(self.lib, self.lib.jh_world_set_shapes(self.handle, ptr(s), len(s), ptr(v), nv), "jh_world_set_shapes")
+
+    def step(self, dt: float) -> None:
+        check(self.lib, self.lib.jh_world_step(self.handle, ctypes.c_float(dt)), "jh_world_step")
+
+    def bodies(self) -> np.ndarray:
+        out = np.zeros(self.n_bodies, BODY_DTYPE)
+        check(self.lib, self.lib.jh_world_get_bodies(self.handle, ptr(out), self.n_bodies), "jh_world_get_bodies")
+        return out
+
+    def rigidbodydata(self, stride: int = 128) -> np.ndarray:
+        out = np.zeros(self.n_bodies * stride, np.uint8)
+        check(self.lib, self.lib.jh_world_get_rigidbodydata(self.handle, ptr(out), self.n_bodies, stride),
+              "jh_world_get_rigidbodydata")
+        return out.reshape(self.n_bodies, stride)
+
+    def compute_pairs(self) -> np.ndarray:
+        n = ctypes.c_int32(0)
+        self.lib.jh_world_compute_pairs(self.handle, None, 0, ctypes.byref(n))
+        out = np.zeros((max(n.value, 1), 2), np.uint32)
+        check(self.lib, self.lib.jh_world_compute_pairs(self.handle, ptr(out), n.value, ctypes.byref(n)),
+              "jh_world_compute_pairs")
+        return out[:n.value]
+
+    def manifolds(self) -> np.ndarray:
+        n = ctypes.c_int32(0)
+        self.lib.jh_world_get_manifolds(self.handle, None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), MANIFOLD_DTYPE)
+        check(self.lib, self.lib.jh_world_get_manifolds(self.handle, ptr(out), n.value, ctypes.byref(n)),
+              "jh_world_get_manifolds")
+        return out[:n.value]
+
+    def arbiters(self) -> np.ndarray:
+        n = ctypes.c_int32(0)
+        self.lib.jh_world_get_arbiters(self.handle, None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), ARBITER_DTYPE)
+        check(self.lib, self.lib.jh_world_get_arbiters(self.handle, ptr(out), n.value, ctypes.byref(n)),
+              "jh_world_get_arbiters")
+        return out[:n.value]
+
+    def set_arbiters(self, arbiters: np.ndarray) -> None:
+        a = np.ascontiguousarray(arbiters, ARBITER_DTYPE)
+        check(self.lib, self.lib.jh_world_set_arbiters(self.handle, ptr(a), len(a)), "jh_world_set_arbiters")
+
+    def solve_order(self) -> tuple[np.ndarray, np.ndarray]:
+        """(keys (K, 2) uint32 in solve order, batch offsets (B+1,)) of the last step."""
+        nk, nb = ctypes.c_int32(0), ctypes.c_int32(0)
+        self.lib.jh_world_get_solve_order(self.handle, None, 0, None, 0, ctypes.byref(nk), ctypes.byref(nb))
+        keys = np.zeros((max(nk.value, 1), 2), np.uint32)
+        offs = np.zeros(nb.value + 1, np.int32)
+        check(self.lib, self.lib.jh_world_get_solve_order(self.handle, ptr(keys), nk.value, ptr(offs), nb.value + 1,
+                                                          ctypes.byref(nk), ctypes.byref(nb)), "jh_world_get_solve_order")
+        return keys[:nk.value], offs
+
+    def stats(self) -> dict:
+        s = np.zeros(1, STATS_DTYPE)
+        check(self.lib, self.lib.jh_world_get_stats(self.handle, ptr(s)), "jh_world_get_stats")
+        return {k: s[k][0].item() for k in STATS_DTYPE.names}

@@ -15,7 +15,7 @@ import math
 
 import numpy as np
 
-from ._abi import (BODY_DTYPE, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE,
+from ._abi import (BODY_DTYPE, default_config, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE,
                    SHAPE_POINTCLOUD, SHAPE_SPHERE, default_config)
 
 f32 = np.float32

@@ -207,6 +207,53 @@ Shape<double> shape_d(const jh_shape& s, const float* verts) {
     o.nverts = s.vertex_count;
     return o;
 }
+
+// Contact-level restatement for known-answer tests: bodies[0..1] and one arbiter (body1 = 0,
+// body2 = 1).  op 0: AddContact(p1, p2, n) after ResetMode; op 1: PrepareForIteration; op 2:
+// Iterate(applyBias = true); op 3: Iterate(applyBias = false); op 4: UpdatePosition.  The bodies and
+// the arbiter are updated in place (FP32 or FP64 evaluation, stored back as float).
+template <class R> int contact_op_t(jh_body* bodies, jh_arbiter* arb, int op, float dt, int substeps,
+                                     const float* p1, const float* p2, const float* n) {
+    World<R> w;
+    set_bodies(w, bodies, 2);
+    jh_arbiter a0 = *arb;
+    a0.shape1 = 0;
+    a0.shape2 = 1;
+    set_arbiters(w, &a0, 1);
+    Arbiter<R>& a = w.arbiters.begin()->second;
+    a.b1 = arb->body1;
+    a.b2 = arb->body2;
+    StepTiming<R> ts(R(dt), substeps);
+    switch (op) {
+        case 0:
+            w.reset_mode(a);
+            w.add_contact(a, vin<R>(p1), vin<R>(p2), vin<R>(n));
+            break;
+        case 1:
+            for (int i = 0; i < 4; ++i)
+                if (a.usage & (1u << i)) w.prepare_contact(a, a.c[i], ts);
+            break;
+        case 2:
+        case 3:
+            for (int i = 0; i < 4; ++i)
+                if (a.usage & (1u << i)) w.iterate_contact(a, a.c[i], op == 2);
+            break;
+        case 4:
+            w.arbiter_update_position(a);
+            break;
+        default:
+            return -1;
+    }
+    get_bodies(w, bodies, 2);
+    jh_arbiter out;
+    get_arbiters(w, &out, 1);
+    out.body1 = arb->body1;
+    out.body2 = arb->body2;
+    out.shape1 = arb->shape1;
+    out.shape2 = arb->shape2;
+    *arb = out;
+    return 0;
+}
 }  // namespace
 
 #define DISPATCH(h, expr) ((h)->fp64 ? ([&](World<double>& W) { return expr; })(*(h)->wd) \
@@ -381,6 +428,11 @@ int orc_build_manifold(int fp64, const jh_shape* a, const jh_shape* b, const flo
     out[0] = float(m.count);
     for (int i = 0; i < m.count; ++i) { vout(m.mA[i], out + 1 + 3 * i); vout(m.mB[i], out + 13 + 3 * i); }
     return m.count;
+}
+int orc_contact_op(int fp64, jh_body* bodies, jh_arbiter* arb, int op, float dt, int substeps, const float* p1,
+                   const float* p2, const float* n) {
+    if (fp64) return contact_op_t<double>(bodies, arb, op, dt, substeps, p1, p2, n);
+    return contact_op_t<float>(bodies, arb, op, dt, substeps, p1, p2, n);
 }
 // StableMath.SinCos / Atan2 in FP32 (float) for the trigonometry checks.
 void orc_sincos_f(float a, float* s, float* c) { StableMath<float>::sincos(a, *s, *c); }

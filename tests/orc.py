@@ -41,6 +41,7 @@ def lib() -> ctypes.CDLL:
             "orc_mpr_manifold": [i32, vp, vp, vp, vp, vp, vp, vp, f, vp],
             "orc_build_manifold": [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "orc_exact_side": [vp, vp, vp, vp],
+            "orc_contact_op": [i32, vp, vp, i32, f, i32, vp, vp, vp],
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32

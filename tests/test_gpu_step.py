@@ -1,0 +1,112 @@
+"""GPU parity of the whole step against the oracle.
+
+The GPU solves contacts in graph-colour order; the oracle replays the same order (downloaded with
+jh_world_get_solve_order), so bodies, contacts and impulses must agree bit for bit in FP32
+(tolerance 0: the device code keeps the reference's operation order, FP contraction is off and
+division/sqrt are correctly rounded).  Pair sets are compared with the oracle's brute-force pair
+set.  FP64 error (oracle in double ≙ Jitter2.Double) is reported alongside.
+"""
+import numpy as np
+import pytest
+
+from jitterphysics2_amd import scenes
+from jitterphysics2_amd.gpu import GpuWorld
+from tests.orc import OracleWorld
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("position", "orientation", "velocity", "angular_velocity", "delta_velocity", "delta_angular_velocity")
+
+
+def assert_bodies_equal(g, o, what):
+    for f in STATE:
+        if not np.array_equal(g[f], o[f]):
+            bad = np.argwhere(np.any(g[f] != o[f], axis=-1)).ravel()
+            d = np.abs(g[f].astype(np.float64) - o[f]).max()
+            raise AssertionError(f"{what}: {f} differs in {len(bad)} bodies (first {bad[:5]}), max abs {d:.3e}")
+
+
+def assert_arbiters_equal(ga, oa, what):
+    assert len(ga) == len(oa), f"{what}: arbiter count {len(ga)} != {len(oa)}"
+    for f in ("shape1", "shape2", "body1", "body2", "usage_mask", "mode", "friction", "restitution"):
+        assert np.array_equal(ga[f], oa[f]), f"{what}: arbiter field {f} differs"
+    gc, oc = ga["contacts"], oa["contacts"]
+    used = ((ga["usage_mask"][:, None] >> np.arange(4)[None, :]) & 1).astype(bool)
+    for f in gc.dtype.names:
+        a, b = gc[f][used], oc[f][used]
+        assert np.array_equal(a, b), f"{what}: contact field {f} differs"
+
+
+def assert_manifolds_equal(gm, om, what):
+    assert len(gm) == len(om), f"{what}: manifold count {len(gm)} != {len(om)}"
+    for f in ("shape1", "shape2", "count", "flags", "normal", "penetration", "point_a", "point_b"):
+        assert np.array_equal(gm[f], om[f]), f"{what}: manifold field {f} differs"
+    for i in range(4):
+        m = gm["count"] > i
+        assert np.array_equal(gm["manifold_a"][m, i], om["manifold_a"][m, i]), f"{what}: manifold_a[{i}]"
+        assert np.array_equal(gm["manifold_b"][m, i], om["manifold_b"][m, i]), f"{what}: manifold_b[{i}]"
+
+
+def run_replay(scene, steps, check_every=1, brute_pairs=False):
+    bodies, shapes, verts, cfg, dt = scene
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    ow = OracleWorld(bodies, shapes, verts, cfg)
+    for s in range(steps):
+        pg = gw.compute_pairs()
+        po = ow.pairs(brute=brute_pairs)
+        assert np.array_equal(pg, po), f"step {s}: pair set differs ({len(pg)} vs {len(po)})"
+        gw.step(dt)
+        keys, offs = gw.solve_order()
+        ow.step(dt, keys)
+        if s % check_every == 0 or s == steps - 1:
+            assert_manifolds_equal(gw.manifolds(), ow.manifolds(), f"step {s}")
+            assert_bodies_equal(gw.bodies(), ow.bodies(), f"step {s}")
+            assert_arbiters_equal(gw.arbiters(), ow.arbiters(), f"step {s}")
+    return gw, ow
+
+
+def test_pyramid_small_bitexact():
+    run_replay(scenes.pyramid(rows=6), steps=20, brute_pairs=True)
+
+
+def test_pyramid_20_rows_bitexact():
+    gw, ow = run_replay(scenes.pyramid(rows=20), steps=30, check_every=10)
+    st = gw.stats()
+    assert st["arbiters"] > 300 and st["colours"] >= 2
+
+
+def test_heap_mixed_shapes_bitexact():
+    # spheres, boxes and capsules with random orientations; deep impacts exercise EPA
+    gw, ow = run_replay(scenes.heap(1500, seed=7), steps=120, check_every=20)
+    assert gw.stats()["arbiters"] > 0
+
+
+def test_solve_order_is_colouring():
+    bodies, shapes, verts, cfg, dt = scenes.heap(1200, seed=3)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    for _ in range(60):
+        gw.step(dt)
+    keys, offs = gw.solve_order()
+    arbs = gw.arbiters()
+    assert len(keys) >= len(arbs)  # arbiters were compacted after the solve
+    body_of = shapes["body"]
+    dyn = bodies["motion_type"] == 0
+    for b in range(len(offs) - 1):
+        k = keys[offs[b]:offs[b + 1]]
+        bl = np.concatenate([body_of[k[:, 0]], body_of[k[:, 1]]])
+        bl = bl[dyn[bl]]
+        serial = b == len(offs) - 2 and len(offs) - 1 > gw.stats()["colours"]
+        if not serial:
+            assert len(np.unique(bl)) == len(bl), f"batch {b} shares a dynamic body"
+
+
+def test_step_argument_errors():
+    bodies, shapes, verts, cfg, dt = scenes.pyramid(rows=3)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    with pytest.raises(RuntimeError, match="invalid argument"):
+        gw.step(-0.01)
+    with pytest.raises(RuntimeError, match="non-finite"):
+        gw.step(float("nan"))
+    b0 = gw.bodies()
+    gw.step(0.0)  # dt < epsilon: nothing happens (World.Step.cs:163-164)
+    assert np.array_equal(gw.bodies()["position"], b0["position"])
