@@ -95,6 +95,7 @@ def main() -> int:
     avg_solve_s = float(np.mean(solve_ms)) / 1000.0
     achieved = float(np.mean(bytes_per_launch)) / avg_solve_s / 1e9
     final = gw.stats()
+    gw.close()
 
     cpu = None
     if not args.no_cpu:
@@ -124,9 +125,10 @@ def main() -> int:
                                f"{substeps} substeps, {iters} iterations, {relax} relaxations (plain warm start)",
                    "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],
                    "contacts": final["contacts"], "colours": final["colours"],
+                   "colliding_pairs": final["colliding_pairs"], "epa_pairs": final["epa_calls"],
                    "contacts_per_s": float(np.mean(contacts)) * args.steps / wall,
                    "stage_ms": {k: final[k] for k in ("ms_broadphase", "ms_narrowphase", "ms_contact_cache",
-                                                      "ms_colouring", "ms_solve", "ms_update")}},
+                                                      "ms_colouring", "ms_solve", "ms_update", "ms_epa")}},
         "roofline": {"bound": "hbm", "kernel": "k_solve", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_ms": 1000.0 * avg_solve_s,

@@ -136,7 +136,7 @@ typedef struct jh_arbiter {
     float friction;        /* max of the two bodies (Contact.cs:276)                         */
     float restitution;     /* max of the two bodies (Contact.cs:277)                         */
     float speculative_relaxation_factor;
-    uint32_t reserved;
+    uint32_t reserved;     /* solver colour kept between steps (internal; ignored on upload) */
     jh_contact contacts[4];
 } jh_arbiter;
 
@@ -161,6 +161,7 @@ typedef struct jh_stats {
     int64_t arbiters_solved;  /* arbiters in the last solve (before contact refresh removed broken ones) */
     double ms_broadphase, ms_narrowphase, ms_contact_cache, ms_colouring, ms_solve, ms_update;
     double ms_step;
+    double ms_epa;            /* part of ms_narrowphase spent in the EPA kernel */
 } jh_stats;
 
 typedef struct jh_world jh_world;

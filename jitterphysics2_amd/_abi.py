@@ -115,7 +115,7 @@ STATS_DTYPE = np.dtype([
     ("arbiters", "<i8"), ("contacts", "<i8"), ("colours", "<i8"), ("epa_calls", "<i8"),
     ("steps", "<i8"), ("pair_overflow", "<i8"), ("arbiters_solved", "<i8"),
     ("ms_broadphase", "<f8"), ("ms_narrowphase", "<f8"), ("ms_contact_cache", "<f8"),
-    ("ms_colouring", "<f8"), ("ms_solve", "<f8"), ("ms_update", "<f8"), ("ms_step", "<f8"),
+    ("ms_colouring", "<f8"), ("ms_solve", "<f8"), ("ms_update", "<f8"), ("ms_step", "<f8"), ("ms_epa", "<f8"),
 ])
 
 STRUCT_SIZES = {

@@ -1,7 +1,7 @@
 // EPA refinement on the GPU: the FP64 polytope with filtered exact orientation predicates of
 // ConvexPolytope.cs / PlaneFilter.cs / ExpansionArithmetic.cs.  One thread owns one deep pair and a
-// 40 KB workspace slab in HBM (the polytope is far too large for registers); deep pairs are rare,
-// so the EPA kernel runs on a compacted queue with a small fixed grid.
+// 38 KB polytope (far too large for registers) in an HBM workspace interleaved across threads; the
+// EPA kernel runs over the compacted queue of deep pairs.
 #pragma once
 #include "jh_collide.h"
 
@@ -24,13 +24,109 @@ struct Face {
     double bx, by, bz;  // barycentric
     double dist_sq;
 };
+
+// Per-thread polytope storage in HBM, interleaved across threads: element e of field f of thread t
+// lives at [(f * Elems + e) * T + t], so the lanes of a wave that touch the same element together
+// (face scans, vertex reads) issue coalesced accesses.  Scalars stay in registers.
+constexpr size_t kVertBytes = size_t(9) * MaxVertices * 4;
+constexpr size_t kTriIdxBytes = size_t(MaxTriangles) * 8;
+constexpr size_t kTriFBytes = size_t(8) * MaxTriangles * 4;
+constexpr size_t kFaceBytes = size_t(10) * MaxTriangles * 8;
+constexpr size_t kEdgeBytes = size_t(3) * MaxTriangles * 4;
+constexpr size_t kLitBytes = size_t(MaxTriangles);
+constexpr size_t kBytesPerThread = kVertBytes + kTriIdxBytes + kTriFBytes + kFaceBytes + kEdgeBytes + kLitBytes;
+
 struct Work {
-    MVertex v[MaxVertices];
-    Tri t[MaxTriangles];
-    Face f[MaxTriangles];
-    short ea[3 * MaxTriangles], eb[3 * MaxTriangles];
-    unsigned char lit[MaxTriangles];
-    int vcount, tcount, enclosed, pad;
+    float* vf;            // 9 x MaxVertices
+    unsigned long long* ti;  // MaxTriangles: packed a, b, c, facing
+    float* tf;            // 8 x MaxTriangles: normal, closest, normal_sq, closest_sq
+    double* fd;           // 10 x MaxTriangles: plane ex ey ez nx ny nz, bary x y z, dist_sq
+    unsigned* ed;         // 3 x MaxTriangles edges (a | b << 16)
+    unsigned char* lt;    // MaxTriangles
+    int T, tid;
+    int vcount, tcount, enclosed;
+
+    __device__ void bind(char* base, int threads, int t) {
+        T = threads;
+        tid = t;
+        size_t TT = size_t(threads);
+        vf = (float*)base;
+        ti = (unsigned long long*)(base + kVertBytes * TT);
+        tf = (float*)(base + (kVertBytes + kTriIdxBytes) * TT);
+        fd = (double*)(base + (kVertBytes + kTriIdxBytes + kTriFBytes) * TT);
+        ed = (unsigned*)(base + (kVertBytes + kTriIdxBytes + kTriFBytes + kFaceBytes) * TT);
+        lt = (unsigned char*)(base + (kVertBytes + kTriIdxBytes + kTriFBytes + kFaceBytes + kEdgeBytes) * TT);
+    }
+    __device__ __forceinline__ size_t at(int field, int elems, int e) const {
+        return (size_t(field) * elems + e) * size_t(T) + tid;
+    }
+    __device__ __forceinline__ vec3 V(int i) const {
+        return {vf[at(0, MaxVertices, i)], vf[at(1, MaxVertices, i)], vf[at(2, MaxVertices, i)]};
+    }
+    __device__ __forceinline__ MVertex vertex(int i) const {
+        MVertex m;
+        m.V = V(i);
+        m.A = {vf[at(3, MaxVertices, i)], vf[at(4, MaxVertices, i)], vf[at(5, MaxVertices, i)]};
+        m.B = {vf[at(6, MaxVertices, i)], vf[at(7, MaxVertices, i)], vf[at(8, MaxVertices, i)]};
+        return m;
+    }
+    __device__ __forceinline__ void set_vertex(int i, const MVertex& m) {
+        vf[at(0, MaxVertices, i)] = m.V.x; vf[at(1, MaxVertices, i)] = m.V.y; vf[at(2, MaxVertices, i)] = m.V.z;
+        vf[at(3, MaxVertices, i)] = m.A.x; vf[at(4, MaxVertices, i)] = m.A.y; vf[at(5, MaxVertices, i)] = m.A.z;
+        vf[at(6, MaxVertices, i)] = m.B.x; vf[at(7, MaxVertices, i)] = m.B.y; vf[at(8, MaxVertices, i)] = m.B.z;
+    }
+    __device__ __forceinline__ void tri_idx(int i, short& a, short& b, short& c, short& facing) const {
+        unsigned long long p = ti[at(0, MaxTriangles, i)];
+        a = short(p & 0xffff); b = short((p >> 16) & 0xffff); c = short((p >> 32) & 0xffff); facing = short(p >> 48);
+    }
+    __device__ __forceinline__ Tri tri(int i) const {
+        Tri t;
+        tri_idx(i, t.a, t.b, t.c, t.facing);
+        t.normal = {tf[at(0, MaxTriangles, i)], tf[at(1, MaxTriangles, i)], tf[at(2, MaxTriangles, i)]};
+        t.closest = {tf[at(3, MaxTriangles, i)], tf[at(4, MaxTriangles, i)], tf[at(5, MaxTriangles, i)]};
+        t.normal_sq = tf[at(6, MaxTriangles, i)];
+        t.closest_sq = tf[at(7, MaxTriangles, i)];
+        return t;
+    }
+    __device__ __forceinline__ void set_tri(int i, const Tri& t) {
+        ti[at(0, MaxTriangles, i)] = (unsigned long long)(unsigned short)t.a | ((unsigned long long)(unsigned short)t.b << 16) |
+                                     ((unsigned long long)(unsigned short)t.c << 32) |
+                                     ((unsigned long long)(unsigned short)t.facing << 48);
+        tf[at(0, MaxTriangles, i)] = t.normal.x; tf[at(1, MaxTriangles, i)] = t.normal.y; tf[at(2, MaxTriangles, i)] = t.normal.z;
+        tf[at(3, MaxTriangles, i)] = t.closest.x; tf[at(4, MaxTriangles, i)] = t.closest.y; tf[at(5, MaxTriangles, i)] = t.closest.z;
+        tf[at(6, MaxTriangles, i)] = t.normal_sq;
+        tf[at(7, MaxTriangles, i)] = t.closest_sq;
+    }
+    __device__ __forceinline__ Plane plane(int i) const {
+        return {fd[at(0, MaxTriangles, i)], fd[at(1, MaxTriangles, i)], fd[at(2, MaxTriangles, i)],
+                fd[at(3, MaxTriangles, i)], fd[at(4, MaxTriangles, i)], fd[at(5, MaxTriangles, i)]};
+    }
+    __device__ __forceinline__ double dist_sq(int i) const { return fd[at(9, MaxTriangles, i)]; }
+    __device__ __forceinline__ Face face(int i) const {
+        Face f;
+        f.f = plane(i);
+        f.bx = fd[at(6, MaxTriangles, i)];
+        f.by = fd[at(7, MaxTriangles, i)];
+        f.bz = fd[at(8, MaxTriangles, i)];
+        f.dist_sq = fd[at(9, MaxTriangles, i)];
+        return f;
+    }
+    __device__ __forceinline__ void set_face(int i, const Face& f) {
+        fd[at(0, MaxTriangles, i)] = f.f.ex; fd[at(1, MaxTriangles, i)] = f.f.ey; fd[at(2, MaxTriangles, i)] = f.f.ez;
+        fd[at(3, MaxTriangles, i)] = f.f.nx; fd[at(4, MaxTriangles, i)] = f.f.ny; fd[at(5, MaxTriangles, i)] = f.f.nz;
+        fd[at(6, MaxTriangles, i)] = f.bx; fd[at(7, MaxTriangles, i)] = f.by; fd[at(8, MaxTriangles, i)] = f.bz;
+        fd[at(9, MaxTriangles, i)] = f.dist_sq;
+    }
+    __device__ __forceinline__ void edge(int e, short& a, short& b) const {
+        unsigned p = ed[at(0, 3 * MaxTriangles, e)];
+        a = short(p & 0xffff);
+        b = short(p >> 16);
+    }
+    __device__ __forceinline__ void set_edge(int e, short a, short b) {
+        ed[at(0, 3 * MaxTriangles, e)] = (unsigned)(unsigned short)a | ((unsigned)(unsigned short)b << 16);
+    }
+    __device__ __forceinline__ bool lit(int i) const { return lt[at(0, MaxTriangles, i)] != 0; }
+    __device__ __forceinline__ void set_lit(int i, bool v) { lt[at(0, MaxTriangles, i)] = v ? 1 : 0; }
 };
 
 constexpr double ErrorFactor = 16 * 1.1102230246251565e-16;
@@ -148,11 +244,12 @@ enum { Added = 0, NoHorizon, Degenerate, CapacityExceeded, NumericalFailure };
 
 // ConvexPolytope.Side (:320-326)
 __device__ __forceinline__ int side(const Work& w, const Plane& pf, int a, int b, int c, int pi) {
-    vec3 pt = pi < 0 ? vec3{0.0f, 0.0f, 0.0f} : w.v[pi].V;
-    if (veq(pt, w.v[a].V) || veq(pt, w.v[b].V) || veq(pt, w.v[c].V)) return 0;
+    vec3 pt = pi < 0 ? vec3{0.0f, 0.0f, 0.0f} : w.V(pi);
+    vec3 va = w.V(a), vb = w.V(b), vc = w.V(c);
+    if (veq(pt, va) || veq(pt, vb) || veq(pt, vc)) return 0;
     int s;
-    if (plane_try_side(pf, w.v[a].V, pt, s)) return s;
-    return exact_side(w.v[a].V, w.v[b].V, w.v[c].V, pt);
+    if (plane_try_side(pf, va, pt, s)) return s;
+    return exact_side(va, vb, vc, pt);
 }
 
 __device__ __forceinline__ double ddot(double ax, double ay, double az, double bx, double by, double bz) {
@@ -163,7 +260,7 @@ __device__ __forceinline__ double ddot(double ax, double ay, double az, double b
 __device__ __noinline__ int create_triangle(Work& w, short a, short b, short c, int slot) {
     Tri tri{};
     Face fd{};
-    plane_init(fd.f, w.v[a].V, w.v[b].V, w.v[c].V);
+    plane_init(fd.f, w.V(a), w.V(b), w.V(c));
     int orientation = 0;
     for (short i = 0; i < 4 && orientation == 0; ++i) {
         if (i == a || i == b || i == c) continue;
@@ -180,14 +277,15 @@ __device__ __noinline__ int create_triangle(Work& w, short a, short b, short c, 
     }
     bool facing = side(w, fd.f, a, b, c, -1) <= 0;
     double rx = fd.f.nx, ry = fd.f.ny, rz = fd.f.nz;
-    if (!plane_reliable(fd.f)) exact_normal(w.v[a].V, w.v[b].V, w.v[c].V, rx, ry, rz);
+    if (!plane_reliable(fd.f)) exact_normal(w.V(a), w.V(b), w.V(c), rx, ry, rz);
     double ns = fmax(fabs(rx), fmax(fabs(ry), fabs(rz)));
     if (!(ns > 0) || !isfinite(ns)) return NumericalFailure;
     double nx = rx / ns, ny = ry / ns, nz = rz / ns;
     double nsq = ddot(nx, ny, nz, nx, ny, nz);
-    double ax = w.v[a].V.x, ay = w.v[a].V.y, az = w.v[a].V.z;
-    double bx = w.v[b].V.x, by = w.v[b].V.y, bz = w.v[b].V.z;
-    double cx = w.v[c].V.x, cy = w.v[c].V.y, cz = w.v[c].V.z;
+    vec3 Va = w.V(a), Vb = w.V(b), Vc = w.V(c);
+    double ax = Va.x, ay = Va.y, az = Va.z;
+    double bx = Vb.x, by = Vb.y, bz = Vb.z;
+    double cx = Vc.x, cy = Vc.y, cz = Vc.z;
     // CalcBarycentric
     double ux = ax - bx, uy = ay - by, uz = az - bz;
     double vx = ax - cx, vy = ay - cy, vz = az - cz;
@@ -265,8 +363,8 @@ __device__ __noinline__ int create_triangle(Work& w, short a, short b, short c, 
     tri.normal_sq = len_sq(tri.normal);
     tri.closest = vec3{float(clx), float(cly), float(clz)};
     tri.closest_sq = float(fd.dist_sq);
-    w.t[slot] = tri;
-    w.f[slot] = fd;
+    w.set_tri(slot, tri);
+    w.set_face(slot, fd);
     bool fin = isfinite(tri.closest.x) && isfinite(tri.closest.y) && isfinite(tri.closest.z) &&
                isfinite(tri.closest_sq);
     return fin ? Added : NumericalFailure;
@@ -278,7 +376,7 @@ __device__ __forceinline__ bool init_tetrahedron(Work& w) {
     w.tcount = 0;
     w.enclosed = 0;
     for (int i = 0; i < 4; ++i)
-        if (!is_safe(w.v[i].V)) return false;
+        if (!is_safe(w.V(i))) return false;
     const short idx[12] = {0, 2, 1, 0, 1, 3, 0, 3, 2, 1, 2, 3};
     for (int i = 0; i < 4; ++i)
         if (create_triangle(w, idx[3 * i], idx[3 * i + 1], idx[3 * i + 2], i) != Added) return false;
@@ -293,45 +391,50 @@ __device__ __forceinline__ Tri closest_triangle(Work& w) {
     int skip = w.enclosed;
     int enclosed = 1;
     for (int i = 0; i < w.tcount; ++i) {
-        if (w.f[i].dist_sq < minimum) {
-            minimum = w.f[i].dist_sq;
+        double d = w.dist_sq(i);
+        if (d < minimum) {
+            minimum = d;
             closest = i;
         }
-        if (!w.t[i].facing) enclosed = skip;
+        short a, b, c, facing;
+        w.tri_idx(i, a, b, c, facing);
+        if (!facing) enclosed = skip;
     }
     w.enclosed = enclosed;
-    return w.t[closest];
+    return w.tri(closest);
 }
 
 // ConvexPolytope.AddVertexDetailed (:199-265)
 __device__ __noinline__ int add_vertex(Work& w, const MVertex& vx) {
     if (w.tcount == 0 || !is_safe(vx.V)) return NumericalFailure;
     if (w.vcount == MaxVertices) return CapacityExceeded;
-    w.v[w.vcount] = vx;
+    w.set_vertex(w.vcount, vx);
     int ecount = 0, litcount = 0;
     for (int index = w.tcount; index-- > 0;) {
-        Tri f = w.t[index];
-        bool lit = side(w, w.f[index].f, f.a, f.b, f.c, w.vcount) > 0;
-        w.lit[index] = lit ? 1 : 0;
+        short fa, fb, fc, ffacing;
+        w.tri_idx(index, fa, fb, fc, ffacing);
+        bool lit = side(w, w.plane(index), fa, fb, fc, w.vcount) > 0;
+        w.set_lit(index, lit);
         if (!lit) continue;
         litcount++;
-        short corners[3] = {f.a, f.b, f.c};
+        short corners[3] = {fa, fb, fc};
         for (int k = 0; k < 3; ++k) {
             short A = corners[k], B = corners[(k + 1) % 3];
             bool added = true;
             for (int e = ecount; e-- > 0;) {
-                short ea = w.ea[e], eb = w.eb[e];
+                short ea, eb;
+                w.edge(e, ea, eb);
                 bool m = (ea == A && eb == B) || (ea == B && eb == A);
                 if (!m) continue;
                 --ecount;
-                w.ea[e] = w.ea[ecount];
-                w.eb[e] = w.eb[ecount];
+                short la, lb;
+                w.edge(ecount, la, lb);
+                w.set_edge(e, la, lb);
                 added = false;
                 break;
             }
             if (added) {
-                w.ea[ecount] = A;
-                w.eb[ecount] = B;
+                w.set_edge(ecount, A, B);
                 ecount++;
             }
         }
@@ -340,15 +443,17 @@ __device__ __noinline__ int add_vertex(Work& w, const MVertex& vx) {
     if (w.tcount - litcount + ecount > MaxTriangles) return CapacityExceeded;
     int surv = 0;
     for (int i = 0; i < w.tcount; ++i) {
-        if (w.lit[i]) continue;
+        if (w.lit(i)) continue;
         if (surv != i) {
-            w.t[surv] = w.t[i];
-            w.f[surv] = w.f[i];
+            w.set_tri(surv, w.tri(i));
+            w.set_face(surv, w.face(i));
         }
         surv++;
     }
     for (int i = 0; i < ecount; ++i) {
-        int r = create_triangle(w, w.ea[i], w.eb[i], short(w.vcount), surv + i);
+        short ea, eb;
+        w.edge(i, ea, eb);
+        int r = create_triangle(w, ea, eb, short(w.vcount), surv + i);
         if (r != Added) {
             w.tcount = 0;
             w.enclosed = 0;
@@ -360,7 +465,7 @@ __device__ __noinline__ int add_vertex(Work& w, const MVertex& vx) {
     return Added;
 }
 
-// MprEpaSolver.SolveMprEpa (NarrowPhase.cs:39-95); w.v[0..3] hold the MPR portal.
+// MprEpaSolver.SolveMprEpa (NarrowPhase.cs:39-95); vertices 0..3 hold the MPR portal.
 __device__ __noinline__ bool solve(Work& w, const MinkowskiPair& mk, vec3& p1, vec3& p2, vec3& normal, float& pen) {
     const float eps = 1e-5f;
     const float NumericEpsilon = 1e-16f;
@@ -393,9 +498,12 @@ __device__ __noinline__ bool solve(Work& w, const MinkowskiPair& mk, vec3& p1, v
     if (!converged || !w.enclosed) return false;
     // CalculatePoints (:169-186)
     for (int i = 0; i < w.tcount; ++i) {
-        if (w.t[i].a != ctri.a || w.t[i].b != ctri.b || w.t[i].c != ctri.c) continue;
-        double bx = w.f[i].bx, by = w.f[i].by, bz = w.f[i].bz;
-        const MVertex &A = w.v[ctri.a], &B = w.v[ctri.b], &C = w.v[ctri.c];
+        short ta, tb, tc, tfacing;
+        w.tri_idx(i, ta, tb, tc, tfacing);
+        if (ta != ctri.a || tb != ctri.b || tc != ctri.c) continue;
+        Face fc = w.face(i);
+        double bx = fc.bx, by = fc.by, bz = fc.bz;
+        const MVertex A = w.vertex(ctri.a), B = w.vertex(ctri.b), C = w.vertex(ctri.c);
         p1 = vec3{float((double)A.A.x * bx + (double)B.A.x * by + (double)C.A.x * bz),
                   float((double)A.A.y * bx + (double)B.A.y * by + (double)C.A.y * bz),
                   float((double)A.A.z * bx + (double)B.A.z * by + (double)C.A.z * bz)};

@@ -12,6 +12,8 @@ namespace cg = cooperative_groups;
 
 namespace jh {
 
+constexpr uint32_t kNoColour = 0xffffffffu;  // arbiter without a colour from a previous step
+
 // ---- Device state ---------------------------------------------------------------------------
 struct Bodies {
     float4 *pos, *quat, *vel, *angvel, *dvel, *dangvel;
@@ -32,6 +34,24 @@ struct StepTime {
     float dt, substep_dt, inv_dt, inv_substep_dt;
     int substeps;
 };
+
+// Grid barrier for a cooperative launch: one arrival per workgroup, sense by generation counter.
+__device__ __forceinline__ void grid_barrier(unsigned* bar) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* count = bar;
+        unsigned* gen = bar + 32;
+        unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
 
 __device__ __forceinline__ sym3 ld_iw(const Bodies& B, int b) {
     float4 a = B.iw0[b];
@@ -429,22 +449,25 @@ __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pai
     }
 }
 
-// EPA over the deep queue: a successful EPA replaces the MPR answer and its manifold.
+// EPA over the deep queue: a successful EPA replaces the MPR answer and its manifold.  `work`
+// holds gridDim.x * blockDim.x interleaved polytopes (epa::kBytesPerThread each).
 __global__ void __launch_bounds__(64) k_epa(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
                                             const float* __restrict__ verts, Bodies B, int aux,
                                             jh_manifold* __restrict__ out, const DeepRec* __restrict__ deep,
-                                            const int* __restrict__ deep_count, epa::Work* __restrict__ work) {
+                                            const int* __restrict__ deep_count, char* __restrict__ work) {
     int n = *deep_count;
+    int T = gridDim.x * blockDim.x;
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    epa::Work& w = work[tid];
-    for (int q = tid; q < n; q += gridDim.x * blockDim.x) {
+    epa::Work w;
+    w.bind(work, T, tid);
+    for (int q = tid; q < n; q += T) {
         const DeepRec& d = deep[q];
         uint64_t key = pairs[d.pair];
         MinkowskiPair mk;
         quat qa, qb;
         vec3 pa, pb;
         pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
-        for (int k = 0; k < 4; ++k) w.v[k] = d.portal[k];
+        for (int k = 0; k < 4; ++k) w.set_vertex(k, d.portal[k]);
         MprOut r;
         r.hit = 1;
         if (!epa::solve(w, mk, r.pa, r.pb, r.normal, r.pen)) continue;
@@ -523,6 +546,7 @@ __global__ void k_merge_new(const uint64_t* __restrict__ akey, const int* __rest
         a.friction = fmaxr(m1.y, m2.y);
         a.restitution = fmaxr(m1.z, m2.z);
         a.speculative_relaxation_factor = specrel;
+        a.reserved = kNoColour;
         aout[pos] = a;
         okey[pos] = k;
     }
@@ -666,7 +690,7 @@ __device__ __forceinline__ unsigned long long colour_prio(const jh_arbiter& r, i
 }
 
 struct ColourArgs {
-    const jh_arbiter* arb;
+    jh_arbiter* arb;     // jh_arbiter.reserved carries the arbiter's colour from step to step
     const int* a_count;
     const int* motion;
     int* colour;
@@ -676,10 +700,16 @@ struct ColourArgs {
     int* remaining;  // [2]
     int* ncolours;
     int* hist;       // [128]
+    unsigned* bar;   // grid barrier state
+    int keep;        // reuse the colours of arbiters that persist from the previous step
 };
 
+// Graph colouring of the arbiters: edges are arbiters, nodes are non-static bodies (static bodies
+// never change, so arbiters through them never conflict).  Arbiters keep their colour between
+// steps: the bodies of an arbiter never change, so persisting colours stay a proper colouring, and
+// only new arbiters are coloured, in Jones-Plassmann rounds (local priority maxima take the first
+// colour free on both bodies).  Colours >= 127 fall into a serial batch.
 __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
-    cg::grid_group grid = cg::this_grid();
     int A = *P.a_count;
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -696,23 +726,37 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
     }
     if (t0 < 128) P.hist[t0] = 0;
     if (t0 == 0) { P.remaining[0] = 0; P.remaining[1] = 0; *P.ncolours = 0; }
-    grid.sync();
+    grid_barrier(P.bar);
+    if (P.keep) {
+        for (int a = t0; a < A; a += stride) {
+            if (P.colour[a] != -1) continue;
+            const jh_arbiter& r = P.arb[a];
+            uint32_t c = r.reserved;
+            if (c >= uint32_t(kMaxColours)) continue;
+            P.colour[a] = int(c);
+            unsigned long long bit = 1ull << (c & 63);
+            unsigned long long* m1 = c < 64 ? P.mask_lo : P.mask_hi;
+            if (P.motion[r.body1] != JH_MOTION_STATIC) atomicOr(&m1[r.body1], bit);
+            if (P.motion[r.body2] != JH_MOTION_STATIC) atomicOr(&m1[r.body2], bit);
+            atomicAdd(&P.hist[c], 1);
+        }
+        grid_barrier(P.bar);
+    }
     for (int round = 0;; ++round) {
         for (int a = t0; a < A; a += stride) {
             if (P.colour[a] != -1) continue;
             const jh_arbiter& r = P.arb[a];
             unsigned long long prio = colour_prio(r, a);
-            if (P.motion[r.body1] == JH_MOTION_DYNAMIC) atomicMax(&P.body_max[r.body1], prio);
-            if (P.motion[r.body2] == JH_MOTION_DYNAMIC) atomicMax(&P.body_max[r.body2], prio);
+            if (P.motion[r.body1] != JH_MOTION_STATIC) atomicMax(&P.body_max[r.body1], prio);
+            if (P.motion[r.body2] != JH_MOTION_STATIC) atomicMax(&P.body_max[r.body2], prio);
         }
-        __threadfence();
-        grid.sync();
+        grid_barrier(P.bar);
         int* rem = &P.remaining[round & 1];
         for (int a = t0; a < A; a += stride) {
             if (P.colour[a] != -1) continue;
             const jh_arbiter& r = P.arb[a];
             unsigned long long prio = colour_prio(r, a);
-            bool d1 = P.motion[r.body1] == JH_MOTION_DYNAMIC, d2 = P.motion[r.body2] == JH_MOTION_DYNAMIC;
+            bool d1 = P.motion[r.body1] != JH_MOTION_STATIC, d2 = P.motion[r.body2] != JH_MOTION_STATIC;
             bool win = (!d1 || P.body_max[r.body1] == prio) && (!d2 || P.body_max[r.body2] == prio);
             if (!win) {
                 atomicAdd(rem, 1);
@@ -736,8 +780,7 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
             P.colour[a] = c;
             atomicAdd(&P.hist[c], 1);
         }
-        __threadfence();
-        grid.sync();
+        grid_barrier(P.bar);
         int left = atomicAdd(rem, 0);
         if (t0 == 0) P.remaining[(round + 1) & 1] = 0;
         for (int a = t0; a < A; a += stride) {
@@ -746,8 +789,12 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
             P.body_max[r.body1] = 0;
             P.body_max[r.body2] = 0;
         }
-        grid.sync();
+        grid_barrier(P.bar);
         if (left == 0) break;
+    }
+    for (int a = t0; a < A; a += stride) {
+        int c = P.colour[a];
+        P.arb[a].reserved = (c >= 0 && c < kMaxColours) ? uint32_t(c) : kNoColour;
     }
     if (t0 == 0) {
         int nc = 0;
@@ -772,34 +819,50 @@ __global__ void k_colour_keys(const int* __restrict__ colour, const jh_arbiter* 
     if (local) atomicAdd(contacts, local);
 }
 
-__global__ void k_colour_offsets(const int* __restrict__ hist, int* __restrict__ offs) {
+// Colour offsets, and chunk bases for the dataflow solver (64-arbiter chunks, serial colour: 1 each).
+__global__ void k_colour_offsets(const int* __restrict__ hist, int* __restrict__ offs, int* __restrict__ cbeg,
+                                 int* __restrict__ nchunks) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        int s = 0;
+        int s = 0, cs = 0;
         for (int c = 0; c < 128; ++c) {
             offs[c] = s;
+            cbeg[c] = cs;
             s += hist[c];
+            cs += c == kMaxColours ? hist[c] : (hist[c] + 63) / 64;
         }
         offs[128] = s;
+        cbeg[128] = cs;
+        *nchunks = cs;
     }
 }
 
 // ---- Solver ---------------------------------------------------------------------------------
+// Body state a contact needs, held in registers while one arbiter's slots are processed (the
+// reference updates the same RigidBodyData slot by slot; values are identical).
+struct BodyReg {
+    vec3 v, w, imv;
+    sym3 I;
+};
+
+__device__ __forceinline__ void load_static_part(const Bodies& B, int b, BodyReg& r) {
+    r.imv = ld3(B.imv[b]);
+    r.I = ld_iw(B, b);
+}
+
 // Contact.PrepareForIterationAccelerated (Contact.cs:1038-1180), no linear locks.
-__device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact& c, const Bodies& B,
-                                                const StepTime& ts) {
-    int b1 = cd.body1, b2 = cd.body2;
+__device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact& c, BodyReg& B1, BodyReg& B2,
+                                                vec3 x1, quat q1, vec3 x2, quat q2, const StepTime& ts) {
     uint32_t mode = cd.mode;
     bool skip = (c.flag & 4u) != 0;
     c.flag &= ~4u;
     float ws[3] = {skip ? 0.0f : c.accumulated[0], skip ? 0.0f : c.accumulated[1], skip ? 0.0f : c.accumulated[2]};
     vec3 pos1{c.position1[0], c.position1[1], c.position1[2]};
     vec3 pos2{c.position2[0], c.position2[1], c.position2[2]};
-    vec3 x1 = ld3(B.pos[b1]), x2 = ld3(B.pos[b2]);
-    vec3 cr1 = rot(pos1, ldq(B.quat[b1]));
-    vec3 cr2 = rot(pos2, ldq(B.quat[b2]));
+    vec3 cr1 = rot(pos1, q1);
+    vec3 cr2 = rot(pos2, q2);
     vec3 imv{0.0f, 0.0f, 0.0f};
-    if (mode & JH_SOLVE_LINEAR1) imv = ld3(B.imv[b1]);
-    if (mode & JH_SOLVE_LINEAR2) imv = imv + ld3(B.imv[b2]);
+    if (mode & JH_SOLVE_LINEAR1) imv = B1.imv;
+    if (mode & JH_SOLVE_LINEAR2) imv = imv + B2.imv;
     float k[3] = {imv.x, imv.x, imv.x};
     if ((c.flag & 2u) == 0) c.bias = 0.0f;
     vec3 r1 = cr1 + x1, r2 = cr2 + x2;
@@ -814,13 +877,10 @@ __device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact
     float ktx[3] = {0, 0, 0}, kty[3] = {0, 0, 0}, ktz[3] = {0, 0, 0};
     vec3 lin{ws[0] * NX[0] + ws[1] * NX[1] + ws[2] * NX[2], ws[0] * NY[0] + ws[1] * NY[1] + ws[2] * NY[2],
              ws[0] * NZ[0] + ws[1] * NZ[1] + ws[2] * NZ[2]};
-    if (mode & JH_SOLVE_LINEAR1) {
-        vec3 v = ld3(B.vel[b1]) - cmul(lin, ld3(B.imv[b1]));
-        B.vel[b1] = st3(v);
-    }
+    if (mode & JH_SOLVE_LINEAR1) B1.v = B1.v - cmul(lin, B1.imv);
     if (mode & JH_SOLVE_ANGULAR1) {
         vec3 rp{c.relative_position1[0], c.relative_position1[1], c.relative_position1[2]};
-        sym3 I = ld_iw(B, b1);
+        const sym3& I = B1.I;
         float e1[3], e2[3], e3[3];
         for (int l = 0; l < 3; ++l) {
             float rrx = rp.y * NZ[l] - rp.z * NY[l];
@@ -832,20 +892,17 @@ __device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact
         }
         vec3 ai{ws[0] * e1[0] + ws[1] * e1[1] + ws[2] * e1[2], ws[0] * e2[0] + ws[1] * e2[1] + ws[2] * e2[2],
                 ws[0] * e3[0] + ws[1] * e3[1] + ws[2] * e3[2]};
-        B.angvel[b1] = st3(ld3(B.angvel[b1]) - ai);
+        B1.w = B1.w - ai;
         for (int l = 0; l < 3; ++l) {
             ktx[l] = e2[l] * rp.z - e3[l] * rp.y;
             kty[l] = e3[l] * rp.x - e1[l] * rp.z;
             ktz[l] = e1[l] * rp.y - e2[l] * rp.x;
         }
     }
-    if (mode & JH_SOLVE_LINEAR2) {
-        vec3 v = ld3(B.vel[b2]) + cmul(lin, ld3(B.imv[b2]));
-        B.vel[b2] = st3(v);
-    }
+    if (mode & JH_SOLVE_LINEAR2) B2.v = B2.v + cmul(lin, B2.imv);
     if (mode & JH_SOLVE_ANGULAR2) {
         vec3 rp{c.relative_position2[0], c.relative_position2[1], c.relative_position2[2]};
-        sym3 I = ld_iw(B, b2);
+        const sym3& I = B2.I;
         float f1[3], f2[3], f3[3];
         for (int l = 0; l < 3; ++l) {
             float rrx = rp.y * NZ[l] - rp.z * NY[l];
@@ -857,7 +914,7 @@ __device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact
         }
         vec3 ai{ws[0] * f1[0] + ws[1] * f1[1] + ws[2] * f1[2], ws[0] * f2[0] + ws[1] * f2[1] + ws[2] * f2[2],
                 ws[0] * f3[0] + ws[1] * f3[1] + ws[2] * f3[2]};
-        B.angvel[b2] = st3(ld3(B.angvel[b2]) + ai);
+        B2.w = B2.w + ai;
         for (int l = 0; l < 3; ++l) {
             ktx[l] = ktx[l] + (f2[l] * rp.z - f3[l] * rp.y);
             kty[l] = kty[l] + (f3[l] * rp.x - f1[l] * rp.z);
@@ -874,17 +931,15 @@ __device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact
 }
 
 // Contact.IterateAccelerated (Contact.cs:1183-1248)
-__device__ __forceinline__ void iterate_contact(const jh_arbiter& cd, jh_contact& c, const Bodies& B, bool apply_bias) {
-    int b1 = cd.body1, b2 = cd.body2;
+__device__ __forceinline__ void iterate_contact(const jh_arbiter& cd, jh_contact& c, BodyReg& B1, BodyReg& B2,
+                                                bool apply_bias) {
     uint32_t mode = cd.mode;
-    vec3 v1 = ld3(B.vel[b1]), v2 = ld3(B.vel[b2]);
     vec3 rel1{c.relative_position1[0], c.relative_position1[1], c.relative_position1[2]};
     vec3 rel2{c.relative_position2[0], c.relative_position2[1], c.relative_position2[2]};
-    vec3 dv = v2 - v1;
+    vec3 dv = B2.v - B1.v;
     if (mode & (JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2)) {
-        vec3 w1 = ld3(B.angvel[b1]), w2 = ld3(B.angvel[b2]);
-        dv = v2 + cross(w2, rel2);
-        dv = dv - (v1 + cross(w1, rel1));
+        dv = B2.v + cross(B2.w, rel2);
+        dv = dv - (B1.v + cross(B1.w, rel1));
     }
     float bias = c.bias;
     if (apply_bias && c.penalty_bias > 0) bias = fmaxr(c.penalty_bias, c.bias);
@@ -908,10 +963,24 @@ __device__ __forceinline__ void iterate_contact(const jh_arbiter& cd, jh_contact
     }
     vec3 lin{imp[0] * NX[0] + imp[1] * NX[1] + imp[2] * NX[2], imp[0] * NY[0] + imp[1] * NY[1] + imp[2] * NY[2],
              imp[0] * NZ[0] + imp[1] * NZ[1] + imp[2] * NZ[2]};
-    if (mode & JH_SOLVE_LINEAR1) B.vel[b1] = st3(v1 - cmul(lin, ld3(B.imv[b1])));
-    if (mode & JH_SOLVE_ANGULAR1) B.angvel[b1] = st3(ld3(B.angvel[b1]) - sym_mul(cross(rel1, lin), ld_iw(B, b1)));
-    if (mode & JH_SOLVE_LINEAR2) B.vel[b2] = st3(v2 + cmul(lin, ld3(B.imv[b2])));
-    if (mode & JH_SOLVE_ANGULAR2) B.angvel[b2] = st3(ld3(B.angvel[b2]) + sym_mul(cross(rel2, lin), ld_iw(B, b2)));
+    if (mode & JH_SOLVE_LINEAR1) B1.v = B1.v - cmul(lin, B1.imv);
+    if (mode & JH_SOLVE_ANGULAR1) B1.w = B1.w - sym_mul(cross(rel1, lin), B1.I);
+    if (mode & JH_SOLVE_LINEAR2) B2.v = B2.v + cmul(lin, B2.imv);
+    if (mode & JH_SOLVE_ANGULAR2) B2.w = B2.w + sym_mul(cross(rel2, lin), B2.I);
+}
+
+// phase 0: prepare, 1: iterate with bias, 2: iterate without bias (relaxation)
+__device__ __forceinline__ void arbiter_pass(jh_arbiter& a, BodyReg& B1, BodyReg& B2, const Bodies& B, int phase,
+                                             const StepTime& ts) {
+    if (phase == 0) {
+        vec3 x1 = ld3(B.pos[a.body1]), x2 = ld3(B.pos[a.body2]);
+        quat q1 = ldq(B.quat[a.body1]), q2 = ldq(B.quat[a.body2]);
+        for (int s = 0; s < 4; ++s)
+            if (a.usage_mask & (1u << s)) prepare_contact(a, a.contacts[s], B1, B2, x1, q1, x2, q2, ts);
+    } else {
+        for (int s = 0; s < 4; ++s)
+            if (a.usage_mask & (1u << s)) iterate_contact(a, a.contacts[s], B1, B2, phase == 1);
+    }
 }
 
 struct SolveArgs {
@@ -919,74 +988,405 @@ struct SolveArgs {
     const uint64_t* order;   // (colour << 32) | arbiter, sorted
     const int* offs;         // 129 colour offsets
     const int* ncolours;
+    const int2* chunks;      // [begin, end) of the colour-sorted order; one colour per chunk
+    const int* nchunks;
+    const int2* rank;        // per order position: rank of the arbiter in each body's update sequence
+    const int* deg;          // per body: arbiters touching it
+    int* counter;            // per body: updates applied in this step
+    unsigned* bar;           // grid barrier state
     Bodies B;
     int nb;
     StepTime ts;
     int iterations, relaxations;
 };
 
-__device__ __forceinline__ void solve_batches(const SolveArgs& P, cg::grid_group& grid, int phase, int nc) {
+__device__ __forceinline__ void integrate_forces(const SolveArgs& P, int t0, int stride) {
+    const Bodies& B = P.B;
+    for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
+        if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
+        B.angvel[b] = st3(ld3(B.angvel[b]) + ld3(B.dangvel[b]));
+        B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
+    }
+}
+__device__ __forceinline__ void integrate_velocities(const SolveArgs& P, int t0, int stride) {
+    const Bodies& B = P.B;
+    for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
+        if (B.motion[b] == JH_MOTION_STATIC) continue;
+        vec3 p = ld3(B.pos[b]) + ld3(B.vel[b]) * P.ts.substep_dt;
+        B.pos[b] = st3(p);
+        quat rq = rotation_quaternion(ld3(B.angvel[b]), P.ts.substep_dt);
+        B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+    }
+}
+
+// Barrier solver: one grid barrier per colour batch per pass.
+__device__ __forceinline__ void solve_batches(const SolveArgs& P, int phase, int nc) {
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const Bodies& B = P.B;
     for (int c = 0; c <= nc; ++c) {
         int col = c < nc ? c : kMaxColours;
         int beg = P.offs[col], end = P.offs[col + 1];
         if (beg == end) continue;
-        if (col == kMaxColours) {
-            if (t0 == 0)
-                for (int k = beg; k < end; ++k) {
-                    jh_arbiter& a = P.arb[uint32_t(P.order[k])];
-                    for (int s = 0; s < 4; ++s)
-                        if (a.usage_mask & (1u << s)) {
-                            if (phase == 0) prepare_contact(a, a.contacts[s], P.B, P.ts);
-                            else iterate_contact(a, a.contacts[s], P.B, phase == 1);
-                        }
-                }
-        } else {
-            for (int k = beg + t0; k < end; k += stride) {
+        int first = col == kMaxColours ? 0 : t0;
+        int step = col == kMaxColours ? 1 : stride;
+        if (col != kMaxColours || t0 == 0)
+            for (int k = beg + first; k < end; k += step) {
                 jh_arbiter& a = P.arb[uint32_t(P.order[k])];
-                for (int s = 0; s < 4; ++s)
-                    if (a.usage_mask & (1u << s)) {
-                        if (phase == 0) prepare_contact(a, a.contacts[s], P.B, P.ts);
-                        else iterate_contact(a, a.contacts[s], P.B, phase == 1);
-                    }
+                BodyReg b1, b2;
+                b1.v = ld3(B.vel[a.body1]); b1.w = ld3(B.angvel[a.body1]); load_static_part(B, a.body1, b1);
+                b2.v = ld3(B.vel[a.body2]); b2.w = ld3(B.angvel[a.body2]); load_static_part(B, a.body2, b2);
+                arbiter_pass(a, b1, b2, B, phase, P.ts);
+                if (a.mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) { B.vel[a.body1] = st3(b1.v); B.angvel[a.body1] = st3(b1.w); }
+                if (a.mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) { B.vel[a.body2] = st3(b2.v); B.angvel[a.body2] = st3(b2.w); }
             }
-        }
-        __threadfence();
-        grid.sync();
+        grid_barrier(P.bar);
     }
 }
 
-// One launch per step: for each substep IntegrateForces, SolveVelocities (prepare + iterations),
-// IntegrateVelocities, RelaxVelocities (World.Step.cs:234-253), colour batch by colour batch.
 __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
-    cg::grid_group grid = cg::this_grid();
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = *P.ncolours;
-    const Bodies& B = P.B;
     for (int s = 0; s < P.ts.substeps; ++s) {
-        // IntegrateForces (World.Step.cs:1060-1072)
-        for (int b = t0; b < P.nb; b += stride) {
-            if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
-            B.angvel[b] = st3(ld3(B.angvel[b]) + ld3(B.dangvel[b]));
-            B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
+        integrate_forces(P, t0, stride);
+        grid_barrier(P.bar);
+        solve_batches(P, 0, nc);
+        for (int it = 0; it < P.iterations; ++it) solve_batches(P, 1, nc);
+        integrate_velocities(P, t0, stride);
+        grid_barrier(P.bar);
+        for (int it = 0; it < P.relaxations; ++it) solve_batches(P, 2, nc);
+    }
+}
+
+// ---- Dataflow solver ------------------------------------------------------------------------
+// Every non-static body counts the arbiter visits applied to it in this step.  In global pass g
+// (prepare, iterations and relaxations of every substep, in order), the arbiter at colour-order
+// position k waits until each of its bodies has seen g * deg(b) + rank(k, b) visits: exactly the
+// visits that precede it in the colour-ordered Gauss-Seidel sequence.  Each body therefore sees its
+// updates in the same order as the barrier solver (bit-identical results) without grid barriers
+// between colours.  Shared velocities move through agent-coherent (sc1) 64-bit atomics; a completed
+// store is visible device-wide before the counter increment that publishes it (s_waitcnt).
+// Waves own chunks statically (chunk j -> wave j mod W, for every pass), processed in global order,
+// so the earliest unfinished chunk always has its dependencies met: the loop cannot deadlock on a
+// cooperative (fully resident) grid.  Arbiter contact data stays with its owning wave.
+__device__ __forceinline__ void coh_load_vw(const Bodies& B, int b, vec3& v, vec3& w) {
+    const unsigned long long* pv = (const unsigned long long*)&B.vel[b];
+    const unsigned long long* pw = (const unsigned long long*)&B.angvel[b];
+    unsigned long long v0 = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long v1 = __hip_atomic_load(pv + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long w0 = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long w1 = __hip_atomic_load(pw + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = {__uint_as_float(uint32_t(v0)), __uint_as_float(uint32_t(v0 >> 32)), __uint_as_float(uint32_t(v1))};
+    w = {__uint_as_float(uint32_t(w0)), __uint_as_float(uint32_t(w0 >> 32)), __uint_as_float(uint32_t(w1))};
+}
+__device__ __forceinline__ unsigned long long pack2(float a, float b) {
+    return (unsigned long long)__float_as_uint(a) | ((unsigned long long)__float_as_uint(b) << 32);
+}
+__device__ __forceinline__ void coh_store_vw(const Bodies& B, int b, vec3 v, vec3 w) {
+    unsigned long long* pv = (unsigned long long*)&B.vel[b];
+    unsigned long long* pw = (unsigned long long*)&B.angvel[b];
+    __hip_atomic_store(pv, pack2(v.x, v.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(pv + 1, pack2(v.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(pw, pack2(w.x, w.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(pw + 1, pack2(w.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Contact fields the solver reads/writes, held in registers for one arbiter visit.
+struct SlotReg {
+    uint32_t flag;
+    float bias, penalty;
+    float nx[3], ny[3], nz[3];  // lanes: normal, tangent1, tangent2
+    float mass[3], acc[3];
+    vec3 rel1, rel2, pos1, pos2;
+};
+
+__device__ __forceinline__ void load_slot(const jh_contact& c, SlotReg& r, bool prepare) {
+    r.flag = c.flag;
+    r.bias = c.bias;
+    r.penalty = c.penalty_bias;
+    r.nx[0] = c.normal[0]; r.ny[0] = c.normal[1]; r.nz[0] = c.normal[2];
+    r.nx[1] = c.tangent1[0]; r.ny[1] = c.tangent1[1]; r.nz[1] = c.tangent1[2];
+    r.nx[2] = c.tangent2[0]; r.ny[2] = c.tangent2[1]; r.nz[2] = c.tangent2[2];
+    for (int l = 0; l < 3; ++l) { r.mass[l] = c.mass_normal_tangent[l]; r.acc[l] = c.accumulated[l]; }
+    r.rel1 = vec3{c.relative_position1[0], c.relative_position1[1], c.relative_position1[2]};
+    r.rel2 = vec3{c.relative_position2[0], c.relative_position2[1], c.relative_position2[2]};
+    if (prepare) {
+        r.pos1 = vec3{c.position1[0], c.position1[1], c.position1[2]};
+        r.pos2 = vec3{c.position2[0], c.position2[1], c.position2[2]};
+    }
+}
+__device__ __forceinline__ void store_slot(jh_contact& c, const SlotReg& r, bool prepare) {
+    if (prepare) {
+        c.flag = r.flag;
+        c.bias = r.bias;
+        c.penalty_bias = r.penalty;
+        for (int l = 0; l < 3; ++l) c.mass_normal_tangent[l] = r.mass[l];
+    }
+    for (int l = 0; l < 3; ++l) c.accumulated[l] = r.acc[l];
+}
+
+// Same arithmetic as prepare_contact, on register copies.
+__device__ __forceinline__ void prepare_reg(uint32_t mode, float specrel, SlotReg& c, BodyReg& B1, BodyReg& B2,
+                                            vec3 x1, quat q1, vec3 x2, quat q2, const StepTime& ts) {
+    bool skip = (c.flag & 4u) != 0;
+    c.flag &= ~4u;
+    float ws[3] = {skip ? 0.0f : c.acc[0], skip ? 0.0f : c.acc[1], skip ? 0.0f : c.acc[2]};
+    vec3 cr1 = rot(c.pos1, q1);
+    vec3 cr2 = rot(c.pos2, q2);
+    vec3 imv{0.0f, 0.0f, 0.0f};
+    if (mode & JH_SOLVE_LINEAR1) imv = B1.imv;
+    if (mode & JH_SOLVE_LINEAR2) imv = imv + B2.imv;
+    float k[3] = {imv.x, imv.x, imv.x};
+    if ((c.flag & 2u) == 0) c.bias = 0.0f;
+    vec3 r1 = cr1 + x1, r2 = cr2 + x2;
+    vec3 dist = r1 - r2;
+    const float* NX = c.nx;
+    const float* NY = c.ny;
+    const float* NZ = c.nz;
+    vec3 n{NX[0], NY[0], NZ[0]};
+    float pen = dot(dist, n);
+    if (pen < -0.02f) c.bias = pen * ts.inv_dt * specrel;
+    c.flag &= ~2u;
+    float ktx[3] = {0, 0, 0}, kty[3] = {0, 0, 0}, ktz[3] = {0, 0, 0};
+    vec3 lin{ws[0] * NX[0] + ws[1] * NX[1] + ws[2] * NX[2], ws[0] * NY[0] + ws[1] * NY[1] + ws[2] * NY[2],
+             ws[0] * NZ[0] + ws[1] * NZ[1] + ws[2] * NZ[2]};
+    if (mode & JH_SOLVE_LINEAR1) B1.v = B1.v - cmul(lin, B1.imv);
+    if (mode & JH_SOLVE_ANGULAR1) {
+        vec3 rp = c.rel1;
+        const sym3& I = B1.I;
+        float e1[3], e2[3], e3[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            float rrx = rp.y * NZ[l] - rp.z * NY[l];
+            float rry = rp.z * NX[l] - rp.x * NZ[l];
+            float rrz = rp.x * NY[l] - rp.y * NX[l];
+            e1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+            e2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+            e3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
         }
-        __threadfence();
-        grid.sync();
-        solve_batches(P, grid, 0, nc);
-        for (int it = 0; it < P.iterations; ++it) solve_batches(P, grid, 1, nc);
-        // IntegrateVelocities (World.Step.cs:1106-1133)
-        for (int b = t0; b < P.nb; b += stride) {
-            if (B.motion[b] == JH_MOTION_STATIC) continue;
-            vec3 p = ld3(B.pos[b]) + ld3(B.vel[b]) * P.ts.substep_dt;
-            B.pos[b] = st3(p);
-            quat rq = rotation_quaternion(ld3(B.angvel[b]), P.ts.substep_dt);
-            B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+        vec3 ai{ws[0] * e1[0] + ws[1] * e1[1] + ws[2] * e1[2], ws[0] * e2[0] + ws[1] * e2[1] + ws[2] * e2[2],
+                ws[0] * e3[0] + ws[1] * e3[1] + ws[2] * e3[2]};
+        B1.w = B1.w - ai;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            ktx[l] = e2[l] * rp.z - e3[l] * rp.y;
+            kty[l] = e3[l] * rp.x - e1[l] * rp.z;
+            ktz[l] = e1[l] * rp.y - e2[l] * rp.x;
         }
-        __threadfence();
-        grid.sync();
-        for (int it = 0; it < P.relaxations; ++it) solve_batches(P, grid, 2, nc);
+    }
+    if (mode & JH_SOLVE_LINEAR2) B2.v = B2.v + cmul(lin, B2.imv);
+    if (mode & JH_SOLVE_ANGULAR2) {
+        vec3 rp = c.rel2;
+        const sym3& I = B2.I;
+        float f1[3], f2[3], f3[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            float rrx = rp.y * NZ[l] - rp.z * NY[l];
+            float rry = rp.z * NX[l] - rp.x * NZ[l];
+            float rrz = rp.x * NY[l] - rp.y * NX[l];
+            f1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+            f2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+            f3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+        }
+        vec3 ai{ws[0] * f1[0] + ws[1] * f1[1] + ws[2] * f1[2], ws[0] * f2[0] + ws[1] * f2[1] + ws[2] * f2[2],
+                ws[0] * f3[0] + ws[1] * f3[1] + ws[2] * f3[2]};
+        B2.w = B2.w + ai;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            ktx[l] = ktx[l] + (f2[l] * rp.z - f3[l] * rp.y);
+            kty[l] = kty[l] + (f3[l] * rp.x - f1[l] * rp.z);
+            ktz[l] = ktz[l] + (f1[l] * rp.y - f2[l] * rp.x);
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        float kres = NX[l] * ktx[l] + NY[l] * kty[l] + NZ[l] * ktz[l];
+        float kk = k[l] + kres;
+        c.mass[l] = kk > 0.0f ? 1.0f / kk : 0.0f;
+    }
+    float pb = 0.2f * ts.inv_dt * fmaxr(0.0f, pen - 0.01f);
+    c.penalty = fminr(pb, 100.0f);
+}
+
+// Same arithmetic as iterate_contact, on register copies.
+__device__ __forceinline__ void iterate_reg(uint32_t mode, float friction, SlotReg& c, BodyReg& B1, BodyReg& B2,
+                                            bool apply_bias) {
+    vec3 dv = B2.v - B1.v;
+    if (mode & (JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2)) {
+        dv = B2.v + cross(B2.w, c.rel2);
+        dv = dv - (B1.v + cross(B1.w, c.rel1));
+    }
+    float bias = c.bias;
+    if (apply_bias && c.penalty > 0) bias = fmaxr(c.penalty, c.bias);
+    float old[3] = {c.acc[0], c.acc[1], c.acc[2]};
+    float bv[3] = {bias, 0.0f, 0.0f};
+    float imp[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        float vd = c.nx[l] * dv.x + c.ny[l] * dv.y + c.nz[l] * dv.z;
+        imp[l] = c.mass[l] * (bv[l] - vd);
+    }
+    float maxT = friction * old[0];
+    float lo[3] = {0.0f, -maxT, -maxT}, hi[3] = {3.402823466e+38f, maxT, maxT};
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        float a = old[l] + imp[l];
+        a = fminr(fmaxr(a, lo[l]), hi[l]);
+        c.acc[l] = a;
+        imp[l] = a - old[l];
+    }
+    vec3 lin{imp[0] * c.nx[0] + imp[1] * c.nx[1] + imp[2] * c.nx[2], imp[0] * c.ny[0] + imp[1] * c.ny[1] + imp[2] * c.ny[2],
+             imp[0] * c.nz[0] + imp[1] * c.nz[1] + imp[2] * c.nz[2]};
+    if (mode & JH_SOLVE_LINEAR1) B1.v = B1.v - cmul(lin, B1.imv);
+    if (mode & JH_SOLVE_ANGULAR1) B1.w = B1.w - sym_mul(cross(c.rel1, lin), B1.I);
+    if (mode & JH_SOLVE_LINEAR2) B2.v = B2.v + cmul(lin, B2.imv);
+    if (mode & JH_SOLVE_ANGULAR2) B2.w = B2.w + sym_mul(cross(c.rel2, lin), B2.I);
+}
+
+// One dataflow visit of the arbiter at colour-order position k in global pass g.  Everything private
+// (contact slots, body inertia, poses) is fetched before the dependency wait; after the wait only
+// the shared velocities are read, updated in registers, stored coherently and published; the
+// private contact results are written back after publishing.
+template <bool PREPARE>
+__device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, bool apply_bias) {
+    const Bodies& B = P.B;
+    jh_arbiter& a = P.arb[uint32_t(P.order[k])];
+    int b1 = a.body1, b2 = a.body2;
+    uint32_t usage = a.usage_mask, mode = a.mode;
+    float friction = a.friction, specrel = a.speculative_relaxation_factor;
+    bool m1 = B.motion[b1] != JH_MOTION_STATIC, m2 = B.motion[b2] != JH_MOTION_STATIC;
+    int2 r = P.rank[k];
+    int need1 = m1 ? g * P.deg[b1] + r.x : 0;
+    int need2 = m2 ? g * P.deg[b2] + r.y : 0;
+    SlotReg sr[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (usage & (1u << s)) load_slot(a.contacts[s], sr[s], PREPARE);
+    BodyReg R1, R2;
+    load_static_part(B, b1, R1);
+    load_static_part(B, b2, R2);
+    vec3 x1, x2;
+    quat q1, q2;
+    if (PREPARE) {
+        x1 = ld3(B.pos[b1]);
+        x2 = ld3(B.pos[b2]);
+        q1 = ldq(B.quat[b1]);
+        q2 = ldq(B.quat[b2]);
+    }
+    if (!m1) { R1.v = ld3(B.vel[b1]); R1.w = ld3(B.angvel[b1]); }
+    if (!m2) { R2.v = ld3(B.vel[b2]); R2.w = ld3(B.angvel[b2]); }
+    while (true) {
+        int c1 = m1 ? __hip_atomic_load(&P.counter[b1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        int c2 = m2 ? __hip_atomic_load(&P.counter[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        if (c1 >= need1 && c2 >= need2) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (m1) coh_load_vw(B, b1, R1.v, R1.w);
+    if (m2) coh_load_vw(B, b2, R2.v, R2.w);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (!(usage & (1u << s))) continue;
+        if (PREPARE) prepare_reg(mode, specrel, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
+        else iterate_reg(mode, friction, sr[s], R1, R2, apply_bias);
+    }
+    if (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) coh_store_vw(B, b1, R1.v, R1.w);
+    if (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) coh_store_vw(B, b2, R2.v, R2.w);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (m1) __hip_atomic_fetch_add(&P.counter[b1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (m2) __hip_atomic_fetch_add(&P.counter[b2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (usage & (1u << s)) store_slot(a.contacts[s], sr[s], PREPARE);
+}
+
+__device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int npass, int phase_first,
+                                            int phase_rest) {
+    int lane = threadIdx.x & 63;
+    int W = (gridDim.x * blockDim.x) >> 6;
+    int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int J = *P.nchunks;
+    for (int p = 0; p < npass; ++p) {
+        int g = g_first + p;
+        int phase = p == 0 ? phase_first : phase_rest;
+        for (int j = gw; j < J; j += W) {
+            int2 ch = P.chunks[j];
+            int k = ch.x + lane;
+            if (k >= ch.y) continue;
+            if (phase == 0) flow_visit<true>(P, k, g, false);
+            else flow_visit<false>(P, k, g, phase == 1);
+        }
+    }
+}
+
+// One launch per step (World.Step.cs:234-253): per substep IntegrateForces | prepare + iterations
+// (dataflow) | IntegrateVelocities | relaxations (dataflow), grid barriers only between these phases.
+__global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
+    int stride = gridDim.x * blockDim.x;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    int per_sub = 1 + P.iterations + P.relaxations;
+    for (int s = 0; s < P.ts.substeps; ++s) {
+        integrate_forces(P, t0, stride);
+        grid_barrier(P.bar);
+        flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
+        grid_barrier(P.bar);
+        integrate_velocities(P, t0, stride);
+        grid_barrier(P.bar);
+        if (P.relaxations > 0) {
+            flow_passes(P, s * per_sub + 1 + P.iterations, P.relaxations, 2, 2);
+            grid_barrier(P.bar);
+        }
+    }
+}
+
+// Per-step set-up of the dataflow solver.
+// Rank keys: (body << 32) | (order position << 1 | which) for each non-static body of each solved arbiter.
+__global__ void k_rank_keys(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
+                            const int* __restrict__ offs, const int* __restrict__ motion, uint64_t* __restrict__ keys,
+                            int* __restrict__ nkeys) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        const jh_arbiter& a = arb[uint32_t(order[k])];
+        if (motion[a.body1] != JH_MOTION_STATIC) {
+            int s = atomicAdd(nkeys, 1);
+            keys[s] = (uint64_t(uint32_t(a.body1)) << 32) | (uint64_t(k) << 1);
+        }
+        if (motion[a.body2] != JH_MOTION_STATIC) {
+            int s = atomicAdd(nkeys, 1);
+            keys[s] = (uint64_t(uint32_t(a.body2)) << 32) | (uint64_t(k) << 1) | 1u;
+        }
+    }
+}
+// After sorting the rank keys: rank = position within the body's segment; deg = segment length.
+__global__ void k_rank_assign(const uint64_t* __restrict__ keys, const int* __restrict__ nkeys,
+                              int2* __restrict__ rank, int* __restrict__ deg, int* __restrict__ counter) {
+    int n = *nkeys;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t key = keys[i];
+        uint64_t b = key >> 32;
+        int lo = lower_bound_u64(keys, n, b << 32);
+        uint32_t pos = uint32_t(key & 0xffffffffu);
+        int k = int(pos >> 1);
+        if (pos & 1u) rank[k].y = i - lo;
+        else rank[k].x = i - lo;
+        if (i == lo) {
+            deg[b] = lower_bound_u64(keys, n, (b + 1) << 32) - lo;
+            counter[b] = 0;
+        }
+    }
+}
+// Chunks of at most 64 arbiters of one colour; the serial overflow colour gets one arbiter per chunk.
+__global__ void k_chunks(const uint64_t* __restrict__ order, const int* __restrict__ offs, const int* __restrict__ cbeg,
+                         int2* __restrict__ chunks) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        int c = int(order[k] >> 32);
+        int rel = k - offs[c];
+        if (c == kMaxColours) chunks[cbeg[c] + rel] = make_int2(k, k + 1);
+        else if ((rel & 63) == 0) chunks[cbeg[c] + (rel >> 6)] = make_int2(k, min(k + 64, offs[c + 1]));
     }
 }
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -75,10 +76,12 @@ enum {
     C_NS = 10,   // shape count
     C_NB = 11,   // body count
     C_CONTACTS = 12,  // contact points solved
+    C_NCHUNK = 13,    // dataflow solver chunks
+    C_NRANK = 14,     // rank keys
     C_COUNT = 16
 };
 
-constexpr int kEpaThreads = 2048;
+constexpr int kEpaMaxThreads = 32768;  // polytopes resident at once (epa::kBytesPerThread each)
 
 int bits_for(uint32_t v) {
     int b = 0;
@@ -126,7 +129,8 @@ struct jh_world {
     // narrowphase
     DBuf<jh_manifold> man;
     DBuf<DeepRec> deep;
-    DBuf<epa::Work> epa_work;
+    DBuf<char> epa_work;
+    int epa_threads = 0;
     // contact cache
     DBuf<uint32_t> flags, offs, sums, cnew, noffs;
     DBuf<uint64_t> ckey, nkey;
@@ -139,11 +143,18 @@ struct jh_world {
     DBuf<int> colour, chist, coffs;
     DBuf<unsigned long long> bmax, mlo, mhi;
     DBuf<uint64_t> ok0, ok1, solve_keys;
+    // dataflow solver
+    DBuf<uint64_t> rk0, rk1;
+    DBuf<int2> rank, chunks;
+    DBuf<int> deg, bcounter, cbeg;
+    DBuf<unsigned> bar;
+    bool flow_solver = true;
+    bool recolour = true;  // next colouring ignores the colours stored in the arbiters
     int solve_count = 0, solve_batches = 0;
     std::vector<int> solve_offsets;
     // cooperative occupancy
-    int coop_colour = 0, coop_solve = 0;
-    hipEvent_t ev[8];
+    int coop_colour = 0, coop_solve = 0, flow_blocks = 0, colour_blocks = 64;
+    hipEvent_t ev[8];  // 0 start, 1 broadphase, 2 narrowphase, 3 cache, 4 colouring, 5 solve, 6 end, 7 EPA start
     jh_stats stats;
     bool have_step = false;
 
@@ -160,7 +171,8 @@ namespace {
 
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
-    if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS) cnt[t] = 0;
+    if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK)
+        cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
     else if (t < 6) bounds[t] = 0u;
 }
@@ -274,6 +286,10 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->ok0.ensure(na));
     HIPCHK(w->ok1.ensure(na));
     HIPCHK(w->solve_keys.ensure(na));
+    HIPCHK(w->rk0.ensure(2 * na));
+    HIPCHK(w->rk1.ensure(2 * na));
+    HIPCHK(w->rank.ensure(na));
+    HIPCHK(w->chunks.ensure(na));
     HIPCHK(w->flags.ensure(na));
     HIPCHK(w->offs.ensure(na));
     HIPCHK(w->sums.ensure(na / 256 + 2));
@@ -312,8 +328,14 @@ jh_status do_step(jh_world* w, float dt) {
     if (np > 0) {
         k_narrow<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->verts.p, B, aux,
                                                              w->man.p, w->deep.p, cnt + C_DEEP);
-        k_epa<<<kEpaThreads / 64, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p, w->deep.p,
-                                               cnt + C_DEEP, w->epa_work.p);
+        HIPCHK(hipEventRecord(w->ev[7], st));
+        int et = std::min(kEpaMaxThreads, ((np + 63) / 64) * 64);
+        if (et > w->epa_threads) {
+            HIPCHK(w->epa_work.ensure(size_t(et) * epa::kBytesPerThread));
+            w->epa_threads = et;
+        }
+        k_epa<<<w->epa_threads / 64, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p, w->deep.p,
+                                                  cnt + C_DEEP, w->epa_work.p);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(w->ev[2], st));
@@ -358,7 +380,12 @@ jh_status do_step(jh_world* w, float dt) {
         P.remaining = cnt + C_REM;
         P.ncolours = cnt + C_NCOL;
         P.hist = w->chist.p;
-        int grid = std::min(w->coop_colour, grid_for(std::max(amax_, 128), 256, 1 << 20));
+        P.bar = w->bar.p;
+        // colours persist across steps; recolour everything after body changes or when the palette
+        // has spread (first-fit over many steps can leave more colours than a fresh colouring)
+        P.keep = (w->recolour || w->solve_batches > 48) ? 0 : 1;
+        w->recolour = false;
+        int grid = std::min(w->colour_blocks, grid_for(std::max(amax_, 128), 256, 1 << 20));
         void* args[] = {&P};
         s = launch_coop((const void*)k_colour, grid, 256, args, st);
         if (s != JH_OK) return s;
@@ -367,9 +394,26 @@ jh_status do_step(jh_world* w, float dt) {
         int csh[1] = {32};
         bool in1 = rsort::sort_pairs(w->ok0.p, w->ok1.p, nullptr, nullptr, a_cur, std::max(amax_, 1), csh, 1,
                                      w->hist.p, st);
-        k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p);
+        k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p, w->cbeg.p, cnt + C_NCHUNK);
         const uint64_t* order = in1 ? w->ok1.p : w->ok0.p;
         k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
+        if (w->flow_solver) {
+            // per-body update sequence: rank of each arbiter visit and body degree
+            int ga = grid_for(std::max(amax_, 1));
+            k_set_int<<<1, 1, 0, st>>>(cnt + C_NRANK, 0);
+            k_rank_keys<<<ga, 256, 0, st>>>(w->arb[cur].p, order, w->coffs.p, w->motion.p, w->rk0.p, cnt + C_NRANK);
+            int rsh[8];
+            int nrs = 0;
+            int kb = bits_for(uint32_t(2 * std::max(amax_, 1)));
+            for (int b = 0; b < kb; b += 8) rsh[nrs++] = b;
+            int bb = bits_for(uint32_t(std::max(w->nb - 1, 1)));
+            for (int b = 32; b < 32 + bb; b += 8) rsh[nrs++] = b;
+            bool rin1 = rsort::sort_pairs(w->rk0.p, w->rk1.p, nullptr, nullptr, cnt + C_NRANK, 2 * std::max(amax_, 1),
+                                          rsh, nrs, w->hist.p, st);
+            k_rank_assign<<<grid_for(2 * std::max(amax_, 1)), 256, 0, st>>>(rin1 ? w->rk1.p : w->rk0.p, cnt + C_NRANK,
+                                                                            w->rank.p, w->deg.p, w->bcounter.p);
+            k_chunks<<<ga, 256, 0, st>>>(order, w->coffs.p, w->cbeg.p, w->chunks.p);
+        }
         HIPCHK(hipEventRecord(w->ev[4], st));
         // persistent solver over all substeps
         SolveArgs S;
@@ -377,14 +421,24 @@ jh_status do_step(jh_world* w, float dt) {
         S.order = order;
         S.offs = w->coffs.p;
         S.ncolours = cnt + C_NCOL;
+        S.chunks = w->chunks.p;
+        S.nchunks = cnt + C_NCHUNK;
+        S.rank = w->rank.p;
+        S.deg = w->deg.p;
+        S.counter = w->bcounter.p;
+        S.bar = w->bar.p;
         S.B = B;
         S.nb = w->nb;
         S.ts = ts;
         S.iterations = cfg.solver_iterations;
         S.relaxations = cfg.relaxation_iterations;
-        int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(amax_, w->nb), 256), 256, 1 << 20));
         void* sargs[] = {&S};
-        s = launch_coop((const void*)k_solve, sgrid, 256, sargs, st);
+        if (w->flow_solver) {
+            s = launch_coop((const void*)k_solve_flow, w->flow_blocks, 256, sargs, st);
+        } else {
+            int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(amax_, w->nb), 256), 256, 1 << 20));
+            s = launch_coop((const void*)k_solve, sgrid, 256, sargs, st);
+        }
         if (s != JH_OK) return s;
     }
     HIPCHK(hipEventRecord(w->ev[5], st));
@@ -428,6 +482,9 @@ jh_status do_step(jh_world* w, float dt) {
     S.ms_colouring = ms[3];
     S.ms_solve = ms[4];
     S.ms_update = ms[5];
+    float epa_ms = 0;
+    if (np > 0) HIPCHK(hipEventElapsedTime(&epa_ms, w->ev[7], w->ev[2]));
+    S.ms_epa = np > 0 ? epa_ms : 0.0;
     float tot;
     HIPCHK(hipEventElapsedTime(&tot, w->ev[0], w->ev[6]));
     S.ms_step = tot;
@@ -495,12 +552,24 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     HIPCHK(w->bounds.ensure(8));
     HIPCHK(w->chist.ensure(128));
     HIPCHK(w->coffs.ensure(129));
-    HIPCHK(w->epa_work.ensure(kEpaThreads));
     int occ = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_colour, 256, 0));
     w->coop_colour = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve, 256, 0));
     w->coop_solve = std::max(1, occ) * w->cu_count;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_flow, 256, 0));
+    w->flow_blocks = std::min(std::max(1, occ) * w->cu_count, w->cu_count);
+    HIPCHK(w->bar.ensure(64));
+    HIPCHK(hipMemset(w->bar.p, 0, 64 * sizeof(unsigned)));
+    HIPCHK(w->cbeg.ensure(130));
+    // launch-shape overrides for tuning sweeps (clamped to the co-resident maximum)
+    if (const char* e = getenv("JH_SOLVER")) w->flow_solver = strcmp(e, "barrier") != 0;
+    if (const char* e = getenv("JH_SOLVE_BLOCKS")) {
+        w->coop_solve = std::max(1, std::min(atoi(e), w->coop_solve));
+        w->flow_blocks = std::max(1, std::min(atoi(e), std::max(1, occ) * w->cu_count));
+    }
+    w->colour_blocks = std::min(64, w->coop_colour);
+    if (const char* e = getenv("JH_COLOUR_BLOCKS")) w->colour_blocks = std::max(1, std::min(atoi(e), w->coop_colour));
     *out = w;
     return JH_OK;
 }
@@ -524,6 +593,8 @@ jh_status jh_world_destroy(jh_world* w) {
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
     w->colour.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
+    w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
+    w->bcounter.release(); w->cbeg.release(); w->bar.release();
     for (auto& e : w->ev) (void)hipEventDestroy(e);
     if (w->hcnt) (void)hipHostFree(w->hcnt);
     (void)hipStreamDestroy(w->st);
@@ -567,12 +638,15 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     HIPCHK(w->bmax.ensure(nn));
     HIPCHK(w->mlo.ensure(nn));
     HIPCHK(w->mhi.ensure(nn));
+    HIPCHK(w->deg.ensure(nn));
+    HIPCHK(w->bcounter.ensure(nn));
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(w->body_io.p, bodies, n * sizeof(jh_body), hipMemcpyHostToDevice, w->st));
         k_unpack_bodies<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(w->body_io.p, n, w->bodies());
         HIPCHK(hipGetLastError());
     }
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_NB, n);
+    w->recolour = true;  // motion types may have changed
     HIPCHK(hipStreamSynchronize(w->st));
     return JH_OK;
 }
@@ -740,6 +814,7 @@ jh_status jh_world_set_arbiters(jh_world* w, const jh_arbiter* in, int32_t n) {
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + (w->cur == 0 ? C_A0 : C_A1), n);
     HIPCHK(hipStreamSynchronize(w->st));
     w->a_host = n;
+    w->recolour = true;  // colours in uploaded arbiters are not trusted
     return JH_OK;
 }
 

@@ -81,6 +81,19 @@ def test_heap_mixed_shapes_bitexact():
     assert gw.stats()["arbiters"] > 0
 
 
+def test_heap_bitexact_barrier_solver(monkeypatch):
+    # the colour-batch solver with a grid barrier per batch must agree with the dataflow solver
+    monkeypatch.setenv("JH_SOLVER", "barrier")
+    run_replay(scenes.heap(900, seed=5), steps=60, check_every=20)
+
+
+def test_dataflow_solver_launch_shapes(monkeypatch):
+    # the dataflow solver is exact for any number of resident waves (chunk ownership changes)
+    for blocks in ("1", "7", "64"):
+        monkeypatch.setenv("JH_SOLVE_BLOCKS", blocks)
+        run_replay(scenes.heap(700, seed=11), steps=40, check_every=39)
+
+
 def test_solve_order_is_colouring():
     bodies, shapes, verts, cfg, dt = scenes.heap(1200, seed=3)
     gw = GpuWorld(bodies, shapes, verts, cfg)
