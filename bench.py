@@ -28,6 +28,31 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+class _Roctx:
+    """roctx range around the timed steps, so a kernel-trace profile of this command can be cut to
+    them (rocpd2summary --start-marker/--end-marker timed_steps).  No-op without the library."""
+
+    def __init__(self):
+        import ctypes
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "libroctx64.so.4"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+
+    def push(self, name: str):
+        if self.lib is not None:
+            self.lib.roctxRangePushA(name.encode())
+
+    def pop(self):
+        if self.lib is not None:
+            self.lib.roctxRangePop()
+
+
 def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: int, relax: int) -> float:
     """Algorithmic HBM bytes of one k_solve launch (one step, all substeps).
     Per arbiter per pass (prepare, each iteration, each relaxation): 152 B of body traffic
@@ -80,6 +105,8 @@ def main() -> int:
     start_arbiters = gw.arbiters()
 
     solve_ms, step_ms, bytes_per_launch, contacts = [], [], [], []
+    marker = _Roctx()
+    marker.push("timed_steps")
     t_start = time.perf_counter()
     for _ in range(args.steps):
         gw.step(dt)  # returns after the step finished on the GPU
@@ -89,6 +116,7 @@ def main() -> int:
         contacts.append(s["contacts"])
         bytes_per_launch.append(solver_bytes(s, n_dyn, n_moving, substeps, iters, relax))
     t_end = time.perf_counter()
+    marker.pop()
     wall = t_end - t_start
     value = n_dyn * args.steps / wall
     ms_per_step = 1000.0 * wall / args.steps

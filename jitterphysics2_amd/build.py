@@ -7,6 +7,7 @@ import subprocess
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libjitterhip.so")
+CLI = os.path.join(PKG, "jh_cli")
 
 # -ffp-contract=off: no FMA contraction, so FP32 results round like the reference's scalar code.
 # The packed-FP32 feature is disabled for all device code.
@@ -25,6 +26,13 @@ def build(verbose: bool = False) -> str:
     os.replace(OUT + ".tmp", OUT)
     if verbose:
         print(" ".join(cmd))
+    # native driver (host C++ only, links the library next to it)
+    cli = ["g++", "-O2", "-std=c++17", "-I/opt/rocm/include", "-o", CLI + ".tmp", os.path.join(CSRC, "jh_cli.cpp"),
+           "-L" + PKG, "-ljitterhip", "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,$ORIGIN:/opt/rocm/lib"]
+    r = subprocess.run(cli, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ (jh_cli) failed ({r.returncode}):\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
     return OUT
 
 

@@ -1,0 +1,89 @@
+// jh_cli: native driver of libjitterhip for profiling and scripted runs.
+//   jh_cli <scene.bin> <warmup> <steps>
+// scene.bin (written by jitterphysics2_amd.scenes.save_scene): magic "JHSC", version 1, then
+// int32 counts (bodies, shapes, vertices), float dt, jh_config, jh_body[], jh_shape[], float[3*nv].
+// Prints the mean per-stage timings (HIP events inside the library) and the wall time of the timed
+// steps, which are bracketed by the roctx range "timed_steps" for kernel-trace profiles.
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/jitterhip.h"
+
+static int die(const char* what, jh_status s) {
+    fprintf(stderr, "%s failed (%d): %s\n", what, s, jh_last_error());
+    return 1;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 4) {
+        fprintf(stderr, "usage: %s scene.bin warmup steps\n", argv[0]);
+        return 2;
+    }
+    FILE* f = fopen(argv[1], "rb");
+    if (!f) {
+        perror("open scene");
+        return 2;
+    }
+    char magic[4];
+    int32_t ver, nb, ns, nv;
+    float dt;
+    jh_config cfg;
+    if (fread(magic, 1, 4, f) != 4 || memcmp(magic, "JHSC", 4) != 0 || fread(&ver, 4, 1, f) != 1 || ver != 1 ||
+        fread(&nb, 4, 1, f) != 1 || fread(&ns, 4, 1, f) != 1 || fread(&nv, 4, 1, f) != 1 || fread(&dt, 4, 1, f) != 1 ||
+        fread(&cfg, sizeof(cfg), 1, f) != 1 || nb < 0 || ns < 0 || nv < 0) {
+        fprintf(stderr, "bad scene file\n");
+        return 2;
+    }
+    std::vector<jh_body> bodies(nb);
+    std::vector<jh_shape> shapes(ns);
+    std::vector<float> verts(3 * size_t(nv));
+    if (fread(bodies.data(), sizeof(jh_body), nb, f) != size_t(nb) ||
+        fread(shapes.data(), sizeof(jh_shape), ns, f) != size_t(ns) ||
+        fread(verts.data(), sizeof(float), verts.size(), f) != verts.size()) {
+        fprintf(stderr, "truncated scene file\n");
+        return 2;
+    }
+    fclose(f);
+    int warmup = atoi(argv[2]), steps = atoi(argv[3]);
+    jh_world* w = nullptr;
+    jh_status s = jh_world_create(&cfg, &w);
+    if (s) return die("jh_world_create", s);
+    if ((s = jh_world_set_bodies(w, bodies.data(), nb))) return die("jh_world_set_bodies", s);
+    if ((s = jh_world_set_shapes(w, shapes.data(), ns, nv ? verts.data() : nullptr, nv)))
+        return die("jh_world_set_shapes", s);
+    for (int i = 0; i < warmup; ++i)
+        if ((s = jh_world_step(w, dt))) return die("jh_world_step", s);
+    printf("# warmup %d steps done\n", warmup);
+    fflush(stdout);
+    double sum[8] = {0};
+    jh_stats st;
+    roctxRangePush("timed_steps");
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < steps; ++i) {
+        if ((s = jh_world_step(w, dt))) return die("jh_world_step", s);
+        jh_world_get_stats(w, &st);
+        const double v[8] = {st.ms_broadphase, st.ms_narrowphase, st.ms_epa, st.ms_contact_cache,
+                             st.ms_colouring,  st.ms_solve,       st.ms_update, st.ms_step};
+        for (int k = 0; k < 8; ++k) sum[k] += v[k];
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    roctxRangePop();
+    double wall = std::chrono::duration<double>(t1 - t0).count();
+    for (double& x : sum) x /= steps > 0 ? steps : 1;
+    int ndyn = 0;
+    for (auto& b : bodies) ndyn += b.motion_type == JH_MOTION_DYNAMIC;
+    printf("steps %d wall_s %.6f ms_per_step %.4f body_steps_per_s %.1f pairs %lld arbiters %lld contacts %lld "
+           "colours %lld epa %lld | mean ms: broadphase %.3f narrowphase %.3f (epa %.3f) cache %.3f colouring %.3f "
+           "solve %.3f update %.3f step %.3f\n",
+           steps, wall, 1000.0 * wall / steps, ndyn * steps / wall, (long long)st.pairs, (long long)st.arbiters,
+           (long long)st.contacts, (long long)st.colours, (long long)st.epa_calls, sum[0], sum[1], sum[2], sum[3], sum[4],
+           sum[5], sum[6], sum[7]);
+    fflush(stdout);
+    jh_world_destroy(w);
+    return 0;
+}

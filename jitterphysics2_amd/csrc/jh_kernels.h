@@ -257,7 +257,26 @@ __global__ void k_lbvh_build(const uint64_t* __restrict__ keys, int n, int* __re
     if (i == 0) parent[0] = -1;
 }
 
-// Leaf boxes in sorted order, then bottom-up union of child boxes (second arrival continues).
+// Boxes shared between waves of the refit go through agent-coherent 64-bit accesses (relaxed
+// atomics: they bypass the per-XCD L2), so no device-wide cache write-back or invalidate is needed.
+__device__ __forceinline__ void coh_st4(float4* p, float4 v) {
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    __hip_atomic_store(q, (unsigned long long)__float_as_uint(v.x) | ((unsigned long long)__float_as_uint(v.y) << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, (unsigned long long)__float_as_uint(v.z) | ((unsigned long long)__float_as_uint(v.w) << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float4 coh_ld4(const float4* p) {
+    unsigned long long* q = (unsigned long long*)(p);
+    unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float(uint32_t(a)), __uint_as_float(uint32_t(a >> 32)), __uint_as_float(uint32_t(b)),
+                       __uint_as_float(uint32_t(b >> 32)));
+}
+
+// Leaf boxes in sorted order, then bottom-up union of child boxes (second arrival continues).  A
+// thread publishes its box (coherent stores, waited for with s_waitcnt) before it bumps the node's
+// arrival counter; the second arrival reads both children with coherent loads.
 __global__ void k_lbvh_refit(int n, const uint32_t* __restrict__ order, const float4* __restrict__ amin,
                              const float4* __restrict__ amax, float4* __restrict__ lmin, float4* __restrict__ lmax,
                              const int* __restrict__ left, const int* __restrict__ right, const int* __restrict__ parent,
@@ -266,23 +285,24 @@ __global__ void k_lbvh_refit(int n, const uint32_t* __restrict__ order, const fl
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t s = order[j];
-    lmin[j] = amin[s];
-    lmax[j] = amax[s];
+    coh_st4(&lmin[j], amin[s]);
+    coh_st4(&lmax[j], amax[s]);
     if (n == 1) return;
-    __threadfence();
     int node = leaf_parent[j];
     while (node >= 0) {
-        // the first child to arrive stops; the second sees both child boxes after the acquire fence
-        if (atomicAdd(&flags[node], 1) == 0) return;
-        __threadfence();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        // the first child to arrive stops; the second sees both child boxes
+        if (__hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         int l = left[node], r = right[node];
-        float4 a0 = l < 0 ? lmin[~l] : nmin[l];
-        float4 a1 = l < 0 ? lmax[~l] : nmax[l];
-        float4 b0 = r < 0 ? lmin[~r] : nmin[r];
-        float4 b1 = r < 0 ? lmax[~r] : nmax[r];
-        nmin[node] = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0);
-        nmax[node] = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0);
-        __threadfence();
+        float4 a0 = coh_ld4(l < 0 ? &lmin[~l] : &nmin[l]);
+        float4 a1 = coh_ld4(l < 0 ? &lmax[~l] : &nmax[l]);
+        float4 b0 = coh_ld4(r < 0 ? &lmin[~r] : &nmin[r]);
+        float4 b1 = coh_ld4(r < 0 ? &lmax[~r] : &nmax[r]);
+        coh_st4(&nmin[node], make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0));
+        coh_st4(&nmax[node], make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0));
         node = parent[node];
     }
 }
@@ -306,7 +326,33 @@ constexpr int kTraverseStack = 96;
 
 // One thread per sorted leaf whose body is dynamic: walks the LBVH and emits each pair of the
 // reference's pair set once (DynamicTree.cs:266-286 + World.cs:447-455 + World.Detect.cs:158-160):
-// different bodies of the same world, at least one dynamic, world AABBs not disjoint.
+// different bodies of the same world, at least one dynamic, world AABBs not disjoint.  Pairs are
+// appended to a per-wave LDS buffer (LDS atomics); the wave flushes it to the global list with one
+// global atomic whenever it holds kPairFlush or more pairs (each loop turn adds at most 2 per lane,
+// so 256 slots never overflow) and once more after the walk.  The list is sorted afterwards.
+constexpr int kPairBuf = 256;
+constexpr int kPairFlush = kPairBuf - 128;
+
+__device__ __forceinline__ void flush_pairs(uint64_t* wbuf, int* wcnt, uint64_t* __restrict__ pairs,
+                                            int* __restrict__ pair_count, int cap) {
+    // called by the currently active lanes of one wave, all together
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    int c = __hip_atomic_load(wcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    uint64_t act = __ballot(1);
+    int lane = threadIdx.x & 63;
+    int me = __popcll(act & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    int nact = __popcll(act);
+    int leader = __ffsll((long long)act) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(pair_count, c);
+    base = __shfl(base, leader, 64);
+    for (int k = me; k < c; k += nact)
+        if (base + k < cap) pairs[base + k] = wbuf[k];
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == leader) __hip_atomic_store(wcnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restrict__ keys,
                                                   const uint32_t* __restrict__ order,
                                                   const ShapeRec* __restrict__ shapes, const int* __restrict__ motion,
@@ -315,44 +361,63 @@ __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restr
                                                   const int2* __restrict__ range, const float4* __restrict__ nmin,
                                                   const float4* __restrict__ nmax, uint64_t* __restrict__ pairs,
                                                   int* __restrict__ pair_count, int cap, int* __restrict__ err) {
+    __shared__ uint64_t sbuf[4][kPairBuf];
+    __shared__ int scnt[4];
+    int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) scnt[wv] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    uint64_t* wbuf = sbuf[wv];
+    int* wcnt = &scnt[wv];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || n < 2) return;
-    uint32_t si = order[i];
-    int bi = shapes[si].body;
-    if (motion[bi] != JH_MOTION_DYNAMIC) return;
-    uint64_t w = keys[i] >> 32;
-    int lo = lower_bound_u64(keys, n, w << 32);
-    int hi = lower_bound_u64(keys, n, (w + 1) << 32);
-    float4 qmn = lmin[i], qmx = lmax[i];
-    int stack[kTraverseStack];
-    int sp = 0;
-    int node = 0;
-    while (true) {
-        int ch[2] = {left[node], right[node]};
-        for (int k = 0; k < 2; ++k) {
-            int c = ch[k];
-            if (c < 0) {
-                int j = ~c;
-                if (j < lo || j >= hi || j == i) continue;
-                if (box_disjoint(qmn, qmx, lmin[j], lmax[j])) continue;
-                uint32_t sj = order[j];
-                int bj = shapes[sj].body;
-                if (bj == bi) continue;
-                if (motion[bj] == JH_MOTION_DYNAMIC && j < i) continue;
-                uint32_t a = min(si, sj), b = max(si, sj);
-                int slot = atomicAdd(pair_count, 1);
-                if (slot < cap) pairs[slot] = (uint64_t(a) << 32) | b;
-            } else {
-                int2 r = range[c];
-                if (r.y < lo || r.x >= hi) continue;
-                if (box_disjoint(qmn, qmx, nmin[c], nmax[c])) continue;
-                if (sp < kTraverseStack) stack[sp++] = c;
-                else atomicOr(err, 1);
-            }
-        }
-        if (sp == 0) break;
-        node = stack[--sp];
+    bool active = i < n && n >= 2;
+    uint32_t si = 0;
+    int bi = 0;
+    if (active) {
+        si = order[i];
+        bi = shapes[si].body;
+        active = motion[bi] == JH_MOTION_DYNAMIC;
     }
+    if (active) {
+        uint64_t w = keys[i] >> 32;
+        int lo = lower_bound_u64(keys, n, w << 32);
+        int hi = lower_bound_u64(keys, n, (w + 1) << 32);
+        float4 qmn = lmin[i], qmx = lmax[i];
+        int stack[kTraverseStack];
+        int sp = 0;
+        int node = 0;
+        while (true) {
+            int ch[2] = {left[node], right[node]};
+            for (int k = 0; k < 2; ++k) {
+                int c = ch[k];
+                if (c < 0) {
+                    int j = ~c;
+                    if (j < lo || j >= hi || j == i) continue;
+                    if (box_disjoint(qmn, qmx, lmin[j], lmax[j])) continue;
+                    uint32_t sj = order[j];
+                    int bj = shapes[sj].body;
+                    if (bj == bi) continue;
+                    if (motion[bj] == JH_MOTION_DYNAMIC && j < i) continue;
+                    uint32_t a = min(si, sj), b = max(si, sj);
+                    int slot = __hip_atomic_fetch_add(wcnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    wbuf[slot] = (uint64_t(a) << 32) | b;
+                } else {
+                    int2 r = range[c];
+                    if (r.y < lo || r.x >= hi) continue;
+                    if (box_disjoint(qmn, qmx, nmin[c], nmax[c])) continue;
+                    if (sp < kTraverseStack) stack[sp++] = c;
+                    else atomicOr(err, 1);
+                }
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (__hip_atomic_load(wcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >= kPairFlush)
+                flush_pairs(wbuf, wcnt, pairs, pair_count, cap);
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    // all lanes of the wave are back together here
+    flush_pairs(wbuf, wcnt, pairs, pair_count, cap);
 }
 
 // ---- Narrowphase ----------------------------------------------------------------------------

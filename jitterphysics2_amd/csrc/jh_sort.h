@@ -2,7 +2,9 @@
 // The element count lives in device memory (it is produced by an earlier kernel), so no host
 // synchronisation is needed; grids are sized by the buffer capacity and tiles past the count exit.
 // Per pass: tile histograms (LDS atomics) -> one exclusive scan over digit-major counts -> stable
-// scatter where each wave ranks equal digits with eight ballots.
+// scatter where each wave ranks equal digits with eight ballots.  The histogram holds
+// 256 * ceil(n / kTile) entries for the current count n (kTile = 2048 keys), so the scan costs what
+// the data needs, not what the buffer could hold.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -14,53 +16,73 @@ constexpr int kThreads = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;
 
+__device__ __forceinline__ int tiles_for(int n) { return (n + kTile - 1) / kTile; }
+
+// Tile histograms, digit-major: hist[d * tiles + tile] with tiles = ceil(n / kTile) from the device count.
 __global__ void __launch_bounds__(kThreads) upsweep(const uint64_t* __restrict__ keys, const int* __restrict__ n_ptr,
-                                                    int shift, uint32_t* __restrict__ hist, int num_tiles) {
+                                                    int shift, uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
     int tid = threadIdx.x;
-    h[tid] = 0;
-    __syncthreads();
     int n = *n_ptr;
     int base = blockIdx.x * kTile;
-    if (base < n) {
-        for (int c = 0; c < kItems; ++c) {
-            int idx = base + c * kThreads + tid;
-            if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
-        }
+    if (base >= n) return;
+    h[tid] = 0;
+    __syncthreads();
+    for (int c = 0; c < kItems; ++c) {
+        int idx = base + c * kThreads + tid;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[tid * num_tiles + blockIdx.x] = h[tid];
+    hist[tid * tiles_for(n) + blockIdx.x] = h[tid];
 }
 
-// Exclusive scan of `count` uint32 in place by one workgroup of 1024 threads.
-__global__ void __launch_bounds__(1024) scan_exclusive(uint32_t* __restrict__ data, int count) {
-    __shared__ uint32_t part[1024];
-    int tid = threadIdx.x;
-    int per = (count + 1023) / 1024;
-    int beg = tid * per;
-    int end = min(beg + per, count);
-    uint32_t s = 0;
-    for (int i = beg; i < end; ++i) s += data[i];
-    part[tid] = s;
+// Exclusive scan in place of the 256 * ceil(n / kTile) histogram entries by one workgroup of 1024
+// threads: rounds of 4096 entries, four consecutive entries per thread (one 16-byte load), wave scans
+// with shuffles, then the 16 wave totals.
+__global__ void __launch_bounds__(1024) scan_exclusive(uint32_t* __restrict__ data, const int* __restrict__ n_ptr) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
+    int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int count = tiles_for(*n_ptr) * 256;
+    if (tid == 0) carry_s = 0;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = tid >= off ? part[tid - off] : 0;
+    for (int base = 0; base < count; base += 4096) {
+        int idx = base + tid * 4;
+        uint4 v = idx < count ? *reinterpret_cast<const uint4*>(data + idx) : make_uint4(0, 0, 0, 0);
+        uint32_t t = v.x + v.y + v.z + v.w;
+        uint32_t inc = t;
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t o = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += o;
+        }
+        if (lane == 63) wsum[wave] = inc;
         __syncthreads();
-        part[tid] += v;
+        uint32_t carry = carry_s;
+        uint32_t wpre = 0, total = 0;
+        for (int w = 0; w < 16; ++w) {
+            uint32_t x = wsum[w];
+            if (w < wave) wpre += x;
+            total += x;
+        }
+        uint32_t run = carry + wpre + inc - t;
+        if (idx < count) {
+            uint4 o;
+            o.x = run; run += v.x;
+            o.y = run; run += v.y;
+            o.z = run; run += v.z;
+            o.w = run;
+            *reinterpret_cast<uint4*>(data + idx) = o;
+        }
         __syncthreads();
-    }
-    uint32_t run = part[tid] - s;
-    for (int i = beg; i < end; ++i) {
-        uint32_t v = data[i];
-        data[i] = run;
-        run += v;
+        if (tid == 0) carry_s = carry + total;
+        __syncthreads();
     }
 }
 
 __global__ void __launch_bounds__(kThreads) scatter(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                     const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                     const int* __restrict__ n_ptr, int shift,
-                                                    const uint32_t* __restrict__ hist, int num_tiles) {
+                                                    const uint32_t* __restrict__ hist) {
     __shared__ uint32_t running[256];
     __shared__ uint32_t wcnt[kThreads / 64][256];
     int tid = threadIdx.x;
@@ -68,7 +90,7 @@ __global__ void __launch_bounds__(kThreads) scatter(const uint64_t* __restrict__
     int n = *n_ptr;
     int base = blockIdx.x * kTile;
     if (base >= n) return;
-    running[tid] = hist[tid * num_tiles + blockIdx.x];
+    running[tid] = hist[tid * tiles_for(n) + blockIdx.x];
     uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int c = 0; c < kItems; ++c) {
         for (int w = 0; w < kThreads / 64; ++w) wcnt[w][tid] = 0;
@@ -114,9 +136,9 @@ inline bool sort_pairs(uint64_t* k0, uint64_t* k1, uint32_t* v0, uint32_t* v1, c
         uint64_t* ko = flip ? k0 : k1;
         const uint32_t* vi = flip ? v1 : v0;
         uint32_t* vo = flip ? v0 : v1;
-        upsweep<<<tiles, kThreads, 0, st>>>(ki, n_ptr, shift, hist, tiles);
-        scan_exclusive<<<1, 1024, 0, st>>>(hist, tiles * 256);
-        scatter<<<tiles, kThreads, 0, st>>>(ki, ko, vi, vo, n_ptr, shift, hist, tiles);
+        upsweep<<<tiles, kThreads, 0, st>>>(ki, n_ptr, shift, hist);
+        scan_exclusive<<<1, 1024, 0, st>>>(hist, n_ptr);
+        scatter<<<tiles, kThreads, 0, st>>>(ki, ko, vi, vo, n_ptr, shift, hist);
         flip = !flip;
     }
     return flip;

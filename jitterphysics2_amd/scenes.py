@@ -15,7 +15,7 @@ import math
 
 import numpy as np
 
-from ._abi import (BODY_DTYPE, default_config, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE,
+from ._abi import (BODY_DTYPE, CONFIG_DTYPE, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE,
                    SHAPE_POINTCLOUD, SHAPE_SPHERE, default_config)
 
 f32 = np.float32
@@ -290,3 +290,20 @@ def two_bodies(shape_a, shape_b, pos_a, pos_b, q_a=(0, 0, 0, 1), q_b=(0, 0, 0, 1
             sb.bodies[b]["inverse_mass"] = 0.0
             sb.bodies[b]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
     return sb.arrays()
+
+
+def save_scene(path: str, scene) -> None:
+    """Write (bodies, shapes, verts, config, dt) in the binary form read by the native driver
+    jh_cli: "JHSC", int32 version 1, int32 body/shape/vertex counts, float32 dt, the 64-byte
+    jh_config, then the jh_body, jh_shape and float32[3*nv] arrays."""
+    bodies, shapes, verts, cfg, dt = scene
+    v = np.zeros((0, 3), np.float32) if verts is None else np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    cfg = np.asarray(cfg, dtype=CONFIG_DTYPE).reshape(())
+    with open(path, "wb") as f:
+        f.write(b"JHSC")
+        f.write(np.array([1, len(bodies), len(shapes), len(v)], np.int32).tobytes())
+        f.write(np.float32(dt).tobytes())
+        f.write(cfg.tobytes())
+        f.write(np.ascontiguousarray(bodies, BODY_DTYPE).tobytes())
+        f.write(np.ascontiguousarray(shapes, SHAPE_DTYPE).tobytes())
+        f.write(v.tobytes())
