@@ -69,8 +69,14 @@ typedef struct jh_config {
     int32_t enable_auxiliary_contact_points; /* World.EnableAuxiliaryContactPoints, default 1    */
     int32_t persistent_contact_manifold; /* World.PersistentContactManifold, default 1           */
     int32_t pair_capacity;               /* broadphase pair buffer size; 0 = 24 per shape         */
-    int32_t reserved[6];
+    int32_t warm_start;                  /* JH_WARM_START_*; no counterpart in the reference      */
+    int32_t reserved[5];
 } jh_config;
+
+/* Warm start of the contact impulses at the start of a step.  PREVIOUS is the reference's
+ * (PrepareForIteration replays the previous frame's Accumulated, Contact.cs:802-817); LEARNED first
+ * replaces Accumulated with the prediction of the model given to jh_world_set_warm_start_model. */
+enum { JH_WARM_START_PREVIOUS = 0, JH_WARM_START_LEARNED = 1 };
 
 /* One rigid body.  Position/orientation/velocities and the delta velocities are the solver state
  * of RigidBodyData (RigidBody.cs:107-250); the rest are the managed RigidBody fields that
@@ -217,6 +223,71 @@ jh_status jh_world_get_solve_order(jh_world* world, uint32_t* keys, int32_t key_
                                    int32_t* out_key_count, int32_t* out_batch_count);
 
 jh_status jh_world_get_stats(jh_world* world, jh_stats* stats);
+
+/* ---- Learned warm start and residual probe ---------------------------------------------------
+ * No counterpart in the reference (it runs fixed iteration counts, World.cs:346-368, and warm-starts
+ * from the previous frame).  A per-contact-point MLP 24 -> 128 -> 128 -> 3 (GELU; BF16 weights and
+ * activations, FP32 accumulation on MFMA) predicts the accumulated impulse the solver converges to.
+ *
+ * Features (24 floats per contact point, at the warm start of the first substep, velocities after
+ * IntegrateForces): normal(3), tangent1(3), tangent2(3), relative contact velocity along normal,
+ * tangent1, tangent2 (3), penetration(1), MassNormalTangent(3), previous Accumulated(3), substep
+ * index(1), friction(1), Bias(1), PenaltyBias(1), NewContact(1).
+ * Network output y (3 floats): predicted impulse = previous Accumulated + y * target_scale.
+ * Parameters (JH_LWS_PARAMS floats): W1[128][24], b1[128], W2[128][128], b2[128], W3[3][128], b3[3]. */
+#define JH_LWS_FEATURES 24
+#define JH_LWS_PARAMS 20099
+#define JH_PROBE_MAX_ITERATIONS 256
+
+typedef struct jh_lws jh_lws;
+
+/* Model with seeded initial weights (uniform +-1/sqrt(fan_in), SplitMix64 stream of `seed`),
+ * zero mean / unit scale normalisation. */
+jh_status jh_lws_create(uint64_t seed, jh_lws** out_model);
+jh_status jh_lws_destroy(jh_lws* model);
+jh_status jh_lws_set_params(jh_lws* model, const float* params, int32_t count);
+jh_status jh_lws_get_params(jh_lws* model, float* params, int32_t count);
+/* mean[24], inv_std[24]: feature normalisation (f - mean) * inv_std; target_scale[3]. */
+jh_status jh_lws_set_normalisation(jh_lws* model, const float* mean, const float* inv_std, const float* target_scale);
+/* Training data made resident on the device: features [n][24] and targets [n][3] (the accumulated
+ * impulse after many plain iterations); normalised with the current normalisation. */
+jh_status jh_lws_set_data(jh_lws* model, const float* features, const float* targets, int64_t count);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) on the mean squared error of the normalised outputs over the
+ * resident samples [begin, begin + count), in minibatches of `batch` taken in order; epoch_loss
+ * (may be NULL) receives the mean loss of each epoch. */
+jh_status jh_lws_train(jh_lws* model, int64_t begin, int64_t count, int32_t epochs, int32_t batch, float lr,
+                       double* epoch_loss);
+/* Mean squared error of the normalised outputs over resident samples [begin, begin + count). */
+jh_status jh_lws_evaluate(jh_lws* model, int64_t begin, int64_t count, double* mse);
+/* Gradient of that loss with respect to every parameter (JH_LWS_PARAMS floats), no update. */
+jh_status jh_lws_gradient(jh_lws* model, int64_t begin, int64_t count, float* grad, double* loss);
+/* Network outputs y [n][3] for raw features [n][24]. */
+jh_status jh_lws_predict(jh_lws* model, const float* features, int64_t count, float* outputs);
+
+/* Copies the model (weights and normalisation) into the world for JH_WARM_START_LEARNED. */
+jh_status jh_world_set_warm_start_model(jh_world* world, jh_lws* model);
+
+/* Residual probe: iters-to-residual of the first substep of this step, per warm start, on identical
+ * states.  Per contact point, after PGS iteration k: e_n = bias - v_n if lambda_n > 0 else
+ * max(0, bias - v_n); e_t = |v_t| if |lambda_t| < friction * lambda_n else 0 (both tangents).
+ * residual[m][k] = sum of squares over all points after iteration k (k = 0: after the warm start),
+ * for m = 0 previous-frame warm start, 1 learned, 2 cold (zero impulses); R_k = sqrt(residual[m][k] /
+ * residual[2][0]).  Each probe runs on a scratch copy; the step itself then runs as jh_world_step.
+ * With capacity > 0, the features of every contact point and its accumulated impulse after
+ * `iterations` previous-frame-warm-started iterations are written (training traces). */
+typedef struct jh_probe {
+    int32_t iterations;      /* K, 1 .. JH_PROBE_MAX_ITERATIONS                             */
+    int32_t modes;           /* bit m set: run probe m (m as above)                         */
+    int32_t capacity;        /* rows available in features / targets                        */
+    int32_t reserved;
+    float* features;         /* [capacity][24]                                              */
+    float* targets;          /* [capacity][3]                                               */
+    int64_t points;          /* out: contact points                                          */
+    int64_t rows;            /* out: rows written (min(points, capacity))                   */
+    double residual[3][JH_PROBE_MAX_ITERATIONS + 1];  /* out                                 */
+} jh_probe;
+
+jh_status jh_world_step_probe(jh_world* world, float dt, jh_probe* probe);
 
 #ifdef __cplusplus
 }

@@ -35,7 +35,24 @@ CONFIG_DTYPE = np.dtype([
     ("enable_auxiliary_contact_points", "<i4"),
     ("persistent_contact_manifold", "<i4"),
     ("pair_capacity", "<i4"),
-    ("reserved", "<i4", 6),
+    ("warm_start", "<i4"),
+    ("reserved", "<i4", 5),
+])
+
+WARM_START_PREVIOUS, WARM_START_LEARNED = 0, 1
+LWS_FEATURES, LWS_PARAMS, PROBE_MAX_ITERATIONS = 24, 20099, 256
+PROBE_PREVIOUS, PROBE_LEARNED, PROBE_COLD = 1, 2, 4  # jh_probe.modes bits
+
+PROBE_DTYPE = np.dtype([
+    ("iterations", "<i4"),
+    ("modes", "<i4"),
+    ("capacity", "<i4"),
+    ("reserved", "<i4"),
+    ("features", "<u8"),
+    ("targets", "<u8"),
+    ("points", "<i8"),
+    ("rows", "<i8"),
+    ("residual", "<f8", (3, PROBE_MAX_ITERATIONS + 1)),
 ])
 
 BODY_DTYPE = np.dtype([
@@ -121,7 +138,7 @@ STATS_DTYPE = np.dtype([
 STRUCT_SIZES = {
     0: CONFIG_DTYPE.itemsize, 1: BODY_DTYPE.itemsize, 2: SHAPE_DTYPE.itemsize,
     3: CONTACT_DTYPE.itemsize, 4: ARBITER_DTYPE.itemsize, 5: MANIFOLD_DTYPE.itemsize,
-    6: STATS_DTYPE.itemsize,
+    6: STATS_DTYPE.itemsize, 7: PROBE_DTYPE.itemsize,
 }
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -186,6 +203,18 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_world_set_arbiters": ([vp, vp, i32], i32),
         "jh_world_get_solve_order": ([vp, vp, i32, vp, i32, pi32, pi32], i32),
         "jh_world_get_stats": ([vp, vp], i32),
+        "jh_lws_create": ([ctypes.c_uint64, ctypes.POINTER(vp)], i32),
+        "jh_lws_destroy": ([vp], i32),
+        "jh_lws_set_params": ([vp, vp, i32], i32),
+        "jh_lws_get_params": ([vp, vp, i32], i32),
+        "jh_lws_set_normalisation": ([vp, vp, vp, vp], i32),
+        "jh_lws_set_data": ([vp, vp, vp, ctypes.c_int64], i32),
+        "jh_lws_train": ([vp, ctypes.c_int64, ctypes.c_int64, i32, i32, ctypes.c_float, vp], i32),
+        "jh_lws_evaluate": ([vp, ctypes.c_int64, ctypes.c_int64, vp], i32),
+        "jh_lws_gradient": ([vp, ctypes.c_int64, ctypes.c_int64, vp, vp], i32),
+        "jh_lws_predict": ([vp, vp, ctypes.c_int64, vp], i32),
+        "jh_world_set_warm_start_model": ([vp, vp], i32),
+        "jh_world_step_probe": ([vp, ctypes.c_float, vp], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(lib, name)

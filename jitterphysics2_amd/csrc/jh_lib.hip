@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "jh_kernels.h"
+#include "jh_lws.h"
 #include "jh_scan.h"
 #include "jh_sort.h"
 
@@ -93,6 +94,38 @@ int grid_for(int n, int block = 256, int cap = 4096) {
     return std::max(1, std::min(g, cap));
 }
 
+// Device copy of a warm-start model (BF16 weights in both layouts, FP32 biases, normalisation).
+struct LwsWeights {
+    DBuf<uint16_t> w1, w2, w3, w2t, w3t;
+    DBuf<float> b1, b2, b3, mean, istd, tscale;
+    bool ready = false;
+    hipError_t alloc() {
+        hipError_t e;
+        if ((e = w1.ensure(lws::kHid * lws::kInPad)) != hipSuccess) return e;
+        if ((e = w2.ensure(lws::kHid * lws::kHid)) != hipSuccess) return e;
+        if ((e = w3.ensure(lws::kOutPad * lws::kHid)) != hipSuccess) return e;
+        if ((e = w2t.ensure(lws::kHid * lws::kHid)) != hipSuccess) return e;
+        if ((e = w3t.ensure(lws::kHid * lws::kOutPad)) != hipSuccess) return e;
+        if ((e = b1.ensure(lws::kHid)) != hipSuccess) return e;
+        if ((e = b2.ensure(lws::kHid)) != hipSuccess) return e;
+        if ((e = b3.ensure(lws::kOutPad)) != hipSuccess) return e;
+        if ((e = mean.ensure(lws::kIn)) != hipSuccess) return e;
+        if ((e = istd.ensure(lws::kIn)) != hipSuccess) return e;
+        return tscale.ensure(3);
+    }
+    lws::Model model() const {
+        lws::Model M;
+        M.w1 = w1.p; M.w2 = w2.p; M.w3 = w3.p; M.w2t = w2t.p; M.w3t = w3t.p;
+        M.b1 = b1.p; M.b2 = b2.p; M.b3 = b3.p; M.mean = mean.p; M.istd = istd.p; M.tscale = tscale.p;
+        return M;
+    }
+    void release() {
+        w1.release(); w2.release(); w3.release(); w2t.release(); w3t.release();
+        b1.release(); b2.release(); b3.release(); mean.release(); istd.release(); tscale.release();
+        ready = false;
+    }
+};
+
 }  // namespace
 
 struct jh_world {
@@ -155,6 +188,14 @@ struct jh_world {
     // cooperative occupancy
     int coop_colour = 0, coop_solve = 0, flow_blocks = 0, colour_blocks = 64;
     hipEvent_t ev[8];  // 0 start, 1 broadphase, 2 narrowphase, 3 cache, 4 colouring, 5 solve, 6 end, 7 EPA start
+    // learned warm start and residual probe
+    LwsWeights lws;
+    DBuf<float4> pvel, pang;
+    DBuf<jh_arbiter> parb;
+    DBuf<double> ppart;
+    DBuf<float> pfeat, ptarg;
+    DBuf<uint8_t> pvalid;
+    int coop_probe = 0;
     jh_stats stats;
     bool have_step = false;
 
@@ -191,6 +232,8 @@ jh_status check_config(const jh_config* c) {
     if (c->relaxation_iterations < 0) return fail(JH_ERR_INVALID_ARG, "relaxation_iterations must be >= 0");
     for (int k = 0; k < 3; ++k)
         if (!std::isfinite(c->gravity[k])) return fail(JH_ERR_NON_FINITE, "gravity must be finite");
+    if (c->warm_start != JH_WARM_START_PREVIOUS && c->warm_start != JH_WARM_START_LEARNED)
+        return fail(JH_ERR_INVALID_ARG, "warm_start must be JH_WARM_START_PREVIOUS or JH_WARM_START_LEARNED");
     return JH_OK;
 }
 
@@ -302,7 +345,92 @@ jh_status launch_coop(const void* fn, int grid, int block, void** args, hipStrea
     return JH_OK;
 }
 
-jh_status do_step(jh_world* w, float dt) {
+// Residual probes and traces of the first substep (see jh_world_step_probe).  S describes the solve
+// about to run; every probe works on scratch copies of the velocities and the arbiters.
+jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int amax_, jh_probe* pr) {
+    hipStream_t st = w->st;
+    Bodies B = w->bodies();
+    int K = pr->iterations;
+    HIPCHK(hipMemcpyAsync(w->hcnt + C_COUNT - 1, a_count, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int A = w->hcnt[C_COUNT - 1];
+    int grid = std::min(w->coop_probe, grid_for(std::max(std::max(amax_, w->nb), 256), 256, 1 << 20));
+    HIPCHK(w->pvel.ensure(std::max(w->nb, 1)));
+    HIPCHK(w->pang.ensure(std::max(w->nb, 1)));
+    HIPCHK(w->parb.ensure(std::max(A, 1)));
+    HIPCHK(w->ppart.ensure(size_t(K + 1) * grid));
+    bool trace = pr->capacity > 0;
+    int ga = grid_for(std::max(4 * A, 1));
+    if (trace) {
+        HIPCHK(w->pfeat.ensure(size_t(std::max(4 * A, 1)) * lws::kIn));
+        HIPCHK(w->ptarg.ensure(size_t(std::max(4 * A, 1)) * 3));
+        HIPCHK(w->pvalid.ensure(std::max(4 * A, 1)));
+        lws::k_features<<<ga, 256, 0, st>>>(S.arb, a_count, B, w->pfeat.p, w->pvalid.p);
+    }
+    std::vector<double> part(size_t(K + 1) * grid);
+    for (int m = 0; m < 3; ++m) {
+        for (int k = 0; k <= JH_PROBE_MAX_ITERATIONS; ++k) pr->residual[m][k] = 0.0;
+        if (!(pr->modes & (1 << m))) continue;
+        if (w->nb > 0) {
+            HIPCHK(hipMemcpyAsync(w->pvel.p, w->vel.p, size_t(w->nb) * sizeof(float4), hipMemcpyDeviceToDevice, st));
+            HIPCHK(hipMemcpyAsync(w->pang.p, w->angvel.p, size_t(w->nb) * sizeof(float4), hipMemcpyDeviceToDevice, st));
+        }
+        if (A > 0)
+            HIPCHK(hipMemcpyAsync(w->parb.p, S.arb, size_t(A) * sizeof(jh_arbiter), hipMemcpyDeviceToDevice, st));
+        SolveArgs P = S;
+        P.arb = w->parb.p;
+        P.B.vel = w->pvel.p;
+        P.B.angvel = w->pang.p;
+        if (m == 2) lws::k_zero_acc<<<ga, 256, 0, st>>>(w->parb.p, a_count);
+        if (m == 1) lws::k_infer<<<grid_for(std::max(4 * A, 1), 256, 2048), 256, 0, st>>>(w->parb.p, a_count, P.B,
+                                                                                        w->lws.model());
+        HIPCHK(hipGetLastError());
+        void* args[] = {&P, &K, &w->ppart.p};
+        HIPCHK(hipLaunchCooperativeKernel((const void*)lws::k_probe, dim3(grid), dim3(256), args, 0, st));
+        if (m == 0 && trace) lws::k_gather_acc<<<ga, 256, 0, st>>>(w->parb.p, a_count, w->ptarg.p);
+        HIPCHK(hipMemcpyAsync(part.data(), w->ppart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int k = 0; k <= K; ++k) {
+            double sum = 0.0;
+            for (int b = 0; b < grid; ++b) sum += part[size_t(k) * grid + b];
+            pr->residual[m][k] = sum;
+        }
+    }
+    pr->points = 0;
+    pr->rows = 0;
+    if (A > 0) {
+        std::vector<uint8_t> valid(4 * size_t(A));
+        std::vector<float> feat, targ;
+        if (trace) {
+            feat.resize(4 * size_t(A) * lws::kIn);
+            targ.resize(4 * size_t(A) * 3);
+            HIPCHK(hipMemcpyAsync(valid.data(), w->pvalid.p, valid.size(), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(feat.data(), w->pfeat.p, feat.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(targ.data(), w->ptarg.p, targ.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            int64_t rows = 0;
+            for (size_t i = 0; i < valid.size(); ++i) {
+                if (!valid[i]) continue;
+                if (rows < pr->capacity) {
+                    memcpy(pr->features + rows * lws::kIn, &feat[i * lws::kIn], lws::kIn * sizeof(float));
+                    memcpy(pr->targets + rows * 3, &targ[i * 3], 3 * sizeof(float));
+                }
+                ++rows;
+            }
+            pr->points = rows;
+            pr->rows = std::min<int64_t>(rows, pr->capacity);
+        } else {
+            // count of contact points from the arbiter usage masks
+            std::vector<jh_arbiter> a(A);
+            HIPCHK(hipMemcpyAsync(a.data(), S.arb, size_t(A) * sizeof(jh_arbiter), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            for (auto& x : a) pr->points += __builtin_popcount(x.usage_mask & 15u);
+        }
+    }
+    return JH_OK;
+}
+
+jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     hipStream_t st = w->st;
     Bodies B = w->bodies();
     int* cnt = w->cnt.p;
@@ -432,6 +560,14 @@ jh_status do_step(jh_world* w, float dt) {
         S.ts = ts;
         S.iterations = cfg.solver_iterations;
         S.relaxations = cfg.relaxation_iterations;
+        if (probe) {
+            s = run_probe(w, S, a_cur, amax_, probe);
+            if (s != JH_OK) return s;
+        }
+        if (cfg.warm_start == JH_WARM_START_LEARNED) {
+            lws::k_infer<<<grid_for(std::max(4 * amax_, 1), 256, 2048), 256, 0, st>>>(S.arb, a_cur, B, w->lws.model());
+            HIPCHK(hipGetLastError());
+        }
         void* sargs[] = {&S};
         if (w->flow_solver) {
             s = launch_coop((const void*)k_solve_flow, w->flow_blocks, 256, sargs, st);
@@ -510,6 +646,7 @@ int32_t jh_struct_size(int32_t which) {
         case 4: return sizeof(jh_arbiter);
         case 5: return sizeof(jh_manifold);
         case 6: return sizeof(jh_stats);
+        case 7: return sizeof(jh_probe);
         default: return -1;
     }
 }
@@ -557,6 +694,8 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     w->coop_colour = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve, 256, 0));
     w->coop_solve = std::max(1, occ) * w->cu_count;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)lws::k_probe, 256, 0));
+    w->coop_probe = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_flow, 256, 0));
     w->flow_blocks = std::min(std::max(1, occ) * w->cu_count, w->cu_count);
     HIPCHK(w->bar.ensure(64));
@@ -595,6 +734,8 @@ jh_status jh_world_destroy(jh_world* w) {
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
     w->bcounter.release(); w->cbeg.release(); w->bar.release();
+    w->lws.release(); w->pvel.release(); w->pang.release(); w->parb.release(); w->ppart.release();
+    w->pfeat.release(); w->ptarg.release(); w->pvalid.release();
     for (auto& e : w->ev) (void)hipEventDestroy(e);
     if (w->hcnt) (void)hipHostFree(w->hcnt);
     (void)hipStreamDestroy(w->st);
@@ -733,7 +874,9 @@ jh_status jh_world_step(jh_world* w, float dt) {
     if (dt < 0.0f) return fail(JH_ERR_INVALID_ARG, "Time step cannot be negative.");
     if (dt < 1.401298464e-45f) return JH_OK;
     if (w->nb == 0) return JH_OK;
-    return do_step(w, dt);
+    if (w->cfg.warm_start == JH_WARM_START_LEARNED && !w->lws.ready)
+        return fail(JH_ERR_STATE, "learned warm start selected but no model was set (jh_world_set_warm_start_model)");
+    return do_step(w, dt, nullptr);
 }
 
 jh_status jh_world_compute_pairs(jh_world* w, uint32_t* pairs, int32_t capacity, int32_t* out_count) {
@@ -858,6 +1001,349 @@ jh_status jh_world_get_stats(jh_world* w, jh_stats* s) {
     if (!w || !s) return fail(JH_ERR_INVALID_ARG, "bad arguments");
     *s = w->stats;
     return JH_OK;
+}
+
+}  // extern "C"
+
+struct jh_lws {
+    hipStream_t st = nullptr;
+    LwsWeights W;
+    DBuf<float> params, grad, m, v;
+    float hmean[lws::kIn], histd[lws::kIn], htscale[3];
+    // resident data
+    DBuf<uint16_t> xb;
+    DBuf<float> tn;
+    int64_t n = 0;
+    // workspace
+    DBuf<uint16_t> xt, h1t, h2t, dz1t, dz2t, dyt;
+    DBuf<float> bias_part, gw1, gw2, gw3, tmp_f, tmp_t, out;
+    DBuf<double> loss_part, loss_acc;
+    int64_t adam_t = 0;
+};
+
+namespace {
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+jh_status lws_refresh(jh_lws* L) {
+    lws::CopyArgs C;
+    C.params = L->params.p;
+    C.w1 = L->W.w1.p; C.w2 = L->W.w2.p; C.w3 = L->W.w3.p; C.w2t = L->W.w2t.p; C.w3t = L->W.w3t.p;
+    C.b1 = L->W.b1.p; C.b2 = L->W.b2.p; C.b3 = L->W.b3.p;
+    lws::k_refresh<<<(lws::kHid * lws::kHid + 255) / 256, 256, 0, L->st>>>(C);
+    HIPCHK(hipGetLastError());
+    return JH_OK;
+}
+
+jh_status lws_upload_norm(jh_lws* L) {
+    HIPCHK(hipMemcpyAsync(L->W.mean.p, L->hmean, sizeof(L->hmean), hipMemcpyHostToDevice, L->st));
+    HIPCHK(hipMemcpyAsync(L->W.istd.p, L->histd, sizeof(L->histd), hipMemcpyHostToDevice, L->st));
+    HIPCHK(hipMemcpyAsync(L->W.tscale.p, L->htscale, sizeof(L->htscale), hipMemcpyHostToDevice, L->st));
+    return JH_OK;
+}
+
+// One minibatch of resident samples [begin, begin + count): forward/backward, gradients, and an Adam
+// update when `adam`; the batch's sum of squared errors is added to loss_acc[slot].
+jh_status lws_batch(jh_lws* L, int64_t begin, int count, bool grads, bool adam, float lr, int slot) {
+    hipStream_t st = L->st;
+    int ld = (count + 15) / 16 * 16;
+    int waves = ld / 16;
+    HIPCHK(L->xt.ensure(size_t(lws::kInPad) * ld));
+    HIPCHK(L->h1t.ensure(size_t(lws::kHid) * ld));
+    HIPCHK(L->h2t.ensure(size_t(lws::kHid) * ld));
+    HIPCHK(L->dz1t.ensure(size_t(lws::kHid) * ld));
+    HIPCHK(L->dz2t.ensure(size_t(lws::kHid) * ld));
+    HIPCHK(L->dyt.ensure(size_t(lws::kOutPad) * ld));
+    HIPCHK(L->bias_part.ensure(size_t(waves) * lws::kBiasCols));
+    HIPCHK(L->loss_part.ensure(waves));
+    int ksplit = std::min(ld, 512);
+    int splits = (ld + ksplit - 1) / ksplit;
+    HIPCHK(L->gw1.ensure(size_t(splits) * lws::kHid * lws::kInPad));
+    HIPCHK(L->gw2.ensure(size_t(splits) * lws::kHid * lws::kHid));
+    HIPCHK(L->gw3.ensure(size_t(splits) * lws::kOutPad * lws::kHid));
+    lws::Batch T;
+    T.xb = L->xb.p;
+    T.tn = L->tn.p;
+    T.begin = int(begin);
+    T.count = count;
+    T.ld = ld;
+    T.gscale = 2.0f / (3.0f * float(count));
+    T.xt = L->xt.p; T.h1t = L->h1t.p; T.h2t = L->h2t.p; T.dz1t = L->dz1t.p; T.dz2t = L->dz2t.p; T.dyt = L->dyt.p;
+    T.bias_part = L->bias_part.p;
+    T.loss_part = L->loss_part.p;
+    lws::Model M = L->W.model();
+    lws::k_fwd_bwd<<<(waves + 3) / 4, 256, 0, st>>>(M, T);
+    lws::k_sum_loss<<<1, 64, 0, st>>>(L->loss_part.p, waves, L->loss_acc.p + slot);
+    if (grads) {
+        lws::GemmArgs G;
+        G.job[0] = {L->dz1t.p, L->xt.p, L->gw1.p, lws::kHid, lws::kInPad};
+        G.job[1] = {L->dz2t.p, L->h1t.p, L->gw2.p, lws::kHid, lws::kHid};
+        G.job[2] = {L->dyt.p, L->h2t.p, L->gw3.p, lws::kOutPad, lws::kHid};
+        for (int j = 0; j < 3; ++j) G.tiles[j] = (G.job[j].M / 16) * (G.job[j].N / 16);
+        G.splits = splits;
+        G.ksplit = ksplit;
+        G.ld = ld;
+        int gw = (G.tiles[0] + G.tiles[1] + G.tiles[2]) * splits;
+        lws::k_grad_gemm<<<(gw + 3) / 4, 256, 0, st>>>(G);
+        lws::OptArgs O;
+        O.params = L->params.p; O.grad = L->grad.p; O.m = L->m.p; O.v = L->v.p;
+        O.gw1 = L->gw1.p; O.gw2 = L->gw2.p; O.gw3 = L->gw3.p; O.bias_part = L->bias_part.p;
+        O.splits = splits;
+        O.waves = waves;
+        O.adam = adam ? 1 : 0;
+        O.lr = lr; O.beta1 = 0.9f; O.beta2 = 0.999f; O.eps = 1e-8f;
+        if (adam) L->adam_t++;
+        O.bc1 = float(1.0 - std::pow(0.9, double(std::max<int64_t>(L->adam_t, 1))));
+        O.bc2 = float(1.0 - std::pow(0.999, double(std::max<int64_t>(L->adam_t, 1))));
+        lws::k_optimizer<<<(lws::kParams + 255) / 256, 256, 0, st>>>(O);
+        if (adam) {
+            jh_status s = lws_refresh(L);
+            if (s != JH_OK) return s;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return JH_OK;
+}
+
+jh_status lws_check_range(jh_lws* L, int64_t begin, int64_t count) {
+    if (begin < 0 || count < 1 || begin + count > L->n)
+        return fail(JH_ERR_INVALID_ARG, "sample range [%lld, %lld) outside the %lld resident samples", (long long)begin,
+                    (long long)(begin + count), (long long)L->n);
+    return JH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+jh_status jh_lws_create(uint64_t seed, jh_lws** out) {
+    if (!out) return fail(JH_ERR_INVALID_ARG, "out_model is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(JH_ERR_NO_DEVICE, "no HIP device visible");
+    jh_lws* L = new jh_lws();
+    HIPCHK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+    HIPCHK(L->W.alloc());
+    HIPCHK(L->params.ensure(lws::kParams));
+    HIPCHK(L->grad.ensure(lws::kParams));
+    HIPCHK(L->m.ensure(lws::kParams));
+    HIPCHK(L->v.ensure(lws::kParams));
+    HIPCHK(L->loss_acc.ensure(64));
+    HIPCHK(hipMemset(L->m.p, 0, lws::kParams * sizeof(float)));
+    HIPCHK(hipMemset(L->v.p, 0, lws::kParams * sizeof(float)));
+    // uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) for every weight and bias of a layer
+    std::vector<float> p(lws::kParams);
+    uint64_t st = seed;
+    auto fill = [&](int off, int count, int fan_in) {
+        float b = 1.0f / std::sqrt(float(fan_in));
+        for (int i = 0; i < count; ++i) {
+            double u = double(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+            p[off + i] = float((2.0 * u - 1.0) * b);
+        }
+    };
+    fill(lws::oW1, lws::kHid * lws::kIn + lws::kHid, lws::kIn);
+    fill(lws::oW2, lws::kHid * lws::kHid + lws::kHid, lws::kHid);
+    fill(lws::oW3, lws::kOut * lws::kHid + lws::kOut, lws::kHid);
+    HIPCHK(hipMemcpy(L->params.p, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
+    for (int k = 0; k < lws::kIn; ++k) { L->hmean[k] = 0.0f; L->histd[k] = 1.0f; }
+    for (int k = 0; k < 3; ++k) L->htscale[k] = 1.0f;
+    jh_status s = lws_upload_norm(L);
+    if (s == JH_OK) s = lws_refresh(L);
+    if (s != JH_OK) {
+        delete L;
+        return s;
+    }
+    HIPCHK(hipStreamSynchronize(L->st));
+    L->W.ready = true;
+    *out = L;
+    return JH_OK;
+}
+
+jh_status jh_lws_destroy(jh_lws* L) {
+    if (!L) return fail(JH_ERR_INVALID_ARG, "model is NULL");
+    (void)hipStreamSynchronize(L->st);
+    L->W.release();
+    DBuf<float>* f[] = {&L->params, &L->grad, &L->m, &L->v, &L->tn, &L->bias_part, &L->gw1, &L->gw2, &L->gw3,
+                        &L->tmp_f, &L->tmp_t, &L->out};
+    for (auto* b : f) b->release();
+    DBuf<uint16_t>* h[] = {&L->xb, &L->xt, &L->h1t, &L->h2t, &L->dz1t, &L->dz2t, &L->dyt};
+    for (auto* b : h) b->release();
+    L->loss_part.release();
+    L->loss_acc.release();
+    (void)hipStreamDestroy(L->st);
+    delete L;
+    return JH_OK;
+}
+
+jh_status jh_lws_set_params(jh_lws* L, const float* params, int32_t count) {
+    if (!L || !params || count != lws::kParams)
+        return fail(JH_ERR_INVALID_ARG, "expected %d parameters", lws::kParams);
+    for (int i = 0; i < count; ++i)
+        if (!std::isfinite(params[i])) return fail(JH_ERR_NON_FINITE, "parameter %d is not finite", i);
+    HIPCHK(hipMemcpyAsync(L->params.p, params, count * sizeof(float), hipMemcpyHostToDevice, L->st));
+    HIPCHK(hipMemsetAsync(L->m.p, 0, lws::kParams * sizeof(float), L->st));
+    HIPCHK(hipMemsetAsync(L->v.p, 0, lws::kParams * sizeof(float), L->st));
+    L->adam_t = 0;
+    jh_status s = lws_refresh(L);
+    if (s != JH_OK) return s;
+    HIPCHK(hipStreamSynchronize(L->st));
+    return JH_OK;
+}
+
+jh_status jh_lws_get_params(jh_lws* L, float* params, int32_t count) {
+    if (!L || !params || count != lws::kParams)
+        return fail(JH_ERR_INVALID_ARG, "expected %d parameters", lws::kParams);
+    HIPCHK(hipMemcpyAsync(params, L->params.p, count * sizeof(float), hipMemcpyDeviceToHost, L->st));
+    HIPCHK(hipStreamSynchronize(L->st));
+    return JH_OK;
+}
+
+jh_status jh_lws_set_normalisation(jh_lws* L, const float* mean, const float* inv_std, const float* tscale) {
+    if (!L || !mean || !inv_std || !tscale) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    for (int k = 0; k < lws::kIn; ++k)
+        if (!std::isfinite(mean[k]) || !std::isfinite(inv_std[k])) return fail(JH_ERR_NON_FINITE, "normalisation");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(tscale[k]) || !(tscale[k] > 0.0f))
+            return fail(JH_ERR_INVALID_ARG, "target_scale must be finite and > 0");
+    memcpy(L->hmean, mean, sizeof(L->hmean));
+    memcpy(L->histd, inv_std, sizeof(L->histd));
+    memcpy(L->htscale, tscale, sizeof(L->htscale));
+    jh_status s = lws_upload_norm(L);
+    if (s != JH_OK) return s;
+    HIPCHK(hipStreamSynchronize(L->st));
+    return JH_OK;
+}
+
+jh_status jh_lws_set_data(jh_lws* L, const float* features, const float* targets, int64_t count) {
+    if (!L || !features || !targets || count < 1 || count > (int64_t(1) << 28))
+        return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    HIPCHK(L->tmp_f.ensure(size_t(count) * lws::kIn));
+    HIPCHK(L->tmp_t.ensure(size_t(count) * 3));
+    HIPCHK(L->xb.ensure(size_t(count) * lws::kInPad));
+    HIPCHK(L->tn.ensure(size_t(count) * 3));
+    HIPCHK(hipMemcpyAsync(L->tmp_f.p, features, size_t(count) * lws::kIn * sizeof(float), hipMemcpyHostToDevice, L->st));
+    HIPCHK(hipMemcpyAsync(L->tmp_t.p, targets, size_t(count) * 3 * sizeof(float), hipMemcpyHostToDevice, L->st));
+    lws::k_prepare_data<<<int((count + 255) / 256), 256, 0, L->st>>>(L->tmp_f.p, L->tmp_t.p, int(count), L->W.mean.p,
+                                                                      L->W.istd.p, L->W.tscale.p, L->xb.p, L->tn.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(L->st));
+    L->tmp_f.release();
+    L->tmp_t.release();
+    L->n = count;
+    return JH_OK;
+}
+
+jh_status jh_lws_train(jh_lws* L, int64_t begin, int64_t count, int32_t epochs, int32_t batch, float lr,
+                       double* epoch_loss) {
+    if (!L) return fail(JH_ERR_INVALID_ARG, "model is NULL");
+    jh_status s = lws_check_range(L, begin, count);
+    if (s != JH_OK) return s;
+    if (epochs < 1 || epochs > 64 || batch < 16 || batch > (1 << 16) || !(lr > 0.0f) || !std::isfinite(lr))
+        return fail(JH_ERR_INVALID_ARG, "epochs 1..64, batch 16..65536, lr > 0");
+    HIPCHK(hipMemsetAsync(L->loss_acc.p, 0, 64 * sizeof(double), L->st));
+    for (int e = 0; e < epochs; ++e)
+        for (int64_t b = begin; b < begin + count; b += batch) {
+            int c = int(std::min<int64_t>(batch, begin + count - b));
+            s = lws_batch(L, b, c, true, true, lr, e);
+            if (s != JH_OK) return s;
+        }
+    std::vector<double> acc(64);
+    HIPCHK(hipMemcpyAsync(acc.data(), L->loss_acc.p, 64 * sizeof(double), hipMemcpyDeviceToHost, L->st));
+    HIPCHK(hipStreamSynchronize(L->st));
+    if (epoch_loss)
+        for (int e = 0; e < epochs; ++e) epoch_loss[e] = acc[e] / (3.0 * double(count));
+    return JH_OK;
+}
+
+jh_status jh_lws_evaluate(jh_lws* L, int64_t begin, int64_t count, double* mse) {
+    if (!L || !mse) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    jh_status s = lws_check_range(L, begin, count);
+    if (s != JH_OK) return s;
+    HIPCHK(hipMemsetAsync(L->loss_acc.p, 0, sizeof(double), L->st));
+    for (int64_t b = begin; b < begin + count; b += 65536) {
+        s = lws_batch(L, b, int(std::min<int64_t>(65536, begin + count - b)), false, false, 0.0f, 0);
+        if (s != JH_OK) return s;
+    }
+    double acc = 0;
+    HIPCHK(hipMemcpyAsync(&acc, L->loss_acc.p, sizeof(double), hipMemcpyDeviceToHost, L->st));
+    HIPCHK(hipStreamSynchronize(L->st));
+    *mse = acc / (3.0 * double(count));
+    return JH_OK;
+}
+
+jh_status jh_lws_gradient(jh_lws* L, int64_t begin, int64_t count, float* grad, double* loss) {
+    if (!L || !grad) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    jh_status s = lws_check_range(L, begin, count);
+    if (s != JH_OK) return s;
+    if (count > (1 << 20)) return fail(JH_ERR_INVALID_ARG, "gradient over at most 2^20 samples");
+    HIPCHK(hipMemsetAsync(L->loss_acc.p, 0, sizeof(double), L->st));
+    s = lws_batch(L, begin, int(count), true, false, 0.0f, 0);
+    if (s != JH_OK) return s;
+    double acc = 0;
+    HIPCHK(hipMemcpyAsync(grad, L->grad.p, lws::kParams * sizeof(float), hipMemcpyDeviceToHost, L->st));
+    HIPCHK(hipMemcpyAsync(&acc, L->loss_acc.p, sizeof(double), hipMemcpyDeviceToHost, L->st));
+    HIPCHK(hipStreamSynchronize(L->st));
+    if (loss) *loss = acc / (3.0 * double(count));
+    return JH_OK;
+}
+
+jh_status jh_lws_predict(jh_lws* L, const float* features, int64_t count, float* outputs) {
+    if (!L || !features || !outputs || count < 1 || count > (int64_t(1) << 26))
+        return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    HIPCHK(L->tmp_f.ensure(size_t(count) * lws::kIn));
+    HIPCHK(L->out.ensure(size_t(count) * 3));
+    DBuf<uint16_t> xb;
+    HIPCHK(xb.ensure(size_t(count) * lws::kInPad));
+    HIPCHK(hipMemcpyAsync(L->tmp_f.p, features, size_t(count) * lws::kIn * sizeof(float), hipMemcpyHostToDevice, L->st));
+    lws::k_prepare_data<<<int((count + 255) / 256), 256, 0, L->st>>>(L->tmp_f.p, nullptr, int(count), L->W.mean.p,
+                                                                      L->W.istd.p, L->W.tscale.p, xb.p, nullptr);
+    int waves = int((count + 15) / 16);
+    lws::k_predict<<<(waves + 3) / 4, 256, 0, L->st>>>(L->W.model(), xb.p, int(count), L->out.p);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(outputs, L->out.p, size_t(count) * 3 * sizeof(float), hipMemcpyDeviceToHost, L->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(L->st);
+    xb.release();
+    if (e != hipSuccess) return fail(JH_ERR_HIP, "jh_lws_predict: %s", hipGetErrorString(e));
+    return JH_OK;
+}
+
+jh_status jh_world_set_warm_start_model(jh_world* w, jh_lws* L) {
+    if (!w || !L) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    HIPCHK(hipStreamSynchronize(L->st));
+    HIPCHK(w->lws.alloc());
+    struct { void* d; const void* s; size_t n; } cp[] = {
+        {w->lws.w1.p, L->W.w1.p, size_t(lws::kHid) * lws::kInPad * 2}, {w->lws.w2.p, L->W.w2.p, size_t(lws::kHid) * lws::kHid * 2},
+        {w->lws.w3.p, L->W.w3.p, size_t(lws::kOutPad) * lws::kHid * 2}, {w->lws.w2t.p, L->W.w2t.p, size_t(lws::kHid) * lws::kHid * 2},
+        {w->lws.w3t.p, L->W.w3t.p, size_t(lws::kHid) * lws::kOutPad * 2}, {w->lws.b1.p, L->W.b1.p, lws::kHid * 4},
+        {w->lws.b2.p, L->W.b2.p, lws::kHid * 4}, {w->lws.b3.p, L->W.b3.p, lws::kOutPad * 4},
+        {w->lws.mean.p, L->W.mean.p, lws::kIn * 4}, {w->lws.istd.p, L->W.istd.p, lws::kIn * 4},
+        {w->lws.tscale.p, L->W.tscale.p, 3 * 4}};
+    for (auto& c : cp) HIPCHK(hipMemcpyAsync(c.d, c.s, c.n, hipMemcpyDeviceToDevice, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    w->lws.ready = true;
+    return JH_OK;
+}
+
+jh_status jh_world_step_probe(jh_world* w, float dt, jh_probe* probe) {
+    if (!w || !probe) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (probe->iterations < 1 || probe->iterations > JH_PROBE_MAX_ITERATIONS)
+        return fail(JH_ERR_INVALID_ARG, "iterations must be in 1..%d", JH_PROBE_MAX_ITERATIONS);
+    if (probe->modes < 1 || probe->modes > 7) return fail(JH_ERR_INVALID_ARG, "modes must be a non-empty subset of 0..2");
+    if (probe->capacity < 0 || (probe->capacity > 0 && (!probe->features || !probe->targets)))
+        return fail(JH_ERR_INVALID_ARG, "capacity > 0 needs features and targets");
+    if ((probe->modes & 2) && !w->lws.ready) return fail(JH_ERR_STATE, "learned probe needs a warm-start model");
+    if (!std::isfinite(dt)) return fail(JH_ERR_NON_FINITE, "dt must be finite");
+    if (!(dt >= 1.401298464e-45f)) return fail(JH_ERR_INVALID_ARG, "probe needs dt > 0");
+    if (w->nb == 0) return fail(JH_ERR_STATE, "no bodies");
+    if (w->cfg.warm_start == JH_WARM_START_LEARNED && !w->lws.ready)
+        return fail(JH_ERR_STATE, "learned warm start selected but no model was set");
+    return do_step(w, dt, probe);
 }
 
 }  // extern "C"

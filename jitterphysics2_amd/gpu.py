@@ -9,8 +9,8 @@ import ctypes
 
 import numpy as np
 
-from ._abi import (ARBITER_DTYPE, BODY_DTYPE, CONFIG_DTYPE, MANIFOLD_DTYPE, STATS_DTYPE, check,
-                   default_config, load_library, ptr)
+from ._abi import (ARBITER_DTYPE, BODY_DTYPE, CONFIG_DTYPE, LWS_FEATURES, MANIFOLD_DTYPE, PROBE_DTYPE,
+                   STATS_DTYPE, check, default_config, load_library, ptr)
 
 
 class GpuWorld:
@@ -109,3 +109,26 @@ class GpuWorld:
         s = np.zeros(1, STATS_DTYPE)
         check(self.lib, self.lib.jh_world_get_stats(self.handle, ptr(s)), "jh_world_get_stats")
         return {k: s[k][0].item() for k in STATS_DTYPE.names}
+
+    def set_warm_start_model(self, model) -> None:
+        """Copy a LearnedWarmStart model into the world (used when config.warm_start = 1)."""
+        check(self.lib, self.lib.jh_world_set_warm_start_model(self.handle, model.handle),
+              "jh_world_set_warm_start_model")
+
+    def step_probe(self, dt: float, iterations: int, modes: int, capacity: int = 0) -> dict:
+        """One step with residual probes of its first substep (jh_world_step_probe).
+        Returns residual (3, iterations + 1) sums of squares per mode (previous, learned, cold),
+        points, and with capacity > 0 the trace rows: features (rows, 24), targets (rows, 3)."""
+        pr = np.zeros(1, PROBE_DTYPE)
+        pr["iterations"] = iterations
+        pr["modes"] = modes
+        pr["capacity"] = capacity
+        feats = np.zeros((max(capacity, 1), LWS_FEATURES), np.float32)
+        targs = np.zeros((max(capacity, 1), 3), np.float32)
+        if capacity > 0:
+            pr["features"] = feats.ctypes.data
+            pr["targets"] = targs.ctypes.data
+        check(self.lib, self.lib.jh_world_step_probe(self.handle, ctypes.c_float(dt), ptr(pr)), "jh_world_step_probe")
+        rows = int(pr["rows"][0])
+        return {"residual": pr["residual"][0][:, :iterations + 1].copy(), "points": int(pr["points"][0]),
+                "features": feats[:rows], "targets": targs[:rows]}

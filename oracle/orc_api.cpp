@@ -362,6 +362,31 @@ int orc_step(void* p, float dt, const uint32_t* order, int nkeys) {
     DISPATCH(h, (W.step(decltype(W.gravity.x)(dt), order ? &ord : nullptr), 0));
     return 0;
 }
+// Step with residual probes of the first substep (modes: bit 0 previous-frame warm start, bit 2
+// cold); residual[3][iterations + 1], features [capacity][24], targets [capacity][3];
+// counts[0] = rows written, counts[1] = contact points.
+int orc_step_probe(void* p, float dt, const uint32_t* order, int nkeys, int iterations, int modes, double* residual,
+                   float* features, float* targets, long capacity, long* counts) {
+    Handle* h = (Handle*)p;
+    std::vector<std::pair<uint32_t, uint32_t>> ord(nkeys);
+    for (int i = 0; i < nkeys; ++i) ord[i] = {order[2 * i], order[2 * i + 1]};
+    for (int i = 0; i < 3 * (iterations + 1); ++i) residual[i] = 0.0;
+    auto run = [&](auto& W) {
+        typename std::remove_reference<decltype(W)>::type::Probe pr;
+        pr.iterations = iterations;
+        pr.modes = modes;
+        pr.capacity = capacity;
+        pr.residual = residual;
+        pr.features = features;
+        pr.targets = targets;
+        W.step_impl(decltype(W.gravity.x)(dt), &ord, &pr);
+        counts[0] = pr.rows;
+        counts[1] = pr.points;
+        return 0;
+    };
+    DISPATCH(h, run(W));
+    return 0;
+}
 long orc_epa_calls(void* p) {
     Handle* h = (Handle*)p;
     return DISPATCH(h, W.epa_calls);

@@ -426,7 +426,119 @@ template <class R> struct World {
     }
 
     // World.Step (World.Step.cs:141-296) with a fixed solve order.
-    void step(R dt, const std::vector<std::pair<uint32_t, uint32_t>>* order) {
+    // ---- Learned-warm-start features and residual probe --------------------------------------
+    // These are definitions of this project (the reference runs fixed iteration counts and has no
+    // residual); the arithmetic mirrors the device code operation for operation.
+    struct Probe {
+        int iterations = 0, modes = 0;
+        long capacity = 0;
+        double* residual = nullptr;  // [3][iterations + 1]
+        float* features = nullptr;   // [capacity][24]
+        float* targets = nullptr;    // [capacity][3]
+        long rows = 0, points = 0;
+    };
+
+    // 24 features of slot s at the warm start of the first substep (velocities after IntegrateForces).
+    void contact_features(const Arbiter<R>& a, int s, float* f) const {
+        const Contact<R>& c = a.c[s];
+        const Body<R>& b1 = bodies[a.b1];
+        const Body<R>& b2 = bodies[a.b2];
+        V3<R> v1 = b1.vel, w1 = b1.angvel, v2 = b2.vel, w2 = b2.angvel;
+        if (b1.motion == JH_MOTION_DYNAMIC) { w1 = w1 + b1.dangvel; v1 = v1 + b1.dvel; }
+        if (b2.motion == JH_MOTION_DYNAMIC) { w2 = w2 + b2.dangvel; v2 = v2 + b2.dvel; }
+        V3<R> dv = v2 + cross(w2, c.rel2);
+        dv = dv - (v1 + cross(w1, c.rel1));
+        V3<R> r1 = rotate(c.pos1, b1.q) + b1.pos;
+        V3<R> r2 = rotate(c.pos2, b2.q) + b2.pos;
+        R v[24] = {c.n.x, c.n.y, c.n.z, c.t1.x, c.t1.y, c.t1.z, c.t2.x, c.t2.y, c.t2.z,
+                   dot(dv, c.n), dot(dv, c.t1), dot(dv, c.t2), dot(r1 - r2, c.n),
+                   c.mass[0], c.mass[1], c.mass[2], c.acc[0], c.acc[1], c.acc[2], R(0), a.friction,
+                   c.bias, c.penalty, (c.flag & Contact<R>::NewContact) ? R(1) : R(0)};
+        for (int k = 0; k < 24; ++k) f[k] = float(v[k]);
+    }
+
+    // Sum over contact points of e_n^2 + e_t1^2 + e_t2^2 (see the device definition).
+    double residual(const std::vector<Arbiter<R>>& pa) const {
+        double sum = 0.0;
+        for (const auto& a : pa) {
+            const Body<R>& b1 = bodies[a.b1];
+            const Body<R>& b2 = bodies[a.b2];
+            for (int k = 0; k < 4; ++k) {
+                if (!(a.usage & (1u << k))) continue;
+                const Contact<R>& c = a.c[k];
+                V3<R> dv = b2.vel + cross(b2.angvel, c.rel2);
+                dv = dv - (b1.vel + cross(b1.angvel, c.rel1));
+                R vn = c.n.x * dv.x + c.n.y * dv.y + c.n.z * dv.z;
+                R vt1 = c.t1.x * dv.x + c.t1.y * dv.y + c.t1.z * dv.z;
+                R vt2 = c.t2.x * dv.x + c.t2.y * dv.y + c.t2.z * dv.z;
+                R bias = c.penalty > R(0) ? std::max(c.penalty, c.bias) : c.bias;
+                R ln = c.acc[0];
+                R en = ln > R(0) ? bias - vn : std::max(R(0), bias - vn);
+                R lim = a.friction * ln;
+                R et1 = std::fabs(c.acc[1]) < lim ? std::fabs(vt1) : R(0);
+                R et2 = std::fabs(c.acc[2]) < lim ? std::fabs(vt2) : R(0);
+                sum += double(en) * double(en) + double(et1) * double(et1) + double(et2) * double(et2);
+            }
+        }
+        return sum;
+    }
+
+    // Probes of the first substep on copies (modes: bit 0 previous-frame warm start, bit 2 cold).
+    void run_probe(const std::vector<Arbiter<R>*>& list, const StepTiming<R>& ts, Probe& pr) {
+        pr.rows = pr.points = 0;
+        for (auto& kv : arbiters)
+            for (int s = 0; s < 4; ++s) {
+                if (!(kv.second.usage & (1u << s))) continue;
+                if (pr.features && pr.rows < pr.capacity) contact_features(kv.second, s, pr.features + 24 * pr.rows);
+                if (pr.rows < pr.capacity) ++pr.rows;
+                ++pr.points;
+            }
+        int K = pr.iterations;
+        for (int m : {0, 2}) {
+            if (!(pr.modes & (1 << m))) continue;
+            std::vector<Body<R>> saved = bodies;
+            std::vector<Arbiter<R>> pa;
+            pa.reserve(list.size());
+            for (auto* a : list) pa.push_back(*a);
+            if (m == 2)
+                for (auto& a : pa)
+                    for (auto& c : a.c) c.acc[0] = c.acc[1] = c.acc[2] = R(0);
+            for (auto& b : bodies) {
+                if (b.motion != JH_MOTION_DYNAMIC) continue;
+                b.angvel = b.angvel + b.dangvel;
+                b.vel = b.vel + b.dvel;
+            }
+            for (auto& a : pa)
+                for (int i = 0; i < 4; ++i)
+                    if (a.usage & (1u << i)) prepare_contact(a, a.c[i], ts);
+            pr.residual[m * (K + 1)] = residual(pa);
+            for (int k = 1; k <= K; ++k) {
+                for (auto& a : pa)
+                    for (int i = 0; i < 4; ++i)
+                        if (a.usage & (1u << i)) iterate_contact(a, a.c[i], true);
+                pr.residual[m * (K + 1) + k] = residual(pa);
+            }
+            if (m == 0 && pr.targets) {
+                std::map<std::pair<uint32_t, uint32_t>, const Arbiter<R>*> by_key;
+                for (auto& a : pa) by_key[{a.key1, a.key2}] = &a;
+                long row = 0;
+                for (auto& kv : arbiters)
+                    for (int s = 0; s < 4; ++s) {
+                        if (!(kv.second.usage & (1u << s))) continue;
+                        if (row < pr.capacity) {
+                            const Contact<R>& c = by_key.at(kv.first)->c[s];
+                            for (int r = 0; r < 3; ++r) pr.targets[3 * row + r] = float(c.acc[r]);
+                        }
+                        ++row;
+                    }
+            }
+            bodies = saved;
+        }
+    }
+
+    void step(R dt, const std::vector<std::pair<uint32_t, uint32_t>>* order) { step_impl(dt, order, nullptr); }
+
+    void step_impl(R dt, const std::vector<std::pair<uint32_t, uint32_t>>* order, Probe* probe) {
         if (!(dt >= R(0)) || !std::isfinite(dt)) return;
         if (dt < std::numeric_limits<R>::denorm_min()) return;
         StepTiming<R> ts(dt, substeps);
@@ -445,6 +557,7 @@ template <class R> struct World {
         } else {
             for (auto& kv : arbiters) list.push_back(&kv.second);
         }
+        if (probe) run_probe(list, ts, *probe);
         for (int s = 0; s < ts.substeps; ++s) {
             // IntegrateForces (World.Step.cs:1060-1072)
             for (auto& b : bodies) {

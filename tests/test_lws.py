@@ -1,0 +1,139 @@
+"""GPU tests of the learned warm start: BF16 MFMA forward and backward against the CPU oracle
+(oracle/mlp_ref.py), training, the residual probe against the oracle's replay, and the fallback
+(a network that predicts no change reproduces the previous-frame warm start bit for bit)."""
+import numpy as np
+import pytest
+
+from jitterphysics2_amd import scenes
+from jitterphysics2_amd.gpu import GpuWorld
+from jitterphysics2_amd.lws import (PROBE_COLD, PROBE_LEARNED, PROBE_PREVIOUS, LearnedWarmStart, iterations_to,
+                                    relative_residuals)
+from oracle import mlp_ref
+from tests.orc import OracleWorld
+
+pytestmark = pytest.mark.gpu
+
+
+def _norm(rng):
+    return rng.normal(size=24).astype(np.float32) * 0.1, (0.5 + rng.random(24)).astype(np.float32), \
+        np.array([0.02, 0.01, 0.01], np.float32)
+
+
+def test_init_matches_oracle():
+    m = LearnedWarmStart(seed=0x1234)
+    assert np.array_equal(m.params(), mlp_ref.init_params(0x1234))
+
+
+def test_forward_matches_oracle():
+    # also pins the MFMA lane layout: a wrong A/B/D mapping would scramble the outputs
+    rng = np.random.default_rng(5)
+    m = LearnedWarmStart(seed=11)
+    mean, istd, ts = _norm(rng)
+    m.set_normalisation(mean, istd, ts)
+    f = rng.normal(size=(1000, 24)).astype(np.float32)
+    y = m.predict(f)
+    want = mlp_ref.forward(m.params(), mlp_ref.normalise(f, mean, istd))["Y"]
+    err = np.abs(y - want)
+    assert err.max() < 2e-2 * max(1.0, np.abs(want).max()), err.max()
+    assert err.mean() < 2e-3 * max(1.0, np.abs(want).mean() * 10), err.mean()
+
+
+def test_gradient_matches_oracle():
+    rng = np.random.default_rng(6)
+    m = LearnedWarmStart(seed=12)
+    mean, istd, ts = _norm(rng)
+    m.set_normalisation(mean, istd, ts)
+    n = 2000
+    f = rng.normal(size=(n, 24)).astype(np.float32)
+    t = (f[:, 16:19] + rng.normal(size=(n, 3)) * ts).astype(np.float32)
+    m.set_data(f, t)
+    g, loss = m.gradient(0, n)
+    x = mlp_ref.normalise(f, mean, istd)
+    tn = ((t - f[:, 16:19]) / ts).astype(np.float32)
+    want_loss, want = mlp_ref.loss_and_grad(m.params(), x, tn, quantise=True)
+    assert abs(loss - want_loss) <= 1e-3 * want_loss
+    G, W = mlp_ref.unpack(g), mlp_ref.unpack(want)
+    for name in G:
+        rel = np.linalg.norm(G[name] - W[name]) / max(np.linalg.norm(W[name]), 1e-30)
+        assert rel < 2e-2, (name, rel)
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(7)
+    m = LearnedWarmStart(seed=13)
+    n = 40000
+    f = rng.normal(size=(n, 24)).astype(np.float32)
+    delta = np.stack([np.tanh(f[:, 9] - f[:, 12]), 0.5 * f[:, 10] * f[:, 20], np.sin(f[:, 11])], 1)
+    t = (f[:, 16:19] + 0.01 * delta).astype(np.float32)
+    m.fit_normalisation(f, t)
+    m.set_data(f, t)
+    before = m.evaluate(36000, 4000)
+    loss = m.train(0, 36000, epochs=8, batch=2048, lr=2e-3)
+    after = m.evaluate(36000, 4000)
+    assert np.all(np.isfinite(loss)) and loss[-1] < loss[0]
+    assert after < 0.2 * before, (before, after)
+
+
+def _settled_pyramid(rows=6, steps=30):
+    bodies, shapes, verts, cfg, dt = scenes.pyramid(rows=rows)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    ow = OracleWorld(bodies, shapes, verts, cfg)
+    for _ in range(steps):
+        gw.step(dt)
+        ow.step(dt, gw.solve_order()[0])
+    return gw, ow, dt
+
+
+def test_probe_matches_oracle_replay():
+    gw, ow, dt = _settled_pyramid()
+    r = gw.step_probe(dt, 24, PROBE_PREVIOUS | PROBE_COLD, capacity=4096)
+    keys, _ = gw.solve_order()
+    o = ow.step_probe(dt, keys, 24, PROBE_PREVIOUS | PROBE_COLD, capacity=4096)
+    assert r["points"] == o["points"] > 0
+    assert np.array_equal(r["features"], o["features"])
+    assert np.array_equal(r["targets"], o["targets"])
+    for m in (0, 2):
+        assert np.allclose(r["residual"][m], o["residual"][m], rtol=1e-9, atol=0), m
+    # the probe runs on copies: the step itself is unchanged
+    for f in ("position", "orientation", "velocity", "angular_velocity"):
+        assert np.array_equal(gw.bodies()[f], ow.bodies()[f])
+    R = relative_residuals(r["residual"])
+    assert R[2, 0] == 1.0 and R[0, 24] < R[0, 0]
+
+
+def test_zero_network_equals_previous_warm_start():
+    # W3 = 0, b3 = 0: the prediction is the previous impulse, so learned == previous, bit for bit
+    m = LearnedWarmStart(seed=3)
+    p = m.params()
+    P = mlp_ref.unpack(p)
+    P["W3"][:] = 0
+    P["b3"][:] = 0
+    m.set_params(mlp_ref.pack(P).astype(np.float32))
+    gw, _, dt = _settled_pyramid(steps=20)
+    gw.set_warm_start_model(m)
+    r = gw.step_probe(dt, 16, PROBE_PREVIOUS | PROBE_LEARNED | PROBE_COLD)
+    assert np.array_equal(r["residual"][0], r["residual"][1])
+    # and a world stepping with the learned warm start follows the plain world exactly
+    bodies, shapes, verts, cfg, dt = scenes.pyramid(rows=6)
+    plain = GpuWorld(bodies, shapes, verts, cfg)
+    cfg2 = cfg.copy()
+    cfg2["warm_start"] = 1
+    learned = GpuWorld(bodies, shapes, verts, cfg2)
+    learned.set_warm_start_model(m)
+    for _ in range(15):
+        plain.step(dt)
+        learned.step(dt)
+    for f in ("position", "velocity", "angular_velocity"):
+        assert np.array_equal(plain.bodies()[f], learned.bodies()[f])
+
+
+def test_learned_without_model_is_an_error():
+    bodies, shapes, verts, cfg, dt = scenes.pyramid(rows=3)
+    cfg["warm_start"] = 1
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    with pytest.raises(RuntimeError, match="invalid state"):
+        gw.step(dt)
+    cfg["warm_start"] = 0
+    gw.set_config(cfg)
+    with pytest.raises(RuntimeError, match="invalid state"):
+        gw.step_probe(dt, 4, PROBE_LEARNED)
