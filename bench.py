@@ -1,4 +1,5 @@
-"""Benchmark: body-steps/s of World.Step on one MI355X for a 131,072-body heap.
+"""Benchmark: body-steps/s of World.Step on one MI355X for a 131,072-body heap, and iterations to a
+1e-3 velocity residual with the learned vs the previous-frame (plain) warm start.
 
 Workload: `scenes.heap(131072)` — dynamic spheres/boxes/capsules dropped into a static container,
 dt = 1/60, 4 substeps, 10 solver iterations, 4 relaxation iterations (the 32,768-body heap
@@ -11,6 +12,7 @@ Besides the JSON line's contract fields it reports:
                  per-step counts (see solver_bytes()).
   cpu_baseline — the oracle (CPU restatement, 1 thread) timed on a bounded sample: a few steps of the
                  same heap continued from the GPU state at the start of the timed region.
+  learned_warm_start — residual probes of the settled heap (see learned_warm_start_leg).
 """
 from __future__ import annotations
 
@@ -65,6 +67,49 @@ def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: i
     return substeps * per_substep
 
 
+def learned_warm_start_leg(gw, dt: float, cfg, settle_to: int, probe_iterations: int, timed: int) -> dict:
+    """iters to 1e-3 residual, learned vs previous-frame warm start, on the benchmark heap once it
+    has settled (step `settle_to`): one probe step runs the first substep from identical states with
+    each warm start (and a cold start, the normaliser of R_k).  Then, at equal accuracy (the learned
+    warm start with the fewest iterations whose residual is at most the previous-frame one's at the
+    configured count), `timed` steps of each are timed."""
+    from jitterphysics2_amd.lws import (PROBE_COLD, PROBE_LEARNED, PROBE_PREVIOUS, LearnedWarmStart, iterations_to,
+                                        relative_residuals)
+    from jitterphysics2_amd.lws_train import DEFAULT_MODEL
+
+    if not os.path.exists(DEFAULT_MODEL):
+        return {"error": f"no trained model at {DEFAULT_MODEL} (python -m jitterphysics2_amd.lws_train)"}
+    model = LearnedWarmStart.load(DEFAULT_MODEL)
+    gw.set_warm_start_model(model)
+    while gw.stats()["steps"] < settle_to:
+        gw.step(dt)
+    r = gw.step_probe(dt, probe_iterations, PROBE_PREVIOUS | PROBE_LEARNED | PROBE_COLD)
+    R = relative_residuals(r["residual"])
+    it = int(cfg["solver_iterations"][0])
+    out = {"state": f"benchmark heap after {settle_to} steps (settled)", "contact_points": r["points"],
+           "probe_iterations": probe_iterations, "tolerance": 1e-3}
+    for m, name in ((0, "previous"), (1, "learned"), (2, "cold")):
+        out[name] = {"iters_to_1e-3": iterations_to(R[m], 1e-3),
+                     "R": {str(k): float(R[m, k]) for k in (0, 4, 8, 16, 32, 64, probe_iterations)}}
+    k_learned = iterations_to(R[1], float(R[0, it]))
+    out["equal_accuracy"] = {"previous_iterations": it, "R_previous": float(R[0, it]), "learned_iterations": k_learned}
+    if k_learned is not None and k_learned >= 1 and timed > 0:
+        rates = {}
+        for name, ws, iters in (("previous", 0, it), ("learned", 1, k_learned)):
+            c = cfg.copy()
+            c["warm_start"] = ws
+            c["solver_iterations"] = iters
+            gw.set_config(c)
+            n_dyn = gw.stats()["bodies"]
+            t0 = time.perf_counter()
+            for _ in range(timed):
+                gw.step(dt)
+            rates[name] = {"iterations": iters, "body_steps_per_s": n_dyn * timed / (time.perf_counter() - t0)}
+        gw.set_config(cfg)
+        out["equal_accuracy"]["throughput"] = rates
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +118,9 @@ def main() -> int:
     ap.add_argument("--bodies", type=int, default=131072)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lws", action="store_true", help="skip the learned-warm-start residual leg")
+    ap.add_argument("--lws-settle", type=int, default=700)
+    ap.add_argument("--lws-iterations", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
@@ -123,6 +171,8 @@ def main() -> int:
     avg_solve_s = float(np.mean(solve_ms)) / 1000.0
     achieved = float(np.mean(bytes_per_launch)) / avg_solve_s / 1e9
     final = gw.stats()
+    lws = None if args.no_lws else learned_warm_start_leg(gw, dt, cfg, max(args.lws_settle, args.warmup + args.steps),
+                                                          args.lws_iterations, 20)
     gw.close()
 
     cpu = None
@@ -137,7 +187,7 @@ def main() -> int:
                          f"the start of the timed region (oracle FP32, key-order solve, 1 thread)"}
 
     line = {
-        "metric": "body-steps/sec at 100k bodies",
+        "metric": "body-steps/sec at 100k bodies; iters to 1e-3 residual, learned vs plain warm start",
         "value": value,
         "unit": "body-steps/s",
         "n_gpus": 1,
@@ -162,6 +212,7 @@ def main() -> int:
                      "avg_launch_ms": 1000.0 * avg_solve_s,
                      "algorithmic_bytes_per_launch": float(np.mean(bytes_per_launch))},
         "cpu_baseline": cpu,
+        "learned_warm_start": lws,
     }
     print(json.dumps(line), flush=True)
     return 0

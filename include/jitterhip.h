@@ -233,7 +233,8 @@ jh_status jh_world_get_stats(jh_world* world, jh_stats* stats);
  * IntegrateForces): normal(3), tangent1(3), tangent2(3), relative contact velocity along normal,
  * tangent1, tangent2 (3), penetration(1), MassNormalTangent(3), previous Accumulated(3), substep
  * index(1), friction(1), Bias(1), PenaltyBias(1), NewContact(1).
- * Network output y (3 floats): predicted impulse = previous Accumulated + y * target_scale.
+ * Network output y (3 floats): predicted impulse = previous Accumulated + clamp(y, +-3) * target_scale *
+ * s, with the per-contact scale s = |MassNormal| (|v.n| + |v.t1| + |v.t2| + |PenaltyBias| + 0.05).
  * Parameters (JH_LWS_PARAMS floats): W1[128][24], b1[128], W2[128][128], b2[128], W3[3][128], b3[3]. */
 #define JH_LWS_FEATURES 24
 #define JH_LWS_PARAMS 20099
@@ -247,7 +248,8 @@ jh_status jh_lws_create(uint64_t seed, jh_lws** out_model);
 jh_status jh_lws_destroy(jh_lws* model);
 jh_status jh_lws_set_params(jh_lws* model, const float* params, int32_t count);
 jh_status jh_lws_get_params(jh_lws* model, float* params, int32_t count);
-/* mean[24], inv_std[24]: feature normalisation (f - mean) * inv_std; target_scale[3]. */
+/* mean[24], inv_std[24]: feature normalisation (f - mean) * inv_std; target_scale[3] >= 0 (0 turns
+ * the correction off: the prediction is then the previous-frame impulse). */
 jh_status jh_lws_set_normalisation(jh_lws* model, const float* mean, const float* inv_std, const float* target_scale);
 /* Training data made resident on the device: features [n][24] and targets [n][3] (the accumulated
  * impulse after many plain iterations); normalised with the current normalisation. */
@@ -269,7 +271,8 @@ jh_status jh_world_set_warm_start_model(jh_world* world, jh_lws* model);
 
 /* Residual probe: iters-to-residual of the first substep of this step, per warm start, on identical
  * states.  Per contact point, after PGS iteration k: e_n = bias - v_n if lambda_n > 0 else
- * max(0, bias - v_n); e_t = |v_t| if |lambda_t| < friction * lambda_n else 0 (both tangents).
+ * max(0, bias - v_n); e_t = |v_t| if |lambda_t| < friction * lambda_n(old) else 0 (both tangents),
+ * lambda_n(old) = the normal impulse before iteration k (the one its friction clamp used).
  * residual[m][k] = sum of squares over all points after iteration k (k = 0: after the warm start),
  * for m = 0 previous-frame warm start, 1 learned, 2 cold (zero impulses); R_k = sqrt(residual[m][k] /
  * residual[2][0]).  Each probe runs on a scratch copy; the step itself then runs as jh_world_step.

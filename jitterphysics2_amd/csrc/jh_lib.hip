@@ -193,7 +193,7 @@ struct jh_world {
     DBuf<float4> pvel, pang;
     DBuf<jh_arbiter> parb;
     DBuf<double> ppart;
-    DBuf<float> pfeat, ptarg;
+    DBuf<float> pfeat, ptarg, plam;
     DBuf<uint8_t> pvalid;
     int coop_probe = 0;
     jh_stats stats;
@@ -359,6 +359,7 @@ jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int ama
     HIPCHK(w->pang.ensure(std::max(w->nb, 1)));
     HIPCHK(w->parb.ensure(std::max(A, 1)));
     HIPCHK(w->ppart.ensure(size_t(K + 1) * grid));
+    HIPCHK(w->plam.ensure(4 * size_t(std::max(A, 1))));
     bool trace = pr->capacity > 0;
     int ga = grid_for(std::max(4 * A, 1));
     if (trace) {
@@ -385,7 +386,7 @@ jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int ama
         if (m == 1) lws::k_infer<<<grid_for(std::max(4 * A, 1), 256, 2048), 256, 0, st>>>(w->parb.p, a_count, P.B,
                                                                                         w->lws.model());
         HIPCHK(hipGetLastError());
-        void* args[] = {&P, &K, &w->ppart.p};
+        void* args[] = {&P, &K, &w->ppart.p, &w->plam.p};
         HIPCHK(hipLaunchCooperativeKernel((const void*)lws::k_probe, dim3(grid), dim3(256), args, 0, st));
         if (m == 0 && trace) lws::k_gather_acc<<<ga, 256, 0, st>>>(w->parb.p, a_count, w->ptarg.p);
         HIPCHK(hipMemcpyAsync(part.data(), w->ppart.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -735,7 +736,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
     w->bcounter.release(); w->cbeg.release(); w->bar.release();
     w->lws.release(); w->pvel.release(); w->pang.release(); w->parb.release(); w->ppart.release();
-    w->pfeat.release(); w->ptarg.release(); w->pvalid.release();
+    w->pfeat.release(); w->ptarg.release(); w->pvalid.release(); w->plam.release();
     for (auto& e : w->ev) (void)hipEventDestroy(e);
     if (w->hcnt) (void)hipHostFree(w->hcnt);
     (void)hipStreamDestroy(w->st);
@@ -1208,8 +1209,8 @@ jh_status jh_lws_set_normalisation(jh_lws* L, const float* mean, const float* in
     for (int k = 0; k < lws::kIn; ++k)
         if (!std::isfinite(mean[k]) || !std::isfinite(inv_std[k])) return fail(JH_ERR_NON_FINITE, "normalisation");
     for (int k = 0; k < 3; ++k)
-        if (!std::isfinite(tscale[k]) || !(tscale[k] > 0.0f))
-            return fail(JH_ERR_INVALID_ARG, "target_scale must be finite and > 0");
+        if (!std::isfinite(tscale[k]) || !(tscale[k] >= 0.0f))
+            return fail(JH_ERR_INVALID_ARG, "target_scale must be finite and >= 0");
     memcpy(L->hmean, mean, sizeof(L->hmean));
     memcpy(L->histd, inv_std, sizeof(L->histd));
     memcpy(L->htscale, tscale, sizeof(L->htscale));

@@ -162,6 +162,13 @@ __device__ __forceinline__ void contact_features(const jh_arbiter& a, int s, con
     f[23] = (c.flag & 2u) ? 1.0f : 0.0f;
 }
 
+// Per-contact impulse scale of the network output: effective normal mass x (relative speed +
+// penalty bias + 0.05 m/s), from the raw features.  Zero for contacts without an effective mass yet
+// (new contacts), which therefore keep the previous-frame warm start.
+__host__ __device__ __forceinline__ float impulse_scale(const float* f) {
+    return fabsf(f[13]) * (fabsf(f[9]) + fabsf(f[10]) + fabsf(f[11]) + fabsf(f[22]) + 0.05f);
+}
+
 // Raw features of every slot (row = 4 * arbiter + slot); valid[row] = slot in use (other rows of
 // `out` are left untouched).
 template <class BodiesT>
@@ -204,8 +211,11 @@ __global__ void __launch_bounds__(256) k_zero_acc(jh_arbiter* __restrict__ arb, 
 
 // Warm start from the network: each wave takes 64 slots, every lane builds the normalised BF16
 // features of one slot in LDS, then four 16-sample MLP passes; the prediction replaces Accumulated
-// for used slots that will be warm-started (not SkipWarmStart).  prediction = previous + y * tscale,
-// normal impulse clamped at 0; a non-finite prediction keeps the previous impulse.
+// for used slots that will be warm-started (not SkipWarmStart).  prediction = previous + y * tscale
+// * impulse_scale with y limited to +-kTrust standard deviations of the training corrections (a trust region around
+// the previous-frame impulse for states unlike the training data), normal impulse clamped at 0; a
+// non-finite prediction keeps the previous impulse.
+constexpr float kTrust = 3.0f;
 template <class BodiesT>
 __global__ void __launch_bounds__(256) k_infer(jh_arbiter* __restrict__ arb, const int* __restrict__ a_count, BodiesT B,
                                                Model M) {
@@ -217,12 +227,16 @@ __global__ void __launch_bounds__(256) k_infer(jh_arbiter* __restrict__ arb, con
     for (int base = (blockIdx.x * 4 + wv) * 64; base < n4; base += nwaves * 64) {
         int i = base + lane;
         bool use = false;
+        float sc = 0.0f;
         if (i < n4) {
             const jh_arbiter& a = arb[i >> 2];
             int s = i & 3;
             use = (a.usage_mask & (1u << s)) && !(a.contacts[s].flag & 4u);
             float f[kIn];
-            if (use) contact_features(a, s, B, f);
+            if (use) {
+                contact_features(a, s, B, f);
+                sc = impulse_scale(f);
+            }
             for (int k = 0; k < kInPad; ++k)
                 xs[wv][lane][k] = (use && k < kIn) ? f2bf((f[k] - M.mean[k]) * M.istd[k]) : uint16_t(0);
         } else {
@@ -247,7 +261,8 @@ __global__ void __launch_bounds__(256) k_infer(jh_arbiter* __restrict__ arb, con
             float p[3];
             bool ok = true;
             for (int r = 0; r < 3; ++r) {
-                p[r] = c.accumulated[r] + ys[wv][lane][r] * M.tscale[r];
+                float y = fminf(fmaxf(ys[wv][lane][r], -kTrust), kTrust);
+                p[r] = c.accumulated[r] + y * (M.tscale[r] * sc);
                 ok = ok && (p[r] - p[r] == 0.0f);
             }
             if (ok) {
@@ -483,7 +498,8 @@ __global__ void __launch_bounds__(256) k_refresh(CopyArgs C) {
 }
 
 // Normalised training tensors from raw features [n][24] and raw targets [n][3] (accumulated impulse
-// after the reference iterations): xb = bf16((f - mean) * istd), tn = (target - f[16..18]) / tscale.
+// after the reference iterations): xb = bf16((f - mean) * istd),
+// tn = (target - f[16..18]) / (tscale * impulse_scale(f)) (0 where the scale is 0).
 __global__ void __launch_bounds__(256) k_prepare_data(const float* __restrict__ f, const float* __restrict__ t, int n,
                                                       const float* __restrict__ mean, const float* __restrict__ istd,
                                                       const float* __restrict__ tscale, uint16_t* __restrict__ xb,
@@ -492,9 +508,14 @@ __global__ void __launch_bounds__(256) k_prepare_data(const float* __restrict__ 
     if (i >= n) return;
     for (int k = 0; k < kInPad; ++k)
         xb[size_t(i) * kInPad + k] = k < kIn ? f2bf((f[size_t(i) * kIn + k] - mean[k]) * istd[k]) : uint16_t(0);
-    if (t)
-        for (int r = 0; r < 3; ++r)
-            tn[size_t(i) * 3 + r] = (t[size_t(i) * 3 + r] - f[size_t(i) * kIn + 16 + r]) / tscale[r];
+    if (t) {
+        float sc = impulse_scale(f + size_t(i) * kIn);
+        for (int r = 0; r < 3; ++r) {
+            float d = t[size_t(i) * 3 + r] - f[size_t(i) * kIn + 16 + r];
+            float den = tscale[r] * sc;
+            tn[size_t(i) * 3 + r] = den > 0.0f ? d / den : 0.0f;
+        }
+    }
 }
 
 // Forward only (predictions in the normalised output space) for evaluation.
@@ -516,11 +537,13 @@ __global__ void __launch_bounds__(256) k_predict(Model M, const uint16_t* __rest
 }
 
 // ---- Residual of the velocity problem ---------------------------------------------------------
-// Per contact point, with the bias SolveVelocities uses (max(PenaltyBias, Bias) when PenaltyBias > 0,
-// Contact.cs:955-960): e_n = bias - v_n if lambda_n > 0 else max(0, bias - v_n); e_t = |v_t| if
-// |lambda_t| < friction * lambda_n else 0, for both tangents.  Returns sum of squares.
+// Per contact point, with the bias SolveVelocities uses (max(PenaltyBias, Bias) when PenaltyBias > 0):
+// e_n = bias - v_n if lambda_n > 0 else max(0, bias - v_n); e_t = |v_t| if |lambda_t| < friction *
+// lambda_n(old) else 0, for both tangents, where lambda_n(old) is the normal impulse the friction
+// clamp of the last iteration used (the value before that iteration; the current one when lam_old is
+// NULL).  Returns the sum of squares.
 template <class BodiesT>
-__device__ __forceinline__ double arbiter_residual(const jh_arbiter& a, const BodiesT& B) {
+__device__ __forceinline__ double arbiter_residual(const jh_arbiter& a, const BodiesT& B, const float* lam_old) {
     vec3 v1 = ld3(B.vel[a.body1]), w1 = ld3(B.angvel[a.body1]);
     vec3 v2 = ld3(B.vel[a.body2]), w2 = ld3(B.angvel[a.body2]);
     double s = 0.0;
@@ -537,7 +560,7 @@ __device__ __forceinline__ double arbiter_residual(const jh_arbiter& a, const Bo
         float bias = c.penalty_bias > 0.0f ? fmaxf(c.penalty_bias, c.bias) : c.bias;
         float ln = c.accumulated[0];
         float en = ln > 0.0f ? bias - vn : fmaxf(0.0f, bias - vn);
-        float lim = a.friction * ln;
+        float lim = a.friction * (lam_old ? lam_old[k] : ln);
         float et1 = fabsf(c.accumulated[1]) < lim ? fabsf(vt1) : 0.0f;
         float et2 = fabsf(c.accumulated[2]) < lim ? fabsf(vt2) : 0.0f;
         s += double(en) * en + double(et1) * et1 + double(et2) * et2;
@@ -549,7 +572,8 @@ __device__ __forceinline__ double arbiter_residual(const jh_arbiter& a, const Bo
 // IntegrateForces, PrepareForIteration with whatever Accumulated the copy holds (the warm start
 // under test), then K iterations of SolveVelocities in the colour batches of k_solve, recording the
 // residual sum of squares after the prepare (k = 0) and after each iteration: part[k][block].
-__global__ void __launch_bounds__(256) k_probe(SolveArgs P, int K, double* __restrict__ part) {
+// lam_old[4 * arbiter + slot] keeps the normal impulses from before the current iteration.
+__global__ void __launch_bounds__(256) k_probe(SolveArgs P, int K, double* __restrict__ part, float* __restrict__ lam_old) {
     __shared__ double red[256];
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -558,9 +582,19 @@ __global__ void __launch_bounds__(256) k_probe(SolveArgs P, int K, double* __res
     integrate_forces(P, t0, stride);
     grid_barrier(P.bar);
     for (int k = 0; k <= K; ++k) {
+        if (k > 0) {
+            for (int i = t0; i < na; i += stride) {
+                uint32_t ai = uint32_t(P.order[i]);
+                for (int s = 0; s < 4; ++s) lam_old[4 * size_t(ai) + s] = P.arb[ai].contacts[s].accumulated[0];
+            }
+            grid_barrier(P.bar);
+        }
         solve_batches(P, k == 0 ? 0 : 1, nc);
         double s = 0.0;
-        for (int i = t0; i < na; i += stride) s += arbiter_residual(P.arb[uint32_t(P.order[i])], P.B);
+        for (int i = t0; i < na; i += stride) {
+            uint32_t ai = uint32_t(P.order[i]);
+            s += arbiter_residual(P.arb[ai], P.B, k > 0 ? lam_old + 4 * size_t(ai) : nullptr);
+        }
         red[threadIdx.x] = s;
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {

@@ -24,6 +24,15 @@ FEATURE_NAMES = (
 assert len(FEATURE_NAMES) == LWS_FEATURES
 
 
+def impulse_scale(features: np.ndarray) -> np.ndarray:
+    """Per-contact impulse scale of the network output (float32, as the device computes it):
+    |MassNormal| * (|v.n| + |v.t1| + |v.t2| + |PenaltyBias| + 0.05); predicted impulse =
+    previous + y * target_scale * impulse_scale."""
+    f = np.asarray(features, np.float32)
+    a = np.abs
+    return a(f[:, 13]) * ((((a(f[:, 9]) + a(f[:, 10])) + a(f[:, 11])) + a(f[:, 22])) + np.float32(0.05))
+
+
 class LearnedWarmStart:
     """One jh_lws model: parameters, normalisation and (optionally) resident training data."""
 
@@ -65,12 +74,16 @@ class LearnedWarmStart:
 
     def fit_normalisation(self, features: np.ndarray, targets: np.ndarray) -> None:
         """Feature mean / inverse standard deviation (constant features get scale 1) and the output
-        scale = standard deviation of (target - previous impulse), floored at 1e-6."""
+        scale = standard deviation of (target - previous impulse) / impulse_scale over the contacts
+        with a non-zero impulse scale, floored at 1e-6."""
         f = np.asarray(features, np.float64)
-        d = np.asarray(targets, np.float64) - f[:, 16:19]
+        sc = impulse_scale(features).astype(np.float64)
+        ok = sc > 0
+        d = (np.asarray(targets, np.float64) - f[:, 16:19])[ok] / sc[ok, None]
         sd = f.std(axis=0)
         inv = np.where(sd > 1e-12, 1.0 / np.maximum(sd, 1e-12), 1.0)
-        self.set_normalisation(f.mean(axis=0), inv, np.maximum(d.std(axis=0), 1e-6))
+        ts = np.maximum(d.std(axis=0), 1e-6) if len(d) else np.ones(3)
+        self.set_normalisation(f.mean(axis=0), inv, ts)
 
     # data and training
     def set_data(self, features: np.ndarray, targets: np.ndarray) -> None:
@@ -99,7 +112,8 @@ class LearnedWarmStart:
         return g, loss.value
 
     def predict(self, features: np.ndarray) -> np.ndarray:
-        """Network outputs y (n, 3); predicted impulse = features[:, 16:19] + y * target_scale."""
+        """Network outputs y (n, 3); predicted impulse = features[:, 16:19] + y * target_scale *
+        impulse_scale(features) (before the trust-region limit the world applies)."""
         f = np.ascontiguousarray(features, np.float32)
         out = np.zeros((len(f), 3), np.float32)
         check(self.lib, self.lib.jh_lws_predict(self.handle, ptr(f), len(f), ptr(out)), "jh_lws_predict")
@@ -147,5 +161,5 @@ def collect_traces(gw, dt: float, steps: int, every: int, iterations: int = 64, 
     return np.concatenate(fs), np.concatenate(ts)
 
 
-__all__ = ["FEATURE_NAMES", "LearnedWarmStart", "relative_residuals", "iterations_to", "collect_traces",
+__all__ = ["FEATURE_NAMES", "LearnedWarmStart", "impulse_scale", "relative_residuals", "iterations_to", "collect_traces",
            "PROBE_PREVIOUS", "PROBE_LEARNED", "PROBE_COLD"]

@@ -458,9 +458,11 @@ template <class R> struct World {
     }
 
     // Sum over contact points of e_n^2 + e_t1^2 + e_t2^2 (see the device definition).
-    double residual(const std::vector<Arbiter<R>>& pa) const {
+    // lam_old: normal impulses before the last iteration, per arbiter (4 slots), or empty.
+    double residual(const std::vector<Arbiter<R>>& pa, const std::vector<R>& lam_old) const {
         double sum = 0.0;
-        for (const auto& a : pa) {
+        for (size_t ai = 0; ai < pa.size(); ++ai) {
+            const Arbiter<R>& a = pa[ai];
             const Body<R>& b1 = bodies[a.b1];
             const Body<R>& b2 = bodies[a.b2];
             for (int k = 0; k < 4; ++k) {
@@ -474,7 +476,7 @@ template <class R> struct World {
                 R bias = c.penalty > R(0) ? std::max(c.penalty, c.bias) : c.bias;
                 R ln = c.acc[0];
                 R en = ln > R(0) ? bias - vn : std::max(R(0), bias - vn);
-                R lim = a.friction * ln;
+                R lim = a.friction * (lam_old.empty() ? ln : lam_old[4 * ai + k]);
                 R et1 = std::fabs(c.acc[1]) < lim ? std::fabs(vt1) : R(0);
                 R et2 = std::fabs(c.acc[2]) < lim ? std::fabs(vt2) : R(0);
                 sum += double(en) * double(en) + double(et1) * double(et1) + double(et2) * double(et2);
@@ -511,12 +513,15 @@ template <class R> struct World {
             for (auto& a : pa)
                 for (int i = 0; i < 4; ++i)
                     if (a.usage & (1u << i)) prepare_contact(a, a.c[i], ts);
-            pr.residual[m * (K + 1)] = residual(pa);
+            pr.residual[m * (K + 1)] = residual(pa, {});
+            std::vector<R> lam(4 * pa.size());
             for (int k = 1; k <= K; ++k) {
+                for (size_t ai = 0; ai < pa.size(); ++ai)
+                    for (int i = 0; i < 4; ++i) lam[4 * ai + i] = pa[ai].c[i].acc[0];
                 for (auto& a : pa)
                     for (int i = 0; i < 4; ++i)
                         if (a.usage & (1u << i)) iterate_contact(a, a.c[i], true);
-                pr.residual[m * (K + 1) + k] = residual(pa);
+                pr.residual[m * (K + 1) + k] = residual(pa, lam);
             }
             if (m == 0 && pr.targets) {
                 std::map<std::pair<uint32_t, uint32_t>, const Arbiter<R>*> by_key;

@@ -6,8 +6,8 @@ import pytest
 
 from jitterphysics2_amd import scenes
 from jitterphysics2_amd.gpu import GpuWorld
-from jitterphysics2_amd.lws import (PROBE_COLD, PROBE_LEARNED, PROBE_PREVIOUS, LearnedWarmStart, iterations_to,
-                                    relative_residuals)
+from jitterphysics2_amd.lws import (PROBE_COLD, PROBE_LEARNED, PROBE_PREVIOUS, LearnedWarmStart, impulse_scale,
+                                    iterations_to, relative_residuals)
 from oracle import mlp_ref
 from tests.orc import OracleWorld
 
@@ -49,7 +49,8 @@ def test_gradient_matches_oracle():
     m.set_data(f, t)
     g, loss = m.gradient(0, n)
     x = mlp_ref.normalise(f, mean, istd)
-    tn = ((t - f[:, 16:19]) / ts).astype(np.float32)
+    sc = impulse_scale(f)
+    tn = np.where(sc[:, None] > 0, (t - f[:, 16:19]) / (ts[None, :] * sc[:, None]), 0).astype(np.float32)
     want_loss, want = mlp_ref.loss_and_grad(m.params(), x, tn, quantise=True)
     assert abs(loss - want_loss) <= 1e-3 * want_loss
     G, W = mlp_ref.unpack(g), mlp_ref.unpack(want)
@@ -64,7 +65,7 @@ def test_training_reduces_loss():
     n = 40000
     f = rng.normal(size=(n, 24)).astype(np.float32)
     delta = np.stack([np.tanh(f[:, 9] - f[:, 12]), 0.5 * f[:, 10] * f[:, 20], np.sin(f[:, 11])], 1)
-    t = (f[:, 16:19] + 0.01 * delta).astype(np.float32)
+    t = (f[:, 16:19] + 0.01 * delta * impulse_scale(f)[:, None]).astype(np.float32)
     m.fit_normalisation(f, t)
     m.set_data(f, t)
     before = m.evaluate(36000, 4000)
