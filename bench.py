@@ -116,6 +116,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=240)
     ap.add_argument("--bodies", type=int, default=131072)
+    ap.add_argument("--workload", choices=("heap", "batched"), default="heap",
+                    help="heap: one 131,072-body heap (the metric's configuration); batched: 4,096 independent "
+                         "worlds of 16..2,048 bodies (log-uniform) in one world batch")
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lws", action="store_true", help="skip the learned-warm-start residual leg")
@@ -131,7 +136,11 @@ def main() -> int:
     from jitterphysics2_amd.gpu import GpuWorld
 
     t0 = time.time()
-    bodies, shapes, verts, cfg, dt = scenes.heap(args.bodies)
+    if args.workload == "batched":
+        bodies, shapes, verts, cfg, dt = scenes.batched_worlds(args.worlds)
+        args.no_lws = True  # the residual leg is defined on the heap
+    else:
+        bodies, shapes, verts, cfg, dt = scenes.heap(args.bodies)
     print(f"# scene built: {len(bodies)} bodies, {len(shapes)} shapes in {time.time() - t0:.1f}s", flush=True)
     n_dyn = int(np.sum(bodies["motion_type"] == 0))
     n_moving = int(np.sum(bodies["motion_type"] != 2))
@@ -148,6 +157,12 @@ def main() -> int:
             print(f"# warmup {i + 1}/{args.warmup}: pairs {s['pairs']} arbiters {s['arbiters']} "
                   f"colours {s['colours']} ms/step {s['ms_step']:.2f}", flush=True)
     print(f"# warmup done in {time.time() - t0:.1f}s", flush=True)
+    tuning = None
+    if not args.no_autotune:
+        tuning = gw.autotune(dt, rounds=2)
+        print(f"# autotune: solve blocks {tuning['solve_blocks']} ({tuning['ms_solve_best']:.2f} ms vs "
+              f"{tuning['ms_solve_default']:.2f} default), colour blocks {tuning['colour_blocks']} "
+              f"({tuning['ms_colour_best']:.2f} ms vs {tuning['ms_colour_default']:.2f})", flush=True)
 
     start_bodies = gw.bodies()
     start_arbiters = gw.arbiters()
@@ -199,8 +214,12 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"heap of {n_dyn} dynamic spheres/boxes/capsules in a static container, dt=1/60, "
-                               f"{substeps} substeps, {iters} iterations, {relax} relaxations (plain warm start)",
+        "config": {"workload": (f"heap of {n_dyn} dynamic spheres/boxes/capsules in a static container, dt=1/60, "
+                                f"{substeps} substeps, {iters} iterations, {relax} relaxations (plain warm start)")
+                   if args.workload == "heap" else
+                   (f"{args.worlds} independent worlds, {n_dyn} dynamic boxes/spheres in total (16..2048 per "
+                    f"world, log-uniform), dt=1/60, {substeps} substeps, {iters} iterations, {relax} relaxations"),
+                   "autotune": tuning,
                    "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],
                    "contacts": final["contacts"], "colours": final["colours"],
                    "colliding_pairs": final["colliding_pairs"], "epa_pairs": final["epa_calls"],

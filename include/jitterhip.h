@@ -177,7 +177,7 @@ const char* jh_last_error(void);
 /* ABI version: 1. */
 int32_t jh_version(void);
 /* sizeof of the ABI structs, for binding checks: 0 jh_config, 1 jh_body, 2 jh_shape, 3 jh_contact,
- * 4 jh_arbiter, 5 jh_manifold, 6 jh_stats; -1 otherwise. */
+ * 4 jh_arbiter, 5 jh_manifold, 6 jh_stats, 7 jh_probe, 8 jh_tuning; -1 otherwise. */
 int32_t jh_struct_size(int32_t which);
 
 /* World lifetime (World ctor World.cs:423-441, Dispose World.cs:977-990).  config may be NULL for
@@ -291,6 +291,23 @@ typedef struct jh_probe {
 } jh_probe;
 
 jh_status jh_world_step_probe(jh_world* world, float dt, jh_probe* probe);
+
+/* ---- Launch-shape autotuner (no counterpart in the reference) --------------------------------
+ * Times candidate workgroup counts of the persistent solver (1/16 .. 2x the CU count, clamped to what
+ * can be co-resident) and of the colouring kernel (16 .. 512) on real steps, round-robin over the
+ * candidates for `rounds` rounds each, and keeps the fastest.  The world is stepped
+ * rounds * candidates times (the simulation advances); results are deterministic whatever the
+ * launch shape. */
+typedef struct jh_tuning {
+    int32_t solve_blocks;      /* chosen persistent-solver workgroups  */
+    int32_t colour_blocks;     /* chosen colouring workgroups          */
+    int32_t candidates;
+    int32_t steps;             /* steps taken                          */
+    double ms_solve_best, ms_solve_default;    /* mean solver time: chosen / default shape      */
+    double ms_colour_best, ms_colour_default;  /* mean colouring time: chosen / default shape   */
+} jh_tuning;
+
+jh_status jh_world_autotune(jh_world* world, float dt, int32_t rounds, jh_tuning* out);
 
 #ifdef __cplusplus
 }

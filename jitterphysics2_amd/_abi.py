@@ -135,10 +135,15 @@ STATS_DTYPE = np.dtype([
     ("ms_colouring", "<f8"), ("ms_solve", "<f8"), ("ms_update", "<f8"), ("ms_step", "<f8"), ("ms_epa", "<f8"),
 ])
 
+TUNING_DTYPE = np.dtype([
+    ("solve_blocks", "<i4"), ("colour_blocks", "<i4"), ("candidates", "<i4"), ("steps", "<i4"),
+    ("ms_solve_best", "<f8"), ("ms_solve_default", "<f8"), ("ms_colour_best", "<f8"), ("ms_colour_default", "<f8"),
+])
+
 STRUCT_SIZES = {
     0: CONFIG_DTYPE.itemsize, 1: BODY_DTYPE.itemsize, 2: SHAPE_DTYPE.itemsize,
     3: CONTACT_DTYPE.itemsize, 4: ARBITER_DTYPE.itemsize, 5: MANIFOLD_DTYPE.itemsize,
-    6: STATS_DTYPE.itemsize, 7: PROBE_DTYPE.itemsize,
+    6: STATS_DTYPE.itemsize, 7: PROBE_DTYPE.itemsize, 8: TUNING_DTYPE.itemsize,
 }
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -215,6 +220,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_lws_predict": ([vp, vp, ctypes.c_int64, vp], i32),
         "jh_world_set_warm_start_model": ([vp, vp], i32),
         "jh_world_step_probe": ([vp, ctypes.c_float, vp], i32),
+        "jh_world_autotune": ([vp, ctypes.c_float, i32, vp], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(lib, name)

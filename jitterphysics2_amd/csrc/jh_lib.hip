@@ -186,7 +186,7 @@ struct jh_world {
     int solve_count = 0, solve_batches = 0;
     std::vector<int> solve_offsets;
     // cooperative occupancy
-    int coop_colour = 0, coop_solve = 0, flow_blocks = 0, colour_blocks = 64;
+    int coop_colour = 0, coop_solve = 0, flow_blocks = 0, colour_blocks = 64, flow_max = 0;
     hipEvent_t ev[8];  // 0 start, 1 broadphase, 2 narrowphase, 3 cache, 4 colouring, 5 solve, 6 end, 7 EPA start
     // learned warm start and residual probe
     LwsWeights lws;
@@ -648,6 +648,7 @@ int32_t jh_struct_size(int32_t which) {
         case 5: return sizeof(jh_manifold);
         case 6: return sizeof(jh_stats);
         case 7: return sizeof(jh_probe);
+        case 8: return sizeof(jh_tuning);
         default: return -1;
     }
 }
@@ -698,7 +699,8 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)lws::k_probe, 256, 0));
     w->coop_probe = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_flow, 256, 0));
-    w->flow_blocks = std::min(std::max(1, occ) * w->cu_count, w->cu_count);
+    w->flow_max = std::max(1, occ) * w->cu_count;
+    w->flow_blocks = std::min(w->flow_max, w->cu_count);
     HIPCHK(w->bar.ensure(64));
     HIPCHK(hipMemset(w->bar.p, 0, 64 * sizeof(unsigned)));
     HIPCHK(w->cbeg.ensure(130));
@@ -1345,6 +1347,58 @@ jh_status jh_world_step_probe(jh_world* w, float dt, jh_probe* probe) {
     if (w->cfg.warm_start == JH_WARM_START_LEARNED && !w->lws.ready)
         return fail(JH_ERR_STATE, "learned warm start selected but no model was set");
     return do_step(w, dt, probe);
+}
+
+jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* out) {
+    if (!w || !out || rounds < 1 || rounds > 64) return fail(JH_ERR_INVALID_ARG, "bad arguments (rounds 1..64)");
+    if (!std::isfinite(dt) || !(dt >= 1.401298464e-45f)) return fail(JH_ERR_INVALID_ARG, "autotune needs dt > 0");
+    if (w->nb == 0) return fail(JH_ERR_STATE, "no bodies");
+    if (w->cfg.warm_start == JH_WARM_START_LEARNED && !w->lws.ready)
+        return fail(JH_ERR_STATE, "learned warm start selected but no model was set");
+    memset(out, 0, sizeof(*out));
+    int cu = w->cu_count;
+    // candidates, de-duplicated after clamping to what can be co-resident
+    auto uniq = [](std::vector<int> v, int hi) {
+        for (int& x : v) x = std::max(1, std::min(x, hi));
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        return v;
+    };
+    std::vector<int> sc = w->flow_solver ? uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->flow_max)
+                                         : uniq({cu / 2, cu, 2 * cu, 4 * cu, 8 * cu}, w->coop_solve);
+    std::vector<int> cc = uniq({16, 32, 64, 128, 256, 512}, w->coop_colour);
+    int& solve_knob = w->flow_solver ? w->flow_blocks : w->coop_solve;
+    int solve_default = solve_knob, colour_default = w->colour_blocks;
+    // round-robin over the candidates so that the evolving scene weighs them equally
+    auto tune = [&](std::vector<int>& cand, int& knob, bool solve, double& best_ms, double& default_ms,
+                    int def) -> jh_status {
+        std::vector<double> tot(cand.size(), 0.0);
+        for (int r = 0; r < rounds; ++r)
+            for (size_t i = 0; i < cand.size(); ++i) {
+                knob = cand[i];
+                jh_status s = do_step(w, dt, nullptr);
+                if (s != JH_OK) return s;
+                tot[i] += solve ? w->stats.ms_solve : w->stats.ms_colouring;
+            }
+        size_t bi = 0;
+        for (size_t i = 1; i < cand.size(); ++i)
+            if (tot[i] < tot[bi]) bi = i;
+        knob = cand[bi];
+        best_ms = tot[bi] / rounds;
+        default_ms = best_ms;
+        for (size_t i = 0; i < cand.size(); ++i)
+            if (cand[i] == def) default_ms = tot[i] / rounds;
+        return JH_OK;
+    };
+    jh_status s = tune(sc, solve_knob, true, out->ms_solve_best, out->ms_solve_default, solve_default);
+    if (s != JH_OK) return s;
+    s = tune(cc, w->colour_blocks, false, out->ms_colour_best, out->ms_colour_default, colour_default);
+    if (s != JH_OK) return s;
+    out->solve_blocks = solve_knob;
+    out->colour_blocks = w->colour_blocks;
+    out->candidates = int(sc.size() + cc.size());
+    out->steps = rounds * out->candidates;
+    return JH_OK;
 }
 
 }  // extern "C"

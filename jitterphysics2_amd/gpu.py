@@ -10,7 +10,7 @@ import ctypes
 import numpy as np
 
 from ._abi import (ARBITER_DTYPE, BODY_DTYPE, CONFIG_DTYPE, LWS_FEATURES, MANIFOLD_DTYPE, PROBE_DTYPE,
-                   STATS_DTYPE, check, default_config, load_library, ptr)
+                   STATS_DTYPE, TUNING_DTYPE, check, default_config, load_library, ptr)
 
 
 class GpuWorld:
@@ -132,3 +132,10 @@ class GpuWorld:
         rows = int(pr["rows"][0])
         return {"residual": pr["residual"][0][:, :iterations + 1].copy(), "points": int(pr["points"][0]),
                 "features": feats[:rows], "targets": targs[:rows]}
+
+    def autotune(self, dt: float, rounds: int = 2) -> dict:
+        """Pick the solver and colouring workgroup counts by timing candidates on real steps
+        (jh_world_autotune; advances the simulation)."""
+        t = np.zeros(1, TUNING_DTYPE)
+        check(self.lib, self.lib.jh_world_autotune(self.handle, ctypes.c_float(dt), rounds, ptr(t)), "jh_world_autotune")
+        return {k: t[k][0].item() for k in TUNING_DTYPE.names}

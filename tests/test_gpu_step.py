@@ -123,3 +123,30 @@ def test_step_argument_errors():
     b0 = gw.bodies()
     gw.step(0.0)  # dt < epsilon: nothing happens (World.Step.cs:163-164)
     assert np.array_equal(gw.bodies()["position"], b0["position"])
+
+
+def test_batched_worlds_bitexact():
+    # independent worlds sharing coordinates: pairs never cross world ids; every world is solved in
+    # the same persistent launch (the oracle filters pairs by world id by brute force)
+    gw, ow = run_replay(scenes.batched_worlds(n_worlds=24, min_bodies=16, max_bodies=160, seed=9), steps=40,
+                        check_every=13, brute_pairs=True)
+    st = gw.stats()
+    assert st["arbiters"] > 0
+
+
+def test_autotune_keeps_results_exact():
+    bodies, shapes, verts, cfg, dt = scenes.heap(1200, seed=21)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    for _ in range(60):
+        gw.step(dt)
+    t = gw.autotune(dt, rounds=1)
+    assert t["solve_blocks"] >= 1 and t["colour_blocks"] >= 1 and t["steps"] == t["candidates"]
+    assert t["ms_solve_best"] <= t["ms_solve_default"] + 1e-9
+    # continue from the tuned world's state: the oracle replay stays bit-exact with the new shapes
+    ow = OracleWorld(gw.bodies(), shapes, verts, cfg)
+    ow.set_arbiters(gw.arbiters())
+    for s in range(20):
+        gw.step(dt)
+        ow.step(dt, gw.solve_order()[0])
+    assert_bodies_equal(gw.bodies(), ow.bodies(), "after autotune")
+    assert_arbiters_equal(gw.arbiters(), ow.arbiters(), "after autotune")
