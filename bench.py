@@ -116,9 +116,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=240)
     ap.add_argument("--bodies", type=int, default=131072)
-    ap.add_argument("--workload", choices=("heap", "batched"), default="heap",
+    ap.add_argument("--workload", choices=("heap", "batched", "full"), default="heap",
                     help="heap: one 131,072-body heap (the metric's configuration); batched: 4,096 independent "
-                         "worlds of 16..2,048 bodies (log-uniform) in one world batch")
+                         "worlds of 16..2,048 bodies (log-uniform) in one world batch; full: 262,144 bodies incl. "
+                         "32-point hulls over a 512x512-cell triangle heightfield split into 64 sub-worlds, "
+                         "TriangleEdgeCollisionFilter on")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -139,6 +141,9 @@ def main() -> int:
     if args.workload == "batched":
         bodies, shapes, verts, cfg, dt = scenes.batched_worlds(args.worlds)
         args.no_lws = True  # the residual leg is defined on the heap
+    elif args.workload == "full":
+        bodies, shapes, verts, cfg, dt = scenes.full_form()
+        args.no_lws = True
     else:
         bodies, shapes, verts, cfg, dt = scenes.heap(args.bodies)
     print(f"# scene built: {len(bodies)} bodies, {len(shapes)} shapes in {time.time() - t0:.1f}s", flush=True)
@@ -218,7 +223,11 @@ def main() -> int:
                                 f"{substeps} substeps, {iters} iterations, {relax} relaxations (plain warm start)")
                    if args.workload == "heap" else
                    (f"{args.worlds} independent worlds, {n_dyn} dynamic boxes/spheres in total (16..2048 per "
-                    f"world, log-uniform), dt=1/60, {substeps} substeps, {iters} iterations, {relax} relaxations"),
+                    f"world, log-uniform), dt=1/60, {substeps} substeps, {iters} iterations, {relax} relaxations")
+                   if args.workload == "batched" else
+                   (f"{n_dyn} dynamic spheres/boxes/capsules/32-point hulls over a 512x512-cell Perlin heightfield "
+                    f"(524,288 triangles) in 64 sub-worlds, triangle edge filter, dt=1/60, {substeps} substeps, "
+                    f"{iters} iterations, {relax} relaxations"),
                    "autotune": tuning,
                    "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],
                    "contacts": final["contacts"], "colours": final["colours"],

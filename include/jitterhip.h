@@ -53,7 +53,9 @@ enum {
     JH_SHAPE_POINTCLOUD = 3, /* PointCloudShape.cs / VertexSupportMap.cs: params[0..2] = center (GetCenter),
                                 params[3..5] = shift, params[6..8] = cached local box min,
                                 params[9..11] = cached local box max; vertices[vertex_offset ..]           */
-    JH_SHAPE_TRIANGLE = 4    /* TriangleShape.cs (static meshes): vertices[vertex_offset .. +3]          */
+    JH_SHAPE_TRIANGLE = 4    /* TriangleShape.cs (static meshes): vertices[vertex_offset .. +3] = A, B, C;
+                              * params: triangle normal(3), normals of NeighborA, B, C (3 each, zero =
+                              * none), body frame (TriangleMesh.cs:56-63, 300-319)                     */
 };
 
 /* ContactData.SolveMode bits, Dynamics/Contact.cs:36-58. */
@@ -70,8 +72,14 @@ typedef struct jh_config {
     int32_t persistent_contact_manifold; /* World.PersistentContactManifold, default 1           */
     int32_t pair_capacity;               /* broadphase pair buffer size; 0 = 24 per shape         */
     int32_t warm_start;                  /* JH_WARM_START_*; no counterpart in the reference      */
-    int32_t reserved[5];
+    int32_t narrowphase_filter;          /* World.NarrowPhaseFilter: JH_FILTER_*, default none     */
+    int32_t reserved[4];
 } jh_config;
+
+/* World.NarrowPhaseFilter (World.Detect.cs:197-203).  TRIANGLE_EDGE = TriangleEdgeCollisionFilter
+ * (Collision/CollisionFilter/TriangleEdgeCollisionFilter.cs) with its defaults: EdgeThreshold 0.01,
+ * AngleThreshold acos(0.999), ProjectionThreshold 0.5, FilterDynamicBodies false. */
+enum { JH_FILTER_NONE = 0, JH_FILTER_TRIANGLE_EDGE = 1 };
 
 /* Warm start of the contact impulses at the start of a step.  PREVIOUS is the reference's
  * (PrepareForIteration replays the previous frame's Accumulated, Contact.cs:802-817); LEARNED first

@@ -36,8 +36,11 @@ CONFIG_DTYPE = np.dtype([
     ("persistent_contact_manifold", "<i4"),
     ("pair_capacity", "<i4"),
     ("warm_start", "<i4"),
-    ("reserved", "<i4", 5),
+    ("narrowphase_filter", "<i4"),
+    ("reserved", "<i4", 4),
 ])
+
+FILTER_NONE, FILTER_TRIANGLE_EDGE = 0, 1
 
 WARM_START_PREVIOUS, WARM_START_LEARNED = 0, 1
 LWS_FEATURES, LWS_PARAMS, PROBE_MAX_ITERATIONS = 24, 20099, 256

@@ -7,6 +7,7 @@
 
 #include "jh_collide.h"
 #include "jh_epa.h"
+#include "jh_filter.h"
 
 namespace cg = cooperative_groups;
 
@@ -423,7 +424,7 @@ __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restr
 // ---- Narrowphase ----------------------------------------------------------------------------
 struct DeepRec {
     int pair;
-    int pad;
+    int kind;  // kDeepPlain / kDeepFilter / kDeepRerun
     MVertex portal[4];
 };
 
@@ -483,11 +484,69 @@ __device__ __forceinline__ void manifold_for(const MinkowskiPair& mk, quat qa, q
     }
 }
 
+// Filtered pairs (TriangleEdgeCollisionFilter): the world-space answer is filtered before the
+// manifold is built (World.Detect.cs:197-215).
+__device__ __forceinline__ tef::WRes to_world(const MprOut& r, quat qa, vec3 posA) {
+    tef::WRes w;
+    w.pa = rot(r.pa, qa) + posA;
+    w.pb = rot(r.pb, qa) + posA;
+    w.n = rot(r.normal, qa);
+    w.pen = r.pen;
+    return w;
+}
+
+__device__ __forceinline__ void write_filtered(jh_manifold& o, uint64_t key, const tef::WRes& w, bool keep,
+                                               const MinkowskiPair& mk, quat qa, quat qb, vec3 posA, vec3 posB,
+                                               int aux, int epa) {
+    o.shape1 = uint32_t(key >> 32);
+    o.shape2 = uint32_t(key & 0xffffffffu);
+    o.flags = epa;
+    o.normal[0] = w.n.x; o.normal[1] = w.n.y; o.normal[2] = w.n.z;
+    o.penetration = w.pen;
+    o.point_a[0] = w.pa.x; o.point_a[1] = w.pa.y; o.point_a[2] = w.pa.z;
+    o.point_b[0] = w.pb.x; o.point_b[1] = w.pb.y; o.point_b[2] = w.pb.z;
+    ManifoldBuf m;
+    m.count = 0;
+    if (keep) {
+        if (aux) {
+            build_manifold(mk.a, mk.b, qa, qb, posA, posB, w.pa, w.pb, w.n, m);
+        } else {
+            m.count = 1;
+            m.mA[0] = w.pa;
+            m.mB[0] = w.pb;
+        }
+    }
+    o.count = m.count;
+    for (int i = 0; i < 4; ++i) {
+        vec3 a = i < m.count ? m.mA[i] : vec3{0, 0, 0};
+        vec3 b = i < m.count ? m.mB[i] : vec3{0, 0, 0};
+        o.manifold_a[i][0] = a.x; o.manifold_a[i][1] = a.y; o.manifold_a[i][2] = a.z;
+        o.manifold_b[i][0] = b.x; o.manifold_b[i][1] = b.y; o.manifold_b[i][2] = b.z;
+    }
+}
+
+__device__ __forceinline__ tef::Ctx filter_ctx(int side, const ShapeRec* shapes, uint64_t key, const float* verts,
+                                               quat qa, quat qb, vec3 pa, vec3 pb) {
+    tef::Ctx c;
+    c.tri = side == 1 ? &shapes[uint32_t(key >> 32)] : &shapes[uint32_t(key & 0xffffffffu)];
+    c.verts = verts;
+    c.q = side == 1 ? qa : qb;
+    c.pos = side == 1 ? pa : pb;
+    c.tri_is_b = side == 2;
+    return c;
+}
+
+// Deep-queue entry kinds (DeepRec.kind): plain EPA refinement; EPA then the whole filter; EPA as the
+// filter's re-run with threshold 0 (TriangleEdgeCollisionFilter.cs:186-210), then its second half.
+enum { kDeepPlain = 0, kDeepFilter = 1, kDeepRerun = 2 };
+
 // World.Detect (World.Detect.cs:125-218) per broadphase pair: MPR, then the manifold; pairs deeper
-// than EpaPenetrationThreshold (NarrowPhase.cs:877) go to the EPA queue.
+// than EpaPenetrationThreshold (NarrowPhase.cs:877) go to the EPA queue.  With the triangle edge
+// filter on, filtered pairs run MPR with threshold 0 so the portal is kept for the filter's re-run;
+// the MPR answer is the same for any threshold.
 __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
                                                 const ShapeRec* __restrict__ shapes, const float* __restrict__ verts,
-                                                Bodies B, int aux, jh_manifold* __restrict__ out,
+                                                Bodies B, int aux, int filter, jh_manifold* __restrict__ out,
                                                 DeepRec* __restrict__ deep, int* __restrict__ deep_count) {
     int n = *n_ptr;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
@@ -496,21 +555,55 @@ __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pai
         quat qa, qb;
         vec3 pa, pb;
         pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
+        int side = filter ? tef::applies(*mk.a.s, *mk.b.s, B.motion) : 0;
         MprOut r;
-        solve_mpr(mk, 0.02f, r);
+        solve_mpr(mk, side ? 0.0f : 0.02f, r);
         if (!r.hit) {
             write_manifold(out[p], key, r, qa, pa, nullptr, 0);
             continue;
         }
-        if (r.deep) {
+        if (!side) {
+            if (r.deep) {
+                int slot = atomicAdd(deep_count, 1);
+                DeepRec& d = deep[slot];
+                d.pair = p;
+                d.kind = kDeepPlain;
+                for (int k = 0; k < 4; ++k) d.portal[k] = r.portal[k];
+            }
+            ManifoldBuf m;
+            manifold_for(mk, qa, qb, pa, pb, r, aux, m);
+            write_manifold(out[p], key, r, qa, pa, &m, 0);
+            continue;
+        }
+        tef::Ctx c = filter_ctx(side, shapes, key, verts, qa, qb, pa, pb);
+        tef::WRes w = to_world(r, qa, pa);
+        bool keep;
+        int kind = -1;
+        if (r.pen > 0.02f) {  // MprEpa ran EPA already: the re-run repeats the same answer
+            kind = kDeepFilter;
+            keep = tef::filter_same(c, w);
+        } else {
+            vec3 tn, nn;
+            int e = tef::eval_res(c, w, tn, nn);
+            if (e == tef::Discard) {
+                keep = false;
+            } else if (e == tef::Keep) {
+                keep = true;
+            } else if (w.pen < 0.0f) {
+                keep = tef::adjust(w, tn, nn);
+            } else {
+                if (r.deep) kind = kDeepRerun;  // the re-run refines with EPA (penetration > 0)
+                keep = tef::after_rerun(c, w);  // answer if EPA fails: the MPR one again
+            }
+        }
+        if (kind >= 0) {
             int slot = atomicAdd(deep_count, 1);
             DeepRec& d = deep[slot];
             d.pair = p;
+            d.kind = kind;
             for (int k = 0; k < 4; ++k) d.portal[k] = r.portal[k];
         }
-        ManifoldBuf m;
-        manifold_for(mk, qa, qb, pa, pb, r, aux, m);
-        write_manifold(out[p], key, r, qa, pa, &m, 0);
+        write_filtered(out[p], key, w, keep, mk, qa, qb, pa, pb, aux, 0);
     }
 }
 
@@ -536,9 +629,17 @@ __global__ void __launch_bounds__(64) k_epa(const uint64_t* __restrict__ pairs, 
         MprOut r;
         r.hit = 1;
         if (!epa::solve(w, mk, r.pa, r.pb, r.normal, r.pen)) continue;
-        ManifoldBuf m;
-        manifold_for(mk, qa, qb, pa, pb, r, aux, m);
-        write_manifold(out[d.pair], key, r, qa, pa, &m, 1);
+        if (d.kind == kDeepPlain) {
+            ManifoldBuf m;
+            manifold_for(mk, qa, qb, pa, pb, r, aux, m);
+            write_manifold(out[d.pair], key, r, qa, pa, &m, 1);
+            continue;
+        }
+        int side = tef::applies(*mk.a.s, *mk.b.s, B.motion);
+        tef::Ctx c = filter_ctx(side, shapes, key, verts, qa, qb, pa, pb);
+        tef::WRes wr = to_world(r, qa, pa);
+        bool keep = d.kind == kDeepFilter ? tef::filter_same(c, wr) : tef::after_rerun(c, wr);
+        write_filtered(out[d.pair], key, wr, keep, mk, qa, qb, pa, pb, aux, 1);
     }
 }
 

@@ -232,6 +232,8 @@ jh_status check_config(const jh_config* c) {
     if (c->relaxation_iterations < 0) return fail(JH_ERR_INVALID_ARG, "relaxation_iterations must be >= 0");
     for (int k = 0; k < 3; ++k)
         if (!std::isfinite(c->gravity[k])) return fail(JH_ERR_NON_FINITE, "gravity must be finite");
+    if (c->narrowphase_filter != JH_FILTER_NONE && c->narrowphase_filter != JH_FILTER_TRIANGLE_EDGE)
+        return fail(JH_ERR_INVALID_ARG, "narrowphase_filter must be JH_FILTER_NONE or JH_FILTER_TRIANGLE_EDGE");
     if (c->warm_start != JH_WARM_START_PREVIOUS && c->warm_start != JH_WARM_START_LEARNED)
         return fail(JH_ERR_INVALID_ARG, "warm_start must be JH_WARM_START_PREVIOUS or JH_WARM_START_LEARNED");
     return JH_OK;
@@ -456,7 +458,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     // narrowphase
     if (np > 0) {
         k_narrow<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->verts.p, B, aux,
-                                                             w->man.p, w->deep.p, cnt + C_DEEP);
+                                                             cfg.narrowphase_filter, w->man.p, w->deep.p,
+                                                             cnt + C_DEEP);
         HIPCHK(hipEventRecord(w->ev[7], st));
         int et = std::min(kEpaMaxThreads, ((np + 63) / 64) * 64);
         if (et > w->epa_threads) {

@@ -15,8 +15,8 @@ import math
 
 import numpy as np
 
-from ._abi import (BODY_DTYPE, CONFIG_DTYPE, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE,
-                   SHAPE_POINTCLOUD, SHAPE_SPHERE, default_config)
+from ._abi import (BODY_DTYPE, CONFIG_DTYPE, FILTER_TRIANGLE_EDGE, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX,
+                   SHAPE_CAPSULE, SHAPE_DTYPE, SHAPE_POINTCLOUD, SHAPE_SPHERE, SHAPE_TRIANGLE, default_config)
 
 f32 = np.float32
 
@@ -103,6 +103,25 @@ class SceneBuilder:
         self.shapes: list[np.void] = []
         self.vertices: list[float] = []
         self._next_id = 1
+        self._shape_blocks: list[np.ndarray] = []   # earlier shapes, in order (bulk additions)
+        self._vert_blocks: list[np.ndarray] = []
+        self._n_vert_blocks = 0                     # vertices held in _vert_blocks
+
+    def _flush(self):
+        if self.shapes:
+            self._shape_blocks.append(np.array(self.shapes, dtype=SHAPE_DTYPE))
+            self.shapes = []
+        if self.vertices:
+            v = np.array(self.vertices, np.float32)
+            self._vert_blocks.append(v)
+            self._n_vert_blocks += len(v) // 3
+            self.vertices = []
+
+    def n_shapes(self) -> int:
+        return sum(len(b) for b in self._shape_blocks) + len(self.shapes)
+
+    def n_vertices(self) -> int:
+        return self._n_vert_blocks + len(self.vertices) // 3
 
     def body(self, position, orientation=(0.0, 0.0, 0.0, 1.0), velocity=(0, 0, 0), angular_velocity=(0, 0, 0),
              motion=MOTION_DYNAMIC, friction=0.2, restitution=0.0, world=0) -> int:
@@ -133,7 +152,7 @@ class SceneBuilder:
         p[:len(params)] = params
         s["params"] = p
         if verts is not None:
-            s["vertex_offset"] = len(self.vertices) // 3
+            s["vertex_offset"] = self.n_vertices()
             s["vertex_count"] = len(verts)
             self.vertices.extend(np.asarray(verts, np.float32).ravel().tolist())
         self.shapes.append(s)
@@ -142,7 +161,7 @@ class SceneBuilder:
         b = self.bodies[body]
         b["inverse_inertia"] = _sym_inverse(*inertia)
         b["inverse_mass"] = f32(1.0) / f32(mass)
-        return len(self.shapes) - 1
+        return self.n_shapes() - 1
 
     def box(self, body: int, size) -> int:
         half = [f32(0.5) * f32(v) for v in size]
@@ -154,10 +173,66 @@ class SceneBuilder:
     def capsule(self, body: int, radius, length) -> int:
         return self._shape(body, SHAPE_CAPSULE, [radius, f32(0.5) * f32(length)], capsule_mass(radius, length))
 
+    def hull(self, body: int, points) -> int:
+        """PointCloudShape over `points` (VertexSupportMap, PointCloudShape.cs:45-49).  GetCenter is
+        the vertex mean (an interior point; the reference uses the hull's centre of mass) and the
+        cached local box is the support box along +-x/y/z (CalcInitBox).  Mass properties: a solid
+        sphere of the mean vertex radius (density 1) — body inputs, as with SetMassInertia."""
+        pts = np.asarray(points, np.float32).reshape(-1, 3)
+        c = pts.mean(axis=0, dtype=np.float64).astype(np.float32)
+        lo, hi = pts.min(axis=0), pts.max(axis=0)
+        r = float(np.mean(np.linalg.norm(pts.astype(np.float64) - c, axis=1)))
+        return self._shape(body, SHAPE_POINTCLOUD, [*c, 0.0, 0.0, 0.0, *lo, *hi], sphere_mass(r), verts=pts)
+
+    def triangle_mesh(self, body: int, vertices, triangles) -> np.ndarray:
+        """TriangleShapes of a TriangleMesh (TriangleMesh.cs:117-319) on a static body: one shape per
+        triangle, vertices A, B, C copied per triangle, params = triangle normal and the normals of
+        NeighborA / B / C (AssignNeighbors: the triangle across the directed edge C->B, A->C, B->A;
+        zero vector at a boundary).  Normals: normalize((B - A) x (C - A)) in float32.  Returns the
+        shape indices."""
+        self._flush()
+        v = np.asarray(vertices, np.float32).reshape(-1, 3)
+        t = np.asarray(triangles, np.int64).reshape(-1, 3)
+        A, B, C = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+        n = np.cross(B - A, C - A).astype(np.float32)
+        n = (n * (np.float32(1.0) / np.sqrt((n * n).sum(axis=1, dtype=np.float32)))[:, None]).astype(np.float32)
+        first = {}
+        for i, (a, b, c) in enumerate(t.tolist()):
+            for e in ((a, b), (b, c), (c, a)):
+                first.setdefault(e, i)
+        nb = np.full((len(t), 3), -1, np.int64)
+        for i, (a, b, c) in enumerate(t.tolist()):
+            nb[i, 0] = first.get((c, b), -1)
+            nb[i, 1] = first.get((a, c), -1)
+            nb[i, 2] = first.get((b, a), -1)
+        params = np.zeros((len(t), 12), np.float32)
+        params[:, 0:3] = n
+        for k in range(3):
+            ok = nb[:, k] >= 0
+            params[ok, 3 + 3 * k:6 + 3 * k] = n[nb[ok, k]]
+        out = np.zeros(len(t), SHAPE_DTYPE)
+        out["shape_id"] = self._next_id + np.arange(len(t), dtype=np.uint64)
+        self._next_id += len(t)
+        out["body"] = body
+        out["type"] = SHAPE_TRIANGLE
+        out["params"] = params
+        out["vertex_offset"] = self.n_vertices() + 3 * np.arange(len(t))
+        out["vertex_count"] = 3
+        start = self.n_shapes()
+        self._shape_blocks.append(out)
+        tri_verts = np.stack([A, B, C], axis=1).reshape(-1)
+        self._vert_blocks.append(tri_verts.astype(np.float32))
+        self._n_vert_blocks += len(tri_verts) // 3
+        return np.arange(start, start + len(t))
+
     def arrays(self):
+        self._flush()
         bodies = np.array(self.bodies, dtype=BODY_DTYPE) if self.bodies else np.zeros(0, BODY_DTYPE)
-        shapes = np.array(self.shapes, dtype=SHAPE_DTYPE) if self.shapes else np.zeros(0, SHAPE_DTYPE)
-        verts = np.array(self.vertices, dtype=np.float32) if self.vertices else None
+        shapes = np.concatenate(self._shape_blocks) if self._shape_blocks else np.zeros(0, SHAPE_DTYPE)
+        verts = np.concatenate(self._vert_blocks) if self._vert_blocks else None
+        # keep later builder calls consistent with what was returned
+        self._shape_blocks = [shapes] if len(shapes) else []
+        self._vert_blocks = [verts] if verts is not None else []
         return bodies, shapes, verts
 
 
@@ -272,6 +347,108 @@ def batched_worlds(n_worlds: int = 4096, seed: int = 0x4A2D0003, min_bodies: int
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 10
     cfg["relaxation_iterations"] = 4
+    return (*sb.arrays(), cfg, 1.0 / 60.0)
+
+
+def perlin2(x: np.ndarray, z: np.ndarray, seed: int) -> np.ndarray:
+    """Perlin's improved gradient noise in 2-D (quintic fade, 8 gradient directions, permutation of
+    0..255 shuffled by the SplitMix64 stream of `seed`); values in about [-1, 1]."""
+    rng = SplitMix64(seed)
+    perm = list(range(256))
+    for i in range(255, 0, -1):
+        j = int(rng.next_u64() % (i + 1))
+        perm[i], perm[j] = perm[j], perm[i]
+    P = np.array(perm + perm, np.int64)
+    g = np.array([[1, 0], [-1, 0], [0, 1], [0, -1], [0.7071, 0.7071], [-0.7071, 0.7071], [0.7071, -0.7071],
+                  [-0.7071, -0.7071]])
+    xi, zi = np.floor(x).astype(np.int64), np.floor(z).astype(np.int64)
+    xf, zf = x - xi, z - zi
+    xi &= 255
+    zi &= 255
+
+    def grad(h, dx, dz):
+        gg = g[h & 7]
+        return gg[..., 0] * dx + gg[..., 1] * dz
+
+    fade = lambda t: t * t * t * (t * (t * 6 - 15) + 10)  # noqa: E731
+    u, w = fade(xf), fade(zf)
+    aa, ab = P[P[xi] + zi], P[P[xi] + zi + 1]
+    ba, bb = P[P[xi + 1] + zi], P[P[xi + 1] + zi + 1]
+    x1 = grad(aa, xf, zf) * (1 - u) + grad(ba, xf - 1, zf) * u
+    x2 = grad(ab, xf, zf - 1) * (1 - u) + grad(bb, xf - 1, zf - 1) * u
+    return x1 * (1 - w) + x2 * w
+
+
+def heightfield(cells: int, cell_size: float, amplitude: float, seed: int, x0: float = 0.0, z0: float = 0.0,
+                feature: float = 48.0):
+    """(cells + 1)^2 grid vertices with heights amplitude * (perlin(x / feature) + 0.5 perlin(x /
+    (feature / 2))) / 1.5 and 2 cells^2 triangles, wound so their normals point up (+y)."""
+    i = np.arange(cells + 1)
+    X, Z = np.meshgrid(x0 + i * cell_size, z0 + i * cell_size, indexing="ij")
+    H = (perlin2(X / feature, Z / feature, seed) + 0.5 * perlin2(X / (feature / 2), Z / (feature / 2), seed ^ 0x5A5A))
+    H = amplitude * H / 1.5
+    verts = np.stack([X, H, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    k = np.arange(cells)
+    a, b = np.meshgrid(k, k, indexing="ij")
+    v00 = a * (cells + 1) + b          # (x0, z0)
+    v10 = (a + 1) * (cells + 1) + b    # (x1, z0)
+    v01 = a * (cells + 1) + b + 1      # (x0, z1)
+    v11 = (a + 1) * (cells + 1) + b + 1
+    t1 = np.stack([v00, v01, v10], axis=-1).reshape(-1, 3)
+    t2 = np.stack([v10, v01, v11], axis=-1).reshape(-1, 3)
+    tris = np.stack([t1, t2], axis=1).reshape(-1, 3)
+    return verts, tris, H
+
+
+def full_form(n_bodies: int = 262144, cells: int = 512, worlds: int = 64, seed: int = 0x4A2D0005,
+              cell_size: float = 1.0, amplitude: float = 5.0, pitch: float = 2.2, feature: float = 48.0):
+    """Bodies over a static triangle-mesh heightfield: `cells` x `cells` cells (Perlin heights,
+    amplitude 5, main feature size `feature` metres) split into `worlds` square tiles, each tile the static ground of its own sub-world
+    (a TriangleMesh body) with an uneven, log-uniform share of the bodies dropped on a jittered grid
+    above it.  Shapes uniform over sphere, box, capsule (as in the heap) and 32-point hulls (points
+    uniform on the unit sphere, scaled U(0.3, 1.0), PointCloudShape).  TriangleEdgeCollisionFilter
+    on; dt = 1/60, 4 substeps, 10 iterations."""
+    rng = SplitMix64(seed)
+    tiles = int(round(math.sqrt(worlds)))
+    assert tiles * tiles == worlds and cells % tiles == 0
+    tc = cells // tiles
+    wts = np.array([math.exp(rng.uniform(0.0, math.log(16.0))) for _ in range(worlds)])
+    counts = np.floor(wts / wts.sum() * n_bodies).astype(int)
+    counts[: n_bodies - counts.sum()] += 1
+    sb = SceneBuilder()
+    side = tc * cell_size
+    per_row = max(1, int((side - 2.0) // pitch))
+    for w in range(worlds):
+        tx, tz = divmod(w, tiles)
+        x0, z0 = tx * side, tz * side
+        verts, tris, H = heightfield(tc, cell_size, amplitude, seed, x0, z0, feature)
+        g = sb.body((0.0, 0.0, 0.0), motion=MOTION_STATIC, friction=0.5, world=w)
+        sb.bodies[g]["inverse_mass"] = 0.0
+        sb.bodies[g]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+        sb.triangle_mesh(g, verts, tris)
+        top = float(H.max()) + 2.0
+        for i in range(int(counts[w])):
+            layer, rem = divmod(i, per_row * per_row)
+            iz, ix = divmod(rem, per_row)
+            pos = (x0 + 1.0 + (ix + 0.5) * pitch + rng.uniform(-0.05, 0.05), top + layer * pitch,
+                   z0 + 1.0 + (iz + 0.5) * pitch + rng.uniform(-0.05, 0.05))
+            b = sb.body(pos, orientation=rng.quaternion(), velocity=(rng.normal(), rng.normal(), rng.normal()),
+                        world=w)
+            k = int(rng.uniform() * 4) % 4
+            if k == 3:
+                pts = []
+                scale = rng.uniform(0.3, 1.0)
+                for _ in range(32):
+                    d = np.array([rng.normal(), rng.normal(), rng.normal()])
+                    pts.append(d / np.linalg.norm(d) * scale)
+                sb.hull(b, pts)
+            else:
+                _random_shape(sb, rng, b, (SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE)[k:k + 1])
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    cfg["solver_iterations"] = 10
+    cfg["relaxation_iterations"] = 4
+    cfg["narrowphase_filter"] = FILTER_TRIANGLE_EDGE
     return (*sb.arrays(), cfg, 1.0 / 60.0)
 
 

@@ -26,6 +26,7 @@ template <class R> void set_config(World<R>& w, const jh_config* c) {
     w.specrel = R(c->speculative_relaxation_factor);
     w.aux_contacts = c->enable_auxiliary_contact_points != 0;
     w.persistent = c->persistent_contact_manifold != 0;
+    w.triangle_edge_filter = c->narrowphase_filter == JH_FILTER_TRIANGLE_EDGE;
 }
 
 template <class R> V3<R> vin(const float* f) { return {R(f[0]), R(f[1]), R(f[2])}; }

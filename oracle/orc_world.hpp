@@ -92,7 +92,7 @@ template <class R> struct World {
     V3<R> gravity{R(0), R(-9.81), R(0)};
     int substeps = 1, iterations = 6, relaxations = 4;
     R specrel = R(0.9);
-    bool aux_contacts = true, persistent = true;
+    bool aux_contacts = true, persistent = true, triangle_edge_filter = false;
     std::vector<ManifoldOut<R>> last_manifolds;
     std::vector<std::pair<uint32_t, uint32_t>> last_pairs;
     long epa_calls = 0;
@@ -250,6 +250,97 @@ template <class R> struct World {
 
     // World.Detect (World.Detect.cs:125-218) for one broadphase pair (k1 < k2), default filters,
     // no speculative contacts; the default TriangleEdgeCollisionFilter only acts on TriangleShape.
+    // ---- TriangleEdgeCollisionFilter (Collision/CollisionFilter/TriangleEdgeCollisionFilter.cs) ----
+    // Triangle shapes carry their normal and the normals of NeighborA/B/C in p[0..11] (zero = none).
+    enum { TefKeep = 0, TefDiscard = 1, TefEdge = 2 };
+    static constexpr R kEdgeThreshold = R(0.01), kCosAngle = R(0.999), kProjectionThreshold = R(0.5);
+
+    // EvaluateTriangleContact (:86-147)
+    int tef_evaluate(const Shape<R>& t, const Body<R>& tb, bool tri_is_b, V3<R> cp, V3<R> normal, V3<R>& tri_n,
+                     V3<R>& nb_n) const {
+        tri_n = rotate(V3<R>{t.p[0], t.p[1], t.p[2]}, tb.q);
+        nb_n = V3<R>{R(0), R(0), R(0)};
+        if (tri_is_b) tri_n = -tri_n;
+        if (dot(normal, tri_n) < -kCosAngle) return TefDiscard;
+        V3<R> a = rotate(V3<R>{R(t.verts[0]), R(t.verts[1]), R(t.verts[2])}, tb.q);
+        V3<R> b = rotate(V3<R>{R(t.verts[3]), R(t.verts[4]), R(t.verts[5])}, tb.q);
+        V3<R> c = rotate(V3<R>{R(t.verts[6]), R(t.verts[7]), R(t.verts[8])}, tb.q);
+        a = a + tb.pos;
+        b = b + tb.pos;
+        c = c + tb.pos;
+        V3<R> nrm = cross(b - a, c - a);
+        R inv = R(1.0) / std::sqrt(len_sq(nrm));
+        nrm = V3<R>{nrm.x * inv, nrm.y * inv, nrm.z * inv};
+        R dist = dot(cp - a, nrm);
+        V3<R> pp = cp - nrm * dist;
+        V3<R> n = b - a;
+        V3<R> pma = pp - a;
+        R d0 = len_sq(pma - (n * dot(pma, n)) * (R(1.0) / len_sq(n)));
+        n = c - a;
+        pma = pp - a;
+        R d1 = len_sq(pma - (n * dot(pma, n)) * (R(1.0) / len_sq(n)));
+        n = c - b;
+        pma = pp - b;
+        R d2 = len_sq(pma - (n * dot(pma, n)) * (R(1.0) / len_sq(n)));
+        if (std::min(std::min(d0, d1), d2) > kEdgeThreshold * kEdgeThreshold) return TefKeep;
+        int k = (d0 < d1 && d0 < d2) ? 3 : (d1 < d2 ? 2 : 1);  // NeighborC / NeighborB / NeighborA
+        V3<R> nn{t.p[3 * k], t.p[3 * k + 1], t.p[3 * k + 2]};
+        if (nn.x == R(0) && nn.y == R(0) && nn.z == R(0)) return TefKeep;
+        nb_n = rotate(nn, tb.q);
+        if (tri_is_b) nb_n = -nb_n;
+        return TefEdge;
+    }
+
+    // Filter (:149-323); false = discard.  r may be replaced by the re-run with EPA threshold 0.
+    bool tef_filter(const Shape<R>& sA, const Shape<R>& sB, const Body<R>& b1, const Body<R>& b2, MprResult<R>& r) {
+        bool c1 = sA.type == JH_SHAPE_TRIANGLE, c2 = sB.type == JH_SHAPE_TRIANGLE;
+        if (c1 == c2) return true;
+        const Shape<R>& t = c1 ? sA : sB;
+        const Body<R>& tb = c1 ? b1 : b2;
+        if (tb.motion == JH_MOTION_DYNAMIC) return true;
+        V3<R> tn, nn;
+        int e = tef_evaluate(t, tb, c2, c1 ? r.pa : r.pb, r.normal, tn, nn);
+        if (e == TefDiscard) return false;
+        if (e == TefKeep) return true;
+        bool spec = r.penetration < R(0);
+        if (!spec) {
+            MprResult<R> r2 = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, poly, R(0));
+            if (r2.epa) epa_calls++;
+            if (!r2.hit) return false;
+            r = r2;
+            e = tef_evaluate(t, tb, c2, c1 ? r.pa : r.pb, r.normal, tn, nn);
+            if (e == TefDiscard) return false;
+            if (e == TefKeep) return true;
+        }
+        V3<R> mid = (r.pa + r.pb) * R(0.5);
+        if (dot(tn, nn) > kCosAngle) {
+            R f5 = dot(r.normal, nn), f6 = dot(r.normal, tn);
+            if (!spec) {
+                r.penetration = R(0);
+                r.pa = mid;
+                r.pb = mid;
+            }
+            r.normal = f5 > f6 ? nn : tn;
+            return true;
+        }
+        V3<R> cr = cross(nn, tn);
+        R cl = len_sq(cr);
+        V3<R> proj = r.normal - cr * (dot(cr, r.normal) / cl);
+        if (len_sq(proj) < kProjectionThreshold * kProjectionThreshold) return false;
+        R f1 = dot(cross(proj, nn), cr), f2 = dot(cross(proj, tn), cr);
+        bool between = f1 <= R(0) && f2 >= R(0);
+        if (!between) {
+            R f3 = dot(r.normal, nn), f4 = dot(r.normal, tn);
+            if (!spec) {
+                r.penetration = R(0);
+                r.pa = mid;
+                r.pb = mid;
+            }
+            r.normal = f3 > f4 ? nn : tn;
+        }
+        return true;
+    }
+
     void detect(uint32_t k1, uint32_t k2) {
         const Shape<R>& sA = shapes[k1];
         const Shape<R>& sB = shapes[k2];
@@ -259,6 +350,7 @@ template <class R> struct World {
         MprResult<R> r = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, poly);
         if (r.epa) epa_calls++;
         if (!r.hit) return;
+        if (triangle_edge_filter && !tef_filter(sA, sB, b1, b2, r)) return;
         ManifoldOut<R> mo{};
         mo.k1 = k1;
         mo.k2 = k2;

@@ -150,3 +150,35 @@ def test_autotune_keeps_results_exact():
         ow.step(dt, gw.solve_order()[0])
     assert_bodies_equal(gw.bodies(), ow.bodies(), "after autotune")
     assert_arbiters_equal(gw.arbiters(), ow.arbiters(), "after autotune")
+
+
+def _seam_scene(pos):
+    sb = scenes.SceneBuilder()
+    g = sb.body((0.0, 0.0, 0.0), motion=scenes.MOTION_STATIC, friction=0.5)
+    sb.bodies[g]["inverse_mass"] = 0.0
+    sb.bodies[g]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    v = np.array([[-2, 0, -2], [2, 0, -2], [-2, 0, 2], [2, 0, 2]], np.float32)
+    sb.triangle_mesh(g, v, np.array([[0, 2, 1], [1, 2, 3]]))
+    b = sb.body(pos, velocity=(0.5, 0.0, -0.3))
+    sb.sphere(b, 0.5)
+    c = sb.body((1.0, 0.6, 1.0), orientation=(0.1, 0.2, 0.0, 0.9747), velocity=(-0.4, 0.0, -0.4))
+    sb.box(c, (0.8, 0.8, 0.8))
+    bodies, shapes, verts = sb.arrays()
+    cfg = scenes.default_config()
+    cfg["narrowphase_filter"] = 1
+    cfg["substep_count"] = 2
+    return bodies, shapes, verts, cfg, 0.01
+
+
+def test_triangle_edge_filter_seam_bitexact():
+    # contacts on the shared edge of a two-triangle mesh go through the filter's EPA re-run
+    gw, ow = run_replay(_seam_scene((0.0, 0.45, 0.0)), steps=40, brute_pairs=True)
+    assert gw.stats()["epa_calls"] >= 0
+
+
+def test_heightfield_hulls_filter_bitexact():
+    # the full form at small scale: 4 sub-worlds, each a Perlin heightfield tile (TriangleMesh with
+    # the edge filter) under spheres, boxes, capsules and 32-point hulls
+    scene = scenes.full_form(n_bodies=240, cells=32, worlds=4, feature=8.0, pitch=2.4)
+    gw, ow = run_replay(scene, steps=90, check_every=15, brute_pairs=True)
+    assert gw.stats()["arbiters"] > 50
