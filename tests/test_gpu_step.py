@@ -182,3 +182,20 @@ def test_heightfield_hulls_filter_bitexact():
     scene = scenes.full_form(n_bodies=240, cells=32, worlds=4, feature=8.0, pitch=2.4)
     gw, ow = run_replay(scene, steps=90, check_every=15, brute_pairs=True)
     assert gw.stats()["arbiters"] > 50
+
+
+def test_fp32_error_against_fp64_oracle():
+    # the FP64 oracle (Jitter2.Double) replays the GPU's colour order; the FP32 error is reported and
+    # bounded (tolerance: 1e-4 m in position, 1e-3 m/s in velocity after 30 steps)
+    bodies, shapes, verts, cfg, dt = scenes.pyramid(rows=20)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    o64 = OracleWorld(bodies, shapes, verts, cfg, fp64=True)
+    for _ in range(30):
+        gw.step(dt)
+        o64.step(dt, gw.solve_order()[0])
+    g = gw.bodies()
+    s64 = o64.state64()
+    dp = np.abs(g["position"].astype(np.float64) - s64[:, 0:3]).max()
+    dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10]).max()
+    print(f"FP32 GPU vs FP64 oracle after 30 steps: max |dp| {dp:.3e} m, max |dv| {dv:.3e} m/s")
+    assert dp < 1e-4 and dv < 1e-3
