@@ -7,6 +7,7 @@
 
 #include "jh_collide.h"
 #include "jh_epa.h"
+#include "jh_epa_wave.h"
 #include "jh_filter.h"
 
 namespace cg = cooperative_groups;
@@ -607,6 +608,24 @@ __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pai
     }
 }
 
+// A successful EPA answer replaces the MPR one for the pair (filtered pairs go through the triangle
+// edge filter first).
+__device__ __forceinline__ void epa_finish(const DeepRec& d, uint64_t key, const ShapeRec* shapes, const float* verts,
+                                           const Bodies& B, const MinkowskiPair& mk, quat qa, quat qb, vec3 pa,
+                                           vec3 pb, const MprOut& r, int aux, jh_manifold* out) {
+    if (d.kind == kDeepPlain) {
+        ManifoldBuf m;
+        manifold_for(mk, qa, qb, pa, pb, r, aux, m);
+        write_manifold(out[d.pair], key, r, qa, pa, &m, 1);
+        return;
+    }
+    int side = tef::applies(*mk.a.s, *mk.b.s, B.motion);
+    tef::Ctx c = filter_ctx(side, shapes, key, verts, qa, qb, pa, pb);
+    tef::WRes wr = to_world(r, qa, pa);
+    bool keep = d.kind == kDeepFilter ? tef::filter_same(c, wr) : tef::after_rerun(c, wr);
+    write_filtered(out[d.pair], key, wr, keep, mk, qa, qb, pa, pb, aux, 1);
+}
+
 // EPA over the deep queue: a successful EPA replaces the MPR answer and its manifold.  `work`
 // holds gridDim.x * blockDim.x interleaved polytopes (epa::kBytesPerThread each).
 __global__ void __launch_bounds__(64) k_epa(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
@@ -629,17 +648,38 @@ __global__ void __launch_bounds__(64) k_epa(const uint64_t* __restrict__ pairs, 
         MprOut r;
         r.hit = 1;
         if (!epa::solve(w, mk, r.pa, r.pb, r.normal, r.pen)) continue;
-        if (d.kind == kDeepPlain) {
-            ManifoldBuf m;
-            manifold_for(mk, qa, qb, pa, pb, r, aux, m);
-            write_manifold(out[d.pair], key, r, qa, pa, &m, 1);
-            continue;
-        }
-        int side = tef::applies(*mk.a.s, *mk.b.s, B.motion);
-        tef::Ctx c = filter_ctx(side, shapes, key, verts, qa, qb, pa, pb);
-        tef::WRes wr = to_world(r, qa, pa);
-        bool keep = d.kind == kDeepFilter ? tef::filter_same(c, wr) : tef::after_rerun(c, wr);
-        write_filtered(out[d.pair], key, wr, keep, mk, qa, qb, pa, pb, aux, 1);
+        epa_finish(d, key, shapes, verts, B, mk, qa, qb, pa, pb, r, aux, out);
+    }
+}
+
+// The wave-cooperative form: one wavefront (workgroup of 64) per deep pair, polytope in LDS, the
+// per-wave HBM area of jh_epa_wave.h in `side`.  Same results as k_epa bit for bit.
+__global__ void __launch_bounds__(64) k_epa_wave(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
+                                                 const float* __restrict__ verts, Bodies B, int aux,
+                                                 jh_manifold* __restrict__ out, const DeepRec* __restrict__ deep,
+                                                 const int* __restrict__ deep_count, char* __restrict__ side) {
+    __shared__ epw::Lds L;
+    int lane = threadIdx.x;
+    epw::Poly w;
+    w.L = &L;
+    char* base = side + size_t(blockIdx.x) * epw::kSideBytes;
+    w.tf = (float*)base;
+    w.bary = (double*)(base + epw::kTfBytes);
+    w.ab = (float*)(base + epw::kTfBytes + epw::kBaryBytes);
+    int n = *deep_count;
+    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+        const DeepRec& d = deep[q];
+        uint64_t key = pairs[d.pair];
+        MinkowskiPair mk;
+        quat qa, qb;
+        vec3 pa, pb;
+        pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
+        for (int k = 0; k < 4; ++k) epw::set_vertex(w, k, d.portal[k], lane);
+        MprOut r;
+        r.hit = 1;
+        bool ok = epw::solve(w, mk, lane, r.pa, r.pb, r.normal, r.pen);
+        if (ok && lane == 0) epa_finish(d, key, shapes, verts, B, mk, qa, qb, pa, pb, r, aux, out);
+        epw::wave_sync();
     }
 }
 

@@ -162,8 +162,10 @@ struct jh_world {
     // narrowphase
     DBuf<jh_manifold> man;
     DBuf<DeepRec> deep;
-    DBuf<char> epa_work;
+    DBuf<char> epa_work, epa_side;
     int epa_threads = 0;
+    bool epa_wave = true;      // wave-cooperative EPA (k_epa_wave); false: one thread per pair (k_epa)
+    int epa_wave_blocks = 0;   // resident single-wave workgroups of k_epa_wave
     // contact cache
     DBuf<uint32_t> flags, offs, sums, cnew, noffs;
     DBuf<uint64_t> ckey, nkey;
@@ -461,13 +463,20 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
                                                              cfg.narrowphase_filter, w->man.p, w->deep.p,
                                                              cnt + C_DEEP);
         HIPCHK(hipEventRecord(w->ev[7], st));
-        int et = std::min(kEpaMaxThreads, ((np + 63) / 64) * 64);
-        if (et > w->epa_threads) {
-            HIPCHK(w->epa_work.ensure(size_t(et) * epa::kBytesPerThread));
-            w->epa_threads = et;
+        if (w->epa_wave) {
+            int blocks = std::max(1, std::min(w->epa_wave_blocks, np));
+            HIPCHK(w->epa_side.ensure(size_t(w->epa_wave_blocks) * epw::kSideBytes));
+            k_epa_wave<<<blocks, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p, w->deep.p,
+                                              cnt + C_DEEP, w->epa_side.p);
+        } else {
+            int et = std::min(kEpaMaxThreads, ((np + 63) / 64) * 64);
+            if (et > w->epa_threads) {
+                HIPCHK(w->epa_work.ensure(size_t(et) * epa::kBytesPerThread));
+                w->epa_threads = et;
+            }
+            k_epa<<<w->epa_threads / 64, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p, w->deep.p,
+                                                      cnt + C_DEEP, w->epa_work.p);
         }
-        k_epa<<<w->epa_threads / 64, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p, w->deep.p,
-                                                  cnt + C_DEEP, w->epa_work.p);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(w->ev[2], st));
@@ -699,6 +708,9 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     w->coop_colour = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve, 256, 0));
     w->coop_solve = std::max(1, occ) * w->cu_count;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_epa_wave, 64, 0));
+    w->epa_wave_blocks = std::max(1, occ) * w->cu_count;
+    if (const char* e = getenv("JH_EPA")) w->epa_wave = strcmp(e, "thread") != 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)lws::k_probe, 256, 0));
     w->coop_probe = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_flow, 256, 0));
@@ -732,7 +744,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->mk0.release(); w->mk1.release(); w->mv0.release(); w->mv1.release();
     w->left.release(); w->right.release(); w->parent.release(); w->leaf_parent.release(); w->nflags.release();
     w->range.release(); w->bounds.release(); w->hist.release(); w->pk0.release(); w->pk1.release();
-    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release();
+    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release();
     w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
