@@ -1455,8 +1455,9 @@ __device__ __forceinline__ void iterate_reg(uint32_t mode, float friction, SlotR
 // the shared velocities are read, updated in registers, stored coherently and published; the
 // private contact results are written back after publishing.
 template <bool PREPARE>
-__device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, bool apply_bias) {
+__device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, bool apply_bias, bool active) {
     const Bodies& B = P.B;
+    if (!active) k = 0;  // inactive lanes of the wave's last chunk mirror a valid arbiter but do nothing
     jh_arbiter& a = P.arb[uint32_t(P.order[k])];
     int b1 = a.body1, b2 = a.body2;
     uint32_t usage = a.usage_mask, mode = a.mode;
@@ -1482,31 +1483,39 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     }
     if (!m1) { R1.v = ld3(B.vel[b1]); R1.w = ld3(B.angvel[b1]); }
     if (!m2) { R2.v = ld3(B.vel[b2]); R2.w = ld3(B.angvel[b2]); }
+    // Each lane computes and publishes as soon as its own bodies are ready; the wave leaves when all
+    // of its lanes have (a lane never waits for an unrelated lane's dependencies).
+    bool done = !active;
     while (true) {
-        int c1 = m1 ? __hip_atomic_load(&P.counter[b1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-        int c2 = m2 ? __hip_atomic_load(&P.counter[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-        if (c1 >= need1 && c2 >= need2) break;
+        if (!done) {
+            int c1 = m1 ? __hip_atomic_load(&P.counter[b1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            int c2 = m2 ? __hip_atomic_load(&P.counter[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+            if (c1 >= need1 && c2 >= need2) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (m1) coh_load_vw(B, b1, R1.v, R1.w);
+                if (m2) coh_load_vw(B, b2, R2.v, R2.w);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (!(usage & (1u << s))) continue;
+                    if (PREPARE) prepare_reg(mode, specrel, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
+                    else iterate_reg(mode, friction, sr[s], R1, R2, apply_bias);
+                }
+                if (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) coh_store_vw(B, b1, R1.v, R1.w);
+                if (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) coh_store_vw(B, b2, R2.v, R2.w);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (m1) __hip_atomic_fetch_add(&P.counter[b1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (m2) __hip_atomic_fetch_add(&P.counter[b2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    if (usage & (1u << s)) store_slot(a.contacts[s], sr[s], PREPARE);
+                done = true;
+            }
+        }
+        if (__all(done)) break;
         __builtin_amdgcn_s_sleep(1);
     }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (m1) coh_load_vw(B, b1, R1.v, R1.w);
-    if (m2) coh_load_vw(B, b2, R2.v, R2.w);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        if (!(usage & (1u << s))) continue;
-        if (PREPARE) prepare_reg(mode, specrel, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
-        else iterate_reg(mode, friction, sr[s], R1, R2, apply_bias);
-    }
-    if (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) coh_store_vw(B, b1, R1.v, R1.w);
-    if (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) coh_store_vw(B, b2, R2.v, R2.w);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (m1) __hip_atomic_fetch_add(&P.counter[b1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (m2) __hip_atomic_fetch_add(&P.counter[b2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-        if (usage & (1u << s)) store_slot(a.contacts[s], sr[s], PREPARE);
 }
 
 __device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int npass, int phase_first,
@@ -1521,9 +1530,9 @@ __device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int
         for (int j = gw; j < J; j += W) {
             int2 ch = P.chunks[j];
             int k = ch.x + lane;
-            if (k >= ch.y) continue;
-            if (phase == 0) flow_visit<true>(P, k, g, false);
-            else flow_visit<false>(P, k, g, phase == 1);
+            bool active = k < ch.y;
+            if (phase == 0) flow_visit<true>(P, k, g, false, active);
+            else flow_visit<false>(P, k, g, phase == 1, active);
         }
     }
 }
