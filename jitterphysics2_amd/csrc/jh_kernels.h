@@ -900,72 +900,88 @@ struct ColourArgs {
     const int* a_count;
     const int* motion;
     int* colour;
-    unsigned long long* body_max;
+    unsigned long long* body_max;  // [2][nb]: Jones-Plassmann maxima, double-buffered by round parity
+    int nb;
     unsigned long long* mask_lo;
     unsigned long long* mask_hi;
-    int* remaining;  // [2]
+    int* work;           // [2][A]: uncoloured arbiters, double-buffered by round parity
+    int* wcount;         // [2] worklist sizes
     int* ncolours;
-    int* hist;       // [128]
-    unsigned* bar;   // grid barrier state
-    int keep;        // reuse the colours of arbiters that persist from the previous step
+    int* hist;           // [128]
+    unsigned* bar;       // grid barrier state
+    int keep;            // reuse the colours of arbiters that persist from the previous step
 };
 
 // Graph colouring of the arbiters: edges are arbiters, nodes are non-static bodies (static bodies
 // never change, so arbiters through them never conflict).  Arbiters keep their colour between
 // steps: the bodies of an arbiter never change, so persisting colours stay a proper colouring, and
-// only new arbiters are coloured, in Jones-Plassmann rounds (local priority maxima take the first
-// colour free on both bodies).  Colours >= 127 fall into a serial batch.
+// only the others are coloured, in Jones-Plassmann rounds over a worklist (local priority maxima take
+// the first colour free on both bodies; the losers form the next round's worklist).  Two grid
+// barriers per round: the priority maxima of round r live in body_max[r & 1], and the decide sweep of
+// round r clears body_max[(r + 1) & 1] for its bodies, which are the only ones round r + 1 touches.
+// Colours >= 127 fall into a serial batch.
 __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
     int A = *P.a_count;
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long* bmax[2] = {P.body_max, P.body_max + P.nb};
+    int* work[2] = {P.work, P.work + A};
     for (int a = t0; a < A; a += stride) {
         const jh_arbiter& r = P.arb[a];
         bool active = (r.usage_mask & 15u) != 0;
         P.colour[a] = active ? -1 : -2;
-        P.body_max[r.body1] = 0;
-        P.body_max[r.body2] = 0;
+        bmax[0][r.body1] = 0;
+        bmax[0][r.body2] = 0;
         P.mask_lo[r.body1] = 0;
         P.mask_lo[r.body2] = 0;
         P.mask_hi[r.body1] = 0;
         P.mask_hi[r.body2] = 0;
     }
     if (t0 < 128) P.hist[t0] = 0;
-    if (t0 == 0) { P.remaining[0] = 0; P.remaining[1] = 0; *P.ncolours = 0; }
+    if (t0 == 0) { P.wcount[0] = 0; P.wcount[1] = 0; *P.ncolours = 0; }
     grid_barrier(P.bar);
-    if (P.keep) {
-        for (int a = t0; a < A; a += stride) {
-            if (P.colour[a] != -1) continue;
-            const jh_arbiter& r = P.arb[a];
-            uint32_t c = r.reserved;
-            if (c >= uint32_t(kMaxColours)) continue;
+    // persisting colours, and the initial worklist
+    for (int a = t0; a < A; a += stride) {
+        if (P.colour[a] != -1) continue;
+        const jh_arbiter& r = P.arb[a];
+        uint32_t c = r.reserved;
+        if (P.keep && c < uint32_t(kMaxColours)) {
             P.colour[a] = int(c);
             unsigned long long bit = 1ull << (c & 63);
             unsigned long long* m1 = c < 64 ? P.mask_lo : P.mask_hi;
             if (P.motion[r.body1] != JH_MOTION_STATIC) atomicOr(&m1[r.body1], bit);
             if (P.motion[r.body2] != JH_MOTION_STATIC) atomicOr(&m1[r.body2], bit);
             atomicAdd(&P.hist[c], 1);
+        } else {
+            work[0][atomicAdd(&P.wcount[0], 1)] = a;
         }
-        grid_barrier(P.bar);
     }
+    grid_barrier(P.bar);
     for (int round = 0;; ++round) {
-        for (int a = t0; a < A; a += stride) {
-            if (P.colour[a] != -1) continue;
+        int cur = round & 1, nxt = cur ^ 1;
+        int n = __hip_atomic_load(&P.wcount[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (n == 0) break;
+        unsigned long long* bm = bmax[cur];
+        unsigned long long* bn = bmax[nxt];
+        for (int i = t0; i < n; i += stride) {
+            int a = work[cur][i];
             const jh_arbiter& r = P.arb[a];
             unsigned long long prio = colour_prio(r, a);
-            if (P.motion[r.body1] != JH_MOTION_STATIC) atomicMax(&P.body_max[r.body1], prio);
-            if (P.motion[r.body2] != JH_MOTION_STATIC) atomicMax(&P.body_max[r.body2], prio);
+            if (P.motion[r.body1] != JH_MOTION_STATIC) atomicMax(&bm[r.body1], prio);
+            if (P.motion[r.body2] != JH_MOTION_STATIC) atomicMax(&bm[r.body2], prio);
         }
+        if (t0 == 0) P.wcount[nxt] = 0;
         grid_barrier(P.bar);
-        int* rem = &P.remaining[round & 1];
-        for (int a = t0; a < A; a += stride) {
-            if (P.colour[a] != -1) continue;
+        for (int i = t0; i < n; i += stride) {
+            int a = work[cur][i];
             const jh_arbiter& r = P.arb[a];
             unsigned long long prio = colour_prio(r, a);
             bool d1 = P.motion[r.body1] != JH_MOTION_STATIC, d2 = P.motion[r.body2] != JH_MOTION_STATIC;
-            bool win = (!d1 || P.body_max[r.body1] == prio) && (!d2 || P.body_max[r.body2] == prio);
+            bool win = (!d1 || bm[r.body1] == prio) && (!d2 || bm[r.body2] == prio);
+            bn[r.body1] = 0;
+            bn[r.body2] = 0;
             if (!win) {
-                atomicAdd(rem, 1);
+                work[nxt][atomicAdd(&P.wcount[nxt], 1)] = a;
                 continue;
             }
             unsigned long long lo = 0, hi = 0;
@@ -987,16 +1003,6 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
             atomicAdd(&P.hist[c], 1);
         }
         grid_barrier(P.bar);
-        int left = atomicAdd(rem, 0);
-        if (t0 == 0) P.remaining[(round + 1) & 1] = 0;
-        for (int a = t0; a < A; a += stride) {
-            if (P.colour[a] != -1) continue;
-            const jh_arbiter& r = P.arb[a];
-            P.body_max[r.body1] = 0;
-            P.body_max[r.body2] = 0;
-        }
-        grid_barrier(P.bar);
-        if (left == 0) break;
     }
     for (int a = t0; a < A; a += stride) {
         int c = P.colour[a];

@@ -176,7 +176,8 @@ struct jh_world {
     int a_host = 0;  // arbiters in arb[cur] (after the last step / upload)
     // colouring + order
     DBuf<int> colour, chist, coffs;
-    DBuf<unsigned long long> bmax, mlo, mhi;
+    DBuf<unsigned long long> bmax, mlo, mhi;  // bmax: 2 x body capacity
+    DBuf<int> cwork;                          // colouring worklists: 2 x arbiter capacity
     DBuf<uint64_t> ok0, ok1, solve_keys;
     // dataflow solver
     DBuf<uint64_t> rk0, rk1;
@@ -330,6 +331,7 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->arb[1 - w->cur].ensure(na));
     HIPCHK(w->akey[1 - w->cur].ensure(na));
     HIPCHK(w->colour.ensure(na));
+    HIPCHK(w->cwork.ensure(2 * na));
     HIPCHK(w->ok0.ensure(na));
     HIPCHK(w->ok1.ensure(na));
     HIPCHK(w->solve_keys.ensure(na));
@@ -516,9 +518,11 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         P.motion = w->motion.p;
         P.colour = w->colour.p;
         P.body_max = w->bmax.p;
+        P.nb = int(w->bmax.cap / 2);
         P.mask_lo = w->mlo.p;
         P.mask_hi = w->mhi.p;
-        P.remaining = cnt + C_REM;
+        P.work = w->cwork.p;
+        P.wcount = cnt + C_REM;
         P.ncolours = cnt + C_NCOL;
         P.hist = w->chist.p;
         P.bar = w->bar.p;
@@ -748,7 +752,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
-    w->colour.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
+    w->colour.release(); w->cwork.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
     w->bcounter.release(); w->cbeg.release(); w->bar.release();
@@ -794,7 +798,7 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     HIPCHK(w->gravity.ensure(nn));
     HIPCHK(w->world.ensure(nn));
     HIPCHK(w->body_io.ensure(nn));
-    HIPCHK(w->bmax.ensure(nn));
+    HIPCHK(w->bmax.ensure(2 * nn));
     HIPCHK(w->mlo.ensure(nn));
     HIPCHK(w->mhi.ensure(nn));
     HIPCHK(w->deg.ensure(nn));
