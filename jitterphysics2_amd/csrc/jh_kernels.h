@@ -55,6 +55,20 @@ __device__ __forceinline__ void grid_barrier(unsigned* bar) {
     __syncthreads();
 }
 
+// Slot in a list appended to by many threads: one global atomic per wave (ballot + prefix count)
+// instead of one per element.  Call with the lanes that may append active; returns -1 for !pred.
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+    unsigned long long m = __ballot(pred);
+    if (!m) return -1;
+    int lane = threadIdx.x & 63;
+    int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, 64);
+    unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    return pred ? base + __popcll(m & below) : -1;
+}
+
 __device__ __forceinline__ sym3 ld_iw(const Bodies& B, int b) {
     float4 a = B.iw0[b];
     float2 c = B.iw1[b];
@@ -921,9 +935,11 @@ struct ColourArgs {
 // round r clears body_max[(r + 1) & 1] for its bodies, which are the only ones round r + 1 touches.
 // Colours >= 127 fall into a serial batch.
 __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
+    __shared__ int lhist[128];  // per-workgroup colour counts, added to P.hist once at the end
     int A = *P.a_count;
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < 128) lhist[threadIdx.x] = 0;
     unsigned long long* bmax[2] = {P.body_max, P.body_max + P.nb};
     int* work[2] = {P.work, P.work + A};
     for (int a = t0; a < A; a += stride) {
@@ -942,19 +958,23 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
     grid_barrier(P.bar);
     // persisting colours, and the initial worklist
     for (int a = t0; a < A; a += stride) {
-        if (P.colour[a] != -1) continue;
-        const jh_arbiter& r = P.arb[a];
-        uint32_t c = r.reserved;
-        if (P.keep && c < uint32_t(kMaxColours)) {
-            P.colour[a] = int(c);
-            unsigned long long bit = 1ull << (c & 63);
-            unsigned long long* m1 = c < 64 ? P.mask_lo : P.mask_hi;
-            if (P.motion[r.body1] != JH_MOTION_STATIC) atomicOr(&m1[r.body1], bit);
-            if (P.motion[r.body2] != JH_MOTION_STATIC) atomicOr(&m1[r.body2], bit);
-            atomicAdd(&P.hist[c], 1);
-        } else {
-            work[0][atomicAdd(&P.wcount[0], 1)] = a;
+        bool uncoloured = false;
+        if (P.colour[a] == -1) {
+            const jh_arbiter& r = P.arb[a];
+            uint32_t c = r.reserved;
+            if (P.keep && c < uint32_t(kMaxColours)) {
+                P.colour[a] = int(c);
+                unsigned long long bit = 1ull << (c & 63);
+                unsigned long long* m1 = c < 64 ? P.mask_lo : P.mask_hi;
+                if (P.motion[r.body1] != JH_MOTION_STATIC) atomicOr(&m1[r.body1], bit);
+                if (P.motion[r.body2] != JH_MOTION_STATIC) atomicOr(&m1[r.body2], bit);
+                atomicAdd(&lhist[c], 1);
+            } else {
+                uncoloured = true;
+            }
         }
+        int slot = wave_append(&P.wcount[0], uncoloured);
+        if (uncoloured) work[0][slot] = a;
     }
     grid_barrier(P.bar);
     for (int round = 0;; ++round) {
@@ -980,8 +1000,9 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
             bool win = (!d1 || bm[r.body1] == prio) && (!d2 || bm[r.body2] == prio);
             bn[r.body1] = 0;
             bn[r.body2] = 0;
+            int slot = wave_append(&P.wcount[nxt], !win);
             if (!win) {
-                work[nxt][atomicAdd(&P.wcount[nxt], 1)] = a;
+                work[nxt][slot] = a;
                 continue;
             }
             unsigned long long lo = 0, hi = 0;
@@ -1000,7 +1021,7 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
                 if (d2) P.mask_hi[r.body2] |= (1ull << (c - 64));
             }
             P.colour[a] = c;
-            atomicAdd(&P.hist[c], 1);
+            atomicAdd(&lhist[c], 1);
         }
         grid_barrier(P.bar);
     }
@@ -1008,6 +1029,9 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
         int c = P.colour[a];
         P.arb[a].reserved = (c >= 0 && c < kMaxColours) ? uint32_t(c) : kNoColour;
     }
+    __syncthreads();
+    if (threadIdx.x < 128 && lhist[threadIdx.x]) atomicAdd(&P.hist[threadIdx.x], lhist[threadIdx.x]);
+    grid_barrier(P.bar);
     if (t0 == 0) {
         int nc = 0;
         for (int c = 0; c < kMaxColours; ++c)
@@ -1571,14 +1595,11 @@ __global__ void k_rank_keys(const jh_arbiter* __restrict__ arb, const uint64_t* 
     int A = offs[128];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
         const jh_arbiter& a = arb[uint32_t(order[k])];
-        if (motion[a.body1] != JH_MOTION_STATIC) {
-            int s = atomicAdd(nkeys, 1);
-            keys[s] = (uint64_t(uint32_t(a.body1)) << 32) | (uint64_t(k) << 1);
-        }
-        if (motion[a.body2] != JH_MOTION_STATIC) {
-            int s = atomicAdd(nkeys, 1);
-            keys[s] = (uint64_t(uint32_t(a.body2)) << 32) | (uint64_t(k) << 1) | 1u;
-        }
+        bool d1 = motion[a.body1] != JH_MOTION_STATIC, d2 = motion[a.body2] != JH_MOTION_STATIC;
+        int s1 = wave_append(nkeys, d1);
+        if (d1) keys[s1] = (uint64_t(uint32_t(a.body1)) << 32) | (uint64_t(k) << 1);
+        int s2 = wave_append(nkeys, d2);
+        if (d2) keys[s2] = (uint64_t(uint32_t(a.body2)) << 32) | (uint64_t(k) << 1) | 1u;
     }
 }
 // After sorting the rank keys: rank = position within the body's segment; deg = segment length.

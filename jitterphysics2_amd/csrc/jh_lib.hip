@@ -302,7 +302,7 @@ jh_status broadphase(jh_world* w) {
         int sb = bits_for(uint32_t(ns - 1));
         for (int s = 0; s < sb; s += 8) pshift[npsh++] = s;
         for (int s = 32; s < 32 + sb; s += 8) pshift[npsh++] = s;
-        HIPCHK(w->hist.ensure(size_t((w->pair_cap + rsort::kTile - 1) / rsort::kTile + 1) * 256));
+        HIPCHK(w->hist.ensure(rsort::hist_entries(size_t(w->pair_cap))));
         w->pairs_in_k1 = rsort::sort_pairs(w->pk0.p, w->pk1.p, nullptr, nullptr, cnt + C_PAIRS, np, pshift, npsh,
                                            w->hist.p, st);
     } else {
@@ -342,7 +342,7 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->flags.ensure(na));
     HIPCHK(w->offs.ensure(na));
     HIPCHK(w->sums.ensure(na / 256 + 2));
-    HIPCHK(w->hist.ensure(size_t((na + rsort::kTile - 1) / rsort::kTile + 1) * 256));
+    HIPCHK(w->hist.ensure(rsort::hist_entries(2 * na)));  // the rank-key sort holds up to 2 keys per arbiter
     return JH_OK;
 }
 
@@ -871,7 +871,7 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
     HIPCHK(w->mk0.ensure(nn)); HIPCHK(w->mk1.ensure(nn)); HIPCHK(w->mv0.ensure(nn)); HIPCHK(w->mv1.ensure(nn));
     HIPCHK(w->left.ensure(nn)); HIPCHK(w->right.ensure(nn)); HIPCHK(w->parent.ensure(nn));
     HIPCHK(w->leaf_parent.ensure(nn)); HIPCHK(w->nflags.ensure(nn)); HIPCHK(w->range.ensure(nn));
-    HIPCHK(w->hist.ensure(size_t((nn + rsort::kTile - 1) / rsort::kTile + 1) * 256));
+    HIPCHK(w->hist.ensure(rsort::hist_entries(size_t(nn))));
     int want = w->cfg.pair_capacity > 0 ? w->cfg.pair_capacity : 24 * n + 1024;
     if (w->pair_cap < want) {
         jh_status s = ensure_pair_buffers(w, want);
