@@ -27,8 +27,9 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(" ".join(cmd))
     # native driver (host C++ only, links the library next to it)
-    cli = ["g++", "-O2", "-std=c++17", "-I/opt/rocm/include", "-o", CLI + ".tmp", os.path.join(CSRC, "jh_cli.cpp"),
-           "-L" + PKG, "-ljitterhip", "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,$ORIGIN:/opt/rocm/lib"]
+    cli = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-o", CLI + ".tmp",
+           os.path.join(CSRC, "jh_cli.cpp"), "-L" + PKG, "-ljitterhip", "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx",
+           "-lamdhip64", "-Wl,-rpath,$ORIGIN:/opt/rocm/lib"]
     r = subprocess.run(cli, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"g++ (jh_cli) failed ({r.returncode}):\n{r.stderr}")

@@ -1,9 +1,13 @@
 // jh_cli: native driver of libjitterhip for profiling and scripted runs.
-//   jh_cli <scene.bin> <warmup> <steps>
+//   jh_cli <scene.bin> <warmup> <steps> [copy_mib]
+// copy_mib > 0 first copies that many MiB device to device (hipMemcpy), a transfer of known size that
+// calibrates HBM traffic counters against the kernels.
 // scene.bin (written by jitterphysics2_amd.scenes.save_scene): magic "JHSC", version 1, then
 // int32 counts (bodies, shapes, vertices), float dt, jh_config, jh_body[], jh_shape[], float[3*nv].
-// Prints the mean per-stage timings (HIP events inside the library) and the wall time of the timed
-// steps, which are bracketed by the roctx range "timed_steps" for kernel-trace profiles.
+// After the warmup the launch shapes are autotuned as in the benchmark.  Prints the mean per-stage
+// timings (HIP events inside the library) and the wall time of the timed steps, which are bracketed
+// by the roctx range "timed_steps" for kernel-trace profiles.
+#include <hip/hip_runtime_api.h>
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <chrono>
@@ -50,6 +54,23 @@ int main(int argc, char** argv) {
     }
     fclose(f);
     int warmup = atoi(argv[2]), steps = atoi(argv[3]);
+    int copy_mib = argc > 4 ? atoi(argv[4]) : 0;
+    if (copy_mib > 0) {
+        size_t bytes = size_t(copy_mib) << 20;
+        void *src = nullptr, *dst = nullptr;
+        if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess) {
+            fprintf(stderr, "copy buffers: hipMalloc failed\n");
+            return 1;
+        }
+        if (hipMemset(src, 1, bytes) != hipSuccess || hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            fprintf(stderr, "copy failed\n");
+            return 1;
+        }
+        printf("# copied %zu bytes device to device\n", bytes);
+        (void)hipFree(src);
+        (void)hipFree(dst);
+    }
     jh_world* w = nullptr;
     jh_status s = jh_world_create(&cfg, &w);
     if (s) return die("jh_world_create", s);
@@ -59,6 +80,10 @@ int main(int argc, char** argv) {
     for (int i = 0; i < warmup; ++i)
         if ((s = jh_world_step(w, dt))) return die("jh_world_step", s);
     printf("# warmup %d steps done\n", warmup);
+    // launch shapes as the benchmark picks them (2 autotune rounds, which advance the simulation)
+    jh_tuning tune;
+    if ((s = jh_world_autotune(w, dt, 2, &tune))) return die("jh_world_autotune", s);
+    printf("# autotune: solve blocks %d, colour blocks %d\n", tune.solve_blocks, tune.colour_blocks);
     fflush(stdout);
     double sum[8] = {0};
     jh_stats st;

@@ -1219,6 +1219,25 @@ __device__ __forceinline__ void arbiter_pass(jh_arbiter& a, BodyReg& B1, BodyReg
     }
 }
 
+// Colour-ordered structure-of-arrays copy of the contact rows for the dataflow solver: position k of
+// the solve order holds arbiter order[k]; field f of slot s lives at f_[(s * kSoaFields + f) * cap + k],
+// so the 64 lanes of a chunk (consecutive positions) read each field with one contiguous access.
+// Gathered from the arbiters after colouring, scattered back (the fields the solve writes) after it.
+constexpr int kSoaFields = 29;  // bias, penalty, normal 3, tangent1 3, tangent2 3, mass 3, acc 3, rel1 3, rel2 3, pos1 3, pos2 3
+enum { kFBias = 0, kFPen = 1, kFN = 2, kFT1 = 5, kFT2 = 8, kFMass = 11, kFAcc = 14, kFRel1 = 17, kFRel2 = 20,
+       kFPos1 = 23, kFPos2 = 26 };
+struct ContactSoA {
+    int2* bodies;    // [cap] body1, body2
+    uint2* um;       // [cap] usage mask, mode
+    float2* fs;      // [cap] friction, speculative relaxation factor
+    uint32_t* flag;  // [4][cap]
+    float* f;        // [4][kSoaFields][cap]
+    int cap;
+    __device__ __forceinline__ float& at(int s, int fld, int k) const {
+        return f[(size_t(s) * kSoaFields + fld) * cap + k];
+    }
+};
+
 struct SolveArgs {
     jh_arbiter* arb;
     const uint64_t* order;   // (colour << 32) | arbiter, sorted
@@ -1234,6 +1253,7 @@ struct SolveArgs {
     int nb;
     StepTime ts;
     int iterations, relaxations;
+    ContactSoA soa;          // dataflow solver: colour-ordered contact rows
 };
 
 __device__ __forceinline__ void integrate_forces(const SolveArgs& P, int t0, int stride) {
@@ -1328,6 +1348,55 @@ __device__ __forceinline__ void coh_store_vw(const Bodies& B, int b, vec3 v, vec
     __hip_atomic_store(pw + 1, pack2(w.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__global__ void __launch_bounds__(256) k_soa_gather(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
+                                                    const int* __restrict__ offs, ContactSoA S) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        const jh_arbiter& a = arb[uint32_t(order[k])];
+        S.bodies[k] = make_int2(a.body1, a.body2);
+        S.um[k] = make_uint2(a.usage_mask, a.mode);
+        S.fs[k] = make_float2(a.friction, a.speculative_relaxation_factor);
+        for (int s = 0; s < 4; ++s) {
+            if (!(a.usage_mask & (1u << s))) continue;
+            const jh_contact& c = a.contacts[s];
+            S.flag[size_t(s) * S.cap + k] = c.flag;
+            S.at(s, kFBias, k) = c.bias;
+            S.at(s, kFPen, k) = c.penalty_bias;
+            for (int l = 0; l < 3; ++l) {
+                S.at(s, kFN + l, k) = c.normal[l];
+                S.at(s, kFT1 + l, k) = c.tangent1[l];
+                S.at(s, kFT2 + l, k) = c.tangent2[l];
+                S.at(s, kFMass + l, k) = c.mass_normal_tangent[l];
+                S.at(s, kFAcc + l, k) = c.accumulated[l];
+                S.at(s, kFRel1 + l, k) = c.relative_position1[l];
+                S.at(s, kFRel2 + l, k) = c.relative_position2[l];
+                S.at(s, kFPos1 + l, k) = c.position1[l];
+                S.at(s, kFPos2 + l, k) = c.position2[l];
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_soa_scatter(jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
+                                                     const int* __restrict__ offs, ContactSoA S) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        jh_arbiter& a = arb[uint32_t(order[k])];
+        uint32_t usage = S.um[k].x;
+        for (int s = 0; s < 4; ++s) {
+            if (!(usage & (1u << s))) continue;
+            jh_contact& c = a.contacts[s];
+            c.flag = S.flag[size_t(s) * S.cap + k];
+            c.bias = S.at(s, kFBias, k);
+            c.penalty_bias = S.at(s, kFPen, k);
+            for (int l = 0; l < 3; ++l) {
+                c.mass_normal_tangent[l] = S.at(s, kFMass + l, k);
+                c.accumulated[l] = S.at(s, kFAcc + l, k);
+            }
+        }
+    }
+}
+
 // Contact fields the solver reads/writes, held in registers for one arbiter visit.
 struct SlotReg {
     uint32_t flag;
@@ -1352,6 +1421,34 @@ __device__ __forceinline__ void load_slot(const jh_contact& c, SlotReg& r, bool 
         r.pos2 = vec3{c.position2[0], c.position2[1], c.position2[2]};
     }
 }
+__device__ __forceinline__ void load_slot_soa(const ContactSoA& S, int s, int k, SlotReg& r, bool prepare) {
+    r.flag = S.flag[size_t(s) * S.cap + k];
+    r.bias = S.at(s, kFBias, k);
+    r.penalty = S.at(s, kFPen, k);
+    for (int l = 0; l < 3; ++l) {
+        r.nx[l] = S.at(s, (l == 0 ? kFN : l == 1 ? kFT1 : kFT2) + 0, k);
+        r.ny[l] = S.at(s, (l == 0 ? kFN : l == 1 ? kFT1 : kFT2) + 1, k);
+        r.nz[l] = S.at(s, (l == 0 ? kFN : l == 1 ? kFT1 : kFT2) + 2, k);
+        r.mass[l] = S.at(s, kFMass + l, k);
+        r.acc[l] = S.at(s, kFAcc + l, k);
+    }
+    r.rel1 = vec3{S.at(s, kFRel1, k), S.at(s, kFRel1 + 1, k), S.at(s, kFRel1 + 2, k)};
+    r.rel2 = vec3{S.at(s, kFRel2, k), S.at(s, kFRel2 + 1, k), S.at(s, kFRel2 + 2, k)};
+    if (prepare) {
+        r.pos1 = vec3{S.at(s, kFPos1, k), S.at(s, kFPos1 + 1, k), S.at(s, kFPos1 + 2, k)};
+        r.pos2 = vec3{S.at(s, kFPos2, k), S.at(s, kFPos2 + 1, k), S.at(s, kFPos2 + 2, k)};
+    }
+}
+__device__ __forceinline__ void store_slot_soa(const ContactSoA& S, int s, int k, const SlotReg& r, bool prepare) {
+    if (prepare) {
+        S.flag[size_t(s) * S.cap + k] = r.flag;
+        S.at(s, kFBias, k) = r.bias;
+        S.at(s, kFPen, k) = r.penalty;
+        for (int l = 0; l < 3; ++l) S.at(s, kFMass + l, k) = r.mass[l];
+    }
+    for (int l = 0; l < 3; ++l) S.at(s, kFAcc + l, k) = r.acc[l];
+}
+
 __device__ __forceinline__ void store_slot(jh_contact& c, const SlotReg& r, bool prepare) {
     if (prepare) {
         c.flag = r.flag;
@@ -1488,10 +1585,13 @@ template <bool PREPARE>
 __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, bool apply_bias, bool active) {
     const Bodies& B = P.B;
     if (!active) k = 0;  // inactive lanes of the wave's last chunk mirror a valid arbiter but do nothing
-    jh_arbiter& a = P.arb[uint32_t(P.order[k])];
-    int b1 = a.body1, b2 = a.body2;
-    uint32_t usage = a.usage_mask, mode = a.mode;
-    float friction = a.friction, specrel = a.speculative_relaxation_factor;
+    const ContactSoA& S = P.soa;
+    int2 bb = S.bodies[k];
+    uint2 um = S.um[k];
+    float2 fs = S.fs[k];
+    int b1 = bb.x, b2 = bb.y;
+    uint32_t usage = um.x, mode = um.y;
+    float friction = fs.x, specrel = fs.y;
     bool m1 = B.motion[b1] != JH_MOTION_STATIC, m2 = B.motion[b2] != JH_MOTION_STATIC;
     int2 r = P.rank[k];
     int need1 = m1 ? g * P.deg[b1] + r.x : 0;
@@ -1499,7 +1599,7 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     SlotReg sr[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s)
-        if (usage & (1u << s)) load_slot(a.contacts[s], sr[s], PREPARE);
+        if (usage & (1u << s)) load_slot_soa(S, s, k, sr[s], PREPARE);
     BodyReg R1, R2;
     load_static_part(B, b1, R1);
     load_static_part(B, b2, R2);
@@ -1515,12 +1615,15 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     if (!m2) { R2.v = ld3(B.vel[b2]); R2.w = ld3(B.angvel[b2]); }
     // Each lane computes and publishes as soon as its own bodies are ready; the wave leaves when all
     // of its lanes have (a lane never waits for an unrelated lane's dependencies).
+    // A satisfied counter is not polled again (a counter only grows); polls back off a little so
+    // that waiting lanes do not flood the coherent memory path.
     bool done = !active;
+    bool ok1 = !m1, ok2 = !m2;
     while (true) {
         if (!done) {
-            int c1 = m1 ? __hip_atomic_load(&P.counter[b1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-            int c2 = m2 ? __hip_atomic_load(&P.counter[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-            if (c1 >= need1 && c2 >= need2) {
+            if (!ok1) ok1 = __hip_atomic_load(&P.counter[b1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need1;
+            if (!ok2) ok2 = __hip_atomic_load(&P.counter[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need2;
+            if (ok1 && ok2) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 if (m1) coh_load_vw(B, b1, R1.v, R1.w);
                 if (m2) coh_load_vw(B, b2, R2.v, R2.w);
@@ -1539,12 +1642,12 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
                 if (m2) __hip_atomic_fetch_add(&P.counter[b2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
-                    if (usage & (1u << s)) store_slot(a.contacts[s], sr[s], PREPARE);
+                    if (usage & (1u << s)) store_slot_soa(S, s, k, sr[s], PREPARE);
                 done = true;
             }
         }
         if (__all(done)) break;
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(2);
     }
 }
 

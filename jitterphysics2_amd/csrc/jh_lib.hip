@@ -178,6 +178,13 @@ struct jh_world {
     DBuf<int> colour, chist, coffs;
     DBuf<unsigned long long> bmax, mlo, mhi;  // bmax: 2 x body capacity
     DBuf<int> cwork;                          // colouring worklists: 2 x arbiter capacity
+    // colour-ordered contact rows of the dataflow solver (ContactSoA), capacity soa_cap arbiters
+    DBuf<int2> soa_bodies;
+    DBuf<uint2> soa_um;
+    DBuf<float2> soa_fs;
+    DBuf<uint32_t> soa_flag;
+    DBuf<float> soa_f;
+    int soa_cap = 0;
     DBuf<uint64_t> ok0, ok1, solve_keys;
     // dataflow solver
     DBuf<uint64_t> rk0, rk1;
@@ -332,6 +339,15 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->akey[1 - w->cur].ensure(na));
     HIPCHK(w->colour.ensure(na));
     HIPCHK(w->cwork.ensure(2 * na));
+    if (w->flow_solver && int(na) > w->soa_cap) {
+        size_t cap = na + na / 4;
+        HIPCHK(w->soa_bodies.ensure(cap));
+        HIPCHK(w->soa_um.ensure(cap));
+        HIPCHK(w->soa_fs.ensure(cap));
+        HIPCHK(w->soa_flag.ensure(4 * cap));
+        HIPCHK(w->soa_f.ensure(4 * kSoaFields * cap));
+        w->soa_cap = int(cap);
+    }
     HIPCHK(w->ok0.ensure(na));
     HIPCHK(w->ok1.ensure(na));
     HIPCHK(w->solve_keys.ensure(na));
@@ -577,6 +593,12 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.ts = ts;
         S.iterations = cfg.solver_iterations;
         S.relaxations = cfg.relaxation_iterations;
+        S.soa.bodies = w->soa_bodies.p;
+        S.soa.um = w->soa_um.p;
+        S.soa.fs = w->soa_fs.p;
+        S.soa.flag = w->soa_flag.p;
+        S.soa.f = w->soa_f.p;
+        S.soa.cap = w->soa_cap;
         if (probe) {
             s = run_probe(w, S, a_cur, amax_, probe);
             if (s != JH_OK) return s;
@@ -587,7 +609,11 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         }
         void* sargs[] = {&S};
         if (w->flow_solver) {
+            int gsoa = grid_for(std::max(amax_, 1));
+            k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
             s = launch_coop((const void*)k_solve_flow, w->flow_blocks, 256, sargs, st);
+            if (s != JH_OK) return s;
+            k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
         } else {
             int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(amax_, w->nb), 256), 256, 1 << 20));
             s = launch_coop((const void*)k_solve, sgrid, 256, sargs, st);
@@ -752,7 +778,8 @@ jh_status jh_world_destroy(jh_world* w) {
     w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
-    w->colour.release(); w->cwork.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
+    w->colour.release(); w->cwork.release(); w->soa_bodies.release(); w->soa_um.release();
+    w->soa_fs.release(); w->soa_flag.release(); w->soa_f.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
     w->bcounter.release(); w->cbeg.release(); w->bar.release();
