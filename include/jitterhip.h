@@ -302,12 +302,13 @@ jh_status jh_world_step_probe(jh_world* world, float dt, jh_probe* probe);
 
 /* ---- Launch-shape autotuner (no counterpart in the reference) --------------------------------
  * Times candidate workgroup counts of the persistent solver (1/16 .. 2x the CU count, clamped to what
- * can be co-resident) and of the colouring kernel (16 .. 512) on real steps, round-robin over the
+ * can be co-resident), with several worlds also the world-local solver (one workgroup per world), and
+ * the workgroup count of the colouring kernel (16 .. 512) on real steps, round-robin over the
  * candidates for `rounds` rounds each, and keeps the fastest.  The world is stepped
  * rounds * candidates times (the simulation advances); results are deterministic whatever the
  * launch shape. */
 typedef struct jh_tuning {
-    int32_t solve_blocks;      /* chosen persistent-solver workgroups  */
+    int32_t solve_blocks;      /* chosen persistent-solver workgroups; 0 = world-local solver */
     int32_t colour_blocks;     /* chosen colouring workgroups          */
     int32_t candidates;
     int32_t steps;             /* steps taken                          */

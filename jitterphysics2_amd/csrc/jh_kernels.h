@@ -1690,6 +1690,158 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
     }
 }
 
+// ---- World-local solver --------------------------------------------------------------------
+// For batches of many independent worlds: one workgroup solves one world at a time (worlds taken from
+// a queue, largest first), walking that world's arbiters colour by colour with a workgroup barrier
+// between colours.  Arbiters of one colour touch disjoint non-static bodies, and each body sees its
+// arbiters in colour order (the global solve order restricted to its world), so the results equal
+// the colour-batch solver's bit for bit.  All accesses of a world come from one workgroup on one CU,
+// so plain loads and stores ordered by the barriers are coherent: no device-scope atomics.
+struct WorldArgs {
+    const int* wbody;     // body indices grouped by world
+    const int* wboff;     // [W + 1]
+    const int* wsched;    // worlds, largest first
+    const uint32_t* woff; // [W * 128 + 1]: start of (world, colour) in the world-ordered arbiter list
+    int W;
+    int* queue;           // next world to take
+};
+
+template <bool PREPARE>
+__device__ __forceinline__ void world_visit(const SolveArgs& P, int p, bool apply_bias) {
+    const Bodies& B = P.B;
+    const ContactSoA& S = P.soa;
+    int2 bb = S.bodies[p];
+    uint2 um = S.um[p];
+    float2 fs = S.fs[p];
+    int b1 = bb.x, b2 = bb.y;
+    uint32_t usage = um.x, mode = um.y;
+    SlotReg sr[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (usage & (1u << s)) load_slot_soa(S, s, p, sr[s], PREPARE);
+    BodyReg R1, R2;
+    load_static_part(B, b1, R1);
+    load_static_part(B, b2, R2);
+    R1.v = ld3(B.vel[b1]); R1.w = ld3(B.angvel[b1]);
+    R2.v = ld3(B.vel[b2]); R2.w = ld3(B.angvel[b2]);
+    vec3 x1, x2;
+    quat q1, q2;
+    if (PREPARE) {
+        x1 = ld3(B.pos[b1]);
+        x2 = ld3(B.pos[b2]);
+        q1 = ldq(B.quat[b1]);
+        q2 = ldq(B.quat[b2]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (!(usage & (1u << s))) continue;
+        if (PREPARE) prepare_reg(mode, fs.y, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
+        else iterate_reg(mode, fs.x, sr[s], R1, R2, apply_bias);
+    }
+    if (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) { B.vel[b1] = st3(R1.v); B.angvel[b1] = st3(R1.w); }
+    if (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) { B.vel[b2] = st3(R2.v); B.angvel[b2] = st3(R2.w); }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (usage & (1u << s)) store_slot_soa(S, s, p, sr[s], PREPARE);
+}
+
+__device__ __forceinline__ void world_pass(const SolveArgs& P, const uint32_t* segs, int phase) {
+    for (int c = 0; c < kMaxColours; ++c) {
+        int beg = int(segs[c]), end = int(segs[c + 1]);
+        if (beg == end) continue;
+        for (int p = beg + int(threadIdx.x); p < end; p += blockDim.x) {
+            if (phase == 0) world_visit<true>(P, p, false);
+            else world_visit<false>(P, p, phase == 1);
+        }
+        __syncthreads();
+    }
+    int beg = int(segs[kMaxColours]), end = int(segs[kMaxColours + 1]);
+    if (beg < end) {  // the serial batch, in solve order
+        if (threadIdx.x == 0)
+            for (int p = beg; p < end; ++p) {
+                if (phase == 0) world_visit<true>(P, p, false);
+                else world_visit<false>(P, p, phase == 1);
+            }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(512) k_solve_worlds(SolveArgs P, WorldArgs Wd) {
+    __shared__ uint32_t segs[kMaxColours + 2];
+    __shared__ int take;
+    const Bodies& B = P.B;
+    while (true) {
+        if (threadIdx.x == 0) take = atomicAdd(Wd.queue, 1);
+        __syncthreads();
+        int q = take;
+        if (q >= Wd.W) break;
+        int w = Wd.wsched[q];
+        for (int c = threadIdx.x; c <= kMaxColours + 1; c += blockDim.x) segs[c] = Wd.woff[size_t(w) * 128 + c];
+        int bo = Wd.wboff[w], be = Wd.wboff[w + 1];
+        __syncthreads();
+        for (int s = 0; s < P.ts.substeps; ++s) {
+            for (int i = bo + int(threadIdx.x); i < be; i += blockDim.x) {  // IntegrateForces
+                int b = Wd.wbody[i];
+                if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
+                B.angvel[b] = st3(ld3(B.angvel[b]) + ld3(B.dangvel[b]));
+                B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
+            }
+            __syncthreads();
+            for (int it = 0; it <= P.iterations; ++it) world_pass(P, segs, it == 0 ? 0 : 1);
+            for (int i = bo + int(threadIdx.x); i < be; i += blockDim.x) {  // IntegrateVelocities
+                int b = Wd.wbody[i];
+                if (B.motion[b] == JH_MOTION_STATIC) continue;
+                vec3 pnew = ld3(B.pos[b]) + ld3(B.vel[b]) * P.ts.substep_dt;
+                B.pos[b] = st3(pnew);
+                quat rq = rotation_quaternion(ld3(B.angvel[b]), P.ts.substep_dt);
+                B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+            }
+            __syncthreads();
+            for (int it = 0; it < P.relaxations; ++it) world_pass(P, segs, 2);
+        }
+        __syncthreads();
+    }
+}
+
+// (world, colour) histogram of the solved arbiters; world of an arbiter = world of its first body.
+__global__ void k_world_hist(const uint64_t* __restrict__ order, const int* __restrict__ offs,
+                             const jh_arbiter* __restrict__ arb, const int* __restrict__ colour,
+                             const int* __restrict__ body_world, uint32_t* __restrict__ hist) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        uint32_t a = uint32_t(order[k]);
+        int c = colour[a];
+        atomicAdd(&hist[size_t(body_world[arb[a].body1]) * 128 + c], 1u);
+    }
+}
+// World-ordered arbiter list; within one (world, colour) the slot order is free (the arbiters are
+// independent) except in the serial batch, which k_world_serial fills in solve order.
+__global__ void k_world_scatter(const uint64_t* __restrict__ order, const int* __restrict__ offs,
+                                const jh_arbiter* __restrict__ arb, const int* __restrict__ colour,
+                                const int* __restrict__ body_world, const uint32_t* __restrict__ woff,
+                                uint32_t* __restrict__ cursor, uint64_t* __restrict__ wlist) {
+    int A = offs[kMaxColours];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        uint32_t a = uint32_t(order[k]);
+        size_t cell = size_t(body_world[arb[a].body1]) * 128 + colour[a];
+        wlist[woff[cell] + atomicAdd(&cursor[cell], 1u)] = a;
+    }
+}
+__global__ void k_world_serial(const uint64_t* __restrict__ order, const int* __restrict__ offs,
+                               const jh_arbiter* __restrict__ arb, const int* __restrict__ body_world,
+                               const uint32_t* __restrict__ woff, uint32_t* __restrict__ cursor,
+                               uint64_t* __restrict__ wlist) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int k = offs[kMaxColours]; k < offs[kMaxColours + 1]; ++k) {
+        uint32_t a = uint32_t(order[k]);
+        size_t cell = size_t(body_world[arb[a].body1]) * 128 + kMaxColours;
+        wlist[woff[cell] + cursor[cell]++] = a;
+    }
+}
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
 // Per-step set-up of the dataflow solver.
 // Rank keys: (body << 32) | (order position << 1 | which) for each non-static body of each solved arbiter.
 __global__ void k_rank_keys(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,

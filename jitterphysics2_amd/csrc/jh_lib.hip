@@ -185,6 +185,12 @@ struct jh_world {
     DBuf<uint32_t> soa_flag;
     DBuf<float> soa_f;
     int soa_cap = 0;
+    // world-local solver (many independent worlds)
+    bool world_mode = false;
+    int n_worlds = 0, world_blocks = 0;
+    DBuf<int> wbody, wboff, wsched, wcells, wqueue;
+    DBuf<uint32_t> whist, woff, wcur, wsums;
+    DBuf<uint64_t> wlist;
     DBuf<uint64_t> ok0, ok1, solve_keys;
     // dataflow solver
     DBuf<uint64_t> rk0, rk1;
@@ -558,7 +564,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p, w->cbeg.p, cnt + C_NCHUNK);
         const uint64_t* order = in1 ? w->ok1.p : w->ok0.p;
         k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
-        if (w->flow_solver) {
+        if (w->flow_solver && !w->world_mode) {
             // per-body update sequence: rank of each arbiter visit and body degree
             int ga = grid_for(std::max(amax_, 1));
             k_set_int<<<1, 1, 0, st>>>(cnt + C_NRANK, 0);
@@ -608,7 +614,31 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             HIPCHK(hipGetLastError());
         }
         void* sargs[] = {&S};
-        if (w->flow_solver) {
+        if (w->flow_solver && w->world_mode) {
+            // world-ordered arbiter list: (world, colour) histogram -> offsets -> scatter
+            int gsoa = grid_for(std::max(amax_, 1));
+            size_t cells = size_t(w->n_worlds) * 128 + 1;
+            HIPCHK(w->wlist.ensure(std::max(amax_, 1)));
+            k_fill_u32<<<grid_for(int(std::min<size_t>(cells, 1 << 30))), 256, 0, st>>>(w->whist.p, cells, 0u);
+            k_fill_u32<<<grid_for(int(std::min<size_t>(cells, 1 << 30))), 256, 0, st>>>(w->wcur.p, cells, 0u);
+            k_world_hist<<<gsoa, 256, 0, st>>>(order, w->coffs.p, S.arb, w->colour.p, w->world.p, w->whist.p);
+            scan::exclusive_scan(w->whist.p, w->woff.p, w->wcells.p, int(cells), w->wsums.p, nullptr, st);
+            k_world_scatter<<<gsoa, 256, 0, st>>>(order, w->coffs.p, S.arb, w->colour.p, w->world.p, w->woff.p,
+                                                  w->wcur.p, w->wlist.p);
+            k_world_serial<<<1, 64, 0, st>>>(order, w->coffs.p, S.arb, w->world.p, w->woff.p, w->wcur.p, w->wlist.p);
+            k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, w->wlist.p, w->coffs.p, S.soa);
+            k_set_int<<<1, 1, 0, st>>>(w->wqueue.p, 0);
+            WorldArgs Wd;
+            Wd.wbody = w->wbody.p;
+            Wd.wboff = w->wboff.p;
+            Wd.wsched = w->wsched.p;
+            Wd.woff = w->woff.p;
+            Wd.W = w->n_worlds;
+            Wd.queue = w->wqueue.p;
+            k_solve_worlds<<<std::max(1, std::min(w->world_blocks, w->n_worlds)), 512, 0, st>>>(S, Wd);
+            k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, w->wlist.p, w->coffs.p, S.soa);
+            HIPCHK(hipGetLastError());
+        } else if (w->flow_solver) {
             int gsoa = grid_for(std::max(amax_, 1));
             k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
             s = launch_coop((const void*)k_solve_flow, w->flow_blocks, 256, sargs, st);
@@ -738,6 +768,8 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     w->coop_colour = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve, 256, 0));
     w->coop_solve = std::max(1, occ) * w->cu_count;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_worlds, 512, 0));
+    w->world_blocks = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_epa_wave, 64, 0));
     w->epa_wave_blocks = std::max(1, occ) * w->cu_count;
     if (const char* e = getenv("JH_EPA")) w->epa_wave = strcmp(e, "thread") != 0;
@@ -779,7 +811,9 @@ jh_status jh_world_destroy(jh_world* w) {
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
     w->colour.release(); w->cwork.release(); w->soa_bodies.release(); w->soa_um.release();
-    w->soa_fs.release(); w->soa_flag.release(); w->soa_f.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
+    w->soa_fs.release(); w->soa_flag.release(); w->soa_f.release();
+    w->wbody.release(); w->wboff.release(); w->wsched.release(); w->wcells.release(); w->wqueue.release();
+    w->whist.release(); w->woff.release(); w->wcur.release(); w->wsums.release(); w->wlist.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
     w->bcounter.release(); w->cbeg.release(); w->bar.release();
@@ -815,6 +849,38 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     w->nb = n;
     w->max_world = maxw;
     w->host_bodies.assign(bodies, bodies + n);
+    {
+        // bodies grouped by world (counting sort), worlds scheduled largest first
+        int W = n > 0 ? maxw + 1 : 0;
+        std::vector<int> off(W + 1, 0), list(std::max(n, 1)), sched(std::max(W, 1));
+        for (int i = 0; i < n; ++i) off[bodies[i].world_id + 1]++;
+        for (int k = 0; k < W; ++k) off[k + 1] += off[k];
+        std::vector<int> fill(off.begin(), off.end() - (W > 0 ? 1 : 0));
+        for (int i = 0; i < n; ++i) list[fill[bodies[i].world_id]++] = i;
+        for (int k = 0; k < W; ++k) sched[k] = k;
+        std::stable_sort(sched.begin(), sched.begin() + W,
+                         [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+        w->n_worlds = W;
+        const char* e = getenv("JH_SOLVER");
+        w->world_mode = w->flow_solver && (e ? strcmp(e, "world") == 0 : W >= 64);
+        if (W > 0) {
+            HIPCHK(w->wbody.ensure(std::max(n, 1)));
+            HIPCHK(w->wboff.ensure(W + 1));
+            HIPCHK(w->wsched.ensure(W));
+            HIPCHK(w->wcells.ensure(1));
+            HIPCHK(w->wqueue.ensure(1));
+            size_t cells = size_t(W) * 128 + 1;
+            HIPCHK(w->whist.ensure(cells));
+            HIPCHK(w->woff.ensure(cells));
+            HIPCHK(w->wcur.ensure(cells));
+            HIPCHK(w->wsums.ensure(cells / 256 + 2));
+            HIPCHK(hipMemcpyAsync(w->wbody.p, list.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice, w->st));
+            HIPCHK(hipMemcpyAsync(w->wboff.p, off.data(), size_t(W + 1) * sizeof(int), hipMemcpyHostToDevice, w->st));
+            HIPCHK(hipMemcpyAsync(w->wsched.p, sched.data(), size_t(W) * sizeof(int), hipMemcpyHostToDevice, w->st));
+            k_set_int<<<1, 1, 0, w->st>>>(w->wcells.p, int(cells));
+            HIPCHK(hipStreamSynchronize(w->st));
+        }
+    }
     size_t nn = std::max(n, 1);
     DBuf<float4>* f4[] = {&w->pos, &w->quat, &w->vel, &w->angvel, &w->dvel, &w->dangvel, &w->imv,
                           &w->iw0, &w->il0, &w->mat, &w->mat2, &w->force, &w->torque};
@@ -1436,13 +1502,41 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
             if (cand[i] == def) default_ms = tot[i] / rounds;
         return JH_OK;
     };
-    jh_status s = tune(sc, solve_knob, true, out->ms_solve_best, out->ms_solve_default, solve_default);
-    if (s != JH_OK) return s;
+    // many worlds: the world-local solver is one more candidate (encoded as solve shape 0)
+    bool world_default = w->world_mode;
+    if (w->flow_solver && w->n_worlds > 1) sc.insert(sc.begin(), 0);
+    std::vector<int> flow_shapes = sc;
+    // candidates: 0 = world-local solver, > 0 = dataflow solver with that many workgroups
+    auto apply = [&](int v) {
+        w->world_mode = v == 0;
+        if (v > 0) solve_knob = v;
+    };
+    int keep_solve = solve_knob;
+    {
+        std::vector<double> tot(flow_shapes.size(), 0.0);
+        for (int r = 0; r < rounds; ++r)
+            for (size_t i = 0; i < flow_shapes.size(); ++i) {
+                apply(flow_shapes[i]);
+                jh_status st = do_step(w, dt, nullptr);
+                if (st != JH_OK) return st;
+                tot[i] += w->stats.ms_solve;
+            }
+        size_t bi = 0;
+        for (size_t i = 1; i < flow_shapes.size(); ++i)
+            if (tot[i] < tot[bi]) bi = i;
+        solve_knob = keep_solve;
+        apply(flow_shapes[bi]);
+        out->ms_solve_best = tot[bi] / rounds;
+        out->ms_solve_default = out->ms_solve_best;
+        for (size_t i = 0; i < flow_shapes.size(); ++i)
+            if (flow_shapes[i] == (world_default ? 0 : solve_default)) out->ms_solve_default = tot[i] / rounds;
+    }
+    jh_status s = JH_OK;
     s = tune(cc, w->colour_blocks, false, out->ms_colour_best, out->ms_colour_default, colour_default);
     if (s != JH_OK) return s;
-    out->solve_blocks = solve_knob;
+    out->solve_blocks = w->world_mode ? 0 : solve_knob;  // 0: world-local solver
     out->colour_blocks = w->colour_blocks;
-    out->candidates = int(sc.size() + cc.size());
+    out->candidates = int(flow_shapes.size() + cc.size());
     out->steps = rounds * out->candidates;
     return JH_OK;
 }

@@ -199,3 +199,10 @@ def test_fp32_error_against_fp64_oracle():
     dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10]).max()
     print(f"FP32 GPU vs FP64 oracle after 30 steps: max |dp| {dp:.3e} m, max |dv| {dv:.3e} m/s")
     assert dp < 1e-4 and dv < 1e-3
+
+
+def test_world_solver_bitexact(monkeypatch):
+    # one workgroup per world, colour by colour: same per-body order as the colour batches
+    monkeypatch.setenv("JH_SOLVER", "world")
+    run_replay(scenes.batched_worlds(n_worlds=24, min_bodies=16, max_bodies=160, seed=13), steps=40, check_every=13)
+    run_replay(scenes.full_form(n_bodies=200, cells=32, worlds=4, feature=8.0, pitch=2.4), steps=60, check_every=20)
