@@ -1254,6 +1254,7 @@ struct SolveArgs {
     StepTime ts;
     int iterations, relaxations;
     ContactSoA soa;          // dataflow solver: colour-ordered contact rows
+    float4* vw;              // dataflow solver: per body (v.x v.y v.z w.x)(w.y w.z - -), 32 bytes
 };
 
 __device__ __forceinline__ void integrate_forces(const SolveArgs& P, int t0, int stride) {
@@ -1395,6 +1396,37 @@ __global__ void __launch_bounds__(256) k_soa_scatter(jh_arbiter* __restrict__ ar
             }
         }
     }
+}
+
+// Packed velocities of the dataflow solver: body b's 32-byte record is read and written with two
+// 16-byte buffer accesses carrying the same coherence bit (sc1) as the compiler's agent-scope atomic
+// loads and stores, instead of four 8-byte atomics.
+constexpr int kBufferDword3 = 0x00020000;  // raw buffer resource word 3 for gfx9-family targets
+constexpr int kCohSc1 = 16;                // cache-policy operand: sc1
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vw_rsrc(const SolveArgs& P) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)P.vw, 0, P.nb * 32, kBufferDword3);
+}
+__device__ __forceinline__ void coh_load_vw2(__amdgpu_buffer_rsrc_t r, int b, vec3& v, vec3& w) {
+    float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32, 0, kCohSc1));
+    float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32 + 16, 0, kCohSc1));
+    v = {a.x, a.y, a.z};
+    w = {a.w, c.x, c.y};
+}
+__device__ __forceinline__ void coh_store_vw2(__amdgpu_buffer_rsrc_t r, int b, vec3 v, vec3 w) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(v.x, v.y, v.z, w.x)), r, b * 32, 0,
+                                           kCohSc1);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(w.y, w.z, 0.0f, 0.0f)), r,
+                                           b * 32 + 16, 0, kCohSc1);
+}
+__device__ __forceinline__ void ld_vw(const SolveArgs& P, int b, vec3& v, vec3& w) {
+    float4 a = P.vw[2 * b], c = P.vw[2 * b + 1];
+    v = {a.x, a.y, a.z};
+    w = {a.w, c.x, c.y};
+}
+__device__ __forceinline__ void st_vw(const SolveArgs& P, int b, vec3 v, vec3 w) {
+    P.vw[2 * b] = make_float4(v.x, v.y, v.z, w.x);
+    P.vw[2 * b + 1] = make_float4(w.y, w.z, 0.0f, 0.0f);
 }
 
 // Contact fields the solver reads/writes, held in registers for one arbiter visit.
@@ -1611,8 +1643,9 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
         q1 = ldq(B.quat[b1]);
         q2 = ldq(B.quat[b2]);
     }
-    if (!m1) { R1.v = ld3(B.vel[b1]); R1.w = ld3(B.angvel[b1]); }
-    if (!m2) { R2.v = ld3(B.vel[b2]); R2.w = ld3(B.angvel[b2]); }
+    if (!m1) ld_vw(P, b1, R1.v, R1.w);
+    if (!m2) ld_vw(P, b2, R2.v, R2.w);
+    __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
     // Each lane computes and publishes as soon as its own bodies are ready; the wave leaves when all
     // of its lanes have (a lane never waits for an unrelated lane's dependencies).
     // A satisfied counter is not polled again (a counter only grows); polls back off a little so
@@ -1625,16 +1658,16 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
             if (!ok2) ok2 = __hip_atomic_load(&P.counter[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need2;
             if (ok1 && ok2) {
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                if (m1) coh_load_vw(B, b1, R1.v, R1.w);
-                if (m2) coh_load_vw(B, b2, R2.v, R2.w);
+                if (m1) coh_load_vw2(vr, b1, R1.v, R1.w);
+                if (m2) coh_load_vw2(vr, b2, R2.v, R2.w);
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     if (!(usage & (1u << s))) continue;
                     if (PREPARE) prepare_reg(mode, specrel, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
                     else iterate_reg(mode, friction, sr[s], R1, R2, apply_bias);
                 }
-                if (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) coh_store_vw(B, b1, R1.v, R1.w);
-                if (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) coh_store_vw(B, b2, R2.v, R2.w);
+                if (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) coh_store_vw2(vr, b1, R1.v, R1.w);
+                if (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) coh_store_vw2(vr, b2, R2.v, R2.w);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
                 __builtin_amdgcn_s_waitcnt(0);
                 __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -1676,17 +1709,40 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     int per_sub = 1 + P.iterations + P.relaxations;
+    const Bodies& B = P.B;
+    for (int b = t0; b < P.nb; b += stride) st_vw(P, b, ld3(B.vel[b]), ld3(B.angvel[b]));
     for (int s = 0; s < P.ts.substeps; ++s) {
-        integrate_forces(P, t0, stride);
+        for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
+            if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
+            vec3 v, w;
+            ld_vw(P, b, v, w);
+            w = w + ld3(B.dangvel[b]);
+            v = v + ld3(B.dvel[b]);
+            st_vw(P, b, v, w);
+        }
         grid_barrier(P.bar);
         flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
         grid_barrier(P.bar);
-        integrate_velocities(P, t0, stride);
+        for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
+            if (B.motion[b] == JH_MOTION_STATIC) continue;
+            vec3 v, w;
+            ld_vw(P, b, v, w);
+            vec3 p = ld3(B.pos[b]) + v * P.ts.substep_dt;
+            B.pos[b] = st3(p);
+            quat rq = rotation_quaternion(w, P.ts.substep_dt);
+            B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+        }
         grid_barrier(P.bar);
         if (P.relaxations > 0) {
             flow_passes(P, s * per_sub + 1 + P.iterations, P.relaxations, 2, 2);
             grid_barrier(P.bar);
         }
+    }
+    for (int b = t0; b < P.nb; b += stride) {
+        vec3 v, w;
+        ld_vw(P, b, v, w);
+        B.vel[b] = st3(v);
+        B.angvel[b] = st3(w);
     }
 }
 

@@ -191,6 +191,7 @@ struct jh_world {
     DBuf<int> wbody, wboff, wsched, wcells, wqueue;
     DBuf<uint32_t> whist, woff, wcur, wsums;
     DBuf<uint64_t> wlist;
+    DBuf<float4> vwpack;  // dataflow solver: packed velocities, 2 per body
     DBuf<uint64_t> ok0, ok1, solve_keys;
     // dataflow solver
     DBuf<uint64_t> rk0, rk1;
@@ -605,6 +606,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.soa.flag = w->soa_flag.p;
         S.soa.f = w->soa_f.p;
         S.soa.cap = w->soa_cap;
+        S.vw = w->vwpack.p;
         if (probe) {
             s = run_probe(w, S, a_cur, amax_, probe);
             if (s != JH_OK) return s;
@@ -813,7 +815,8 @@ jh_status jh_world_destroy(jh_world* w) {
     w->colour.release(); w->cwork.release(); w->soa_bodies.release(); w->soa_um.release();
     w->soa_fs.release(); w->soa_flag.release(); w->soa_f.release();
     w->wbody.release(); w->wboff.release(); w->wsched.release(); w->wcells.release(); w->wqueue.release();
-    w->whist.release(); w->woff.release(); w->wcur.release(); w->wsums.release(); w->wlist.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
+    w->whist.release(); w->woff.release(); w->wcur.release(); w->wsums.release(); w->wlist.release();
+    w->vwpack.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
     w->bcounter.release(); w->cbeg.release(); w->bar.release();
@@ -896,6 +899,7 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     HIPCHK(w->mhi.ensure(nn));
     HIPCHK(w->deg.ensure(nn));
     HIPCHK(w->bcounter.ensure(nn));
+    HIPCHK(w->vwpack.ensure(2 * nn));
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(w->body_io.p, bodies, n * sizeof(jh_body), hipMemcpyHostToDevice, w->st));
         k_unpack_bodies<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(w->body_io.p, n, w->bodies());
