@@ -7,9 +7,11 @@ configuration scaled to 131,072 bodies, the body count the metric is quoted at).
 the heap form contacts; then K steps are timed with inputs resident in HBM.
 
 Besides the JSON line's contract fields it reports:
-  roofline     — the persistent solver kernel (k_solve), whose duration is measured with HIP events
-                 on the library's stream (jh_stats.ms_solve); algorithmic bytes per launch from the
-                 per-step counts (see solver_bytes()).
+  roofline     — the solver kernel (k_solve_flow, the persistent dataflow solver, or k_solve_worlds,
+                 one workgroup per world, whichever the autotuner picked), whose duration is measured
+                 with HIP events on the library's stream (jh_stats.ms_solve); algorithmic bytes per
+                 launch from the per-step counts (see solver_bytes()); traffic from the committed
+                 counter summary when this is the profiled run (see measured_traffic()).
   cpu_baseline — the oracle (CPU restatement, 1 thread) timed on a bounded sample: a few steps of the
                  same heap continued from the GPU state at the start of the timed region.
   learned_warm_start — residual probes of the settled heap (see learned_warm_start_leg).
@@ -56,7 +58,7 @@ class _Roctx:
 
 
 def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: int, relax: int) -> float:
-    """Algorithmic HBM bytes of one k_solve launch (one step, all substeps).
+    """Algorithmic HBM bytes of one solver launch (one step, all substeps).
     Per arbiter per pass (prepare, each iteration, each relaxation): 152 B of body traffic
     (2 x (v, w read + write 48 B + inverse mass/inertia 28 B)); per contact point per pass 116 B
     (row data 92 B read, accumulated impulse 12 B read + 12 B write).  Per substep:
@@ -65,6 +67,22 @@ def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: i
     passes = 1 + iters + relax
     per_substep = passes * (152.0 * m + 116.0 * k) + 72.0 * n_dyn + 80.0 * n_moving
     return substeps * per_substep
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solve_flow_heap131072_traffic.json")
+
+
+def measured_traffic(kernel: str, args) -> float | None:
+    """HBM bytes per launch of the solver kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by what it reads for a device-to-device copy of known
+    size).  That summary profiles the default run (131,072-body heap, 240 warmup steps, autotune,
+    60 timed steps, dataflow solver); any other run reports None."""
+    if (kernel != "k_solve_flow" or args.workload != "heap" or args.bodies != 131072 or args.warmup != 240
+            or args.steps != 60 or not os.path.exists(TRAFFIC_PROFILE)):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        prof = json.load(f)
+    return float(prof["traffic_bytes_per_launch"]) if prof.get("kernel") == kernel else None
 
 
 def learned_warm_start_leg(gw, dt: float, cfg, settle_to: int, probe_iterations: int, timed: int) -> dict:
@@ -206,6 +224,10 @@ def main() -> int:
                "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body heap continued from the GPU state at "
                          f"the start of the timed region (oracle FP32, key-order solve, 1 thread)"}
 
+    world_solver = tuning is not None and tuning["solve_blocks"] == 0
+    kernel = "k_solve_worlds" if world_solver else "k_solve_flow"
+    traffic = measured_traffic(kernel, args)
+
     line = {
         "metric": "body-steps/sec at 100k bodies; iters to 1e-3 residual, learned vs plain warm start",
         "value": value,
@@ -235,8 +257,8 @@ def main() -> int:
                    "contacts_per_s": float(np.mean(contacts)) * args.steps / wall,
                    "stage_ms": {k: final[k] for k in ("ms_broadphase", "ms_narrowphase", "ms_contact_cache",
                                                       "ms_colouring", "ms_solve", "ms_update", "ms_epa")}},
-        "roofline": {"bound": "hbm", "kernel": "k_solve", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_ms": 1000.0 * avg_solve_s,
                      "algorithmic_bytes_per_launch": float(np.mean(bytes_per_launch))},
         "cpu_baseline": cpu,
