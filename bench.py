@@ -7,8 +7,9 @@ configuration scaled to 131,072 bodies, the body count the metric is quoted at).
 the heap form contacts; then K steps are timed with inputs resident in HBM.
 
 Besides the JSON line's contract fields it reports:
-  roofline     — the solver kernel (k_solve_flow, the persistent dataflow solver, or k_solve_worlds,
-                 one workgroup per world, whichever the autotuner picked), whose duration is measured
+  roofline     — the solver kernel (k_solve_flow, the persistent dataflow solver; k_solve_tiles, one
+                 workgroup per Morton range of bodies; or k_solve_worlds, one workgroup per world,
+                 whichever the autotuner picked), whose duration is measured
                  with HIP events on the library's stream (jh_stats.ms_solve); algorithmic bytes per
                  launch from the per-step counts (see solver_bytes()); traffic from the committed
                  counter summary when this is the profiled run (see measured_traffic()).
@@ -224,8 +225,7 @@ def main() -> int:
                "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body heap continued from the GPU state at "
                          f"the start of the timed region (oracle FP32, key-order solve, 1 thread)"}
 
-    world_solver = tuning is not None and tuning["solve_blocks"] == 0
-    kernel = "k_solve_worlds" if world_solver else "k_solve_flow"
+    kernel = {0: "k_solve", 1: "k_solve_flow", 2: "k_solve_worlds", 3: "k_solve_tiles"}[int(final["solver"])]
     traffic = measured_traffic(kernel, args)
 
     line = {

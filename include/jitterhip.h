@@ -176,6 +176,8 @@ typedef struct jh_stats {
     double ms_broadphase, ms_narrowphase, ms_contact_cache, ms_colouring, ms_solve, ms_update;
     double ms_step;
     double ms_epa;            /* part of ms_narrowphase spent in the EPA kernel */
+    int64_t solver;           /* solver kernel of the last step: 0 colour batches with grid barriers,
+                                 1 dataflow, 2 world-local, 3 tiles */
 } jh_stats;
 
 typedef struct jh_world jh_world;
@@ -308,7 +310,8 @@ jh_status jh_world_step_probe(jh_world* world, float dt, jh_probe* probe);
  * rounds * candidates times (the simulation advances); results are deterministic whatever the
  * launch shape. */
 typedef struct jh_tuning {
-    int32_t solve_blocks;      /* chosen persistent-solver workgroups; 0 = world-local solver */
+    int32_t solve_blocks;      /* chosen solver: > 0 dataflow-solver workgroups; 0 world-local solver;
+                                  < 0 tile solver with -solve_blocks tiles */
     int32_t colour_blocks;     /* chosen colouring workgroups          */
     int32_t candidates;
     int32_t steps;             /* steps taken                          */

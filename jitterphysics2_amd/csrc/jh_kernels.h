@@ -1247,14 +1247,13 @@ struct SolveArgs {
     const int* nchunks;
     const int2* rank;        // per order position: rank of the arbiter in each body's update sequence
     const int* deg;          // per body: arbiters touching it
-    int* counter;            // per body: updates applied in this step
     unsigned* bar;           // grid barrier state
     Bodies B;
     int nb;
     StepTime ts;
     int iterations, relaxations;
     ContactSoA soa;          // dataflow solver: colour-ordered contact rows
-    float4* vw;              // dataflow solver: per body (v.x v.y v.z w.x)(w.y w.z - -), 32 bytes
+    float4* vw;              // dataflow solvers: per body (v.x v.y v.z n)(w.x w.y w.z n), 32 bytes
 };
 
 __device__ __forceinline__ void integrate_forces(const SolveArgs& P, int t0, int stride) {
@@ -1322,33 +1321,10 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
 // position k waits until each of its bodies has seen g * deg(b) + rank(k, b) visits: exactly the
 // visits that precede it in the colour-ordered Gauss-Seidel sequence.  Each body therefore sees its
 // updates in the same order as the barrier solver (bit-identical results) without grid barriers
-// between colours.  Shared velocities move through agent-coherent (sc1) 64-bit atomics; a completed
-// store is visible device-wide before the counter increment that publishes it (s_waitcnt).
+// between colours.  The count travels inside the body's packed velocity record (see coh_poll_vw).
 // Waves own chunks statically (chunk j -> wave j mod W, for every pass), processed in global order,
 // so the earliest unfinished chunk always has its dependencies met: the loop cannot deadlock on a
 // cooperative (fully resident) grid.  Arbiter contact data stays with its owning wave.
-__device__ __forceinline__ void coh_load_vw(const Bodies& B, int b, vec3& v, vec3& w) {
-    const unsigned long long* pv = (const unsigned long long*)&B.vel[b];
-    const unsigned long long* pw = (const unsigned long long*)&B.angvel[b];
-    unsigned long long v0 = __hip_atomic_load(pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long v1 = __hip_atomic_load(pv + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long w0 = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long w1 = __hip_atomic_load(pw + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    v = {__uint_as_float(uint32_t(v0)), __uint_as_float(uint32_t(v0 >> 32)), __uint_as_float(uint32_t(v1))};
-    w = {__uint_as_float(uint32_t(w0)), __uint_as_float(uint32_t(w0 >> 32)), __uint_as_float(uint32_t(w1))};
-}
-__device__ __forceinline__ unsigned long long pack2(float a, float b) {
-    return (unsigned long long)__float_as_uint(a) | ((unsigned long long)__float_as_uint(b) << 32);
-}
-__device__ __forceinline__ void coh_store_vw(const Bodies& B, int b, vec3 v, vec3 w) {
-    unsigned long long* pv = (unsigned long long*)&B.vel[b];
-    unsigned long long* pw = (unsigned long long*)&B.angvel[b];
-    __hip_atomic_store(pv, pack2(v.x, v.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(pv + 1, pack2(v.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(pw, pack2(w.x, w.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(pw + 1, pack2(w.z, 0.0f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __global__ void __launch_bounds__(256) k_soa_gather(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
                                                     const int* __restrict__ offs, ContactSoA S) {
     int A = offs[128];
@@ -1398,35 +1374,46 @@ __global__ void __launch_bounds__(256) k_soa_scatter(jh_arbiter* __restrict__ ar
     }
 }
 
-// Packed velocities of the dataflow solver: body b's 32-byte record is read and written with two
-// 16-byte buffer accesses carrying the same coherence bit (sc1) as the compiler's agent-scope atomic
-// loads and stores, instead of four 8-byte atomics.
+// Packed velocity record of the dataflow solvers, 32 bytes per body:
+//   (v.x v.y v.z n)(w.x w.y w.z n),  n = visits applied to the body so far in this launch,
+// the count repeated in both 16-byte halves.  A visit is published by storing both halves with the
+// new count; a waiting visit takes the record as soon as both halves carry the count it waits for,
+// so one pair of loads is both the readiness poll and the data, and publishing needs no separate
+// counter update nor a wait for the data stores.  Each half is one aligned 16-byte buffer access
+// (single-copy atomic within a cache line), so a half with the expected count holds that visit's
+// values.  The accesses carry the sc1 bit (device coherence: served past the L2 of the issuing
+// XCD), like the compiler's agent-scope atomics.
 constexpr int kBufferDword3 = 0x00020000;  // raw buffer resource word 3 for gfx9-family targets
 constexpr int kCohSc1 = 16;                // cache-policy operand: sc1
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t vw_rsrc(const SolveArgs& P) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)P.vw, 0, P.nb * 32, kBufferDword3);
 }
-__device__ __forceinline__ void coh_load_vw2(__amdgpu_buffer_rsrc_t r, int b, vec3& v, vec3& w) {
+// True (and v, w set) when body b's record shows exactly `need` visits.
+__device__ __forceinline__ bool coh_poll_vw(__amdgpu_buffer_rsrc_t r, int b, int need, vec3& v, vec3& w) {
     float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32, 0, kCohSc1));
     float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32 + 16, 0, kCohSc1));
     v = {a.x, a.y, a.z};
-    w = {a.w, c.x, c.y};
+    w = {c.x, c.y, c.z};
+    return __float_as_int(a.w) == need && __float_as_int(c.w) == need;
 }
-__device__ __forceinline__ void coh_store_vw2(__amdgpu_buffer_rsrc_t r, int b, vec3 v, vec3 w) {
+__device__ __forceinline__ void coh_publish_vw(__amdgpu_buffer_rsrc_t r, int b, vec3 v, vec3 w, int n) {
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(v.x, v.y, v.z, w.x)), r, b * 32, 0,
+    float fn = __int_as_float(n);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(v.x, v.y, v.z, fn)), r, b * 32, 0,
                                            kCohSc1);
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(w.y, w.z, 0.0f, 0.0f)), r,
-                                           b * 32 + 16, 0, kCohSc1);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(w.x, w.y, w.z, fn)), r, b * 32 + 16,
+                                           0, kCohSc1);
 }
-__device__ __forceinline__ void ld_vw(const SolveArgs& P, int b, vec3& v, vec3& w) {
+// Plain accesses, for phases separated from the dataflow passes by grid barriers.
+__device__ __forceinline__ int ld_vw(const SolveArgs& P, int b, vec3& v, vec3& w) {
     float4 a = P.vw[2 * b], c = P.vw[2 * b + 1];
     v = {a.x, a.y, a.z};
-    w = {a.w, c.x, c.y};
+    w = {c.x, c.y, c.z};
+    return __float_as_int(a.w);
 }
-__device__ __forceinline__ void st_vw(const SolveArgs& P, int b, vec3 v, vec3 w) {
-    P.vw[2 * b] = make_float4(v.x, v.y, v.z, w.x);
-    P.vw[2 * b + 1] = make_float4(w.y, w.z, 0.0f, 0.0f);
+__device__ __forceinline__ void st_vw(const SolveArgs& P, int b, vec3 v, vec3 w, int n) {
+    P.vw[2 * b] = make_float4(v.x, v.y, v.z, __int_as_float(n));
+    P.vw[2 * b + 1] = make_float4(w.x, w.y, w.z, __int_as_float(n));
 }
 
 // Contact fields the solver reads/writes, held in registers for one arbiter visit.
@@ -1647,32 +1634,25 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     if (!m2) ld_vw(P, b2, R2.v, R2.w);
     __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
     // Each lane computes and publishes as soon as its own bodies are ready; the wave leaves when all
-    // of its lanes have (a lane never waits for an unrelated lane's dependencies).
-    // A satisfied counter is not polled again (a counter only grows); polls back off a little so
-    // that waiting lanes do not flood the coherent memory path.
+    // of its lanes have (a lane never waits for an unrelated lane's dependencies).  A record that
+    // showed the awaited count is not polled again (no other visit can change it before this one);
+    // polls back off a little so that waiting lanes do not flood the coherent memory path.
     bool done = !active;
     bool ok1 = !m1, ok2 = !m2;
     while (true) {
+        asm volatile("" ::: "memory");
         if (!done) {
-            if (!ok1) ok1 = __hip_atomic_load(&P.counter[b1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need1;
-            if (!ok2) ok2 = __hip_atomic_load(&P.counter[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need2;
+            if (!ok1) ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w);
+            if (!ok2) ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w);
             if (ok1 && ok2) {
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                if (m1) coh_load_vw2(vr, b1, R1.v, R1.w);
-                if (m2) coh_load_vw2(vr, b2, R2.v, R2.w);
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     if (!(usage & (1u << s))) continue;
                     if (PREPARE) prepare_reg(mode, specrel, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
                     else iterate_reg(mode, friction, sr[s], R1, R2, apply_bias);
                 }
-                if (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) coh_store_vw2(vr, b1, R1.v, R1.w);
-                if (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) coh_store_vw2(vr, b2, R2.v, R2.w);
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                __builtin_amdgcn_s_waitcnt(0);
-                __atomic_signal_fence(__ATOMIC_SEQ_CST);
-                if (m1) __hip_atomic_fetch_add(&P.counter[b1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (m2) __hip_atomic_fetch_add(&P.counter[b2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (m1) coh_publish_vw(vr, b1, R1.v, R1.w, need1 + 1);
+                if (m2) coh_publish_vw(vr, b2, R2.v, R2.w, need2 + 1);
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
                     if (usage & (1u << s)) store_slot_soa(S, s, k, sr[s], PREPARE);
@@ -1710,15 +1690,15 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     int per_sub = 1 + P.iterations + P.relaxations;
     const Bodies& B = P.B;
-    for (int b = t0; b < P.nb; b += stride) st_vw(P, b, ld3(B.vel[b]), ld3(B.angvel[b]));
+    for (int b = t0; b < P.nb; b += stride) st_vw(P, b, ld3(B.vel[b]), ld3(B.angvel[b]), 0);
     for (int s = 0; s < P.ts.substeps; ++s) {
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
             if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
             vec3 v, w;
-            ld_vw(P, b, v, w);
+            int n = ld_vw(P, b, v, w);
             w = w + ld3(B.dangvel[b]);
             v = v + ld3(B.dvel[b]);
-            st_vw(P, b, v, w);
+            st_vw(P, b, v, w, n);
         }
         grid_barrier(P.bar);
         flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
@@ -1915,7 +1895,7 @@ __global__ void k_rank_keys(const jh_arbiter* __restrict__ arb, const uint64_t* 
 }
 // After sorting the rank keys: rank = position within the body's segment; deg = segment length.
 __global__ void k_rank_assign(const uint64_t* __restrict__ keys, const int* __restrict__ nkeys,
-                              int2* __restrict__ rank, int* __restrict__ deg, int* __restrict__ counter) {
+                              int2* __restrict__ rank, int* __restrict__ deg) {
     int n = *nkeys;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t key = keys[i];
@@ -1925,10 +1905,7 @@ __global__ void k_rank_assign(const uint64_t* __restrict__ keys, const int* __re
         int k = int(pos >> 1);
         if (pos & 1u) rank[k].y = i - lo;
         else rank[k].x = i - lo;
-        if (i == lo) {
-            deg[b] = lower_bound_u64(keys, n, (b + 1) << 32) - lo;
-            counter[b] = 0;
-        }
+        if (i == lo) deg[b] = lower_bound_u64(keys, n, (b + 1) << 32) - lo;
     }
 }
 // Chunks of at most 64 arbiters of one colour; the serial overflow colour gets one arbiter per chunk.
@@ -1940,6 +1917,285 @@ __global__ void k_chunks(const uint64_t* __restrict__ order, const int* __restri
         int rel = k - offs[c];
         if (c == kMaxColours) chunks[cbeg[c] + rel] = make_int2(k, k + 1);
         else if ((rel & 63) == 0) chunks[cbeg[c] + (rel >> 6)] = make_int2(k, min(k + 64, offs[c + 1]));
+    }
+}
+
+// ---- Tile solver ----------------------------------------------------------------------------
+// One workgroup per tile: a contiguous range of the non-static bodies in broadphase (Morton) order.
+// Each arbiter belongs to the tile of its first non-static body.  A body is *private* when every
+// arbiter touching it belongs to its own tile: its v/w live in LDS for the whole launch and its
+// visits are ordered by workgroup barriers alone (the tile walks its arbiters colour by colour).
+// Other non-static bodies are *shared*: their v/w live in the packed global records and their
+// visits are ordered by the dataflow counters (need = g * deg + rank, as in k_solve_flow).  Every
+// body therefore sees its visits in colour order, so results equal the colour-batch solver's bit
+// for bit.  No deadlock on a cooperative grid: the tile whose current (pass, colour) is smallest
+// only waits for visits at strictly earlier positions, all finished.
+struct TileArgs {
+    const int* tbody;       // non-static bodies with shapes, Morton order; tile t holds [tile_begin(t), tile_begin(t+1))
+    const int* nmov;        // number of entries of tbody
+    const int* tile_of;     // per body: tile, -1 for static bodies and bodies without shapes
+    const int* tslot;       // per body: index within its tile
+    const uint32_t* shared; // per body: 1 if an arbiter of another tile touches it
+    const uint32_t* toff;   // [T * 128 + 1]: start of (tile, colour) in tlist
+    const uint64_t* tlist;  // (colour-order position << 32) | arbiter, tile-major, colour within tile
+    int* err;               // set when a dependency wait exceeds kTileSpinLimit polls (then all waits stop)
+    int T;
+};
+constexpr int kTileSpinLimit = 1 << 20;
+__device__ __forceinline__ int tile_begin(int t, int M, int T) { return int((int64_t(t) * M + T - 1) / T); }
+
+// Body -> first position of one of its shapes in the sorted broadphase order.
+__global__ void k_body_first(const uint32_t* __restrict__ sorted, const int* __restrict__ ns_ptr,
+                             const ShapeRec* __restrict__ shapes, int* __restrict__ first) {
+    int ns = *ns_ptr;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x)
+        atomicMin(&first[shapes[sorted[i]].body], i);
+}
+__global__ void k_tile_flags(const uint32_t* __restrict__ sorted, const int* __restrict__ ns_ptr,
+                             const ShapeRec* __restrict__ shapes, const int* __restrict__ motion,
+                             const int* __restrict__ first, uint32_t* __restrict__ flags) {
+    int ns = *ns_ptr;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+        int b = shapes[sorted[i]].body;
+        flags[i] = (motion[b] != JH_MOTION_STATIC && first[b] == i) ? 1u : 0u;
+    }
+}
+__global__ void k_tile_assign(const uint32_t* __restrict__ sorted, const int* __restrict__ ns_ptr,
+                              const ShapeRec* __restrict__ shapes, const uint32_t* __restrict__ flags,
+                              const uint32_t* __restrict__ rank, const int* __restrict__ nmov, int T,
+                              int* __restrict__ tbody, int* __restrict__ tile_of, int* __restrict__ tslot) {
+    int ns = *ns_ptr, M = *nmov;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
+        if (!flags[i]) continue;
+        int b = shapes[sorted[i]].body;
+        int r = int(rank[i]);
+        int t = int(int64_t(r) * T / M);
+        tbody[r] = b;
+        tile_of[b] = t;
+        tslot[b] = r - tile_begin(t, M, T);
+    }
+}
+__device__ __forceinline__ int arbiter_tile(const jh_arbiter& a, const int* motion, const int* tile_of) {
+    return motion[a.body1] != JH_MOTION_STATIC ? tile_of[a.body1] : tile_of[a.body2];
+}
+// (tile, colour) histogram of the solved arbiters, and the shared flags of their bodies.
+__global__ void k_tile_hist(const uint64_t* __restrict__ order, const int* __restrict__ offs,
+                            const jh_arbiter* __restrict__ arb, const int* __restrict__ colour,
+                            const int* __restrict__ motion, const int* __restrict__ tile_of,
+                            uint32_t* __restrict__ hist, uint32_t* __restrict__ shared) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        uint32_t ai = uint32_t(order[k]);
+        const jh_arbiter& a = arb[ai];
+        int t = arbiter_tile(a, motion, tile_of);
+        atomicAdd(&hist[size_t(t) * 128 + colour[ai]], 1u);
+        if (motion[a.body1] != JH_MOTION_STATIC && tile_of[a.body1] != t) shared[a.body1] = 1u;
+        if (motion[a.body2] != JH_MOTION_STATIC && tile_of[a.body2] != t) shared[a.body2] = 1u;
+    }
+}
+// Tile-ordered list; within one (tile, colour) the order is free (the arbiters are independent)
+// except in the serial batch, which k_tile_serial fills in solve order.
+__global__ void k_tile_scatter(const uint64_t* __restrict__ order, const int* __restrict__ offs,
+                               const jh_arbiter* __restrict__ arb, const int* __restrict__ colour,
+                               const int* __restrict__ motion, const int* __restrict__ tile_of,
+                               const uint32_t* __restrict__ toff, uint32_t* __restrict__ cursor,
+                               uint64_t* __restrict__ tlist) {
+    int A = offs[kMaxColours];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        uint32_t ai = uint32_t(order[k]);
+        size_t cell = size_t(arbiter_tile(arb[ai], motion, tile_of)) * 128 + colour[ai];
+        tlist[toff[cell] + atomicAdd(&cursor[cell], 1u)] = (uint64_t(uint32_t(k)) << 32) | ai;
+    }
+}
+__global__ void k_tile_serial(const uint64_t* __restrict__ order, const int* __restrict__ offs,
+                              const jh_arbiter* __restrict__ arb, const int* __restrict__ motion,
+                              const int* __restrict__ tile_of, const uint32_t* __restrict__ toff,
+                              uint32_t* __restrict__ cursor, uint64_t* __restrict__ tlist) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int k = offs[kMaxColours]; k < offs[kMaxColours + 1]; ++k) {
+        uint32_t ai = uint32_t(order[k]);
+        size_t cell = size_t(arbiter_tile(arb[ai], motion, tile_of)) * 128 + kMaxColours;
+        tlist[toff[cell] + cursor[cell]++] = (uint64_t(uint32_t(k)) << 32) | ai;
+    }
+}
+
+// One visit of the arbiter at tile-list position p in global pass g.  Private bodies are read and
+// written in LDS; shared bodies wait for their counters and go through the coherent records.
+template <bool PREPARE>
+__device__ __forceinline__ void tile_visit(const SolveArgs& P, const TileArgs& Tl, float4* lv, float2* lw, int p,
+                                           int g, bool apply_bias) {
+    const Bodies& B = P.B;
+    const ContactSoA& S = P.soa;
+    int2 bb = S.bodies[p];
+    uint2 um = S.um[p];
+    float2 fs = S.fs[p];
+    int b1 = bb.x, b2 = bb.y;
+    uint32_t usage = um.x, mode = um.y;
+    bool m1 = B.motion[b1] != JH_MOTION_STATIC, m2 = B.motion[b2] != JH_MOTION_STATIC;
+    bool sh1 = m1 && Tl.shared[b1] != 0u, sh2 = m2 && Tl.shared[b2] != 0u;
+    int need1 = 0, need2 = 0;
+    if (sh1 || sh2) {
+        int2 r = P.rank[int(Tl.tlist[p] >> 32)];
+        if (sh1) need1 = g * P.deg[b1] + r.x;
+        if (sh2) need2 = g * P.deg[b2] + r.y;
+    }
+    SlotReg sr[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        if (usage & (1u << s)) load_slot_soa(S, s, p, sr[s], PREPARE);
+    BodyReg R1, R2;
+    load_static_part(B, b1, R1);
+    load_static_part(B, b2, R2);
+    vec3 x1, x2;
+    quat q1, q2;
+    if (PREPARE) {
+        x1 = ld3(B.pos[b1]);
+        x2 = ld3(B.pos[b2]);
+        q1 = ldq(B.quat[b1]);
+        q2 = ldq(B.quat[b2]);
+    }
+    int s1 = (m1 && !sh1) ? Tl.tslot[b1] : 0, s2 = (m2 && !sh2) ? Tl.tslot[b2] : 0;
+    if (!m1) { R1.v = ld3(B.vel[b1]); R1.w = ld3(B.angvel[b1]); }
+    else if (!sh1) { float4 a = lv[s1]; float2 c = lw[s1]; R1.v = {a.x, a.y, a.z}; R1.w = {a.w, c.x, c.y}; }
+    if (!m2) { R2.v = ld3(B.vel[b2]); R2.w = ld3(B.angvel[b2]); }
+    else if (!sh2) { float4 a = lv[s2]; float2 c = lw[s2]; R2.v = {a.x, a.y, a.z}; R2.w = {a.w, c.x, c.y}; }
+    __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
+    bool done = false, ok1 = !sh1, ok2 = !sh2;
+    int spins = 0;
+    while (true) {
+        asm volatile("" ::: "memory");
+        if (!done && !(ok1 && ok2) && (++spins & 255) == 0) {
+            // bounded wait: a missing visit must end the launch with an error, not hang it
+            if (spins >= kTileSpinLimit) __hip_atomic_store(Tl.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_load(Tl.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok1 = ok2 = true;
+        }
+        if (!done) {
+            if (!ok1) ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w);
+            if (!ok2) ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w);
+            if (ok1 && ok2) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (!(usage & (1u << s))) continue;
+                    if (PREPARE) prepare_reg(mode, fs.y, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
+                    else iterate_reg(mode, fs.x, sr[s], R1, R2, apply_bias);
+                }
+                bool w1 = (mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) != 0;
+                bool w2 = (mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) != 0;
+                if (sh1) coh_publish_vw(vr, b1, R1.v, R1.w, need1 + 1);
+                else if (w1) { lv[s1] = make_float4(R1.v.x, R1.v.y, R1.v.z, R1.w.x); lw[s1] = make_float2(R1.w.y, R1.w.z); }
+                if (sh2) coh_publish_vw(vr, b2, R2.v, R2.w, need2 + 1);
+                else if (w2) { lv[s2] = make_float4(R2.v.x, R2.v.y, R2.v.z, R2.w.x); lw[s2] = make_float2(R2.w.y, R2.w.z); }
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    if (usage & (1u << s)) store_slot_soa(S, s, p, sr[s], PREPARE);
+                done = true;
+            }
+        }
+        if (__all(done)) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// One pass of a tile: its arbiters colour by colour, a workgroup barrier after each non-empty
+// colour; the serial batch by one thread in solve order.
+__device__ __forceinline__ void tile_pass(const SolveArgs& P, const TileArgs& Tl, float4* lv, float2* lw,
+                                          const uint32_t* segs, int nc, int g, int phase) {
+    for (int c = 0; c < nc; ++c) {
+        int beg = int(segs[c]), end = int(segs[c + 1]);
+        if (beg == end) continue;
+        for (int p = beg + int(threadIdx.x); p < end; p += blockDim.x) {
+            if (phase == 0) tile_visit<true>(P, Tl, lv, lw, p, g, false);
+            else tile_visit<false>(P, Tl, lv, lw, p, g, phase == 1);
+        }
+        __syncthreads();
+    }
+    int beg = int(segs[kMaxColours]), end = int(segs[kMaxColours + 1]);
+    if (beg < end) {
+        if (threadIdx.x == 0)
+            for (int p = beg; p < end; ++p) {
+                if (phase == 0) tile_visit<true>(P, Tl, lv, lw, p, g, false);
+                else tile_visit<false>(P, Tl, lv, lw, p, g, phase == 1);
+            }
+        __syncthreads();
+    }
+}
+
+// One launch per step, grid = T tiles (cooperative); dynamic LDS holds 24 bytes per tile body.
+__global__ void __launch_bounds__(256) k_solve_tiles(SolveArgs P, TileArgs Tl) {
+    extern __shared__ float4 tile_lds[];
+    __shared__ uint32_t segs[kMaxColours + 2];
+    const Bodies& B = P.B;
+    int t = blockIdx.x;
+    int M = *Tl.nmov;
+    int tb = tile_begin(t, M, Tl.T), te = tile_begin(t + 1, M, Tl.T);
+    int cap = te - tb;
+    float4* lv = tile_lds;
+    float2* lw = (float2*)(tile_lds + cap);
+    int nc = *P.ncolours;
+    for (int c = threadIdx.x; c <= kMaxColours + 1; c += blockDim.x) segs[c] = Tl.toff[size_t(t) * 128 + c];
+    int per_sub = 1 + P.iterations + P.relaxations;
+    int stride = gridDim.x * blockDim.x;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    // v/w: private bodies into LDS, shared bodies into the packed records
+    for (int i = tb + int(threadIdx.x); i < te; i += blockDim.x) {
+        int b = Tl.tbody[i];
+        vec3 v = ld3(B.vel[b]), w = ld3(B.angvel[b]);
+        if (Tl.shared[b]) st_vw(P, b, v, w, 0);
+        else { lv[i - tb] = make_float4(v.x, v.y, v.z, w.x); lw[i - tb] = make_float2(w.y, w.z); }
+    }
+    for (int s = 0; s < P.ts.substeps; ++s) {
+        // IntegrateForces (World.Step.cs:1060-1072)
+        for (int i = tb + int(threadIdx.x); i < te; i += blockDim.x) {
+            int b = Tl.tbody[i];
+            if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
+            vec3 v, w;
+            bool sh = Tl.shared[b] != 0u;
+            int l = i - tb, n = 0;
+            if (sh) n = ld_vw(P, b, v, w);
+            else { float4 a = lv[l]; float2 c = lw[l]; v = {a.x, a.y, a.z}; w = {a.w, c.x, c.y}; }
+            w = w + ld3(B.dangvel[b]);
+            v = v + ld3(B.dvel[b]);
+            if (sh) st_vw(P, b, v, w, n);
+            else { lv[l] = make_float4(v.x, v.y, v.z, w.x); lw[l] = make_float2(w.y, w.z); }
+        }
+        for (int b = t0; b < P.nb; b += stride) {  // dynamic bodies without shapes touch no arbiter
+            if (B.motion[b] != JH_MOTION_DYNAMIC || Tl.tile_of[b] >= 0) continue;
+            B.angvel[b] = st3(ld3(B.angvel[b]) + ld3(B.dangvel[b]));
+            B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
+        }
+        grid_barrier(P.bar);
+        for (int it = 0; it <= P.iterations; ++it) tile_pass(P, Tl, lv, lw, segs, nc, s * per_sub + it, it == 0 ? 0 : 1);
+        grid_barrier(P.bar);
+        // IntegrateVelocities (World.Step.cs:1106-1133)
+        for (int i = tb + int(threadIdx.x); i < te; i += blockDim.x) {
+            int b = Tl.tbody[i];
+            vec3 v, w;
+            if (Tl.shared[b]) ld_vw(P, b, v, w);
+            else { float4 a = lv[i - tb]; float2 c = lw[i - tb]; v = {a.x, a.y, a.z}; w = {a.w, c.x, c.y}; }
+            B.pos[b] = st3(ld3(B.pos[b]) + v * P.ts.substep_dt);
+            quat rq = rotation_quaternion(w, P.ts.substep_dt);
+            B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+        }
+        for (int b = t0; b < P.nb; b += stride) {
+            if (B.motion[b] == JH_MOTION_STATIC || Tl.tile_of[b] >= 0) continue;
+            B.pos[b] = st3(ld3(B.pos[b]) + ld3(B.vel[b]) * P.ts.substep_dt);
+            quat rq = rotation_quaternion(ld3(B.angvel[b]), P.ts.substep_dt);
+            B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+        }
+        grid_barrier(P.bar);
+        if (P.relaxations > 0) {
+            for (int it = 0; it < P.relaxations; ++it)
+                tile_pass(P, Tl, lv, lw, segs, nc, s * per_sub + 1 + P.iterations + it, 2);
+            grid_barrier(P.bar);
+        }
+    }
+    for (int i = tb + int(threadIdx.x); i < te; i += blockDim.x) {
+        int b = Tl.tbody[i];
+        vec3 v, w;
+        if (Tl.shared[b]) ld_vw(P, b, v, w);
+        else { float4 a = lv[i - tb]; float2 c = lw[i - tb]; v = {a.x, a.y, a.z}; w = {a.w, c.x, c.y}; }
+        B.vel[b] = st3(v);
+        B.angvel[b] = st3(w);
     }
 }
 

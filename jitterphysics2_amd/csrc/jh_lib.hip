@@ -79,7 +79,10 @@ enum {
     C_CONTACTS = 12,  // contact points solved
     C_NCHUNK = 13,    // dataflow solver chunks
     C_NRANK = 14,     // rank keys
-    C_COUNT = 16
+    C_NMOV = 16,      // tile solver: non-static bodies with shapes
+    C_TCELLS = 17,    // tile solver: (tile, colour) cells + 1
+    C_SERR = 18,      // tile solver: a dependency wait gave up
+    C_COUNT = 20
 };
 
 constexpr int kEpaMaxThreads = 32768;  // polytopes resident at once (epa::kBytesPerThread each)
@@ -192,11 +195,20 @@ struct jh_world {
     DBuf<uint32_t> whist, woff, wcur, wsums;
     DBuf<uint64_t> wlist;
     DBuf<float4> vwpack;  // dataflow solver: packed velocities, 2 per body
+    // tile solver (one workgroup per Morton range of bodies, private bodies in LDS)
+    bool tile_mode = false;
+    int tiles = 0;            // tiles = workgroups of k_solve_tiles
+    int n_moving = 0;         // non-static bodies (upper bound of the tiled bodies)
+    int tile_occ_lds = -1, tile_occ = 0;  // occupancy cache: LDS bytes -> resident workgroups per CU
+    int sorted_in1 = 0;       // broadphase order ended in mv1
+    DBuf<int> tfirst, tbody, tile_of, tslot;
+    DBuf<uint32_t> tflags, trank, tshared, thist, toff, tcur, tsums;
+    DBuf<uint64_t> tlist;
     DBuf<uint64_t> ok0, ok1, solve_keys;
     // dataflow solver
     DBuf<uint64_t> rk0, rk1;
     DBuf<int2> rank, chunks;
-    DBuf<int> deg, bcounter, cbeg;
+    DBuf<int> deg, cbeg;
     DBuf<unsigned> bar;
     bool flow_solver = true;
     bool recolour = true;  // next colouring ignores the colours stored in the arbiters
@@ -229,7 +241,8 @@ namespace {
 
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
-    if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK)
+    if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK ||
+        t == C_SERR)
         cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
     else if (t < 6) bounds[t] = 0u;
@@ -287,6 +300,7 @@ jh_status broadphase(jh_world* w) {
     bool in1 = rsort::sort_pairs(w->mk0.p, w->mk1.p, w->mv0.p, w->mv1.p, cnt + C_NS, ns, shifts, nsh, w->hist.p, st);
     uint64_t* keys = in1 ? w->mk1.p : w->mk0.p;
     uint32_t* order = in1 ? w->mv1.p : w->mv0.p;
+    w->sorted_in1 = in1;
     k_lbvh_build<<<grid_for(ns - 1, 256, 1 << 20), 256, 0, st>>>(keys, ns, w->left.p, w->right.p, w->range.p,
                                                                   w->parent.p, w->leaf_parent.p, w->nflags.p);
     k_lbvh_refit<<<g, 256, 0, st>>>(ns, order, w->amin.p, w->amax.p, w->lmin.p, w->lmax.p, w->left.p, w->right.p,
@@ -366,11 +380,79 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->offs.ensure(na));
     HIPCHK(w->sums.ensure(na / 256 + 2));
     HIPCHK(w->hist.ensure(rsort::hist_entries(2 * na)));  // the rank-key sort holds up to 2 keys per arbiter
+    if (w->tile_mode) HIPCHK(w->tlist.ensure(na));
     return JH_OK;
 }
 
 jh_status launch_coop(const void* fn, int grid, int block, void** args, hipStream_t st) {
     HIPCHK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, st));
+    return JH_OK;
+}
+
+// Tile solver: dynamic LDS of k_solve_tiles for T tiles (24 bytes per tile body), 0 if the tiles do
+// not fit (more than 64 KiB per tile, or fewer co-resident workgroups than tiles).
+size_t tile_lds_bytes(jh_world* w, int T) {
+    if (T < 1) return 0;
+    size_t per = (size_t(std::max(w->n_moving, 1)) + T - 1) / T;
+    size_t lds = std::max<size_t>(per, 1) * 24;
+    if (lds > 65536) return 0;
+    if (int(lds) != w->tile_occ_lds) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_tiles, 256, lds) != hipSuccess) occ = 0;
+        w->tile_occ_lds = int(lds);
+        w->tile_occ = occ;
+    }
+    return size_t(w->tile_occ) * w->cu_count >= size_t(T) ? lds : 0;
+}
+
+// Tile solver set-up: tiles from the broadphase order, the shared flags, and the (tile, colour)
+// ordered arbiter list.
+jh_status tile_setup(jh_world* w, const uint64_t* order, const int* a_cur, int amax_) {
+    hipStream_t st = w->st;
+    int* cnt = w->cnt.p;
+    int ns = w->ns, nb = std::max(w->nb, 1), T = w->tiles;
+    size_t cells = size_t(T) * 128 + 1;
+    HIPCHK(w->tfirst.ensure(nb));
+    HIPCHK(w->tile_of.ensure(nb));
+    HIPCHK(w->tslot.ensure(nb));
+    HIPCHK(w->tbody.ensure(nb));
+    HIPCHK(w->tshared.ensure(nb));
+    HIPCHK(w->tflags.ensure(std::max(ns, 1)));
+    HIPCHK(w->trank.ensure(std::max(ns, 1)));
+    HIPCHK(w->tsums.ensure(std::max<size_t>(size_t(std::max(ns, 1)), cells) / 256 + 2));
+    HIPCHK(w->thist.ensure(cells));
+    HIPCHK(w->toff.ensure(cells));
+    HIPCHK(w->tcur.ensure(cells));
+    int gb = grid_for(nb);
+    k_fill_u32<<<gb, 256, 0, st>>>((uint32_t*)w->tfirst.p, size_t(nb), 0x7fffffffu);
+    k_fill_u32<<<gb, 256, 0, st>>>((uint32_t*)w->tile_of.p, size_t(nb), 0xffffffffu);
+    k_fill_u32<<<gb, 256, 0, st>>>(w->tshared.p, size_t(nb), 0u);
+    int gc = grid_for(int(std::min<size_t>(cells, 1 << 30)));
+    k_fill_u32<<<gc, 256, 0, st>>>(w->thist.p, cells, 0u);
+    k_fill_u32<<<gc, 256, 0, st>>>(w->tcur.p, cells, 0u);
+    if (ns >= 2) {  // the broadphase sorted the shapes
+        const uint32_t* sorted = w->sorted_in1 ? w->mv1.p : w->mv0.p;
+        int gs = grid_for(ns);
+        k_body_first<<<gs, 256, 0, st>>>(sorted, cnt + C_NS, w->shapes.p, w->tfirst.p);
+        k_tile_flags<<<gs, 256, 0, st>>>(sorted, cnt + C_NS, w->shapes.p, w->motion.p, w->tfirst.p, w->tflags.p);
+        scan::exclusive_scan(w->tflags.p, w->trank.p, cnt + C_NS, ns, w->tsums.p, cnt + C_NMOV, st);
+        k_tile_assign<<<gs, 256, 0, st>>>(sorted, cnt + C_NS, w->shapes.p, w->tflags.p, w->trank.p, cnt + C_NMOV, T,
+                                          w->tbody.p, w->tile_of.p, w->tslot.p);
+    } else {
+        k_set_int<<<1, 1, 0, st>>>(cnt + C_NMOV, 0);
+    }
+    int ga = grid_for(std::max(amax_, 1));
+    jh_arbiter* arb = w->arb[w->cur].p;
+    k_tile_hist<<<ga, 256, 0, st>>>(order, w->coffs.p, arb, w->colour.p, w->motion.p, w->tile_of.p, w->thist.p,
+                                    w->tshared.p);
+    k_set_int<<<1, 1, 0, st>>>(cnt + C_TCELLS, int(cells));
+    scan::exclusive_scan(w->thist.p, w->toff.p, cnt + C_TCELLS, int(cells), w->tsums.p, nullptr, st);
+    k_tile_scatter<<<ga, 256, 0, st>>>(order, w->coffs.p, arb, w->colour.p, w->motion.p, w->tile_of.p, w->toff.p,
+                                       w->tcur.p, w->tlist.p);
+    k_tile_serial<<<1, 64, 0, st>>>(order, w->coffs.p, arb, w->motion.p, w->tile_of.p, w->toff.p, w->tcur.p,
+                                    w->tlist.p);
+    (void)a_cur;
+    HIPCHK(hipGetLastError());
     return JH_OK;
 }
 
@@ -534,6 +616,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     a_cur = a_nxt;
     HIPCHK(hipEventRecord(w->ev[3], st));
     // colouring of the arbiters (graph edges between dynamic bodies)
+    size_t tile_lds = (w->flow_solver && w->tile_mode && !w->world_mode) ? tile_lds_bytes(w, w->tiles) : 0;
+    bool use_tiles = tile_lds > 0;
     {
         ColourArgs P;
         P.arb = w->arb[cur].p;
@@ -579,8 +663,12 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             bool rin1 = rsort::sort_pairs(w->rk0.p, w->rk1.p, nullptr, nullptr, cnt + C_NRANK, 2 * std::max(amax_, 1),
                                           rsh, nrs, w->hist.p, st);
             k_rank_assign<<<grid_for(2 * std::max(amax_, 1)), 256, 0, st>>>(rin1 ? w->rk1.p : w->rk0.p, cnt + C_NRANK,
-                                                                            w->rank.p, w->deg.p, w->bcounter.p);
+                                                                            w->rank.p, w->deg.p);
             k_chunks<<<ga, 256, 0, st>>>(order, w->coffs.p, w->cbeg.p, w->chunks.p);
+            if (use_tiles) {
+                s = tile_setup(w, order, a_cur, amax_);
+                if (s != JH_OK) return s;
+            }
         }
         HIPCHK(hipEventRecord(w->ev[4], st));
         // persistent solver over all substeps
@@ -593,7 +681,6 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.nchunks = cnt + C_NCHUNK;
         S.rank = w->rank.p;
         S.deg = w->deg.p;
-        S.counter = w->bcounter.p;
         S.bar = w->bar.p;
         S.B = B;
         S.nb = w->nb;
@@ -640,6 +727,24 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             k_solve_worlds<<<std::max(1, std::min(w->world_blocks, w->n_worlds)), 512, 0, st>>>(S, Wd);
             k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, w->wlist.p, w->coffs.p, S.soa);
             HIPCHK(hipGetLastError());
+        } else if (use_tiles) {
+            int gsoa = grid_for(std::max(amax_, 1));
+            k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, w->tlist.p, w->coffs.p, S.soa);
+            TileArgs Tl;
+            Tl.tbody = w->tbody.p;
+            Tl.nmov = cnt + C_NMOV;
+            Tl.tile_of = w->tile_of.p;
+            Tl.tslot = w->tslot.p;
+            Tl.shared = w->tshared.p;
+            Tl.toff = w->toff.p;
+            Tl.tlist = w->tlist.p;
+            Tl.err = cnt + C_SERR;
+            Tl.T = w->tiles;
+            void* targs[] = {&S, &Tl};
+            HIPCHK(hipLaunchCooperativeKernel((const void*)k_solve_tiles, dim3(w->tiles), dim3(256), targs,
+                                              unsigned(tile_lds), st));
+            k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, w->tlist.p, w->coffs.p, S.soa);
+            HIPCHK(hipGetLastError());
         } else if (w->flow_solver) {
             int gsoa = grid_for(std::max(amax_, 1));
             k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
@@ -670,6 +775,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     HIPCHK(hipEventRecord(w->ev[6], st));
     HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (w->hcnt[C_SERR]) return fail(JH_ERR_HIP, "tile solver: a dependency wait did not finish");
     // solve-order bookkeeping and stats
     w->a_host = w->hcnt[w->cur == 0 ? C_A0 : C_A1];
     w->solve_count = w->hcnt[cur == 0 ? C_A0 : C_A1];
@@ -684,6 +790,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     S.contacts = w->hcnt[C_CONTACTS];
     S.colours = w->hcnt[C_NCOL];
     S.epa_calls = w->hcnt[C_DEEP];
+    S.solver = !w->flow_solver ? 0 : w->world_mode ? 2 : use_tiles ? 3 : 1;
     S.steps++;
     float ms[7];
     for (int k = 0; k < 6; ++k) HIPCHK(hipEventElapsedTime(&ms[k], w->ev[k], w->ev[k + 1]));
@@ -789,6 +896,8 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
         w->coop_solve = std::max(1, std::min(atoi(e), w->coop_solve));
         w->flow_blocks = std::max(1, std::min(atoi(e), std::max(1, occ) * w->cu_count));
     }
+    w->tiles = w->cu_count;
+    if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
     w->colour_blocks = std::min(64, w->coop_colour);
     if (const char* e = getenv("JH_COLOUR_BLOCKS")) w->colour_blocks = std::max(1, std::min(atoi(e), w->coop_colour));
     *out = w;
@@ -816,10 +925,13 @@ jh_status jh_world_destroy(jh_world* w) {
     w->soa_fs.release(); w->soa_flag.release(); w->soa_f.release();
     w->wbody.release(); w->wboff.release(); w->wsched.release(); w->wcells.release(); w->wqueue.release();
     w->whist.release(); w->woff.release(); w->wcur.release(); w->wsums.release(); w->wlist.release();
+    w->tfirst.release(); w->tbody.release(); w->tile_of.release(); w->tslot.release(); w->tflags.release();
+    w->trank.release(); w->tshared.release(); w->thist.release(); w->toff.release(); w->tcur.release();
+    w->tsums.release(); w->tlist.release();
     w->vwpack.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
-    w->bcounter.release(); w->cbeg.release(); w->bar.release();
+    w->cbeg.release(); w->bar.release();
     w->lws.release(); w->pvel.release(); w->pang.release(); w->parb.release(); w->ppart.release();
     w->pfeat.release(); w->ptarg.release(); w->pvalid.release(); w->plam.release();
     for (auto& e : w->ev) (void)hipEventDestroy(e);
@@ -866,6 +978,9 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
         w->n_worlds = W;
         const char* e = getenv("JH_SOLVER");
         w->world_mode = w->flow_solver && (e ? strcmp(e, "world") == 0 : W >= 64);
+        w->tile_mode = w->flow_solver && e && strcmp(e, "tiles") == 0;
+        w->n_moving = 0;
+        for (int i = 0; i < n; ++i) w->n_moving += bodies[i].motion_type != JH_MOTION_STATIC;
         if (W > 0) {
             HIPCHK(w->wbody.ensure(std::max(n, 1)));
             HIPCHK(w->wboff.ensure(W + 1));
@@ -898,7 +1013,6 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     HIPCHK(w->mlo.ensure(nn));
     HIPCHK(w->mhi.ensure(nn));
     HIPCHK(w->deg.ensure(nn));
-    HIPCHK(w->bcounter.ensure(nn));
     HIPCHK(w->vwpack.ensure(2 * nn));
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(w->body_io.p, bodies, n * sizeof(jh_body), hipMemcpyHostToDevice, w->st));
@@ -1506,14 +1620,22 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
             if (cand[i] == def) default_ms = tot[i] / rounds;
         return JH_OK;
     };
-    // many worlds: the world-local solver is one more candidate (encoded as solve shape 0)
+    // many worlds: the world-local solver is one more candidate (encoded as solve shape 0); the tile
+    // solver is a candidate for every tile count whose tiles fit (encoded as minus the tile count)
     bool world_default = w->world_mode;
+    int solver_default = w->world_mode ? 0 : (w->tile_mode ? -w->tiles : 0);
     if (w->flow_solver && w->n_worlds > 1) sc.insert(sc.begin(), 0);
+    if (w->flow_solver)
+        for (int T : {cu / 2, cu, 2 * cu})
+            if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back(-T);
     std::vector<int> flow_shapes = sc;
-    // candidates: 0 = world-local solver, > 0 = dataflow solver with that many workgroups
+    // candidates: 0 = world-local solver, > 0 = dataflow solver with that many workgroups, < 0 = tiles
+    int keep_tiles = w->tiles;
     auto apply = [&](int v) {
         w->world_mode = v == 0;
+        w->tile_mode = v < 0;
         if (v > 0) solve_knob = v;
+        if (v < 0) w->tiles = -v;
     };
     int keep_solve = solve_knob;
     {
@@ -1529,16 +1651,18 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         for (size_t i = 1; i < flow_shapes.size(); ++i)
             if (tot[i] < tot[bi]) bi = i;
         solve_knob = keep_solve;
+        w->tiles = keep_tiles;
         apply(flow_shapes[bi]);
         out->ms_solve_best = tot[bi] / rounds;
         out->ms_solve_default = out->ms_solve_best;
         for (size_t i = 0; i < flow_shapes.size(); ++i)
-            if (flow_shapes[i] == (world_default ? 0 : solve_default)) out->ms_solve_default = tot[i] / rounds;
+            if (flow_shapes[i] == (world_default ? 0 : solver_default < 0 ? solver_default : solve_default))
+                out->ms_solve_default = tot[i] / rounds;
     }
     jh_status s = JH_OK;
     s = tune(cc, w->colour_blocks, false, out->ms_colour_best, out->ms_colour_default, colour_default);
     if (s != JH_OK) return s;
-    out->solve_blocks = w->world_mode ? 0 : solve_knob;  // 0: world-local solver
+    out->solve_blocks = w->world_mode ? 0 : w->tile_mode ? -w->tiles : solve_knob;  // 0: world-local, < 0: tiles
     out->colour_blocks = w->colour_blocks;
     out->candidates = int(flow_shapes.size() + cc.size());
     out->steps = rounds * out->candidates;

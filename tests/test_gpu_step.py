@@ -140,7 +140,8 @@ def test_autotune_keeps_results_exact():
     for _ in range(60):
         gw.step(dt)
     t = gw.autotune(dt, rounds=1)
-    assert t["solve_blocks"] >= 1 and t["colour_blocks"] >= 1 and t["steps"] == t["candidates"]
+    assert t["solve_blocks"] != 0 and t["colour_blocks"] >= 1 and t["steps"] == t["candidates"]
+    assert gw.stats()["solver"] == (3 if t["solve_blocks"] < 0 else 1)
     assert t["ms_solve_best"] <= t["ms_solve_default"] + 1e-9
     # continue from the tuned world's state: the oracle replay stays bit-exact with the new shapes
     ow = OracleWorld(gw.bodies(), shapes, verts, cfg)
@@ -199,6 +200,22 @@ def test_fp32_error_against_fp64_oracle():
     dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10]).max()
     print(f"FP32 GPU vs FP64 oracle after 30 steps: max |dp| {dp:.3e} m, max |dv| {dv:.3e} m/s")
     assert dp < 1e-4 and dv < 1e-3
+
+
+def test_tile_solver_bitexact(monkeypatch):
+    # one workgroup per Morton range of bodies: private bodies in LDS, ordered by workgroup barriers;
+    # bodies touched by other tiles' arbiters through the dataflow counters.  Every body still sees its
+    # visits in colour order, so the replay stays bit-exact for any tile count.
+    monkeypatch.setenv("JH_SOLVER", "tiles")
+    for tiles in ("1", "5", "64"):
+        monkeypatch.setenv("JH_TILES", tiles)
+        gw, _ = run_replay(scenes.heap(900, seed=17), steps=50, check_every=49)
+        assert gw.stats()["solver"] == 3
+    monkeypatch.setenv("JH_TILES", "16")
+    run_replay(scenes.pyramid(rows=12), steps=30, check_every=10)
+    gw, _ = run_replay(scenes.full_form(n_bodies=200, cells=32, worlds=4, feature=8.0, pitch=2.4), steps=60,
+                       check_every=20)
+    assert gw.stats()["solver"] == 3
 
 
 def test_world_solver_bitexact(monkeypatch):
