@@ -184,7 +184,7 @@ def main() -> int:
     tuning = None
     if not args.no_autotune:
         tuning = gw.autotune(dt, rounds=2)
-        print(f"# autotune: solve blocks {tuning['solve_blocks']} ({tuning['ms_solve_best']:.2f} ms vs "
+        print(f"# autotune: solver {tuning['solver']} x {tuning['solve_blocks']} ({tuning['ms_solve_best']:.2f} ms vs "
               f"{tuning['ms_solve_default']:.2f} default), colour blocks {tuning['colour_blocks']} "
               f"({tuning['ms_colour_best']:.2f} ms vs {tuning['ms_colour_default']:.2f})", flush=True)
 
@@ -225,7 +225,8 @@ def main() -> int:
                "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body heap continued from the GPU state at "
                          f"the start of the timed region (oracle FP32, key-order solve, 1 thread)"}
 
-    kernel = {0: "k_solve", 1: "k_solve_flow", 2: "k_solve_worlds", 3: "k_solve_tiles"}[int(final["solver"])]
+    kernel = {0: "k_solve", 1: "k_solve_flow", 2: "k_solve_worlds", 3: "k_solve_tiles",
+              4: "k_solve_lanes"}[int(final["solver"])]
     traffic = measured_traffic(kernel, args)
 
     line = {
@@ -252,7 +253,7 @@ def main() -> int:
                     f"{iters} iterations, {relax} relaxations"),
                    "autotune": tuning,
                    "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],
-                   "contacts": final["contacts"], "colours": final["colours"],
+                   "contacts": final["contacts"], "colours": final["colours"], "max_degree": final["max_degree"],
                    "colliding_pairs": final["colliding_pairs"], "epa_pairs": final["epa_calls"],
                    "contacts_per_s": float(np.mean(contacts)) * args.steps / wall,
                    "stage_ms": {k: final[k] for k in ("ms_broadphase", "ms_narrowphase", "ms_contact_cache",

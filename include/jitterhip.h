@@ -177,7 +177,9 @@ typedef struct jh_stats {
     double ms_step;
     double ms_epa;            /* part of ms_narrowphase spent in the EPA kernel */
     int64_t solver;           /* solver kernel of the last step: 0 colour batches with grid barriers,
-                                 1 dataflow, 2 world-local, 3 tiles */
+                                 1 dataflow (wave chunks), 2 world-local, 3 tiles, 4 dataflow (lanes) */
+    int64_t max_degree;       /* most solved arbiters on one non-static body (a lower bound of the
+                                 colour count); dataflow and tile solvers only, else 0 */
 } jh_stats;
 
 typedef struct jh_world jh_world;
@@ -310,11 +312,12 @@ jh_status jh_world_step_probe(jh_world* world, float dt, jh_probe* probe);
  * rounds * candidates times (the simulation advances); results are deterministic whatever the
  * launch shape. */
 typedef struct jh_tuning {
-    int32_t solve_blocks;      /* chosen solver: > 0 dataflow-solver workgroups; 0 world-local solver;
-                                  < 0 tile solver with -solve_blocks tiles */
+    int32_t solver;            /* chosen solver kernel (as jh_stats.solver; 4 = lanes)           */
+    int32_t solve_blocks;      /* its workgroups (tiles for the tile solver)                      */
     int32_t colour_blocks;     /* chosen colouring workgroups          */
     int32_t candidates;
     int32_t steps;             /* steps taken                          */
+    int32_t reserved;
     double ms_solve_best, ms_solve_default;    /* mean solver time: chosen / default shape      */
     double ms_colour_best, ms_colour_default;  /* mean colouring time: chosen / default shape   */
 } jh_tuning;

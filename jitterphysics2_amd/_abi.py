@@ -136,11 +136,12 @@ STATS_DTYPE = np.dtype([
     ("steps", "<i8"), ("pair_overflow", "<i8"), ("arbiters_solved", "<i8"),
     ("ms_broadphase", "<f8"), ("ms_narrowphase", "<f8"), ("ms_contact_cache", "<f8"),
     ("ms_colouring", "<f8"), ("ms_solve", "<f8"), ("ms_update", "<f8"), ("ms_step", "<f8"), ("ms_epa", "<f8"),
-    ("solver", "<i8"),
+    ("solver", "<i8"), ("max_degree", "<i8"),
 ])
 
 TUNING_DTYPE = np.dtype([
-    ("solve_blocks", "<i4"), ("colour_blocks", "<i4"), ("candidates", "<i4"), ("steps", "<i4"),
+    ("solver", "<i4"), ("solve_blocks", "<i4"), ("colour_blocks", "<i4"), ("candidates", "<i4"), ("steps", "<i4"),
+    ("reserved", "<i4"),
     ("ms_solve_best", "<f8"), ("ms_solve_default", "<f8"), ("ms_colour_best", "<f8"), ("ms_colour_default", "<f8"),
 ])
 

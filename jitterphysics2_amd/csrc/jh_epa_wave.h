@@ -262,24 +262,13 @@ __device__ __forceinline__ int closest_triangle(Poly& w, int lane) {
     return w.L->ord[bi];
 }
 
-// ConvexPolytope.AddVertexDetailed (:199-265).
-__device__ __forceinline__ int add_vertex(Poly& w, const MVertex& vx, int lane) {
-    if (w.tcount == 0 || !epa::is_safe(vx.V)) return epa::NumericalFailure;
-    if (w.vcount == MaxVertices) return epa::CapacityExceeded;
-    set_vertex(w, w.vcount, vx, lane);
-    // lit test of every triangle
-    int litcount = 0;
-    for (int i = lane; i < w.tcount; i += 64) {
-        int s = w.L->ord[i];
-        short fa, fb, fc, ff;
-        unpack(w.L->tri[s], fa, fb, fc, ff);
-        bool lit = side(w, w.plane(s), fa, fb, fc, w.vcount) > 0;
-        w.L->lit[i] = lit ? 1 : 0;
-    }
-    wave_sync();
-    // horizon: lit triangles in descending logical order, edges (a,b), (b,c), (c,a); a reversed match
-    // already in the list cancels (the last edge moves into its place), otherwise the edge is appended
+// Horizon of the lit triangles (the sequential walk of ConvexPolytope.AddVertexDetailed :217-246): lit
+// triangles in descending logical order, edges (a,b), (b,c), (c,a); a matching edge (either direction)
+// already in the list cancels (the last edge moves into its place), otherwise the edge is appended.
+// This form keeps the list in LDS (any length); horizon_regs below is the fast path.
+__device__ __noinline__ int horizon_lds(Poly& w, int lane, int& litcount) {
     int ecount = 0;
+    litcount = 0;
     for (int i = w.tcount; i-- > 0;) {
         if (!w.L->lit[i]) continue;
         litcount++;
@@ -307,29 +296,121 @@ __device__ __forceinline__ int add_vertex(Poly& w, const MVertex& vx, int lane) 
             wave_sync();
         }
     }
+    return ecount;
+}
+
+// The same walk with the list in registers: edge e is held by lane e % 64 in register e / 64, so a
+// match search is one compare and ballot per register, a removal one readlane, and the lit triangles
+// come from the ballot masks of the lit test instead of a scan of the lit flags.  Returns -1 when the
+// list would outgrow the registers (the caller then runs horizon_lds).
+constexpr int kEdgeRegs = 4;
+__device__ __forceinline__ unsigned ereg_get(const unsigned (&r)[kEdgeRegs], int q) {
+    return q == 0 ? r[0] : q == 1 ? r[1] : q == 2 ? r[2] : r[3];
+}
+__device__ __forceinline__ void ereg_set(unsigned (&r)[kEdgeRegs], int q, unsigned v) {
+    if (q == 0) r[0] = v;
+    else if (q == 1) r[1] = v;
+    else if (q == 2) r[2] = v;
+    else r[3] = v;
+}
+__device__ __forceinline__ int horizon_regs(Poly& w, int lane, const unsigned long long (&litm)[4], int& litcount) {
+    unsigned ereg[kEdgeRegs] = {0u, 0u, 0u, 0u};
+    int ecount = 0;
+    litcount = 0;
+    for (int j = 3; j >= 0; --j) {
+        unsigned long long m = litm[j];
+        while (m) {
+            int hi = 63 - __clzll(m);
+            m &= ~(1ull << hi);
+            litcount++;
+            short fa, fb, fc, ff;
+            unpack(w.L->tri[w.L->ord[j * 64 + hi]], fa, fb, fc, ff);
+            short corners[3] = {fa, fb, fc};
+            for (int k = 0; k < 3; ++k) {
+                unsigned A = (unsigned short)corners[k], B = (unsigned short)corners[(k + 1) % 3];
+                unsigned fwd = A | (B << 16), rev = B | (A << 16);
+                int found = -1;
+#pragma unroll
+                for (int q = kEdgeRegs - 1; q >= 0; --q) {
+                    if (found >= 0 || q * 64 >= ecount) continue;
+                    unsigned v = ereg[q];
+                    unsigned long long mm = __ballot(q * 64 + lane < ecount && (v == fwd || v == rev));
+                    if (mm) found = q * 64 + 63 - __clzll(mm);
+                }
+                if (found >= 0) {
+                    --ecount;
+                    unsigned last = __builtin_amdgcn_readlane(ereg_get(ereg, ecount >> 6), ecount & 63);
+                    if (lane == (found & 63)) ereg_set(ereg, found >> 6, last);
+                } else {
+                    if (ecount == kEdgeRegs * 64) return -1;
+                    if (lane == (ecount & 63)) ereg_set(ereg, ecount >> 6, fwd);
+                    ++ecount;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kEdgeRegs; ++q)
+        if (q * 64 + lane < ecount) w.L->edge[q * 64 + lane] = ereg[q];
+    wave_sync();
+    return ecount;
+}
+
+// ConvexPolytope.AddVertexDetailed (:199-265).
+__device__ __forceinline__ int add_vertex(Poly& w, const MVertex& vx, int lane) {
+    if (w.tcount == 0 || !epa::is_safe(vx.V)) return epa::NumericalFailure;
+    if (w.vcount == MaxVertices) return epa::CapacityExceeded;
+    set_vertex(w, w.vcount, vx, lane);
+    // lit test of every triangle (MaxTriangles = 4 x 64: lane + 64 j covers them)
+    static_assert(MaxTriangles <= 256, "lit masks hold 256 triangles");
+    unsigned long long litm[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int i = j * 64 + lane;
+        bool lit = false;
+        if (i < w.tcount) {
+            int s = w.L->ord[i];
+            short fa, fb, fc, ff;
+            unpack(w.L->tri[s], fa, fb, fc, ff);
+            lit = side(w, w.plane(s), fa, fb, fc, w.vcount) > 0;
+            w.L->lit[i] = lit ? 1 : 0;
+        }
+        litm[j] = __ballot(lit);
+    }
+    wave_sync();
+    int litcount = 0;
+    int ecount = horizon_regs(w, lane, litm, litcount);
+    if (ecount < 0) ecount = horizon_lds(w, lane, litcount);
     if (ecount == 0) return epa::NoHorizon;
     if (w.tcount - litcount + ecount > MaxTriangles) return epa::CapacityExceeded;
-    // survivors keep their order; lit slots return to the free list
+    // survivors keep their order; lit slots return to the free list (all reads before the writes:
+    // the compaction is in place)
     int surv_before = 0, lit_before = 0;
-    unsigned char keep_slot[4], lit_slot[4];
-    int keep_pos[4], lit_pos[4], nk = 0, nl = 0;
-    for (int base = 0; base < w.tcount; base += 64) {
-        int i = base + lane;
+    unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    unsigned char slot_j[4];
+    int pos_j[4];
+    bool valid_j[4], lit_j[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int i = j * 64 + lane;
         bool valid = i < w.tcount;
-        bool lit = valid && w.L->lit[i];
-        unsigned long long lm = __ballot(lit), sm = __ballot(valid && !lit);
-        unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-        if (valid) {
-            unsigned char s = w.L->ord[i];
-            if (lit) { lit_slot[nl] = s; lit_pos[nl] = lit_before + __popcll(lm & below); nl++; }
-            else { keep_slot[nk] = s; keep_pos[nk] = surv_before + __popcll(sm & below); nk++; }
-        }
+        unsigned long long vm = __ballot(valid);
+        unsigned long long lm = litm[j] & vm, sm = vm & ~litm[j];
+        bool lit = ((litm[j] >> lane) & 1ull) != 0;
+        valid_j[j] = valid;
+        lit_j[j] = lit;
+        slot_j[j] = valid ? w.L->ord[i] : (unsigned char)0;
+        pos_j[j] = lit ? lit_before + __popcll(lm & below) : surv_before + __popcll(sm & below);
         surv_before += __popcll(sm);
         lit_before += __popcll(lm);
     }
     wave_sync();
-    for (int j = 0; j < nk; ++j) w.L->ord[keep_pos[j]] = keep_slot[j];
-    for (int j = 0; j < nl; ++j) w.L->freel[w.nfree + lit_pos[j]] = lit_slot[j];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (!valid_j[j]) continue;
+        if (lit_j[j]) w.L->freel[w.nfree + pos_j[j]] = slot_j[j];
+        else w.L->ord[pos_j[j]] = slot_j[j];
+    }
     int surv = surv_before;
     w.nfree += lit_before;
     wave_sync();

@@ -1042,15 +1042,20 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
 
 // Sort keys for the colour batches: (colour << 32) | arbiter index; inactive arbiters sort last.
 // Also counts the contact points that the solve will visit.
+// Solve-order keys: (colour << 2 | contacts - 1) << 32 | arbiter.  Within a colour the order is free
+// (no two arbiters of one colour share a non-static body), so arbiters are grouped by contact count:
+// the dataflow solver's 64-arbiter chunks then run the same number of contact rows in every lane.
+// Uncoloured (unsolved) arbiters sort last.
 __global__ void k_colour_keys(const int* __restrict__ colour, const jh_arbiter* __restrict__ arb,
                               const int* __restrict__ a_count, uint64_t* __restrict__ keys, int* __restrict__ contacts) {
     int A = *a_count;
     int local = 0;
     for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < A; a += gridDim.x * blockDim.x) {
         int c = colour[a];
-        uint32_t cc = c < 0 ? 255u : uint32_t(c);
+        int n = __popc(arb[a].usage_mask & 15u);
+        uint32_t cc = c < 0 ? 1023u : (uint32_t(c) << 2) | uint32_t(max(n, 1) - 1);
         keys[a] = (uint64_t(cc) << 32) | uint32_t(a);
-        if (c >= 0) local += __popc(arb[a].usage_mask & 15u);
+        if (c >= 0) local += n;
     }
     if (local) atomicAdd(contacts, local);
 }
@@ -1220,7 +1225,8 @@ __device__ __forceinline__ void arbiter_pass(jh_arbiter& a, BodyReg& B1, BodyReg
 }
 
 // Colour-ordered structure-of-arrays copy of the contact rows for the dataflow solver: position k of
-// the solve order holds arbiter order[k]; field f of slot s lives at f_[(s * kSoaFields + f) * cap + k],
+// the solve order holds arbiter order[k]; row r is its r-th used contact slot (used slots compacted in
+// slot order; usage mask (1 << n) - 1), and field f of row r lives at f_[(r * kSoaFields + f) * cap + k],
 // so the 64 lanes of a chunk (consecutive positions) read each field with one contiguous access.
 // Gathered from the arbiters after colouring, scattered back (the fields the solve writes) after it.
 constexpr int kSoaFields = 29;  // bias, penalty, normal 3, tangent1 3, tangent2 3, mass 3, acc 3, rel1 3, rel2 3, pos1 3, pos2 3
@@ -1330,25 +1336,27 @@ __global__ void __launch_bounds__(256) k_soa_gather(const jh_arbiter* __restrict
     int A = offs[128];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
         const jh_arbiter& a = arb[uint32_t(order[k])];
+        int n = __popc(a.usage_mask & 15u);
         S.bodies[k] = make_int2(a.body1, a.body2);
-        S.um[k] = make_uint2(a.usage_mask, a.mode);
+        S.um[k] = make_uint2((1u << n) - 1u, a.mode);  // used slots compacted to rows 0 .. n-1, in slot order
         S.fs[k] = make_float2(a.friction, a.speculative_relaxation_factor);
-        for (int s = 0; s < 4; ++s) {
+        for (int j = 0, s = 0; s < 4; ++s) {
             if (!(a.usage_mask & (1u << s))) continue;
             const jh_contact& c = a.contacts[s];
-            S.flag[size_t(s) * S.cap + k] = c.flag;
-            S.at(s, kFBias, k) = c.bias;
-            S.at(s, kFPen, k) = c.penalty_bias;
+            int r = j++;
+            S.flag[size_t(r) * S.cap + k] = c.flag;
+            S.at(r, kFBias, k) = c.bias;
+            S.at(r, kFPen, k) = c.penalty_bias;
             for (int l = 0; l < 3; ++l) {
-                S.at(s, kFN + l, k) = c.normal[l];
-                S.at(s, kFT1 + l, k) = c.tangent1[l];
-                S.at(s, kFT2 + l, k) = c.tangent2[l];
-                S.at(s, kFMass + l, k) = c.mass_normal_tangent[l];
-                S.at(s, kFAcc + l, k) = c.accumulated[l];
-                S.at(s, kFRel1 + l, k) = c.relative_position1[l];
-                S.at(s, kFRel2 + l, k) = c.relative_position2[l];
-                S.at(s, kFPos1 + l, k) = c.position1[l];
-                S.at(s, kFPos2 + l, k) = c.position2[l];
+                S.at(r, kFN + l, k) = c.normal[l];
+                S.at(r, kFT1 + l, k) = c.tangent1[l];
+                S.at(r, kFT2 + l, k) = c.tangent2[l];
+                S.at(r, kFMass + l, k) = c.mass_normal_tangent[l];
+                S.at(r, kFAcc + l, k) = c.accumulated[l];
+                S.at(r, kFRel1 + l, k) = c.relative_position1[l];
+                S.at(r, kFRel2 + l, k) = c.relative_position2[l];
+                S.at(r, kFPos1 + l, k) = c.position1[l];
+                S.at(r, kFPos2 + l, k) = c.position2[l];
             }
         }
     }
@@ -1359,16 +1367,17 @@ __global__ void __launch_bounds__(256) k_soa_scatter(jh_arbiter* __restrict__ ar
     int A = offs[128];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
         jh_arbiter& a = arb[uint32_t(order[k])];
-        uint32_t usage = S.um[k].x;
-        for (int s = 0; s < 4; ++s) {
+        uint32_t usage = a.usage_mask & 15u;  // row r holds the r-th used slot
+        for (int j = 0, s = 0; s < 4; ++s) {
             if (!(usage & (1u << s))) continue;
+            int r = j++;
             jh_contact& c = a.contacts[s];
-            c.flag = S.flag[size_t(s) * S.cap + k];
-            c.bias = S.at(s, kFBias, k);
-            c.penalty_bias = S.at(s, kFPen, k);
+            c.flag = S.flag[size_t(r) * S.cap + k];
+            c.bias = S.at(r, kFBias, k);
+            c.penalty_bias = S.at(r, kFPen, k);
             for (int l = 0; l < 3; ++l) {
-                c.mass_normal_tangent[l] = S.at(s, kFMass + l, k);
-                c.accumulated[l] = S.at(s, kFAcc + l, k);
+                c.mass_normal_tangent[l] = S.at(r, kFMass + l, k);
+                c.accumulated[l] = S.at(r, kFAcc + l, k);
             }
         }
     }
@@ -1726,6 +1735,228 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
     }
 }
 
+// ---- Lane solver ----------------------------------------------------------------------------
+// The dataflow protocol of k_solve_flow with one sequence per lane instead of one per wave: lane L
+// of the grid visits colour-order positions L, L + NL, L + 2 NL, ... (NL lanes in the grid), pass
+// after pass, each as soon as its two bodies show the awaited counts, independently of the other
+// lanes of its wave.  Deadlock-free on a cooperative grid for the same reason as k_solve_flow: the
+// earliest unfinished visit is at the head of its lane's sequence with all its predecessors done.
+// Each position's data is one contiguous record of 16-byte vectors, so a lane's loads stay
+// vectorised whatever position its neighbours are at.
+//   header h0 (b1, b2, flags, friction)  flags = usage | mode << 4 | non-static1 << 8 | non-static2 << 9
+//          h1 (deg1, rank1, deg2, rank2)  h2 (speculative relaxation factor, -, -, -)
+//   slot s  s0 (flag, bias, penalty, mass.n)  s1 (mass.t1, mass.t2, acc.n, acc.t1)
+//           s2 (acc.t2, n.x, n.y, n.z)        s3 (t1.x, t1.y, t1.z, t2.x)
+//           s4 (t2.y, t2.z, rel1.x, rel1.y)   s5 (rel1.z, rel2.x, rel2.y, rel2.z)
+//           s6 (pos1.x, pos1.y, pos1.z, pos2.x)  s7 (pos2.y, pos2.z, -, -)
+constexpr int kRecHdr = 3, kRecSlot = 8, kRecVec = kRecHdr + 4 * kRecSlot;
+
+__global__ void __launch_bounds__(256) k_rec_gather(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
+                                                    const int* __restrict__ offs, const int2* __restrict__ rank,
+                                                    const int* __restrict__ deg, const int* __restrict__ motion,
+                                                    float4* __restrict__ rec) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        const jh_arbiter& a = arb[uint32_t(order[k])];
+        float4* r = rec + size_t(k) * kRecVec;
+        bool m1 = motion[a.body1] != JH_MOTION_STATIC, m2 = motion[a.body2] != JH_MOTION_STATIC;
+        uint32_t flags = (a.usage_mask & 15u) | ((a.mode & 15u) << 4) | (m1 ? 256u : 0u) | (m2 ? 512u : 0u);
+        int2 rk = rank[k];
+        r[0] = make_float4(__int_as_float(a.body1), __int_as_float(a.body2), __uint_as_float(flags), a.friction);
+        r[1] = make_float4(__int_as_float(m1 ? deg[a.body1] : 0), __int_as_float(rk.x),
+                           __int_as_float(m2 ? deg[a.body2] : 0), __int_as_float(rk.y));
+        r[2] = make_float4(a.speculative_relaxation_factor, 0.0f, 0.0f, 0.0f);
+        for (int s = 0; s < 4; ++s) {
+            if (!(a.usage_mask & (1u << s))) continue;
+            const jh_contact& c = a.contacts[s];
+            float4* q = r + kRecHdr + s * kRecSlot;
+            q[0] = make_float4(__uint_as_float(c.flag), c.bias, c.penalty_bias, c.mass_normal_tangent[0]);
+            q[1] = make_float4(c.mass_normal_tangent[1], c.mass_normal_tangent[2], c.accumulated[0], c.accumulated[1]);
+            q[2] = make_float4(c.accumulated[2], c.normal[0], c.normal[1], c.normal[2]);
+            q[3] = make_float4(c.tangent1[0], c.tangent1[1], c.tangent1[2], c.tangent2[0]);
+            q[4] = make_float4(c.tangent2[1], c.tangent2[2], c.relative_position1[0], c.relative_position1[1]);
+            q[5] = make_float4(c.relative_position1[2], c.relative_position2[0], c.relative_position2[1],
+                               c.relative_position2[2]);
+            q[6] = make_float4(c.position1[0], c.position1[1], c.position1[2], c.position2[0]);
+            q[7] = make_float4(c.position2[1], c.position2[2], 0.0f, 0.0f);
+        }
+    }
+}
+__global__ void __launch_bounds__(256) k_rec_scatter(jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
+                                                     const int* __restrict__ offs, const float4* __restrict__ rec) {
+    int A = offs[128];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        jh_arbiter& a = arb[uint32_t(order[k])];
+        const float4* r = rec + size_t(k) * kRecVec;
+        for (int s = 0; s < 4; ++s) {
+            if (!(a.usage_mask & (1u << s))) continue;
+            jh_contact& c = a.contacts[s];
+            const float4* q = r + kRecHdr + s * kRecSlot;
+            float4 q0 = q[0], q1 = q[1], q2 = q[2];
+            c.flag = __float_as_uint(q0.x);
+            c.bias = q0.y;
+            c.penalty_bias = q0.z;
+            c.mass_normal_tangent[0] = q0.w;
+            c.mass_normal_tangent[1] = q1.x;
+            c.mass_normal_tangent[2] = q1.y;
+            c.accumulated[0] = q1.z;
+            c.accumulated[1] = q1.w;
+            c.accumulated[2] = q2.x;
+        }
+    }
+}
+
+__device__ __forceinline__ void load_slot_rec(const float4* q, SlotReg& r, bool prepare) {
+    float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
+    r.flag = __float_as_uint(q0.x);
+    r.bias = q0.y;
+    r.penalty = q0.z;
+    r.mass[0] = q0.w; r.mass[1] = q1.x; r.mass[2] = q1.y;
+    r.acc[0] = q1.z; r.acc[1] = q1.w; r.acc[2] = q2.x;
+    r.nx[0] = q2.y; r.ny[0] = q2.z; r.nz[0] = q2.w;
+    r.nx[1] = q3.x; r.ny[1] = q3.y; r.nz[1] = q3.z;
+    r.nx[2] = q3.w; r.ny[2] = q4.x; r.nz[2] = q4.y;
+    r.rel1 = vec3{q4.z, q4.w, q5.x};
+    r.rel2 = vec3{q5.y, q5.z, q5.w};
+    if (prepare) {
+        float4 q6 = q[6], q7 = q[7];
+        r.pos1 = vec3{q6.x, q6.y, q6.z};
+        r.pos2 = vec3{q6.w, q7.x, q7.y};
+    }
+}
+__device__ __forceinline__ void store_slot_rec(float4* q, const SlotReg& r, bool prepare) {
+    if (prepare) q[0] = make_float4(__uint_as_float(r.flag), r.bias, r.penalty, r.mass[0]);
+    q[1] = make_float4(r.mass[1], r.mass[2], r.acc[0], r.acc[1]);
+    q[2] = make_float4(r.acc[2], r.nx[0], r.ny[0], r.nz[0]);
+}
+
+// This lane's sequence over passes [g_first, g_first + npass) of one phase: the first pass prepares
+// when prep_first, the others iterate (with the penalty bias when `bias`).
+__device__ __forceinline__ void lane_passes(const SolveArgs& P, float4* __restrict__ rec, int A, int g_first,
+                                            int npass, bool prep_first, bool bias) {
+    const Bodies& B = P.B;
+    int NL = gridDim.x * blockDim.x;
+    int L = blockIdx.x * blockDim.x + threadIdx.x;
+    int per = A > L ? (A - L + NL - 1) / NL : 0;  // positions of this lane per pass
+    int p = 0, i = 0;                              // current pass and position index
+    bool finished = per == 0 || npass == 0;
+    bool loaded = false, ok1 = true, ok2 = true, prep = false, m1 = false, m2 = false;
+    int k = 0, b1 = 0, b2 = 0, need1 = 0, need2 = 0;
+    uint32_t usage = 0, mode = 0;
+    float friction = 0.0f, specrel = 0.0f;
+    SlotReg sr[4];
+    BodyReg R1, R2;
+    vec3 x1{0, 0, 0}, x2{0, 0, 0};
+    quat q1{0, 0, 0, 1}, q2{0, 0, 0, 1};
+    __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
+    while (true) {
+        asm volatile("" ::: "memory");
+        if (!finished && !loaded) {
+            k = L + i * NL;
+            prep = prep_first && p == 0;
+            const float4* r = rec + size_t(k) * kRecVec;
+            float4 h0 = r[0], h1 = r[1];
+            b1 = __float_as_int(h0.x);
+            b2 = __float_as_int(h0.y);
+            uint32_t fl = __float_as_uint(h0.z);
+            friction = h0.w;
+            usage = fl & 15u;
+            mode = (fl >> 4) & 15u;
+            m1 = (fl & 256u) != 0;
+            m2 = (fl & 512u) != 0;
+            int g = g_first + p;
+            need1 = g * __float_as_int(h1.x) + __float_as_int(h1.y);
+            need2 = g * __float_as_int(h1.z) + __float_as_int(h1.w);
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (usage & (1u << s)) load_slot_rec(r + kRecHdr + s * kRecSlot, sr[s], prep);
+            load_static_part(B, b1, R1);
+            load_static_part(B, b2, R2);
+            if (prep) {
+                specrel = r[2].x;
+                x1 = ld3(B.pos[b1]);
+                x2 = ld3(B.pos[b2]);
+                q1 = ldq(B.quat[b1]);
+                q2 = ldq(B.quat[b2]);
+            }
+            if (!m1) ld_vw(P, b1, R1.v, R1.w);
+            if (!m2) ld_vw(P, b2, R2.v, R2.w);
+            ok1 = !m1;
+            ok2 = !m2;
+            loaded = true;
+        }
+        bool progressed = false;
+        if (loaded) {
+            if (!ok1) ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w);
+            if (!ok2) ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w);
+            if (ok1 && ok2) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (!(usage & (1u << s))) continue;
+                    if (prep) prepare_reg(mode, specrel, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
+                    else iterate_reg(mode, friction, sr[s], R1, R2, bias);
+                }
+                if (m1) coh_publish_vw(vr, b1, R1.v, R1.w, need1 + 1);
+                if (m2) coh_publish_vw(vr, b2, R2.v, R2.w, need2 + 1);
+                float4* r = rec + size_t(k) * kRecVec;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    if (usage & (1u << s)) store_slot_rec(r + kRecHdr + s * kRecSlot, sr[s], prep);
+                loaded = false;
+                progressed = true;
+                if (++i == per) {
+                    i = 0;
+                    if (++p == npass) finished = true;
+                }
+            }
+        }
+        if (__all(finished)) break;
+        if (!__any(progressed)) __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// One launch per step, like k_solve_flow, with per-lane sequences in the dataflow phases.
+__global__ void __launch_bounds__(256) k_solve_lanes(SolveArgs P, float4* __restrict__ rec) {
+    int stride = gridDim.x * blockDim.x;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    int per_sub = 1 + P.iterations + P.relaxations;
+    int A = P.offs[128];
+    const Bodies& B = P.B;
+    for (int b = t0; b < P.nb; b += stride) st_vw(P, b, ld3(B.vel[b]), ld3(B.angvel[b]), 0);
+    for (int s = 0; s < P.ts.substeps; ++s) {
+        for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
+            if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
+            vec3 v, w;
+            int n = ld_vw(P, b, v, w);
+            w = w + ld3(B.dangvel[b]);
+            v = v + ld3(B.dvel[b]);
+            st_vw(P, b, v, w, n);
+        }
+        grid_barrier(P.bar);
+        lane_passes(P, rec, A, s * per_sub, 1 + P.iterations, true, true);
+        grid_barrier(P.bar);
+        for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
+            if (B.motion[b] == JH_MOTION_STATIC) continue;
+            vec3 v, w;
+            ld_vw(P, b, v, w);
+            B.pos[b] = st3(ld3(B.pos[b]) + v * P.ts.substep_dt);
+            quat rq = rotation_quaternion(w, P.ts.substep_dt);
+            B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+        }
+        grid_barrier(P.bar);
+        if (P.relaxations > 0) {
+            lane_passes(P, rec, A, s * per_sub + 1 + P.iterations, P.relaxations, false, false);
+            grid_barrier(P.bar);
+        }
+    }
+    for (int b = t0; b < P.nb; b += stride) {
+        vec3 v, w;
+        ld_vw(P, b, v, w);
+        B.vel[b] = st3(v);
+        B.angvel[b] = st3(w);
+    }
+}
+
 // ---- World-local solver --------------------------------------------------------------------
 // For batches of many independent worlds: one workgroup solves one world at a time (worlds taken from
 // a queue, largest first), walking that world's arbiters colour by colour with a workgroup barrier
@@ -1895,7 +2126,7 @@ __global__ void k_rank_keys(const jh_arbiter* __restrict__ arb, const uint64_t* 
 }
 // After sorting the rank keys: rank = position within the body's segment; deg = segment length.
 __global__ void k_rank_assign(const uint64_t* __restrict__ keys, const int* __restrict__ nkeys,
-                              int2* __restrict__ rank, int* __restrict__ deg) {
+                              int2* __restrict__ rank, int* __restrict__ deg, int* __restrict__ max_deg) {
     int n = *nkeys;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t key = keys[i];
@@ -1905,7 +2136,11 @@ __global__ void k_rank_assign(const uint64_t* __restrict__ keys, const int* __re
         int k = int(pos >> 1);
         if (pos & 1u) rank[k].y = i - lo;
         else rank[k].x = i - lo;
-        if (i == lo) deg[b] = lower_bound_u64(keys, n, (b + 1) << 32) - lo;
+        if (i == lo) {
+            int d = lower_bound_u64(keys, n, (b + 1) << 32) - lo;
+            deg[b] = d;
+            atomicMax(max_deg, d);
+        }
     }
 }
 // Chunks of at most 64 arbiters of one colour; the serial overflow colour gets one arbiter per chunk.
@@ -1913,7 +2148,7 @@ __global__ void k_chunks(const uint64_t* __restrict__ order, const int* __restri
                          int2* __restrict__ chunks) {
     int A = offs[128];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
-        int c = int(order[k] >> 32);
+        int c = int(order[k] >> 34);
         int rel = k - offs[c];
         if (c == kMaxColours) chunks[cbeg[c] + rel] = make_int2(k, k + 1);
         else if ((rel & 63) == 0) chunks[cbeg[c] + (rel >> 6)] = make_int2(k, min(k + 64, offs[c + 1]));

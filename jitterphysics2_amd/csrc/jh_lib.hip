@@ -82,8 +82,12 @@ enum {
     C_NMOV = 16,      // tile solver: non-static bodies with shapes
     C_TCELLS = 17,    // tile solver: (tile, colour) cells + 1
     C_SERR = 18,      // tile solver: a dependency wait gave up
-    C_COUNT = 20
+    C_MAXDEG = 19,    // most solved arbiters on one non-static body
+    C_COUNT = 24
 };
+
+// Solver kernels (jh_stats.solver, jh_tuning.solver).
+enum { SOLVER_BARRIER = 0, SOLVER_FLOW = 1, SOLVER_WORLDS = 2, SOLVER_TILES = 3, SOLVER_LANES = 4 };
 
 constexpr int kEpaMaxThreads = 32768;  // polytopes resident at once (epa::kBytesPerThread each)
 
@@ -189,14 +193,17 @@ struct jh_world {
     DBuf<float> soa_f;
     int soa_cap = 0;
     // world-local solver (many independent worlds)
-    bool world_mode = false;
     int n_worlds = 0, world_blocks = 0;
     DBuf<int> wbody, wboff, wsched, wcells, wqueue;
     DBuf<uint32_t> whist, woff, wcur, wsums;
     DBuf<uint64_t> wlist;
     DBuf<float4> vwpack;  // dataflow solver: packed velocities, 2 per body
+    // solver kernel: SOLVER_FLOW unless chosen otherwise (environment, world count, autotuner)
+    int kind = SOLVER_FLOW;
+    // lane solver (k_solve_lanes): workgroups and arbiter records
+    int lane_blocks = 0, lane_max = 0;
+    DBuf<float4> rec;
     // tile solver (one workgroup per Morton range of bodies, private bodies in LDS)
-    bool tile_mode = false;
     int tiles = 0;            // tiles = workgroups of k_solve_tiles
     int n_moving = 0;         // non-static bodies (upper bound of the tiled bodies)
     int tile_occ_lds = -1, tile_occ = 0;  // occupancy cache: LDS bytes -> resident workgroups per CU
@@ -210,7 +217,6 @@ struct jh_world {
     DBuf<int2> rank, chunks;
     DBuf<int> deg, cbeg;
     DBuf<unsigned> bar;
-    bool flow_solver = true;
     bool recolour = true;  // next colouring ignores the colours stored in the arbiters
     int solve_count = 0, solve_batches = 0;
     std::vector<int> solve_offsets;
@@ -242,7 +248,7 @@ namespace {
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
     if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK ||
-        t == C_SERR)
+        t == C_SERR || t == C_MAXDEG)
         cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
     else if (t < 6) bounds[t] = 0u;
@@ -360,7 +366,7 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->akey[1 - w->cur].ensure(na));
     HIPCHK(w->colour.ensure(na));
     HIPCHK(w->cwork.ensure(2 * na));
-    if (w->flow_solver && int(na) > w->soa_cap) {
+    if (w->kind != SOLVER_BARRIER && int(na) > w->soa_cap) {
         size_t cap = na + na / 4;
         HIPCHK(w->soa_bodies.ensure(cap));
         HIPCHK(w->soa_um.ensure(cap));
@@ -380,7 +386,8 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->offs.ensure(na));
     HIPCHK(w->sums.ensure(na / 256 + 2));
     HIPCHK(w->hist.ensure(rsort::hist_entries(2 * na)));  // the rank-key sort holds up to 2 keys per arbiter
-    if (w->tile_mode) HIPCHK(w->tlist.ensure(na));
+    if (w->kind == SOLVER_TILES) HIPCHK(w->tlist.ensure(na));
+    if (w->kind == SOLVER_LANES) HIPCHK(w->rec.ensure(na * kRecVec));
     return JH_OK;
 }
 
@@ -616,8 +623,10 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     a_cur = a_nxt;
     HIPCHK(hipEventRecord(w->ev[3], st));
     // colouring of the arbiters (graph edges between dynamic bodies)
-    size_t tile_lds = (w->flow_solver && w->tile_mode && !w->world_mode) ? tile_lds_bytes(w, w->tiles) : 0;
-    bool use_tiles = tile_lds > 0;
+    // the tile solver needs its tiles to fit; otherwise this step runs the dataflow solver
+    size_t tile_lds = w->kind == SOLVER_TILES ? tile_lds_bytes(w, w->tiles) : 0;
+    int kind = (w->kind == SOLVER_TILES && tile_lds == 0) ? SOLVER_FLOW : w->kind;
+    bool use_tiles = kind == SOLVER_TILES;
     {
         ColourArgs P;
         P.arb = w->arb[cur].p;
@@ -643,13 +652,13 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         if (s != JH_OK) return s;
         k_colour_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(w->colour.p, w->arb[cur].p, a_cur, w->ok0.p,
                                                                      cnt + C_CONTACTS);
-        int csh[1] = {32};
-        bool in1 = rsort::sort_pairs(w->ok0.p, w->ok1.p, nullptr, nullptr, a_cur, std::max(amax_, 1), csh, 1,
+        int csh[2] = {32, 40};  // (colour, contacts) in bits 32..41
+        bool in1 = rsort::sort_pairs(w->ok0.p, w->ok1.p, nullptr, nullptr, a_cur, std::max(amax_, 1), csh, 2,
                                      w->hist.p, st);
         k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p, w->cbeg.p, cnt + C_NCHUNK);
         const uint64_t* order = in1 ? w->ok1.p : w->ok0.p;
         k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
-        if (w->flow_solver && !w->world_mode) {
+        if (kind == SOLVER_FLOW || kind == SOLVER_TILES || kind == SOLVER_LANES) {
             // per-body update sequence: rank of each arbiter visit and body degree
             int ga = grid_for(std::max(amax_, 1));
             k_set_int<<<1, 1, 0, st>>>(cnt + C_NRANK, 0);
@@ -663,7 +672,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             bool rin1 = rsort::sort_pairs(w->rk0.p, w->rk1.p, nullptr, nullptr, cnt + C_NRANK, 2 * std::max(amax_, 1),
                                           rsh, nrs, w->hist.p, st);
             k_rank_assign<<<grid_for(2 * std::max(amax_, 1)), 256, 0, st>>>(rin1 ? w->rk1.p : w->rk0.p, cnt + C_NRANK,
-                                                                            w->rank.p, w->deg.p);
+                                                                            w->rank.p, w->deg.p, cnt + C_MAXDEG);
             k_chunks<<<ga, 256, 0, st>>>(order, w->coffs.p, w->cbeg.p, w->chunks.p);
             if (use_tiles) {
                 s = tile_setup(w, order, a_cur, amax_);
@@ -703,7 +712,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             HIPCHK(hipGetLastError());
         }
         void* sargs[] = {&S};
-        if (w->flow_solver && w->world_mode) {
+        if (kind == SOLVER_WORLDS) {
             // world-ordered arbiter list: (world, colour) histogram -> offsets -> scatter
             int gsoa = grid_for(std::max(amax_, 1));
             size_t cells = size_t(w->n_worlds) * 128 + 1;
@@ -745,7 +754,14 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
                                               unsigned(tile_lds), st));
             k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, w->tlist.p, w->coffs.p, S.soa);
             HIPCHK(hipGetLastError());
-        } else if (w->flow_solver) {
+        } else if (kind == SOLVER_LANES) {
+            int gsoa = grid_for(std::max(amax_, 1));
+            k_rec_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, w->rank.p, w->deg.p, w->motion.p, w->rec.p);
+            void* largs[] = {&S, &w->rec.p};
+            s = launch_coop((const void*)k_solve_lanes, w->lane_blocks, 256, largs, st);
+            if (s != JH_OK) return s;
+            k_rec_scatter<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, w->rec.p);
+        } else if (kind == SOLVER_FLOW) {
             int gsoa = grid_for(std::max(amax_, 1));
             k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
             s = launch_coop((const void*)k_solve_flow, w->flow_blocks, 256, sargs, st);
@@ -790,7 +806,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     S.contacts = w->hcnt[C_CONTACTS];
     S.colours = w->hcnt[C_NCOL];
     S.epa_calls = w->hcnt[C_DEEP];
-    S.solver = !w->flow_solver ? 0 : w->world_mode ? 2 : use_tiles ? 3 : 1;
+    S.max_degree = w->hcnt[C_MAXDEG];
+    S.solver = kind;
     S.steps++;
     float ms[7];
     for (int k = 0; k < 6; ++k) HIPCHK(hipEventElapsedTime(&ms[k], w->ev[k], w->ev[k + 1]));
@@ -891,10 +908,13 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     HIPCHK(hipMemset(w->bar.p, 0, 64 * sizeof(unsigned)));
     HIPCHK(w->cbeg.ensure(130));
     // launch-shape overrides for tuning sweeps (clamped to the co-resident maximum)
-    if (const char* e = getenv("JH_SOLVER")) w->flow_solver = strcmp(e, "barrier") != 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_lanes, 256, 0));
+    w->lane_max = std::max(1, occ) * w->cu_count;
+    w->lane_blocks = std::min(w->lane_max, w->cu_count);
     if (const char* e = getenv("JH_SOLVE_BLOCKS")) {
         w->coop_solve = std::max(1, std::min(atoi(e), w->coop_solve));
-        w->flow_blocks = std::max(1, std::min(atoi(e), std::max(1, occ) * w->cu_count));
+        w->flow_blocks = std::max(1, std::min(atoi(e), w->flow_max));
+        w->lane_blocks = std::max(1, std::min(atoi(e), w->lane_max));
     }
     w->tiles = w->cu_count;
     if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
@@ -927,7 +947,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->whist.release(); w->woff.release(); w->wcur.release(); w->wsums.release(); w->wlist.release();
     w->tfirst.release(); w->tbody.release(); w->tile_of.release(); w->tslot.release(); w->tflags.release();
     w->trank.release(); w->tshared.release(); w->thist.release(); w->toff.release(); w->tcur.release();
-    w->tsums.release(); w->tlist.release();
+    w->tsums.release(); w->tlist.release(); w->rec.release();
     w->vwpack.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
@@ -977,8 +997,12 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
                          [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
         w->n_worlds = W;
         const char* e = getenv("JH_SOLVER");
-        w->world_mode = w->flow_solver && (e ? strcmp(e, "world") == 0 : W >= 64);
-        w->tile_mode = w->flow_solver && e && strcmp(e, "tiles") == 0;
+        if (e && strcmp(e, "barrier") == 0) w->kind = SOLVER_BARRIER;
+        else if (e && strcmp(e, "world") == 0) w->kind = SOLVER_WORLDS;
+        else if (e && strcmp(e, "tiles") == 0) w->kind = SOLVER_TILES;
+        else if (e && strcmp(e, "lanes") == 0) w->kind = SOLVER_LANES;
+        else if (e && strcmp(e, "flow") == 0) w->kind = SOLVER_FLOW;
+        else w->kind = W >= 64 ? SOLVER_WORLDS : SOLVER_FLOW;
         w->n_moving = 0;
         for (int i = 0; i < n; ++i) w->n_moving += bodies[i].motion_type != JH_MOTION_STATIC;
         if (W > 0) {
@@ -1594,77 +1618,73 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         v.erase(std::unique(v.begin(), v.end()), v.end());
         return v;
     };
-    std::vector<int> sc = w->flow_solver ? uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->flow_max)
-                                         : uniq({cu / 2, cu, 2 * cu, 4 * cu, 8 * cu}, w->coop_solve);
-    std::vector<int> cc = uniq({16, 32, 64, 128, 256, 512}, w->coop_colour);
-    int& solve_knob = w->flow_solver ? w->flow_blocks : w->coop_solve;
-    int solve_default = solve_knob, colour_default = w->colour_blocks;
-    // round-robin over the candidates so that the evolving scene weighs them equally
-    auto tune = [&](std::vector<int>& cand, int& knob, bool solve, double& best_ms, double& default_ms,
-                    int def) -> jh_status {
-        std::vector<double> tot(cand.size(), 0.0);
-        for (int r = 0; r < rounds; ++r)
-            for (size_t i = 0; i < cand.size(); ++i) {
-                knob = cand[i];
-                jh_status s = do_step(w, dt, nullptr);
-                if (s != JH_OK) return s;
-                tot[i] += solve ? w->stats.ms_solve : w->stats.ms_colouring;
-            }
-        size_t bi = 0;
-        for (size_t i = 1; i < cand.size(); ++i)
-            if (tot[i] < tot[bi]) bi = i;
-        knob = cand[bi];
-        best_ms = tot[bi] / rounds;
-        default_ms = best_ms;
-        for (size_t i = 0; i < cand.size(); ++i)
-            if (cand[i] == def) default_ms = tot[i] / rounds;
-        return JH_OK;
-    };
-    // many worlds: the world-local solver is one more candidate (encoded as solve shape 0); the tile
-    // solver is a candidate for every tile count whose tiles fit (encoded as minus the tile count)
-    bool world_default = w->world_mode;
-    int solver_default = w->world_mode ? 0 : (w->tile_mode ? -w->tiles : 0);
-    if (w->flow_solver && w->n_worlds > 1) sc.insert(sc.begin(), 0);
-    if (w->flow_solver)
+    // solver candidates (kind, workgroups); the barrier solver only tunes its own grid
+    struct Cand { int kind, n; };
+    std::vector<Cand> sc;
+    if (w->kind == SOLVER_BARRIER) {
+        for (int n : uniq({cu / 2, cu, 2 * cu, 4 * cu, 8 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n});
+    } else {
+        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->flow_max)) sc.push_back({SOLVER_FLOW, n});
+        for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n});
         for (int T : {cu / 2, cu, 2 * cu})
-            if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back(-T);
-    std::vector<int> flow_shapes = sc;
-    // candidates: 0 = world-local solver, > 0 = dataflow solver with that many workgroups, < 0 = tiles
-    int keep_tiles = w->tiles;
-    auto apply = [&](int v) {
-        w->world_mode = v == 0;
-        w->tile_mode = v < 0;
-        if (v > 0) solve_knob = v;
-        if (v < 0) w->tiles = -v;
+            if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back({SOLVER_TILES, T});
+        if (w->n_worlds > 1) sc.push_back({SOLVER_WORLDS, w->world_blocks});
+    }
+    auto width = [&](int kind) -> int& {
+        return kind == SOLVER_BARRIER ? w->coop_solve : kind == SOLVER_LANES ? w->lane_blocks
+             : kind == SOLVER_TILES ? w->tiles : kind == SOLVER_WORLDS ? w->world_blocks : w->flow_blocks;
     };
-    int keep_solve = solve_knob;
+    Cand def{w->kind, width(w->kind)};
+    int keep[5];
+    for (int k = 0; k < 5; ++k) keep[k] = width(k);
+    auto apply = [&](Cand c) {
+        w->kind = c.kind;
+        width(c.kind) = c.n;
+    };
+    // round-robin over the candidates so that the evolving scene weighs them equally
     {
-        std::vector<double> tot(flow_shapes.size(), 0.0);
+        std::vector<double> tot(sc.size(), 0.0);
         for (int r = 0; r < rounds; ++r)
-            for (size_t i = 0; i < flow_shapes.size(); ++i) {
-                apply(flow_shapes[i]);
+            for (size_t i = 0; i < sc.size(); ++i) {
+                apply(sc[i]);
                 jh_status st = do_step(w, dt, nullptr);
                 if (st != JH_OK) return st;
                 tot[i] += w->stats.ms_solve;
             }
         size_t bi = 0;
-        for (size_t i = 1; i < flow_shapes.size(); ++i)
+        for (size_t i = 1; i < sc.size(); ++i)
             if (tot[i] < tot[bi]) bi = i;
-        solve_knob = keep_solve;
-        w->tiles = keep_tiles;
-        apply(flow_shapes[bi]);
+        for (int k = 0; k < 5; ++k) width(k) = keep[k];
+        apply(sc[bi]);
         out->ms_solve_best = tot[bi] / rounds;
         out->ms_solve_default = out->ms_solve_best;
-        for (size_t i = 0; i < flow_shapes.size(); ++i)
-            if (flow_shapes[i] == (world_default ? 0 : solver_default < 0 ? solver_default : solve_default))
-                out->ms_solve_default = tot[i] / rounds;
+        for (size_t i = 0; i < sc.size(); ++i)
+            if (sc[i].kind == def.kind && sc[i].n == def.n) out->ms_solve_default = tot[i] / rounds;
     }
-    jh_status s = JH_OK;
-    s = tune(cc, w->colour_blocks, false, out->ms_colour_best, out->ms_colour_default, colour_default);
-    if (s != JH_OK) return s;
-    out->solve_blocks = w->world_mode ? 0 : w->tile_mode ? -w->tiles : solve_knob;  // 0: world-local, < 0: tiles
+    std::vector<int> cc = uniq({16, 32, 64, 128, 256, 512}, w->coop_colour);
+    int colour_default = w->colour_blocks;
+    {
+        std::vector<double> tot(cc.size(), 0.0);
+        for (int r = 0; r < rounds; ++r)
+            for (size_t i = 0; i < cc.size(); ++i) {
+                w->colour_blocks = cc[i];
+                jh_status st = do_step(w, dt, nullptr);
+                if (st != JH_OK) return st;
+                tot[i] += w->stats.ms_colouring;
+            }
+        size_t bi = 0;
+        for (size_t i = 1; i < cc.size(); ++i)
+            if (tot[i] < tot[bi]) bi = i;
+        w->colour_blocks = cc[bi];
+        out->ms_colour_best = tot[bi] / rounds;
+        out->ms_colour_default = out->ms_colour_best;
+        for (size_t i = 0; i < cc.size(); ++i)
+            if (cc[i] == colour_default) out->ms_colour_default = tot[i] / rounds;
+    }
+    out->solver = w->kind;
+    out->solve_blocks = width(w->kind);
     out->colour_blocks = w->colour_blocks;
-    out->candidates = int(flow_shapes.size() + cc.size());
+    out->candidates = int(sc.size() + cc.size());
     out->steps = rounds * out->candidates;
     return JH_OK;
 }

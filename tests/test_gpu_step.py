@@ -140,8 +140,8 @@ def test_autotune_keeps_results_exact():
     for _ in range(60):
         gw.step(dt)
     t = gw.autotune(dt, rounds=1)
-    assert t["solve_blocks"] != 0 and t["colour_blocks"] >= 1 and t["steps"] == t["candidates"]
-    assert gw.stats()["solver"] == (3 if t["solve_blocks"] < 0 else 1)
+    assert t["solve_blocks"] >= 1 and t["colour_blocks"] >= 1 and t["steps"] == t["candidates"]
+    assert gw.stats()["solver"] == t["solver"] and t["solver"] in (1, 3, 4)
     assert t["ms_solve_best"] <= t["ms_solve_default"] + 1e-9
     # continue from the tuned world's state: the oracle replay stays bit-exact with the new shapes
     ow = OracleWorld(gw.bodies(), shapes, verts, cfg)
@@ -200,6 +200,20 @@ def test_fp32_error_against_fp64_oracle():
     dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10]).max()
     print(f"FP32 GPU vs FP64 oracle after 30 steps: max |dp| {dp:.3e} m, max |dv| {dv:.3e} m/s")
     assert dp < 1e-4 and dv < 1e-3
+
+
+def test_lane_solver_bitexact(monkeypatch):
+    # one dataflow sequence per lane (positions L, L + lanes, ...): exact for any grid size
+    monkeypatch.setenv("JH_SOLVER", "lanes")
+    for blocks in ("1", "3", "64"):
+        monkeypatch.setenv("JH_SOLVE_BLOCKS", blocks)
+        gw, _ = run_replay(scenes.heap(900, seed=19), steps=50, check_every=49)
+        assert gw.stats()["solver"] == 4
+    monkeypatch.delenv("JH_SOLVE_BLOCKS")
+    run_replay(scenes.pyramid(rows=12), steps=30, check_every=10)
+    gw, _ = run_replay(scenes.batched_worlds(n_worlds=24, min_bodies=16, max_bodies=160, seed=23), steps=40,
+                       check_every=13)
+    assert gw.stats()["solver"] == 4
 
 
 def test_tile_solver_bitexact(monkeypatch):
