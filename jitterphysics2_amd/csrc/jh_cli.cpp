@@ -83,7 +83,7 @@ int main(int argc, char** argv) {
     // launch shapes as the benchmark picks them (2 autotune rounds, which advance the simulation)
     jh_tuning tune;
     if ((s = jh_world_autotune(w, dt, 2, &tune))) return die("jh_world_autotune", s);
-    printf("# autotune: solve blocks %d, colour blocks %d\n", tune.solve_blocks, tune.colour_blocks);
+    printf("# autotune: solver %d x %d, colour blocks %d\n", tune.solver, tune.solve_blocks, tune.colour_blocks);
     fflush(stdout);
     double sum[8] = {0};
     jh_stats st;

@@ -234,28 +234,54 @@ __device__ __forceinline__ bool init_tetrahedron(Poly& w, int lane) {
     return true;
 }
 
+// Wave minimum of a 64-bit key with DPP row shifts and row broadcasts (VALU only, no LDS traffic):
+// prefix minima within each 16-lane row, then rows folded into lane 63, which is read back.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long x) {
+    int lo = int(uint32_t(x)), hi = int(uint32_t(x >> 32));
+    lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xf, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xf, false);
+    return (unsigned long long)uint32_t(lo) | ((unsigned long long)uint32_t(hi) << 32);
+}
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
+    unsigned long long y;
+    y = dpp_u64<0x111, 0xf>(x); x = y < x ? y : x;  // row_shr:1
+    y = dpp_u64<0x112, 0xf>(x); x = y < x ? y : x;  // row_shr:2
+    y = dpp_u64<0x114, 0xf>(x); x = y < x ? y : x;  // row_shr:4
+    y = dpp_u64<0x118, 0xf>(x); x = y < x ? y : x;  // row_shr:8
+    y = dpp_u64<0x142, 0xa>(x); x = y < x ? y : x;  // row_bcast:15 into rows 1, 3
+    y = dpp_u64<0x143, 0xc>(x); x = y < x ? y : x;  // row_bcast:31 into rows 2, 3
+    uint32_t lo = __builtin_amdgcn_readlane(int(uint32_t(x)), 63);
+    uint32_t hi = __builtin_amdgcn_readlane(int(uint32_t(x >> 32)), 63);
+    return (unsigned long long)lo | ((unsigned long long)hi << 32);
+}
+
 // ConvexPolytope.GetClosestTriangle (:147-166): the first triangle (logical order) of minimum
-// distance; returns its slot.
+// distance; returns its slot.  Distances are finite and non-negative, so their bit patterns order
+// like the values: the minimum is a wave minimum of the bits, and the first logical index holding it
+// comes from ballots (lane l holds indices l + 64 j, its own first minimum in j).
 __device__ __forceinline__ int closest_triangle(Poly& w, int lane) {
-    double best = __longlong_as_double(0x7ff0000000000000LL);
-    int bi = 0x7fffffff;
+    unsigned long long best = 0x7ff0000000000000ull;  // +inf
+    int bj = 4;
     bool all_facing = true;
-    for (int i = lane; i < w.tcount; i += 64) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int i = j * 64 + lane;
+        if (i >= w.tcount) continue;
         int s = w.L->ord[i];
-        double d = w.L->dist[s];
+        unsigned long long d = (unsigned long long)__double_as_longlong(w.L->dist[s]);
         if (d < best) {
             best = d;
-            bi = i;
+            bj = j;
         }
         if (!(w.L->tri[s] >> 48)) all_facing = false;
     }
-    for (int o = 1; o < 64; o <<= 1) {
-        double ob = __shfl_xor(best, o, 64);
-        int oi = __shfl_xor(bi, o, 64);
-        if (ob < best || (ob == best && oi < bi)) {
-            best = ob;
-            bi = oi;
-        }
+    unsigned long long m = wave_min_u64(best);
+    int bi = 0;
+#pragma unroll
+    for (int j = 3; j >= 0; --j) {
+        unsigned long long mm = __ballot(bj == j && best == m);
+        if (mm) bi = j * 64 + __ffsll((long long)mm) - 1;
     }
     bool every = __all(all_facing);
     w.enclosed = every ? 1 : w.enclosed;

@@ -668,10 +668,17 @@ __global__ void __launch_bounds__(64) k_epa(const uint64_t* __restrict__ pairs, 
 
 // The wave-cooperative form: one wavefront (workgroup of 64) per deep pair, polytope in LDS, the
 // per-wave HBM area of jh_epa_wave.h in `side`.  Same results as k_epa bit for bit.
+// Result of the wave EPA for deep entry q; k_epa_finish turns it into the pair's manifold.
+struct EpaRes {
+    int ok;
+    float pen;
+    vec3 pa, pb, normal;
+};
+
 __global__ void __launch_bounds__(64) k_epa_wave(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
-                                                 const float* __restrict__ verts, Bodies B, int aux,
-                                                 jh_manifold* __restrict__ out, const DeepRec* __restrict__ deep,
-                                                 const int* __restrict__ deep_count, char* __restrict__ side) {
+                                                 const float* __restrict__ verts, Bodies B,
+                                                 const DeepRec* __restrict__ deep, const int* __restrict__ deep_count,
+                                                 char* __restrict__ side, EpaRes* __restrict__ res) {
     __shared__ epw::Lds L;
     int lane = threadIdx.x;
     epw::Poly w;
@@ -689,11 +696,38 @@ __global__ void __launch_bounds__(64) k_epa_wave(const uint64_t* __restrict__ pa
         vec3 pa, pb;
         pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
         for (int k = 0; k < 4; ++k) epw::set_vertex(w, k, d.portal[k], lane);
+        EpaRes r;
+        r.ok = epw::solve(w, mk, lane, r.pa, r.pb, r.normal, r.pen) ? 1 : 0;
+        if (lane == 0) res[q] = r;
+        epw::wave_sync();
+    }
+}
+
+// Manifolds of the deep pairs from the wave EPA's results, one thread per pair (the manifold and the
+// triangle filter are serial per pair; here every lane has a pair of its own).
+__global__ void __launch_bounds__(128) k_epa_finish(const uint64_t* __restrict__ pairs,
+                                                    const ShapeRec* __restrict__ shapes,
+                                                    const float* __restrict__ verts, Bodies B, int aux,
+                                                    jh_manifold* __restrict__ out, const DeepRec* __restrict__ deep,
+                                                    const int* __restrict__ deep_count,
+                                                    const EpaRes* __restrict__ res) {
+    int n = *deep_count;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        EpaRes e = res[q];
+        if (!e.ok) continue;  // EPA failed: the MPR answer stays
+        const DeepRec& d = deep[q];
+        uint64_t key = pairs[d.pair];
+        MinkowskiPair mk;
+        quat qa, qb;
+        vec3 pa, pb;
+        pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
         MprOut r;
         r.hit = 1;
-        bool ok = epw::solve(w, mk, lane, r.pa, r.pb, r.normal, r.pen);
-        if (ok && lane == 0) epa_finish(d, key, shapes, verts, B, mk, qa, qb, pa, pb, r, aux, out);
-        epw::wave_sync();
+        r.pa = e.pa;
+        r.pb = e.pb;
+        r.normal = e.normal;
+        r.pen = e.pen;
+        epa_finish(d, key, shapes, verts, B, mk, qa, qb, pa, pb, r, aux, out);
     }
 }
 

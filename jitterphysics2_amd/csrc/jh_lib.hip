@@ -170,6 +170,7 @@ struct jh_world {
     DBuf<jh_manifold> man;
     DBuf<DeepRec> deep;
     DBuf<char> epa_work, epa_side;
+    DBuf<EpaRes> epa_res;
     int epa_threads = 0;
     bool epa_wave = true;      // wave-cooperative EPA (k_epa_wave); false: one thread per pair (k_epa)
     int epa_wave_blocks = 0;   // resident single-wave workgroups of k_epa_wave
@@ -580,8 +581,11 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         if (w->epa_wave) {
             int blocks = std::max(1, std::min(w->epa_wave_blocks, np));
             HIPCHK(w->epa_side.ensure(size_t(w->epa_wave_blocks) * epw::kSideBytes));
-            k_epa_wave<<<blocks, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p, w->deep.p,
-                                              cnt + C_DEEP, w->epa_side.p);
+            HIPCHK(w->epa_res.ensure(std::max(np, 1)));
+            k_epa_wave<<<blocks, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, w->deep.p, cnt + C_DEEP,
+                                              w->epa_side.p, w->epa_res.p);
+            k_epa_finish<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p,
+                                                                  w->deep.p, cnt + C_DEEP, w->epa_res.p);
         } else {
             int et = std::min(kEpaMaxThreads, ((np + 63) / 64) * 64);
             if (et > w->epa_threads) {
@@ -937,7 +941,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->mk0.release(); w->mk1.release(); w->mv0.release(); w->mv1.release();
     w->left.release(); w->right.release(); w->parent.release(); w->leaf_parent.release(); w->nflags.release();
     w->range.release(); w->bounds.release(); w->hist.release(); w->pk0.release(); w->pk1.release();
-    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release();
+    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release(); w->epa_res.release();
     w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
