@@ -11,10 +11,10 @@
 //  * the horizon edge list is built in exactly the sequential order of insertions and removals, each
 //    step one wave-wide match search (ballot) over the list in LDS;
 //  * the new triangles of the horizon are created one per lane.
-// LDS per wave: vertices (positions), packed triangle indices, squared distances, plane filters,
-// order / free-slot / lit bytes, edge list: kLdsBytes (7 waves per CU fit in 160 KB).  The rest
-// (triangle normals and closest points, barycentric coordinates, vertex support points) is written
-// once per triangle / vertex and read rarely: it lives in a per-wave HBM area.
+// LDS per wave: vertices (positions), packed triangle indices, squared distances, order / free-slot /
+// lit bytes: kLdsBytes (6.3 KB).  The rest (plane filters, triangle normals and closest points,
+// barycentric coordinates, vertex support points, the edge list of the rare long horizon) is written
+// once per triangle / vertex and read at most once per iteration: it lives in a per-wave HBM area.
 #pragma once
 #include "jh_epa.h"
 
@@ -29,31 +29,42 @@ struct Lds {
     float vx[MaxVertices], vy[MaxVertices], vz[MaxVertices];
     unsigned long long tri[MaxTriangles];  // a | b << 16 | c << 32 | facing << 48
     double dist[MaxTriangles];
-    double pl[6][MaxTriangles];            // plane filter ex ey ez nx ny nz
-    unsigned edge[kMaxEdges];              // a | b << 16
     unsigned char ord[MaxTriangles];       // logical position -> slot
     unsigned char freel[MaxTriangles];     // free slots
     unsigned char lit[MaxTriangles];       // by logical position
 };
 constexpr size_t kLdsBytes = sizeof(Lds);
 
-// Per-wave HBM area: tf[8][256] floats (normal xyz, closest xyz, normal_sq, closest_sq), bary[3][256]
-// doubles, ab[6][128] floats (support points on A and B).
-constexpr size_t kTfBytes = size_t(8) * MaxTriangles * 4;
+// Per-wave HBM area: pl[6][256] doubles (plane filter ex ey ez nx ny nz), bary[3][256] doubles,
+// tf[8][256] floats (normal xyz, closest xyz, normal_sq, closest_sq), ab[6][128] floats (support points
+// on A and B), edge[768] (a | b << 16).
+constexpr size_t kPlBytes = size_t(6) * MaxTriangles * 8;
 constexpr size_t kBaryBytes = size_t(3) * MaxTriangles * 8;
+constexpr size_t kTfBytes = size_t(8) * MaxTriangles * 4;
 constexpr size_t kAbBytes = size_t(6) * MaxVertices * 4;
-constexpr size_t kSideBytes = kTfBytes + kBaryBytes + kAbBytes;
+constexpr size_t kEdgeBytes = size_t(kMaxEdges) * 4;
+constexpr size_t kSideBytes = kPlBytes + kBaryBytes + kTfBytes + kAbBytes + kEdgeBytes;
 
 struct Poly {
     Lds* L;
-    float* tf;
+    double* pl;
     double* bary;
+    float* tf;
     float* ab;
+    unsigned* edge;
     int vcount, tcount, enclosed, nfree;
 
+    __device__ __forceinline__ void bind(char* base) {
+        pl = (double*)base;
+        bary = (double*)(base + kPlBytes);
+        tf = (float*)(base + kPlBytes + kBaryBytes);
+        ab = (float*)(base + kPlBytes + kBaryBytes + kTfBytes);
+        edge = (unsigned*)(base + kPlBytes + kBaryBytes + kTfBytes + kAbBytes);
+    }
     __device__ __forceinline__ vec3 V(int i) const { return {L->vx[i], L->vy[i], L->vz[i]}; }
     __device__ __forceinline__ epa::Plane plane(int s) const {
-        return {L->pl[0][s], L->pl[1][s], L->pl[2][s], L->pl[3][s], L->pl[4][s], L->pl[5][s]};
+        return {pl[0 * MaxTriangles + s], pl[1 * MaxTriangles + s], pl[2 * MaxTriangles + s],
+                pl[3 * MaxTriangles + s], pl[4 * MaxTriangles + s], pl[5 * MaxTriangles + s]};
     }
 };
 
@@ -178,8 +189,9 @@ __device__ __noinline__ int create_triangle(Poly& w, short a, short b, short c, 
                      ((unsigned long long)(unsigned short)c << 32) |
                      ((unsigned long long)(unsigned short)(facing ? 1 : 0) << 48);
     w.L->dist[slot] = dsq;
-    w.L->pl[0][slot] = pf.ex; w.L->pl[1][slot] = pf.ey; w.L->pl[2][slot] = pf.ez;
-    w.L->pl[3][slot] = pf.nx; w.L->pl[4][slot] = pf.ny; w.L->pl[5][slot] = pf.nz;
+    double* pl = w.pl;
+    pl[0 * MaxTriangles + slot] = pf.ex; pl[1 * MaxTriangles + slot] = pf.ey; pl[2 * MaxTriangles + slot] = pf.ez;
+    pl[3 * MaxTriangles + slot] = pf.nx; pl[4 * MaxTriangles + slot] = pf.ny; pl[5 * MaxTriangles + slot] = pf.nz;
     float* tf = w.tf;
     tf[0 * MaxTriangles + slot] = nf.x; tf[1 * MaxTriangles + slot] = nf.y; tf[2 * MaxTriangles + slot] = nf.z;
     tf[3 * MaxTriangles + slot] = cf.x; tf[4 * MaxTriangles + slot] = cf.y; tf[5 * MaxTriangles + slot] = cf.z;
@@ -291,7 +303,7 @@ __device__ __forceinline__ int closest_triangle(Poly& w, int lane) {
 // Horizon of the lit triangles (the sequential walk of ConvexPolytope.AddVertexDetailed :217-246): lit
 // triangles in descending logical order, edges (a,b), (b,c), (c,a); a matching edge (either direction)
 // already in the list cancels (the last edge moves into its place), otherwise the edge is appended.
-// This form keeps the list in LDS (any length); horizon_regs below is the fast path.
+// This form keeps the list in the HBM side area (any length); horizon_regs below is the fast path.
 __device__ __noinline__ int horizon_lds(Poly& w, int lane, int& litcount) {
     int ecount = 0;
     litcount = 0;
@@ -307,16 +319,16 @@ __device__ __noinline__ int horizon_lds(Poly& w, int lane, int& litcount) {
             int found = -1;
             for (int base = ((ecount - 1) / 64) * 64; base >= 0 && found < 0 && ecount > 0; base -= 64) {
                 int e = base + lane;
-                unsigned v = e < ecount ? w.L->edge[e] : 0xffffffffu;
+                unsigned v = e < ecount ? w.edge[e] : 0xffffffffu;
                 unsigned long long m = __ballot(e < ecount && (v == fwd || v == rev));
                 if (m) found = base + 63 - __clzll(m);
             }
             wave_sync();
             if (found >= 0) {
                 --ecount;
-                if (lane == 0) w.L->edge[found] = w.L->edge[ecount];
+                if (lane == 0) w.edge[found] = w.edge[ecount];
             } else {
-                if (lane == 0) w.L->edge[ecount] = fwd;
+                if (lane == 0) w.edge[ecount] = fwd;
                 ++ecount;
             }
             wave_sync();
@@ -339,8 +351,8 @@ __device__ __forceinline__ void ereg_set(unsigned (&r)[kEdgeRegs], int q, unsign
     else if (q == 2) r[2] = v;
     else r[3] = v;
 }
-__device__ __forceinline__ int horizon_regs(Poly& w, int lane, const unsigned long long (&litm)[4], int& litcount) {
-    unsigned ereg[kEdgeRegs] = {0u, 0u, 0u, 0u};
+__device__ __forceinline__ int horizon_regs(Poly& w, int lane, const unsigned long long (&litm)[4], int& litcount,
+                                            unsigned (&ereg)[kEdgeRegs]) {
     int ecount = 0;
     litcount = 0;
     for (int j = 3; j >= 0; --j) {
@@ -375,10 +387,6 @@ __device__ __forceinline__ int horizon_regs(Poly& w, int lane, const unsigned lo
             }
         }
     }
-#pragma unroll
-    for (int q = 0; q < kEdgeRegs; ++q)
-        if (q * 64 + lane < ecount) w.L->edge[q * 64 + lane] = ereg[q];
-    wave_sync();
     return ecount;
 }
 
@@ -405,8 +413,10 @@ __device__ __forceinline__ int add_vertex(Poly& w, const MVertex& vx, int lane) 
     }
     wave_sync();
     int litcount = 0;
-    int ecount = horizon_regs(w, lane, litm, litcount);
-    if (ecount < 0) ecount = horizon_lds(w, lane, litcount);
+    unsigned ereg[kEdgeRegs] = {0u, 0u, 0u, 0u};
+    int ecount = horizon_regs(w, lane, litm, litcount, ereg);
+    bool in_regs = ecount >= 0;
+    if (!in_regs) ecount = horizon_lds(w, lane, litcount);
     if (ecount == 0) return epa::NoHorizon;
     if (w.tcount - litcount + ecount > MaxTriangles) return epa::CapacityExceeded;
     // survivors keep their order; lit slots return to the free list (all reads before the writes:
@@ -443,7 +453,7 @@ __device__ __forceinline__ int add_vertex(Poly& w, const MVertex& vx, int lane) 
     // one new triangle per horizon edge, in edge order
     int r = epa::Added;
     for (int e = lane; e < ecount; e += 64) {
-        unsigned ev = w.L->edge[e];
+        unsigned ev = in_regs ? ereg_get(ereg, e >> 6) : w.edge[e];
         int slot = w.L->freel[w.nfree - 1 - e];
         w.L->ord[surv + e] = (unsigned char)slot;
         int rr = create_triangle(w, short(ev & 0xffff), short(ev >> 16), short(w.vcount), slot);

@@ -675,7 +675,7 @@ struct EpaRes {
     vec3 pa, pb, normal;
 };
 
-__global__ void __launch_bounds__(64) k_epa_wave(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
+__global__ void __launch_bounds__(64, 6) k_epa_wave(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
                                                  const float* __restrict__ verts, Bodies B,
                                                  const DeepRec* __restrict__ deep, const int* __restrict__ deep_count,
                                                  char* __restrict__ side, EpaRes* __restrict__ res) {
@@ -683,10 +683,7 @@ __global__ void __launch_bounds__(64) k_epa_wave(const uint64_t* __restrict__ pa
     int lane = threadIdx.x;
     epw::Poly w;
     w.L = &L;
-    char* base = side + size_t(blockIdx.x) * epw::kSideBytes;
-    w.tf = (float*)base;
-    w.bary = (double*)(base + epw::kTfBytes);
-    w.ab = (float*)(base + epw::kTfBytes + epw::kBaryBytes);
+    w.bind(side + size_t(blockIdx.x) * epw::kSideBytes);
     int n = *deep_count;
     for (int q = blockIdx.x; q < n; q += gridDim.x) {
         const DeepRec& d = deep[q];
