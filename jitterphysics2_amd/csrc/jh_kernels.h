@@ -1284,6 +1284,9 @@ struct SolveArgs {
     const int* nchunks;
     const int2* rank;        // per order position: rank of the arbiter in each body's update sequence
     const int* deg;          // per body: arbiters touching it
+    int* queue;              // k_solve_flow: chunk counters, 2 per substep (null: round-robin ownership)
+    int far_links, far_sleep; // dataflow polling: back off far_sleep x 1024 cycles while every waiting
+                              // lane lacks at least far_links visits
     unsigned* bar;           // grid barrier state
     Bodies B;
     int nb;
@@ -1428,13 +1431,19 @@ constexpr int kCohSc1 = 16;                // cache-policy operand: sc1
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t vw_rsrc(const SolveArgs& P) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)P.vw, 0, P.nb * 32, kBufferDword3);
 }
-// True (and v, w set) when body b's record shows exactly `need` visits.
-__device__ __forceinline__ bool coh_poll_vw(__amdgpu_buffer_rsrc_t r, int b, int need, vec3& v, vec3& w) {
+// True (and v, w set) when body b's record shows exactly `need` visits; `seen` gets the smaller of the
+// two counts (how many visits the body still lacks is need - seen).
+__device__ __forceinline__ bool coh_poll_vw(__amdgpu_buffer_rsrc_t r, int b, int need, vec3& v, vec3& w, int& seen) {
     float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32, 0, kCohSc1));
     float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32 + 16, 0, kCohSc1));
     v = {a.x, a.y, a.z};
     w = {c.x, c.y, c.z};
+    seen = min(__float_as_int(a.w), __float_as_int(c.w));
     return __float_as_int(a.w) == need && __float_as_int(c.w) == need;
+}
+__device__ __forceinline__ bool coh_poll_vw(__amdgpu_buffer_rsrc_t r, int b, int need, vec3& v, vec3& w) {
+    int seen;
+    return coh_poll_vw(r, b, need, v, w, seen);
 }
 __device__ __forceinline__ void coh_publish_vw(__amdgpu_buffer_rsrc_t r, int b, vec3 v, vec3 w, int n) {
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -1675,15 +1684,25 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
     // Each lane computes and publishes as soon as its own bodies are ready; the wave leaves when all
     // of its lanes have (a lane never waits for an unrelated lane's dependencies).  A record that
-    // showed the awaited count is not polled again (no other visit can change it before this one);
-    // polls back off a little so that waiting lanes do not flood the coherent memory path.
+    // showed the awaited count is not polled again (no other visit can change it before this one).
+    // Polls back off: briefly while some waiting lane lacks fewer than P.far_links visits, longer
+    // while every waiting lane lacks more (each missing visit is at least one more dependency link
+    // away), so that waiting lanes do not flood the coherent memory path.
     bool done = !active;
     bool ok1 = !m1, ok2 = !m2;
     while (true) {
         asm volatile("" ::: "memory");
+        int lack = 0x7fffffff;
         if (!done) {
-            if (!ok1) ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w);
-            if (!ok2) ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w);
+            int seen;
+            if (!ok1) {
+                ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w, seen);
+                if (!ok1) lack = need1 - seen;
+            }
+            if (!ok2) {
+                ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w, seen);
+                if (!ok2) lack = min(lack, need2 - seen);
+            }
             if (ok1 && ok2) {
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
@@ -1700,7 +1719,57 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
             }
         }
         if (__all(done)) break;
-        __builtin_amdgcn_s_sleep(2);
+        if (__all(done || lack >= P.far_links))
+            for (int z = 0; z < P.far_sleep; ++z) __builtin_amdgcn_s_sleep(16);
+        else
+            __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Chunk ownership decided in the first pass of a phase: waves take chunks from a counter in global
+// order (a wave that frees up takes the oldest chunk nobody has taken, so a ready chunk does not wait
+// behind a slow chunk of the same wave), record them, and replay the same list in the phase's later
+// passes.  Every arbiter therefore stays with one wave for the whole phase, which keeps its contact
+// rows coherent (plain loads and stores by the lanes that own them); phases are separated by grid
+// barriers.  Deadlock-free: the first pass takes chunks in global order, the replays walk each list
+// in increasing order, so the earliest unfinished visit always has its predecessors done.  A wave
+// takes at most kOwnMax chunks per phase; with more chunks than all waves can own, the phase uses the
+// round-robin ownership of flow_passes instead.
+constexpr int kOwnMax = 64;
+__device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int npass, int phase_first,
+                                            int phase_rest);
+__device__ __forceinline__ void flow_passes_own(const SolveArgs& P, int g_first, int npass, int phase_first,
+                                                int phase_rest, int* queue, int* own) {
+    int lane = threadIdx.x & 63;
+    int J = *P.nchunks;
+    if (J > ((gridDim.x * blockDim.x) >> 6) * kOwnMax) {
+        flow_passes(P, g_first, npass, phase_first, phase_rest);
+        return;
+    }
+    int n = 0;
+    for (int p = 0; p < npass; ++p) {
+        int g = g_first + p;
+        int phase = p == 0 ? phase_first : phase_rest;
+        for (int i = 0;; ++i) {
+            int j;
+            if (p == 0) {
+                if (i == kOwnMax) break;
+                int t = 0;
+                if (lane == 0) t = atomicAdd(queue, 1);
+                j = __shfl(t, 0, 64);
+                if (j >= J) break;
+                if (lane == 0) own[i] = j;
+                n = i + 1;
+            } else {
+                if (i == n) break;
+                j = own[i];
+            }
+            int2 ch = P.chunks[j];
+            int k = ch.x + lane;
+            bool active = k < ch.y;
+            if (phase == 0) flow_visit<true>(P, k, g, false, active);
+            else flow_visit<false>(P, k, g, phase == 1, active);
+        }
     }
 }
 
@@ -1726,11 +1795,15 @@ __device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int
 // One launch per step (World.Step.cs:234-253): per substep IntegrateForces | prepare + iterations
 // (dataflow) | IntegrateVelocities | relaxations (dataflow), grid barriers only between these phases.
 __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
+    __shared__ int own_all[4][kOwnMax];  // chunks owned by each wave in the current phase
+    int* own = own_all[threadIdx.x >> 6];
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     int per_sub = 1 + P.iterations + P.relaxations;
     const Bodies& B = P.B;
     for (int b = t0; b < P.nb; b += stride) st_vw(P, b, ld3(B.vel[b]), ld3(B.angvel[b]), 0);
+    if (P.queue)  // visible after the first grid barrier
+        for (int i = t0; i < 2 * P.ts.substeps; i += stride) P.queue[i] = 0;
     for (int s = 0; s < P.ts.substeps; ++s) {
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
             if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
@@ -1741,7 +1814,8 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
             st_vw(P, b, v, w, n);
         }
         grid_barrier(P.bar);
-        flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
+        if (P.queue) flow_passes_own(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s, own);
+        else flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
         grid_barrier(P.bar);
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
             if (B.motion[b] == JH_MOTION_STATIC) continue;
@@ -1754,7 +1828,8 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
         }
         grid_barrier(P.bar);
         if (P.relaxations > 0) {
-            flow_passes(P, s * per_sub + 1 + P.iterations, P.relaxations, 2, 2);
+            if (P.queue) flow_passes_own(P, s * per_sub + 1 + P.iterations, P.relaxations, 2, 2, P.queue + 2 * s + 1, own);
+            else flow_passes(P, s * per_sub + 1 + P.iterations, P.relaxations, 2, 2);
             grid_barrier(P.bar);
         }
     }

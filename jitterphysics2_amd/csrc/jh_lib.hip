@@ -203,6 +203,9 @@ struct jh_world {
     int kind = SOLVER_FLOW;
     // lane solver (k_solve_lanes): workgroups and arbiter records
     int lane_blocks = 0, lane_max = 0;
+    int far_links = 3, far_sleep = 2;  // dataflow polling back-off (SolveArgs)
+    bool flow_queue = true;            // k_solve_flow: chunk ownership from counters (else round-robin)
+    DBuf<int> fqueue;
     DBuf<float4> rec;
     // tile solver (one workgroup per Morton range of bodies, private bodies in LDS)
     int tiles = 0;            // tiles = workgroups of k_solve_tiles
@@ -707,6 +710,10 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.soa.f = w->soa_f.p;
         S.soa.cap = w->soa_cap;
         S.vw = w->vwpack.p;
+        if (w->flow_queue) HIPCHK(w->fqueue.ensure(2 * size_t(cfg.substep_count)));
+        S.queue = w->flow_queue ? w->fqueue.p : nullptr;
+        S.far_links = w->far_links;
+        S.far_sleep = w->far_sleep;
         if (probe) {
             s = run_probe(w, S, a_cur, amax_, probe);
             if (s != JH_OK) return s;
@@ -920,6 +927,9 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
         w->flow_blocks = std::max(1, std::min(atoi(e), w->flow_max));
         w->lane_blocks = std::max(1, std::min(atoi(e), w->lane_max));
     }
+    if (const char* e = getenv("JH_FLOW_QUEUE")) w->flow_queue = atoi(e) != 0;
+    if (const char* e = getenv("JH_FAR_LINKS")) w->far_links = std::max(1, atoi(e));
+    if (const char* e = getenv("JH_FAR_SLEEP")) w->far_sleep = std::max(0, std::min(atoi(e), 64));
     w->tiles = w->cu_count;
     if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
     w->colour_blocks = std::min(64, w->coop_colour);
@@ -941,7 +951,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->mk0.release(); w->mk1.release(); w->mv0.release(); w->mv1.release();
     w->left.release(); w->right.release(); w->parent.release(); w->leaf_parent.release(); w->nflags.release();
     w->range.release(); w->bounds.release(); w->hist.release(); w->pk0.release(); w->pk1.release();
-    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release(); w->epa_res.release();
+    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release(); w->epa_res.release(); w->fqueue.release();
     w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
@@ -1628,7 +1638,8 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
     if (w->kind == SOLVER_BARRIER) {
         for (int n : uniq({cu / 2, cu, 2 * cu, 4 * cu, 8 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n});
     } else {
-        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->flow_max)) sc.push_back({SOLVER_FLOW, n});
+        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, 3 * cu / 4, cu, 2 * cu}, w->flow_max))
+            sc.push_back({SOLVER_FLOW, n});
         for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n});
         for (int T : {cu / 2, cu, 2 * cu})
             if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back({SOLVER_TILES, T});

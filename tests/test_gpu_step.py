@@ -88,10 +88,13 @@ def test_heap_bitexact_barrier_solver(monkeypatch):
 
 
 def test_dataflow_solver_launch_shapes(monkeypatch):
-    # the dataflow solver is exact for any number of resident waves (chunk ownership changes)
-    for blocks in ("1", "7", "64"):
-        monkeypatch.setenv("JH_SOLVE_BLOCKS", blocks)
-        run_replay(scenes.heap(700, seed=11), steps=40, check_every=39)
+    # the dataflow solver is exact for any number of resident waves and either chunk ownership
+    # (taken from counters in each phase's first pass, or round-robin)
+    for queue in ("1", "0"):
+        monkeypatch.setenv("JH_FLOW_QUEUE", queue)
+        for blocks in ("1", "7", "64"):
+            monkeypatch.setenv("JH_SOLVE_BLOCKS", blocks)
+            run_replay(scenes.heap(700, seed=11), steps=40, check_every=39)
 
 
 def test_solve_order_is_colouring():
