@@ -9,8 +9,8 @@ the heap form contacts; then K steps are timed with inputs resident in HBM.
 Besides the JSON line's contract fields it reports:
   roofline     — the solver kernel (k_solve_flow, the persistent dataflow solver; k_solve_tiles, one
                  workgroup per Morton range of bodies; or k_solve_worlds, one workgroup per world,
-                 whichever the autotuner picked), whose duration is measured
-                 with HIP events on the library's stream (jh_stats.ms_solve); algorithmic bytes per
+                 whichever the autotuner picked), whose duration is measured with HIP events around
+                 its launch on the library's stream (jh_stats.ms_solve_kernel); algorithmic bytes per
                  launch from the per-step counts (see solver_bytes()); traffic from the committed
                  counter summary when this is the profiled run (see measured_traffic()).
   cpu_baseline — the oracle (CPU restatement, 1 thread) timed on a bounded sample: a few steps of the
@@ -200,7 +200,7 @@ def main() -> int:
     for _ in range(args.steps):
         gw.step(dt)  # returns after the step finished on the GPU
         s = gw.stats()
-        solve_ms.append(s["ms_solve"])
+        solve_ms.append(s["ms_solve_kernel"])
         for key in stage_keys:
             stage_sum[key] += s[key]
         step_ms.append(s["ms_step"])

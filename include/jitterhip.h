@@ -180,6 +180,7 @@ typedef struct jh_stats {
                                  1 dataflow (wave chunks), 2 world-local, 3 tiles, 4 dataflow (lanes) */
     int64_t max_degree;       /* most solved arbiters on one non-static body (a lower bound of the
                                  colour count); dataflow and tile solvers only, else 0 */
+    double ms_solve_kernel;   /* part of ms_solve spent in the solver kernel itself */
 } jh_stats;
 
 typedef struct jh_world jh_world;

@@ -136,7 +136,7 @@ STATS_DTYPE = np.dtype([
     ("steps", "<i8"), ("pair_overflow", "<i8"), ("arbiters_solved", "<i8"),
     ("ms_broadphase", "<f8"), ("ms_narrowphase", "<f8"), ("ms_contact_cache", "<f8"),
     ("ms_colouring", "<f8"), ("ms_solve", "<f8"), ("ms_update", "<f8"), ("ms_step", "<f8"), ("ms_epa", "<f8"),
-    ("solver", "<i8"), ("max_degree", "<i8"),
+    ("solver", "<i8"), ("max_degree", "<i8"), ("ms_solve_kernel", "<f8"),
 ])
 
 TUNING_DTYPE = np.dtype([

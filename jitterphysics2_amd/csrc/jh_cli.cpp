@@ -85,16 +85,16 @@ int main(int argc, char** argv) {
     if ((s = jh_world_autotune(w, dt, 2, &tune))) return die("jh_world_autotune", s);
     printf("# autotune: solver %d x %d, colour blocks %d\n", tune.solver, tune.solve_blocks, tune.colour_blocks);
     fflush(stdout);
-    double sum[8] = {0};
+    double sum[9] = {0};
     jh_stats st;
     roctxRangePush("timed_steps");
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < steps; ++i) {
         if ((s = jh_world_step(w, dt))) return die("jh_world_step", s);
         jh_world_get_stats(w, &st);
-        const double v[8] = {st.ms_broadphase, st.ms_narrowphase, st.ms_epa, st.ms_contact_cache,
-                             st.ms_colouring,  st.ms_solve,       st.ms_update, st.ms_step};
-        for (int k = 0; k < 8; ++k) sum[k] += v[k];
+        const double v[9] = {st.ms_broadphase, st.ms_narrowphase, st.ms_epa,    st.ms_contact_cache, st.ms_colouring,
+                             st.ms_solve,      st.ms_update,     st.ms_step,   st.ms_solve_kernel};
+        for (int k = 0; k < 9; ++k) sum[k] += v[k];
     }
     auto t1 = std::chrono::steady_clock::now();
     roctxRangePop();
@@ -104,10 +104,10 @@ int main(int argc, char** argv) {
     for (auto& b : bodies) ndyn += b.motion_type == JH_MOTION_DYNAMIC;
     printf("steps %d wall_s %.6f ms_per_step %.4f body_steps_per_s %.1f pairs %lld arbiters %lld contacts %lld "
            "colours %lld epa %lld | mean ms: broadphase %.3f narrowphase %.3f (epa %.3f) cache %.3f colouring %.3f "
-           "solve %.3f update %.3f step %.3f\n",
+           "solve %.3f (kernel %.3f) update %.3f step %.3f\n",
            steps, wall, 1000.0 * wall / steps, ndyn * steps / wall, (long long)st.pairs, (long long)st.arbiters,
            (long long)st.contacts, (long long)st.colours, (long long)st.epa_calls, sum[0], sum[1], sum[2], sum[3], sum[4],
-           sum[5], sum[6], sum[7]);
+           sum[5], sum[8], sum[6], sum[7]);
     fflush(stdout);
     jh_world_destroy(w);
     return 0;

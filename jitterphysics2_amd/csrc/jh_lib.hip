@@ -226,7 +226,8 @@ struct jh_world {
     std::vector<int> solve_offsets;
     // cooperative occupancy
     int coop_colour = 0, coop_solve = 0, flow_blocks = 0, colour_blocks = 64, flow_max = 0;
-    hipEvent_t ev[8];  // 0 start, 1 broadphase, 2 narrowphase, 3 cache, 4 colouring, 5 solve, 6 end, 7 EPA start
+    hipEvent_t ev[10];  // 0 start, 1 broadphase, 2 narrowphase, 3 cache, 4 colouring, 5 solve, 6 end, 7 EPA start,
+                        // 8 / 9 around the solver kernel itself
     // learned warm start and residual probe
     LwsWeights lws;
     DBuf<float4> pvel, pang;
@@ -744,7 +745,9 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             Wd.woff = w->woff.p;
             Wd.W = w->n_worlds;
             Wd.queue = w->wqueue.p;
+            HIPCHK(hipEventRecord(w->ev[8], st));
             k_solve_worlds<<<std::max(1, std::min(w->world_blocks, w->n_worlds)), 512, 0, st>>>(S, Wd);
+            HIPCHK(hipEventRecord(w->ev[9], st));
             k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, w->wlist.p, w->coffs.p, S.soa);
             HIPCHK(hipGetLastError());
         } else if (use_tiles) {
@@ -761,26 +764,35 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             Tl.err = cnt + C_SERR;
             Tl.T = w->tiles;
             void* targs[] = {&S, &Tl};
+            HIPCHK(hipEventRecord(w->ev[8], st));
             HIPCHK(hipLaunchCooperativeKernel((const void*)k_solve_tiles, dim3(w->tiles), dim3(256), targs,
                                               unsigned(tile_lds), st));
+            HIPCHK(hipEventRecord(w->ev[9], st));
             k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, w->tlist.p, w->coffs.p, S.soa);
             HIPCHK(hipGetLastError());
         } else if (kind == SOLVER_LANES) {
             int gsoa = grid_for(std::max(amax_, 1));
             k_rec_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, w->rank.p, w->deg.p, w->motion.p, w->rec.p);
             void* largs[] = {&S, &w->rec.p};
+            HIPCHK(hipEventRecord(w->ev[8], st));
             s = launch_coop((const void*)k_solve_lanes, w->lane_blocks, 256, largs, st);
             if (s != JH_OK) return s;
+            HIPCHK(hipEventRecord(w->ev[9], st));
             k_rec_scatter<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, w->rec.p);
         } else if (kind == SOLVER_FLOW) {
             int gsoa = grid_for(std::max(amax_, 1));
             k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
+            HIPCHK(hipEventRecord(w->ev[8], st));
             s = launch_coop((const void*)k_solve_flow, w->flow_blocks, 256, sargs, st);
             if (s != JH_OK) return s;
+            HIPCHK(hipEventRecord(w->ev[9], st));
             k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
         } else {
             int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(amax_, w->nb), 256), 256, 1 << 20));
+            HIPCHK(hipEventRecord(w->ev[8], st));
             s = launch_coop((const void*)k_solve, sgrid, 256, sargs, st);
+            if (s != JH_OK) return s;
+            HIPCHK(hipEventRecord(w->ev[9], st));
         }
         if (s != JH_OK) return s;
     }
@@ -831,6 +843,9 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     float epa_ms = 0;
     if (np > 0) HIPCHK(hipEventElapsedTime(&epa_ms, w->ev[7], w->ev[2]));
     S.ms_epa = np > 0 ? epa_ms : 0.0;
+    float sk_ms;
+    HIPCHK(hipEventElapsedTime(&sk_ms, w->ev[8], w->ev[9]));
+    S.ms_solve_kernel = sk_ms;
     float tot;
     HIPCHK(hipEventElapsedTime(&tot, w->ev[0], w->ev[6]));
     S.ms_step = tot;
