@@ -132,14 +132,16 @@ def learned_warm_start_leg(gw, dt: float, cfg, settle_to: int, probe_iterations:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=240)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (heap default 60; pyramid 600)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (heap default 240; pyramid 0)")
     ap.add_argument("--bodies", type=int, default=131072)
-    ap.add_argument("--workload", choices=("heap", "batched", "full"), default="heap",
-                    help="heap: one 131,072-body heap (the metric's configuration); batched: 4,096 independent "
-                         "worlds of 16..2,048 bodies (log-uniform) in one world batch; full: 262,144 bodies incl. "
-                         "32-point hulls over a 512x512-cell triangle heightfield split into 64 sub-worlds, "
-                         "TriangleEdgeCollisionFilter on")
+    ap.add_argument("--workload", choices=("heap", "batched", "full", "pyramid"), default="heap",
+                    help="heap: one heap of --bodies bodies (default 131,072, the metric's configuration; 32,768 is "
+                         "the 32k-heap configuration); batched: 4,096 independent worlds of 16..2,048 bodies "
+                         "(log-uniform) in one world batch; full: 262,144 bodies incl. 32-point hulls over a "
+                         "512x512-cell triangle heightfield split into 64 sub-worlds, TriangleEdgeCollisionFilter "
+                         "on; pyramid: the 20-row pyramid of 210 unit boxes, 600 steps, CPU baseline over all of "
+                         "them")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -148,6 +150,10 @@ def main() -> int:
     ap.add_argument("--lws-settle", type=int, default=700)
     ap.add_argument("--lws-iterations", type=int, default=256)
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 600 if args.workload == "pyramid" else 60
+    if args.warmup is None:
+        args.warmup = 0 if args.workload == "pyramid" else 240
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
                           "n_gpus": args.gpus}))
@@ -163,6 +169,11 @@ def main() -> int:
     elif args.workload == "full":
         bodies, shapes, verts, cfg, dt = scenes.full_form()
         args.no_lws = True
+    elif args.workload == "pyramid":
+        bodies, shapes, verts, cfg, dt = scenes.pyramid(rows=20)
+        args.no_lws = True
+        args.cpu_steps = args.steps  # the whole configuration is CPU-sized
+        args.no_autotune = True  # the configuration's 600 steps start from the initial state
     else:
         bodies, shapes, verts, cfg, dt = scenes.heap(args.bodies)
     print(f"# scene built: {len(bodies)} bodies, {len(shapes)} shapes in {time.time() - t0:.1f}s", flush=True)
@@ -226,7 +237,7 @@ def main() -> int:
         ow.set_arbiters(start_arbiters)
         secs = ow.time_steps(dt, args.cpu_steps)
         cpu = {"value": n_dyn * args.cpu_steps / secs, "unit": "body-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body heap continued from the GPU state at "
+               "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body scene continued from the GPU state at "
                          f"the start of the timed region (oracle FP32, key-order solve, 1 thread)"}
 
     kernel = {0: "k_solve", 1: "k_solve_flow", 2: "k_solve_worlds", 3: "k_solve_tiles",
@@ -254,6 +265,9 @@ def main() -> int:
                    if args.workload == "batched" else
                    (f"{n_dyn} dynamic spheres/boxes/capsules/32-point hulls over a 512x512-cell Perlin heightfield "
                     f"(524,288 triangles) in 64 sub-worlds, triangle edge filter, dt=1/60, {substeps} substeps, "
+                    f"{iters} iterations, {relax} relaxations")
+                   if args.workload == "full" else
+                   (f"20-row pyramid of {n_dyn} unit boxes on a static ground box, dt=0.01, {substeps} substeps, "
                     f"{iters} iterations, {relax} relaxations"),
                    "autotune": tuning,
                    "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],

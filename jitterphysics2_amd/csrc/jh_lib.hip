@@ -88,6 +88,7 @@ enum {
 
 // Solver kernels (jh_stats.solver, jh_tuning.solver).
 enum { SOLVER_BARRIER = 0, SOLVER_FLOW = 1, SOLVER_WORLDS = 2, SOLVER_TILES = 3, SOLVER_LANES = 4 };
+constexpr int kSmallWorld = 2048;  // bodies: worlds up to this size default to the world-local solver
 
 constexpr int kEpaMaxThreads = 32768;  // polytopes resident at once (epa::kBytesPerThread each)
 
@@ -1031,7 +1032,13 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
         else if (e && strcmp(e, "tiles") == 0) w->kind = SOLVER_TILES;
         else if (e && strcmp(e, "lanes") == 0) w->kind = SOLVER_LANES;
         else if (e && strcmp(e, "flow") == 0) w->kind = SOLVER_FLOW;
-        else w->kind = W >= 64 ? SOLVER_WORLDS : SOLVER_FLOW;
+        else {
+            // many worlds, or only small ones (one workgroup per world beats a device-wide launch
+            // whose grid barriers and dependency round trips dominate a few hundred arbiters)
+            int largest = 0;
+            for (int k = 0; k < W; ++k) largest = std::max(largest, off[k + 1] - off[k]);
+            w->kind = (W >= 64 || largest <= kSmallWorld) ? SOLVER_WORLDS : SOLVER_FLOW;
+        }
         w->n_moving = 0;
         for (int i = 0; i < n; ++i) w->n_moving += bodies[i].motion_type != JH_MOTION_STATIC;
         if (W > 0) {
@@ -1658,7 +1665,7 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n});
         for (int T : {cu / 2, cu, 2 * cu})
             if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back({SOLVER_TILES, T});
-        if (w->n_worlds > 1) sc.push_back({SOLVER_WORLDS, w->world_blocks});
+        if (w->n_worlds >= 1) sc.push_back({SOLVER_WORLDS, w->world_blocks});
     }
     auto width = [&](int kind) -> int& {
         return kind == SOLVER_BARRIER ? w->coop_solve : kind == SOLVER_LANES ? w->lane_blocks

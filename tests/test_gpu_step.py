@@ -69,16 +69,22 @@ def test_pyramid_small_bitexact():
     run_replay(scenes.pyramid(rows=6), steps=20, brute_pairs=True)
 
 
-def test_pyramid_20_rows_bitexact():
+@pytest.mark.parametrize("solver", ["default", "flow"])
+def test_pyramid_20_rows_bitexact(monkeypatch, solver):
+    # default: a world this small runs the world-local solver; flow: the device-wide dataflow solver
+    if solver != "default":
+        monkeypatch.setenv("JH_SOLVER", solver)
     gw, ow = run_replay(scenes.pyramid(rows=20), steps=30, check_every=10)
     st = gw.stats()
     assert st["arbiters"] > 300 and st["colours"] >= 2
+    assert st["solver"] == (2 if solver == "default" else 1)
 
 
-def test_heap_mixed_shapes_bitexact():
+def test_heap_mixed_shapes_bitexact(monkeypatch):
     # spheres, boxes and capsules with random orientations; deep impacts exercise EPA
+    monkeypatch.setenv("JH_SOLVER", "flow")
     gw, ow = run_replay(scenes.heap(1500, seed=7), steps=120, check_every=20)
-    assert gw.stats()["arbiters"] > 0
+    assert gw.stats()["arbiters"] > 0 and gw.stats()["solver"] == 1
 
 
 def test_heap_bitexact_barrier_solver(monkeypatch):
@@ -90,6 +96,7 @@ def test_heap_bitexact_barrier_solver(monkeypatch):
 def test_dataflow_solver_launch_shapes(monkeypatch):
     # the dataflow solver is exact for any number of resident waves and either chunk ownership
     # (taken from counters in each phase's first pass, or round-robin)
+    monkeypatch.setenv("JH_SOLVER", "flow")
     for queue in ("1", "0"):
         monkeypatch.setenv("JH_FLOW_QUEUE", queue)
         for blocks in ("1", "7", "64"):
@@ -144,7 +151,7 @@ def test_autotune_keeps_results_exact():
         gw.step(dt)
     t = gw.autotune(dt, rounds=1)
     assert t["solve_blocks"] >= 1 and t["colour_blocks"] >= 1 and t["steps"] == t["candidates"]
-    assert gw.stats()["solver"] == t["solver"] and t["solver"] in (1, 3, 4)
+    assert gw.stats()["solver"] == t["solver"] and t["solver"] in (1, 2, 3, 4)
     assert t["ms_solve_best"] <= t["ms_solve_default"] + 1e-9
     # continue from the tuned world's state: the oracle replay stays bit-exact with the new shapes
     ow = OracleWorld(gw.bodies(), shapes, verts, cfg)
