@@ -1269,6 +1269,7 @@ struct ContactSoA {
     float2* fs;      // [cap] friction, speculative relaxation factor
     uint32_t* flag;  // [4][cap]
     float* f;        // [4][kSoaFields][cap]
+    float4* mv;      // [4][3][cap] versioned mutable fields (k_solve_flow), see load_mutable
     int cap;
     __device__ __forceinline__ float& at(int s, int fld, int k) const {
         return f[(size_t(s) * kSoaFields + fld) * cap + k];
@@ -1284,9 +1285,9 @@ struct SolveArgs {
     const int* nchunks;
     const int2* rank;        // per order position: rank of the arbiter in each body's update sequence
     const int* deg;          // per body: arbiters touching it
-    int* queue;              // k_solve_flow: chunk counters, 2 per substep (null: round-robin ownership)
-    int far_links, far_sleep; // dataflow polling: back off far_sleep x 1024 cycles while every waiting
-                              // lane lacks at least far_links visits
+    int* queue;              // k_solve_flow: chunk counters, 2 per substep
+    int qmode;               // k_solve_flow chunk ownership: 0 round-robin, 1 first pass of each phase,
+                             // 2 taken afresh in every pass
     unsigned* bar;           // grid barrier state
     Bodies B;
     int nb;
@@ -1378,6 +1379,13 @@ __global__ void __launch_bounds__(256) k_soa_gather(const jh_arbiter* __restrict
             if (!(a.usage_mask & (1u << s))) continue;
             const jh_contact& c = a.contacts[s];
             int r = j++;
+            if (S.mv) {
+                float4* m = S.mv + size_t(r) * 3 * S.cap + k;
+                float v0 = __int_as_float(-1);
+                m[0] = make_float4(c.accumulated[0], c.accumulated[1], c.accumulated[2], v0);
+                m[S.cap] = make_float4(c.bias, c.penalty_bias, c.mass_normal_tangent[0], v0);
+                m[2 * size_t(S.cap)] = make_float4(c.mass_normal_tangent[1], c.mass_normal_tangent[2], __uint_as_float(c.flag), v0);
+            }
             S.flag[size_t(r) * S.cap + k] = c.flag;
             S.at(r, kFBias, k) = c.bias;
             S.at(r, kFPen, k) = c.penalty_bias;
@@ -1406,6 +1414,14 @@ __global__ void __launch_bounds__(256) k_soa_scatter(jh_arbiter* __restrict__ ar
             if (!(usage & (1u << s))) continue;
             int r = j++;
             jh_contact& c = a.contacts[s];
+            if (S.mv) {
+                const float4* m = S.mv + size_t(r) * 3 * S.cap + k;
+                float4 m0 = m[0], m1 = m[S.cap], m2 = m[2 * size_t(S.cap)];
+                c.accumulated[0] = m0.x; c.accumulated[1] = m0.y; c.accumulated[2] = m0.z;
+                c.bias = m1.x; c.penalty_bias = m1.y; c.mass_normal_tangent[0] = m1.z;
+                c.mass_normal_tangent[1] = m2.x; c.mass_normal_tangent[2] = m2.y; c.flag = __float_as_uint(m2.z);
+                continue;
+            }
             c.flag = S.flag[size_t(r) * S.cap + k];
             c.bias = S.at(r, kFBias, k);
             c.penalty_bias = S.at(r, kFPen, k);
@@ -1645,10 +1661,68 @@ __device__ __forceinline__ void iterate_reg(uint32_t mode, float friction, SlotR
     if (mode & JH_SOLVE_ANGULAR2) B2.w = B2.w + sym_mul(cross(c.rel2, lin), B2.I);
 }
 
-// One dataflow visit of the arbiter at colour-order position k in global pass g.  Everything private
-// (contact slots, body inertia, poses) is fetched before the dependency wait; after the wait only
-// the shared velocities are read, updated in registers, stored coherently and published; the
-// private contact results are written back after publishing.
+// Versioned mutable contact fields of the dataflow solver.  Row r of position k has three 16-byte
+// records, each carrying the pass that wrote it in its last word:
+//   m0 (acc.n, acc.t1, acc.t2, pass of the arbiter's last visit)
+//   m1 (bias, penalty, mass.n, pass of its last prepare)   m2 (mass.t1, mass.t2, flag, same)
+// A visit in pass g needs m0 from pass g - 1 and m1/m2 from the last prepare, so whichever wave
+// visits the arbiter reads them with device-coherent loads, polled together with its two bodies'
+// records until the versions match (usually at once: the previous visit of the arbiter lies a whole
+// pass back).  This keeps every chunk-to-wave assignment exact, including a fresh one in every pass.
+// The gather writes version -1.
+// One buffer resource per row keeps the byte offsets (< 3 x 16 x cap) in 32 bits.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mv_rsrc(const ContactSoA& S, int s) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(S.mv + size_t(s) * 3 * S.cap), 0, 3 * 16 * S.cap, kBufferDword3);
+}
+__device__ __forceinline__ bool load_mutable(const ContactSoA& S, int s, int k, SlotReg& c, int want_visit,
+                                             int want_prepare) {
+    __amdgpu_buffer_rsrc_t r = mv_rsrc(S, s);
+    int o = k * 16;
+    float4 m0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, kCohSc1));
+    float4 m1 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + S.cap * 16, 0, kCohSc1));
+    float4 m2 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 2 * S.cap * 16, 0, kCohSc1));
+    c.acc[0] = m0.x; c.acc[1] = m0.y; c.acc[2] = m0.z;
+    c.bias = m1.x; c.penalty = m1.y; c.mass[0] = m1.z;
+    c.mass[1] = m2.x; c.mass[2] = m2.y; c.flag = __float_as_uint(m2.z);
+    return __float_as_int(m0.w) == want_visit && __float_as_int(m1.w) == want_prepare &&
+           __float_as_int(m2.w) == want_prepare;
+}
+__device__ __forceinline__ void store_mutable(const ContactSoA& S, int s, int k, const SlotReg& c, int g,
+                                              bool prepare) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    __amdgpu_buffer_rsrc_t r = mv_rsrc(S, s);
+    int o = k * 16;
+    float vg = __int_as_float(g);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(c.acc[0], c.acc[1], c.acc[2], vg)), r,
+                                           o, 0, kCohSc1);
+    if (prepare) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(c.bias, c.penalty, c.mass[0], vg)),
+                                               r, o + S.cap * 16, 0, kCohSc1);
+        __builtin_amdgcn_raw_buffer_store_b128(
+            __builtin_bit_cast(u32x4, make_float4(c.mass[1], c.mass[2], __uint_as_float(c.flag), vg)), r,
+            o + 2 * S.cap * 16, 0, kCohSc1);
+    }
+}
+// Only the fields no visit changes (contact frame, anchors).
+__device__ __forceinline__ void load_slot_fixed(const ContactSoA& S, int s, int k, SlotReg& r, bool prepare) {
+    for (int l = 0; l < 3; ++l) {
+        r.nx[l] = S.at(s, (l == 0 ? kFN : l == 1 ? kFT1 : kFT2) + 0, k);
+        r.ny[l] = S.at(s, (l == 0 ? kFN : l == 1 ? kFT1 : kFT2) + 1, k);
+        r.nz[l] = S.at(s, (l == 0 ? kFN : l == 1 ? kFT1 : kFT2) + 2, k);
+    }
+    r.rel1 = vec3{S.at(s, kFRel1, k), S.at(s, kFRel1 + 1, k), S.at(s, kFRel1 + 2, k)};
+    r.rel2 = vec3{S.at(s, kFRel2, k), S.at(s, kFRel2 + 1, k), S.at(s, kFRel2 + 2, k)};
+    if (prepare) {
+        r.pos1 = vec3{S.at(s, kFPos1, k), S.at(s, kFPos1 + 1, k), S.at(s, kFPos1 + 2, k)};
+        r.pos2 = vec3{S.at(s, kFPos2, k), S.at(s, kFPos2 + 1, k), S.at(s, kFPos2 + 2, k)};
+    }
+}
+
+// One dataflow visit of the arbiter at colour-order position k in global pass g.  Everything that
+// does not depend on other visits (contact frames and anchors, body inertia, poses) is fetched before
+// the dependency wait; the bodies' velocity records and the arbiter's versioned mutable fields are
+// polled until they show the awaited visit counts / passes, then the visit computes in registers and
+// publishes both.
 template <bool PREPARE>
 __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, bool apply_bias, bool active) {
     const Bodies& B = P.B;
@@ -1664,10 +1738,15 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     int2 r = P.rank[k];
     int need1 = m1 ? g * P.deg[b1] + r.x : 0;
     int need2 = m2 ? g * P.deg[b2] + r.y : 0;
+    // passes the mutable fields must come from: the arbiter's previous visit (-1: the gather) and the
+    // last prepare (for a prepare: the previous substep's, or the gather)
+    int per_sub = 1 + P.iterations + P.relaxations;
+    int want_visit = g - 1;
+    int want_prep = PREPARE ? (g >= per_sub ? g - per_sub : -1) : (g / per_sub) * per_sub;
     SlotReg sr[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s)
-        if (usage & (1u << s)) load_slot_soa(S, s, k, sr[s], PREPARE);
+        if (usage & (1u << s)) load_slot_fixed(S, s, k, sr[s], PREPARE);
     BodyReg R1, R2;
     load_static_part(B, b1, R1);
     load_static_part(B, b2, R2);
@@ -1682,28 +1761,23 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     if (!m1) ld_vw(P, b1, R1.v, R1.w);
     if (!m2) ld_vw(P, b2, R2.v, R2.w);
     __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
-    // Each lane computes and publishes as soon as its own bodies are ready; the wave leaves when all
-    // of its lanes have (a lane never waits for an unrelated lane's dependencies).  A record that
-    // showed the awaited count is not polled again (no other visit can change it before this one).
-    // Polls back off: briefly while some waiting lane lacks fewer than P.far_links visits, longer
-    // while every waiting lane lacks more (each missing visit is at least one more dependency link
-    // away), so that waiting lanes do not flood the coherent memory path.
+    // Each lane computes and publishes as soon as its own inputs are ready; the wave leaves when all
+    // of its lanes have (a lane never waits for an unrelated lane's dependencies).  An input that
+    // showed the awaited version is not polled again (no other visit can change it before this one).
     bool done = !active;
-    bool ok1 = !m1, ok2 = !m2;
+    bool ok1 = !m1, ok2 = !m2, rows = false;
     while (true) {
         asm volatile("" ::: "memory");
-        int lack = 0x7fffffff;
         if (!done) {
-            int seen;
-            if (!ok1) {
-                ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w, seen);
-                if (!ok1) lack = need1 - seen;
+            if (!rows) {
+                rows = true;
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    if (usage & (1u << s)) rows = load_mutable(S, s, k, sr[s], want_visit, want_prep) && rows;
             }
-            if (!ok2) {
-                ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w, seen);
-                if (!ok2) lack = min(lack, need2 - seen);
-            }
-            if (ok1 && ok2) {
+            if (!ok1) ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w);
+            if (!ok2) ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w);
+            if (ok1 && ok2 && rows) {
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     if (!(usage & (1u << s))) continue;
@@ -1714,24 +1788,42 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
                 if (m2) coh_publish_vw(vr, b2, R2.v, R2.w, need2 + 1);
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
-                    if (usage & (1u << s)) store_slot_soa(S, s, k, sr[s], PREPARE);
+                    if (usage & (1u << s)) store_mutable(S, s, k, sr[s], g, PREPARE);
                 done = true;
             }
         }
         if (__all(done)) break;
-        if (__all(done || lack >= P.far_links))
-            for (int z = 0; z < P.far_sleep; ++z) __builtin_amdgcn_s_sleep(16);
-        else
-            __builtin_amdgcn_s_sleep(2);
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Chunks of a phase handed out in global order from a counter, in every pass: a wave that frees up
+// takes the oldest chunk nobody has taken, so a ready chunk never waits behind a slow chunk of the
+// same wave.  Deadlock-free: the oldest unfinished chunk has been taken and its predecessors are done.
+// Exact because an arbiter's mutable fields are versioned (load_mutable), whichever wave visits it.
+__device__ __forceinline__ void flow_passes_queue(const SolveArgs& P, int g_first, int npass, int phase_first,
+                                                  int phase_rest, int* queue) {
+    int lane = threadIdx.x & 63;
+    int J = *P.nchunks;
+    int total = J * npass;
+    while (true) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(queue, 1);
+        t = __shfl(t, 0, 64);
+        if (t >= total) break;
+        int p = t / J, j = t - p * J;
+        int g = g_first + p;
+        int phase = p == 0 ? phase_first : phase_rest;
+        int2 ch = P.chunks[j];
+        int k = ch.x + lane;
+        bool active = k < ch.y;
+        if (phase == 0) flow_visit<true>(P, k, g, false, active);
+        else flow_visit<false>(P, k, g, phase == 1, active);
     }
 }
 
 // Chunk ownership decided in the first pass of a phase: waves take chunks from a counter in global
-// order (a wave that frees up takes the oldest chunk nobody has taken, so a ready chunk does not wait
-// behind a slow chunk of the same wave), record them, and replay the same list in the phase's later
-// passes.  Every arbiter therefore stays with one wave for the whole phase, which keeps its contact
-// rows coherent (plain loads and stores by the lanes that own them); phases are separated by grid
-// barriers.  Deadlock-free: the first pass takes chunks in global order, the replays walk each list
+// order, record them, and replay the same list in the phase's later passes.  Deadlock-free: the first pass takes chunks in global order, the replays walk each list
 // in increasing order, so the earliest unfinished visit always has its predecessors done.  A wave
 // takes at most kOwnMax chunks per phase; with more chunks than all waves can own, the phase uses the
 // round-robin ownership of flow_passes instead.
@@ -1802,7 +1894,7 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
     int per_sub = 1 + P.iterations + P.relaxations;
     const Bodies& B = P.B;
     for (int b = t0; b < P.nb; b += stride) st_vw(P, b, ld3(B.vel[b]), ld3(B.angvel[b]), 0);
-    if (P.queue)  // visible after the first grid barrier
+    if (P.qmode)  // visible after the first grid barrier
         for (int i = t0; i < 2 * P.ts.substeps; i += stride) P.queue[i] = 0;
     for (int s = 0; s < P.ts.substeps; ++s) {
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
@@ -1814,7 +1906,8 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
             st_vw(P, b, v, w, n);
         }
         grid_barrier(P.bar);
-        if (P.queue) flow_passes_own(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s, own);
+        if (P.qmode == 2) flow_passes_queue(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s);
+        else if (P.qmode == 1) flow_passes_own(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s, own);
         else flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
         grid_barrier(P.bar);
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
@@ -1828,8 +1921,10 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
         }
         grid_barrier(P.bar);
         if (P.relaxations > 0) {
-            if (P.queue) flow_passes_own(P, s * per_sub + 1 + P.iterations, P.relaxations, 2, 2, P.queue + 2 * s + 1, own);
-            else flow_passes(P, s * per_sub + 1 + P.iterations, P.relaxations, 2, 2);
+            int g0 = s * per_sub + 1 + P.iterations;
+            if (P.qmode == 2) flow_passes_queue(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1);
+            else if (P.qmode == 1) flow_passes_own(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1, own);
+            else flow_passes(P, g0, P.relaxations, 2, 2);
             grid_barrier(P.bar);
         }
     }

@@ -204,8 +204,8 @@ struct jh_world {
     int kind = SOLVER_FLOW;
     // lane solver (k_solve_lanes): workgroups and arbiter records
     int lane_blocks = 0, lane_max = 0;
-    int far_links = 3, far_sleep = 2;  // dataflow polling back-off (SolveArgs)
-    bool flow_queue = true;            // k_solve_flow: chunk ownership from counters (else round-robin)
+    int flow_qmode = 2;                // k_solve_flow chunk ownership (SolveArgs.qmode)
+    DBuf<float4> soa_mv;               // versioned mutable contact fields of k_solve_flow
     DBuf<int> fqueue;
     DBuf<float4> rec;
     // tile solver (one workgroup per Morton range of bodies, private bodies in LDS)
@@ -379,6 +379,7 @@ jh_status ensure_step_buffers(jh_world* w) {
         HIPCHK(w->soa_fs.ensure(cap));
         HIPCHK(w->soa_flag.ensure(4 * cap));
         HIPCHK(w->soa_f.ensure(4 * kSoaFields * cap));
+        HIPCHK(w->soa_mv.ensure(4 * 3 * cap));
         w->soa_cap = int(cap);
     }
     HIPCHK(w->ok0.ensure(na));
@@ -712,10 +713,10 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.soa.f = w->soa_f.p;
         S.soa.cap = w->soa_cap;
         S.vw = w->vwpack.p;
-        if (w->flow_queue) HIPCHK(w->fqueue.ensure(2 * size_t(cfg.substep_count)));
-        S.queue = w->flow_queue ? w->fqueue.p : nullptr;
-        S.far_links = w->far_links;
-        S.far_sleep = w->far_sleep;
+        HIPCHK(w->fqueue.ensure(2 * size_t(cfg.substep_count)));
+        S.queue = w->fqueue.p;
+        S.qmode = w->flow_qmode;
+        S.soa.mv = kind == SOLVER_FLOW ? w->soa_mv.p : nullptr;
         if (probe) {
             s = run_probe(w, S, a_cur, amax_, probe);
             if (s != JH_OK) return s;
@@ -943,9 +944,7 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
         w->flow_blocks = std::max(1, std::min(atoi(e), w->flow_max));
         w->lane_blocks = std::max(1, std::min(atoi(e), w->lane_max));
     }
-    if (const char* e = getenv("JH_FLOW_QUEUE")) w->flow_queue = atoi(e) != 0;
-    if (const char* e = getenv("JH_FAR_LINKS")) w->far_links = std::max(1, atoi(e));
-    if (const char* e = getenv("JH_FAR_SLEEP")) w->far_sleep = std::max(0, std::min(atoi(e), 64));
+    if (const char* e = getenv("JH_FLOW_QUEUE")) w->flow_qmode = std::max(0, std::min(atoi(e), 2));
     w->tiles = w->cu_count;
     if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
     w->colour_blocks = std::min(64, w->coop_colour);
@@ -967,7 +966,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->mk0.release(); w->mk1.release(); w->mv0.release(); w->mv1.release();
     w->left.release(); w->right.release(); w->parent.release(); w->leaf_parent.release(); w->nflags.release();
     w->range.release(); w->bounds.release(); w->hist.release(); w->pk0.release(); w->pk1.release();
-    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release(); w->epa_res.release(); w->fqueue.release();
+    w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release(); w->epa_res.release(); w->fqueue.release(); w->soa_mv.release();
     w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }

@@ -94,10 +94,10 @@ def test_heap_bitexact_barrier_solver(monkeypatch):
 
 
 def test_dataflow_solver_launch_shapes(monkeypatch):
-    # the dataflow solver is exact for any number of resident waves and either chunk ownership
-    # (taken from counters in each phase's first pass, or round-robin)
+    # the dataflow solver is exact for any number of resident waves and any chunk ownership (taken
+    # afresh from counters in every pass, taken in each phase's first pass and kept, round-robin)
     monkeypatch.setenv("JH_SOLVER", "flow")
-    for queue in ("1", "0"):
+    for queue in ("2", "1", "0"):
         monkeypatch.setenv("JH_FLOW_QUEUE", queue)
         for blocks in ("1", "7", "64"):
             monkeypatch.setenv("JH_SOLVE_BLOCKS", blocks)
