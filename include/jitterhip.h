@@ -73,7 +73,8 @@ typedef struct jh_config {
     int32_t pair_capacity;               /* broadphase pair buffer size; 0 = 24 per shape         */
     int32_t warm_start;                  /* JH_WARM_START_*; no counterpart in the reference      */
     int32_t narrowphase_filter;          /* World.NarrowPhaseFilter: JH_FILTER_*, default none     */
-    int32_t reserved[4];
+    float speculative_velocity_threshold; /* World.SpeculativeVelocityThreshold, default 10       */
+    int32_t reserved[3];
 } jh_config;
 
 /* World.NarrowPhaseFilter (World.Detect.cs:197-203).  TRIANGLE_EDGE = TriangleEdgeCollisionFilter
@@ -109,7 +110,10 @@ typedef struct jh_body {
     int32_t motion_type;             /* JH_MOTION_*                                                 */
     int32_t affected_by_gravity;     /* RigidBody.AffectedByGravity, default 1                     */
     int32_t world_id;                /* independent sub-world (batched worlds); 0 for one world    */
-    int32_t reserved;
+    int32_t enable_speculative_contacts; /* RigidBody.EnableSpeculativeContacts, default 0:
+                                        the body's broadphase box is swept over dt * |velocity|
+                                        and pairs MPR finds separated are swept for a time of
+                                        impact (World.Detect.cs:162-194)                          */
 } jh_body;
 
 /* One shape attached to a body.  shape_id is Shape.ShapeId (Collision/Shapes/Shape.cs:28); shapes
@@ -158,7 +162,8 @@ typedef struct jh_arbiter {
 typedef struct jh_manifold {
     uint32_t shape1, shape2;
     int32_t count;          /* manifold points (CollisionManifold.Count), 1..4             */
-    int32_t flags;          /* bit 0: EPA refined the MPR answer                           */
+    int32_t flags;          /* bit 0: EPA refined the MPR answer; bit 1: speculative contact
+                               (NarrowPhase.Sweep, one point, SolveMode.Angular removed)      */
     float normal[3];
     float penetration;
     float point_a[3];       /* MPR/EPA deepest point on shape 1                            */

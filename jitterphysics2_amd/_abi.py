@@ -37,7 +37,8 @@ CONFIG_DTYPE = np.dtype([
     ("pair_capacity", "<i4"),
     ("warm_start", "<i4"),
     ("narrowphase_filter", "<i4"),
-    ("reserved", "<i4", 4),
+    ("speculative_velocity_threshold", "<f4"),
+    ("reserved", "<i4", 3),
 ])
 
 FILTER_NONE, FILTER_TRIANGLE_EDGE = 0, 1
@@ -76,7 +77,7 @@ BODY_DTYPE = np.dtype([
     ("motion_type", "<i4"),
     ("affected_by_gravity", "<i4"),
     ("world_id", "<i4"),
-    ("reserved", "<i4"),
+    ("enable_speculative_contacts", "<i4"),
 ])
 
 SHAPE_DTYPE = np.dtype([
@@ -166,6 +167,7 @@ def default_config() -> np.ndarray:
     c["enable_auxiliary_contact_points"] = 1
     c["persistent_contact_manifold"] = 1
     c["pair_capacity"] = 0
+    c["speculative_velocity_threshold"] = 10.0
     return c
 
 

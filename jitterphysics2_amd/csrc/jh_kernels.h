@@ -9,6 +9,7 @@
 #include "jh_epa.h"
 #include "jh_epa_wave.h"
 #include "jh_filter.h"
+#include "jh_sweep.h"
 
 namespace cg = cooperative_groups;
 
@@ -25,7 +26,7 @@ struct Bodies {
     float4* il0;          // body-frame inverse inertia M11 M12 M13 M22
     float2* il1;          // M23 M33
     float4* mat;          // inverse mass, friction, restitution, linear damping multiplier
-    float4* mat2;         // angular damping multiplier, -, -, -
+    float4* mat2;         // angular damping multiplier, EnableSpeculativeContacts (1 or 0), -, -
     float4 *force, *torque;
     int* motion;          // MotionType
     int* gravity;         // AffectedByGravity
@@ -107,7 +108,7 @@ __global__ void k_unpack_bodies(const jh_body* __restrict__ in, int n, Bodies B)
     B.il0[b] = make_float4(s.inverse_inertia[0], s.inverse_inertia[1], s.inverse_inertia[2], s.inverse_inertia[3]);
     B.il1[b] = make_float2(s.inverse_inertia[4], s.inverse_inertia[5]);
     B.mat[b] = make_float4(s.inverse_mass, s.friction, s.restitution, s.linear_damping_multiplier);
-    B.mat2[b] = make_float4(s.angular_damping_multiplier, 0, 0, 0);
+    B.mat2[b] = make_float4(s.angular_damping_multiplier, s.enable_speculative_contacts ? 1.0f : 0.0f, 0, 0);
     B.force[b] = make_float4(s.force[0], s.force[1], s.force[2], 0);
     B.torque[b] = make_float4(s.torque[0], s.torque[1], s.torque[2], 0);
     B.motion[b] = s.motion_type;
@@ -130,13 +131,14 @@ __global__ void k_pack_bodies(Bodies B, int n, jh_body* __restrict__ out) {
     v = B.il0[b]; o.inverse_inertia[0] = v.x; o.inverse_inertia[1] = v.y; o.inverse_inertia[2] = v.z; o.inverse_inertia[3] = v.w;
     float2 w = B.il1[b]; o.inverse_inertia[4] = w.x; o.inverse_inertia[5] = w.y;
     v = B.mat[b]; o.inverse_mass = v.x; o.friction = v.y; o.restitution = v.z; o.linear_damping_multiplier = v.w;
-    o.angular_damping_multiplier = B.mat2[b].x;
+    float2 m2 = make_float2(B.mat2[b].x, B.mat2[b].y);
+    o.angular_damping_multiplier = m2.x;
     v = B.force[b]; o.force[0] = v.x; o.force[1] = v.y; o.force[2] = v.z;
     v = B.torque[b]; o.torque[0] = v.x; o.torque[1] = v.y; o.torque[2] = v.z;
     o.motion_type = B.motion[b];
     o.affected_by_gravity = B.gravity[b];
     o.world_id = B.world[b];
-    o.reserved = 0;
+    o.enable_speculative_contacts = m2.y != 0.0f;
 }
 
 // RigidBodyData in the reference's 128-byte FP32 layout (RigidBody.cs:107-160).
@@ -170,9 +172,12 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-// World AABB per shape; bounds[0..2] = min centre (ordered), bounds[3..5] = max centre.
+// World AABB per shape; bounds[0..2] = min centre (ordered), bounds[3..5] = max centre.  Bodies with
+// speculative contacts get the box grown by bdt * |velocity| on every side, bdt = the length of the
+// previous step (RigidBodyShape.UpdateWorldBoundingBox(BroadphaseStepDt), RigidBodyShape.cs:37-58).
 __global__ void k_aabb(const ShapeRec* __restrict__ shapes, int ns, const float* __restrict__ verts, Bodies B,
-                       float4* __restrict__ amin, float4* __restrict__ amax, uint32_t* __restrict__ bounds) {
+                       float bdt, float4* __restrict__ amin, float4* __restrict__ amax,
+                       uint32_t* __restrict__ bounds) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     vec3 c{0, 0, 0};
     bool valid = i < ns;
@@ -180,6 +185,11 @@ __global__ void k_aabb(const ShapeRec* __restrict__ shapes, int ns, const float*
         ShapeRec s = shapes[i];
         vec3 mn, mx;
         shape_aabb(s, verts, ldq(B.quat[s.body]), ld3(B.pos[s.body]), mn, mx);
+        if (B.mat2[s.body].y != 0.0f) {
+            float swept = bdt * sqrtf(len_sq(ld3(B.vel[s.body])));
+            mn = mn - vec3{swept, swept, swept};
+            mx = mx + vec3{swept, swept, swept};
+        }
         amin[i] = st3(mn);
         amax[i] = st3(mx);
         c = 0.5f * (mn + mx);
@@ -559,10 +569,15 @@ enum { kDeepPlain = 0, kDeepFilter = 1, kDeepRerun = 2 };
 // than EpaPenetrationThreshold (NarrowPhase.cs:877) go to the EPA queue.  With the triangle edge
 // filter on, filtered pairs run MPR with threshold 0 so the portal is kept for the filter's re-run;
 // the MPR answer is the same for any threshold.
+// Separated pairs with a body that has speculative contacts and a relative speed of at least the
+// speculative velocity threshold (spec_thr_sq = its square) go to the sweep queue (World.Detect.cs:
+// 167-176); spec_any = 0 when no body has speculative contacts.
 __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
                                                 const ShapeRec* __restrict__ shapes, const float* __restrict__ verts,
                                                 Bodies B, int aux, int filter, jh_manifold* __restrict__ out,
-                                                DeepRec* __restrict__ deep, int* __restrict__ deep_count) {
+                                                DeepRec* __restrict__ deep, int* __restrict__ deep_count,
+                                                int spec_any, float spec_thr_sq, int* __restrict__ spec_queue,
+                                                int* __restrict__ spec_count) {
     int n = *n_ptr;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         uint64_t key = pairs[p];
@@ -575,6 +590,13 @@ __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pai
         solve_mpr(mk, side ? 0.0f : 0.02f, r);
         if (!r.hit) {
             write_manifold(out[p], key, r, qa, pa, nullptr, 0);
+            if (spec_any) {
+                int b1 = mk.a.s->body, b2 = mk.b.s->body;
+                if (B.mat2[b1].y != 0.0f || B.mat2[b2].y != 0.0f) {
+                    vec3 dv = ld3(B.vel[b2]) - ld3(B.vel[b1]);
+                    if (!(len_sq(dv) < spec_thr_sq)) spec_queue[atomicAdd(spec_count, 1)] = p;
+                }
+            }
             continue;
         }
         if (!side) {
@@ -725,6 +747,55 @@ __global__ void __launch_bounds__(128) k_epa_finish(const uint64_t* __restrict__
         r.normal = e.normal;
         r.pen = e.pen;
         epa_finish(d, key, shapes, verts, B, mk, qa, qb, pa, pb, r, aux, out);
+    }
+}
+
+// The speculative branch of World.Detect (World.Detect.cs:167-194) for the queued pairs, one thread
+// per pair: a linear sweep (NarrowPhase.Sweep, NarrowPhase.cs:1100-1137); a time of impact inside the
+// step (0 < toi <= dt) gives one contact point with penetration normal . (pA - pB) * specrel, after
+// the triangle edge filter when it applies.  Flags bit 1 marks it for RegisterContact with
+// SolveMode.Angular removed.
+__global__ void __launch_bounds__(64) k_sweep(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
+                                              const float* __restrict__ verts, Bodies B, int filter, float dt,
+                                              float specrel, const int* __restrict__ spec_queue,
+                                              const int* __restrict__ spec_count, jh_manifold* __restrict__ out) {
+    int n = *spec_count;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        int p = spec_queue[q];
+        uint64_t key = pairs[p];
+        MinkowskiPair mk;
+        quat qa, qb;
+        vec3 pa, pb;
+        pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
+        vec3 va = ld3(B.vel[mk.a.s->body]), vb = ld3(B.vel[mk.b.s->body]);
+        tef::WRes w;
+        float toi;
+        if (!sweep_pair(mk, qa, pa, va, vb, w.pa, w.pb, w.n, toi)) continue;
+        if (toi > dt || toi == 0.0f) continue;
+        w.pen = dot(w.n, w.pa - w.pb) * specrel;
+        int side = filter ? tef::applies(*mk.a.s, *mk.b.s, B.motion) : 0;
+        if (side) {
+            // TriangleEdgeCollisionFilter.Filter (:149-323): a speculative answer (penetration < 0)
+            // skips the re-run; otherwise the re-run's MPR finds the pair separated again: discard.
+            tef::Ctx c = filter_ctx(side, shapes, key, verts, qa, qb, pa, pb);
+            vec3 tn, nn;
+            int e = tef::eval_res(c, w, tn, nn);
+            if (e == tef::Discard) continue;
+            if (e == tef::Edge && !(w.pen < 0.0f && tef::adjust(w, tn, nn))) continue;
+        }
+        jh_manifold& o = out[p];
+        o.flags = 2;
+        o.count = 1;
+        o.normal[0] = w.n.x; o.normal[1] = w.n.y; o.normal[2] = w.n.z;
+        o.penetration = w.pen;
+        o.point_a[0] = w.pa.x; o.point_a[1] = w.pa.y; o.point_a[2] = w.pa.z;
+        o.point_b[0] = w.pb.x; o.point_b[1] = w.pb.y; o.point_b[2] = w.pb.z;
+        for (int i = 0; i < 4; ++i) {
+            vec3 a = i == 0 ? w.pa : vec3{0, 0, 0};
+            vec3 b = i == 0 ? w.pb : vec3{0, 0, 0};
+            o.manifold_a[i][0] = a.x; o.manifold_a[i][1] = a.y; o.manifold_a[i][2] = a.z;
+            o.manifold_b[i][0] = b.x; o.manifold_b[i][1] = b.y; o.manifold_b[i][2] = b.z;
+        }
     }
 }
 
@@ -912,9 +983,11 @@ __global__ void k_register(const uint64_t* __restrict__ ckey, const int* __restr
         jh_arbiter cd = arb[pos];
         const jh_manifold& m = man[cpair[c]];
         if (!persistent) cd.usage_mask = 0;
-        // ContactData.ResetMode (Contact.cs:291-296)
+        // ContactData.ResetMode (Contact.cs:291-296); speculative contacts remove SolveMode.Angular
+        // (World.Detect.cs:191-192)
         cd.mode = (B.motion[cd.body1] == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1) : 0;
         cd.mode |= (B.motion[cd.body2] == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2) : 0;
+        if (m.flags & 2) cd.mode &= ~uint32_t(JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2);
         vec3 n3{m.normal[0], m.normal[1], m.normal[2]};
         for (int e = 0; e < m.count; ++e) {
             vec3 a{m.manifold_a[e][0], m.manifold_a[e][1], m.manifold_a[e][2]};

@@ -79,6 +79,7 @@ enum {
     C_CONTACTS = 12,  // contact points solved
     C_NCHUNK = 13,    // dataflow solver chunks
     C_NRANK = 14,     // rank keys
+    C_SPEC = 15,      // speculative sweep queue
     C_NMOV = 16,      // tile solver: non-static bodies with shapes
     C_TCELLS = 17,    // tile solver: (tile, colour) cells + 1
     C_SERR = 18,      // tile solver: a dependency wait gave up
@@ -143,6 +144,8 @@ struct jh_world {
     hipStream_t st = nullptr;
     int nb = 0, ns = 0, nv = 0;
     int max_world = 0;
+    int n_spec = 0;              // bodies with speculative contacts
+    float broadphase_dt = 0.1f;  // World.BroadphaseStepDt (World.cs:416-418): dt of the last step
     std::vector<jh_body> host_bodies;
     // bodies
     DBuf<float4> pos, quat, vel, angvel, dvel, dangvel, imv, iw0, il0, mat, mat2, force, torque;
@@ -172,6 +175,7 @@ struct jh_world {
     DBuf<DeepRec> deep;
     DBuf<char> epa_work, epa_side;
     DBuf<EpaRes> epa_res;
+    DBuf<int> specq;  // pairs queued for the speculative sweep
     int epa_threads = 0;
     bool epa_wave = true;      // wave-cooperative EPA (k_epa_wave); false: one thread per pair (k_epa)
     int epa_wave_blocks = 0;   // resident single-wave workgroups of k_epa_wave
@@ -254,7 +258,7 @@ namespace {
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
     if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK ||
-        t == C_SERR || t == C_MAXDEG)
+        t == C_SERR || t == C_MAXDEG || t == C_SPEC)
         cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
     else if (t < 6) bounds[t] = 0u;
@@ -302,7 +306,7 @@ jh_status broadphase(jh_world* w) {
         return JH_OK;
     }
     int g = grid_for(ns, 256, 1 << 20);
-    k_aabb<<<g, 256, 0, st>>>(w->shapes.p, ns, w->verts.p, B, w->amin.p, w->amax.p, w->bounds.p);
+    k_aabb<<<g, 256, 0, st>>>(w->shapes.p, ns, w->verts.p, B, w->broadphase_dt, w->amin.p, w->amax.p, w->bounds.p);
     k_morton<<<g, 256, 0, st>>>(w->shapes.p, ns, w->world.p, w->amin.p, w->amax.p, w->bounds.p, w->mk0.p, w->mv0.p);
     int shifts[8];
     int nsh = 0;
@@ -580,9 +584,17 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     int aux = cfg.enable_auxiliary_contact_points;
     // narrowphase
     if (np > 0) {
+        int spec = w->n_spec > 0;
+        if (spec) HIPCHK(w->specq.ensure(size_t(np)));
+        float thr = cfg.speculative_velocity_threshold;
         k_narrow<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->verts.p, B, aux,
                                                              cfg.narrowphase_filter, w->man.p, w->deep.p,
-                                                             cnt + C_DEEP);
+                                                             cnt + C_DEEP, spec, thr * thr, w->specq.p,
+                                                             cnt + C_SPEC);
+        if (spec)
+            k_sweep<<<grid_for(np, 64, 4096), 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, cfg.narrowphase_filter,
+                                                           dt, cfg.speculative_relaxation_factor, w->specq.p,
+                                                           cnt + C_SPEC, w->man.p);
         HIPCHK(hipEventRecord(w->ev[7], st));
         if (w->epa_wave) {
             int blocks = std::max(1, std::min(w->epa_wave_blocks, np));
@@ -816,6 +828,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     HIPCHK(hipEventRecord(w->ev[6], st));
     HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    // DynamicTree.Update(multiThread, StepDt) (World.Step.cs:274-276)
+    w->broadphase_dt = dt;
     if (w->hcnt[C_SERR]) return fail(JH_ERR_HIP, "tile solver: a dependency wait did not finish");
     // solve-order bookkeeping and stats
     w->a_host = w->hcnt[w->cur == 0 ? C_A0 : C_A1];
@@ -889,6 +903,7 @@ jh_status jh_config_defaults(jh_config* c) {
     c->speculative_relaxation_factor = 0.9f;
     c->enable_auxiliary_contact_points = 1;
     c->persistent_contact_manifold = 1;
+    c->speculative_velocity_threshold = 10.0f;
     return JH_OK;
 }
 
@@ -1012,6 +1027,8 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     }
     w->nb = n;
     w->max_world = maxw;
+    w->n_spec = 0;
+    for (int i = 0; i < n; ++i) w->n_spec += bodies[i].enable_speculative_contacts != 0;
     w->host_bodies.assign(bodies, bodies + n);
     {
         // bodies grouped by world (counting sort), worlds scheduled largest first

@@ -452,6 +452,60 @@ def full_form(n_bodies: int = 262144, cells: int = 512, worlds: int = 64, seed: 
     return (*sb.arrays(), cfg, 1.0 / 60.0)
 
 
+def bullets(n_bodies: int = 512, seed: int = 0x4A2D0006, speed=(15.0, 60.0), speculative: float = 1.0,
+            mesh: bool = False, arena: float = 30.0, hulls: bool = True):
+    """Fast projectiles against thin static slabs: a floor (a flat box, or with mesh=True a wavy
+    triangle-mesh heightfield and the TriangleEdgeCollisionFilter), four walls and a middle slab
+    0.2 thick around an `arena` x `arena` area, and dynamic spheres, boxes, capsules (and 16-point
+    hulls) at heights 1..20 with speeds U(speed) in random directions biased downwards.  A fraction
+    `speculative` of the bodies has speculative contacts on (RigidBody.EnableSpeculativeContacts).
+    At these speeds a body moves further than a slab is thick in one step: without speculative
+    contacts it tunnels.  dt = 1/60, 4 substeps, 10 iterations, speculative velocity threshold 10."""
+    rng = SplitMix64(seed)
+    sb = SceneBuilder()
+    h = arena / 2.0
+    t = 0.2
+    if mesh:
+        cells = 16
+        verts, tris, _ = heightfield(cells, arena / cells, 1.0, seed, -h, -h, feature=8.0)
+        g = sb.body((0.0, 0.0, 0.0), motion=MOTION_STATIC, friction=0.5)
+        sb.bodies[g]["inverse_mass"] = 0.0
+        sb.bodies[g]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+        sb.triangle_mesh(g, verts, tris)
+    else:
+        _static_box(sb, (0.0, -t / 2.0, 0.0), (arena + 2.0 * t, t, arena + 2.0 * t))
+    _static_box(sb, (-(h + t / 2.0), 10.0, 0.0), (t, 20.0, arena + 2.0 * t))
+    _static_box(sb, ((h + t / 2.0), 10.0, 0.0), (t, 20.0, arena + 2.0 * t))
+    _static_box(sb, (0.0, 10.0, -(h + t / 2.0)), (arena, 20.0, t))
+    _static_box(sb, (0.0, 10.0, (h + t / 2.0)), (arena, 20.0, t))
+    _static_box(sb, (0.0, 6.0, 0.0), (t, 12.0, arena * 0.6))
+    kinds = 4 if hulls else 3
+    for _ in range(n_bodies):
+        pos = (rng.uniform(-h + 1.0, h - 1.0), rng.uniform(1.5, 20.0), rng.uniform(-h + 1.0, h - 1.0))
+        d = np.array([rng.normal(), rng.normal() - 0.7, rng.normal()])
+        d = d / np.linalg.norm(d) * rng.uniform(*speed)
+        w = (rng.normal(), rng.normal(), rng.normal())
+        b = sb.body(pos, orientation=rng.quaternion(), velocity=tuple(d), angular_velocity=w)
+        k = int(rng.uniform() * kinds) % kinds
+        if k == 3:
+            pts = []
+            scale = rng.uniform(0.3, 0.8)
+            for _ in range(16):
+                q = np.array([rng.normal(), rng.normal(), rng.normal()])
+                pts.append(q / np.linalg.norm(q) * scale)
+            sb.hull(b, pts)
+        else:
+            _random_shape(sb, rng, b, (SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE)[k:k + 1])
+        sb.bodies[b]["enable_speculative_contacts"] = 1 if rng.uniform() < speculative else 0
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    cfg["solver_iterations"] = 10
+    cfg["relaxation_iterations"] = 4
+    if mesh:
+        cfg["narrowphase_filter"] = FILTER_TRIANGLE_EDGE
+    return (*sb.arrays(), cfg, 1.0 / 60.0)
+
+
 def two_bodies(shape_a, shape_b, pos_a, pos_b, q_a=(0, 0, 0, 1), q_b=(0, 0, 0, 1), static_a=False):
     """Tiny two-body scene for unit tests; shape_x = ("box", size) | ("sphere", r) | ("capsule", r, len)."""
     sb = SceneBuilder()

@@ -24,6 +24,7 @@ template <class R> void set_config(World<R>& w, const jh_config* c) {
     w.iterations = c->solver_iterations;
     w.relaxations = c->relaxation_iterations;
     w.specrel = R(c->speculative_relaxation_factor);
+    w.spec_threshold = R(c->speculative_velocity_threshold);
     w.aux_contacts = c->enable_auxiliary_contact_points != 0;
     w.persistent = c->persistent_contact_manifold != 0;
     w.triangle_edge_filter = c->narrowphase_filter == JH_FILTER_TRIANGLE_EDGE;
@@ -54,6 +55,7 @@ template <class R> void set_bodies(World<R>& w, const jh_body* b, int n) {
         o.motion = b[i].motion_type;
         o.gravity = b[i].affected_by_gravity != 0;
         o.world = b[i].world_id;
+        o.speculative = b[i].enable_speculative_contacts != 0;
         o.update_world_inertia();
     }
 }
@@ -175,7 +177,7 @@ template <class R> int get_manifolds(const World<R>& w, jh_manifold* out, int ca
             o.shape1 = m.k1;
             o.shape2 = m.k2;
             o.count = m.count;
-            o.flags = m.epa ? 1 : 0;
+            o.flags = (m.epa ? 1 : 0) | (m.spec ? 2 : 0);
             vout(m.normal, o.normal);
             o.penetration = float(m.pen);
             vout(m.pa, o.point_a);
@@ -459,6 +461,59 @@ int orc_contact_op(int fp64, jh_body* bodies, jh_arbiter* arb, int op, float dt,
                    const float* p2, const float* n) {
     if (fp64) return contact_op_t<double>(bodies, arb, op, dt, substeps, p1, p2, n);
     return contact_op_t<float>(bodies, arb, op, dt, substeps, p1, p2, n);
+}
+// SimplexSolverAB: AddVertex for n vertices (V, A, B: [n][3]) in order, then GetClosest.  results[i] =
+// AddVertex's return for vertex i; out: [0..2] the last closest point, [3..5] pointA, [6..8] pointB,
+// [9] the used-slot mask.
+}  // extern "C"
+namespace {
+template <class R> int simplex_ab_t(int n, const double* V, const double* A, const double* B, int* results, double* out) {
+    SimplexAB<R> s;
+    V3<R> closest{};
+    for (int i = 0; i < n; ++i) {
+        Vertex<R> x;
+        x.V = {R(V[3 * i]), R(V[3 * i + 1]), R(V[3 * i + 2])};
+        x.A = {R(A[3 * i]), R(A[3 * i + 1]), R(A[3 * i + 2])};
+        x.B = {R(B[3 * i]), R(B[3 * i + 1]), R(B[3 * i + 2])};
+        results[i] = s.add_vertex(x, closest) ? 1 : 0;
+    }
+    V3<R> pa, pb;
+    s.get_closest(pa, pb);
+    const V3<R> o[3] = {closest, pa, pb};
+    for (int k = 0; k < 3; ++k) { out[3 * k] = double(o[k].x); out[3 * k + 1] = double(o[k].y); out[3 * k + 2] = double(o[k].z); }
+    out[9] = double(s.mask);
+    return 0;
+}
+}  // namespace
+extern "C" {
+int orc_simplex_ab(int fp64, int n, const double* V, const double* A, const double* B, int* results, double* out) {
+    if (n < 0) return -1;
+    if (fp64) return simplex_ab_t<double>(n, V, A, B, results, out);
+    return simplex_ab_t<float>(n, V, A, B, results, out);
+}
+// NarrowPhase.Sweep of two posed shapes with linear velocities va, vb.  out: [0..2] pointA, [3..5]
+// pointB, [6..8] normal, [9] lambda.  Returns 1 if the sweep succeeded.
+}  // extern "C"
+namespace {
+template <class R>
+int sweep_t(const jh_shape* a, const jh_shape* b, const float* verts, const float* qa, const float* pa, const float* va,
+            const float* qb, const float* pb, const float* vb, double* out, Shape<R> (*mk)(const jh_shape&, const float*)) {
+    Shape<R> sa = mk(*a, verts), sb = mk(*b, verts);
+    Q4<R> QA{R(qa[0]), R(qa[1]), R(qa[2]), R(qa[3])}, QB{R(qb[0]), R(qb[1]), R(qb[2]), R(qb[3])};
+    V3<R> A, B, N;
+    R lambda;
+    bool ok = sweep(sa, sb, QA, QB, vin<R>(pa), vin<R>(pb), vin<R>(va), vin<R>(vb), A, B, N, lambda);
+    const V3<R> o[3] = {A, B, N};
+    for (int k = 0; k < 3; ++k) { out[3 * k] = double(o[k].x); out[3 * k + 1] = double(o[k].y); out[3 * k + 2] = double(o[k].z); }
+    out[9] = double(lambda);
+    return ok ? 1 : 0;
+}
+}  // namespace
+extern "C" {
+int orc_sweep(int fp64, const jh_shape* a, const jh_shape* b, const float* verts, const float* qa, const float* pa,
+              const float* va, const float* qb, const float* pb, const float* vb, double* out) {
+    if (fp64) return sweep_t<double>(a, b, verts, qa, pa, va, qb, pb, vb, out, shape_d);
+    return sweep_t<float>(a, b, verts, qa, pa, va, qb, pb, vb, out, shape_f);
 }
 // StableMath.SinCos / Atan2 in FP32 (float) for the trigonometry checks.
 void orc_sincos_f(float a, float* s, float* c) { StableMath<float>::sincos(a, *s, *c); }

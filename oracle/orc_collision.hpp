@@ -921,4 +921,233 @@ template <class R> struct Manifold {
     }
 };
 
+// ---- Speculative contacts: linear sweep ---------------------------------------------------------
+// SimplexSolverAB (Collision/NarrowPhase/SimplexSolverAB.cs): the point of a simplex of Minkowski
+// vertices closest to the origin, with barycentric weights kept for the witness points.  Weights are
+// stored per slot; only the slots of `mask` are meaningful (the reference copies whole weight sets).
+template <class R> struct SimplexAB {
+    static constexpr R Epsilon = R(1e-8);
+    Vertex<R> v[4];
+    R bc[4] = {R(0), R(0), R(0), R(0)};
+    uint32_t mask = 0;
+
+    // ClosestSegment (:62-104)
+    V3<R> closest_segment(int i0, int i1, R* b, uint32_t& m) const {
+        V3<R> a = v[i0].V, bb = v[i1].V;
+        V3<R> d = bb - a;
+        R vsq = len_sq(d);
+        if (vsq < Epsilon) {
+            m = 1u << i1;
+            b[i0] = R(0.0);
+            b[i1] = R(1.0);
+            return bb;
+        }
+        R t = -dot(a, d) / vsq;
+        if (t > R(1.0)) {
+            m = 1u << i1;
+            b[i0] = R(0.0);
+            b[i1] = R(1.0);
+            return bb;
+        }
+        if (t < R(0.0)) {
+            m = 1u << i0;
+            b[i0] = R(1.0);
+            b[i1] = R(0.0);
+            return a;
+        }
+        b[i0] = R(1.0) - t;
+        b[i1] = t;
+        m = (1u << i0) | (1u << i1);
+        return cross(d, cross(a, d)) * (R(1.0) / vsq);
+    }
+
+    // ClosestTriangle (:106-185)
+    V3<R> closest_triangle(int i0, int i1, int i2, R* b, uint32_t& m) const {
+        m = 0;
+        V3<R> a = v[i0].V, bb = v[i1].V, c = v[i2].V;
+        V3<R> u = a - bb;
+        V3<R> w = a - c;
+        V3<R> normal = cross(u, w);
+        R t = len_sq(normal);
+        R it = R(1.0) / t;
+        bool degenerate = t < Epsilon;
+        V3<R> c1 = cross(u, a);
+        V3<R> c2 = cross(a, w);
+        R lambda2 = dot(c1, normal) * it;
+        R lambda1 = dot(c2, normal) * it;
+        R lambda0 = R(1.0) - lambda2 - lambda1;
+        R best = std::numeric_limits<R>::max();
+        V3<R> closest{};
+        R b0[4];
+        if (lambda0 < R(0.0) || degenerate) {
+            uint32_t mm;
+            V3<R> cl = closest_segment(i1, i2, b0, mm);
+            R dist = len_sq(cl);
+            if (dist < best) { std::memcpy(b, b0, sizeof(b0)); m = mm; best = dist; closest = cl; }
+        }
+        if (lambda1 < R(0.0) || degenerate) {
+            uint32_t mm;
+            V3<R> cl = closest_segment(i0, i2, b0, mm);
+            R dist = len_sq(cl);
+            if (dist < best) { std::memcpy(b, b0, sizeof(b0)); m = mm; best = dist; closest = cl; }
+        }
+        if (lambda2 < R(0.0) || degenerate) {
+            uint32_t mm;
+            V3<R> cl = closest_segment(i0, i1, b0, mm);
+            R dist = len_sq(cl);
+            if (dist < best) { std::memcpy(b, b0, sizeof(b0)); m = mm; closest = cl; }
+        }
+        if (m != 0) return closest;
+        b[i0] = lambda0;
+        b[i1] = lambda1;
+        b[i2] = lambda2;
+        m = (1u << i0) | (1u << i1) | (1u << i2);
+        return normal * (dot(normal, a) * it);
+    }
+
+    // Determinant of ClosestTetrahedron: (b - a) . ((c - a) x (d - a))
+    static R det(V3<R> a, V3<R> b, V3<R> c, V3<R> d) { return dot(b - a, cross(c - a, d - a)); }
+
+    // ClosestTetrahedron (:187-279)
+    V3<R> closest_tetrahedron(R* b, uint32_t& m) const {
+        const V3<R> z{R(0), R(0), R(0)};
+        R detT = det(v[0].V, v[1].V, v[2].V, v[3].V);
+        R inv = R(1.0) / detT;
+        bool degenerate = detT * detT < Epsilon;
+        R lambda0 = det(z, v[1].V, v[2].V, v[3].V) * inv;
+        R lambda1 = det(v[0].V, z, v[2].V, v[3].V) * inv;
+        R lambda2 = det(v[0].V, v[1].V, z, v[3].V) * inv;
+        R lambda3 = R(1.0) - lambda0 - lambda1 - lambda2;
+        R best = std::numeric_limits<R>::max();
+        V3<R> closest{};
+        R b0[4];
+        m = 0;
+        static const int face[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+        const R lam[4] = {lambda0, lambda1, lambda2, lambda3};
+        for (int f = 0; f < 4; ++f) {
+            if (!(lam[f] < R(0.0) || degenerate)) continue;
+            uint32_t mm;
+            V3<R> cl = closest_triangle(face[f][0], face[f][1], face[f][2], b0, mm);
+            R dist = len_sq(cl);
+            if (dist < best) {
+                std::memcpy(b, b0, sizeof(b0));
+                m = mm;
+                best = dist;
+                closest = cl;
+            }
+        }
+        if (m != 0) return closest;
+        b[0] = lambda0;
+        b[1] = lambda1;
+        b[2] = lambda2;
+        b[3] = lambda3;
+        m = 15u;
+        return z;
+    }
+
+    // GetClosest (:281-298): witness points from the weights of the used slots, in slot order.
+    void get_closest(V3<R>& pa, V3<R>& pb) const {
+        pa = V3<R>{R(0), R(0), R(0)};
+        pb = V3<R>{R(0), R(0), R(0)};
+        for (int i = 0; i < 4; ++i) {
+            if ((mask & (1u << i)) == 0) continue;
+            pa = pa + bc[i] * v[i].A;
+            pb = pb + bc[i] * v[i].B;
+        }
+    }
+
+    // AddVertex (:309-357): the new vertex takes the last free slot; used slots keep their order.
+    bool add_vertex(const Vertex<R>& x, V3<R>& closest) {
+        int ix[4];
+        int use = 0, free_slot = 0;
+        for (int i = 0; i < 4; ++i) {
+            if (mask & (1u << i)) ix[use++] = i;
+            else free_slot = i;
+        }
+        ix[use++] = free_slot;
+        v[free_slot] = x;
+        switch (use) {
+            case 1:
+                closest = v[ix[0]].V;
+                mask = 1u << ix[0];
+                bc[ix[0]] = R(1.0);
+                return true;
+            case 2: closest = closest_segment(ix[0], ix[1], bc, mask); return true;
+            case 3: closest = closest_triangle(ix[0], ix[1], ix[2], bc, mask); return true;
+            default: closest = closest_tetrahedron(bc, mask); return mask != 15u;
+        }
+    }
+};
+
+// JVector.NormalizeSafe (LinearMath/JVector.cs:561-567).
+template <class R> inline V3<R> normalize_safe(V3<R> v, R eps_sq) {
+    R len2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    if (len2 < eps_sq) return V3<R>{R(0), R(0), R(0)};
+    return (R(1.0) / std::sqrt(len2)) * v;
+}
+
+// NarrowPhase.Sweep with A fixed at the origin (NarrowPhase.cs:1155-1221): conservative advancement
+// of B along `sweep` against a SimplexSolverAB distance solver.  mk.pb is B's position at time 0.
+template <class R>
+bool sweep_local(MinkowskiPair<R> mk, V3<R> sweep, V3<R>& pointA, V3<R>& pointB, V3<R>& normal, R& lambda) {
+    const R collideEpsilon = R(1e-4);
+    const int maxIter = 34;
+    SimplexAB<R> s;
+    Vertex<R> center = mk.center();
+    const V3<R> positionB = mk.pb;
+    V3<R> posB = positionB;
+    lambda = R(0.0);
+    pointA = pointB = V3<R>{R(0), R(0), R(0)};
+    V3<R> r = sweep;
+    V3<R> v = -center.V;
+    normal = V3<R>{R(0), R(0), R(0)};
+    int iter = maxIter;
+    R distSq = len_sq(v);
+    while ((distSq > collideEpsilon * collideEpsilon) && (iter-- != 0)) {
+        mk.pb = posB;
+        Vertex<R> vx = mk.support(v);
+        V3<R> w = vx.V;
+        R vDotW = -dot(v, w);
+        if (vDotW > R(0.0)) {
+            R vDotR = dot(v, r);
+            if (vDotR >= -R(1e-12)) {
+                lambda = std::numeric_limits<R>::infinity();
+                return false;
+            }
+            lambda = lambda - vDotW / vDotR;
+            posB = positionB + lambda * r;
+            normal = v;
+        }
+        if (!s.add_vertex(vx, v)) break;
+        v = -v;
+        distSq = len_sq(v);
+    }
+    s.get_closest(pointA, pointB);
+    normal = normalize_safe(normal, R(1e-16));
+    return true;
+}
+
+// NarrowPhase.Sweep for two moving shapes, linear velocities (NarrowPhase.cs:1100-1137): the relative
+// sweep in A's frame, then the witness points back in world space at time 0.
+template <class R>
+bool sweep(const Shape<R>& sa, const Shape<R>& sb, Q4<R> qa, Q4<R> qb, V3<R> pa, V3<R> pb, V3<R> va, V3<R> vb,
+           V3<R>& pointA, V3<R>& pointB, V3<R>& normal, R& lambda) {
+    MinkowskiPair<R> mk;
+    mk.a = &sa;
+    mk.b = &sb;
+    mk.qb = qconjmul(qa, qb);
+    V3<R> position = pb - pa;
+    mk.pb = rotate_inv(position, qa);
+    V3<R> sw = vb - va;
+    sw = rotate_inv(sw, qa);
+    if (!sweep_local(mk, sw, pointA, pointB, normal, lambda)) return false;
+    pointA = rotate(pointA, qa);
+    pointA = pointA + pa;
+    pointB = rotate(pointB, qa);
+    pointB = pointB + pa;
+    normal = rotate(normal, qa);
+    pointB = pointB + lambda * (va - vb);
+    return true;
+}
+
 }  // namespace orc

@@ -27,6 +27,7 @@ template <class R> struct Body {
     R lin_damp = R(0.998), ang_damp = R(0.995);
     V3<R> force{}, torque{};
     bool gravity = true;
+    bool speculative = false;  // RigidBody.EnableSpeculativeContacts (RigidBody.cs:877-900)
     int world = 0;
 
     // RigidBody.UpdateWorldInertia (RigidBody.cs:903-923), no motion locks.
@@ -78,6 +79,7 @@ template <class R> struct StepTiming {
 template <class R> struct ManifoldOut {
     uint32_t k1, k2;
     bool epa;
+    bool spec;  // speculative contact: registered with SolveMode.Angular removed
     V3<R> normal, pa, pb;
     R pen;
     int count;
@@ -92,6 +94,8 @@ template <class R> struct World {
     V3<R> gravity{R(0), R(-9.81), R(0)};
     int substeps = 1, iterations = 6, relaxations = 4;
     R specrel = R(0.9);
+    R spec_threshold = R(10.0);  // World.SpeculativeVelocityThreshold (World.Detect.cs:122)
+    R broadphase_dt = R(0.1);    // World.BroadphaseStepDt (World.cs:416-418): StepDt of the last step
     bool aux_contacts = true, persistent = true, triangle_edge_filter = false;
     std::vector<ManifoldOut<R>> last_manifolds;
     std::vector<std::pair<uint32_t, uint32_t>> last_pairs;
@@ -101,9 +105,16 @@ template <class R> struct World {
     World() { poly = new Polytope<R>(); }
     ~World() { delete poly; }
 
+    // RigidBodyShape.UpdateWorldBoundingBox(BroadphaseStepDt) (RigidBodyShape.cs:37-58): bodies with
+    // speculative contacts get their box grown by dt * |velocity| on every side.
     void world_aabb(const Shape<R>& s, V3<R>& mn, V3<R>& mx) const {
         const Body<R>& b = bodies[s.body];
         s.aabb(b.q, b.pos, mn, mx);
+        if (b.speculative) {
+            R swept = broadphase_dt * length(b.vel);
+            mn = mn - V3<R>{swept, swept, swept};
+            mx = mx + V3<R>{swept, swept, swept};
+        }
     }
 
     // The pair set visited by DynamicTree.EnumerateOverlaps (DynamicTree.cs:266-286) after the
@@ -162,9 +173,10 @@ template <class R> struct World {
         a.usage = 0;
     }
     // ContactData.ResetMode (Contact.cs:291-296).
-    void reset_mode(Arbiter<R>& a) {
+    void reset_mode(Arbiter<R>& a, uint32_t remove = 0) {
         a.mode = (bodies[a.b1].motion == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1) : 0;
         a.mode |= (bodies[a.b2].motion == JH_MOTION_DYNAMIC) ? (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2) : 0;
+        a.mode &= ~remove;
     }
 
     // Contact.Initialize (Contact.cs:703-759).
@@ -341,7 +353,57 @@ template <class R> struct World {
         return true;
     }
 
-    void detect(uint32_t k1, uint32_t k2) {
+    // GetOrCreateArbiter (World.Detect.cs:417-430) with ContactData.Init (Contact.cs:271-284).
+    Arbiter<R>& arbiter_for(uint32_t k1, uint32_t k2) {
+        auto key = std::make_pair(k1, k2);
+        auto it = arbiters.find(key);
+        if (it == arbiters.end()) {
+            Arbiter<R> a;
+            init_arbiter(a, k1, k2);
+            it = arbiters.emplace(key, a).first;
+        }
+        return it->second;
+    }
+
+    // The speculative branch of World.Detect (World.Detect.cs:167-194) for a pair MPR found separated:
+    // a linear sweep; a time of impact inside this step registers one contact point whose solve mode
+    // drops the angular terms (RegisterContact(..., SolveMode.Angular), World.Detect.cs:331-337, 232-250).
+    void detect_speculative(uint32_t k1, uint32_t k2, R step_dt) {
+        const Shape<R>& sA = shapes[k1];
+        const Shape<R>& sB = shapes[k2];
+        const Body<R>& b1 = bodies[sA.body];
+        const Body<R>& b2 = bodies[sB.body];
+        if (!b1.speculative && !b2.speculative) return;
+        V3<R> dv = b2.vel - b1.vel;
+        if (len_sq(dv) < spec_threshold * spec_threshold) return;
+        MprResult<R> r{};
+        R toi;
+        bool ok = sweep(sA, sB, b1.q, b2.q, b1.pos, b2.pos, b1.vel, b2.vel, r.pa, r.pb, r.normal, toi);
+        if (!ok || toi > step_dt || toi == R(0.0)) return;
+        r.hit = true;
+        r.epa = false;
+        r.penetration = dot(r.normal, r.pa - r.pb) * specrel;
+        if (triangle_edge_filter && !tef_filter(sA, sB, b1, b2, r)) return;
+        ManifoldOut<R> mo{};
+        mo.k1 = k1;
+        mo.k2 = k2;
+        mo.epa = false;
+        mo.spec = true;
+        mo.normal = r.normal;
+        mo.pa = r.pa;
+        mo.pb = r.pb;
+        mo.pen = r.penetration;
+        mo.count = 1;
+        mo.mA[0] = r.pa;
+        mo.mB[0] = r.pb;
+        last_manifolds.push_back(mo);
+        Arbiter<R>& cd = arbiter_for(k1, k2);
+        if (!persistent) cd.usage = 0;
+        reset_mode(cd, JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2);
+        add_contact(cd, r.pa, r.pb, r.normal);
+    }
+
+    void detect(uint32_t k1, uint32_t k2, R step_dt) {
         const Shape<R>& sA = shapes[k1];
         const Shape<R>& sB = shapes[k2];
         const Body<R>& b1 = bodies[sA.body];
@@ -349,7 +411,10 @@ template <class R> struct World {
         if (b1.motion != JH_MOTION_DYNAMIC && b2.motion != JH_MOTION_DYNAMIC) return;
         MprResult<R> r = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, poly);
         if (r.epa) epa_calls++;
-        if (!r.hit) return;
+        if (!r.hit) {
+            detect_speculative(k1, k2, step_dt);
+            return;
+        }
         if (triangle_edge_filter && !tef_filter(sA, sB, b1, b2, r)) return;
         ManifoldOut<R> mo{};
         mo.k1 = k1;
@@ -371,14 +436,7 @@ template <class R> struct World {
         for (int i = 0; i < m.count; ++i) { mo.mA[i] = m.mA[i]; mo.mB[i] = m.mB[i]; }
         last_manifolds.push_back(mo);
         // RegisterContact (World.Detect.cs:294-324)
-        auto key = std::make_pair(k1, k2);
-        auto it = arbiters.find(key);
-        if (it == arbiters.end()) {
-            Arbiter<R> a;
-            init_arbiter(a, k1, k2);
-            it = arbiters.emplace(key, a).first;
-        }
-        Arbiter<R>& cd = it->second;
+        Arbiter<R>& cd = arbiter_for(k1, k2);
         if (!persistent) cd.usage = 0;
         reset_mode(cd);
         for (int e = 0; e < m.count; ++e) add_contact(cd, m.mA[e], m.mB[e], r.normal);
@@ -643,7 +701,7 @@ template <class R> struct World {
         last_manifolds.clear();
         epa_calls = 0;
         last_pairs = compute_pairs();
-        for (auto& p : last_pairs) detect(p.first, p.second);
+        for (auto& p : last_pairs) detect(p.first, p.second, ts.dt);
         // Solve order: given, else key order.
         std::vector<Arbiter<R>*> list;
         if (order) {
@@ -703,6 +761,9 @@ template <class R> struct World {
             b.force = {R(0), R(0), R(0)};
             b.torque = {R(0), R(0), R(0)};
         }
+        // DynamicTree.Update(multiThread, StepDt) (World.Step.cs:274-276): the next broadphase sweeps
+        // speculative bodies' boxes over this step's length.
+        broadphase_dt = ts.dt;
     }
 };
 

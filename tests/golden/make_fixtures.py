@@ -4,8 +4,8 @@
 
 step0_<scene>.npz — for scenes whose first step already has contacts: the broadphase pair set at the
     initial state (brute force over world AABBs, World.DefaultDynamicTreeFilter semantics) and the
-    contact manifolds of the first step (normal, depth, points; EPA and the triangle edge filter
-    included).  Both are fixed by the initial state alone, whatever order the solver later uses, so
+    contact manifolds of the first step (normal, depth, points; EPA, the triangle edge filter and
+    speculative contacts included).  Both are fixed by the initial state alone, whatever order the solver later uses, so
     the GPU must reproduce them bit for bit (tests/test_golden.py).
 oracle_trajectory.npz — FNV-1a hashes of all body positions (FP32 bits, as the reference's
     ReproducibilityTest hashes them, Robustness/ReproducibilityTest.cs:107-147) while the oracle steps
@@ -61,10 +61,17 @@ def sunk_heightfield():
     return bodies, shapes, verts, cfg, dt
 
 
+def bullets_mesh():
+    """Projectiles at 15-60 m/s over a triangle-mesh floor with the edge filter, half of them with
+    speculative contacts: the first step's swept broadphase boxes and speculative manifolds."""
+    return scenes.bullets(300, seed=0x4A2D0017, speculative=0.5, mesh=True)
+
+
 STEP0_SCENES = {
     "pyramid20": lambda: scenes.pyramid(rows=20),
     "overlap_heap400": overlapping_heap,
     "sunk_heightfield240": sunk_heightfield,
+    "bullets_mesh300": bullets_mesh,
 }
 
 

@@ -42,6 +42,8 @@ def lib() -> ctypes.CDLL:
             "orc_build_manifold": [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "orc_exact_side": [vp, vp, vp, vp],
             "orc_contact_op": [i32, vp, vp, i32, f, i32, vp, vp, vp],
+            "orc_simplex_ab": [i32, i32, vp, vp, vp, vp, vp],
+            "orc_sweep": [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32
@@ -138,3 +140,27 @@ class OracleWorld:
 
     def time_steps(self, dt: float, k: int) -> float:
         return float(self.L.orc_time_steps(self.h, dt, k))
+
+
+def simplex_ab(V, A=None, B=None, fp64: bool = False):
+    """SimplexSolverAB.AddVertex over the rows of V (A, B: witness points, default 0), then
+    GetClosest.  Returns (AddVertex results, last closest point, pointA, pointB, used-slot mask)."""
+    V = np.ascontiguousarray(V, np.float64).reshape(-1, 3)
+    A = np.zeros_like(V) if A is None else np.ascontiguousarray(A, np.float64).reshape(-1, 3)
+    B = np.zeros_like(V) if B is None else np.ascontiguousarray(B, np.float64).reshape(-1, 3)
+    res = np.zeros(len(V), np.int32)
+    out = np.zeros(10, np.float64)
+    lib().orc_simplex_ab(1 if fp64 else 0, len(V), ptr(V), ptr(A), ptr(B), ptr(res), ptr(out))
+    return res, out[0:3], out[3:6], out[6:9], int(out[9])
+
+
+def sweep(shape_a, shape_b, verts, qa, pa, va, qb, pb, vb, fp64: bool = False):
+    """NarrowPhase.Sweep of two posed shapes (jh_shape records) with linear velocities.  Returns
+    (hit, pointA, pointB, normal, lambda) in world space."""
+    f = lambda x: np.ascontiguousarray(x, np.float32)
+    sa, sb = np.ascontiguousarray(shape_a), np.ascontiguousarray(shape_b)
+    vv = None if verts is None else f(verts)
+    out = np.zeros(10, np.float64)
+    hit = lib().orc_sweep(1 if fp64 else 0, ptr(sa), ptr(sb), ptr(vv), ptr(f(qa)), ptr(f(pa)), ptr(f(va)),
+                          ptr(f(qb)), ptr(f(pb)), ptr(f(vb)), ptr(out))
+    return bool(hit), out[0:3], out[3:6], out[6:9], float(out[9])
