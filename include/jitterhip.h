@@ -229,7 +229,21 @@ jh_status jh_world_compute_pairs(jh_world* world, uint32_t* pairs, int32_t capac
 /* Narrowphase results of the last step, sorted by key. */
 jh_status jh_world_get_manifolds(jh_world* world, jh_manifold* manifolds, int32_t capacity, int32_t* out_count);
 
-/* Contact cache (arbiters sorted by key) after the last step, and restore (checkpoint/resume). */
+/* Collision events of the last step: RigidBody.BeginCollide / EndCollide (Dynamics/RigidBody.cs:330-350),
+ * raised for both bodies of an arbiter at World.Step.cs:881-911 (HandleDeferredArbiters) and :839-860
+ * (RemoveBrokenArbiters).  begin = arbiters the narrowphase created in this step; end = arbiters freed in
+ * this step: left without intact contacts by the previous step's UpdateContacts and not refilled by this
+ * step's narrowphase.  An arbiter that breaks and is refilled in the next step raises neither, as in the
+ * reference.  Pairs are (Key1, Key2) shape indices (2 uint32 each), each list in key order (the
+ * reference raises them in narrowphase order); the counts are the full list lengths, and
+ * JH_ERR_CAPACITY is returned when a capacity is short. */
+jh_status jh_world_get_collision_events(jh_world* world, uint32_t* begin_pairs, int32_t begin_capacity,
+                                        int32_t* out_begin_count, uint32_t* end_pairs, int32_t end_capacity,
+                                        int32_t* out_end_count);
+
+/* Contact cache (arbiters sorted by key) after the last step, and restore (checkpoint/resume).  Arbiters
+ * left without intact contacts are dropped at the end of the step that broke them (the reference frees
+ * them in the next step; only the events above keep its timing), so they are never listed. */
 jh_status jh_world_get_arbiters(jh_world* world, jh_arbiter* arbiters, int32_t capacity, int32_t* out_count);
 jh_status jh_world_set_arbiters(jh_world* world, const jh_arbiter* arbiters, int32_t count);
 

@@ -214,6 +214,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_world_get_arbiters": ([vp, vp, i32, pi32], i32),
         "jh_world_set_arbiters": ([vp, vp, i32], i32),
         "jh_world_get_solve_order": ([vp, vp, i32, vp, i32, pi32, pi32], i32),
+        "jh_world_get_collision_events": ([vp, vp, i32, pi32, vp, i32, pi32], i32),
         "jh_world_get_stats": ([vp, vp], i32),
         "jh_lws_create": ([ctypes.c_uint64, ctypes.POINTER(vp)], i32),
         "jh_lws_destroy": ([vp], i32),

@@ -2744,15 +2744,47 @@ __global__ void k_update_contacts(jh_arbiter* __restrict__ arb, const int* __res
     }
 }
 
+// Arbiters with intact contacts move to the other buffer; the keys of the others (brokenArbiters,
+// World.Step.cs:862-878) go to zkey, in key order, for the next step's EndCollide test.
 __global__ void k_compact_arbiters(const jh_arbiter* __restrict__ ain, const uint64_t* __restrict__ akey,
                                    const int* __restrict__ a_count, const uint32_t* __restrict__ keep,
                                    const uint32_t* __restrict__ offs, jh_arbiter* __restrict__ aout,
-                                   uint64_t* __restrict__ okey) {
+                                   uint64_t* __restrict__ okey, uint64_t* __restrict__ zkey,
+                                   int* __restrict__ zcount) {
     int A = *a_count;
+    if (A == 0 && blockIdx.x == 0 && threadIdx.x == 0) *zcount = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) {
-        if (!keep[i]) continue;
+        if (i == A - 1) *zcount = A - int(offs[i] + keep[i]);
+        if (!keep[i]) {
+            zkey[i - offs[i]] = akey[i];
+            continue;
+        }
         aout[offs[i]] = ain[i];
         okey[offs[i]] = akey[i];
+    }
+}
+
+// Collision events of a step, with the reference's arbiter lifetime (World.Step.cs:839-911): an
+// arbiter broken by the previous step's UpdateContacts (zkey) is freed only if this step's
+// narrowphase did not refill it (EndCollide); a pair registered now with no arbiter from the previous
+// step, broken ones included, creates one (BeginCollide).  nkey: pairs new to the cache (sorted), ckey:
+// all registered pairs (sorted).
+__global__ void k_events(const uint64_t* __restrict__ nkey, const int* __restrict__ n_new,
+                         const uint64_t* __restrict__ zkey, const int* __restrict__ zcount,
+                         const uint64_t* __restrict__ ckey, const int* __restrict__ c_count,
+                         uint64_t* __restrict__ begin, int* __restrict__ n_begin, uint64_t* __restrict__ end,
+                         int* __restrict__ n_end) {
+    int N = *n_new, Z = *zcount, C = *c_count;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    for (int i = t0; i < N; i += stride) {
+        uint64_t k = nkey[i];
+        int f = lower_bound_u64(zkey, Z, k);
+        if (!(f < Z && zkey[f] == k)) begin[atomicAdd(n_begin, 1)] = k;
+    }
+    for (int i = t0; i < Z; i += stride) {
+        uint64_t k = zkey[i];
+        int f = lower_bound_u64(ckey, C, k);
+        if (!(f < C && ckey[f] == k)) end[atomicAdd(n_end, 1)] = k;
     }
 }
 

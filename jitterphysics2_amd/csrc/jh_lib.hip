@@ -84,6 +84,9 @@ enum {
     C_TCELLS = 17,    // tile solver: (tile, colour) cells + 1
     C_SERR = 18,      // tile solver: a dependency wait gave up
     C_MAXDEG = 19,    // most solved arbiters on one non-static body
+    C_ZOMB = 20,      // arbiters removed at the end of the last step (keys in zkey); kept across steps
+    C_EVB = 21,       // BeginCollide events of this step
+    C_EVE = 22,       // EndCollide events of this step
     C_COUNT = 24
 };
 
@@ -176,6 +179,7 @@ struct jh_world {
     DBuf<char> epa_work, epa_side;
     DBuf<EpaRes> epa_res;
     DBuf<int> specq;  // pairs queued for the speculative sweep
+    DBuf<uint64_t> zkey, evb, eve;  // keys of arbiters removed by the last step; collision events
     int epa_threads = 0;
     bool epa_wave = true;      // wave-cooperative EPA (k_epa_wave); false: one thread per pair (k_epa)
     int epa_wave_blocks = 0;   // resident single-wave workgroups of k_epa_wave
@@ -243,6 +247,7 @@ struct jh_world {
     int coop_probe = 0;
     jh_stats stats;
     bool have_step = false;
+    int ev_begin = 0, ev_end = 0;  // collision events of the last step (in evb / eve)
 
     Bodies bodies() {
         Bodies B;
@@ -258,7 +263,7 @@ namespace {
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
     if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK ||
-        t == C_SERR || t == C_MAXDEG || t == C_SPEC)
+        t == C_SERR || t == C_MAXDEG || t == C_SPEC || t == C_EVB || t == C_EVE)
         cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
     else if (t < 6) bounds[t] = 0u;
@@ -376,6 +381,9 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->akey[1 - w->cur].ensure(na));
     HIPCHK(w->colour.ensure(na));
     HIPCHK(w->cwork.ensure(2 * na));
+    HIPCHK(w->zkey.ensure(na, true));  // holds the last step's removed keys until k_events has read them
+    HIPCHK(w->evb.ensure(np));
+    HIPCHK(w->eve.ensure(w->zkey.cap));
     if (w->kind != SOLVER_BARRIER && int(na) > w->soa_cap) {
         size_t cap = na + na / 4;
         HIPCHK(w->soa_bodies.ensure(cap));
@@ -629,6 +637,9 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         scan::exclusive_scan(w->cnew.p, w->noffs.p, cnt + C_COLL, np, w->sums.p, cnt + C_NEW, st);
         k_new_compact<<<g, 256, 0, st>>>(w->ckey.p, cnt + C_COLL, w->cnew.p, w->noffs.p, w->nkey.p);
     }
+    k_events<<<grid_for(std::max(np, w->a_host), 256, 256), 256, 0, st>>>(
+        w->nkey.p, cnt + C_NEW, w->zkey.p, cnt + C_ZOMB, w->ckey.p, cnt + C_COLL, w->evb.p, cnt + C_EVB, w->eve.p,
+        cnt + C_EVE);
     int amax_ = w->a_host + np;
     k_merge_existing<<<grid_for(std::max(w->a_host, 1)), 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->nkey.p,
                                                                        cnt + C_NEW, w->arb[nxt].p, w->akey[nxt].p);
@@ -818,7 +829,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         scan::exclusive_scan(w->flags.p, w->offs.p, a_cur, std::max(amax_, 1), w->sums.p,
                              cnt + (cur == 0 ? C_A1 : C_A0), st);
         k_compact_arbiters<<<g, 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->flags.p, w->offs.p,
-                                              w->arb[1 - cur].p, w->akey[1 - cur].p);
+                                              w->arb[1 - cur].p, w->akey[1 - cur].p, w->zkey.p, cnt + C_ZOMB);
         w->cur = 1 - cur;
         if (w->nb > 0)
             k_update_bodies<<<grid_for(w->nb, 256, 1 << 20), 256, 0, st>>>(B, w->nb, ts, cfg.gravity[0], cfg.gravity[1],
@@ -846,6 +857,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     S.colours = w->hcnt[C_NCOL];
     S.epa_calls = w->hcnt[C_DEEP];
     S.max_degree = w->hcnt[C_MAXDEG];
+    w->ev_begin = w->hcnt[C_EVB];
+    w->ev_end = w->hcnt[C_EVE];
     S.solver = kind;
     S.steps++;
     float ms[7];
@@ -1170,6 +1183,7 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
     // a new shape set invalidates the contact cache (RigidBody.ClearContactCache on shape changes)
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_A0, 0);
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_A1, 0);
+    k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_ZOMB, 0);
     w->a_host = 0;
     w->cur = 0;
     HIPCHK(hipStreamSynchronize(w->st));
@@ -1226,6 +1240,37 @@ jh_status jh_world_get_manifolds(jh_world* w, jh_manifold* out, int32_t capacity
     return JH_OK;
 }
 
+jh_status jh_world_get_collision_events(jh_world* w, uint32_t* begin_pairs, int32_t begin_capacity,
+                                        int32_t* out_begin_count, uint32_t* end_pairs, int32_t end_capacity,
+                                        int32_t* out_end_count) {
+    if (!w || !out_begin_count || !out_end_count || begin_capacity < 0 || end_capacity < 0 ||
+        (begin_capacity > 0 && !begin_pairs) || (end_capacity > 0 && !end_pairs))
+        return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    *out_begin_count = *out_end_count = 0;
+    if (!w->have_step) return JH_OK;
+    int nb = w->ev_begin, ne = w->ev_end;
+    std::vector<uint64_t> kb(nb), ke(ne);
+    if (nb > 0) HIPCHK(hipMemcpyAsync(kb.data(), w->evb.p, nb * sizeof(uint64_t), hipMemcpyDeviceToHost, w->st));
+    if (ne > 0) HIPCHK(hipMemcpyAsync(ke.data(), w->eve.p, ne * sizeof(uint64_t), hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    std::sort(kb.begin(), kb.end());  // appended by many threads: sort into key order
+    std::sort(ke.begin(), ke.end());
+    auto put = [](const std::vector<uint64_t>& k, uint32_t* out, int cap) {
+        for (int i = 0; i < int(k.size()) && i < cap; ++i) {
+            out[2 * i] = uint32_t(k[i] >> 32);
+            out[2 * i + 1] = uint32_t(k[i] & 0xffffffffu);
+        }
+    };
+    put(kb, begin_pairs, begin_capacity);
+    put(ke, end_pairs, end_capacity);
+    *out_begin_count = nb;
+    *out_end_count = ne;
+    if (nb > begin_capacity || ne > end_capacity)
+        return fail(JH_ERR_CAPACITY, "%d begin / %d end events exceed capacities %d / %d", nb, ne, begin_capacity,
+                    end_capacity);
+    return JH_OK;
+}
+
 jh_status jh_world_get_arbiters(jh_world* w, jh_arbiter* out, int32_t capacity, int32_t* out_count) {
     if (!w || !out_count || capacity < 0 || (capacity > 0 && !out)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
     int n = w->a_host;
@@ -1264,6 +1309,7 @@ jh_status jh_world_set_arbiters(jh_world* w, const jh_arbiter* in, int32_t n) {
         HIPCHK(hipMemcpyAsync(w->akey[w->cur].p, keys.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, w->st));
     }
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + (w->cur == 0 ? C_A0 : C_A1), n);
+    k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_ZOMB, 0);  // the restored cache has no pending removals
     HIPCHK(hipStreamSynchronize(w->st));
     w->a_host = n;
     w->recolour = true;  // colours in uploaded arbiters are not trusted

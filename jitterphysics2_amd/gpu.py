@@ -91,6 +91,18 @@ class GpuWorld:
               "jh_world_get_arbiters")
         return out[:n.value]
 
+    def collision_events(self) -> tuple[np.ndarray, np.ndarray]:
+        """(BeginCollide pairs, EndCollide pairs) of the last step, (K, 2) uint32 shape indices in key
+        order (RigidBody.BeginCollide / EndCollide with the reference's arbiter lifetime)."""
+        nb, ne = ctypes.c_int32(0), ctypes.c_int32(0)
+        self.lib.jh_world_get_collision_events(self.handle, None, 0, ctypes.byref(nb), None, 0, ctypes.byref(ne))
+        b = np.zeros((max(nb.value, 1), 2), np.uint32)
+        e = np.zeros((max(ne.value, 1), 2), np.uint32)
+        check(self.lib, self.lib.jh_world_get_collision_events(self.handle, ptr(b), nb.value, ctypes.byref(nb), ptr(e),
+                                                               ne.value, ctypes.byref(ne)),
+              "jh_world_get_collision_events")
+        return b[:nb.value], e[:ne.value]
+
     def set_arbiters(self, arbiters: np.ndarray) -> None:
         a = np.ascontiguousarray(arbiters, ARBITER_DTYPE)
         check(self.lib, self.lib.jh_world_set_arbiters(self.handle, ptr(a), len(a)), "jh_world_set_arbiters")

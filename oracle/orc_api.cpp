@@ -100,6 +100,7 @@ template <class R> void set_shapes(World<R>& w, const jh_shape* s, int n, const 
         o.nverts = s[i].vertex_count;
     }
     w.arbiters.clear();
+    w.broken.clear();
 }
 
 template <class R> void contact_out(const Contact<R>& c, jh_contact& o) {
@@ -129,9 +130,11 @@ template <class R> void contact_in(Contact<R>& c, const jh_contact& o) {
     c.rel2 = vin<R>(o.relative_position2);
 }
 
-template <class R> int get_arbiters(const World<R>& w, jh_arbiter* out, int cap) {
+// Arbiters with intact contacts; those waiting for RemoveBrokenArbiters are listed only with all.
+template <class R> int get_arbiters(const World<R>& w, jh_arbiter* out, int cap, bool all = false) {
     int n = 0;
     for (auto& kv : w.arbiters) {
+        if (!all && (kv.second.usage & 15u) == 0) continue;
         if (n < cap && out) {
             const Arbiter<R>& a = kv.second;
             jh_arbiter& o = out[n];
@@ -153,6 +156,7 @@ template <class R> int get_arbiters(const World<R>& w, jh_arbiter* out, int cap)
 }
 template <class R> void set_arbiters(World<R>& w, const jh_arbiter* in, int n) {
     w.arbiters.clear();
+    w.broken.clear();
     for (int i = 0; i < n; ++i) {
         Arbiter<R> a;
         a.key1 = in[i].shape1;
@@ -249,7 +253,7 @@ template <class R> int contact_op_t(jh_body* bodies, jh_arbiter* arb, int op, fl
     }
     get_bodies(w, bodies, 2);
     jh_arbiter out;
-    get_arbiters(w, &out, 1);
+    get_arbiters(w, &out, 1, true);
     out.body1 = arb->body1;
     out.body2 = arb->body2;
     out.shape1 = arb->shape1;
@@ -396,7 +400,19 @@ long orc_epa_calls(void* p) {
 }
 int orc_arbiter_count(void* p) {
     Handle* h = (Handle*)p;
-    return DISPATCH(h, (int)W.arbiters.size());
+    return DISPATCH(h, get_arbiters(W, nullptr, 0));
+}
+// BeginCollide / EndCollide of the last step as (Key1, Key2) pairs, each list sorted by key.
+// counts[0] / counts[1] receive the full list lengths.
+int orc_get_events(void* p, uint32_t* begin, int begin_cap, uint32_t* end, int end_cap, int* counts) {
+    Handle* h = (Handle*)p;
+    auto put = [](const std::vector<std::pair<uint32_t, uint32_t>>& v, uint32_t* out, int cap) {
+        for (int i = 0; i < (int)v.size() && i < cap && out; ++i) { out[2 * i] = v[i].first; out[2 * i + 1] = v[i].second; }
+        return (int)v.size();
+    };
+    counts[0] = DISPATCH(h, put(W.ev_begin, begin, begin_cap));
+    counts[1] = DISPATCH(h, put(W.ev_end, end, end_cap));
+    return 0;
 }
 
 // ---- pure functions for known-answer tests ------------------------------------------------

@@ -99,6 +99,10 @@ template <class R> struct World {
     bool aux_contacts = true, persistent = true, triangle_edge_filter = false;
     std::vector<ManifoldOut<R>> last_manifolds;
     std::vector<std::pair<uint32_t, uint32_t>> last_pairs;
+    // Arbiter lifecycle: arbiters left without intact contacts by UpdateContacts (brokenArbiters) stay
+    // until the next step's RemoveBrokenArbiters, which frees those still empty (EndCollide); arbiters
+    // created by Detect raise BeginCollide (World.Step.cs:839-914).  Keys of the last step's events.
+    std::vector<std::pair<uint32_t, uint32_t>> broken, ev_begin, ev_end;
     long epa_calls = 0;
     Polytope<R>* poly = nullptr;
 
@@ -361,6 +365,7 @@ template <class R> struct World {
             Arbiter<R> a;
             init_arbiter(a, k1, k2);
             it = arbiters.emplace(key, a).first;
+            ev_begin.push_back(key);  // HandleDeferredArbiters raises BeginCollide (World.Step.cs:881-911)
         }
         return it->second;
     }
@@ -697,6 +702,8 @@ template <class R> struct World {
         if (!(dt >= R(0)) || !std::isfinite(dt)) return;
         if (dt < std::numeric_limits<R>::denorm_min()) return;
         StepTiming<R> ts(dt, substeps);
+        ev_begin.clear();
+        ev_end.clear();
         // Narrowphase over the broadphase pair set.
         last_manifolds.clear();
         epa_calls = 0;
@@ -741,14 +748,22 @@ template <class R> struct World {
                     for (int i = 0; i < 4; ++i)
                         if (a->usage & (1u << i)) iterate_contact(*a, a->c[i], false);
         }
-        // UpdateContacts (World.Step.cs:862-878).  Arbiters left without intact contacts carry no
-        // solver work; the reference frees them one step later (RemoveBrokenArbiters runs before
-        // UpdateContacts, World.Step.cs:259-265), which leaves body state unchanged.
-        for (auto it = arbiters.begin(); it != arbiters.end();) {
-            arbiter_update_position(it->second);
-            if ((it->second.usage & 15u) == 0) it = arbiters.erase(it);
-            else ++it;
+        // RemoveBrokenArbiters (World.Step.cs:839-860): the arbiters broken by the previous step's
+        // UpdateContacts that this step's Detect did not refill are freed (EndCollide).
+        for (auto& k : broken) {
+            auto it = arbiters.find(k);
+            if (it == arbiters.end() || (it->second.usage & 15u) != 0) continue;
+            arbiters.erase(it);
+            ev_end.push_back(k);
         }
+        broken.clear();
+        // UpdateContacts (World.Step.cs:862-878): arbiters left without intact contacts are queued for
+        // removal; until then they carry no solver work.
+        for (auto& kv : arbiters) {
+            arbiter_update_position(kv.second);
+            if ((kv.second.usage & 15u) == 0) broken.push_back(kv.first);
+        }
+        std::sort(ev_begin.begin(), ev_begin.end());
         // ForeachActiveBody -> RigidBody.Update (RigidBody.cs:716-764)
         for (auto& b : bodies) {
             if (b.motion != JH_MOTION_DYNAMIC) continue;

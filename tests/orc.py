@@ -44,6 +44,7 @@ def lib() -> ctypes.CDLL:
             "orc_contact_op": [i32, vp, vp, i32, f, i32, vp, vp, vp],
             "orc_simplex_ab": [i32, i32, vp, vp, vp, vp, vp],
             "orc_sweep": [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+            "orc_get_events": [vp, vp, i32, vp, i32, vp],
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32
@@ -134,6 +135,15 @@ class OracleWorld:
         out = np.zeros(n, MANIFOLD_DTYPE)
         self.L.orc_get_manifolds(self.h, ptr(out), n)
         return out
+
+    def collision_events(self) -> tuple[np.ndarray, np.ndarray]:
+        """(BeginCollide pairs, EndCollide pairs) of the last step, in key order."""
+        counts = np.zeros(2, np.int32)
+        self.L.orc_get_events(self.h, None, 0, None, 0, ptr(counts))
+        b = np.zeros((max(int(counts[0]), 1), 2), np.uint32)
+        e = np.zeros((max(int(counts[1]), 1), 2), np.uint32)
+        self.L.orc_get_events(self.h, ptr(b), int(counts[0]), ptr(e), int(counts[1]), ptr(counts))
+        return b[:int(counts[0])], e[:int(counts[1])]
 
     def epa_calls(self) -> int:
         return int(self.L.orc_epa_calls(self.h))
