@@ -158,6 +158,30 @@ typedef struct jh_arbiter {
     jh_contact contacts[4];
 } jh_arbiter;
 
+/* Constraints solved with the contacts (World.Step.cs:1155-1191: each substep prepares contacts, then
+ * constraints; each iteration solves contacts, then constraints; relaxation solves contacts only).
+ * params / state per kind, the Data fields of the reference's constraint (set by its Initialize and
+ * properties; state carries over between steps):
+ *   BALL_SOCKET (Dynamics/Constraints/BallSocket.cs): params LocalAnchor1[3], LocalAnchor2[3] (body
+ *       frames), BiasFactor, Softness; state AccumulatedImpulse[3].
+ *   HINGE_ANGLE (HingeAngle.cs): params MinAngle, MaxAngle, FixedAngle (half angles as Initialize stores
+ *       them), BiasFactor, LimitBias, LimitSoftness, Softness, Axis[3] (body-2 frame), Q0[4] (x y z w);
+ *       state AccumulatedImpulse[3], Hemisphere (0, +1, -1).
+ *   FIXED_ANGLE (FixedAngle.cs): params BiasFactor, Softness, Q0[4]; state AccumulatedImpulse[3],
+ *       Hemisphere.
+ *   DISTANCE_LIMIT (DistanceLimit.cs): params LocalAnchor1[3], LocalAnchor2[3], BiasFactor, Softness,
+ *       Distance, LimitMin, LimitMax; state AccumulatedImpulse. */
+enum { JH_CONSTRAINT_BALL_SOCKET = 0, JH_CONSTRAINT_HINGE_ANGLE = 1, JH_CONSTRAINT_FIXED_ANGLE = 2,
+       JH_CONSTRAINT_DISTANCE_LIMIT = 3 };
+
+typedef struct jh_constraint {
+    int32_t type;           /* JH_CONSTRAINT_*                                                  */
+    int32_t body1, body2;   /* body indices (Constraint.Body1 / Body2)                           */
+    int32_t enabled;        /* Constraint.IsEnabled                                              */
+    float params[16];
+    float state[8];
+} jh_constraint;
+
 /* Narrowphase result of one colliding broadphase pair in the last step (World.Detect.cs:165-217). */
 typedef struct jh_manifold {
     uint32_t shape1, shape2;
@@ -195,7 +219,7 @@ const char* jh_last_error(void);
 /* ABI version: 1. */
 int32_t jh_version(void);
 /* sizeof of the ABI structs, for binding checks: 0 jh_config, 1 jh_body, 2 jh_shape, 3 jh_contact,
- * 4 jh_arbiter, 5 jh_manifold, 6 jh_stats, 7 jh_probe, 8 jh_tuning; -1 otherwise. */
+ * 4 jh_arbiter, 5 jh_manifold, 6 jh_stats, 7 jh_probe, 8 jh_tuning, 9 jh_constraint; -1 otherwise. */
 int32_t jh_struct_size(int32_t which);
 
 /* World lifetime (World ctor World.cs:423-441, Dispose World.cs:977-990).  config may be NULL for
@@ -240,6 +264,18 @@ jh_status jh_world_get_manifolds(jh_world* world, jh_manifold* manifolds, int32_
 jh_status jh_world_get_collision_events(jh_world* world, uint32_t* begin_pairs, int32_t begin_capacity,
                                         int32_t* out_begin_count, uint32_t* end_pairs, int32_t end_capacity,
                                         int32_t* out_end_count);
+
+/* Constraints (World.CreateConstraint, World.cs:850-888, and Constraint.Initialize): set replaces all
+ * constraints (state included); get returns them with their current state.  With constraints the
+ * step solves with colour batches and grid barriers (jh_stats.solver 0). */
+jh_status jh_world_set_constraints(jh_world* world, const jh_constraint* constraints, int32_t count);
+jh_status jh_world_get_constraints(jh_world* world, jh_constraint* constraints, int32_t capacity,
+                                   int32_t* out_count);
+/* Constraint solve order: constraint indices in the order the solver visits them, in batches of
+ * constraints with disjoint dynamic bodies (batch_offsets: batch_count + 1 entries); replaying the
+ * indices sequentially after the contacts of each pass reproduces the GPU's order. */
+jh_status jh_world_get_constraint_order(jh_world* world, int32_t* order, int32_t capacity, int32_t* batch_offsets,
+                                        int32_t batch_capacity, int32_t* out_count, int32_t* out_batch_count);
 
 /* Contact cache (arbiters sorted by key) after the last step, and restore (checkpoint/resume).  Arbiters
  * left without intact contacts are dropped at the end of the step that broke them (the reference frees

@@ -146,10 +146,17 @@ TUNING_DTYPE = np.dtype([
     ("ms_solve_best", "<f8"), ("ms_solve_default", "<f8"), ("ms_colour_best", "<f8"), ("ms_colour_default", "<f8"),
 ])
 
+CONSTRAINT_BALL_SOCKET, CONSTRAINT_HINGE_ANGLE, CONSTRAINT_FIXED_ANGLE, CONSTRAINT_DISTANCE_LIMIT = 0, 1, 2, 3
+
+CONSTRAINT_DTYPE = np.dtype([
+    ("type", "<i4"), ("body1", "<i4"), ("body2", "<i4"), ("enabled", "<i4"),
+    ("params", "<f4", 16), ("state", "<f4", 8),
+])
+
 STRUCT_SIZES = {
     0: CONFIG_DTYPE.itemsize, 1: BODY_DTYPE.itemsize, 2: SHAPE_DTYPE.itemsize,
     3: CONTACT_DTYPE.itemsize, 4: ARBITER_DTYPE.itemsize, 5: MANIFOLD_DTYPE.itemsize,
-    6: STATS_DTYPE.itemsize, 7: PROBE_DTYPE.itemsize, 8: TUNING_DTYPE.itemsize,
+    6: STATS_DTYPE.itemsize, 7: PROBE_DTYPE.itemsize, 8: TUNING_DTYPE.itemsize, 9: CONSTRAINT_DTYPE.itemsize,
 }
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
@@ -215,6 +222,9 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_world_set_arbiters": ([vp, vp, i32], i32),
         "jh_world_get_solve_order": ([vp, vp, i32, vp, i32, pi32, pi32], i32),
         "jh_world_get_collision_events": ([vp, vp, i32, pi32, vp, i32, pi32], i32),
+        "jh_world_set_constraints": ([vp, vp, i32], i32),
+        "jh_world_get_constraints": ([vp, vp, i32, pi32], i32),
+        "jh_world_get_constraint_order": ([vp, vp, i32, vp, i32, pi32, pi32], i32),
         "jh_world_get_stats": ([vp, vp], i32),
         "jh_lws_create": ([ctypes.c_uint64, ctypes.POINTER(vp)], i32),
         "jh_lws_destroy": ([vp], i32),

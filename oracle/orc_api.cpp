@@ -196,6 +196,33 @@ template <class R> int get_manifolds(const World<R>& w, jh_manifold* out, int ca
     return n;
 }
 
+template <class R> void set_constraints(World<R>& w, const jh_constraint* in, int n) {
+    w.constraints.assign(n, Constraint<R>());
+    for (int i = 0; i < n; ++i) {
+        Constraint<R>& c = w.constraints[i];
+        c.type = in[i].type;
+        c.b1 = in[i].body1;
+        c.b2 = in[i].body2;
+        c.enabled = in[i].enabled;
+        for (int k = 0; k < 16; ++k) c.p[k] = R(in[i].params[k]);
+        for (int k = 0; k < 8; ++k) c.s[k] = R(in[i].state[k]);
+    }
+    w.con_order.clear();
+}
+template <class R> int get_constraints(const World<R>& w, jh_constraint* out, int cap) {
+    int n = (int)w.constraints.size();
+    for (int i = 0; i < n && i < cap && out; ++i) {
+        const Constraint<R>& c = w.constraints[i];
+        out[i].type = c.type;
+        out[i].body1 = c.b1;
+        out[i].body2 = c.b2;
+        out[i].enabled = c.enabled;
+        for (int k = 0; k < 16; ++k) out[i].params[k] = float(c.p[k]);
+        for (int k = 0; k < 8; ++k) out[i].state[k] = float(c.s[k]);
+    }
+    return n;
+}
+
 Shape<float> shape_f(const jh_shape& s, const float* verts) {
     Shape<float> o;
     o.type = s.type;
@@ -401,6 +428,26 @@ long orc_epa_calls(void* p) {
 int orc_arbiter_count(void* p) {
     Handle* h = (Handle*)p;
     return DISPATCH(h, get_arbiters(W, nullptr, 0));
+}
+int orc_set_constraints(void* p, const jh_constraint* c, int n) {
+    Handle* h = (Handle*)p;
+    DISPATCH(h, (set_constraints(W, c, n), 0));
+    return 0;
+}
+int orc_get_constraints(void* p, jh_constraint* c, int cap) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, get_constraints(W, c, cap));
+}
+// Order in which the next steps visit the constraints (indices); n = 0 restores index order.
+int orc_set_constraint_order(void* p, const int* order, int n) {
+    Handle* h = (Handle*)p;
+    auto set = [&](auto& W) {
+        W.con_order.assign(order, order + n);
+        for (int i : W.con_order)
+            if (i < 0 || i >= (int)W.constraints.size()) return -1;
+        return 0;
+    };
+    return h->fp64 ? set(*h->wd) : set(*h->wf);
 }
 // BeginCollide / EndCollide of the last step as (Key1, Key2) pairs, each list sorted by key.
 // counts[0] / counts[1] receive the full list lengths.

@@ -1,0 +1,552 @@
+// ORACLE — test infrastructure only (see orc_math.hpp).  Constraints solved with the contacts —
+// BallSocket, HingeAngle, FixedAngle, DistanceLimit (Dynamics/Constraints/*.cs) — and the JMatrix,
+// JSymmetricMatrix, MathHelper and AngularConstraintMath pieces they use, restated in the reference's
+// operation order and templated on the precision R.
+#pragma once
+#include <cmath>
+
+#include "orc_math.hpp"
+
+namespace orc {
+
+// ---- JMatrix (LinearMath/JMatrix.cs) ---------------------------------------------------------
+template <class R> inline M3<R> mzero() { return M3<R>{R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)}; }
+template <class R> inline M3<R> midentity() {
+    return M3<R>{R(1), R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(1)};
+}
+// operator * (:620-633)
+template <class R> inline M3<R> mmul(const M3<R>& a, const M3<R>& b) {
+    M3<R> r;
+    r.m11 = a.m11 * b.m11 + a.m12 * b.m21 + a.m13 * b.m31;
+    r.m12 = a.m11 * b.m12 + a.m12 * b.m22 + a.m13 * b.m32;
+    r.m13 = a.m11 * b.m13 + a.m12 * b.m23 + a.m13 * b.m33;
+    r.m21 = a.m21 * b.m11 + a.m22 * b.m21 + a.m23 * b.m31;
+    r.m22 = a.m21 * b.m12 + a.m22 * b.m22 + a.m23 * b.m32;
+    r.m23 = a.m21 * b.m13 + a.m22 * b.m23 + a.m23 * b.m33;
+    r.m31 = a.m31 * b.m11 + a.m32 * b.m21 + a.m33 * b.m31;
+    r.m32 = a.m31 * b.m12 + a.m32 * b.m22 + a.m33 * b.m32;
+    r.m33 = a.m31 * b.m13 + a.m32 * b.m23 + a.m33 * b.m33;
+    return r;
+}
+// Add / Subtract (:353-385), Multiply(matrix, scale) (:503-514)
+template <class R> inline M3<R> madd(const M3<R>& a, const M3<R>& b) {
+    return {a.m11 + b.m11, a.m12 + b.m12, a.m13 + b.m13, a.m21 + b.m21, a.m22 + b.m22,
+            a.m23 + b.m23, a.m31 + b.m31, a.m32 + b.m32, a.m33 + b.m33};
+}
+template <class R> inline M3<R> msub(const M3<R>& a, const M3<R>& b) {
+    return {a.m11 - b.m11, a.m12 - b.m12, a.m13 - b.m13, a.m21 - b.m21, a.m22 - b.m22,
+            a.m23 - b.m23, a.m31 - b.m31, a.m32 - b.m32, a.m33 - b.m33};
+}
+template <class R> inline M3<R> mscale(const M3<R>& a, R s) {
+    return {a.m11 * s, a.m12 * s, a.m13 * s, a.m21 * s, a.m22 * s, a.m23 * s, a.m31 * s, a.m32 * s, a.m33 * s};
+}
+// Transpose (:603-614)
+template <class R> inline M3<R> mtranspose(const M3<R>& m) {
+    return {m.m11, m.m21, m.m31, m.m12, m.m22, m.m32, m.m13, m.m23, m.m33};
+}
+// Determinant (:391-395)
+template <class R> inline R mdet(const M3<R>& m) {
+    return m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 - m.m31 * m.m22 * m.m13 -
+           m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
+}
+// CreateCrossProduct (:597-600)
+template <class R> inline M3<R> cross_matrix(V3<R> v) {
+    return {R(0), -v.z, v.y, v.z, R(0), -v.x, -v.y, v.x, R(0)};
+}
+template <class R> inline bool finite_r(R v) { return std::isfinite(v); }
+template <class R> inline bool mfinite(const M3<R>& m) {
+    return finite_r(m.m11) && finite_r(m.m12) && finite_r(m.m13) && finite_r(m.m21) && finite_r(m.m22) &&
+           finite_r(m.m23) && finite_r(m.m31) && finite_r(m.m32) && finite_r(m.m33);
+}
+// TryInverseUnscaled (:455-481)
+template <class R> inline bool try_inverse_unscaled(const M3<R>& m, M3<R>& r) {
+    R idet = R(1.0) / mdet(m);
+    if (!std::isnormal(idet)) {
+        r = mzero<R>();
+        return false;
+    }
+    R n11 = m.m22 * m.m33 - m.m23 * m.m32;
+    R n12 = m.m13 * m.m32 - m.m12 * m.m33;
+    R n13 = m.m12 * m.m23 - m.m22 * m.m13;
+    R n21 = m.m23 * m.m31 - m.m33 * m.m21;
+    R n22 = m.m11 * m.m33 - m.m31 * m.m13;
+    R n23 = m.m13 * m.m21 - m.m23 * m.m11;
+    R n31 = m.m21 * m.m32 - m.m31 * m.m22;
+    R n32 = m.m12 * m.m31 - m.m32 * m.m11;
+    R n33 = m.m11 * m.m22 - m.m21 * m.m12;
+    r = {n11 * idet, n12 * idet, n13 * idet, n21 * idet, n22 * idet, n23 * idet, n31 * idet, n32 * idet, n33 * idet};
+    return mfinite(r);
+}
+// MathR.Max with NaN propagation (Math.Max)
+template <class R> inline R max_nan(R a, R b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    return a > b ? a : b;
+}
+// Inverse (:404-444): the direct inverse, else the inverse of the matrix scaled by its largest entry.
+template <class R> inline bool minverse(const M3<R>& m, M3<R>& out) {
+    M3<R> inv;
+    if (try_inverse_unscaled(m, inv)) {
+        out = inv;
+        return true;
+    }
+    R scale = max_nan(std::fabs(m.m11), max_nan(std::fabs(m.m12), std::fabs(m.m13)));
+    scale = max_nan(scale, max_nan(std::fabs(m.m21), max_nan(std::fabs(m.m22), std::fabs(m.m23))));
+    scale = max_nan(scale, max_nan(std::fabs(m.m31), max_nan(std::fabs(m.m32), std::fabs(m.m33))));
+    if (scale > R(0) && finite_r(scale)) {
+        M3<R> n{m.m11 / scale, m.m12 / scale, m.m13 / scale, m.m21 / scale, m.m22 / scale,
+                m.m23 / scale, m.m31 / scale, m.m32 / scale, m.m33 / scale};
+        if (try_inverse_unscaled(n, inv)) {
+            inv = {inv.m11 / scale, inv.m12 / scale, inv.m13 / scale, inv.m21 / scale, inv.m22 / scale,
+                   inv.m23 / scale, inv.m31 / scale, inv.m32 / scale, inv.m33 / scale};
+            if (mfinite(inv)) {
+                out = inv;
+                return true;
+            }
+        }
+    }
+    out = mzero<R>();
+    return false;
+}
+// JVector.TransposedTransform(v, M) (JVector.cs:308-317): M^T v
+template <class R> inline V3<R> tmat_mul(V3<R> v, const M3<R>& m) {
+    return {v.x * m.m11 + v.y * m.m21 + v.z * m.m31, v.x * m.m12 + v.y * m.m22 + v.z * m.m32,
+            v.x * m.m13 + v.y * m.m23 + v.z * m.m33};
+}
+
+// ---- JSymmetricMatrix (LinearMath/JSymmetricMatrix.cs) ---------------------------------------
+template <class R> inline Sym<R> sym_add(const Sym<R>& a, const Sym<R>& b) {  // Add (:190-196)
+    return {a.m11 + b.m11, a.m12 + b.m12, a.m13 + b.m13, a.m22 + b.m22, a.m23 + b.m23, a.m33 + b.m33};
+}
+template <class R> inline Sym<R> sym_scale_vec(V3<R> s) { return {s.x, R(0), R(0), s.y, R(0), s.z}; }  // CreateScale
+template <class R> inline Sym<R> sym_outer(V3<R> v) {  // Outer (:89-93)
+    return {v.x * v.x, v.x * v.y, v.x * v.z, v.y * v.y, v.y * v.z, v.z * v.z};
+}
+template <class R> inline M3<R> sym_to_matrix(const Sym<R>& s) {  // ToMatrix (:68-71)
+    return {s.m11, s.m12, s.m13, s.m12, s.m22, s.m23, s.m13, s.m23, s.m33};
+}
+template <class R> inline R sym_det(const Sym<R>& s) {  // Determinant (:104-109)
+    return s.m11 * (s.m22 * s.m33 - s.m23 * s.m23) + s.m12 * (s.m23 * s.m13 - s.m12 * s.m33) +
+           s.m13 * (s.m12 * s.m23 - s.m22 * s.m13);
+}
+
+// ---- MathHelper (LinearMath/MathHelper.cs) ---------------------------------------------------
+template <class R> constexpr R solver_tolerance() { return sizeof(R) == 8 ? R(1e-12) : R(1e-6); }
+
+// JacobiRotate (:374-407)
+template <class R> inline bool jacobi_rotate(R* a, R* basis, int p, int q, R tolerance) {
+    R off = a[3 * p + q];
+    if (std::fabs(off) <= tolerance) return false;
+    R tau = (a[3 * q + q] - a[3 * p + p]) / (R(2.0) * off);
+    R tangent = std::copysign(R(1.0), tau) / (std::fabs(tau) + std::sqrt(R(1.0) + tau * tau));
+    R cosine = R(1.0) / std::sqrt(R(1.0) + tangent * tangent);
+    R sine = tangent * cosine;
+    a[3 * p + p] -= tangent * off;
+    a[3 * q + q] += tangent * off;
+    a[3 * p + q] = a[3 * q + p] = R(0);
+    for (int k = 0; k < 3; ++k) {
+        if (k != p && k != q) {
+            R akp = a[3 * k + p], akq = a[3 * k + q];
+            a[3 * k + p] = a[3 * p + k] = cosine * akp - sine * akq;
+            a[3 * k + q] = a[3 * q + k] = sine * akp + cosine * akq;
+        }
+        R vkp = basis[3 * k + p], vkq = basis[3 * k + q];
+        basis[3 * k + p] = cosine * vkp - sine * vkq;
+        basis[3 * k + q] = sine * vkp + cosine * vkq;
+    }
+    return true;
+}
+
+// PseudoInverseSymmetric (:282-325)
+template <class R> inline M3<R> pseudo_inverse_symmetric(const M3<R>& m) {
+    R scale = max_nan(std::fabs(m.m11), max_nan(std::fabs(m.m22), std::fabs(m.m33)));
+    if (!(scale > R(0)) || !finite_r(scale)) return mzero<R>();
+    Sym<R> n{m.m11 / scale, m.m12 / scale, m.m13 / scale, m.m22 / scale, m.m23 / scale, m.m33 / scale};
+    const R tol = solver_tolerance<R>();
+    M3<R> inv;
+    if (std::fabs(sym_det(n)) > tol && minverse(m, inv)) return inv;
+    R a[9] = {n.m11, n.m12, n.m13, n.m12, n.m22, n.m23, n.m13, n.m23, n.m33};
+    R basis[9] = {R(1), R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(1)};
+    for (int sweep = 0; sweep < 8; ++sweep) {
+        bool changed = jacobi_rotate(a, basis, 0, 1, tol);
+        changed |= jacobi_rotate(a, basis, 0, 2, tol);
+        changed |= jacobi_rotate(a, basis, 1, 2, tol);
+        if (!changed) break;
+    }
+    Sym<R> result{R(0), R(0), R(0), R(0), R(0), R(0)};
+    for (int i = 0; i < 3; ++i) {
+        R eig = a[3 * i + i];
+        if (!(eig > tol)) continue;
+        R weight = (R(1.0) / eig) / scale;
+        V3<R> ax{basis[i], basis[3 + i], basis[6 + i]};
+        Sym<R> o{ax.x * ax.x, ax.x * ax.y, ax.x * ax.z, ax.y * ax.y, ax.y * ax.z, ax.z * ax.z};
+        Sym<R> w{o.m11 * weight, o.m12 * weight, o.m13 * weight, o.m22 * weight, o.m23 * weight, o.m33 * weight};
+        result = sym_add(result, w);
+    }
+    return sym_to_matrix(result);
+}
+
+// InverseSymmetric (:328-364)
+template <class R> inline M3<R> inverse_symmetric(const M3<R>& m) {
+    const R tol = solver_tolerance<R>();
+    R diag = m.m11 * m.m22 * m.m33;
+    M3<R> inv;
+    if (mdet(m) / diag > tol && minverse(m, inv)) return inv;
+    R x = m.m11 > R(0) ? std::sqrt(m.m11) : R(0);
+    R y = m.m22 > R(0) ? std::sqrt(m.m22) : R(0);
+    R z = m.m33 > R(0) ? std::sqrt(m.m33) : R(0);
+    if (x == R(0) && y == R(0) && z == R(0)) return mzero<R>();
+    R xy = x > R(0) && y > R(0) ? m.m12 / x / y : R(0);
+    R xz = x > R(0) && z > R(0) ? m.m13 / x / z : R(0);
+    R yz = y > R(0) && z > R(0) ? m.m23 / y / z : R(0);
+    M3<R> normalized{R(1), xy, xz, xy, R(1), yz, xz, yz, R(1)};
+    M3<R> resp = pseudo_inverse_symmetric(normalized);
+    xy = x > R(0) && y > R(0) ? resp.m12 / x / y : R(0);
+    xz = x > R(0) && z > R(0) ? resp.m13 / x / z : R(0);
+    yz = y > R(0) && z > R(0) ? resp.m23 / y / z : R(0);
+    M3<R> r{x > R(0) ? resp.m11 / x / x : R(0), xy, xz, xy, y > R(0) ? resp.m22 / y / y : R(0), yz, xz, yz,
+            z > R(0) ? resp.m33 / z / z : R(0)};
+    bool ok = finite_r(r.m11) && finite_r(r.m12) && finite_r(r.m13) && finite_r(r.m22) && finite_r(r.m23) &&
+              finite_r(r.m33);
+    return ok ? r : mzero<R>();
+}
+
+// InverseBilateralBlock (:367-372): the inverse of the upper-left 2x2 block as (M11, M12, M22).
+template <class R> inline V3<R> inverse_bilateral_block(const M3<R>& m) {
+    M3<R> block{m.m11, m.m12, R(0), m.m12, m.m22, R(0), R(0), R(0), R(0)};
+    M3<R> inv = inverse_symmetric(block);
+    return {inv.m11, inv.m12, inv.m22};
+}
+
+// ---- AngularConstraintMath (Dynamics/Constraints/Internal/AngularConstraintMath.cs) ------------
+template <class R> inline Q4<R> qconj(Q4<R> q) { return {-q.x, -q.y, -q.z, q.w}; }
+template <class R> inline Q4<R> qscale(Q4<R> q, R s) { return {q.x * s, q.y * s, q.z * s, q.w * s}; }
+
+// ShortestAngle (:13-21)
+template <class R> inline R shortest_angle(R angle, R half) {
+    if (angle >= -half && angle <= half) return angle;
+    R period = R(2) * half;
+    angle = std::fmod(angle, period);
+    if (angle > half) angle -= period;
+    else if (angle < -half) angle += period;
+    return angle;
+}
+
+// CalculateJacobian (:63-94): d(rotation coordinate)/d(angular velocity difference) and the error.
+template <class R> inline M3<R> calc_jacobian(Q4<R> left, Q4<R> right, V3<R>& error, int& hemisphere) {
+    Q4<R> q = qmul(left, right);
+    hemisphere = q.w < R(0) ? -1 : 1;
+    if (q.w < R(0)) q = qscale(q, R(-1));
+    V3<R> vec{q.x, q.y, q.z};
+    R sine = length(vec);
+    M3<R> derivative;
+    if (sine > R(0)) {
+        R half_angle = StableMath<R>::atan2(sine, q.w);
+        V3<R> axis = vec * (R(1) / sine);
+        error = axis * half_angle;
+        R diagonal = half_angle * q.w / sine;
+        M3<R> t1 = mscale(midentity<R>(), diagonal);
+        M3<R> t2 = mscale(sym_to_matrix(sym_outer(axis)), R(1) - diagonal);
+        derivative = msub(madd(t1, t2), cross_matrix(error));
+    } else {
+        error = V3<R>{R(0), R(0), R(0)};
+        derivative = midentity<R>();
+    }
+    return mmul(mscale(derivative, -R(0.5)), mat_from_quat(left));
+}
+
+// ---- Constraints ----------------------------------------------------------------------------
+// One constraint: kind, bodies, the Data fields of the reference's constraint struct that persist
+// (Initialize / properties: params; accumulated impulse and hemisphere: state) and the fields its
+// PrepareForIteration computes for Iterate (work).  Layouts per kind as in jh_constraint.
+template <class R> struct Constraint {
+    int type = 0, b1 = 0, b2 = 0, enabled = 1;
+    R p[16] = {};
+    R s[8] = {};
+    // work
+    V3<R> r1{}, r2{}, bias{}, bilateral{};
+    M3<R> em{}, jac{};
+    R em_s = 0, bias_s = 0, cx = 0, cy = 0;
+    V3<R> j0{}, j1{}, j2{}, j3{};
+    int clamp = 0;
+};
+
+template <class R> struct ConstraintBody {
+    V3<R>* vel;
+    V3<R>* angvel;
+    V3<R> pos, imv;
+    Q4<R> q;
+    Sym<R> iw;
+    bool dynamic;
+};
+
+template <class R> inline V3<R> vecp(const R* p) { return {p[0], p[1], p[2]}; }
+template <class R> inline void setv(R* p, V3<R> v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+// BallSocket (BallSocket.cs): params LocalAnchor1 (0-2), LocalAnchor2 (3-5), BiasFactor (6), Softness (7);
+// state AccumulatedImpulse (0-2).
+// PrepareForIterationBallSocket (:123-161)
+template <class R> void prepare_ball_socket(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                            R inv_sub) {
+    c.r1 = rotate(vecp(c.p), A.q);
+    c.r2 = rotate(vecp(c.p + 3), B.q);
+    V3<R> p1 = A.pos + c.r1;
+    V3<R> p2 = B.pos + c.r2;
+    M3<R> cr1 = cross_matrix(c.r1), cr2 = cross_matrix(c.r2);
+    Sym<R> s = sym_add(sym_add(sym_add(sym_scale_vec(A.imv), sym_transform(A.iw, cr1)), sym_scale_vec(B.imv)),
+                       sym_transform(B.iw, cr2));
+    c.em = sym_to_matrix(s);
+    R softness = c.p[7] * inv_sub;
+    c.em.m11 += softness;
+    c.em.m22 += softness;
+    c.em.m33 += softness;
+    c.em = inverse_symmetric(c.em);
+    c.bias = (p2 - p1) * c.p[6] * inv_dt;
+    V3<R> acc = vecp(c.s);
+    if (A.dynamic) {
+        *A.vel = *A.vel - cmul(acc, A.imv);
+        *A.angvel = *A.angvel - sym_mul(mat_mul(acc, cr1), A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(acc, B.imv);
+        *B.angvel = *B.angvel + sym_mul(mat_mul(acc, cr2), B.iw);
+    }
+}
+// IterateBallSocket (:194-224)
+template <class R> void iterate_ball_socket(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
+    M3<R> cr1 = cross_matrix(c.r1), cr2 = cross_matrix(c.r2);
+    V3<R> acc = vecp(c.s);
+    V3<R> soft = acc * c.p[7] * inv_sub;
+    V3<R> jv = -*A.vel + mat_mul(*A.angvel, cr1) + *B.vel - mat_mul(*B.angvel, cr2);
+    V3<R> lambda = -R(1.0) * mat_mul(jv + c.bias + soft, c.em);
+    setv(c.s, acc + lambda);
+    if (A.dynamic) {
+        *A.vel = *A.vel - cmul(lambda, A.imv);
+        *A.angvel = *A.angvel - sym_mul(mat_mul(lambda, cr1), A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(lambda, B.imv);
+        *B.angvel = *B.angvel + sym_mul(mat_mul(lambda, cr2), B.iw);
+    }
+}
+
+// HingeAngle (HingeAngle.cs): params MinAngle (0), MaxAngle (1), FixedAngle (2), BiasFactor (3),
+// LimitBias (4), LimitSoftness (5), Softness (6), Axis (7-9), Q0 (10-13, x y z w); state
+// AccumulatedImpulse (0-2), Hemisphere (3).
+// PrepareForIterationHingeAngle (:140-226)
+template <class R> void prepare_hinge_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                            R inv_sub) {
+    const R kPi = R(3.14159265358979323846);
+    Q4<R> q1 = A.q, q2 = B.q;
+    V3<R> axis = vecp(c.p + 7);
+    Q4<R> Q0{c.p[10], c.p[11], c.p[12], c.p[13]};
+    V3<R> p0 = create_orthonormal(axis);
+    V3<R> p1 = cross(axis, p0);
+    V3<R> rotation;
+    int hemisphere;
+    M3<R> m0 = calc_jacobian(qmul(Q0, qconj(q1)), q2, rotation, hemisphere);
+    int old_h = int(c.s[3]);
+    if (old_h != 0 && old_h != hemisphere) c.s[0] = c.s[1] = c.s[2] = R(0);
+    c.s[3] = R(hemisphere);
+    V3<R> error{dot(p0, rotation), dot(p1, rotation), dot(axis, rotation)};
+    c.clamp = 0;
+    // Jacobian.UnsafeGet(i) is column i (JMatrix stores its columns contiguously, JMatrix.cs:19-27, 75-79)
+    V3<R> j0 = tmat_mul(p0, m0), j1 = tmat_mul(p1, m0), j2 = tmat_mul(axis, m0);
+    c.jac = {j0.x, j1.x, j2.x, j0.y, j1.y, j2.y, j0.z, j1.z, j2.z};
+    c.em = sym_to_matrix(sym_transform(sym_add(A.iw, B.iw), mtranspose(c.jac)));
+    c.em.m11 += c.p[6] * inv_sub;
+    c.em.m22 += c.p[6] * inv_sub;
+    c.em.m33 += c.p[5] * inv_sub;
+    R maxA = c.p[1], minA = c.p[0];
+    bool full_range = minA <= -kPi / R(2) && maxA >= kPi / R(2);
+    if (minA >= maxA) {
+        c.clamp = 3;
+        error.z = shortest_angle(error.z - c.p[2], kPi / R(2));
+    } else if (!full_range && error.z >= maxA) {
+        c.clamp = 1;
+        error.z -= maxA;
+    } else if (!full_range && error.z <= minA) {
+        c.clamp = 2;
+        error.z -= minA;
+    } else {
+        c.s[2] = R(0);
+        c.em.m33 = R(0);
+        c.em.m31 = c.em.m13 = R(0);
+        c.em.m32 = c.em.m23 = R(0);
+        c.jac.m13 = c.jac.m23 = c.jac.m33 = R(0);
+    }
+    if (c.clamp == 1 || c.clamp == 2) {
+        c.s[2] = c.clamp == 1 ? std::min(c.s[2], R(0)) : std::max(c.s[2], R(0));
+        c.bilateral = inverse_bilateral_block(c.em);
+        c.cx = c.em.m13;
+        c.cy = c.em.m23;
+    }
+    c.em = inverse_symmetric(c.em);
+    c.bias = error * inv_dt;
+    c.bias.x *= c.p[3];
+    c.bias.y *= c.p[3];
+    c.bias.z *= c.p[4];
+    V3<R> acc = vecp(c.s);
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(mat_mul(acc, c.jac), A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(mat_mul(acc, c.jac), B.iw);
+}
+// IterateHingeAngle (:282-334)
+template <class R> void iterate_hinge_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
+    V3<R> jv = tmat_mul(*A.angvel - *B.angvel, c.jac);
+    V3<R> acc = vecp(c.s);
+    V3<R> soft = acc * inv_sub;
+    soft.x *= c.p[6];
+    soft.y *= c.p[6];
+    soft.z *= c.p[5];
+    V3<R> residual = jv + c.bias + soft;
+    V3<R> lambda = -R(1.0) * mat_mul(residual, c.em);
+    V3<R> orig = acc;
+    acc = acc + lambda;
+    if ((c.clamp == 1 && acc.z > R(0)) || (c.clamp == 2 && acc.z < R(0))) {
+        lambda.z = -orig.z;
+        R x = residual.x + c.cx * lambda.z;
+        R y = residual.y + c.cy * lambda.z;
+        lambda.x = -(c.bilateral.x * x + c.bilateral.y * y);
+        lambda.y = -(c.bilateral.y * x + c.bilateral.z * y);
+        acc.x = orig.x + lambda.x;
+        acc.y = orig.y + lambda.y;
+        acc.z = R(0);
+    } else if (c.clamp == 0) {
+        orig.z = R(0);
+        acc.z = R(0);
+    }
+    setv(c.s, acc);
+    lambda = acc - orig;
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(mat_mul(lambda, c.jac), A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(mat_mul(lambda, c.jac), B.iw);
+}
+
+// FixedAngle (FixedAngle.cs): params BiasFactor (0), Softness (1), Q0 (2-5, x y z w); state
+// AccumulatedImpulse (0-2), Hemisphere (3).
+// PrepareForIterationFixedAngle (:104-141)
+template <class R> void prepare_fixed_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                            R inv_sub) {
+    Q4<R> Q0{c.p[2], c.p[3], c.p[4], c.p[5]};
+    V3<R> error;
+    int hemisphere;
+    c.clamp = 1024;
+    c.jac = mscale(calc_jacobian(qmul(Q0, qconj(A.q)), B.q, error, hemisphere), -R(2));
+    int old_h = int(c.s[3]);
+    if (old_h != 0 && old_h != hemisphere) c.s[0] = c.s[1] = c.s[2] = R(0);
+    c.s[3] = R(hemisphere);
+    c.em = sym_to_matrix(sym_transform(sym_add(A.iw, B.iw), c.jac));
+    c.em.m11 += c.p[1] * inv_sub;
+    c.em.m22 += c.p[1] * inv_sub;
+    c.em.m33 += c.p[1] * inv_sub;
+    c.em = inverse_symmetric(c.em);
+    c.bias = -error * c.p[0] * inv_dt;
+    V3<R> acc = vecp(c.s);
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(tmat_mul(acc, c.jac), A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(tmat_mul(acc, c.jac), B.iw);
+}
+// IterateFixedAngle (:172-193)
+template <class R> void iterate_fixed_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
+    V3<R> jv = mat_mul(*A.angvel - *B.angvel, c.jac);
+    V3<R> acc = vecp(c.s);
+    V3<R> soft = acc * (c.p[1] * inv_sub);
+    V3<R> lambda = -R(1.0) * mat_mul(jv + c.bias + soft, c.em);
+    setv(c.s, acc + lambda);
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(tmat_mul(lambda, c.jac), A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(tmat_mul(lambda, c.jac), B.iw);
+}
+
+// DistanceLimit (DistanceLimit.cs): params LocalAnchor1 (0-2), LocalAnchor2 (3-5), BiasFactor (6),
+// Softness (7), Distance (8), LimitMin (9), LimitMax (10); state AccumulatedImpulse (0).
+// PrepareForIterationDistanceLimit (:220-309)
+template <class R> void prepare_distance_limit(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                               R inv_sub) {
+    V3<R> r1 = rotate(vecp(c.p), A.q);
+    V3<R> r2 = rotate(vecp(c.p + 3), B.q);
+    V3<R> p1 = A.pos + r1;
+    V3<R> p2 = B.pos + r2;
+    V3<R> dp = p2 - p1;
+    R distance = length(dp);
+    R error = distance - c.p[8];
+    c.clamp = 0;
+    R lmin = c.p[9], lmax = c.p[10];
+    if (lmin >= lmax) {
+        error -= R(0.5) * lmin + R(0.5) * lmax;
+        c.clamp = 3;
+    } else if (error >= lmax) {
+        c.clamp = 1;
+        error -= lmax;
+    } else if (error <= lmin) {
+        c.clamp = 2;
+        error -= lmin;
+    } else {
+        c.s[0] = R(0.0);
+        return;
+    }
+    if (distance == R(0)) {
+        c.clamp = 0;
+        c.s[0] = R(0);
+        return;
+    }
+    V3<R> n = dp * (R(1.0) / distance);
+    c.j0 = -R(1.0) * n;
+    c.j1 = -R(1.0) * cross(r1, n);
+    c.j2 = R(1.0) * n;
+    c.j3 = cross(r2, n);
+    c.em_s = dot(cmul(c.j0, A.imv), c.j0) + dot(cmul(c.j2, B.imv), c.j2) + dot(sym_mul(c.j1, A.iw), c.j1) +
+             dot(sym_mul(c.j3, B.iw), c.j3);
+    c.em_s += c.p[7] * inv_sub;
+    c.em_s = c.em_s > R(0) ? R(1.0) / c.em_s : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    c.bias_s = error * c.p[6] * inv_dt;
+    R acc = c.s[0];
+    if (A.dynamic) {
+        *A.vel = *A.vel + cmul(acc * c.j0, A.imv);
+        *A.angvel = *A.angvel + sym_mul(acc * c.j1, A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(acc * c.j2, B.imv);
+        *B.angvel = *B.angvel + sym_mul(acc * c.j3, B.iw);
+    }
+}
+// IterateDistanceLimit (:350-399)
+template <class R> void iterate_distance_limit(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                               R inv_sub) {
+    if (c.clamp == 0) return;
+    R jv = dot(*A.vel, c.j0) + dot(*A.angvel, c.j1) + dot(*B.vel, c.j2) + dot(*B.angvel, c.j3);
+    R soft = c.s[0] * c.p[7] * inv_sub;
+    R lambda = -c.em_s * (jv + c.bias_s + soft);
+    R old = c.s[0];
+    R acc = old + lambda;
+    if (c.clamp == 1) acc = std::min(acc, R(0.0));
+    else if (c.clamp == 2) acc = std::max(acc, R(0.0));
+    c.s[0] = acc;
+    lambda = acc - old;
+    if (A.dynamic) {
+        *A.vel = *A.vel + cmul(lambda * c.j0, A.imv);
+        *A.angvel = *A.angvel + sym_mul(lambda * c.j1, A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(lambda * c.j2, B.imv);
+        *B.angvel = *B.angvel + sym_mul(lambda * c.j3, B.iw);
+    }
+}
+
+template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                           R inv_sub) {
+    switch (c.type) {
+        case JH_CONSTRAINT_BALL_SOCKET: prepare_ball_socket(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_HINGE_ANGLE: prepare_hinge_angle(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_FIXED_ANGLE: prepare_fixed_angle(c, A, B, inv_dt, inv_sub); break;
+        default: prepare_distance_limit(c, A, B, inv_dt, inv_sub); break;
+    }
+}
+template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
+    switch (c.type) {
+        case JH_CONSTRAINT_BALL_SOCKET: iterate_ball_socket(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_HINGE_ANGLE: iterate_hinge_angle(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_FIXED_ANGLE: iterate_fixed_angle(c, A, B, inv_sub); break;
+        default: iterate_distance_limit(c, A, B, inv_sub); break;
+    }
+}
+
+}  // namespace orc

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "orc_collision.hpp"
+#include "orc_constraints.hpp"
 
 namespace orc {
 
@@ -103,6 +104,25 @@ template <class R> struct World {
     // until the next step's RemoveBrokenArbiters, which frees those still empty (EndCollide); arbiters
     // created by Detect raise BeginCollide (World.Step.cs:839-914).  Keys of the last step's events.
     std::vector<std::pair<uint32_t, uint32_t>> broken, ev_begin, ev_end;
+    // Constraints, solved after the contacts in each pass (World.Step.cs:1155-1191), in con_order (a
+    // replay of an order produced elsewhere) or, when empty, in index order.
+    std::vector<Constraint<R>> constraints;
+    std::vector<int> con_order;
+
+    ConstraintBody<R> cbody(int b) {
+        Body<R>& x = bodies[b];
+        return ConstraintBody<R>{&x.vel, &x.angvel, x.pos, x.inv_mass_vec, x.q, x.inv_inertia_world,
+                                 x.motion == JH_MOTION_DYNAMIC};
+    }
+    template <class F> void for_constraints(F f) {
+        if (con_order.empty()) {
+            for (auto& c : constraints)
+                if (c.enabled) f(c);
+        } else {
+            for (int i : con_order)
+                if (constraints[i].enabled) f(constraints[i]);
+        }
+    }
     long epa_calls = 0;
     Polytope<R>* poly = nullptr;
 
@@ -731,10 +751,19 @@ template <class R> struct World {
             for (auto* a : list)
                 for (int i = 0; i < 4; ++i)
                     if (a->usage & (1u << i)) prepare_contact(*a, a->c[i], ts);
-            for (int it = 0; it < iterations; ++it)
+            for_constraints([&](Constraint<R>& c) {  // PrepareConstraints (World.Step.cs:621-668)
+                ConstraintBody<R> A = cbody(c.b1), B = cbody(c.b2);
+                constraint_prepare(c, A, B, ts.inv_dt, ts.inv_substep_dt);
+            });
+            for (int it = 0; it < iterations; ++it) {
                 for (auto* a : list)
                     for (int i = 0; i < 4; ++i)
                         if (a->usage & (1u << i)) iterate_contact(*a, a->c[i], true);
+                for_constraints([&](Constraint<R>& c) {  // IterateConstraints (World.Step.cs:670-716)
+                    ConstraintBody<R> A = cbody(c.b1), B = cbody(c.b2);
+                    constraint_iterate(c, A, B, ts.inv_substep_dt);
+                });
+            }
             // IntegrateVelocities (World.Step.cs:1106-1133), no gyroscopic term
             for (auto& b : bodies) {
                 if (b.motion == JH_MOTION_STATIC) continue;

@@ -8,7 +8,7 @@ import subprocess
 
 import numpy as np
 
-from jitterphysics2_amd._abi import ARBITER_DTYPE, BODY_DTYPE, MANIFOLD_DTYPE, ptr
+from jitterphysics2_amd._abi import ARBITER_DTYPE, BODY_DTYPE, CONSTRAINT_DTYPE, MANIFOLD_DTYPE, ptr
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORC_DIR = os.path.join(ROOT, "oracle")
@@ -45,6 +45,8 @@ def lib() -> ctypes.CDLL:
             "orc_simplex_ab": [i32, i32, vp, vp, vp, vp, vp],
             "orc_sweep": [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
             "orc_get_events": [vp, vp, i32, vp, i32, vp],
+            "orc_set_constraints": [vp, vp, i32], "orc_get_constraints": [vp, vp, i32],
+            "orc_set_constraint_order": [vp, vp, i32],
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32
@@ -135,6 +137,21 @@ class OracleWorld:
         out = np.zeros(n, MANIFOLD_DTYPE)
         self.L.orc_get_manifolds(self.h, ptr(out), n)
         return out
+
+    def set_constraints(self, constraints: np.ndarray) -> None:
+        c = np.ascontiguousarray(constraints, CONSTRAINT_DTYPE)
+        self.L.orc_set_constraints(self.h, ptr(c), len(c))
+
+    def constraints(self) -> np.ndarray:
+        n = self.L.orc_get_constraints(self.h, None, 0)
+        out = np.zeros(n, CONSTRAINT_DTYPE)
+        self.L.orc_get_constraints(self.h, ptr(out), n)
+        return out
+
+    def set_constraint_order(self, order) -> None:
+        """Replay order of the constraints (indices); empty = index order."""
+        o = np.ascontiguousarray(order, np.int32)
+        assert self.L.orc_set_constraint_order(self.h, ptr(o), len(o)) == 0
 
     def collision_events(self) -> tuple[np.ndarray, np.ndarray]:
         """(BeginCollide pairs, EndCollide pairs) of the last step, in key order."""

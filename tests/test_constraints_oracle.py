@@ -1,0 +1,235 @@
+"""Constraints in the oracle (BallSocket, HingeAngle, FixedAngle, DistanceLimit), pinned against the
+reference's own outcome tests (Constraints/ConstraintSolverOutcomeTests.cs) and against what the
+mechanics fixes: a pendulum keeps its length and swings with the small-angle period, a hinge keeps
+the relative rotation on its axis and inside its limits, a weld keeps the relative orientation, a
+distance limit keeps the distance inside [min, max], and the solved constraints leave linear
+momentum unchanged (equal and opposite impulses) when nothing else acts.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from jitterphysics2_amd import constraints as C
+from jitterphysics2_amd import scenes
+from jitterphysics2_amd._abi import default_config
+from tests.orc import OracleWorld
+
+
+def rot_y(angle):
+    return (0.0, math.sin(angle / 2), 0.0, math.cos(angle / 2))
+
+
+def two_spheres(p1, p2, q2=(0, 0, 0, 1), gravity=False, static1=False, r=1.0):
+    sb = scenes.SceneBuilder()
+    a = sb.body(p1, motion=scenes.MOTION_STATIC if static1 else scenes.MOTION_DYNAMIC)
+    sb.sphere(a, r)
+    if static1:
+        sb.bodies[a]["inverse_mass"] = 0.0
+        sb.bodies[a]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    b = sb.body(p2, orientation=q2)
+    sb.sphere(b, r)
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    if not gravity:
+        cfg["gravity"] = (0.0, 0.0, 0.0)
+    return bodies, shapes, verts, cfg
+
+
+def advance(w, seconds, dt=1.0 / 100.0):
+    for _ in range(int(np.float32(seconds) / np.float32(dt))):
+        w.step(dt)
+
+
+def quat_rotate(v, q):
+    x, y, z, w = q
+    u = np.array([x, y, z], float)
+    v = np.asarray(v, float)
+    return v + 2 * w * np.cross(u, v) + 2 * np.cross(u, np.cross(u, v))
+
+
+def qconj(q):
+    return np.array([-q[0], -q[1], -q[2], q[3]], float)
+
+
+def qmul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + bw * ax + ay * bz - az * by, aw * by + bw * ay + az * bx - ax * bz,
+                     aw * bz + bw * az + ax * by - ay * bx, aw * bw - (ax * bx + ay * by + az * bz)])
+
+
+# ---- ConstraintSolverOutcomeTests.cs ----------------------------------------------------------
+def test_distance_limit_pulls_bodies_to_target_distance():  # :7-33
+    bodies, shapes, verts, cfg = two_spheres((-3, 0, 0), (3, 0, 0))
+    cfg["substep_count"] = 4
+    c = C.distance_limit(bodies, 0, 1, (-3, 0, 0), (3, 0, 0))
+    c["params"][8] = 2.0  # TargetDistance
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array([c]))
+    advance(w, 2)
+    p = w.bodies()["position"]
+    assert abs(np.linalg.norm(p[1] - p[0]) - 2.0) < 0.1
+
+
+def test_ball_socket_aligns_anchors():  # :35-62
+    bodies, shapes, verts, cfg = two_spheres((0, 0, 0), (3, 1, 0))
+    cfg["substep_count"] = 4
+    c = C.ball_socket(bodies, 0, 1, (1, 0, 0))
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array([c]))
+    advance(w, 2)
+    b = w.bodies()
+    a1 = quat_rotate(c["params"][0:3], b["orientation"][0]) + b["position"][0]
+    a2 = quat_rotate(c["params"][3:6], b["orientation"][1]) + b["position"][1]
+    assert np.linalg.norm(a2 - a1) < 0.1
+
+
+def test_fixed_angle_reduces_relative_orientation_error():  # :64-98
+    def run(with_constraint):
+        bodies, shapes, verts, cfg = two_spheres((0, 0, 0), (0, 0, 0), q2=rot_y(math.pi / 4))
+        cfg["substep_count"] = 4
+        shapes["body"][:] = shapes["body"]  # overlapping spheres: contacts as in the reference test
+        bodies["angular_velocity"][1] = (0, 8, 0)
+        w = OracleWorld(bodies, shapes, verts, cfg)
+        if with_constraint:
+            w.set_constraints(C.as_array([C.fixed_angle(bodies, 0, 1)]))
+        advance(w, 1)
+        b = w.bodies()
+        return abs(float(np.dot(b["orientation"][0], b["orientation"][1])))
+
+    assert run(True) > run(False)
+
+
+# ---- mechanics ----------------------------------------------------------------------------------
+def pendulum(L=2.0, angle=0.2, gravity=True):
+    sb = scenes.SceneBuilder()
+    a = sb.body((0, 10, 0), motion=scenes.MOTION_STATIC)
+    sb.sphere(a, 0.1)
+    sb.bodies[a]["inverse_mass"] = 0.0
+    sb.bodies[a]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    b = sb.body((L * math.sin(angle), 10 - L * math.cos(angle), 0))
+    sb.sphere(b, 0.2)
+    sb.bodies[b]["linear_damping_multiplier"] = 1.0
+    sb.bodies[b]["angular_damping_multiplier"] = 1.0
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    if not gravity:
+        cfg["gravity"] = (0, 0, 0)
+    return bodies, shapes, verts, cfg
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_pendulum_keeps_length_and_period(fp64):
+    L, angle, dt = 2.0, 0.15, 1.0 / 240.0
+    bodies, shapes, verts, cfg = pendulum(L, angle)
+    w = OracleWorld(bodies, shapes, verts, cfg, fp64=fp64)
+    w.set_constraints(C.as_array([C.ball_socket(bodies, 0, 1, (0, 10, 0))]))
+    xs, worst = [], 0.0
+    for _ in range(int(6.0 / dt)):
+        w.step(dt)
+        p = w.bodies()["position"][1].astype(float)
+        worst = max(worst, abs(np.linalg.norm(p - (0, 10, 0)) - L))
+        xs.append(p[0])
+    assert worst < 2e-3
+    xs = np.array(xs)
+    crossings = np.where(np.sign(xs[:-1]) != np.sign(xs[1:]))[0]
+    assert len(crossings) >= 4
+    period = 2 * np.diff(crossings).mean() * dt
+    # small-angle period of a physical pendulum: point mass on a massless rod plus the sphere's inertia
+    r = 0.2
+    T = 2 * math.pi * math.sqrt((L * L + 0.4 * r * r) / (9.81 * L))
+    assert abs(period - T) / T < 0.02
+
+
+def test_hinge_keeps_rotation_on_axis_and_inside_limits():
+    # a box hanging from a static body by a hinge about z, limited to [-30, 45] degrees, kicked hard
+    sb = scenes.SceneBuilder()
+    a = sb.body((0, 8, 0), motion=scenes.MOTION_STATIC)  # clear of the hanging box: no contacts
+    sb.box(a, (0.2, 0.2, 0.2))
+    sb.bodies[a]["inverse_mass"] = 0.0
+    sb.bodies[a]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    b = sb.body((0, 4, 0), angular_velocity=(3.0, 2.0, 12.0))
+    sb.box(b, (0.3, 2.0, 0.3))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    lo, hi = math.radians(-30), math.radians(45)
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array(C.hinge_joint(bodies, 0, 1, (0, 5, 0), (0, 0, 1), (lo, hi))))
+    worst_axis, angles = 0.0, []
+    for _ in range(300):
+        w.step(1.0 / 120.0)
+        q = w.bodies()["orientation"][1].astype(float)
+        axis_now = quat_rotate((0, 0, 1), q)
+        worst_axis = max(worst_axis, math.acos(min(1.0, abs(axis_now[2]))))
+        angles.append(2 * math.atan2(q[2], q[3]))
+    assert worst_axis < math.radians(2.0)
+    assert min(angles) > lo - math.radians(3) and max(angles) < hi + math.radians(3)
+    assert max(angles) > hi - math.radians(5)  # the kick reaches the upper limit
+
+
+def test_weld_keeps_relative_orientation():
+    sb = scenes.SceneBuilder()
+    a = sb.body((0, 0, 0), angular_velocity=(0, 0, 4.0))
+    sb.box(a, (1, 1, 1))
+    b = sb.body((1.5, 0, 0), orientation=rot_y(0.3))
+    sb.box(b, (1, 0.5, 0.5))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["gravity"] = (0, 0, 0)
+    cfg["substep_count"] = 4
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array(C.weld_joint(bodies, 0, 1, (0.75, 0, 0))))
+    q0 = qmul(qconj(bodies["orientation"][1]), bodies["orientation"][0])
+    mom0 = bodies["velocity"][0] / bodies["inverse_mass"][0] + bodies["velocity"][1] / bodies["inverse_mass"][1]
+    worst = 0.0
+    for _ in range(200):
+        w.step(1.0 / 100.0)
+        b2 = w.bodies()
+        q = qmul(qconj(b2["orientation"][1]), b2["orientation"][0])
+        worst = max(worst, 2 * math.acos(min(1.0, abs(float(np.dot(q, q0))))))
+    assert worst < math.radians(1.0)
+    b2 = w.bodies()
+    mom = b2["velocity"][0] / bodies["inverse_mass"][0] + b2["velocity"][1] / bodies["inverse_mass"][1]
+    np.testing.assert_allclose(mom, mom0, atol=1e-3)  # internal impulses cancel (damping only scales)
+
+
+@pytest.mark.parametrize("limits", [(1e-6, -1e-6), (-0.5, 0.3)])
+def test_distance_limit_keeps_distance_in_range(limits):
+    bodies, shapes, verts, cfg = pendulum(L=2.0, angle=1.0)
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    c = C.distance_limit(bodies, 0, 1, (0, 10, 0), bodies["position"][1])
+    c["params"][9], c["params"][10] = limits
+    w.set_constraints(C.as_array([c]))
+    ds = []
+    for _ in range(400):
+        w.step(1.0 / 100.0)
+        ds.append(float(np.linalg.norm(w.bodies()["position"][1].astype(float) - (0, 10, 0))))
+    lo, hi = (2.0, 2.0) if limits[0] >= limits[1] else (2.0 + limits[0], 2.0 + limits[1])
+    assert min(ds) > lo - 0.02 and max(ds) < hi + 0.02
+    if limits[0] < limits[1]:
+        assert max(ds) > hi - 0.05  # the swing stretches it to the upper limit
+
+
+def test_constraint_state_round_trip_and_order_replay():
+    # the accumulated impulses persist between steps (warm start) and a replayed order that is a
+    # permutation within body-disjoint batches gives the same result as index order
+    bodies, shapes, verts, cfg = pendulum()
+    recs = [C.ball_socket(bodies, 0, 1, (0, 10, 0)), C.distance_limit(bodies, 0, 1, (0, 10, 0), bodies["position"][1])]
+    def run(order, steps=3):
+        w = OracleWorld(bodies, shapes, verts, cfg)
+        w.set_constraints(C.as_array(recs))
+        if order is not None:
+            w.set_constraint_order(order)
+        for _ in range(steps):  # gravity enters from the second step (DeltaVelocity of RigidBody.Update)
+            w.step(0.01)
+        return w
+
+    w1 = run(None)
+    st = w1.constraints()
+    assert np.any(st["state"][0][:3] != 0) and st["state"][1][0] != 0
+    assert np.array_equal(w1.bodies()["position"], run([0, 1]).bodies()["position"])
+    # the two share body 1: the other Gauss-Seidel order gives other bits
+    assert not np.array_equal(w1.bodies()["velocity"], run([1, 0]).bodies()["velocity"])
