@@ -1,0 +1,544 @@
+// Constraints on the device: BallSocket, HingeAngle, FixedAngle, DistanceLimit
+// (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
+// pieces they use, FP32 in the reference's operation order.  One thread solves one constraint; the
+// solver visits constraints in batches with disjoint dynamic bodies.
+#pragma once
+#include "jh_math.h"
+#include "../../include/jitterhip.h"
+
+namespace jh {
+namespace con {
+
+// ---- JMatrix (LinearMath/JMatrix.cs), row-major fields m11 .. m33 -----------------------------
+__device__ __forceinline__ mat3 mzero() { return {0, 0, 0, 0, 0, 0, 0, 0, 0}; }
+__device__ __forceinline__ mat3 mmul(const mat3& a, const mat3& b) {  // operator * (:620-633)
+    mat3 r;
+    r.m11 = a.m11 * b.m11 + a.m12 * b.m21 + a.m13 * b.m31;
+    r.m12 = a.m11 * b.m12 + a.m12 * b.m22 + a.m13 * b.m32;
+    r.m13 = a.m11 * b.m13 + a.m12 * b.m23 + a.m13 * b.m33;
+    r.m21 = a.m21 * b.m11 + a.m22 * b.m21 + a.m23 * b.m31;
+    r.m22 = a.m21 * b.m12 + a.m22 * b.m22 + a.m23 * b.m32;
+    r.m23 = a.m21 * b.m13 + a.m22 * b.m23 + a.m23 * b.m33;
+    r.m31 = a.m31 * b.m11 + a.m32 * b.m21 + a.m33 * b.m31;
+    r.m32 = a.m31 * b.m12 + a.m32 * b.m22 + a.m33 * b.m32;
+    r.m33 = a.m31 * b.m13 + a.m32 * b.m23 + a.m33 * b.m33;
+    return r;
+}
+__device__ __forceinline__ mat3 madd(const mat3& a, const mat3& b) {  // Add (:353-364)
+    return {a.m11 + b.m11, a.m12 + b.m12, a.m13 + b.m13, a.m21 + b.m21, a.m22 + b.m22,
+            a.m23 + b.m23, a.m31 + b.m31, a.m32 + b.m32, a.m33 + b.m33};
+}
+__device__ __forceinline__ mat3 msub(const mat3& a, const mat3& b) {  // Subtract (:373-384)
+    return {a.m11 - b.m11, a.m12 - b.m12, a.m13 - b.m13, a.m21 - b.m21, a.m22 - b.m22,
+            a.m23 - b.m23, a.m31 - b.m31, a.m32 - b.m32, a.m33 - b.m33};
+}
+__device__ __forceinline__ mat3 mscale(const mat3& a, float s) {  // Multiply(matrix, scale) (:503-514)
+    return {a.m11 * s, a.m12 * s, a.m13 * s, a.m21 * s, a.m22 * s, a.m23 * s, a.m31 * s, a.m32 * s, a.m33 * s};
+}
+__device__ __forceinline__ mat3 mtranspose(const mat3& m) {  // Transpose (:603-614)
+    return {m.m11, m.m21, m.m31, m.m12, m.m22, m.m32, m.m13, m.m23, m.m33};
+}
+__device__ __forceinline__ float mdet(const mat3& m) {  // Determinant (:391-395)
+    return m.m11 * m.m22 * m.m33 + m.m12 * m.m23 * m.m31 + m.m13 * m.m21 * m.m32 - m.m31 * m.m22 * m.m13 -
+           m.m32 * m.m23 * m.m11 - m.m33 * m.m21 * m.m12;
+}
+__device__ __forceinline__ mat3 cross_matrix(vec3 v) {  // CreateCrossProduct (:597-600)
+    return {0.0f, -v.z, v.y, v.z, 0.0f, -v.x, -v.y, v.x, 0.0f};
+}
+__device__ __forceinline__ bool fin(float v) { return isfinite(v); }
+__device__ __forceinline__ bool mfinite(const mat3& m) {
+    return fin(m.m11) && fin(m.m12) && fin(m.m13) && fin(m.m21) && fin(m.m22) && fin(m.m23) && fin(m.m31) &&
+           fin(m.m32) && fin(m.m33);
+}
+// Real.IsNormal: finite, non-zero, not subnormal
+__device__ __forceinline__ bool is_normal(float v) {
+    float a = fabsf(v);
+    return a >= 1.17549435e-38f && a <= 3.40282347e+38f;
+}
+// TryInverseUnscaled (:455-481)
+__device__ __forceinline__ bool try_inverse_unscaled(const mat3& m, mat3& r) {
+    float idet = 1.0f / mdet(m);
+    if (!is_normal(idet)) {
+        r = mzero();
+        return false;
+    }
+    float n11 = m.m22 * m.m33 - m.m23 * m.m32;
+    float n12 = m.m13 * m.m32 - m.m12 * m.m33;
+    float n13 = m.m12 * m.m23 - m.m22 * m.m13;
+    float n21 = m.m23 * m.m31 - m.m33 * m.m21;
+    float n22 = m.m11 * m.m33 - m.m31 * m.m13;
+    float n23 = m.m13 * m.m21 - m.m23 * m.m11;
+    float n31 = m.m21 * m.m32 - m.m31 * m.m22;
+    float n32 = m.m12 * m.m31 - m.m32 * m.m11;
+    float n33 = m.m11 * m.m22 - m.m21 * m.m12;
+    r = {n11 * idet, n12 * idet, n13 * idet, n21 * idet, n22 * idet, n23 * idet, n31 * idet, n32 * idet, n33 * idet};
+    return mfinite(r);
+}
+// MathR.Max with NaN propagation
+__device__ __forceinline__ float max_nan(float a, float b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    return a > b ? a : b;
+}
+// Inverse (:404-444)
+__device__ __forceinline__ bool minverse(const mat3& m, mat3& out) {
+    mat3 inv;
+    if (try_inverse_unscaled(m, inv)) {
+        out = inv;
+        return true;
+    }
+    float scale = max_nan(fabsf(m.m11), max_nan(fabsf(m.m12), fabsf(m.m13)));
+    scale = max_nan(scale, max_nan(fabsf(m.m21), max_nan(fabsf(m.m22), fabsf(m.m23))));
+    scale = max_nan(scale, max_nan(fabsf(m.m31), max_nan(fabsf(m.m32), fabsf(m.m33))));
+    if (scale > 0.0f && fin(scale)) {
+        mat3 n{m.m11 / scale, m.m12 / scale, m.m13 / scale, m.m21 / scale, m.m22 / scale,
+               m.m23 / scale, m.m31 / scale, m.m32 / scale, m.m33 / scale};
+        if (try_inverse_unscaled(n, inv)) {
+            inv = {inv.m11 / scale, inv.m12 / scale, inv.m13 / scale, inv.m21 / scale, inv.m22 / scale,
+                   inv.m23 / scale, inv.m31 / scale, inv.m32 / scale, inv.m33 / scale};
+            if (mfinite(inv)) {
+                out = inv;
+                return true;
+            }
+        }
+    }
+    out = mzero();
+    return false;
+}
+// JVector.Transform(v, JMatrix) (JVector.cs:290-298): M v
+__device__ __forceinline__ vec3 mat_mul(vec3 v, const mat3& m) {
+    return {v.x * m.m11 + v.y * m.m12 + v.z * m.m13, v.x * m.m21 + v.y * m.m22 + v.z * m.m23,
+            v.x * m.m31 + v.y * m.m32 + v.z * m.m33};
+}
+// JVector.TransposedTransform(v, JMatrix) (JVector.cs:308-317): M^T v
+__device__ __forceinline__ vec3 tmat_mul(vec3 v, const mat3& m) {
+    return {v.x * m.m11 + v.y * m.m21 + v.z * m.m31, v.x * m.m12 + v.y * m.m22 + v.z * m.m32,
+            v.x * m.m13 + v.y * m.m23 + v.z * m.m33};
+}
+
+// ---- JSymmetricMatrix (LinearMath/JSymmetricMatrix.cs) ---------------------------------------
+__device__ __forceinline__ sym3 sym_add(const sym3& a, const sym3& b) {  // Add (:190-196)
+    return {a.m11 + b.m11, a.m12 + b.m12, a.m13 + b.m13, a.m22 + b.m22, a.m23 + b.m23, a.m33 + b.m33};
+}
+__device__ __forceinline__ sym3 sym_scale_vec(vec3 s) { return {s.x, 0.0f, 0.0f, s.y, 0.0f, s.z}; }  // CreateScale
+__device__ __forceinline__ sym3 sym_outer(vec3 v) {  // Outer (:89-93)
+    return {v.x * v.x, v.x * v.y, v.x * v.z, v.y * v.y, v.y * v.z, v.z * v.z};
+}
+__device__ __forceinline__ mat3 sym_to_matrix(const sym3& s) {  // ToMatrix (:68-71)
+    return {s.m11, s.m12, s.m13, s.m12, s.m22, s.m23, s.m13, s.m23, s.m33};
+}
+__device__ __forceinline__ float sym_det(const sym3& s) {  // Determinant (:104-109)
+    return s.m11 * (s.m22 * s.m33 - s.m23 * s.m23) + s.m12 * (s.m23 * s.m13 - s.m12 * s.m33) +
+           s.m13 * (s.m12 * s.m23 - s.m22 * s.m13);
+}
+
+// ---- MathHelper (LinearMath/MathHelper.cs) ---------------------------------------------------
+constexpr float kTolerance = 1e-6f;  // single precision
+
+// JacobiRotate (:374-407) on a[9] / basis[9] held in registers (p < q, both compile-time)
+template <int p, int q>
+__device__ __forceinline__ bool jacobi_rotate(float* a, float* basis) {
+    float off = a[3 * p + q];
+    if (fabsf(off) <= kTolerance) return false;
+    float tau = (a[3 * q + q] - a[3 * p + p]) / (2.0f * off);
+    float tangent = copysignf(1.0f, tau) / (fabsf(tau) + sqrtf(1.0f + tau * tau));
+    float cosine = 1.0f / sqrtf(1.0f + tangent * tangent);
+    float sine = tangent * cosine;
+    a[3 * p + p] -= tangent * off;
+    a[3 * q + q] += tangent * off;
+    a[3 * p + q] = a[3 * q + p] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k != p && k != q) {
+            float akp = a[3 * k + p], akq = a[3 * k + q];
+            a[3 * k + p] = a[3 * p + k] = cosine * akp - sine * akq;
+            a[3 * k + q] = a[3 * q + k] = sine * akp + cosine * akq;
+        }
+        float vkp = basis[3 * k + p], vkq = basis[3 * k + q];
+        basis[3 * k + p] = cosine * vkp - sine * vkq;
+        basis[3 * k + q] = sine * vkp + cosine * vkq;
+    }
+    return true;
+}
+
+// PseudoInverseSymmetric (:282-325)
+__device__ __forceinline__ mat3 pseudo_inverse_symmetric(const mat3& m) {
+    float scale = max_nan(fabsf(m.m11), max_nan(fabsf(m.m22), fabsf(m.m33)));
+    if (!(scale > 0.0f) || !fin(scale)) return mzero();
+    sym3 n{m.m11 / scale, m.m12 / scale, m.m13 / scale, m.m22 / scale, m.m23 / scale, m.m33 / scale};
+    mat3 inv;
+    if (fabsf(sym_det(n)) > kTolerance && minverse(m, inv)) return inv;
+    float a[9] = {n.m11, n.m12, n.m13, n.m12, n.m22, n.m23, n.m13, n.m23, n.m33};
+    float basis[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int sweep = 0; sweep < 8; ++sweep) {
+        bool changed = jacobi_rotate<0, 1>(a, basis);
+        changed |= jacobi_rotate<0, 2>(a, basis);
+        changed |= jacobi_rotate<1, 2>(a, basis);
+        if (!changed) break;
+    }
+    sym3 result{0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float eig = a[3 * i + i];
+        if (!(eig > kTolerance)) continue;
+        float weight = (1.0f / eig) / scale;
+        vec3 ax{basis[i], basis[3 + i], basis[6 + i]};
+        sym3 o{ax.x * ax.x, ax.x * ax.y, ax.x * ax.z, ax.y * ax.y, ax.y * ax.z, ax.z * ax.z};
+        sym3 w{o.m11 * weight, o.m12 * weight, o.m13 * weight, o.m22 * weight, o.m23 * weight, o.m33 * weight};
+        result = sym_add(result, w);
+    }
+    return sym_to_matrix(result);
+}
+
+// InverseSymmetric (:328-364)
+__device__ __forceinline__ mat3 inverse_symmetric(const mat3& m) {
+    float diag = m.m11 * m.m22 * m.m33;
+    mat3 inv;
+    if (mdet(m) / diag > kTolerance && minverse(m, inv)) return inv;
+    float x = m.m11 > 0.0f ? sqrtf(m.m11) : 0.0f;
+    float y = m.m22 > 0.0f ? sqrtf(m.m22) : 0.0f;
+    float z = m.m33 > 0.0f ? sqrtf(m.m33) : 0.0f;
+    if (x == 0.0f && y == 0.0f && z == 0.0f) return mzero();
+    float xy = x > 0.0f && y > 0.0f ? m.m12 / x / y : 0.0f;
+    float xz = x > 0.0f && z > 0.0f ? m.m13 / x / z : 0.0f;
+    float yz = y > 0.0f && z > 0.0f ? m.m23 / y / z : 0.0f;
+    mat3 normalized{1.0f, xy, xz, xy, 1.0f, yz, xz, yz, 1.0f};
+    mat3 resp = pseudo_inverse_symmetric(normalized);
+    xy = x > 0.0f && y > 0.0f ? resp.m12 / x / y : 0.0f;
+    xz = x > 0.0f && z > 0.0f ? resp.m13 / x / z : 0.0f;
+    yz = y > 0.0f && z > 0.0f ? resp.m23 / y / z : 0.0f;
+    mat3 r{x > 0.0f ? resp.m11 / x / x : 0.0f, xy, xz, xy, y > 0.0f ? resp.m22 / y / y : 0.0f, yz, xz, yz,
+           z > 0.0f ? resp.m33 / z / z : 0.0f};
+    bool ok = fin(r.m11) && fin(r.m12) && fin(r.m13) && fin(r.m22) && fin(r.m23) && fin(r.m33);
+    return ok ? r : mzero();
+}
+
+// InverseBilateralBlock (:367-372)
+__device__ __forceinline__ vec3 inverse_bilateral_block(const mat3& m) {
+    mat3 block{m.m11, m.m12, 0.0f, m.m12, m.m22, 0.0f, 0.0f, 0.0f, 0.0f};
+    mat3 inv = inverse_symmetric(block);
+    return {inv.m11, inv.m12, inv.m22};
+}
+
+// ---- AngularConstraintMath (Dynamics/Constraints/Internal/AngularConstraintMath.cs) ------------
+__device__ __forceinline__ quat qconj(quat q) { return {-q.x, -q.y, -q.z, q.w}; }
+__device__ __forceinline__ quat qscale(quat q, float s) { return {q.x * s, q.y * s, q.z * s, q.w * s}; }
+
+// ShortestAngle (:13-21)
+__device__ __forceinline__ float shortest_angle(float angle, float half) {
+    if (angle >= -half && angle <= half) return angle;
+    float period = 2.0f * half;
+    angle = fmodf(angle, period);
+    if (angle > half) angle -= period;
+    else if (angle < -half) angle += period;
+    return angle;
+}
+
+// CalculateJacobian (:63-94)
+__device__ __forceinline__ mat3 calc_jacobian(quat left, quat right, vec3& error, int& hemisphere) {
+    quat q = qmul(left, right);
+    hemisphere = q.w < 0.0f ? -1 : 1;
+    if (q.w < 0.0f) q = qscale(q, -1.0f);
+    vec3 vec{q.x, q.y, q.z};
+    float sine = sqrtf(len_sq(vec));
+    mat3 derivative;
+    if (sine > 0.0f) {
+        float half_angle = stable::atan2(sine, q.w);
+        vec3 axis = vec * (1.0f / sine);
+        error = axis * half_angle;
+        float diagonal = half_angle * q.w / sine;
+        mat3 ident{1, 0, 0, 0, 1, 0, 0, 0, 1};
+        mat3 t1 = mscale(ident, diagonal);
+        mat3 t2 = mscale(sym_to_matrix(sym_outer(axis)), 1.0f - diagonal);
+        derivative = msub(madd(t1, t2), cross_matrix(error));
+    } else {
+        error = vec3{0.0f, 0.0f, 0.0f};
+        derivative = mat3{1, 0, 0, 0, 1, 0, 0, 0, 1};
+    }
+    return mmul(mscale(derivative, -0.5f), mat_from_quat(left));
+}
+
+// std::min / std::max forms (the oracle's)
+__device__ __forceinline__ float cmin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float cmax(float a, float b) { return a < b ? b : a; }
+
+// ---- Constraints ----------------------------------------------------------------------------
+// What PrepareForIteration leaves for Iterate, per constraint in device memory (48 floats).
+struct Work {
+    vec3 r1, r2, bias, bilateral;  // 12
+    mat3 em, jac;                  // 18
+    float em_s, bias_s, cx, cy;    // 4
+    vec3 j0, j1, j2, j3;           // 12
+    int clamp;                     // 1
+    float pad;
+};
+static_assert(sizeof(Work) == 48 * sizeof(float), "constraint work record is 48 floats");
+
+struct CBody {
+    vec3 v, w, pos, imv;
+    quat q;
+    sym3 I;
+    bool dyn;
+};
+
+__device__ __forceinline__ vec3 p3(const float* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ void set3(float* p, vec3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+// PrepareForIterationBallSocket (BallSocket.cs:123-161)
+__device__ __forceinline__ void prepare_ball_socket(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                    float inv_sub) {
+    k.r1 = rot(p3(c.params), A.q);
+    k.r2 = rot(p3(c.params + 3), B.q);
+    vec3 p1 = A.pos + k.r1;
+    vec3 p2 = B.pos + k.r2;
+    mat3 cr1 = cross_matrix(k.r1), cr2 = cross_matrix(k.r2);
+    sym3 s = sym_add(sym_add(sym_add(sym_scale_vec(A.imv), sym_rotate(A.I, cr1)), sym_scale_vec(B.imv)),
+                     sym_rotate(B.I, cr2));
+    k.em = sym_to_matrix(s);
+    float softness = c.params[7] * inv_sub;
+    k.em.m11 += softness;
+    k.em.m22 += softness;
+    k.em.m33 += softness;
+    k.em = inverse_symmetric(k.em);
+    k.bias = (p2 - p1) * c.params[6] * inv_dt;
+    vec3 acc = p3(c.state);
+    if (A.dyn) {
+        A.v = A.v - cmul(acc, A.imv);
+        A.w = A.w - sym_mul(mat_mul(acc, cr1), A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(acc, B.imv);
+        B.w = B.w + sym_mul(mat_mul(acc, cr2), B.I);
+    }
+}
+// IterateBallSocket (BallSocket.cs:194-224)
+__device__ __forceinline__ void iterate_ball_socket(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    mat3 cr1 = cross_matrix(k.r1), cr2 = cross_matrix(k.r2);
+    vec3 acc = p3(c.state);
+    vec3 soft = acc * c.params[7] * inv_sub;
+    vec3 jv = -A.v + mat_mul(A.w, cr1) + B.v - mat_mul(B.w, cr2);
+    vec3 lambda = -1.0f * mat_mul(jv + k.bias + soft, k.em);
+    set3(c.state, acc + lambda);
+    if (A.dyn) {
+        A.v = A.v - cmul(lambda, A.imv);
+        A.w = A.w - sym_mul(mat_mul(lambda, cr1), A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(lambda, B.imv);
+        B.w = B.w + sym_mul(mat_mul(lambda, cr2), B.I);
+    }
+}
+
+// PrepareForIterationHingeAngle (HingeAngle.cs:140-226)
+__device__ __forceinline__ void prepare_hinge_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                    float inv_sub) {
+    const float* p = c.params;
+    vec3 axis = p3(p + 7);
+    quat Q0{p[10], p[11], p[12], p[13]};
+    vec3 p0 = create_orthonormal(axis);
+    vec3 p1 = cross(axis, p0);
+    vec3 rotation;
+    int hemisphere;
+    mat3 m0 = calc_jacobian(qmul(Q0, qconj(A.q)), B.q, rotation, hemisphere);
+    int old_h = int(c.state[3]);
+    if (old_h != 0 && old_h != hemisphere) c.state[0] = c.state[1] = c.state[2] = 0.0f;
+    c.state[3] = float(hemisphere);
+    vec3 error{dot(p0, rotation), dot(p1, rotation), dot(axis, rotation)};
+    k.clamp = 0;
+    vec3 j0 = tmat_mul(p0, m0), j1 = tmat_mul(p1, m0), j2 = tmat_mul(axis, m0);  // Jacobian columns
+    k.jac = {j0.x, j1.x, j2.x, j0.y, j1.y, j2.y, j0.z, j1.z, j2.z};
+    k.em = sym_to_matrix(sym_rotate(sym_add(A.I, B.I), mtranspose(k.jac)));
+    k.em.m11 += p[6] * inv_sub;
+    k.em.m22 += p[6] * inv_sub;
+    k.em.m33 += p[5] * inv_sub;
+    float maxA = p[1], minA = p[0];
+    bool full_range = minA <= -stable::Pi / 2.0f && maxA >= stable::Pi / 2.0f;
+    if (minA >= maxA) {
+        k.clamp = 3;
+        error.z = shortest_angle(error.z - p[2], stable::Pi / 2.0f);
+    } else if (!full_range && error.z >= maxA) {
+        k.clamp = 1;
+        error.z -= maxA;
+    } else if (!full_range && error.z <= minA) {
+        k.clamp = 2;
+        error.z -= minA;
+    } else {
+        c.state[2] = 0.0f;
+        k.em.m33 = 0.0f;
+        k.em.m31 = k.em.m13 = 0.0f;
+        k.em.m32 = k.em.m23 = 0.0f;
+        k.jac.m13 = k.jac.m23 = k.jac.m33 = 0.0f;
+    }
+    if (k.clamp == 1 || k.clamp == 2) {
+        c.state[2] = k.clamp == 1 ? cmin(c.state[2], 0.0f) : cmax(c.state[2], 0.0f);
+        k.bilateral = inverse_bilateral_block(k.em);
+        k.cx = k.em.m13;
+        k.cy = k.em.m23;
+    }
+    k.em = inverse_symmetric(k.em);
+    k.bias = error * inv_dt;
+    k.bias.x *= p[3];
+    k.bias.y *= p[3];
+    k.bias.z *= p[4];
+    vec3 acc = p3(c.state);
+    if (A.dyn) A.w = A.w + sym_mul(mat_mul(acc, k.jac), A.I);
+    if (B.dyn) B.w = B.w - sym_mul(mat_mul(acc, k.jac), B.I);
+}
+// IterateHingeAngle (HingeAngle.cs:282-334)
+__device__ __forceinline__ void iterate_hinge_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    const float* p = c.params;
+    vec3 jv = tmat_mul(A.w - B.w, k.jac);
+    vec3 acc = p3(c.state);
+    vec3 soft = acc * inv_sub;
+    soft.x *= p[6];
+    soft.y *= p[6];
+    soft.z *= p[5];
+    vec3 residual = jv + k.bias + soft;
+    vec3 lambda = -1.0f * mat_mul(residual, k.em);
+    vec3 orig = acc;
+    acc = acc + lambda;
+    if ((k.clamp == 1 && acc.z > 0.0f) || (k.clamp == 2 && acc.z < 0.0f)) {
+        lambda.z = -orig.z;
+        float x = residual.x + k.cx * lambda.z;
+        float y = residual.y + k.cy * lambda.z;
+        lambda.x = -(k.bilateral.x * x + k.bilateral.y * y);
+        lambda.y = -(k.bilateral.y * x + k.bilateral.z * y);
+        acc.x = orig.x + lambda.x;
+        acc.y = orig.y + lambda.y;
+        acc.z = 0.0f;
+    } else if (k.clamp == 0) {
+        orig.z = 0.0f;
+        acc.z = 0.0f;
+    }
+    set3(c.state, acc);
+    lambda = acc - orig;
+    if (A.dyn) A.w = A.w + sym_mul(mat_mul(lambda, k.jac), A.I);
+    if (B.dyn) B.w = B.w - sym_mul(mat_mul(lambda, k.jac), B.I);
+}
+
+// PrepareForIterationFixedAngle (FixedAngle.cs:104-141)
+__device__ __forceinline__ void prepare_fixed_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                    float inv_sub) {
+    const float* p = c.params;
+    quat Q0{p[2], p[3], p[4], p[5]};
+    vec3 error;
+    int hemisphere;
+    k.clamp = 1024;
+    k.jac = mscale(calc_jacobian(qmul(Q0, qconj(A.q)), B.q, error, hemisphere), -2.0f);
+    int old_h = int(c.state[3]);
+    if (old_h != 0 && old_h != hemisphere) c.state[0] = c.state[1] = c.state[2] = 0.0f;
+    c.state[3] = float(hemisphere);
+    k.em = sym_to_matrix(sym_rotate(sym_add(A.I, B.I), k.jac));
+    k.em.m11 += p[1] * inv_sub;
+    k.em.m22 += p[1] * inv_sub;
+    k.em.m33 += p[1] * inv_sub;
+    k.em = inverse_symmetric(k.em);
+    k.bias = -error * p[0] * inv_dt;
+    vec3 acc = p3(c.state);
+    if (A.dyn) A.w = A.w + sym_mul(tmat_mul(acc, k.jac), A.I);
+    if (B.dyn) B.w = B.w - sym_mul(tmat_mul(acc, k.jac), B.I);
+}
+// IterateFixedAngle (FixedAngle.cs:172-193)
+__device__ __forceinline__ void iterate_fixed_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    vec3 jv = mat_mul(A.w - B.w, k.jac);
+    vec3 acc = p3(c.state);
+    vec3 soft = acc * (c.params[1] * inv_sub);
+    vec3 lambda = -1.0f * mat_mul(jv + k.bias + soft, k.em);
+    set3(c.state, acc + lambda);
+    if (A.dyn) A.w = A.w + sym_mul(tmat_mul(lambda, k.jac), A.I);
+    if (B.dyn) B.w = B.w - sym_mul(tmat_mul(lambda, k.jac), B.I);
+}
+
+// PrepareForIterationDistanceLimit (DistanceLimit.cs:220-309)
+__device__ __forceinline__ void prepare_distance_limit(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                       float inv_sub) {
+    const float* p = c.params;
+    vec3 r1 = rot(p3(p), A.q);
+    vec3 r2 = rot(p3(p + 3), B.q);
+    vec3 p1 = A.pos + r1;
+    vec3 p2 = B.pos + r2;
+    vec3 dp = p2 - p1;
+    float distance = sqrtf(len_sq(dp));
+    float error = distance - p[8];
+    k.clamp = 0;
+    float lmin = p[9], lmax = p[10];
+    if (lmin >= lmax) {
+        error -= 0.5f * lmin + 0.5f * lmax;
+        k.clamp = 3;
+    } else if (error >= lmax) {
+        k.clamp = 1;
+        error -= lmax;
+    } else if (error <= lmin) {
+        k.clamp = 2;
+        error -= lmin;
+    } else {
+        c.state[0] = 0.0f;
+        return;
+    }
+    if (distance == 0.0f) {
+        k.clamp = 0;
+        c.state[0] = 0.0f;
+        return;
+    }
+    vec3 n = dp * (1.0f / distance);
+    k.j0 = -1.0f * n;
+    k.j1 = -1.0f * cross(r1, n);
+    k.j2 = 1.0f * n;
+    k.j3 = cross(r2, n);
+    k.em_s = dot(cmul(k.j0, A.imv), k.j0) + dot(cmul(k.j2, B.imv), k.j2) + dot(sym_mul(k.j1, A.I), k.j1) +
+             dot(sym_mul(k.j3, B.I), k.j3);
+    k.em_s += p[7] * inv_sub;
+    k.em_s = k.em_s > 0.0f ? 1.0f / k.em_s : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    k.bias_s = error * p[6] * inv_dt;
+    float acc = c.state[0];
+    if (A.dyn) {
+        A.v = A.v + cmul(acc * k.j0, A.imv);
+        A.w = A.w + sym_mul(acc * k.j1, A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(acc * k.j2, B.imv);
+        B.w = B.w + sym_mul(acc * k.j3, B.I);
+    }
+}
+// IterateDistanceLimit (DistanceLimit.cs:350-399)
+__device__ __forceinline__ void iterate_distance_limit(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    if (k.clamp == 0) return;
+    float jv = dot(A.v, k.j0) + dot(A.w, k.j1) + dot(B.v, k.j2) + dot(B.w, k.j3);
+    float soft = c.state[0] * c.params[7] * inv_sub;
+    float lambda = -k.em_s * (jv + k.bias_s + soft);
+    float old = c.state[0];
+    float acc = old + lambda;
+    if (k.clamp == 1) acc = cmin(acc, 0.0f);
+    else if (k.clamp == 2) acc = cmax(acc, 0.0f);
+    c.state[0] = acc;
+    lambda = acc - old;
+    if (A.dyn) {
+        A.v = A.v + cmul(lambda * k.j0, A.imv);
+        A.w = A.w + sym_mul(lambda * k.j1, A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(lambda * k.j2, B.imv);
+        B.w = B.w + sym_mul(lambda * k.j3, B.I);
+    }
+}
+
+__device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt, float inv_sub) {
+    switch (c.type) {
+        case JH_CONSTRAINT_BALL_SOCKET: prepare_ball_socket(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_HINGE_ANGLE: prepare_hinge_angle(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_FIXED_ANGLE: prepare_fixed_angle(c, k, A, B, inv_dt, inv_sub); break;
+        default: prepare_distance_limit(c, k, A, B, inv_dt, inv_sub); break;
+    }
+}
+__device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    switch (c.type) {
+        case JH_CONSTRAINT_BALL_SOCKET: iterate_ball_socket(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_HINGE_ANGLE: iterate_hinge_angle(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_FIXED_ANGLE: iterate_fixed_angle(c, k, A, B, inv_sub); break;
+        default: iterate_distance_limit(c, k, A, B, inv_sub); break;
+    }
+}
+
+}  // namespace con
+}  // namespace jh

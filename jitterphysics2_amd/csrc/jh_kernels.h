@@ -6,6 +6,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include "jh_collide.h"
+#include "jh_constraints.h"
 #include "jh_epa.h"
 #include "jh_epa_wave.h"
 #include "jh_filter.h"
@@ -1349,6 +1350,15 @@ struct ContactSoA {
     }
 };
 
+// Constraints of the barrier solver: enabled constraints in batches with disjoint dynamic bodies.
+struct ConArgs {
+    jh_constraint* c;
+    con::Work* work;     // what each constraint's prepare leaves for its iterations
+    const int* order;    // constraint indices, batch by batch
+    const int* offs;     // nbatch + 1 batch offsets into order
+    int nbatch;
+};
+
 struct SolveArgs {
     jh_arbiter* arb;
     const uint64_t* order;   // (colour << 32) | arbiter, sorted
@@ -1368,6 +1378,7 @@ struct SolveArgs {
     int iterations, relaxations;
     ContactSoA soa;          // dataflow solver: colour-ordered contact rows
     float4* vw;              // dataflow solvers: per body (v.x v.y v.z n)(w.x w.y w.z n), 32 bytes
+    ConArgs con;             // barrier solver: constraints (nbatch 0: none)
 };
 
 __device__ __forceinline__ void integrate_forces(const SolveArgs& P, int t0, int stride) {
@@ -1414,6 +1425,42 @@ __device__ __forceinline__ void solve_batches(const SolveArgs& P, int phase, int
     }
 }
 
+// PrepareConstraints / IterateConstraints (World.Step.cs:621-716) after the contacts of the same pass:
+// phase 0 prepares, phase 1 iterates.  One grid barrier per batch.
+__device__ __forceinline__ void con_load(const Bodies& B, int b, con::CBody& r) {
+    r.v = ld3(B.vel[b]);
+    r.w = ld3(B.angvel[b]);
+    r.pos = ld3(B.pos[b]);
+    r.q = ldq(B.quat[b]);
+    r.imv = ld3(B.imv[b]);
+    r.I = ld_iw(B, b);
+    r.dyn = B.motion[b] == JH_MOTION_DYNAMIC;
+}
+__device__ __forceinline__ void solve_constraint_batches(const SolveArgs& P, int phase) {
+    const ConArgs& C = P.con;
+    int stride = gridDim.x * blockDim.x;
+    int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const Bodies& B = P.B;
+    for (int b = 0; b < C.nbatch; ++b) {
+        int beg = C.offs[b], end = C.offs[b + 1];
+        for (int k = beg + t0; k < end; k += stride) {
+            int i = C.order[k];
+            jh_constraint c = C.c[i];
+            con::Work w = C.work[i];
+            con::CBody A, Bb;
+            con_load(B, c.body1, A);
+            con_load(B, c.body2, Bb);
+            if (phase == 0) con::prepare(c, w, A, Bb, P.ts.inv_dt, P.ts.inv_substep_dt);
+            else con::iterate(c, w, A, Bb, P.ts.inv_substep_dt);
+            for (int q = 0; q < 8; ++q) C.c[i].state[q] = c.state[q];
+            if (phase == 0) C.work[i] = w;
+            if (A.dyn) { B.vel[c.body1] = st3(A.v); B.angvel[c.body1] = st3(A.w); }
+            if (Bb.dyn) { B.vel[c.body2] = st3(Bb.v); B.angvel[c.body2] = st3(Bb.w); }
+        }
+        grid_barrier(P.bar);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1422,7 +1469,11 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
         integrate_forces(P, t0, stride);
         grid_barrier(P.bar);
         solve_batches(P, 0, nc);
-        for (int it = 0; it < P.iterations; ++it) solve_batches(P, 1, nc);
+        solve_constraint_batches(P, 0);
+        for (int it = 0; it < P.iterations; ++it) {
+            solve_batches(P, 1, nc);
+            solve_constraint_batches(P, 1);
+        }
         integrate_velocities(P, t0, stride);
         grid_barrier(P.bar);
         for (int it = 0; it < P.relaxations; ++it) solve_batches(P, 2, nc);

@@ -180,6 +180,13 @@ struct jh_world {
     DBuf<EpaRes> epa_res;
     DBuf<int> specq;  // pairs queued for the speculative sweep
     DBuf<uint64_t> zkey, evb, eve;  // keys of arbiters removed by the last step; collision events
+    // constraints (solved by the barrier solver after the contacts of each pass)
+    std::vector<jh_constraint> host_con;
+    DBuf<jh_constraint> con;
+    DBuf<con::Work> con_work;
+    DBuf<int> con_order, con_offs;
+    std::vector<int> con_order_h, con_offs_h;  // batch order of the enabled constraints
+    bool con_dirty = false;                    // recolour before the next step (constraints or bodies changed)
     int epa_threads = 0;
     bool epa_wave = true;      // wave-cooperative EPA (k_epa_wave); false: one thread per pair (k_epa)
     int epa_wave_blocks = 0;   // resident single-wave workgroups of k_epa_wave
@@ -409,6 +416,64 @@ jh_status ensure_step_buffers(jh_world* w) {
     if (w->kind == SOLVER_LANES) HIPCHK(w->rec.ensure(na * kRecVec));
     return JH_OK;
 }
+
+// Batches of the enabled constraints: first fit in index order over the colours of their dynamic
+// bodies (static and kinematic bodies are only read by constraints, so they never conflict), then
+// the indices grouped colour by colour.  Recomputed when constraints or body motion types change.
+jh_status colour_constraints(jh_world* w) {
+    int n = int(w->host_con.size());
+    std::vector<int> colour(n, -1);
+    std::vector<std::vector<uint64_t>> used(w->nb);
+    int ncol = 0;
+    auto is_dyn = [&](int b) { return w->host_bodies[b].motion_type == JH_MOTION_DYNAMIC; };
+    for (int i = 0; i < n; ++i) {
+        const jh_constraint& c = w->host_con[i];
+        if (c.body1 >= w->nb || c.body2 >= w->nb)
+            return fail(JH_ERR_STATE, "constraint %d refers to a body beyond the %d bodies set", i, w->nb);
+        if (!c.enabled) continue;
+        int b[2] = {c.body1, c.body2};
+        for (int col = 0;; ++col) {
+            bool free = true;
+            for (int e = 0; e < 2 && free; ++e) {
+                if (!is_dyn(b[e])) continue;
+                const auto& u = used[b[e]];
+                size_t word = size_t(col) >> 6;
+                if (word < u.size() && (u[word] >> (col & 63) & 1)) free = false;
+            }
+            if (!free) continue;
+            for (int e = 0; e < 2; ++e) {
+                if (!is_dyn(b[e])) continue;
+                auto& u = used[b[e]];
+                size_t word = size_t(col) >> 6;
+                if (u.size() <= word) u.resize(word + 1, 0);
+                u[word] |= 1ull << (col & 63);
+            }
+            colour[i] = col;
+            ncol = std::max(ncol, col + 1);
+            break;
+        }
+    }
+    std::vector<int> count(ncol + 1, 0);
+    for (int i = 0; i < n; ++i)
+        if (colour[i] >= 0) count[colour[i] + 1]++;
+    for (int k = 0; k < ncol; ++k) count[k + 1] += count[k];
+    w->con_offs_h.assign(count.begin(), count.end());
+    w->con_order_h.assign(count[ncol], 0);
+    std::vector<int> fill(count.begin(), count.end() - 1);
+    for (int i = 0; i < n; ++i)
+        if (colour[i] >= 0) w->con_order_h[fill[colour[i]]++] = i;
+    HIPCHK(w->con_order.ensure(std::max<size_t>(w->con_order_h.size(), 1)));
+    HIPCHK(w->con_offs.ensure(w->con_offs_h.size()));
+    if (!w->con_order_h.empty())
+        HIPCHK(hipMemcpyAsync(w->con_order.p, w->con_order_h.data(), w->con_order_h.size() * sizeof(int),
+                              hipMemcpyHostToDevice, w->st));
+    HIPCHK(hipMemcpyAsync(w->con_offs.p, w->con_offs_h.data(), w->con_offs_h.size() * sizeof(int),
+                          hipMemcpyHostToDevice, w->st));
+    w->con_dirty = false;
+    return JH_OK;
+}
+
+int constraint_batches(const jh_world* w) { return w->con_offs_h.empty() ? 0 : int(w->con_offs_h.size()) - 1; }
 
 jh_status launch_coop(const void* fn, int grid, int block, void** args, hipStream_t st) {
     HIPCHK(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, st));
@@ -659,6 +724,13 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     // the tile solver needs its tiles to fit; otherwise this step runs the dataflow solver
     size_t tile_lds = w->kind == SOLVER_TILES ? tile_lds_bytes(w, w->tiles) : 0;
     int kind = (w->kind == SOLVER_TILES && tile_lds == 0) ? SOLVER_FLOW : w->kind;
+    if (!w->host_con.empty()) {
+        if (w->con_dirty) {
+            s = colour_constraints(w);
+            if (s != JH_OK) return s;
+        }
+        if (constraint_batches(w) > 0) kind = SOLVER_BARRIER;  // constraints run in the barrier solver
+    }
     bool use_tiles = kind == SOLVER_TILES;
     {
         ColourArgs P;
@@ -740,6 +812,11 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.queue = w->fqueue.p;
         S.qmode = w->flow_qmode;
         S.soa.mv = kind == SOLVER_FLOW ? w->soa_mv.p : nullptr;
+        S.con.c = w->con.p;
+        S.con.work = w->con_work.p;
+        S.con.order = w->con_order.p;
+        S.con.offs = w->con_offs.p;
+        S.con.nbatch = kind == SOLVER_BARRIER ? constraint_batches(w) : 0;
         if (probe) {
             s = run_probe(w, S, a_cur, amax_, probe);
             if (s != JH_OK) return s;
@@ -813,7 +890,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             HIPCHK(hipEventRecord(w->ev[9], st));
             k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
         } else {
-            int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(amax_, w->nb), 256), 256, 1 << 20));
+            int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(std::max(amax_, w->nb), 256),
+                                                                  int(w->con_order_h.size())), 256, 1 << 20));
             HIPCHK(hipEventRecord(w->ev[8], st));
             s = launch_coop((const void*)k_solve, sgrid, 256, sargs, st);
             if (s != JH_OK) return s;
@@ -902,6 +980,7 @@ int32_t jh_struct_size(int32_t which) {
         case 6: return sizeof(jh_stats);
         case 7: return sizeof(jh_probe);
         case 8: return sizeof(jh_tuning);
+        case 9: return sizeof(jh_constraint);
         default: return -1;
     }
 }
@@ -1042,6 +1121,7 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     w->max_world = maxw;
     w->n_spec = 0;
     for (int i = 0; i < n; ++i) w->n_spec += bodies[i].enable_speculative_contacts != 0;
+    w->con_dirty = true;  // constraint batches depend on which bodies are dynamic
     w->host_bodies.assign(bodies, bodies + n);
     {
         // bodies grouped by world (counting sort), worlds scheduled largest first
@@ -1237,6 +1317,65 @@ jh_status jh_world_get_manifolds(jh_world* w, jh_manifold* out, int32_t capacity
     }
     *out_count = n;
     if (n > capacity) return fail(JH_ERR_CAPACITY, "manifold count %d exceeds capacity %d", n, capacity);
+    return JH_OK;
+}
+
+jh_status jh_world_set_constraints(jh_world* w, const jh_constraint* in, int32_t n) {
+    if (!w || n < 0 || (n > 0 && !in)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    for (int i = 0; i < n; ++i) {
+        const jh_constraint& c = in[i];
+        if (c.type < JH_CONSTRAINT_BALL_SOCKET || c.type > JH_CONSTRAINT_DISTANCE_LIMIT)
+            return fail(JH_ERR_INVALID_ARG, "constraint %d: unknown type %d", i, c.type);
+        if (c.body1 < 0 || c.body1 >= w->nb || c.body2 < 0 || c.body2 >= w->nb)
+            return fail(JH_ERR_INVALID_ARG, "constraint %d: body index out of range", i);
+        if (c.body1 == c.body2)  // SameBodyException (World.CreateConstraint, World.cs:860-861)
+            return fail(JH_ERR_INVALID_ARG, "constraint %d: body1 and body2 are the same body", i);
+        for (int k = 0; k < 16; ++k)
+            if (std::isnan(c.params[k])) return fail(JH_ERR_NON_FINITE, "constraint %d: NaN parameter", i);
+    }
+    w->host_con.assign(in, in + n);
+    size_t cap = std::max(n, 1);
+    HIPCHK(w->con.ensure(cap));
+    HIPCHK(w->con_work.ensure(cap));
+    if (n > 0) HIPCHK(hipMemcpyAsync(w->con.p, in, n * sizeof(jh_constraint), hipMemcpyHostToDevice, w->st));
+    HIPCHK(hipMemsetAsync(w->con_work.p, 0, cap * sizeof(con::Work), w->st));
+    w->con_dirty = true;
+    jh_status s = colour_constraints(w);
+    if (s != JH_OK) return s;
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_get_constraints(jh_world* w, jh_constraint* out, int32_t capacity, int32_t* out_count) {
+    if (!w || !out_count || capacity < 0 || (capacity > 0 && !out)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    int n = int(w->host_con.size());
+    *out_count = n;
+    int k = std::min(n, capacity);
+    if (k > 0) {
+        HIPCHK(hipMemcpyAsync(out, w->con.p, k * sizeof(jh_constraint), hipMemcpyDeviceToHost, w->st));
+        HIPCHK(hipStreamSynchronize(w->st));
+    }
+    if (n > capacity) return fail(JH_ERR_CAPACITY, "constraint count %d exceeds capacity %d", n, capacity);
+    return JH_OK;
+}
+
+jh_status jh_world_get_constraint_order(jh_world* w, int32_t* order, int32_t capacity, int32_t* batch_offsets,
+                                        int32_t batch_capacity, int32_t* out_count, int32_t* out_batch_count) {
+    if (!w || !out_count || !out_batch_count || capacity < 0 || batch_capacity < 0 || (capacity > 0 && !order) ||
+        (batch_capacity > 0 && !batch_offsets))
+        return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (w->con_dirty && !w->host_con.empty()) {
+        jh_status s = colour_constraints(w);
+        if (s != JH_OK) return s;
+        HIPCHK(hipStreamSynchronize(w->st));
+    }
+    int n = int(w->con_order_h.size()), nb = constraint_batches(w);
+    *out_count = n;
+    *out_batch_count = nb;
+    for (int i = 0; i < std::min(n, capacity); ++i) order[i] = w->con_order_h[i];
+    for (int i = 0; i < std::min(nb + 1, batch_capacity); ++i) batch_offsets[i] = w->con_offs_h[i];
+    if (n > capacity || nb + 1 > batch_capacity)
+        return fail(JH_ERR_CAPACITY, "constraint order (%d, %d batches) exceeds capacity", n, nb);
     return JH_OK;
 }
 
@@ -1719,7 +1858,8 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
     // solver candidates (kind, workgroups); the barrier solver only tunes its own grid
     struct Cand { int kind, n; };
     std::vector<Cand> sc;
-    if (w->kind == SOLVER_BARRIER) {
+    if (w->kind == SOLVER_BARRIER || !w->host_con.empty()) {  // constraints run in the barrier solver
+        if (!w->host_con.empty()) w->kind = SOLVER_BARRIER;
         for (int n : uniq({cu / 2, cu, 2 * cu, 4 * cu, 8 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n});
     } else {
         for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, 3 * cu / 4, cu, 2 * cu}, w->flow_max))

@@ -9,7 +9,7 @@ import ctypes
 
 import numpy as np
 
-from ._abi import (ARBITER_DTYPE, BODY_DTYPE, CONFIG_DTYPE, LWS_FEATURES, MANIFOLD_DTYPE, PROBE_DTYPE,
+from ._abi import (ARBITER_DTYPE, BODY_DTYPE, CONFIG_DTYPE, CONSTRAINT_DTYPE, LWS_FEATURES, MANIFOLD_DTYPE, PROBE_DTYPE,
                    STATS_DTYPE, TUNING_DTYPE, check, default_config, load_library, ptr)
 
 
@@ -90,6 +90,29 @@ class GpuWorld:
         check(self.lib, self.lib.jh_world_get_arbiters(self.handle, ptr(out), n.value, ctypes.byref(n)),
               "jh_world_get_arbiters")
         return out[:n.value]
+
+    def set_constraints(self, constraints: np.ndarray) -> None:
+        c = np.ascontiguousarray(constraints, CONSTRAINT_DTYPE)
+        check(self.lib, self.lib.jh_world_set_constraints(self.handle, ptr(c), len(c)), "jh_world_set_constraints")
+
+    def constraints(self) -> np.ndarray:
+        n = ctypes.c_int32(0)
+        self.lib.jh_world_get_constraints(self.handle, None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1), CONSTRAINT_DTYPE)
+        check(self.lib, self.lib.jh_world_get_constraints(self.handle, ptr(out), n.value, ctypes.byref(n)),
+              "jh_world_get_constraints")
+        return out[:n.value]
+
+    def constraint_order(self) -> tuple[np.ndarray, np.ndarray]:
+        """(constraint indices in solve order, batch offsets) of the enabled constraints."""
+        n, nb = ctypes.c_int32(0), ctypes.c_int32(0)
+        self.lib.jh_world_get_constraint_order(self.handle, None, 0, None, 0, ctypes.byref(n), ctypes.byref(nb))
+        order = np.zeros(max(n.value, 1), np.int32)
+        offs = np.zeros(nb.value + 1, np.int32)
+        check(self.lib, self.lib.jh_world_get_constraint_order(self.handle, ptr(order), n.value, ptr(offs),
+                                                               nb.value + 1, ctypes.byref(n), ctypes.byref(nb)),
+              "jh_world_get_constraint_order")
+        return order[:n.value], offs
 
     def collision_events(self) -> tuple[np.ndarray, np.ndarray]:
         """(BeginCollide pairs, EndCollide pairs) of the last step, (K, 2) uint32 shape indices in key

@@ -506,6 +506,78 @@ def bullets(n_bodies: int = 512, seed: int = 0x4A2D0006, speed=(15.0, 60.0), spe
     return (*sb.arrays(), cfg, 1.0 / 60.0)
 
 
+def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
+    """Jointed bodies over a static floor, for the constraint solver: per group one of (cycling)
+    a hinge chain of boxes hanging from a static post (HingeJoint, limits +-60 degrees), a rope of
+    spheres (BallSocket chain), welded box pairs (WeldJoint) dropped on the floor, and capsules tied to a
+    post by distance limits with a slack range; free boxes fall between them.  Returns (bodies, shapes,
+    verts, cfg, dt, constraints).  dt = 1/60, 4 substeps, 10 iterations."""
+    from . import constraints as C
+
+    rng = SplitMix64(seed)
+    sb = SceneBuilder()
+    _static_box(sb, (0.0, -0.5, 0.0), (200.0, 1.0, 200.0))
+    recs: list = []
+    plan = []  # deferred: constraints need the final body array
+    side = int(math.ceil(math.sqrt(n_groups)))
+    for g in range(n_groups):
+        gx, gz = (g % side - side / 2.0) * 8.0, (g // side - side / 2.0) * 8.0
+        kind = g % 4
+        top = 14.0
+        if kind in (0, 1, 3):
+            post = _static_box(sb, (gx, top + 0.5, gz), (0.4, 1.0, 0.4))
+        if kind == 0:  # hinge chain
+            prev, anchor_y = post, top
+            for k in range(links):
+                b = sb.body((gx, anchor_y - 0.55, gz), velocity=(rng.normal(), 0.0, rng.normal()))
+                sb.box(b, (0.3, 1.0, 0.3))
+                plan.append(("hinge", prev, b, (gx, anchor_y, gz)))
+                prev, anchor_y = b, anchor_y - 1.1
+        elif kind == 1:  # rope of spheres
+            prev, anchor_y = post, top
+            for k in range(links):
+                b = sb.body((gx + 0.6 * (k + 1), anchor_y, gz))
+                sb.sphere(b, 0.25)
+                ax = gx + 0.6 * k
+                plan.append(("ball", prev, b, (ax + 0.3 if k else gx, anchor_y, gz)))
+                prev = b
+        elif kind == 2:  # welded pairs dropping onto the floor
+            for k in range(links // 2):
+                y = 2.0 + 2.2 * k
+                a = sb.body((gx - 0.5, y, gz), orientation=rng.quaternion())
+                sb.box(a, (1.0, 0.5, 0.5))
+                qa = sb.bodies[a]["orientation"].copy()
+                b = sb.body((gx + 0.5, y, gz), orientation=qa)
+                sb.box(b, (1.0, 0.5, 0.5))
+                plan.append(("weld", a, b, (gx, y, gz)))
+        else:  # capsules on slack distance limits
+            for k in range(links // 2):
+                b = sb.body((gx + rng.uniform(-2, 2), top - 2.0 - 1.5 * k, gz + rng.uniform(-2, 2)),
+                            orientation=rng.quaternion())
+                sb.capsule(b, 0.25, 0.8)
+                plan.append(("dist", post, b, None))
+        for k in range(3):  # free boxes
+            b = sb.body((gx + rng.uniform(-3, 3), 4.0 + 3.0 * k, gz + rng.uniform(-3, 3)), orientation=rng.quaternion())
+            sb.box(b, [2.0 * rng.uniform(0.2, 0.5) for _ in range(3)])
+    bodies, shapes, verts = sb.arrays()
+    for kind, a, b, anchor in plan:
+        if kind == "hinge":
+            recs += C.hinge_joint(bodies, a, b, anchor, (1.0, 0.0, 0.0), (-math.pi / 3, math.pi / 3))
+        elif kind == "ball":
+            recs.append(C.ball_socket(bodies, a, b, anchor))
+        elif kind == "weld":
+            recs += C.weld_joint(bodies, a, b, anchor)
+        else:
+            pa = bodies["position"][a].astype(np.float32)
+            pb = bodies["position"][b].astype(np.float32)
+            recs.append(C.distance_limit(bodies, a, b, pa, pb, (-1.0, 0.5)))
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    cfg["solver_iterations"] = 10
+    cfg["relaxation_iterations"] = 4
+    return bodies, shapes, verts, cfg, 1.0 / 60.0, C.as_array(recs)
+
+
 def two_bodies(shape_a, shape_b, pos_a, pos_b, q_a=(0, 0, 0, 1), q_b=(0, 0, 0, 1), static_a=False):
     """Tiny two-body scene for unit tests; shape_x = ("box", size) | ("sphere", r) | ("capsule", r, len)."""
     sb = SceneBuilder()
