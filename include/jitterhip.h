@@ -170,9 +170,16 @@ typedef struct jh_arbiter {
  *   FIXED_ANGLE (FixedAngle.cs): params BiasFactor, Softness, Q0[4]; state AccumulatedImpulse[3],
  *       Hemisphere.
  *   DISTANCE_LIMIT (DistanceLimit.cs): params LocalAnchor1[3], LocalAnchor2[3], BiasFactor, Softness,
- *       Distance, LimitMin, LimitMax; state AccumulatedImpulse. */
+ *       Distance, LimitMin, LimitMax; state AccumulatedImpulse.
+ *   TWIST_ANGLE (TwistAngle.cs): params B[3] (body-2 frame), Q0[4], Angle1, Angle2, FixedAngle, BiasFactor,
+ *       Softness; state AccumulatedImpulse, Hemisphere.
+ *   CONE_LIMIT (ConeLimit.cs): params LocalAxis1[3], LocalAxis2[3], BiasFactor, Softness, MinAngle,
+ *       MaxAngle; state AccumulatedImpulse, the last Jacobian J0[3].
+ *   ANGULAR_MOTOR (AngularMotor.cs), LINEAR_MOTOR (LinearMotor.cs): params LocalAxis1[3], LocalAxis2[3],
+ *       Velocity (TargetVelocity), MaxForce (MaximumForce / MaximumTorque); state AccumulatedImpulse. */
 enum { JH_CONSTRAINT_BALL_SOCKET = 0, JH_CONSTRAINT_HINGE_ANGLE = 1, JH_CONSTRAINT_FIXED_ANGLE = 2,
-       JH_CONSTRAINT_DISTANCE_LIMIT = 3 };
+       JH_CONSTRAINT_DISTANCE_LIMIT = 3, JH_CONSTRAINT_TWIST_ANGLE = 4, JH_CONSTRAINT_CONE_LIMIT = 5,
+       JH_CONSTRAINT_ANGULAR_MOTOR = 6, JH_CONSTRAINT_LINEAR_MOTOR = 7 };
 
 typedef struct jh_constraint {
     int32_t type;           /* JH_CONSTRAINT_*                                                  */

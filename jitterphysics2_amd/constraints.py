@@ -11,8 +11,9 @@ import math
 
 import numpy as np
 
-from ._abi import (CONSTRAINT_BALL_SOCKET, CONSTRAINT_DISTANCE_LIMIT, CONSTRAINT_DTYPE, CONSTRAINT_FIXED_ANGLE,
-                   CONSTRAINT_HINGE_ANGLE)
+from ._abi import (CONSTRAINT_ANGULAR_MOTOR, CONSTRAINT_BALL_SOCKET, CONSTRAINT_CONE_LIMIT, CONSTRAINT_DISTANCE_LIMIT,
+                   CONSTRAINT_DTYPE, CONSTRAINT_FIXED_ANGLE, CONSTRAINT_HINGE_ANGLE, CONSTRAINT_LINEAR_MOTOR,
+                   CONSTRAINT_TWIST_ANGLE)
 
 f32 = np.float32
 
@@ -37,6 +38,56 @@ def _rot_inv(v, q):
     n11 = qx * n2 - qz * n0
     n22 = qy * n0 - qx * n1
     return np.array([x + qw * n0 + n00, y + qw * n1 + n11, z + qw * n2 + n22], np.float32)
+
+
+def _rot(v, q):
+    """JVector.Transform(v, q) in FP32."""
+    x, y, z = (f32(c) for c in v)
+    qx, qy, qz, qw = (f32(c) for c in q)
+    two = f32(2.0)
+    n0 = two * (qy * z - qz * y)
+    n1 = two * (qz * x - qx * z)
+    n2 = two * (qx * y - qy * x)
+    n00 = qy * n2 - qz * n1
+    n11 = qz * n0 - qx * n2
+    n22 = qx * n1 - qy * n0
+    return np.array([x + qw * n0 + n00, y + qw * n1 + n11, z + qw * n2 + n22], np.float32)
+
+
+def _normalize(v):
+    a = np.asarray(v, np.float32)
+    return a * (f32(1.0) / np.sqrt((a * a).sum(dtype=np.float32)))
+
+
+def _create_orthonormal(v):
+    """MathHelper.CreateOrthonormal (MathHelper.cs:204-255) in FP32."""
+    v = np.asarray(v, np.float32)
+    ax, ay, az = np.abs(v)
+    if ax <= ay and ax <= az:
+        y, z = v[2], -v[1]
+        inv = f32(1.0) / np.sqrt(y * y + z * z)
+        return np.array([0.0, y * inv, z * inv], np.float32)
+    if ay <= az:
+        x, z = -v[2], v[0]
+        inv = f32(1.0) / np.sqrt(x * x + z * z)
+        return np.array([x * inv, 0.0, z * inv], np.float32)
+    x, y = v[1], -v[0]
+    inv = f32(1.0) / np.sqrt(x * x + y * y)
+    return np.array([x * inv, y * inv, 0.0], np.float32)
+
+
+def _from_to(a, b):
+    """JQuaternion.CreateFromToRotation (JQuaternion.cs:84-101) in FP32, (x, y, z, w)."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    d = a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+    if d < -f32(1.0) + f32(1e-6):
+        ax = _create_orthonormal(a)
+        return np.array([ax[0], ax[1], ax[2], 0.0], np.float32)
+    s = np.sqrt((f32(1.0) + d) * f32(2.0))
+    inv = f32(1.0) / s
+    c = np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], np.float32)
+    return np.array([c[0] * inv, c[1] * inv, c[2] * inv, s * f32(0.5)], np.float32)
 
 
 def _qmul(a, b):
@@ -122,10 +173,77 @@ def distance_limit(bodies, b1, b2, anchor1, anchor2, limit=(1e-6, -1e-6)):
     return c
 
 
-def hinge_joint(bodies, b1, b2, anchor, axis, limit=(-math.pi, math.pi)):
-    """HingeJoint (Dynamics/Joints/HingeJoint.cs:58-63): HingeAngle about the axis, then BallSocket at
-    the anchor (the order the joint creates them, which is the solve order)."""
-    return [hinge_angle(bodies, b1, b2, axis, limit), ball_socket(bodies, b1, b2, anchor)]
+ANGULAR_FIXED = (1e-6, -1e-6)  # AngularLimit.Fixed (Limit.cs:38-39)
+
+
+def twist_angle(bodies, b1, b2, axis1, axis2, limit=ANGULAR_FIXED):
+    """TwistAngle.Initialize(axis1, axis2, limit) (TwistAngle.cs:76-115)."""
+    c = _record(CONSTRAINT_TWIST_ANGLE, b1, b2)
+    p = c["params"]
+    q1, q2 = bodies["orientation"][b1], bodies["orientation"][b2]
+    u1 = _rot_inv(_normalize(axis1), q1)
+    B = _rot_inv(_normalize(axis2), q2)
+    q_rel = _qmul(_conj(q2), q1)
+    u1_in_b2 = _rot(u1, q_rel)
+    p[0:3] = B
+    p[3:7] = _qmul(_from_to(u1_in_b2, B), q_rel)
+    lo, hi = f32(limit[0]), f32(limit[1])
+    p[7], p[8] = lo, hi
+    p[9] = f32(0.5) * lo + f32(0.5) * hi
+    p[10] = DEFAULT_ANGULAR_BIAS
+    p[11] = DEFAULT_ANGULAR_SOFTNESS
+    return c
+
+
+def cone_limit(bodies, b1, b2, axis1, axis2=None, limit=(0.0, math.pi / 4)):
+    """ConeLimit.Initialize(axisBody1, axisBody2, limit) (ConeLimit.cs:82-107); with one axis both bodies
+    use it (rest angle 0)."""
+    c = _record(CONSTRAINT_CONE_LIMIT, b1, b2)
+    p = c["params"]
+    p[0:3] = _rot_inv(_normalize(axis1), bodies["orientation"][b1])
+    p[3:6] = _rot_inv(_normalize(axis1 if axis2 is None else axis2), bodies["orientation"][b2])
+    p[6] = DEFAULT_ANGULAR_BIAS
+    p[7] = DEFAULT_ANGULAR_SOFTNESS
+    p[8], p[9] = f32(limit[0]), f32(limit[1])
+    return c
+
+
+def _motor(kind, bodies, b1, b2, axis1, axis2, velocity, max_force):
+    c = _record(kind, b1, b2)
+    p = c["params"]
+    p[0:3] = _rot_inv(_normalize(axis1), bodies["orientation"][b1])
+    p[3:6] = _rot_inv(_normalize(axis2), bodies["orientation"][b2])
+    p[6], p[7] = f32(velocity), f32(max_force)
+    return c
+
+
+def angular_motor(bodies, b1, b2, axis1, axis2=None, velocity=0.0, max_torque=0.0):
+    """AngularMotor.Initialize(axis1, axis2) (AngularMotor.cs:67-83) with TargetVelocity and MaximumTorque;
+    HingeJoint passes the hinge axis for both (HingeJoint.cs:68-69)."""
+    return _motor(CONSTRAINT_ANGULAR_MOTOR, bodies, b1, b2, axis1, axis1 if axis2 is None else axis2, velocity,
+                  max_torque)
+
+
+def linear_motor(bodies, b1, b2, axis1, axis2=None, velocity=0.0, max_force=0.0):
+    """LinearMotor.Initialize(axis1, axis2) (LinearMotor.cs:97-113) with TargetVelocity and MaximumForce."""
+    return _motor(CONSTRAINT_LINEAR_MOTOR, bodies, b1, b2, axis1, axis1 if axis2 is None else axis2, velocity,
+                  max_force)
+
+
+def ragdoll_joint(bodies, b1, b2, anchor, axis, cone=math.pi / 4, twist=(-math.pi / 6, math.pi / 6)):
+    """A ragdoll shoulder: BallSocket at the anchor, ConeLimit of the bone axis, TwistAngle about it."""
+    return [ball_socket(bodies, b1, b2, anchor), cone_limit(bodies, b1, b2, axis, limit=(0.0, cone)),
+            twist_angle(bodies, b1, b2, axis, axis, twist)]
+
+
+def hinge_joint(bodies, b1, b2, anchor, axis, limit=(-math.pi, math.pi), motor=None):
+    """HingeJoint (Dynamics/Joints/HingeJoint.cs:58-70): HingeAngle about the axis, then BallSocket at
+    the anchor, then (motor = (velocity, max torque)) an AngularMotor about the axis — the order the
+    joint creates them, which is the solve order."""
+    out = [hinge_angle(bodies, b1, b2, axis, limit), ball_socket(bodies, b1, b2, anchor)]
+    if motor is not None:
+        out.append(angular_motor(bodies, b1, b2, axis, axis, motor[0], motor[1]))
+    return out
 
 
 def weld_joint(bodies, b1, b2, anchor):

@@ -1,5 +1,5 @@
-// Constraints on the device: BallSocket, HingeAngle, FixedAngle, DistanceLimit
-// (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
+// Constraints on the device: BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit,
+// AngularMotor, LinearMotor (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
 // pieces they use, FP32 in the reference's operation order.  One thread solves one constraint; the
 // solver visits constraints in batches with disjoint dynamic bodies.
 #pragma once
@@ -258,6 +258,48 @@ __device__ __forceinline__ mat3 calc_jacobian(quat left, quat right, vec3& error
     return mmul(mscale(derivative, -0.5f), mat_from_quat(left));
 }
 
+// QMatrix.ProjectMultiplyLeftRight (Dynamics/Constraints/Internal/QMatrix.cs:109-124)
+__device__ __forceinline__ mat3 project_multiply_left_right(quat l, quat r) {
+    mat3 m;
+    m.m11 = -l.x * r.x + l.w * r.w + l.z * r.z + l.y * r.y;
+    m.m12 = -l.x * r.y + l.w * r.z - l.z * r.w - l.y * r.x;
+    m.m13 = -l.x * r.z - l.w * r.y - l.z * r.x + l.y * r.w;
+    m.m21 = -l.y * r.x + l.z * r.w - l.w * r.z - l.x * r.y;
+    m.m22 = -l.y * r.y + l.z * r.z + l.w * r.w + l.x * r.x;
+    m.m23 = -l.y * r.z - l.z * r.y + l.w * r.x - l.x * r.w;
+    m.m31 = -l.z * r.x - l.y * r.w - l.x * r.z + l.w * r.y;
+    m.m32 = -l.z * r.y - l.y * r.z + l.x * r.w - l.w * r.x;
+    m.m33 = -l.z * r.z + l.y * r.y + l.x * r.x + l.w * r.w;
+    return m;
+}
+
+// CalculateTwist (AngularConstraintMath.cs:31-61)
+__device__ __forceinline__ float calc_twist(quat left, quat right, vec3 axis, vec3& jacobian, int& hemisphere) {
+    mat3 m = mscale(project_multiply_left_right(left, right), -float(1.0 / 2.0));
+    quat q = qmul(left, right);
+    jacobian = tmat_mul(axis, m);
+    float projection = dot(axis, vec3{q.x, q.y, q.z});
+    hemisphere = q.w < 0.0f ? -1 : 1;
+    if (q.w < 0.0f) {
+        q = qscale(q, -1.0f);
+        projection *= -1.0f;
+        jacobian = jacobian * -1.0f;
+    }
+    float dsq = len_sq(jacobian);
+    if (dsq > 0.0f) jacobian = jacobian * (1.0f / sqrtf(dsq));
+    else jacobian = -rot(axis, right);
+    vec3 perpendicular = vec3{q.x, q.y, q.z} - axis * projection;
+    float cosine = sqrtf(q.w * q.w + len_sq(perpendicular));
+    return 2.0f * stable::atan2(projection, cosine);
+}
+
+// Math.Clamp(value, min, max) for min <= max
+__device__ __forceinline__ float clamp_cs(float v, float lo, float hi) {
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
 // std::min / std::max forms (the oracle's)
 __device__ __forceinline__ float cmin(float a, float b) { return b < a ? b : a; }
 __device__ __forceinline__ float cmax(float a, float b) { return a < b ? b : a; }
@@ -284,7 +326,7 @@ struct CBody {
 __device__ __forceinline__ vec3 p3(const float* p) { return {p[0], p[1], p[2]}; }
 __device__ __forceinline__ void set3(float* p, vec3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 
-// PrepareForIterationBallSocket (BallSocket.cs:123-161)
+// PrepareForIterationBallSocket (BallSocket.cs:140-183)
 __device__ __forceinline__ void prepare_ball_socket(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
                                                     float inv_sub) {
     k.r1 = rot(p3(c.params), A.q);
@@ -311,7 +353,7 @@ __device__ __forceinline__ void prepare_ball_socket(jh_constraint& c, Work& k, C
         B.w = B.w + sym_mul(mat_mul(acc, cr2), B.I);
     }
 }
-// IterateBallSocket (BallSocket.cs:194-224)
+// IterateBallSocket (BallSocket.cs:223-252)
 __device__ __forceinline__ void iterate_ball_socket(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
     mat3 cr1 = cross_matrix(k.r1), cr2 = cross_matrix(k.r2);
     vec3 acc = p3(c.state);
@@ -329,7 +371,7 @@ __device__ __forceinline__ void iterate_ball_socket(jh_constraint& c, Work& k, C
     }
 }
 
-// PrepareForIterationHingeAngle (HingeAngle.cs:140-226)
+// PrepareForIterationHingeAngle (HingeAngle.cs:133-220)
 __device__ __forceinline__ void prepare_hinge_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
                                                     float inv_sub) {
     const float* p = c.params;
@@ -384,7 +426,7 @@ __device__ __forceinline__ void prepare_hinge_angle(jh_constraint& c, Work& k, C
     if (A.dyn) A.w = A.w + sym_mul(mat_mul(acc, k.jac), A.I);
     if (B.dyn) B.w = B.w - sym_mul(mat_mul(acc, k.jac), B.I);
 }
-// IterateHingeAngle (HingeAngle.cs:282-334)
+// IterateHingeAngle (HingeAngle.cs:313-364)
 __device__ __forceinline__ void iterate_hinge_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
     const float* p = c.params;
     vec3 jv = tmat_mul(A.w - B.w, k.jac);
@@ -416,7 +458,7 @@ __device__ __forceinline__ void iterate_hinge_angle(jh_constraint& c, Work& k, C
     if (B.dyn) B.w = B.w - sym_mul(mat_mul(lambda, k.jac), B.I);
 }
 
-// PrepareForIterationFixedAngle (FixedAngle.cs:104-141)
+// PrepareForIterationFixedAngle (FixedAngle.cs:83-122)
 __device__ __forceinline__ void prepare_fixed_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
                                                     float inv_sub) {
     const float* p = c.params;
@@ -438,7 +480,7 @@ __device__ __forceinline__ void prepare_fixed_angle(jh_constraint& c, Work& k, C
     if (A.dyn) A.w = A.w + sym_mul(tmat_mul(acc, k.jac), A.I);
     if (B.dyn) B.w = B.w - sym_mul(tmat_mul(acc, k.jac), B.I);
 }
-// IterateFixedAngle (FixedAngle.cs:172-193)
+// IterateFixedAngle (FixedAngle.cs:161-181)
 __device__ __forceinline__ void iterate_fixed_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
     vec3 jv = mat_mul(A.w - B.w, k.jac);
     vec3 acc = p3(c.state);
@@ -449,7 +491,7 @@ __device__ __forceinline__ void iterate_fixed_angle(jh_constraint& c, Work& k, C
     if (B.dyn) B.w = B.w - sym_mul(tmat_mul(lambda, k.jac), B.I);
 }
 
-// PrepareForIterationDistanceLimit (DistanceLimit.cs:220-309)
+// PrepareForIterationDistanceLimit (DistanceLimit.cs:210-291)
 __device__ __forceinline__ void prepare_distance_limit(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
                                                        float inv_sub) {
     const float* p = c.params;
@@ -501,7 +543,7 @@ __device__ __forceinline__ void prepare_distance_limit(jh_constraint& c, Work& k
         B.w = B.w + sym_mul(acc * k.j3, B.I);
     }
 }
-// IterateDistanceLimit (DistanceLimit.cs:350-399)
+// IterateDistanceLimit (DistanceLimit.cs:330-376)
 __device__ __forceinline__ void iterate_distance_limit(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
     if (k.clamp == 0) return;
     float jv = dot(A.v, k.j0) + dot(A.w, k.j1) + dot(B.v, k.j2) + dot(B.w, k.j3);
@@ -523,12 +565,175 @@ __device__ __forceinline__ void iterate_distance_limit(jh_constraint& c, Work& k
     }
 }
 
-__device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt, float inv_sub) {
+// PrepareForIterationTwistAngle (TwistAngle.cs:152-210)
+__device__ __forceinline__ void prepare_twist_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                    float inv_sub) {
+    const float* p = c.params;
+    quat Q0{p[3], p[4], p[5], p[6]};
+    int hemisphere;
+    float error = calc_twist(qmul(Q0, qconj(A.q)), B.q, p3(p), k.j0, hemisphere);
+    float lower = p[7], upper = p[8], target = p[9];
+    int old_h = int(c.state[1]);
+    if (old_h != 0 && old_h != hemisphere) c.state[0] = 0.0f;
+    c.state[1] = float(hemisphere);
+    k.clamp = 0;
+    bool full_range = lower <= -stable::Pi && upper >= stable::Pi;
+    if (lower >= upper) {
+        k.clamp = 3;
+        error = shortest_angle(error - target, stable::Pi);
+    } else if (!full_range && error >= upper) {
+        k.clamp = 1;
+        error -= upper;
+    } else if (!full_range && error <= lower) {
+        k.clamp = 2;
+        error -= lower;
+    } else {
+        c.state[0] = 0.0f;
+        return;
+    }
+    if (k.clamp == 1) c.state[0] = cmin(c.state[0], 0.0f);
+    else if (k.clamp == 2) c.state[0] = cmax(c.state[0], 0.0f);
+    k.em_s = dot(sym_mul(k.j0, sym_add(A.I, B.I)), k.j0);
+    k.em_s += p[11] * inv_sub;
+    k.em_s = k.em_s > 0.0f ? 1.0f / k.em_s : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    k.bias_s = error * p[10] * inv_dt;
+    float acc = c.state[0];
+    if (A.dyn) A.w = A.w + sym_mul(acc * k.j0, A.I);
+    if (B.dyn) B.w = B.w - sym_mul(acc * k.j0, B.I);
+}
+// IterateTwistAngle (TwistAngle.cs:292-328)
+__device__ __forceinline__ void iterate_twist_angle(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    if (k.clamp == 0) return;
+    float jv = dot(A.w - B.w, k.j0);
+    float soft = c.state[0] * (c.params[11] * inv_sub);
+    float lambda = -k.em_s * (jv + k.bias_s + soft);
+    float orig = c.state[0];
+    float acc = orig + lambda;
+    if (k.clamp == 1) acc = cmin(acc, 0.0f);
+    else if (k.clamp == 2) acc = cmax(acc, 0.0f);
+    c.state[0] = acc;
+    lambda = acc - orig;
+    if (A.dyn) A.w = A.w + sym_mul(lambda * k.j0, A.I);
+    if (B.dyn) B.w = B.w - sym_mul(lambda * k.j0, B.I);
+}
+
+// PrepareForIterationConeLimit (ConeLimit.cs:256-347)
+__device__ __forceinline__ void prepare_cone_limit(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                   float inv_sub) {
+    const float* p = c.params;
+    vec3 a1 = rot(p3(p), A.q);
+    vec3 a2 = rot(p3(p + 3), B.q);
+    vec3 previous = p3(c.state + 1);
+    vec3 j0 = cross(a2, a1);
+    k.clamp = 0;
+    float error = dot(a1, a2);
+    float lower = -p[8], upper = -p[9];
+    float sine = sqrtf(len_sq(j0));
+    if (sine > 0.0f) {
+        j0 = j0 * (1.0f / sine);
+    } else {
+        vec3 axis = create_orthonormal(a1);
+        vec3 other = cross(a1, axis);
+        sym3 ii = sym_add(A.I, B.I);
+        j0 = dot(sym_mul(axis, ii), axis) >= dot(sym_mul(other, ii), other) ? axis : other;
+    }
+    vec3 j1 = -j0;
+    set3(c.state + 1, j0);
+    if (dot(previous, j0) < 0.0f) c.state[0] = 0.0f;
+    error = -stable::atan2(sine, error);
+    if (lower == upper) {
+        k.clamp = 3;
+        error -= upper;
+    } else if (error <= upper && p[9] < stable::Pi) {
+        k.clamp = 1;
+        error -= upper;
+    } else if (error >= lower && p[8] > 0.0f) {
+        k.clamp = 2;
+        error -= lower;
+    } else {
+        c.state[0] = 0.0f;
+        return;
+    }
+    k.j0 = j0;
+    k.j1 = j1;
+    if (k.clamp == 1) c.state[0] = cmin(c.state[0], 0.0f);
+    else if (k.clamp == 2) c.state[0] = cmax(c.state[0], 0.0f);
+    k.em_s = dot(sym_mul(j0, A.I), j0) + dot(sym_mul(j1, B.I), j1);
+    k.em_s += p[7] * inv_sub;
+    k.em_s = k.em_s > 0.0f ? 1.0f / k.em_s : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    k.bias_s = -error * p[6] * inv_dt;
+    float acc = c.state[0];
+    if (A.dyn) A.w = A.w + sym_mul(acc * j0, A.I);
+    if (B.dyn) B.w = B.w + sym_mul(acc * j1, B.I);
+}
+// IterateConeLimit (ConeLimit.cs:386-427)
+__device__ __forceinline__ void iterate_cone_limit(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    if (k.clamp == 0) return;
+    float jv = dot(A.w, k.j0) + dot(B.w, k.j1);
+    float soft = c.state[0] * c.params[7] * inv_sub;
+    float lambda = -k.em_s * (jv + k.bias_s + soft);
+    float old = c.state[0];
+    float acc = old + lambda;
+    if (k.clamp == 1) acc = cmin(acc, 0.0f);
+    else if (k.clamp == 2) acc = cmax(acc, 0.0f);
+    c.state[0] = acc;
+    lambda = acc - old;
+    if (A.dyn) A.w = A.w + sym_mul(lambda * k.j0, A.I);
+    if (B.dyn) B.w = B.w + sym_mul(lambda * k.j1, B.I);
+}
+
+// PrepareForIterationAngularMotor (AngularMotor.cs:150-175) / PrepareForIterationLinearMotor
+// (LinearMotor.cs:152-176); MaxLambda kept in bias_s
+__device__ __forceinline__ void prepare_motor(jh_constraint& c, Work& k, CBody& A, CBody& B, float sub_dt,
+                                              bool linear) {
+    vec3 j1 = rot(p3(c.params), A.q);
+    vec3 j2 = rot(p3(c.params + 3), B.q);
+    if (linear) k.em_s = dot(cmul(j1, A.imv), j1) + dot(cmul(j2, B.imv), j2);
+    else k.em_s = dot(sym_mul(j1, A.I), j1) + dot(sym_mul(j2, B.I), j2);
+    k.em_s = k.em_s > 0.0f ? 1.0f / k.em_s : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    k.bias_s = sub_dt * c.params[7];
+    float acc = c.state[0];
+    if (linear) {
+        if (A.dyn) A.v = A.v - cmul(j1 * acc, A.imv);
+        if (B.dyn) B.v = B.v + cmul(j2 * acc, B.imv);
+    } else {
+        if (A.dyn) A.w = A.w - sym_mul(j1 * acc, A.I);
+        if (B.dyn) B.w = B.w + sym_mul(j2 * acc, B.I);
+    }
+}
+// IterateAngularMotor (AngularMotor.cs:177-207) / IterateLinearMotor (LinearMotor.cs:203-232)
+__device__ __forceinline__ void iterate_motor(jh_constraint& c, Work& k, CBody& A, CBody& B, bool linear) {
+    vec3 j1 = rot(p3(c.params), A.q);
+    vec3 j2 = rot(p3(c.params + 3), B.q);
+    float jv = linear ? dot(-j1, A.v) + dot(j2, B.v) : dot(-j1, A.w) + dot(j2, B.w);
+    float lambda = -(jv - c.params[6]) * k.em_s;
+    float old = c.state[0];
+    float acc = clamp_cs(old + lambda, -k.bias_s, k.bias_s);
+    c.state[0] = acc;
+    lambda = acc - old;
+    if (linear) {
+        if (A.dyn) A.v = A.v - cmul(j1 * lambda, A.imv);
+        if (B.dyn) B.v = B.v + cmul(j2 * lambda, B.imv);
+    } else {
+        if (A.dyn) A.w = A.w - sym_mul(j1 * lambda, A.I);
+        if (B.dyn) B.w = B.w + sym_mul(j2 * lambda, B.I);
+    }
+}
+
+__device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt, float inv_sub,
+                                        float sub_dt) {
     switch (c.type) {
         case JH_CONSTRAINT_BALL_SOCKET: prepare_ball_socket(c, k, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_HINGE_ANGLE: prepare_hinge_angle(c, k, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_FIXED_ANGLE: prepare_fixed_angle(c, k, A, B, inv_dt, inv_sub); break;
-        default: prepare_distance_limit(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_DISTANCE_LIMIT: prepare_distance_limit(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_TWIST_ANGLE: prepare_twist_angle(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_CONE_LIMIT: prepare_cone_limit(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_MOTOR: prepare_motor(c, k, A, B, sub_dt, false); break;
+        default: prepare_motor(c, k, A, B, sub_dt, true); break;
     }
 }
 __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
@@ -536,7 +741,11 @@ __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBo
         case JH_CONSTRAINT_BALL_SOCKET: iterate_ball_socket(c, k, A, B, inv_sub); break;
         case JH_CONSTRAINT_HINGE_ANGLE: iterate_hinge_angle(c, k, A, B, inv_sub); break;
         case JH_CONSTRAINT_FIXED_ANGLE: iterate_fixed_angle(c, k, A, B, inv_sub); break;
-        default: iterate_distance_limit(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_DISTANCE_LIMIT: iterate_distance_limit(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_TWIST_ANGLE: iterate_twist_angle(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_CONE_LIMIT: iterate_cone_limit(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_MOTOR: iterate_motor(c, k, A, B, false); break;
+        default: iterate_motor(c, k, A, B, true); break;
     }
 }
 

@@ -1450,7 +1450,7 @@ __device__ __forceinline__ void solve_constraint_batches(const SolveArgs& P, int
             con::CBody A, Bb;
             con_load(B, c.body1, A);
             con_load(B, c.body2, Bb);
-            if (phase == 0) con::prepare(c, w, A, Bb, P.ts.inv_dt, P.ts.inv_substep_dt);
+            if (phase == 0) con::prepare(c, w, A, Bb, P.ts.inv_dt, P.ts.inv_substep_dt, P.ts.substep_dt);
             else con::iterate(c, w, A, Bb, P.ts.inv_substep_dt);
             for (int q = 0; q < 8; ++q) C.c[i].state[q] = c.state[q];
             if (phase == 0) C.work[i] = w;

@@ -509,8 +509,10 @@ def bullets(n_bodies: int = 512, seed: int = 0x4A2D0006, speed=(15.0, 60.0), spe
 def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
     """Jointed bodies over a static floor, for the constraint solver: per group one of (cycling)
     a hinge chain of boxes hanging from a static post (HingeJoint, limits +-60 degrees), a rope of
-    spheres (BallSocket chain), welded box pairs (WeldJoint) dropped on the floor, and capsules tied to a
-    post by distance limits with a slack range; free boxes fall between them.  Returns (bodies, shapes,
+    spheres (BallSocket chain), welded box pairs (WeldJoint) dropped on the floor, capsules tied to a
+    post by distance limits with a slack range, a ragdoll-like limb chain (BallSocket + ConeLimit +
+    TwistAngle), and motor-driven wheels on hinges (AngularMotor) with a block pulled by a LinearMotor;
+    free boxes fall between them.  Returns (bodies, shapes,
     verts, cfg, dt, constraints).  dt = 1/60, 4 substeps, 10 iterations."""
     from . import constraints as C
 
@@ -522,9 +524,9 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
     side = int(math.ceil(math.sqrt(n_groups)))
     for g in range(n_groups):
         gx, gz = (g % side - side / 2.0) * 8.0, (g // side - side / 2.0) * 8.0
-        kind = g % 4
+        kind = g % 6
         top = 14.0
-        if kind in (0, 1, 3):
+        if kind in (0, 1, 3, 4, 5):
             post = _static_box(sb, (gx, top + 0.5, gz), (0.4, 1.0, 0.4))
         if kind == 0:  # hinge chain
             prev, anchor_y = post, top
@@ -550,6 +552,22 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
                 b = sb.body((gx + 0.5, y, gz), orientation=qa)
                 sb.box(b, (1.0, 0.5, 0.5))
                 plan.append(("weld", a, b, (gx, y, gz)))
+        elif kind == 4:  # limb chain: ball sockets with cone and twist limits
+            prev, anchor_y = post, top
+            for k in range(links):
+                b = sb.body((gx, anchor_y - 0.6, gz), velocity=(rng.normal(), rng.normal(), rng.normal()),
+                            angular_velocity=(rng.normal(), 3.0 * rng.normal(), rng.normal()))
+                sb.capsule(b, 0.2, 0.8)
+                plan.append(("limb", prev, b, (gx, anchor_y, gz)))
+                prev, anchor_y = b, anchor_y - 1.2
+        elif kind == 5:  # motor-driven wheels and a block on a linear motor
+            for k in range(links // 2):
+                b = sb.body((gx + 1.2 * k - 2.0, top - 1.0, gz + 0.6))
+                sb.box(b, (1.0, 1.0, 0.2))
+                plan.append(("wheel", post, b, (gx + 1.2 * k - 2.0, top - 1.0, gz + 0.6)))
+            b = sb.body((gx + 2.5, top - 3.0, gz))
+            sb.box(b, (0.5, 0.5, 0.5))
+            plan.append(("slider", post, b, None))
         else:  # capsules on slack distance limits
             for k in range(links // 2):
                 b = sb.body((gx + rng.uniform(-2, 2), top - 2.0 - 1.5 * k, gz + rng.uniform(-2, 2)),
@@ -567,6 +585,12 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
             recs.append(C.ball_socket(bodies, a, b, anchor))
         elif kind == "weld":
             recs += C.weld_joint(bodies, a, b, anchor)
+        elif kind == "limb":
+            recs += C.ragdoll_joint(bodies, a, b, anchor, (0.0, -1.0, 0.0), math.pi / 5, (-math.pi / 8, math.pi / 8))
+        elif kind == "wheel":
+            recs += C.hinge_joint(bodies, a, b, anchor, (0.0, 0.0, 1.0), motor=(4.0, 5.0))
+        elif kind == "slider":
+            recs.append(C.linear_motor(bodies, a, b, (0.0, 1.0, 0.0), (0.0, 1.0, 0.0), 1.0, 20.0))
         else:
             pa = bodies["position"][a].astype(np.float32)
             pb = bodies["position"][b].astype(np.float32)

@@ -1,5 +1,6 @@
 // ORACLE — test infrastructure only (see orc_math.hpp).  Constraints solved with the contacts —
-// BallSocket, HingeAngle, FixedAngle, DistanceLimit (Dynamics/Constraints/*.cs) — and the JMatrix,
+// BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit, AngularMotor, LinearMotor
+// (Dynamics/Constraints/*.cs) — and the JMatrix,
 // JSymmetricMatrix, MathHelper and AngularConstraintMath pieces they use, restated in the reference's
 // operation order and templated on the precision R.
 #pragma once
@@ -255,6 +256,48 @@ template <class R> inline M3<R> calc_jacobian(Q4<R> left, Q4<R> right, V3<R>& er
     return mmul(mscale(derivative, -R(0.5)), mat_from_quat(left));
 }
 
+// QMatrix.ProjectMultiplyLeftRight (Dynamics/Constraints/Internal/QMatrix.cs:109-124)
+template <class R> inline M3<R> project_multiply_left_right(Q4<R> l, Q4<R> r) {
+    M3<R> m;
+    m.m11 = -l.x * r.x + l.w * r.w + l.z * r.z + l.y * r.y;
+    m.m12 = -l.x * r.y + l.w * r.z - l.z * r.w - l.y * r.x;
+    m.m13 = -l.x * r.z - l.w * r.y - l.z * r.x + l.y * r.w;
+    m.m21 = -l.y * r.x + l.z * r.w - l.w * r.z - l.x * r.y;
+    m.m22 = -l.y * r.y + l.z * r.z + l.w * r.w + l.x * r.x;
+    m.m23 = -l.y * r.z - l.z * r.y + l.w * r.x - l.x * r.w;
+    m.m31 = -l.z * r.x - l.y * r.w - l.x * r.z + l.w * r.y;
+    m.m32 = -l.z * r.y - l.y * r.z + l.x * r.w - l.w * r.x;
+    m.m33 = -l.z * r.z + l.y * r.y + l.x * r.x + l.w * r.w;
+    return m;
+}
+
+// CalculateTwist (AngularConstraintMath.cs:31-61): twist angle about `axis` and its unit Jacobian.
+template <class R> inline R calc_twist(Q4<R> left, Q4<R> right, V3<R> axis, V3<R>& jacobian, int& hemisphere) {
+    M3<R> m = mscale(project_multiply_left_right(left, right), -R(1.0 / 2.0));
+    Q4<R> q = qmul(left, right);
+    jacobian = tmat_mul(axis, m);
+    R projection = dot(axis, V3<R>{q.x, q.y, q.z});
+    hemisphere = q.w < R(0) ? -1 : 1;
+    if (q.w < R(0)) {
+        q = qscale(q, R(-1));
+        projection *= -R(1);
+        jacobian = jacobian * R(-1);
+    }
+    R dsq = len_sq(jacobian);
+    if (dsq > R(0)) jacobian = jacobian * (R(1) / std::sqrt(dsq));
+    else jacobian = -rotate(axis, right);
+    V3<R> perpendicular = V3<R>{q.x, q.y, q.z} - axis * projection;
+    R cosine = std::sqrt(q.w * q.w + len_sq(perpendicular));
+    return R(2) * StableMath<R>::atan2(projection, cosine);
+}
+
+// Math.Clamp(value, min, max) for min <= max
+template <class R> inline R clamp_cs(R v, R lo, R hi) {
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
 // ---- Constraints ----------------------------------------------------------------------------
 // One constraint: kind, bodies, the Data fields of the reference's constraint struct that persist
 // (Initialize / properties: params; accumulated impulse and hemisphere: state) and the fields its
@@ -285,7 +328,7 @@ template <class R> inline void setv(R* p, V3<R> v) { p[0] = v.x; p[1] = v.y; p[2
 
 // BallSocket (BallSocket.cs): params LocalAnchor1 (0-2), LocalAnchor2 (3-5), BiasFactor (6), Softness (7);
 // state AccumulatedImpulse (0-2).
-// PrepareForIterationBallSocket (:123-161)
+// PrepareForIterationBallSocket (:140-183)
 template <class R> void prepare_ball_socket(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                             R inv_sub) {
     c.r1 = rotate(vecp(c.p), A.q);
@@ -312,7 +355,7 @@ template <class R> void prepare_ball_socket(Constraint<R>& c, ConstraintBody<R>&
         *B.angvel = *B.angvel + sym_mul(mat_mul(acc, cr2), B.iw);
     }
 }
-// IterateBallSocket (:194-224)
+// IterateBallSocket (:223-252)
 template <class R> void iterate_ball_socket(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
     M3<R> cr1 = cross_matrix(c.r1), cr2 = cross_matrix(c.r2);
     V3<R> acc = vecp(c.s);
@@ -333,7 +376,7 @@ template <class R> void iterate_ball_socket(Constraint<R>& c, ConstraintBody<R>&
 // HingeAngle (HingeAngle.cs): params MinAngle (0), MaxAngle (1), FixedAngle (2), BiasFactor (3),
 // LimitBias (4), LimitSoftness (5), Softness (6), Axis (7-9), Q0 (10-13, x y z w); state
 // AccumulatedImpulse (0-2), Hemisphere (3).
-// PrepareForIterationHingeAngle (:140-226)
+// PrepareForIterationHingeAngle (:133-220)
 template <class R> void prepare_hinge_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                             R inv_sub) {
     const R kPi = R(3.14159265358979323846);
@@ -390,7 +433,7 @@ template <class R> void prepare_hinge_angle(Constraint<R>& c, ConstraintBody<R>&
     if (A.dynamic) *A.angvel = *A.angvel + sym_mul(mat_mul(acc, c.jac), A.iw);
     if (B.dynamic) *B.angvel = *B.angvel - sym_mul(mat_mul(acc, c.jac), B.iw);
 }
-// IterateHingeAngle (:282-334)
+// IterateHingeAngle (:313-364)
 template <class R> void iterate_hinge_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
     V3<R> jv = tmat_mul(*A.angvel - *B.angvel, c.jac);
     V3<R> acc = vecp(c.s);
@@ -423,7 +466,7 @@ template <class R> void iterate_hinge_angle(Constraint<R>& c, ConstraintBody<R>&
 
 // FixedAngle (FixedAngle.cs): params BiasFactor (0), Softness (1), Q0 (2-5, x y z w); state
 // AccumulatedImpulse (0-2), Hemisphere (3).
-// PrepareForIterationFixedAngle (:104-141)
+// PrepareForIterationFixedAngle (:83-122)
 template <class R> void prepare_fixed_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                             R inv_sub) {
     Q4<R> Q0{c.p[2], c.p[3], c.p[4], c.p[5]};
@@ -444,7 +487,7 @@ template <class R> void prepare_fixed_angle(Constraint<R>& c, ConstraintBody<R>&
     if (A.dynamic) *A.angvel = *A.angvel + sym_mul(tmat_mul(acc, c.jac), A.iw);
     if (B.dynamic) *B.angvel = *B.angvel - sym_mul(tmat_mul(acc, c.jac), B.iw);
 }
-// IterateFixedAngle (:172-193)
+// IterateFixedAngle (:161-181)
 template <class R> void iterate_fixed_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
     V3<R> jv = mat_mul(*A.angvel - *B.angvel, c.jac);
     V3<R> acc = vecp(c.s);
@@ -457,7 +500,7 @@ template <class R> void iterate_fixed_angle(Constraint<R>& c, ConstraintBody<R>&
 
 // DistanceLimit (DistanceLimit.cs): params LocalAnchor1 (0-2), LocalAnchor2 (3-5), BiasFactor (6),
 // Softness (7), Distance (8), LimitMin (9), LimitMax (10); state AccumulatedImpulse (0).
-// PrepareForIterationDistanceLimit (:220-309)
+// PrepareForIterationDistanceLimit (:210-291)
 template <class R> void prepare_distance_limit(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                                R inv_sub) {
     V3<R> r1 = rotate(vecp(c.p), A.q);
@@ -508,7 +551,7 @@ template <class R> void prepare_distance_limit(Constraint<R>& c, ConstraintBody<
         *B.angvel = *B.angvel + sym_mul(acc * c.j3, B.iw);
     }
 }
-// IterateDistanceLimit (:350-399)
+// IterateDistanceLimit (:330-376)
 template <class R> void iterate_distance_limit(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
                                                R inv_sub) {
     if (c.clamp == 0) return;
@@ -531,13 +574,181 @@ template <class R> void iterate_distance_limit(Constraint<R>& c, ConstraintBody<
     }
 }
 
-template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+// TwistAngle (TwistAngle.cs): params B (0-2, body-2 frame), Q0 (3-6), Angle1 (7), Angle2 (8),
+// FixedAngle (9), BiasFactor (10), Softness (11); state AccumulatedImpulse (0), Hemisphere (1).
+// PrepareForIterationTwistAngle (:152-210)
+template <class R> void prepare_twist_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                            R inv_sub) {
+    const R kPi = R(3.14159265358979323846);
+    Q4<R> Q0{c.p[3], c.p[4], c.p[5], c.p[6]};
+    int hemisphere;
+    R error = calc_twist(qmul(Q0, qconj(A.q)), B.q, vecp(c.p), c.j0, hemisphere);
+    R lower = c.p[7], upper = c.p[8], target = c.p[9];
+    int old_h = int(c.s[1]);
+    if (old_h != 0 && old_h != hemisphere) c.s[0] = R(0);
+    c.s[1] = R(hemisphere);
+    c.clamp = 0;
+    bool full_range = lower <= -kPi && upper >= kPi;
+    if (lower >= upper) {
+        c.clamp = 3;
+        error = shortest_angle(error - target, kPi);
+    } else if (!full_range && error >= upper) {
+        c.clamp = 1;
+        error -= upper;
+    } else if (!full_range && error <= lower) {
+        c.clamp = 2;
+        error -= lower;
+    } else {
+        c.s[0] = R(0.0);
+        return;
+    }
+    if (c.clamp == 1) c.s[0] = std::min(c.s[0], R(0));
+    else if (c.clamp == 2) c.s[0] = std::max(c.s[0], R(0));
+    c.em_s = dot(sym_mul(c.j0, sym_add(A.iw, B.iw)), c.j0);
+    c.em_s += c.p[11] * inv_sub;
+    c.em_s = c.em_s > R(0) ? R(1.0) / c.em_s : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    c.bias_s = error * c.p[10] * inv_dt;
+    R acc = c.s[0];
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(acc * c.j0, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(acc * c.j0, B.iw);
+}
+// IterateTwistAngle (:292-328)
+template <class R> void iterate_twist_angle(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
+    if (c.clamp == 0) return;
+    R jv = dot(*A.angvel - *B.angvel, c.j0);
+    R soft = c.s[0] * (c.p[11] * inv_sub);
+    R lambda = -c.em_s * (jv + c.bias_s + soft);
+    R orig = c.s[0];
+    R acc = orig + lambda;
+    if (c.clamp == 1) acc = std::min(acc, R(0.0));
+    else if (c.clamp == 2) acc = std::max(acc, R(0.0));
+    c.s[0] = acc;
+    lambda = acc - orig;
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(lambda * c.j0, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(lambda * c.j0, B.iw);
+}
+
+// ConeLimit (ConeLimit.cs): params LocalAxis1 (0-2), LocalAxis2 (3-5), BiasFactor (6), Softness (7),
+// MinAngle (8), MaxAngle (9); state AccumulatedImpulse (0), the previous step's Jacobian J0 (1-3).
+// PrepareForIterationConeLimit (:256-347)
+template <class R> void prepare_cone_limit(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                            R inv_sub) {
+    const R kPi = R(3.14159265358979323846);
+    V3<R> a1 = rotate(vecp(c.p), A.q);
+    V3<R> a2 = rotate(vecp(c.p + 3), B.q);
+    V3<R> previous = vecp(c.s + 1);
+    V3<R> j0 = cross(a2, a1);
+    c.clamp = 0;
+    R error = dot(a1, a2);
+    R lower = -c.p[8], upper = -c.p[9];
+    R sine = std::sqrt(len_sq(j0));
+    if (sine > R(0)) {
+        j0 = j0 * (R(1.0) / sine);
+    } else {
+        V3<R> axis = create_orthonormal(a1);
+        V3<R> other = cross(a1, axis);
+        Sym<R> ii = sym_add(A.iw, B.iw);
+        j0 = dot(sym_mul(axis, ii), axis) >= dot(sym_mul(other, ii), other) ? axis : other;
+    }
+    V3<R> j1 = -j0;
+    setv(c.s + 1, j0);
+    if (dot(previous, j0) < R(0)) c.s[0] = R(0);
+    error = -StableMath<R>::atan2(sine, error);
+    if (lower == upper) {
+        c.clamp = 3;
+        error -= upper;
+    } else if (error <= upper && c.p[9] < kPi) {
+        c.clamp = 1;
+        error -= upper;
+    } else if (error >= lower && c.p[8] > R(0)) {
+        c.clamp = 2;
+        error -= lower;
+    } else {
+        c.s[0] = R(0.0);
+        return;
+    }
+    c.j0 = j0;
+    c.j1 = j1;
+    if (c.clamp == 1) c.s[0] = std::min(c.s[0], R(0));
+    else if (c.clamp == 2) c.s[0] = std::max(c.s[0], R(0));
+    c.em_s = dot(sym_mul(j0, A.iw), j0) + dot(sym_mul(j1, B.iw), j1);
+    c.em_s += c.p[7] * inv_sub;
+    c.em_s = c.em_s > R(0) ? R(1.0) / c.em_s : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    c.bias_s = -error * c.p[6] * inv_dt;
+    R acc = c.s[0];
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(acc * j0, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel + sym_mul(acc * j1, B.iw);
+}
+// IterateConeLimit (:386-427)
+template <class R> void iterate_cone_limit(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
+    if (c.clamp == 0) return;
+    R jv = dot(*A.angvel, c.j0) + dot(*B.angvel, c.j1);
+    R soft = c.s[0] * c.p[7] * inv_sub;
+    R lambda = -c.em_s * (jv + c.bias_s + soft);
+    R old = c.s[0];
+    R acc = old + lambda;
+    if (c.clamp == 1) acc = std::min(acc, R(0.0));
+    else if (c.clamp == 2) acc = std::max(acc, R(0.0));
+    c.s[0] = acc;
+    lambda = acc - old;
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(lambda * c.j0, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel + sym_mul(lambda * c.j1, B.iw);
+}
+
+// AngularMotor (AngularMotor.cs) / LinearMotor (LinearMotor.cs): params LocalAxis1 (0-2), LocalAxis2
+// (3-5), Velocity (6), MaxForce (7); state AccumulatedImpulse (0).  MaxLambda is kept in bias_s.
+// PrepareForIterationAngularMotor (AngularMotor.cs:150-175), PrepareForIterationLinearMotor
+// (LinearMotor.cs:152-176)
+template <class R> void prepare_motor(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R sub_dt,
+                                      bool linear) {
+    V3<R> j1 = rotate(vecp(c.p), A.q);
+    V3<R> j2 = rotate(vecp(c.p + 3), B.q);
+    if (linear) c.em_s = dot(cmul(j1, A.imv), j1) + dot(cmul(j2, B.imv), j2);
+    else c.em_s = dot(sym_mul(j1, A.iw), j1) + dot(sym_mul(j2, B.iw), j2);
+    c.em_s = c.em_s > R(0) ? R(1.0) / c.em_s : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    c.bias_s = sub_dt * c.p[7];
+    R acc = c.s[0];
+    if (linear) {
+        if (A.dynamic) *A.vel = *A.vel - cmul(j1 * acc, A.imv);
+        if (B.dynamic) *B.vel = *B.vel + cmul(j2 * acc, B.imv);
+    } else {
+        if (A.dynamic) *A.angvel = *A.angvel - sym_mul(j1 * acc, A.iw);
+        if (B.dynamic) *B.angvel = *B.angvel + sym_mul(j2 * acc, B.iw);
+    }
+}
+// IterateAngularMotor (AngularMotor.cs:177-207), IterateLinearMotor (LinearMotor.cs:203-232)
+template <class R> void iterate_motor(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, bool linear) {
+    V3<R> j1 = rotate(vecp(c.p), A.q);
+    V3<R> j2 = rotate(vecp(c.p + 3), B.q);
+    R jv = linear ? dot(-j1, *A.vel) + dot(j2, *B.vel) : dot(-j1, *A.angvel) + dot(j2, *B.angvel);
+    R lambda = -(jv - c.p[6]) * c.em_s;
+    R old = c.s[0];
+    R acc = clamp_cs(old + lambda, -c.bias_s, c.bias_s);
+    c.s[0] = acc;
+    lambda = acc - old;
+    if (linear) {
+        if (A.dynamic) *A.vel = *A.vel - cmul(j1 * lambda, A.imv);
+        if (B.dynamic) *B.vel = *B.vel + cmul(j2 * lambda, B.imv);
+    } else {
+        if (A.dynamic) *A.angvel = *A.angvel - sym_mul(j1 * lambda, A.iw);
+        if (B.dynamic) *B.angvel = *B.angvel + sym_mul(j2 * lambda, B.iw);
+    }
+}
+
+template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                           R inv_sub, R sub_dt) {
     switch (c.type) {
         case JH_CONSTRAINT_BALL_SOCKET: prepare_ball_socket(c, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_HINGE_ANGLE: prepare_hinge_angle(c, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_FIXED_ANGLE: prepare_fixed_angle(c, A, B, inv_dt, inv_sub); break;
-        default: prepare_distance_limit(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_DISTANCE_LIMIT: prepare_distance_limit(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_TWIST_ANGLE: prepare_twist_angle(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_CONE_LIMIT: prepare_cone_limit(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_MOTOR: prepare_motor(c, A, B, sub_dt, false); break;
+        default: prepare_motor(c, A, B, sub_dt, true); break;
     }
 }
 template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
@@ -545,7 +756,11 @@ template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& 
         case JH_CONSTRAINT_BALL_SOCKET: iterate_ball_socket(c, A, B, inv_sub); break;
         case JH_CONSTRAINT_HINGE_ANGLE: iterate_hinge_angle(c, A, B, inv_sub); break;
         case JH_CONSTRAINT_FIXED_ANGLE: iterate_fixed_angle(c, A, B, inv_sub); break;
-        default: iterate_distance_limit(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_DISTANCE_LIMIT: iterate_distance_limit(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_TWIST_ANGLE: iterate_twist_angle(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_CONE_LIMIT: iterate_cone_limit(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_MOTOR: iterate_motor(c, A, B, false); break;
+        default: iterate_motor(c, A, B, true); break;
     }
 }
 

@@ -753,7 +753,7 @@ template <class R> struct World {
                     if (a->usage & (1u << i)) prepare_contact(*a, a->c[i], ts);
             for_constraints([&](Constraint<R>& c) {  // PrepareConstraints (World.Step.cs:621-668)
                 ConstraintBody<R> A = cbody(c.b1), B = cbody(c.b2);
-                constraint_prepare(c, A, B, ts.inv_dt, ts.inv_substep_dt);
+                constraint_prepare(c, A, B, ts.inv_dt, ts.inv_substep_dt, ts.substep_dt);
             });
             for (int it = 0; it < iterations; ++it) {
                 for (auto* a : list)

@@ -233,3 +233,101 @@ def test_constraint_state_round_trip_and_order_replay():
     assert np.array_equal(w1.bodies()["position"], run([0, 1]).bodies()["position"])
     # the two share body 1: the other Gauss-Seidel order gives other bits
     assert not np.array_equal(w1.bodies()["velocity"], run([1, 0]).bodies()["velocity"])
+
+
+def static_post(sb, pos):
+    a = sb.body(pos, motion=scenes.MOTION_STATIC)
+    sb.box(a, (0.2, 0.2, 0.2))
+    sb.bodies[a]["inverse_mass"] = 0.0
+    sb.bodies[a]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    return a
+
+
+def test_twist_limit_bounds_spin_about_the_axis():
+    # a bar on a ball socket with a twist limit of +-30 degrees about its own axis (x), spun hard
+    sb = scenes.SceneBuilder()
+    post = static_post(sb, (0, 10, 0))
+    b = sb.body((1.2, 10, 0), angular_velocity=(15.0, 0.0, 0.0))
+    sb.box(b, (2.0, 0.3, 0.3))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["gravity"] = (0, 0, 0)
+    cfg["substep_count"] = 4
+    lim = math.radians(30)
+    recs = [C.ball_socket(bodies, post, b, (0.2, 10, 0)), C.twist_angle(bodies, post, b, (1, 0, 0), (1, 0, 0), (-lim, lim))]
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array(recs))
+    worst = 0.0
+    for _ in range(240):
+        w.step(1.0 / 120.0)
+        q = w.bodies()["orientation"][1].astype(float)
+        worst = max(worst, abs(2 * math.atan2(q[0], q[3])))  # rotation about x (the only one driven)
+    assert worst < lim + math.radians(3)
+    assert worst > lim - math.radians(5)
+
+
+def test_cone_limit_bounds_the_swing():
+    # a pendulum bar hanging along -y under a 30 degree cone, released at 70 degrees with a push
+    sb = scenes.SceneBuilder()
+    post = static_post(sb, (0, 10, 0))
+    ang = math.radians(70)
+    d = np.array([math.sin(ang), -math.cos(ang), 0.0])
+    b = sb.body(tuple(np.array([0, 10, 0]) + 1.2 * d), orientation=(0, 0, math.sin(ang / 2), math.cos(ang / 2)),
+                velocity=(0, 0, 3.0))
+    sb.box(b, (0.3, 2.0, 0.3))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    lim = math.radians(30)
+    cone = C.cone_limit(bodies, post, b, (0, -1, 0), tuple(-d), limit=(0.0, lim))
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array([C.ball_socket(bodies, post, b, (0, 10, 0)), cone]))
+    angles = []
+    for _ in range(360):
+        w.step(1.0 / 120.0)
+        q = w.bodies()["orientation"][1].astype(float)
+        axis = quat_rotate(-d, qmul(q, qconj(np.array(bodies["orientation"][1], float))))
+        angles.append(math.acos(max(-1.0, min(1.0, -axis[1]))))
+    assert max(angles[120:]) < lim + math.radians(3)  # pulled inside the cone, then held there
+
+
+@pytest.mark.parametrize("max_torque, expect_reach", [(50.0, True), (0.05, False)])
+def test_angular_motor_drives_to_target_velocity(max_torque, expect_reach):
+    sb = scenes.SceneBuilder()
+    post = static_post(sb, (0, 5, 0))
+    b = sb.body((0, 5, 0.6))
+    sb.box(b, (1.0, 1.0, 0.2))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["gravity"] = (0, 0, 0)
+    cfg["substep_count"] = 4
+    recs = C.hinge_joint(bodies, post, b, (0, 5, 0.6), (0, 0, 1), motor=(5.0, max_torque))
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array(recs))
+    dt = 1.0 / 100.0
+    wz = []
+    for _ in range(100):
+        w.step(dt)
+        wz.append(float(w.bodies()["angular_velocity"][1][2]))
+    if expect_reach:
+        assert abs(wz[-1] - 5.0) < 0.05
+    else:
+        # the impulse per substep is bounded by MaximumTorque * SubstepDt: the spin-up is slower
+        Izz = 1.0 / bodies["inverse_inertia"][1][5]
+        assert wz[-1] < 1.05 * max_torque * 1.0 / Izz + 1e-3 and wz[-1] > 0
+
+
+def test_linear_motor_drives_to_target_velocity():
+    sb = scenes.SceneBuilder()
+    post = static_post(sb, (0, 5, 0))
+    b = sb.body((3, 5, 0))
+    sb.box(b, (0.5, 0.5, 0.5))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["gravity"] = (0, 0, 0)
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array([C.linear_motor(bodies, post, b, (1, 0, 0), (1, 0, 0), 2.0, 100.0)]))
+    for _ in range(30):
+        w.step(1.0 / 100.0)
+    v = w.bodies()["velocity"][1]
+    assert abs(v[0] * 0.998 ** 0 - 2.0) < 0.02 and abs(v[1]) < 1e-6
