@@ -176,10 +176,15 @@ typedef struct jh_arbiter {
  *   CONE_LIMIT (ConeLimit.cs): params LocalAxis1[3], LocalAxis2[3], BiasFactor, Softness, MinAngle,
  *       MaxAngle; state AccumulatedImpulse, the last Jacobian J0[3].
  *   ANGULAR_MOTOR (AngularMotor.cs), LINEAR_MOTOR (LinearMotor.cs): params LocalAxis1[3], LocalAxis2[3],
- *       Velocity (TargetVelocity), MaxForce (MaximumForce / MaximumTorque); state AccumulatedImpulse. */
+ *       Velocity (TargetVelocity), MaxForce (MaximumForce / MaximumTorque); state AccumulatedImpulse.
+ *   POINT_ON_LINE (PointOnLine.cs): params LocalAxis[3] (body-1 frame), LocalAnchor1[3], LocalAnchor2[3],
+ *       BiasFactor, LimitBias, Softness, LimitSoftness, Min, Max; state AccumulatedImpulse[3].
+ *   POINT_ON_PLANE (PointOnPlane.cs): params LocalAxis[3] (body-1 frame), LocalAnchor1[3], LocalAnchor2[3],
+ *       BiasFactor, Softness, Min, Max; state AccumulatedImpulse. */
 enum { JH_CONSTRAINT_BALL_SOCKET = 0, JH_CONSTRAINT_HINGE_ANGLE = 1, JH_CONSTRAINT_FIXED_ANGLE = 2,
        JH_CONSTRAINT_DISTANCE_LIMIT = 3, JH_CONSTRAINT_TWIST_ANGLE = 4, JH_CONSTRAINT_CONE_LIMIT = 5,
-       JH_CONSTRAINT_ANGULAR_MOTOR = 6, JH_CONSTRAINT_LINEAR_MOTOR = 7 };
+       JH_CONSTRAINT_ANGULAR_MOTOR = 6, JH_CONSTRAINT_LINEAR_MOTOR = 7, JH_CONSTRAINT_POINT_ON_LINE = 8,
+       JH_CONSTRAINT_POINT_ON_PLANE = 9 };
 
 typedef struct jh_constraint {
     int32_t type;           /* JH_CONSTRAINT_*                                                  */

@@ -13,7 +13,7 @@ import numpy as np
 
 from ._abi import (CONSTRAINT_ANGULAR_MOTOR, CONSTRAINT_BALL_SOCKET, CONSTRAINT_CONE_LIMIT, CONSTRAINT_DISTANCE_LIMIT,
                    CONSTRAINT_DTYPE, CONSTRAINT_FIXED_ANGLE, CONSTRAINT_HINGE_ANGLE, CONSTRAINT_LINEAR_MOTOR,
-                   CONSTRAINT_TWIST_ANGLE)
+                   CONSTRAINT_POINT_ON_LINE, CONSTRAINT_POINT_ON_PLANE, CONSTRAINT_TWIST_ANGLE)
 
 f32 = np.float32
 
@@ -228,6 +228,62 @@ def linear_motor(bodies, b1, b2, axis1, axis2=None, velocity=0.0, max_force=0.0)
     """LinearMotor.Initialize(axis1, axis2) (LinearMotor.cs:97-113) with TargetVelocity and MaximumForce."""
     return _motor(CONSTRAINT_LINEAR_MOTOR, bodies, b1, b2, axis1, axis1 if axis2 is None else axis2, velocity,
                   max_force)
+
+
+LINEAR_FULL = (-math.inf, math.inf)  # LinearLimit.Full (Limit.cs:85-86)
+DEFAULT_LINEAR_LIMIT_SOFTNESS = f32(0.0001)
+DEFAULT_LINEAR_LIMIT_BIAS = f32(0.2)
+
+
+def point_on_line(bodies, b1, b2, axis, anchor1, anchor2, limit=(1e-6, -1e-6)):
+    """PointOnLine.Initialize(axis, anchor1, anchor2, limit) (PointOnLine.cs:92-117): anchor 2 stays on the
+    line through anchor 1 along the axis (body-1 frame), its travel along the line inside the limit (the
+    three-argument overload uses LinearLimit.Fixed, PointOnLine.cs:69-72)."""
+    c = _record(CONSTRAINT_POINT_ON_LINE, b1, b2)
+    p = c["params"]
+    p[0:3] = _rot_inv(_normalize(axis), bodies["orientation"][b1])
+    p[3:6] = _local_anchor(bodies, b1, anchor1)
+    p[6:9] = _local_anchor(bodies, b2, anchor2)
+    p[9] = DEFAULT_LINEAR_BIAS
+    p[10] = DEFAULT_LINEAR_LIMIT_BIAS
+    p[11] = DEFAULT_LINEAR_SOFTNESS
+    p[12] = DEFAULT_LINEAR_LIMIT_SOFTNESS
+    p[13], p[14] = f32(limit[0]), f32(limit[1])
+    return c
+
+
+def point_on_plane(bodies, b1, b2, axis, anchor1, anchor2, limit=(1e-6, -1e-6)):
+    """PointOnPlane.Initialize(axis, anchor1, anchor2, limit) (PointOnPlane.cs:89-112): the distance of
+    anchor 2 from the plane through anchor 1 with normal axis (body-1 frame) inside the limit."""
+    c = _record(CONSTRAINT_POINT_ON_PLANE, b1, b2)
+    p = c["params"]
+    p[0:3] = _rot_inv(_normalize(axis), bodies["orientation"][b1])
+    p[3:6] = _local_anchor(bodies, b1, anchor1)
+    p[6:9] = _local_anchor(bodies, b2, anchor2)
+    p[9] = DEFAULT_LINEAR_BIAS
+    p[10] = DEFAULT_LINEAR_SOFTNESS
+    p[11], p[12] = f32(limit[0]), f32(limit[1])
+    return c
+
+
+def prismatic_joint(bodies, b1, b2, center, axis, limit=LINEAR_FULL, pinned=True, motor=None):
+    """PrismaticJoint (Dynamics/Joints/PrismaticJoint.cs:95-140): PointOnLine through the centre, then
+    FixedAngle (pinned) or HingeAngle about the axis (full range), then (motor = (velocity, max force))
+    a LinearMotor along the axis."""
+    out = [point_on_line(bodies, b1, b2, axis, center, center, limit)]
+    out.append(fixed_angle(bodies, b1, b2) if pinned else hinge_angle(bodies, b1, b2, axis))
+    if motor is not None:
+        out.append(linear_motor(bodies, b1, b2, axis, axis, motor[0], motor[1]))
+    return out
+
+
+def universal_joint(bodies, b1, b2, center, axis1, axis2, motor=None):
+    """UniversalJoint (Dynamics/Joints/UniversalJoint.cs:30-72): TwistAngle between the two rotation axes
+    (fixed), BallSocket at the centre, then (motor = (velocity, max torque)) an AngularMotor."""
+    out = [twist_angle(bodies, b1, b2, axis1, axis2), ball_socket(bodies, b1, b2, center)]
+    if motor is not None:
+        out.append(angular_motor(bodies, b1, b2, axis1, axis2, motor[0], motor[1]))
+    return out
 
 
 def ragdoll_joint(bodies, b1, b2, anchor, axis, cone=math.pi / 4, twist=(-math.pi / 6, math.pi / 6)):

@@ -1,5 +1,5 @@
 // Constraints on the device: BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit,
-// AngularMotor, LinearMotor (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
+// AngularMotor, LinearMotor, PointOnLine, PointOnPlane (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
 // pieces they use, FP32 in the reference's operation order.  One thread solves one constraint; the
 // solver visits constraints in batches with disjoint dynamic bodies.
 #pragma once
@@ -723,6 +723,222 @@ __device__ __forceinline__ void iterate_motor(jh_constraint& c, Work& k, CBody& 
     }
 }
 
+// Linear rows of the slider constraints: for direction d, Jacobian (-d, -((r1 + u) x d), d, r2 x d).
+struct LinRow {
+    vec3 j0, j1, j2, j3;
+};
+__device__ __forceinline__ LinRow lin_row(vec3 r1, vec3 r2, vec3 u, vec3 d) {
+    return {-d, -cross(r1 + u, d), d, cross(r2, d)};
+}
+__device__ __forceinline__ float lin_mass(vec3 a, vec3 imv1, vec3 b, vec3 c, vec3 imv2, vec3 e) {
+    return dot(cmul(a, imv1), b) + dot(cmul(c, imv2), e);
+}
+__device__ __forceinline__ float ang_mass(vec3 a, const sym3& i1, vec3 b, vec3 c, const sym3& i2, vec3 e) {
+    return dot(sym_mul(a, i1), b) + dot(sym_mul(c, i2), e);
+}
+
+// PrepareForIterationPointOnPlane (PointOnPlane.cs:118-192)
+__device__ __forceinline__ void prepare_point_on_plane(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                       float inv_sub) {
+    const float* p = c.params;
+    vec3 axis = rot(p3(p), A.q);
+    vec3 r1 = rot(p3(p + 3), A.q);
+    vec3 r2 = rot(p3(p + 6), B.q);
+    vec3 p1 = A.pos + r1;
+    vec3 p2 = B.pos + r2;
+    k.clamp = 0;
+    vec3 u = p2 - p1;
+    k.j0 = -axis;
+    k.j1 = -cross(r1 + u, axis);
+    k.j2 = axis;
+    k.j3 = cross(r2, axis);
+    float error = dot(u, axis);
+    k.em_s = 1.0f;
+    float mn = p[11], mx = p[12];
+    if (mn >= mx) {
+        error -= 0.5f * mn + 0.5f * mx;
+        k.clamp = 3;
+    } else if (error >= mx) {
+        error -= mx;
+        k.clamp = 1;
+    } else if (error <= mn) {
+        error -= mn;
+        k.clamp = 2;
+    } else {
+        c.state[0] = 0.0f;
+        return;
+    }
+    k.em_s = dot(cmul(k.j0, A.imv), k.j0) + dot(cmul(k.j2, B.imv), k.j2) + dot(sym_mul(k.j1, A.I), k.j1) +
+             dot(sym_mul(k.j3, B.I), k.j3);
+    k.em_s += (p[10] * inv_sub);
+    k.em_s = k.em_s > 0.0f ? 1.0f / k.em_s : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    k.bias_s = error * p[9] * inv_dt;
+    float acc = c.state[0];
+    if (A.dyn) {
+        A.v = A.v + cmul(k.j0 * acc, A.imv);
+        A.w = A.w + sym_mul(k.j1 * acc, A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(k.j2 * acc, B.imv);
+        B.w = B.w + sym_mul(k.j3 * acc, B.I);
+    }
+}
+// IteratePointOnPlane (PointOnPlane.cs:231-274)
+__device__ __forceinline__ void iterate_point_on_plane(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    if (k.clamp == 0) return;
+    float jv = dot(k.j0, A.v) + dot(k.j1, A.w) + dot(k.j2, B.v) + dot(k.j3, B.w);
+    float soft = c.state[0] * c.params[10] * inv_sub;
+    float lambda = -1.0f * (jv + k.bias_s + soft) * k.em_s;
+    float orig = c.state[0];
+    float acc = orig + lambda;
+    if (k.clamp == 1) acc = cmin(acc, 0.0f);
+    else if (k.clamp == 2) acc = cmax(acc, 0.0f);
+    c.state[0] = acc;
+    lambda = acc - orig;
+    if (A.dyn) {
+        A.v = A.v + cmul(k.j0 * lambda, A.imv);
+        A.w = A.w + sym_mul(k.j1 * lambda, A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(k.j2 * lambda, B.imv);
+        B.w = B.w + sym_mul(k.j3 * lambda, B.I);
+    }
+}
+
+// PointOnLine: the three rows rebuilt from the poses, as in prepare and iterate of PointOnLine.cs
+__device__ __forceinline__ void point_on_line_rows(const jh_constraint& c, const CBody& A, const CBody& B, LinRow* J,
+                                                   vec3& err) {
+    const float* p = c.params;
+    vec3 aw = rot(p3(p), A.q);
+    vec3 n1 = create_orthonormal(p3(p));
+    n1 = rot(n1, A.q);
+    vec3 n2 = cross(aw, n1);
+    vec3 r1 = rot(p3(p + 3), A.q);
+    vec3 r2 = rot(p3(p + 6), B.q);
+    vec3 p1 = A.pos + r1;
+    vec3 p2 = B.pos + r2;
+    vec3 u = p2 - p1;
+    J[0] = lin_row(r1, r2, u, n1);
+    J[1] = lin_row(r1, r2, u, n2);
+    J[2] = lin_row(r1, r2, u, aw);
+    err = vec3{dot(u, n1), dot(u, n2), dot(u, aw)};
+}
+// PrepareForIterationPointOnLine (PointOnLine.cs:149-300)
+__device__ __forceinline__ void prepare_point_on_line(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                      float inv_sub) {
+    const float* p = c.params;
+    LinRow J[3];
+    vec3 error;
+    point_on_line_rows(c, A, B, J, error);
+    k.clamp = 0;
+    k.em = mat3{1, 0, 0, 0, 1, 0, 0, 0, 1};
+    k.em.m11 = lin_mass(J[0].j0, A.imv, J[0].j0, J[0].j2, B.imv, J[0].j2) + dot(sym_mul(J[0].j1, A.I), J[0].j1) +
+               dot(sym_mul(J[0].j3, B.I), J[0].j3);
+    k.em.m22 = lin_mass(J[1].j0, A.imv, J[1].j0, J[1].j2, B.imv, J[1].j2) + dot(sym_mul(J[1].j1, A.I), J[1].j1) +
+               dot(sym_mul(J[1].j3, B.I), J[1].j3);
+    k.em.m12 = ang_mass(J[0].j1, A.I, J[1].j1, J[0].j3, B.I, J[1].j3);
+    k.em.m21 = ang_mass(J[1].j1, A.I, J[0].j1, J[1].j3, B.I, J[0].j3);
+    float coupling12 = lin_mass(J[0].j0, A.imv, J[1].j0, J[0].j2, B.imv, J[1].j2);
+    k.em.m12 += coupling12;
+    k.em.m21 += coupling12;
+    float mn = p[13], mx = p[14];
+    if (mn >= mx) {
+        error.z -= 0.5f * mn + 0.5f * mx;
+        k.clamp = 3;
+    } else if (error.z >= mx) {
+        error.z -= mx;
+        k.clamp = 1;
+    } else if (error.z <= mn) {
+        error.z -= mn;
+        k.clamp = 2;
+    } else {
+        c.state[2] = 0.0f;
+    }
+    if (k.clamp != 0) {
+        k.em.m33 = lin_mass(J[2].j0, A.imv, J[2].j0, J[2].j2, B.imv, J[2].j2) + dot(sym_mul(J[2].j1, A.I), J[2].j1) +
+                   dot(sym_mul(J[2].j3, B.I), J[2].j3);
+        k.em.m13 = ang_mass(J[0].j1, A.I, J[2].j1, J[0].j3, B.I, J[2].j3);
+        k.em.m31 = ang_mass(J[2].j1, A.I, J[0].j1, J[2].j3, B.I, J[0].j3);
+        k.em.m23 = ang_mass(J[1].j1, A.I, J[2].j1, J[1].j3, B.I, J[2].j3);
+        k.em.m32 = ang_mass(J[2].j1, A.I, J[1].j1, J[2].j3, B.I, J[1].j3);
+        float coupling13 = lin_mass(J[0].j0, A.imv, J[2].j0, J[0].j2, B.imv, J[2].j2);
+        float coupling23 = lin_mass(J[1].j0, A.imv, J[2].j0, J[1].j2, B.imv, J[2].j2);
+        k.em.m13 += coupling13;
+        k.em.m31 += coupling13;
+        k.em.m23 += coupling23;
+        k.em.m32 += coupling23;
+    }
+    k.em.m11 += p[11] * inv_sub;
+    k.em.m22 += p[11] * inv_sub;
+    k.em.m33 += p[12] * inv_sub;
+    if (k.clamp == 1 || k.clamp == 2) {
+        k.bilateral = inverse_bilateral_block(k.em);
+        k.cx = k.em.m13;
+        k.cy = k.em.m23;
+    }
+    if (k.clamp == 0) k.em.m33 = 0.0f;
+    k.em = inverse_symmetric(k.em);
+    k.bias = error * inv_dt;
+    k.bias.x *= p[9];
+    k.bias.y *= p[9];
+    k.bias.z *= p[10];
+    if (k.clamp == 1) c.state[2] = cmin(0.0f, c.state[2]);
+    else if (k.clamp == 2) c.state[2] = cmax(0.0f, c.state[2]);
+    vec3 acc = p3(c.state);
+    if (A.dyn) {
+        A.v = A.v + cmul(J[0].j0 * acc.x + J[1].j0 * acc.y + J[2].j0 * acc.z, A.imv);
+        A.w = A.w + sym_mul(J[0].j1 * acc.x + J[1].j1 * acc.y + J[2].j1 * acc.z, A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(J[0].j2 * acc.x + J[1].j2 * acc.y + J[2].j2 * acc.z, B.imv);
+        B.w = B.w + sym_mul(J[0].j3 * acc.x + J[1].j3 * acc.y + J[2].j3 * acc.z, B.I);
+    }
+}
+// IteratePointOnLine (PointOnLine.cs:372-465)
+__device__ __forceinline__ void iterate_point_on_line(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    const float* p = c.params;
+    LinRow J[3];
+    vec3 unused;
+    point_on_line_rows(c, A, B, J, unused);
+    vec3 jv;
+    jv.x = dot(J[0].j0, A.v) + dot(J[0].j1, A.w) + dot(J[0].j2, B.v) + dot(J[0].j3, B.w);
+    jv.y = dot(J[1].j0, A.v) + dot(J[1].j1, A.w) + dot(J[1].j2, B.v) + dot(J[1].j3, B.w);
+    jv.z = dot(J[2].j0, A.v) + dot(J[2].j1, A.w) + dot(J[2].j2, B.v) + dot(J[2].j3, B.w);
+    vec3 acc = p3(c.state);
+    vec3 soft = acc * inv_sub;
+    soft.x *= p[11];
+    soft.y *= p[11];
+    soft.z *= p[12];
+    vec3 residual = jv + k.bias + soft;
+    vec3 lambda = -1.0f * mat_mul(residual, k.em);
+    vec3 orig = acc;
+    acc = acc + lambda;
+    if ((k.clamp == 1 && acc.z > 0.0f) || (k.clamp == 2 && acc.z < 0.0f)) {
+        lambda.z = -orig.z;
+        float x = residual.x + k.cx * lambda.z;
+        float y = residual.y + k.cy * lambda.z;
+        lambda.x = -(k.bilateral.x * x + k.bilateral.y * y);
+        lambda.y = -(k.bilateral.y * x + k.bilateral.z * y);
+        acc.x = orig.x + lambda.x;
+        acc.y = orig.y + lambda.y;
+        acc.z = 0.0f;
+    } else if (k.clamp == 0) {
+        acc.z = 0.0f;
+        orig.z = 0.0f;
+    }
+    set3(c.state, acc);
+    lambda = acc - orig;
+    if (A.dyn) {
+        A.v = A.v + cmul(J[0].j0 * lambda.x + J[1].j0 * lambda.y + J[2].j0 * lambda.z, A.imv);
+        A.w = A.w + sym_mul(J[0].j1 * lambda.x + J[1].j1 * lambda.y + J[2].j1 * lambda.z, A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(J[0].j2 * lambda.x + J[1].j2 * lambda.y + J[2].j2 * lambda.z, B.imv);
+        B.w = B.w + sym_mul(J[0].j3 * lambda.x + J[1].j3 * lambda.y + J[2].j3 * lambda.z, B.I);
+    }
+}
+
 __device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt, float inv_sub,
                                         float sub_dt) {
     switch (c.type) {
@@ -733,7 +949,9 @@ __device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBo
         case JH_CONSTRAINT_TWIST_ANGLE: prepare_twist_angle(c, k, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_CONE_LIMIT: prepare_cone_limit(c, k, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_ANGULAR_MOTOR: prepare_motor(c, k, A, B, sub_dt, false); break;
-        default: prepare_motor(c, k, A, B, sub_dt, true); break;
+        case JH_CONSTRAINT_LINEAR_MOTOR: prepare_motor(c, k, A, B, sub_dt, true); break;
+        case JH_CONSTRAINT_POINT_ON_LINE: prepare_point_on_line(c, k, A, B, inv_dt, inv_sub); break;
+        default: prepare_point_on_plane(c, k, A, B, inv_dt, inv_sub); break;
     }
 }
 __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
@@ -745,7 +963,9 @@ __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBo
         case JH_CONSTRAINT_TWIST_ANGLE: iterate_twist_angle(c, k, A, B, inv_sub); break;
         case JH_CONSTRAINT_CONE_LIMIT: iterate_cone_limit(c, k, A, B, inv_sub); break;
         case JH_CONSTRAINT_ANGULAR_MOTOR: iterate_motor(c, k, A, B, false); break;
-        default: iterate_motor(c, k, A, B, true); break;
+        case JH_CONSTRAINT_LINEAR_MOTOR: iterate_motor(c, k, A, B, true); break;
+        case JH_CONSTRAINT_POINT_ON_LINE: iterate_point_on_line(c, k, A, B, inv_sub); break;
+        default: iterate_point_on_plane(c, k, A, B, inv_sub); break;
     }
 }
 

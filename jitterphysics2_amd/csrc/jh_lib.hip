@@ -1324,7 +1324,7 @@ jh_status jh_world_set_constraints(jh_world* w, const jh_constraint* in, int32_t
     if (!w || n < 0 || (n > 0 && !in)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
     for (int i = 0; i < n; ++i) {
         const jh_constraint& c = in[i];
-        if (c.type < JH_CONSTRAINT_BALL_SOCKET || c.type > JH_CONSTRAINT_LINEAR_MOTOR)
+        if (c.type < JH_CONSTRAINT_BALL_SOCKET || c.type > JH_CONSTRAINT_POINT_ON_PLANE)
             return fail(JH_ERR_INVALID_ARG, "constraint %d: unknown type %d", i, c.type);
         if (c.body1 < 0 || c.body1 >= w->nb || c.body2 < 0 || c.body2 >= w->nb)
             return fail(JH_ERR_INVALID_ARG, "constraint %d: body index out of range", i);

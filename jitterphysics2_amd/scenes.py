@@ -511,8 +511,9 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
     a hinge chain of boxes hanging from a static post (HingeJoint, limits +-60 degrees), a rope of
     spheres (BallSocket chain), welded box pairs (WeldJoint) dropped on the floor, capsules tied to a
     post by distance limits with a slack range, a ragdoll-like limb chain (BallSocket + ConeLimit +
-    TwistAngle), and motor-driven wheels on hinges (AngularMotor) with a block pulled by a LinearMotor;
-    free boxes fall between them.  Returns (bodies, shapes,
+    TwistAngle), motor-driven wheels on hinges (AngularMotor) with a block pulled by a LinearMotor,
+    prismatic sliders (PointOnLine + FixedAngle or HingeAngle, limits, motors), and universal-joint chains
+    with a body held on a plane (PointOnPlane); free boxes fall between them.  Returns (bodies, shapes,
     verts, cfg, dt, constraints).  dt = 1/60, 4 substeps, 10 iterations."""
     from . import constraints as C
 
@@ -524,9 +525,9 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
     side = int(math.ceil(math.sqrt(n_groups)))
     for g in range(n_groups):
         gx, gz = (g % side - side / 2.0) * 8.0, (g // side - side / 2.0) * 8.0
-        kind = g % 6
+        kind = g % 8
         top = 14.0
-        if kind in (0, 1, 3, 4, 5):
+        if kind != 2:
             post = _static_box(sb, (gx, top + 0.5, gz), (0.4, 1.0, 0.4))
         if kind == 0:  # hinge chain
             prev, anchor_y = post, top
@@ -568,6 +569,21 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
             b = sb.body((gx + 2.5, top - 3.0, gz))
             sb.box(b, (0.5, 0.5, 0.5))
             plan.append(("slider", post, b, None))
+        elif kind == 6:  # prismatic sliders
+            for k in range(links // 2):
+                b = sb.body((gx - 2.0 + 1.3 * k, top - 1.5, gz), velocity=(rng.normal(), rng.normal(), rng.normal()))
+                sb.box(b, (0.5, 0.5, 0.5))
+                plan.append(("slider_joint", post, b, (gx - 2.0 + 1.3 * k, top - 1.5, gz), k))
+        elif kind == 7:  # universal-joint chain, and a sphere held on a plane
+            prev, anchor_y = post, top
+            for k in range(links // 2):
+                b = sb.body((gx, anchor_y - 0.55, gz), angular_velocity=(rng.normal(), 2.0, rng.normal()))
+                sb.box(b, (0.3, 1.0, 0.3))
+                plan.append(("universal", prev, b, (gx, anchor_y, gz)))
+                prev, anchor_y = b, anchor_y - 1.1
+            b = sb.body((gx + 2.0, top - 2.0, gz + 1.0), velocity=(1.0, 0.0, -1.0))
+            sb.sphere(b, 0.3)
+            plan.append(("plane", post, b, (gx + 2.0, top - 2.0, gz + 1.0)))
         else:  # capsules on slack distance limits
             for k in range(links // 2):
                 b = sb.body((gx + rng.uniform(-2, 2), top - 2.0 - 1.5 * k, gz + rng.uniform(-2, 2)),
@@ -578,8 +594,18 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
             b = sb.body((gx + rng.uniform(-3, 3), 4.0 + 3.0 * k, gz + rng.uniform(-3, 3)), orientation=rng.quaternion())
             sb.box(b, [2.0 * rng.uniform(0.2, 0.5) for _ in range(3)])
     bodies, shapes, verts = sb.arrays()
-    for kind, a, b, anchor in plan:
-        if kind == "hinge":
+    for item in plan:
+        kind, a, b, anchor = item[:4]
+        if kind == "slider_joint":
+            k = item[4]
+            ax = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (1.0, 1.0, 0.0), (0.0, 0.0, 1.0))[k % 4]
+            recs += C.prismatic_joint(bodies, a, b, anchor, ax, limit=(-0.6, 0.8), pinned=k % 2 == 0,
+                                      motor=(0.5, 10.0) if k % 3 == 0 else None)
+        elif kind == "universal":
+            recs += C.universal_joint(bodies, a, b, anchor, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0))
+        elif kind == "plane":
+            recs.append(C.point_on_plane(bodies, a, b, (0.0, 1.0, 0.0), bodies["position"][a], anchor))
+        elif kind == "hinge":
             recs += C.hinge_joint(bodies, a, b, anchor, (1.0, 0.0, 0.0), (-math.pi / 3, math.pi / 3))
         elif kind == "ball":
             recs.append(C.ball_socket(bodies, a, b, anchor))

@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only (see orc_math.hpp).  Constraints solved with the contacts —
-// BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit, AngularMotor, LinearMotor
-// (Dynamics/Constraints/*.cs) — and the JMatrix,
+// BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit, AngularMotor, LinearMotor,
+// PointOnLine, PointOnPlane (Dynamics/Constraints/*.cs) — and the JMatrix,
 // JSymmetricMatrix, MathHelper and AngularConstraintMath pieces they use, restated in the reference's
 // operation order and templated on the precision R.
 #pragma once
@@ -738,6 +738,226 @@ template <class R> void iterate_motor(Constraint<R>& c, ConstraintBody<R>& A, Co
     }
 }
 
+// Linear rows of the slider constraints: for direction d, Jacobian (-d, -((r1 + u) x d), d, r2 x d).
+template <class R> struct LinRow {
+    V3<R> j0, j1, j2, j3;
+};
+template <class R> inline LinRow<R> lin_row(V3<R> r1, V3<R> r2, V3<R> u, V3<R> d) {
+    return {-d, -cross(r1 + u, d), d, cross(r2, d)};
+}
+// JVector.Multiply(a, imv1) * b + JVector.Multiply(c, imv2) * e: the linear part of a mass entry
+template <class R> inline R lin_mass(V3<R> a, V3<R> imv1, V3<R> b, V3<R> c, V3<R> imv2, V3<R> e) {
+    return dot(cmul(a, imv1), b) + dot(cmul(c, imv2), e);
+}
+template <class R> inline R ang_mass(V3<R> a, Sym<R> i1, V3<R> b, V3<R> c, Sym<R> i2, V3<R> e) {
+    return dot(sym_mul(a, i1), b) + dot(sym_mul(c, i2), e);
+}
+
+// PointOnPlane (PointOnPlane.cs): params LocalAxis (0-2, body-1 frame), LocalAnchor1 (3-5), LocalAnchor2
+// (6-8), BiasFactor (9), Softness (10), Min (11), Max (12); state AccumulatedImpulse (0).
+// PrepareForIterationPointOnPlane (:118-192)
+template <class R> void prepare_point_on_plane(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                               R inv_sub) {
+    V3<R> axis = rotate(vecp(c.p), A.q);
+    V3<R> r1 = rotate(vecp(c.p + 3), A.q);
+    V3<R> r2 = rotate(vecp(c.p + 6), B.q);
+    V3<R> p1 = A.pos + r1;
+    V3<R> p2 = B.pos + r2;
+    c.clamp = 0;
+    V3<R> u = p2 - p1;
+    c.j0 = -axis;
+    c.j1 = -cross(r1 + u, axis);
+    c.j2 = axis;
+    c.j3 = cross(r2, axis);
+    R error = dot(u, axis);
+    c.em_s = R(1.0);
+    R mn = c.p[11], mx = c.p[12];
+    if (mn >= mx) {
+        error -= R(0.5) * mn + R(0.5) * mx;
+        c.clamp = 3;
+    } else if (error >= mx) {
+        error -= mx;
+        c.clamp = 1;
+    } else if (error <= mn) {
+        error -= mn;
+        c.clamp = 2;
+    } else {
+        c.s[0] = R(0);
+        return;
+    }
+    c.em_s = dot(cmul(c.j0, A.imv), c.j0) + dot(cmul(c.j2, B.imv), c.j2) + dot(sym_mul(c.j1, A.iw), c.j1) +
+             dot(sym_mul(c.j3, B.iw), c.j3);
+    c.em_s += (c.p[10] * inv_sub);
+    c.em_s = c.em_s > R(0) ? R(1.0) / c.em_s : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    c.bias_s = error * c.p[9] * inv_dt;
+    R acc = c.s[0];
+    if (A.dynamic) {
+        *A.vel = *A.vel + cmul(c.j0 * acc, A.imv);
+        *A.angvel = *A.angvel + sym_mul(c.j1 * acc, A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(c.j2 * acc, B.imv);
+        *B.angvel = *B.angvel + sym_mul(c.j3 * acc, B.iw);
+    }
+}
+// IteratePointOnPlane (:231-274)
+template <class R> void iterate_point_on_plane(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                               R inv_sub) {
+    if (c.clamp == 0) return;
+    R jv = dot(c.j0, *A.vel) + dot(c.j1, *A.angvel) + dot(c.j2, *B.vel) + dot(c.j3, *B.angvel);
+    R soft = c.s[0] * c.p[10] * inv_sub;
+    R lambda = -R(1.0) * (jv + c.bias_s + soft) * c.em_s;
+    R orig = c.s[0];
+    R acc = orig + lambda;
+    if (c.clamp == 1) acc = std::min(acc, R(0.0));
+    else if (c.clamp == 2) acc = std::max(acc, R(0.0));
+    c.s[0] = acc;
+    lambda = acc - orig;
+    if (A.dynamic) {
+        *A.vel = *A.vel + cmul(c.j0 * lambda, A.imv);
+        *A.angvel = *A.angvel + sym_mul(c.j1 * lambda, A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(c.j2 * lambda, B.imv);
+        *B.angvel = *B.angvel + sym_mul(c.j3 * lambda, B.iw);
+    }
+}
+
+// PointOnLine (PointOnLine.cs): params LocalAxis (0-2, body-1 frame), LocalAnchor1 (3-5), LocalAnchor2
+// (6-8), BiasFactor (9), LimitBias (10), Softness (11), LimitSoftness (12), Min (13), Max (14); state
+// AccumulatedImpulse (0-2).  The three rows (two across the line, one along it) are rebuilt from the
+// bodies' poses in prepare and in every iteration, as the reference does.
+template <class R> void point_on_line_rows(const Constraint<R>& c, const ConstraintBody<R>& A,
+                                           const ConstraintBody<R>& B, LinRow<R>* rows, V3<R>& err) {
+    V3<R> aw = rotate(vecp(c.p), A.q);
+    V3<R> n1 = create_orthonormal(vecp(c.p));
+    n1 = rotate(n1, A.q);
+    V3<R> n2 = cross(aw, n1);
+    V3<R> r1 = rotate(vecp(c.p + 3), A.q);
+    V3<R> r2 = rotate(vecp(c.p + 6), B.q);
+    V3<R> p1 = A.pos + r1;
+    V3<R> p2 = B.pos + r2;
+    V3<R> u = p2 - p1;
+    rows[0] = lin_row(r1, r2, u, n1);
+    rows[1] = lin_row(r1, r2, u, n2);
+    rows[2] = lin_row(r1, r2, u, aw);
+    err = V3<R>{dot(u, n1), dot(u, n2), dot(u, aw)};
+}
+// PrepareForIterationPointOnLine (:149-300)
+template <class R> void prepare_point_on_line(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                              R inv_sub) {
+    LinRow<R> J[3];
+    V3<R> error;
+    point_on_line_rows(c, A, B, J, error);
+    c.clamp = 0;
+    c.em = midentity<R>();
+    c.em.m11 = lin_mass(J[0].j0, A.imv, J[0].j0, J[0].j2, B.imv, J[0].j2) + dot(sym_mul(J[0].j1, A.iw), J[0].j1) +
+               dot(sym_mul(J[0].j3, B.iw), J[0].j3);
+    c.em.m22 = lin_mass(J[1].j0, A.imv, J[1].j0, J[1].j2, B.imv, J[1].j2) + dot(sym_mul(J[1].j1, A.iw), J[1].j1) +
+               dot(sym_mul(J[1].j3, B.iw), J[1].j3);
+    c.em.m12 = ang_mass(J[0].j1, A.iw, J[1].j1, J[0].j3, B.iw, J[1].j3);
+    c.em.m21 = ang_mass(J[1].j1, A.iw, J[0].j1, J[1].j3, B.iw, J[0].j3);
+    R coupling12 = lin_mass(J[0].j0, A.imv, J[1].j0, J[0].j2, B.imv, J[1].j2);
+    c.em.m12 += coupling12;
+    c.em.m21 += coupling12;
+    R mn = c.p[13], mx = c.p[14];
+    if (mn >= mx) {
+        error.z -= R(0.5) * mn + R(0.5) * mx;
+        c.clamp = 3;
+    } else if (error.z >= mx) {
+        error.z -= mx;
+        c.clamp = 1;
+    } else if (error.z <= mn) {
+        error.z -= mn;
+        c.clamp = 2;
+    } else {
+        c.s[2] = R(0);
+    }
+    if (c.clamp != 0) {
+        c.em.m33 = lin_mass(J[2].j0, A.imv, J[2].j0, J[2].j2, B.imv, J[2].j2) + dot(sym_mul(J[2].j1, A.iw), J[2].j1) +
+                   dot(sym_mul(J[2].j3, B.iw), J[2].j3);
+        c.em.m13 = ang_mass(J[0].j1, A.iw, J[2].j1, J[0].j3, B.iw, J[2].j3);
+        c.em.m31 = ang_mass(J[2].j1, A.iw, J[0].j1, J[2].j3, B.iw, J[0].j3);
+        c.em.m23 = ang_mass(J[1].j1, A.iw, J[2].j1, J[1].j3, B.iw, J[2].j3);
+        c.em.m32 = ang_mass(J[2].j1, A.iw, J[1].j1, J[2].j3, B.iw, J[1].j3);
+        R coupling13 = lin_mass(J[0].j0, A.imv, J[2].j0, J[0].j2, B.imv, J[2].j2);
+        R coupling23 = lin_mass(J[1].j0, A.imv, J[2].j0, J[1].j2, B.imv, J[2].j2);
+        c.em.m13 += coupling13;
+        c.em.m31 += coupling13;
+        c.em.m23 += coupling23;
+        c.em.m32 += coupling23;
+    }
+    c.em.m11 += c.p[11] * inv_sub;
+    c.em.m22 += c.p[11] * inv_sub;
+    c.em.m33 += c.p[12] * inv_sub;
+    if (c.clamp == 1 || c.clamp == 2) {
+        c.bilateral = inverse_bilateral_block(c.em);
+        c.cx = c.em.m13;
+        c.cy = c.em.m23;
+    }
+    if (c.clamp == 0) c.em.m33 = R(0);
+    c.em = inverse_symmetric(c.em);
+    c.bias = error * inv_dt;
+    c.bias.x *= c.p[9];
+    c.bias.y *= c.p[9];
+    c.bias.z *= c.p[10];
+    if (c.clamp == 1) c.s[2] = std::min(R(0), c.s[2]);
+    else if (c.clamp == 2) c.s[2] = std::max(R(0), c.s[2]);
+    V3<R> acc = vecp(c.s);
+    if (A.dynamic) {
+        *A.vel = *A.vel + cmul(J[0].j0 * acc.x + J[1].j0 * acc.y + J[2].j0 * acc.z, A.imv);
+        *A.angvel = *A.angvel + sym_mul(J[0].j1 * acc.x + J[1].j1 * acc.y + J[2].j1 * acc.z, A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(J[0].j2 * acc.x + J[1].j2 * acc.y + J[2].j2 * acc.z, B.imv);
+        *B.angvel = *B.angvel + sym_mul(J[0].j3 * acc.x + J[1].j3 * acc.y + J[2].j3 * acc.z, B.iw);
+    }
+}
+// IteratePointOnLine (:372-465)
+template <class R> void iterate_point_on_line(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                              R inv_sub) {
+    LinRow<R> J[3];
+    V3<R> unused;
+    point_on_line_rows(c, A, B, J, unused);
+    V3<R> jv;
+    jv.x = dot(J[0].j0, *A.vel) + dot(J[0].j1, *A.angvel) + dot(J[0].j2, *B.vel) + dot(J[0].j3, *B.angvel);
+    jv.y = dot(J[1].j0, *A.vel) + dot(J[1].j1, *A.angvel) + dot(J[1].j2, *B.vel) + dot(J[1].j3, *B.angvel);
+    jv.z = dot(J[2].j0, *A.vel) + dot(J[2].j1, *A.angvel) + dot(J[2].j2, *B.vel) + dot(J[2].j3, *B.angvel);
+    V3<R> acc = vecp(c.s);
+    V3<R> soft = acc * inv_sub;
+    soft.x *= c.p[11];
+    soft.y *= c.p[11];
+    soft.z *= c.p[12];
+    V3<R> residual = jv + c.bias + soft;
+    V3<R> lambda = -R(1.0) * mat_mul(residual, c.em);
+    V3<R> orig = acc;
+    acc = acc + lambda;
+    if ((c.clamp == 1 && acc.z > R(0)) || (c.clamp == 2 && acc.z < R(0))) {
+        lambda.z = -orig.z;
+        R x = residual.x + c.cx * lambda.z;
+        R y = residual.y + c.cy * lambda.z;
+        lambda.x = -(c.bilateral.x * x + c.bilateral.y * y);
+        lambda.y = -(c.bilateral.y * x + c.bilateral.z * y);
+        acc.x = orig.x + lambda.x;
+        acc.y = orig.y + lambda.y;
+        acc.z = R(0);
+    } else if (c.clamp == 0) {
+        acc.z = R(0.0);
+        orig.z = R(0.0);
+    }
+    setv(c.s, acc);
+    lambda = acc - orig;
+    if (A.dynamic) {
+        *A.vel = *A.vel + cmul(J[0].j0 * lambda.x + J[1].j0 * lambda.y + J[2].j0 * lambda.z, A.imv);
+        *A.angvel = *A.angvel + sym_mul(J[0].j1 * lambda.x + J[1].j1 * lambda.y + J[2].j1 * lambda.z, A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(J[0].j2 * lambda.x + J[1].j2 * lambda.y + J[2].j2 * lambda.z, B.imv);
+        *B.angvel = *B.angvel + sym_mul(J[0].j3 * lambda.x + J[1].j3 * lambda.y + J[2].j3 * lambda.z, B.iw);
+    }
+}
+
 template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                            R inv_sub, R sub_dt) {
     switch (c.type) {
@@ -748,7 +968,9 @@ template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& 
         case JH_CONSTRAINT_TWIST_ANGLE: prepare_twist_angle(c, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_CONE_LIMIT: prepare_cone_limit(c, A, B, inv_dt, inv_sub); break;
         case JH_CONSTRAINT_ANGULAR_MOTOR: prepare_motor(c, A, B, sub_dt, false); break;
-        default: prepare_motor(c, A, B, sub_dt, true); break;
+        case JH_CONSTRAINT_LINEAR_MOTOR: prepare_motor(c, A, B, sub_dt, true); break;
+        case JH_CONSTRAINT_POINT_ON_LINE: prepare_point_on_line(c, A, B, inv_dt, inv_sub); break;
+        default: prepare_point_on_plane(c, A, B, inv_dt, inv_sub); break;
     }
 }
 template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
@@ -760,7 +982,9 @@ template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& 
         case JH_CONSTRAINT_TWIST_ANGLE: iterate_twist_angle(c, A, B, inv_sub); break;
         case JH_CONSTRAINT_CONE_LIMIT: iterate_cone_limit(c, A, B, inv_sub); break;
         case JH_CONSTRAINT_ANGULAR_MOTOR: iterate_motor(c, A, B, false); break;
-        default: iterate_motor(c, A, B, true); break;
+        case JH_CONSTRAINT_LINEAR_MOTOR: iterate_motor(c, A, B, true); break;
+        case JH_CONSTRAINT_POINT_ON_LINE: iterate_point_on_line(c, A, B, inv_sub); break;
+        default: iterate_point_on_plane(c, A, B, inv_sub); break;
     }
 }
 

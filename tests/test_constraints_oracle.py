@@ -331,3 +331,71 @@ def test_linear_motor_drives_to_target_velocity():
         w.step(1.0 / 100.0)
     v = w.bodies()["velocity"][1]
     assert abs(v[0] * 0.998 ** 0 - 2.0) < 0.02 and abs(v[1]) < 1e-6
+
+
+@pytest.mark.parametrize("pinned", [True, False])
+def test_prismatic_joint_slides_on_its_axis_within_limits(pinned):
+    # a block on a prismatic joint along (1, 1, 0)/sqrt2 from a static post, travel limited to
+    # [-0.5, 1.5], pushed along the axis and across it, spun (pinned: the orientation is held)
+    sb = scenes.SceneBuilder()
+    post = static_post(sb, (0, 10, 0))
+    b = sb.body((0, 10, 2), velocity=(6.0, 2.0, -3.0), angular_velocity=(1.0, 2.0, 3.0))
+    sb.box(b, (0.5, 0.5, 0.5))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    axis = np.array([1.0, 1.0, 0.0]) / math.sqrt(2)
+    recs = C.prismatic_joint(bodies, post, b, (0, 10, 2), axis, limit=(-0.5, 1.5), pinned=pinned)
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array(recs))
+    travel, off, tilt = [], 0.0, 0.0
+    for _ in range(300):
+        w.step(1.0 / 120.0)
+        bb = w.bodies()
+        d = bb["position"][1].astype(float) - (0, 10, 2)
+        t = float(d @ axis)
+        travel.append(t)
+        off = max(off, float(np.linalg.norm(d - t * axis)))
+        if pinned:
+            q = bb["orientation"][1].astype(float)
+            tilt = max(tilt, 2 * math.acos(min(1.0, abs(q[3]))))
+    assert off < 0.02
+    assert min(travel) > -0.5 - 0.03 and max(travel) < 1.5 + 0.03
+    assert max(travel) > 1.4 or min(travel) < -0.4  # the limits are reached
+    if pinned:
+        assert tilt < math.radians(1.0)
+
+
+def test_point_on_plane_holds_the_plane_distance():
+    sb = scenes.SceneBuilder()
+    post = static_post(sb, (0, 10, 0))
+    b = sb.body((1, 10, 1), velocity=(2.0, 0.0, -1.0))
+    sb.sphere(b, 0.3)
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["substep_count"] = 4
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array([C.point_on_plane(bodies, post, b, (0, 1, 0), (0, 10, 0), (1, 10, 1))]))
+    ys = []
+    for _ in range(200):
+        w.step(1.0 / 100.0)
+        ys.append(float(w.bodies()["position"][1][1]))
+    assert max(abs(y - 10.0) for y in ys) < 0.02  # gravity pulls down, the plane holds it
+
+
+def test_prismatic_motor_drives_along_the_axis():
+    sb = scenes.SceneBuilder()
+    post = static_post(sb, (0, 10, 0))
+    b = sb.body((0, 10, 2))
+    sb.box(b, (0.5, 0.5, 0.5))
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["gravity"] = (0, 0, 0)
+    cfg["substep_count"] = 4
+    recs = C.prismatic_joint(bodies, post, b, (0, 10, 2), (0, 0, 1), motor=(1.5, 50.0))
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array(recs))
+    for _ in range(50):
+        w.step(1.0 / 100.0)
+    v = w.bodies()["velocity"][1]
+    assert abs(float(v[2]) - 1.5) < 0.02 and abs(float(v[0])) < 1e-3 and abs(float(v[1])) < 1e-3
