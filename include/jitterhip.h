@@ -180,11 +180,21 @@ typedef struct jh_arbiter {
  *   POINT_ON_LINE (PointOnLine.cs): params LocalAxis[3] (body-1 frame), LocalAnchor1[3], LocalAnchor2[3],
  *       BiasFactor, LimitBias, Softness, LimitSoftness, Min, Max; state AccumulatedImpulse[3].
  *   POINT_ON_PLANE (PointOnPlane.cs): params LocalAxis[3] (body-1 frame), LocalAnchor1[3], LocalAnchor2[3],
- *       BiasFactor, Softness, Min, Max; state AccumulatedImpulse. */
+ *       BiasFactor, Softness, Min, Max; state AccumulatedImpulse.
+ *   ANGULAR_SERVO (AngularServo.cs): params TargetRelativeOrientation[4] (x y z w), MaximumTorque,
+ *       Softness, BiasFactor; state AccumulatedImpulse[3], Hemisphere.
+ *   LINEAR_SERVO (LinearServo.cs): params LocalAnchor1[3], LocalAnchor2[3], TargetPosition[3] (body-1
+ *       frame), MaximumForce, Softness, BiasFactor; state AccumulatedImpulse[3].
+ *   LINEAR_AXIS_SERVO (LinearAxisServo.cs): params LocalAxis[3] (body-1 frame), LocalAnchor1[3],
+ *       LocalAnchor2[3], TargetPosition, MaximumForce, Softness, BiasFactor; state AccumulatedImpulse.
+ *   TWIST_SERVO (TwistServo.cs): params LocalAxis2[3] (body-2 frame), ReferenceOrientation[4],
+ *       TargetAngle, MaximumTorque, Softness, BiasFactor; state AccumulatedImpulse, Hemisphere.
+ * A servo's MaximumForce / MaximumTorque may be +inf (unlimited); it must not be negative or NaN. */
 enum { JH_CONSTRAINT_BALL_SOCKET = 0, JH_CONSTRAINT_HINGE_ANGLE = 1, JH_CONSTRAINT_FIXED_ANGLE = 2,
        JH_CONSTRAINT_DISTANCE_LIMIT = 3, JH_CONSTRAINT_TWIST_ANGLE = 4, JH_CONSTRAINT_CONE_LIMIT = 5,
        JH_CONSTRAINT_ANGULAR_MOTOR = 6, JH_CONSTRAINT_LINEAR_MOTOR = 7, JH_CONSTRAINT_POINT_ON_LINE = 8,
-       JH_CONSTRAINT_POINT_ON_PLANE = 9 };
+       JH_CONSTRAINT_POINT_ON_PLANE = 9, JH_CONSTRAINT_ANGULAR_SERVO = 10, JH_CONSTRAINT_LINEAR_SERVO = 11,
+       JH_CONSTRAINT_LINEAR_AXIS_SERVO = 12, JH_CONSTRAINT_TWIST_SERVO = 13 };
 
 typedef struct jh_constraint {
     int32_t type;           /* JH_CONSTRAINT_*                                                  */

@@ -11,9 +11,10 @@ import math
 
 import numpy as np
 
-from ._abi import (CONSTRAINT_ANGULAR_MOTOR, CONSTRAINT_BALL_SOCKET, CONSTRAINT_CONE_LIMIT, CONSTRAINT_DISTANCE_LIMIT,
-                   CONSTRAINT_DTYPE, CONSTRAINT_FIXED_ANGLE, CONSTRAINT_HINGE_ANGLE, CONSTRAINT_LINEAR_MOTOR,
-                   CONSTRAINT_POINT_ON_LINE, CONSTRAINT_POINT_ON_PLANE, CONSTRAINT_TWIST_ANGLE)
+from ._abi import (CONSTRAINT_ANGULAR_MOTOR, CONSTRAINT_ANGULAR_SERVO, CONSTRAINT_BALL_SOCKET, CONSTRAINT_CONE_LIMIT,
+                   CONSTRAINT_DISTANCE_LIMIT, CONSTRAINT_DTYPE, CONSTRAINT_FIXED_ANGLE, CONSTRAINT_HINGE_ANGLE,
+                   CONSTRAINT_LINEAR_AXIS_SERVO, CONSTRAINT_LINEAR_MOTOR, CONSTRAINT_LINEAR_SERVO,
+                   CONSTRAINT_POINT_ON_LINE, CONSTRAINT_POINT_ON_PLANE, CONSTRAINT_TWIST_ANGLE, CONSTRAINT_TWIST_SERVO)
 
 f32 = np.float32
 
@@ -300,6 +301,69 @@ def hinge_joint(bodies, b1, b2, anchor, axis, limit=(-math.pi, math.pi), motor=N
     if motor is not None:
         out.append(angular_motor(bodies, b1, b2, axis, axis, motor[0], motor[1]))
     return out
+
+
+def angular_servo(bodies, b1, b2, target=None, max_torque=0.0):
+    """AngularServo.Initialize() / Initialize(target) (AngularServo.cs:53-73): drives body 2's orientation
+    relative to body 1 toward target (x y z w; default the current relative orientation) with at most
+    max_torque (+inf: unlimited)."""
+    c = _record(CONSTRAINT_ANGULAR_SERVO, b1, b2)
+    p = c["params"]
+    if target is None:
+        target = _qmul(_conj(bodies["orientation"][b1]), bodies["orientation"][b2])
+    p[0:4] = np.asarray(target, np.float32)
+    p[4] = f32(max_torque)
+    p[5] = DEFAULT_ANGULAR_SOFTNESS
+    p[6] = DEFAULT_ANGULAR_BIAS
+    return c
+
+
+def linear_servo(bodies, b1, b2, anchor1, anchor2=None, target=(0.0, 0.0, 0.0), max_force=0.0):
+    """LinearServo.Initialize(anchor1, anchor2) (LinearServo.cs:55-71) with TargetPosition (body-1 frame
+    offset from anchor 1 to anchor 2) and MaximumForce."""
+    c = _record(CONSTRAINT_LINEAR_SERVO, b1, b2)
+    p = c["params"]
+    p[0:3] = _local_anchor(bodies, b1, anchor1)
+    p[3:6] = _local_anchor(bodies, b2, anchor1 if anchor2 is None else anchor2)
+    p[6:9] = np.asarray(target, np.float32)
+    p[9] = f32(max_force)
+    p[10] = DEFAULT_LINEAR_SOFTNESS
+    p[11] = DEFAULT_LINEAR_BIAS
+    return c
+
+
+def linear_axis_servo(bodies, b1, b2, axis, anchor1, anchor2, target=0.0, max_force=0.0):
+    """LinearAxisServo.Initialize(axis, anchor1, anchor2, targetPosition) (LinearAxisServo.cs:61-78): drives
+    the signed separation of anchor 2 from anchor 1 along axis (body-1 frame) toward target."""
+    c = _record(CONSTRAINT_LINEAR_AXIS_SERVO, b1, b2)
+    p = c["params"]
+    p[0:3] = _rot_inv(_normalize(axis), bodies["orientation"][b1])
+    p[3:6] = _local_anchor(bodies, b1, anchor1)
+    p[6:9] = _local_anchor(bodies, b2, anchor2)
+    p[9] = f32(target)
+    p[10] = f32(max_force)
+    p[11] = DEFAULT_LINEAR_SOFTNESS
+    p[12] = DEFAULT_LINEAR_BIAS
+    return c
+
+
+def twist_servo(bodies, b1, b2, axis1, axis2=None, target=0.0, max_torque=0.0):
+    """TwistServo.Initialize(axis1, axis2, targetAngle) (TwistServo.cs:69-88): drives the twist of body 2
+    about axis2 relative to the initial pose toward target (radians in [-pi, pi])."""
+    if not -math.pi <= target <= math.pi:
+        raise ValueError("twist servo target must be in [-pi, pi]")
+    c = _record(CONSTRAINT_TWIST_SERVO, b1, b2)
+    p = c["params"]
+    q1, q2 = bodies["orientation"][b1], bodies["orientation"][b2]
+    local2 = _rot_inv(_normalize(axis1 if axis2 is None else axis2), q2)
+    first_in_b2 = _rot_inv(_normalize(axis1), q2)
+    p[0:3] = local2
+    p[3:7] = _qmul(_from_to(first_in_b2, local2), _qmul(_conj(q2), q1))
+    p[7] = f32(target)
+    p[8] = f32(max_torque)
+    p[9] = DEFAULT_ANGULAR_SOFTNESS
+    p[10] = DEFAULT_ANGULAR_BIAS
+    return c
 
 
 def weld_joint(bodies, b1, b2, anchor):

@@ -1,5 +1,6 @@
 // Constraints on the device: BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit,
-// AngularMotor, LinearMotor, PointOnLine, PointOnPlane (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
+// AngularMotor, LinearMotor, PointOnLine, PointOnPlane, AngularServo, LinearServo, LinearAxisServo,
+// TwistServo (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
 // pieces they use, FP32 in the reference's operation order.  One thread solves one constraint; the
 // solver visits constraints in batches with disjoint dynamic bodies.
 #pragma once
@@ -939,6 +940,182 @@ __device__ __forceinline__ void iterate_point_on_line(jh_constraint& c, Work& k,
     }
 }
 
+// ---- Servos (Dynamics/Constraints/Internal/ServoMath.cs); the substep's maximum impulse is kept in cx --
+// MaximumImpulse (:30)
+__device__ __forceinline__ float servo_max_impulse(float maximum, float dt) { return dt > 0.0f ? maximum * dt : 0.0f; }
+__device__ __forceinline__ bool pos_inf(float v) { return v == __builtin_inff(); }
+// ClampMagnitude (:32-38)
+__device__ __forceinline__ vec3 servo_clamp_magnitude(vec3 impulse, float maximum) {
+    if (pos_inf(maximum)) return impulse;
+    float l2 = len_sq(impulse);
+    if (l2 > maximum * maximum) impulse = impulse * (maximum / sqrtf(l2));
+    return impulse;
+}
+// ClampAngular (:40-48)
+__device__ __forceinline__ vec3 servo_clamp_angular(vec3 impulse, const mat3& jac, float maximum) {
+    if (pos_inf(maximum)) return impulse;
+    float l2 = len_sq(tmat_mul(impulse, jac));
+    if (l2 > maximum * maximum) impulse = impulse * (maximum / sqrtf(l2));
+    return impulse;
+}
+
+// AngularServo.ApplyImpulse (AngularServo.cs:164-172)
+__device__ __forceinline__ void angular_servo_apply(CBody& A, CBody& B, vec3 impulse) {
+    if (A.dyn) A.w = A.w + sym_mul(impulse, A.I);
+    if (B.dyn) B.w = B.w - sym_mul(impulse, B.I);
+}
+// PrepareForIterationAngularServo (AngularServo.cs:125-149)
+__device__ __forceinline__ void prepare_angular_servo(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                      float inv_sub, float sub_dt) {
+    const float* p = c.params;
+    quat target{p[0], p[1], p[2], p[3]};
+    vec3 error;
+    int hemisphere;
+    k.jac = mscale(calc_jacobian(qmul(qconj(target), qconj(A.q)), B.q, error, hemisphere), -2.0f);
+    int old_h = int(c.state[3]);
+    if (old_h != 0 && old_h != hemisphere) c.state[0] = c.state[1] = c.state[2] = 0.0f;
+    c.state[3] = float(hemisphere);
+    k.em = sym_to_matrix(sym_rotate(sym_add(A.I, B.I), k.jac));
+    float softness = p[5] * inv_sub;
+    k.em.m11 += softness;
+    k.em.m22 += softness;
+    k.em.m33 += softness;
+    k.em = inverse_symmetric(k.em);
+    k.bias = -error * p[6] * inv_dt;
+    k.cx = servo_max_impulse(p[4], sub_dt);
+    vec3 acc = servo_clamp_angular(p3(c.state), k.jac, k.cx);
+    set3(c.state, acc);
+    angular_servo_apply(A, B, tmat_mul(acc, k.jac));
+}
+// IterateAngularServo (AngularServo.cs:151-162)
+__device__ __forceinline__ void iterate_angular_servo(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    vec3 jv = mat_mul(A.w - B.w, k.jac);
+    vec3 previous = p3(c.state);
+    vec3 softness = previous * (c.params[5] * inv_sub);
+    vec3 lambda = -mat_mul(jv + k.bias + softness, k.em);
+    vec3 acc = servo_clamp_angular(previous + lambda, k.jac, k.cx);
+    set3(c.state, acc);
+    angular_servo_apply(A, B, tmat_mul(acc - previous, k.jac));
+}
+
+// LinearServo.ApplyImpulse (LinearServo.cs:189-203)
+__device__ __forceinline__ void linear_servo_apply(Work& k, CBody& A, CBody& B, vec3 impulse) {
+    if (A.dyn) {
+        A.v = A.v - cmul(impulse, A.imv);
+        A.w = A.w - sym_mul(cross(k.r1, impulse), A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(impulse, B.imv);
+        B.w = B.w + sym_mul(cross(k.r2, impulse), B.I);
+    }
+}
+// PrepareForIterationLinearServo (LinearServo.cs:149-171)
+__device__ __forceinline__ void prepare_linear_servo(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                     float inv_sub, float sub_dt) {
+    const float* p = c.params;
+    k.r1 = rot(p3(p) + p3(p + 6), A.q);
+    k.r2 = rot(p3(p + 3), B.q);
+    mat3 cr1 = cross_matrix(k.r1), cr2 = cross_matrix(k.r2);
+    sym3 s = sym_add(sym_add(sym_add(sym_scale_vec(A.imv), sym_rotate(A.I, cr1)), sym_scale_vec(B.imv)),
+                     sym_rotate(B.I, cr2));
+    k.em = sym_to_matrix(s);
+    float softness = p[10] * inv_sub;
+    k.em.m11 += softness;
+    k.em.m22 += softness;
+    k.em.m33 += softness;
+    k.em = inverse_symmetric(k.em);
+    k.bias = (B.pos + k.r2 - A.pos - k.r1) * (p[11] * inv_dt);
+    k.cx = servo_max_impulse(p[9], sub_dt);
+    vec3 acc = servo_clamp_magnitude(p3(c.state), k.cx);
+    set3(c.state, acc);
+    linear_servo_apply(k, A, B, acc);
+}
+// IterateLinearServo (LinearServo.cs:173-187)
+__device__ __forceinline__ void iterate_linear_servo(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    vec3 jv = -A.v - cross(A.w, k.r1) + B.v + cross(B.w, k.r2);
+    vec3 previous = p3(c.state);
+    vec3 softness = previous * (c.params[10] * inv_sub);
+    vec3 lambda = -mat_mul(jv + k.bias + softness, k.em);
+    vec3 acc = servo_clamp_magnitude(previous + lambda, k.cx);
+    set3(c.state, acc);
+    linear_servo_apply(k, A, B, acc - previous);
+}
+
+// LinearAxisServo.ApplyImpulse (LinearAxisServo.cs:211-226)
+__device__ __forceinline__ void linear_axis_servo_apply(Work& k, CBody& A, CBody& B, float impulse) {
+    if (A.dyn) {
+        A.v = A.v + cmul(k.j0 * impulse, A.imv);
+        A.w = A.w + sym_mul(k.j1 * impulse, A.I);
+    }
+    if (B.dyn) {
+        B.v = B.v + cmul(k.j2 * impulse, B.imv);
+        B.w = B.w + sym_mul(k.j3 * impulse, B.I);
+    }
+}
+// PrepareForIterationLinearAxisServo (LinearAxisServo.cs:169-194)
+__device__ __forceinline__ void prepare_linear_axis_servo(jh_constraint& c, Work& k, CBody& A, CBody& B,
+                                                          float inv_dt, float inv_sub, float sub_dt) {
+    const float* p = c.params;
+    vec3 axis = rot(p3(p), A.q);
+    vec3 r1 = rot(p3(p + 3), A.q);
+    vec3 r2 = rot(p3(p + 6), B.q);
+    vec3 separation = B.pos + r2 - A.pos - r1;
+    k.j0 = -axis;
+    k.j1 = -cross(r1 + separation, axis);
+    k.j2 = axis;
+    k.j3 = cross(r2, axis);
+    float mass = dot(cmul(k.j0, A.imv), k.j0) + dot(cmul(k.j2, B.imv), k.j2) + dot(sym_mul(k.j1, A.I), k.j1) +
+                 dot(sym_mul(k.j3, B.I), k.j3) + p[11] * inv_sub;
+    k.em_s = mass > 0.0f ? 1.0f / mass : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    k.bias_s = (dot(separation, axis) - p[9]) * p[12] * inv_dt;
+    k.cx = servo_max_impulse(p[10], sub_dt);
+    c.state[0] = clamp_cs(c.state[0], -k.cx, k.cx);
+    linear_axis_servo_apply(k, A, B, c.state[0]);
+}
+// IterateLinearAxisServo (LinearAxisServo.cs:196-209)
+__device__ __forceinline__ void iterate_linear_axis_servo(jh_constraint& c, Work& k, CBody& A, CBody& B,
+                                                          float inv_sub) {
+    float jv = dot(k.j0, A.v) + dot(k.j1, A.w) + dot(k.j2, B.v) + dot(k.j3, B.w);
+    float lambda = -(jv + k.bias_s + c.state[0] * c.params[11] * inv_sub) * k.em_s;
+    float previous = c.state[0];
+    c.state[0] = clamp_cs(previous + lambda, -k.cx, k.cx);
+    linear_axis_servo_apply(k, A, B, c.state[0] - previous);
+}
+
+// TwistServo.ApplyImpulse (TwistServo.cs:184-192)
+__device__ __forceinline__ void twist_servo_apply(Work& k, CBody& A, CBody& B, float impulse) {
+    if (A.dyn) A.w = A.w + sym_mul(k.j0 * impulse, A.I);
+    if (B.dyn) B.w = B.w - sym_mul(k.j0 * impulse, B.I);
+}
+// PrepareForIterationTwistServo (TwistServo.cs:150-170)
+__device__ __forceinline__ void prepare_twist_servo(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                    float inv_sub, float sub_dt) {
+    const float* p = c.params;
+    quat ref{p[3], p[4], p[5], p[6]};
+    int hemisphere;
+    float angle = calc_twist(qmul(ref, qconj(A.q)), B.q, p3(p), k.j0, hemisphere);
+    int old_h = int(c.state[1]);
+    if (old_h != 0 && old_h != hemisphere) c.state[0] = 0.0f;
+    c.state[1] = float(hemisphere);
+    float mass = dot(sym_mul(k.j0, sym_add(A.I, B.I)), k.j0) + p[9] * inv_sub;
+    k.em_s = mass > 0.0f ? 1.0f / mass : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    float error = angle - p[7];
+    k.bias_s = error * p[10] * inv_dt;
+    k.cx = servo_max_impulse(p[8], sub_dt);
+    c.state[0] = clamp_cs(c.state[0], -k.cx, k.cx);
+    twist_servo_apply(k, A, B, c.state[0]);
+}
+// IterateTwistServo (TwistServo.cs:172-182)
+__device__ __forceinline__ void iterate_twist_servo(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    float jv = dot(A.w - B.w, k.j0);
+    float lambda = -k.em_s * (jv + k.bias_s + c.state[0] * c.params[9] * inv_sub);
+    float previous = c.state[0];
+    c.state[0] = clamp_cs(previous + lambda, -k.cx, k.cx);
+    twist_servo_apply(k, A, B, c.state[0] - previous);
+}
+
 __device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt, float inv_sub,
                                         float sub_dt) {
     switch (c.type) {
@@ -951,7 +1128,11 @@ __device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBo
         case JH_CONSTRAINT_ANGULAR_MOTOR: prepare_motor(c, k, A, B, sub_dt, false); break;
         case JH_CONSTRAINT_LINEAR_MOTOR: prepare_motor(c, k, A, B, sub_dt, true); break;
         case JH_CONSTRAINT_POINT_ON_LINE: prepare_point_on_line(c, k, A, B, inv_dt, inv_sub); break;
-        default: prepare_point_on_plane(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_POINT_ON_PLANE: prepare_point_on_plane(c, k, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_SERVO: prepare_angular_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
+        case JH_CONSTRAINT_LINEAR_SERVO: prepare_linear_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
+        case JH_CONSTRAINT_LINEAR_AXIS_SERVO: prepare_linear_axis_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
+        default: prepare_twist_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
     }
 }
 __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
@@ -965,7 +1146,11 @@ __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBo
         case JH_CONSTRAINT_ANGULAR_MOTOR: iterate_motor(c, k, A, B, false); break;
         case JH_CONSTRAINT_LINEAR_MOTOR: iterate_motor(c, k, A, B, true); break;
         case JH_CONSTRAINT_POINT_ON_LINE: iterate_point_on_line(c, k, A, B, inv_sub); break;
-        default: iterate_point_on_plane(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_POINT_ON_PLANE: iterate_point_on_plane(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_SERVO: iterate_angular_servo(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_LINEAR_SERVO: iterate_linear_servo(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_LINEAR_AXIS_SERVO: iterate_linear_axis_servo(c, k, A, B, inv_sub); break;
+        default: iterate_twist_servo(c, k, A, B, inv_sub); break;
     }
 }
 

@@ -1324,7 +1324,7 @@ jh_status jh_world_set_constraints(jh_world* w, const jh_constraint* in, int32_t
     if (!w || n < 0 || (n > 0 && !in)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
     for (int i = 0; i < n; ++i) {
         const jh_constraint& c = in[i];
-        if (c.type < JH_CONSTRAINT_BALL_SOCKET || c.type > JH_CONSTRAINT_POINT_ON_PLANE)
+        if (c.type < JH_CONSTRAINT_BALL_SOCKET || c.type > JH_CONSTRAINT_TWIST_SERVO)
             return fail(JH_ERR_INVALID_ARG, "constraint %d: unknown type %d", i, c.type);
         if (c.body1 < 0 || c.body1 >= w->nb || c.body2 < 0 || c.body2 >= w->nb)
             return fail(JH_ERR_INVALID_ARG, "constraint %d: body index out of range", i);
@@ -1332,6 +1332,14 @@ jh_status jh_world_set_constraints(jh_world* w, const jh_constraint* in, int32_t
             return fail(JH_ERR_INVALID_ARG, "constraint %d: body1 and body2 are the same body", i);
         for (int k = 0; k < 16; ++k)
             if (std::isnan(c.params[k])) return fail(JH_ERR_NON_FINITE, "constraint %d: NaN parameter", i);
+        // ServoMath.CheckMaximum (Internal/ServoMath.cs:15-20): a servo's force or torque limit is >= 0
+        int imax = c.type == JH_CONSTRAINT_ANGULAR_SERVO       ? 4
+                   : c.type == JH_CONSTRAINT_LINEAR_SERVO      ? 9
+                   : c.type == JH_CONSTRAINT_LINEAR_AXIS_SERVO ? 10
+                   : c.type == JH_CONSTRAINT_TWIST_SERVO       ? 8
+                                                               : -1;
+        if (imax >= 0 && c.params[imax] < 0.0f)
+            return fail(JH_ERR_INVALID_ARG, "constraint %d: negative servo maximum", i);
     }
     w->host_con.assign(in, in + n);
     size_t cap = std::max(n, 1);

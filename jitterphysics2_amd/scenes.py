@@ -512,8 +512,9 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
     spheres (BallSocket chain), welded box pairs (WeldJoint) dropped on the floor, capsules tied to a
     post by distance limits with a slack range, a ragdoll-like limb chain (BallSocket + ConeLimit +
     TwistAngle), motor-driven wheels on hinges (AngularMotor) with a block pulled by a LinearMotor,
-    prismatic sliders (PointOnLine + FixedAngle or HingeAngle, limits, motors), and universal-joint chains
-    with a body held on a plane (PointOnPlane); free boxes fall between them.  Returns (bodies, shapes,
+    prismatic sliders (PointOnLine + FixedAngle or HingeAngle, limits, motors), universal-joint chains
+    with a body held on a plane (PointOnPlane), and servos (TwistServo on a hinged arm, AngularServo on a
+    ball-socketed box, LinearServo and LinearAxisServo on free blocks); free boxes fall between them.  Returns (bodies, shapes,
     verts, cfg, dt, constraints).  dt = 1/60, 4 substeps, 10 iterations."""
     from . import constraints as C
 
@@ -525,7 +526,7 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
     side = int(math.ceil(math.sqrt(n_groups)))
     for g in range(n_groups):
         gx, gz = (g % side - side / 2.0) * 8.0, (g // side - side / 2.0) * 8.0
-        kind = g % 8
+        kind = g % 9
         top = 14.0
         if kind != 2:
             post = _static_box(sb, (gx, top + 0.5, gz), (0.4, 1.0, 0.4))
@@ -584,6 +585,20 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
             b = sb.body((gx + 2.0, top - 2.0, gz + 1.0), velocity=(1.0, 0.0, -1.0))
             sb.sphere(b, 0.3)
             plan.append(("plane", post, b, (gx + 2.0, top - 2.0, gz + 1.0)))
+        elif kind == 8:  # servos: a twist-driven arm, an oriented box, a block pulled to a point and along an axis
+            arm = sb.body((gx, top - 0.55, gz), angular_velocity=(0.0, rng.normal(), 0.0))
+            sb.box(arm, (0.3, 1.0, 0.3))
+            plan.append(("twist_servo", post, arm, (gx, top, gz)))
+            b = sb.body((gx + 2.0, top - 1.0, gz), orientation=rng.quaternion(),
+                        angular_velocity=(rng.normal(), rng.normal(), rng.normal()))
+            sb.box(b, (0.6, 0.6, 0.6))
+            plan.append(("angular_servo", post, b, (gx + 2.0, top - 1.0, gz)))
+            b = sb.body((gx - 2.0, top - 2.0, gz + 1.0), velocity=(rng.normal(), 0.0, rng.normal()))
+            sb.box(b, (0.5, 0.5, 0.5))
+            plan.append(("linear_servo", post, b, (gx - 2.0, top - 2.0, gz + 1.0)))
+            b = sb.body((gx, top - 3.0, gz - 2.0))
+            sb.sphere(b, 0.3)
+            plan.append(("axis_servo", post, b, (gx, top - 3.0, gz - 2.0)))
         else:  # capsules on slack distance limits
             for k in range(links // 2):
                 b = sb.body((gx + rng.uniform(-2, 2), top - 2.0 - 1.5 * k, gz + rng.uniform(-2, 2)),
@@ -605,6 +620,17 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
             recs += C.universal_joint(bodies, a, b, anchor, (1.0, 0.0, 0.0), (0.0, 0.0, 1.0))
         elif kind == "plane":
             recs.append(C.point_on_plane(bodies, a, b, (0.0, 1.0, 0.0), bodies["position"][a], anchor))
+        elif kind == "twist_servo":  # hinge about y driven to +1 rad with a torque limit
+            recs += C.hinge_joint(bodies, a, b, anchor, (0.0, 1.0, 0.0))
+            recs.append(C.twist_servo(bodies, a, b, (0.0, 1.0, 0.0), target=1.0, max_torque=40.0))
+        elif kind == "angular_servo":  # ball socket; servo to the post's orientation, unlimited torque
+            recs.append(C.ball_socket(bodies, a, b, anchor))
+            recs.append(C.angular_servo(bodies, a, b, target=(0.0, 0.0, 0.0, 1.0), max_torque=math.inf))
+        elif kind == "linear_servo":  # pulled 1 m toward the post's side, limited force
+            recs.append(C.linear_servo(bodies, a, b, anchor, target=(1.0, 0.0, -1.0), max_force=60.0))
+        elif kind == "axis_servo":  # held 2 m below the post along y (the other directions are free)
+            recs.append(C.linear_axis_servo(bodies, a, b, (0.0, 1.0, 0.0), bodies["position"][a], anchor,
+                                            target=-4.0, max_force=math.inf))
         elif kind == "hinge":
             recs += C.hinge_joint(bodies, a, b, anchor, (1.0, 0.0, 0.0), (-math.pi / 3, math.pi / 3))
         elif kind == "ball":

@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only (see orc_math.hpp).  Constraints solved with the contacts —
 // BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit, AngularMotor, LinearMotor,
-// PointOnLine, PointOnPlane (Dynamics/Constraints/*.cs) — and the JMatrix,
+// PointOnLine, PointOnPlane, AngularServo, LinearServo, LinearAxisServo, TwistServo (Dynamics/Constraints/*.cs) — and the JMatrix,
 // JSymmetricMatrix, MathHelper and AngularConstraintMath pieces they use, restated in the reference's
 // operation order and templated on the precision R.
 #pragma once
@@ -958,6 +958,191 @@ template <class R> void iterate_point_on_line(Constraint<R>& c, ConstraintBody<R
     }
 }
 
+// ---- Servos (Dynamics/Constraints/Internal/ServoMath.cs) ----------------------------------------
+// MaximumImpulse (:30): the largest impulse a force or torque limit allows in one substep
+template <class R> inline R servo_max_impulse(R maximum, R dt) { return dt > R(0) ? maximum * dt : R(0); }
+// ClampMagnitude (:32-38)
+template <class R> inline V3<R> servo_clamp_magnitude(V3<R> impulse, R maximum) {
+    if (std::isinf(maximum) && maximum > R(0)) return impulse;
+    R l2 = len_sq(impulse);
+    if (l2 > maximum * maximum) impulse = impulse * (maximum / std::sqrt(l2));
+    return impulse;
+}
+// ClampAngular (:40-48): limits the angular impulse J^T lambda a body receives
+template <class R> inline V3<R> servo_clamp_angular(V3<R> impulse, const M3<R>& jac, R maximum) {
+    if (std::isinf(maximum) && maximum > R(0)) return impulse;
+    R l2 = len_sq(tmat_mul(impulse, jac));
+    if (l2 > maximum * maximum) impulse = impulse * (maximum / std::sqrt(l2));
+    return impulse;
+}
+
+// AngularServo (AngularServo.cs): params TargetRelativeOrientation (0-3, x y z w), MaximumTorque (4),
+// Softness (5), BiasFactor (6); state AccumulatedImpulse (0-2), Hemisphere (3).  The substep's maximum
+// impulse is kept in cx.
+template <class R> inline void angular_servo_apply(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                   V3<R> impulse) {  // ApplyImpulse (:164-172)
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(impulse, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(impulse, B.iw);
+}
+// PrepareForIterationAngularServo (:125-149)
+template <class R> void prepare_angular_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                              R inv_sub, R sub_dt) {
+    Q4<R> target{c.p[0], c.p[1], c.p[2], c.p[3]};
+    V3<R> error;
+    int hemisphere;
+    c.jac = mscale(calc_jacobian(qmul(qconj(target), qconj(A.q)), B.q, error, hemisphere), -R(2));
+    int old_h = int(c.s[3]);
+    if (old_h != 0 && old_h != hemisphere) c.s[0] = c.s[1] = c.s[2] = R(0);
+    c.s[3] = R(hemisphere);
+    c.em = sym_to_matrix(sym_transform(sym_add(A.iw, B.iw), c.jac));
+    R softness = c.p[5] * inv_sub;
+    c.em.m11 += softness;
+    c.em.m22 += softness;
+    c.em.m33 += softness;
+    c.em = inverse_symmetric(c.em);
+    c.bias = -error * c.p[6] * inv_dt;
+    c.cx = servo_max_impulse(c.p[4], sub_dt);
+    V3<R> acc = servo_clamp_angular(vecp(c.s), c.jac, c.cx);
+    setv(c.s, acc);
+    angular_servo_apply(c, A, B, tmat_mul(acc, c.jac));
+}
+// IterateAngularServo (:151-162)
+template <class R> void iterate_angular_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                              R inv_sub) {
+    V3<R> jv = mat_mul(*A.angvel - *B.angvel, c.jac);
+    V3<R> previous = vecp(c.s);
+    V3<R> softness = previous * (c.p[5] * inv_sub);
+    V3<R> lambda = -mat_mul(jv + c.bias + softness, c.em);
+    V3<R> acc = servo_clamp_angular(previous + lambda, c.jac, c.cx);
+    setv(c.s, acc);
+    angular_servo_apply(c, A, B, tmat_mul(acc - previous, c.jac));
+}
+
+// LinearServo (LinearServo.cs): params LocalAnchor1 (0-2), LocalAnchor2 (3-5), TargetPosition (6-8, body-1
+// frame), MaximumForce (9), Softness (10), BiasFactor (11); state AccumulatedImpulse (0-2).  The substep's
+// maximum impulse is kept in cx.
+template <class R> inline void linear_servo_apply(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                  V3<R> impulse) {  // ApplyImpulse (:189-203)
+    if (A.dynamic) {
+        *A.vel = *A.vel - cmul(impulse, A.imv);
+        *A.angvel = *A.angvel - sym_mul(cross(c.r1, impulse), A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(impulse, B.imv);
+        *B.angvel = *B.angvel + sym_mul(cross(c.r2, impulse), B.iw);
+    }
+}
+// PrepareForIterationLinearServo (:149-171)
+template <class R> void prepare_linear_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                             R inv_sub, R sub_dt) {
+    c.r1 = rotate(vecp(c.p) + vecp(c.p + 6), A.q);
+    c.r2 = rotate(vecp(c.p + 3), B.q);
+    M3<R> cr1 = cross_matrix(c.r1), cr2 = cross_matrix(c.r2);
+    Sym<R> s = sym_add(sym_add(sym_add(sym_scale_vec(A.imv), sym_transform(A.iw, cr1)), sym_scale_vec(B.imv)),
+                       sym_transform(B.iw, cr2));
+    c.em = sym_to_matrix(s);
+    R softness = c.p[10] * inv_sub;
+    c.em.m11 += softness;
+    c.em.m22 += softness;
+    c.em.m33 += softness;
+    c.em = inverse_symmetric(c.em);
+    c.bias = (B.pos + c.r2 - A.pos - c.r1) * (c.p[11] * inv_dt);
+    c.cx = servo_max_impulse(c.p[9], sub_dt);
+    V3<R> acc = servo_clamp_magnitude(vecp(c.s), c.cx);
+    setv(c.s, acc);
+    linear_servo_apply(c, A, B, acc);
+}
+// IterateLinearServo (:173-187)
+template <class R> void iterate_linear_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                             R inv_sub) {
+    V3<R> jv = -*A.vel - cross(*A.angvel, c.r1) + *B.vel + cross(*B.angvel, c.r2);
+    V3<R> previous = vecp(c.s);
+    V3<R> softness = previous * (c.p[10] * inv_sub);
+    V3<R> lambda = -mat_mul(jv + c.bias + softness, c.em);
+    V3<R> acc = servo_clamp_magnitude(previous + lambda, c.cx);
+    setv(c.s, acc);
+    linear_servo_apply(c, A, B, acc - previous);
+}
+
+// LinearAxisServo (LinearAxisServo.cs): params LocalAxis (0-2, body-1 frame), LocalAnchor1 (3-5),
+// LocalAnchor2 (6-8), TargetPosition (9), MaximumForce (10), Softness (11), BiasFactor (12); state
+// AccumulatedImpulse (0).  The substep's maximum impulse is kept in cx.
+template <class R> inline void linear_axis_servo_apply(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                       R impulse) {  // ApplyImpulse (:211-226)
+    if (A.dynamic) {
+        *A.vel = *A.vel + cmul(c.j0 * impulse, A.imv);
+        *A.angvel = *A.angvel + sym_mul(c.j1 * impulse, A.iw);
+    }
+    if (B.dynamic) {
+        *B.vel = *B.vel + cmul(c.j2 * impulse, B.imv);
+        *B.angvel = *B.angvel + sym_mul(c.j3 * impulse, B.iw);
+    }
+}
+// PrepareForIterationLinearAxisServo (:169-194)
+template <class R> void prepare_linear_axis_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                  R inv_dt, R inv_sub, R sub_dt) {
+    V3<R> axis = rotate(vecp(c.p), A.q);
+    V3<R> r1 = rotate(vecp(c.p + 3), A.q);
+    V3<R> r2 = rotate(vecp(c.p + 6), B.q);
+    V3<R> separation = B.pos + r2 - A.pos - r1;
+    c.j0 = -axis;
+    c.j1 = -cross(r1 + separation, axis);
+    c.j2 = axis;
+    c.j3 = cross(r2, axis);
+    R mass = dot(cmul(c.j0, A.imv), c.j0) + dot(cmul(c.j2, B.imv), c.j2) + dot(sym_mul(c.j1, A.iw), c.j1) +
+             dot(sym_mul(c.j3, B.iw), c.j3) + c.p[11] * inv_sub;
+    c.em_s = mass > R(0) ? R(1) / mass : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    c.bias_s = (dot(separation, axis) - c.p[9]) * c.p[12] * inv_dt;
+    c.cx = servo_max_impulse(c.p[10], sub_dt);
+    c.s[0] = clamp_cs(c.s[0], -c.cx, c.cx);
+    linear_axis_servo_apply(c, A, B, c.s[0]);
+}
+// IterateLinearAxisServo (:196-209)
+template <class R> void iterate_linear_axis_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                  R inv_sub) {
+    R jv = dot(c.j0, *A.vel) + dot(c.j1, *A.angvel) + dot(c.j2, *B.vel) + dot(c.j3, *B.angvel);
+    R lambda = -(jv + c.bias_s + c.s[0] * c.p[11] * inv_sub) * c.em_s;
+    R previous = c.s[0];
+    c.s[0] = clamp_cs(previous + lambda, -c.cx, c.cx);
+    linear_axis_servo_apply(c, A, B, c.s[0] - previous);
+}
+
+// TwistServo (TwistServo.cs): params LocalAxis2 (0-2, body-2 frame), ReferenceOrientation (3-6, x y z w),
+// TargetAngle (7), MaximumTorque (8), Softness (9), BiasFactor (10); state AccumulatedImpulse (0),
+// Hemisphere (1).  The substep's maximum impulse is kept in cx.
+template <class R> inline void twist_servo_apply(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                 R impulse) {  // ApplyImpulse (:184-192)
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(c.j0 * impulse, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(c.j0 * impulse, B.iw);
+}
+// PrepareForIterationTwistServo (:150-170)
+template <class R> void prepare_twist_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
+                                            R inv_sub, R sub_dt) {
+    Q4<R> ref{c.p[3], c.p[4], c.p[5], c.p[6]};
+    int hemisphere;
+    R angle = calc_twist(qmul(ref, qconj(A.q)), B.q, vecp(c.p), c.j0, hemisphere);
+    int old_h = int(c.s[1]);
+    if (old_h != 0 && old_h != hemisphere) c.s[0] = R(0);
+    c.s[1] = R(hemisphere);
+    R mass = dot(sym_mul(c.j0, sym_add(A.iw, B.iw)), c.j0) + c.p[9] * inv_sub;
+    c.em_s = mass > R(0) ? R(1) / mass : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    R error = angle - c.p[7];
+    c.bias_s = error * c.p[10] * inv_dt;
+    c.cx = servo_max_impulse(c.p[8], sub_dt);
+    c.s[0] = clamp_cs(c.s[0], -c.cx, c.cx);
+    twist_servo_apply(c, A, B, c.s[0]);
+}
+// IterateTwistServo (:172-182)
+template <class R> void iterate_twist_servo(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
+    R jv = dot(*A.angvel - *B.angvel, c.j0);
+    R lambda = -c.em_s * (jv + c.bias_s + c.s[0] * c.p[9] * inv_sub);
+    R previous = c.s[0];
+    c.s[0] = clamp_cs(previous + lambda, -c.cx, c.cx);
+    twist_servo_apply(c, A, B, c.s[0] - previous);
+}
+
 template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                            R inv_sub, R sub_dt) {
     switch (c.type) {
@@ -970,7 +1155,11 @@ template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& 
         case JH_CONSTRAINT_ANGULAR_MOTOR: prepare_motor(c, A, B, sub_dt, false); break;
         case JH_CONSTRAINT_LINEAR_MOTOR: prepare_motor(c, A, B, sub_dt, true); break;
         case JH_CONSTRAINT_POINT_ON_LINE: prepare_point_on_line(c, A, B, inv_dt, inv_sub); break;
-        default: prepare_point_on_plane(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_POINT_ON_PLANE: prepare_point_on_plane(c, A, B, inv_dt, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_SERVO: prepare_angular_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
+        case JH_CONSTRAINT_LINEAR_SERVO: prepare_linear_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
+        case JH_CONSTRAINT_LINEAR_AXIS_SERVO: prepare_linear_axis_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
+        default: prepare_twist_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
     }
 }
 template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
@@ -984,7 +1173,11 @@ template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& 
         case JH_CONSTRAINT_ANGULAR_MOTOR: iterate_motor(c, A, B, false); break;
         case JH_CONSTRAINT_LINEAR_MOTOR: iterate_motor(c, A, B, true); break;
         case JH_CONSTRAINT_POINT_ON_LINE: iterate_point_on_line(c, A, B, inv_sub); break;
-        default: iterate_point_on_plane(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_POINT_ON_PLANE: iterate_point_on_plane(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_ANGULAR_SERVO: iterate_angular_servo(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_LINEAR_SERVO: iterate_linear_servo(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_LINEAR_AXIS_SERVO: iterate_linear_axis_servo(c, A, B, inv_sub); break;
+        default: iterate_twist_servo(c, A, B, inv_sub); break;
     }
 }
 

@@ -1,6 +1,6 @@
-"""GPU parity of the constraint solver (BallSocket, HingeAngle, FixedAngle, DistanceLimit in the
-barrier solver's batches after the contacts of each pass) against the oracle replaying the GPU's
-contact and constraint orders: bodies, contacts and constraint impulses bit for bit in FP32."""
+"""GPU parity of the constraint solver (the fourteen constraint and servo kinds, solved in the barrier
+solver's batches after the contacts of each pass) against the oracle replaying the GPU's contact and
+constraint orders: bodies, contacts and constraint impulses bit for bit in FP32."""
 import numpy as np
 import pytest
 
@@ -68,6 +68,9 @@ def test_constraint_input_errors():
     bad["body1"][2] = len(bodies)
     with pytest.raises(RuntimeError, match="invalid argument"):
         gw.set_constraints(bad)
+    servo = C.as_array([C.linear_servo(bodies, 0, 1, bodies["position"][0], max_force=-1.0)])
+    with pytest.raises(RuntimeError, match="invalid argument"):  # ServoMath.CheckMaximum
+        gw.set_constraints(servo)
     gw.set_constraints(cons)  # the valid set is accepted after the rejected ones
     assert len(gw.constraints()) == len(cons)
 

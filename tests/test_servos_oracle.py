@@ -1,0 +1,280 @@
+"""Servos (AngularServo, LinearServo, LinearAxisServo, TwistServo) in the oracle, pinned by the outcomes
+the reference's own tests assert (JitterTests/Constraints/ServoTests.cs): capped acceleration over
+elapsed time, zero maximum, unlimited servos matching the equivalent constraint, tracking of targets in
+body 1's frame, the axis servo leaving the other directions free, the angular servo taking the short
+way and correcting a half turn, the twist servo keeping the signed angle difference.
+
+The reference's tests run PrepareForIteration + Iterate directly on a TimeStep; here one world step
+with gravity off, no damping and no contacts does the same (its velocity change is that of one
+prepare followed by the iterations).  world.NullBody is a static body at the origin without shapes.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from jitterphysics2_amd import constraints as C
+from jitterphysics2_amd import scenes
+from jitterphysics2_amd._abi import MOTION_KINEMATIC, default_config
+from tests.orc import OracleWorld
+
+KINDS = ["angular", "twist", "linear", "axis"]
+
+
+def axis_angle(axis, angle):
+    a = np.asarray(axis, float)
+    a = a / np.linalg.norm(a)
+    s = math.sin(angle / 2)
+    return np.array([a[0] * s, a[1] * s, a[2] * s, math.cos(angle / 2)], np.float32)
+
+
+def rot_x(angle):
+    return axis_angle((1, 0, 0), angle)
+
+
+def qmul(a, b):
+    ax, ay, az, aw = (float(t) for t in a)
+    bx, by, bz, bw = (float(t) for t in b)
+    return np.array([aw * bx + bw * ax + ay * bz - az * by, aw * by + bw * ay + az * bx - ax * bz,
+                     aw * bz + bw * az + ax * by - ay * bx, aw * bw - (ax * bx + ay * by + az * bz)])
+
+
+def quat_rotate(v, q):
+    x, y, z, w = (float(t) for t in q)
+    u = np.array([x, y, z])
+    v = np.asarray(v, float)
+    return v + 2 * w * np.cross(u, v) + 2 * np.cross(u, np.cross(u, v))
+
+
+def unit_bodies(second=(0.0, 0.0, 0.0), q2=(0, 0, 0, 1), first_motion=scenes.MOTION_STATIC, first_pos=(0, 0, 0),
+                first_q=(0, 0, 0, 1)):
+    """world.NullBody (or a kinematic reference) and a body with unit mass and inertia, no damping
+    (ServoTests.CreateBody)."""
+    sb = scenes.SceneBuilder()
+    a = sb.body(first_pos, orientation=first_q, motion=first_motion)
+    sb.bodies[a]["inverse_mass"] = 0.0
+    sb.bodies[a]["inverse_inertia"] = (0, 0, 0, 0, 0, 0)
+    b = sb.body(second, orientation=q2)
+    for i in (a, b):
+        sb.bodies[i]["linear_damping_multiplier"] = 1.0
+        sb.bodies[i]["angular_damping_multiplier"] = 1.0
+    bodies, shapes, verts = sb.arrays()
+    return bodies, shapes, verts
+
+
+def make_servo(bodies, kind, maximum):
+    """ServoTests.CreateServo: targets far out of reach so that the limit decides."""
+    if kind == "angular":
+        return C.angular_servo(bodies, 0, 1, target=axis_angle((1, 2, 3), 2.8), max_torque=maximum)
+    if kind == "twist":
+        return C.twist_servo(bodies, 0, 1, (1, 0, 0), target=1.5, max_torque=maximum)
+    if kind == "linear":
+        p0, p1 = bodies["position"][0], bodies["position"][1]
+        return C.linear_servo(bodies, 0, 1, p0, p1, target=(1000, 2000, 3000), max_force=maximum)
+    p0, p1 = bodies["position"][0], bodies["position"][1]
+    return C.linear_axis_servo(bodies, 0, 1, (1, 0, 0), p0, p1, target=1000.0, max_force=maximum)
+
+
+def world(bodies, shapes, verts, recs, substeps=1, iterations=12):
+    cfg = default_config()
+    cfg["gravity"] = (0.0, 0.0, 0.0)
+    cfg["substep_count"] = substeps
+    cfg["solver_iterations"] = iterations
+    cfg["relaxation_iterations"] = 0
+    w = OracleWorld(bodies, shapes, verts, cfg)
+    w.set_constraints(C.as_array(recs))
+    return w
+
+
+def speed(w, kind):
+    b = w.bodies()
+    v = b["angular_velocity"][1] if kind in ("angular", "twist") else b["velocity"][1]
+    return float(np.linalg.norm(v.astype(float)))
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_capped_acceleration_uses_elapsed_time(kind):  # ServoTests.cs:66-90
+    for steps in (30, 120):
+        for substeps in (1, 4):
+            for iterations in (1, 12):
+                bodies, shapes, verts = unit_bodies()
+                w = world(bodies, shapes, verts, [make_servo(bodies, kind, 2.0)], substeps, iterations)
+                for _ in range(steps):
+                    w.step(1.0 / steps)
+                # maximum 2 on a unit mass / inertia for 1 s of simulated time: speed 2
+                assert speed(w, kind) == pytest.approx(2.0, abs=1e-4), (steps, substeps, iterations)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_zero_maximum_applies_no_impulse(kind):  # ServoTests.cs:348-359
+    bodies, shapes, verts = unit_bodies()
+    bodies["velocity"][1] = (1, 2, 3)
+    bodies["angular_velocity"][1] = (3, 2, 1)
+    w = world(bodies, shapes, verts, [make_servo(bodies, kind, 0.0)])
+    w.step(0.01)
+    b = w.bodies()
+    assert np.array_equal(b["velocity"][1], np.float32([1, 2, 3]))
+    assert np.array_equal(b["angular_velocity"][1], np.float32([3, 2, 1]))
+
+
+def equivalent_constraint(bodies, kind, servo):
+    """UnlimitedServoMatchesExistingConstraintResponse (ServoTests.cs:361-405): FixedAngle at the target
+    orientation, TwistAngle fixed at the target angle, BallSocket at the target offset, PointOnPlane with
+    the target as both limits."""
+    if kind == "angular":
+        b2 = bodies.copy()
+        b2["orientation"][1] = servo["params"][0:4]
+        c = C.fixed_angle(b2, 0, 1)
+        return c
+    if kind == "twist":
+        t = float(servo["params"][7])
+        return C.twist_angle(bodies, 0, 1, (1, 0, 0), (1, 0, 0), (t, t))
+    if kind == "linear":
+        c = C.ball_socket(bodies, 0, 1, servo["params"][6:9])
+        c["params"][3:6] = C._local_anchor(bodies, 1, (0.0, 0.0, 0.0))  # socket.Anchor2 = JVector.Zero
+        return c
+    t = float(servo["params"][9])
+    return C.point_on_plane(bodies, 0, 1, (1, 0, 0), (0, 0, 0), (0, 0, 0), (t, t))
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_unlimited_servo_matches_the_equivalent_constraint(kind):
+    bodies, shapes, verts = unit_bodies()
+    bodies["velocity"][1] = (2, 3, -4)
+    bodies["angular_velocity"][1] = (-3, 1, 2)
+    servo = make_servo(bodies, kind, math.inf)
+    ws = world(bodies, shapes, verts, [servo], substeps=4)
+    wc = world(bodies, shapes, verts, [equivalent_constraint(bodies, kind, servo)], substeps=4)
+    ws.step(0.01)
+    wc.step(0.01)
+    bs, bc = ws.bodies(), wc.bodies()
+    assert np.linalg.norm(bs["velocity"][1] - bc["velocity"][1]) < 1e-4
+    assert np.linalg.norm(bs["angular_velocity"][1] - bc["angular_velocity"][1]) < 1e-4
+    # and the servo really acted
+    assert not np.array_equal(bs["velocity"][1], np.float32([2, 3, -4])) or \
+        not np.array_equal(bs["angular_velocity"][1], np.float32([-3, 1, 2]))
+
+
+def test_angular_servo_tracks_targets_in_body1_frame():  # ServoTests.cs:115-138
+    ref_q = qmul(axis_angle((0, 1, 0), 0.7), axis_angle((1, 0, 0), 0.3)).astype(np.float32)
+    bodies, shapes, verts = unit_bodies(q2=axis_angle((0, 0, 1), -0.8), first_motion=MOTION_KINEMATIC, first_q=ref_q)
+    target = qmul(axis_angle((1, 0, 0), 0.9), axis_angle((0, 0, 1), 0.5)).astype(np.float32)
+    w = world(bodies, shapes, verts, [C.angular_servo(bodies, 0, 1, target=target, max_torque=20.0)], substeps=4)
+    for change in range(2):
+        for _ in range(200):
+            w.step(0.01)
+        b = w.bodies()
+        expected = qmul(b["orientation"][0], target)
+        assert abs(float(np.dot(expected, b["orientation"][1].astype(float)))) == pytest.approx(1.0, abs=1e-5)
+        target = qmul(axis_angle((0, 0, 1), -0.6), axis_angle((0, 1, 0), 0.4)).astype(np.float32)
+        cons = w.constraints()
+        cons["params"][0][0:4] = target
+        cons["state"][0][:] = 0  # the TargetRelativeOrientation setter resets hemisphere and warm start
+        w.set_constraints(cons)
+
+
+def test_linear_servo_tracks_offsets_in_body1_frame():  # ServoTests.cs:140-163
+    ref_q = axis_angle((0, 0, 1), 0.7)
+    bodies, shapes, verts = unit_bodies(second=(-1, 2, 3), first_motion=MOTION_KINEMATIC, first_pos=(4, 3, 2),
+                                        first_q=ref_q)
+    servo = C.linear_servo(bodies, 0, 1, bodies["position"][0], bodies["position"][1], max_force=100.0)
+    w = world(bodies, shapes, verts, [servo], substeps=4)
+    for change in range(2):
+        target = (0.0, 0.0, 0.0) if change == 0 else (1.0, 2.0, -1.0)
+        cons = w.constraints()
+        cons["params"][0][6:9] = target
+        w.set_constraints(cons)
+        for _ in range(300):
+            w.step(0.01)
+        b = w.bodies()
+        expected = b["position"][0].astype(float) + quat_rotate(target, b["orientation"][0])
+        assert np.linalg.norm(b["position"][1].astype(float) - expected) < 1e-3
+
+
+def test_twist_servo_coexists_with_hinge_limits():  # ServoTests.cs:165-180
+    bodies, shapes, verts = unit_bodies()
+    hinge = C.hinge_angle(bodies, 0, 1, (1, 0, 0), (-0.5, 0.5))
+    servo = C.twist_servo(bodies, 0, 1, (1, 0, 0), target=0.9, max_torque=5.0)
+    w = world(bodies, shapes, verts, [hinge, servo], substeps=4)
+
+    def hinge_angle():
+        q = w.bodies()["orientation"][1].astype(float)
+        return 2 * math.atan2(q[0], q[3])
+
+    for _ in range(300):
+        w.step(0.01)
+    assert hinge_angle() == pytest.approx(0.5, abs=0.01)
+    cons = w.constraints()
+    cons["params"][1][7] = -0.25
+    w.set_constraints(cons)
+    for _ in range(300):
+        w.step(0.01)
+    assert hinge_angle() == pytest.approx(-0.25, abs=1e-3)
+
+
+@pytest.mark.parametrize("angular", [False, True])
+def test_axis_servo_leaves_uncontrolled_motion_free(angular):  # ServoTests.cs:182-197
+    bodies, shapes, verts = unit_bodies()
+    bodies["velocity"][1] = (0, 2, 3)
+    bodies["angular_velocity"][1] = (0, 4, 5)
+    w = world(bodies, shapes, verts, [make_servo(bodies, "twist" if angular else "axis", 2.0)])
+    w.step(0.01)
+    b = w.bodies()
+    v, av = b["velocity"][1], b["angular_velocity"][1]
+    assert (v[1], v[2], av[1], av[2]) == (2, 3, 4, 5)
+    assert (v[0] if angular else av[0]) == 0
+    assert (av[0] if angular else v[0]) > 0
+
+
+def test_linear_axis_servo_tracks_signed_position_only():  # ServoTests.cs:268-281
+    bodies, shapes, verts = unit_bodies(second=(3, 4, 5))
+    servo = C.linear_axis_servo(bodies, 0, 1, (1, 0, 0), (0, 0, 0), bodies["position"][1], target=-1.0,
+                                max_force=100.0)
+    w = world(bodies, shapes, verts, [servo], substeps=4)
+    for _ in range(300):
+        w.step(0.01)
+    p = w.bodies()["position"][1]
+    assert float(p[0]) == pytest.approx(-1.0, abs=1e-3)
+    assert (p[1], p[2]) == (4, 5)
+
+
+@pytest.mark.parametrize("current,target", [(-179, 179), (179, -179)])
+def test_twist_servo_preserves_signed_angle_difference(current, target):  # ServoTests.cs:283-295
+    bodies, shapes, verts = unit_bodies()
+    servo = C.twist_servo(bodies, 0, 1, (1, 0, 0), target=math.radians(target), max_torque=math.inf)
+    bodies["orientation"][1] = rot_x(math.radians(current))
+    w = world(bodies, shapes, verts, [servo])
+    w.step(0.01)
+    assert float(w.bodies()["angular_velocity"][1][0]) * np.sign(target - current) > 1
+
+
+@pytest.mark.parametrize("current,target", [(-179, 179), (179, -179)])
+def test_angular_servo_takes_the_shortest_path(current, target):  # ServoTests.cs:297-310
+    bodies, shapes, verts = unit_bodies(q2=rot_x(math.radians(current)))
+    servo = C.angular_servo(bodies, 0, 1, target=rot_x(math.radians(target)), max_torque=math.inf)
+    w = world(bodies, shapes, verts, [servo])
+    w.step(0.01)
+    wx = float(w.bodies()["angular_velocity"][1][0])
+    assert wx * np.sign(current - target) > 0
+    assert abs(wx) < 1
+
+
+def test_angular_servo_corrects_an_initial_half_turn():  # ServoTests.cs:407-418
+    bodies, shapes, verts = unit_bodies(q2=(0, 1, 0, 0))
+    w = world(bodies, shapes, verts, [C.angular_servo(bodies, 0, 1, target=(0, 0, 0, 1), max_torque=20.0)],
+              substeps=4)
+    for _ in range(300):
+        w.step(0.01)
+    assert abs(float(w.bodies()["orientation"][1][3])) == pytest.approx(1.0, abs=1e-5)
+
+
+def test_servo_defaults_follow_initialize():
+    bodies, shapes, verts = unit_bodies(second=(1, 2, 3), q2=axis_angle((0, 1, 0), 0.4))
+    a = C.angular_servo(bodies, 0, 1)
+    # Initialize(): the current relative orientation is the target
+    assert np.allclose(a["params"][0:4], bodies["orientation"][1], atol=1e-7)
+    assert a["params"][4] == 0 and a["params"][5] == np.float32(0.001) and a["params"][6] == np.float32(0.2)
+    t = C.twist_servo(bodies, 0, 1, (0, 0, 2))
+    assert np.allclose(t["params"][0:3], quat_rotate((0, 0, 1), (0, -0.19866933, 0, 0.98006658)), atol=1e-6)
+    with pytest.raises(ValueError):
+        C.twist_servo(bodies, 0, 1, (1, 0, 0), target=math.pi + 0.01)
