@@ -267,23 +267,29 @@ def point_on_plane(bodies, b1, b2, axis, anchor1, anchor2, limit=(1e-6, -1e-6)):
     return c
 
 
-def prismatic_joint(bodies, b1, b2, center, axis, limit=LINEAR_FULL, pinned=True, motor=None):
+def prismatic_joint(bodies, b1, b2, center, axis, limit=LINEAR_FULL, pinned=True, motor=None, servo=None):
     """PrismaticJoint (Dynamics/Joints/PrismaticJoint.cs:95-140): PointOnLine through the centre, then
     FixedAngle (pinned) or HingeAngle about the axis (full range), then (motor = (velocity, max force))
-    a LinearMotor along the axis."""
+    a LinearMotor along the axis or (servo = (target position, max force), JointDrive.Servo) a
+    LinearAxisServo along it."""
     out = [point_on_line(bodies, b1, b2, axis, center, center, limit)]
     out.append(fixed_angle(bodies, b1, b2) if pinned else hinge_angle(bodies, b1, b2, axis))
     if motor is not None:
         out.append(linear_motor(bodies, b1, b2, axis, axis, motor[0], motor[1]))
+    elif servo is not None:
+        out.append(linear_axis_servo(bodies, b1, b2, axis, center, center, servo[0], servo[1]))
     return out
 
 
-def universal_joint(bodies, b1, b2, center, axis1, axis2, motor=None):
+def universal_joint(bodies, b1, b2, center, axis1, axis2, motor=None, servo=None):
     """UniversalJoint (Dynamics/Joints/UniversalJoint.cs:30-72): TwistAngle between the two rotation axes
-    (fixed), BallSocket at the centre, then (motor = (velocity, max torque)) an AngularMotor."""
+    (fixed), BallSocket at the centre, then (motor = (velocity, max torque)) an AngularMotor or
+    (servo = max torque, JointDrive.Servo) an AngularServo holding the initial relative orientation."""
     out = [twist_angle(bodies, b1, b2, axis1, axis2), ball_socket(bodies, b1, b2, center)]
     if motor is not None:
         out.append(angular_motor(bodies, b1, b2, axis1, axis2, motor[0], motor[1]))
+    elif servo is not None:
+        out.append(angular_servo(bodies, b1, b2, max_torque=servo))
     return out
 
 
@@ -293,13 +299,16 @@ def ragdoll_joint(bodies, b1, b2, anchor, axis, cone=math.pi / 4, twist=(-math.p
             twist_angle(bodies, b1, b2, axis, axis, twist)]
 
 
-def hinge_joint(bodies, b1, b2, anchor, axis, limit=(-math.pi, math.pi), motor=None):
-    """HingeJoint (Dynamics/Joints/HingeJoint.cs:58-70): HingeAngle about the axis, then BallSocket at
-    the anchor, then (motor = (velocity, max torque)) an AngularMotor about the axis — the order the
-    joint creates them, which is the solve order."""
+def hinge_joint(bodies, b1, b2, anchor, axis, limit=(-math.pi, math.pi), motor=None, servo=None):
+    """HingeJoint (Dynamics/Joints/HingeJoint.cs:58-77): HingeAngle about the axis, then BallSocket at
+    the anchor, then (motor = (velocity, max torque)) an AngularMotor about the axis or (servo = (target
+    angle, max torque), JointDrive.Servo) a TwistServo about it — the order the joint creates them,
+    which is the solve order."""
     out = [hinge_angle(bodies, b1, b2, axis, limit), ball_socket(bodies, b1, b2, anchor)]
     if motor is not None:
         out.append(angular_motor(bodies, b1, b2, axis, axis, motor[0], motor[1]))
+    elif servo is not None:
+        out.append(twist_servo(bodies, b1, b2, axis, target=servo[0], max_torque=servo[1]))
     return out
 
 

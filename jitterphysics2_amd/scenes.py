@@ -621,8 +621,7 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
         elif kind == "plane":
             recs.append(C.point_on_plane(bodies, a, b, (0.0, 1.0, 0.0), bodies["position"][a], anchor))
         elif kind == "twist_servo":  # hinge about y driven to +1 rad with a torque limit
-            recs += C.hinge_joint(bodies, a, b, anchor, (0.0, 1.0, 0.0))
-            recs.append(C.twist_servo(bodies, a, b, (0.0, 1.0, 0.0), target=1.0, max_torque=40.0))
+            recs += C.hinge_joint(bodies, a, b, anchor, (0.0, 1.0, 0.0), servo=(1.0, 40.0))
         elif kind == "angular_servo":  # ball socket; servo to the post's orientation, unlimited torque
             recs.append(C.ball_socket(bodies, a, b, anchor))
             recs.append(C.angular_servo(bodies, a, b, target=(0.0, 0.0, 0.0, 1.0), max_torque=math.inf))

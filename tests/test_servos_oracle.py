@@ -278,3 +278,35 @@ def test_servo_defaults_follow_initialize():
     assert np.allclose(t["params"][0:3], quat_rotate((0, 0, 1), (0, -0.19866933, 0, 0.98006658)), atol=1e-6)
     with pytest.raises(ValueError):
         C.twist_servo(bodies, 0, 1, (1, 0, 0), target=math.pi + 0.01)
+
+
+def test_joint_servo_drives():
+    # HingeJoint / PrismaticJoint / UniversalJoint with JointDrive.Servo (JointDriveTests.cs): the hinge is
+    # driven to its target angle, the slider to its target position, the universal joint held at its
+    # initial relative orientation against a spin
+    bodies, shapes, verts = unit_bodies(second=(0.0, 0.0, 0.0))
+    recs = C.hinge_joint(bodies, 0, 1, (0, 0, 0), (0, 0, 1), servo=(0.6, 50.0))
+    w = world(bodies, shapes, verts, recs, substeps=4)
+    for _ in range(300):
+        w.step(0.01)
+    q = w.bodies()["orientation"][1].astype(float)
+    assert 2 * math.atan2(q[2], q[3]) == pytest.approx(0.6, abs=1e-3)
+    assert abs(q[0]) < 1e-4 and abs(q[1]) < 1e-4
+
+    bodies, shapes, verts = unit_bodies(second=(1.0, 2.0, 0.0))
+    recs = C.prismatic_joint(bodies, 0, 1, (1.0, 2.0, 0.0), (1, 0, 0), servo=(0.75, 50.0))
+    w = world(bodies, shapes, verts, recs, substeps=4)
+    for _ in range(300):
+        w.step(0.01)
+    p = w.bodies()["position"][1].astype(float)
+    assert p[0] == pytest.approx(1.75, abs=1e-3)
+    assert abs(p[1] - 2.0) < 1e-3 and abs(p[2]) < 1e-3
+
+    bodies, shapes, verts = unit_bodies(second=(0.0, -1.0, 0.0))
+    bodies["angular_velocity"][1] = (2.0, 0.0, 3.0)
+    recs = C.universal_joint(bodies, 0, 1, (0, 0, 0), (1, 0, 0), (0, 0, 1), servo=math.inf)
+    w = world(bodies, shapes, verts, recs, substeps=4)
+    for _ in range(200):
+        w.step(0.01)
+    q = w.bodies()["orientation"][1].astype(float)
+    assert abs(q[3]) == pytest.approx(1.0, abs=1e-4)
