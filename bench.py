@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -58,15 +59,19 @@ class _Roctx:
             self.lib.roctxRangePop()
 
 
-def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: int, relax: int) -> float:
+def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: int, relax: int,
+                 n_cons: int = 0) -> float:
     """Algorithmic HBM bytes of one solver launch (one step, all substeps).
     Per arbiter per pass (prepare, each iteration, each relaxation): 152 B of body traffic
     (2 x (v, w read + write 48 B + inverse mass/inertia 28 B)); per contact point per pass 116 B
-    (row data 92 B read, accumulated impulse 12 B read + 12 B write).  Per substep:
-    IntegrateForces 72 B per dynamic body, IntegrateVelocities 80 B per non-static body."""
+    (row data 92 B read, accumulated impulse 12 B read + 12 B write).  Per constraint per pass
+    (prepare and each iteration; relaxation skips constraints): 544 B (record 112 B read, state 32 B
+    write, work record 192 B, 2 x 104 B of body velocities, inverse mass/inertia, position and
+    orientation).  Per substep: IntegrateForces 72 B per dynamic body, IntegrateVelocities 80 B per
+    non-static body."""
     m, k = stats["arbiters_solved"], stats["contacts"]
     passes = 1 + iters + relax
-    per_substep = passes * (152.0 * m + 116.0 * k) + 72.0 * n_dyn + 80.0 * n_moving
+    per_substep = passes * (152.0 * m + 116.0 * k) + (1 + iters) * 544.0 * n_cons + 72.0 * n_dyn + 80.0 * n_moving
     return substeps * per_substep
 
 
@@ -135,16 +140,19 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=None, help="timed steps (heap default 60; pyramid 600)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (heap default 240; pyramid 0)")
     ap.add_argument("--bodies", type=int, default=131072)
-    ap.add_argument("--workload", choices=("heap", "batched", "full", "pyramid"), default="heap",
+    ap.add_argument("--workload", choices=("heap", "batched", "full", "pyramid", "joints", "bullets"), default="heap",
                     help="heap: one heap of --bodies bodies (default 131,072, the metric's configuration; 32,768 is "
                          "the 32k-heap configuration); batched: 4,096 independent worlds of 16..2,048 bodies "
                          "(log-uniform) in one world batch; full: 262,144 bodies incl. 32-point hulls over a "
                          "512x512-cell triangle heightfield split into 64 sub-worlds, TriangleEdgeCollisionFilter "
                          "on; pyramid: the 20-row pyramid of 210 unit boxes, 600 steps, CPU baseline over all of "
-                         "them")
+                         "them; joints: --groups jointed groups (hinge chains, ropes, welds, distance limits, "
+                         "limbs, motors, sliders, universal joints, servos) over a floor; bullets: --bodies "
+                         "projectiles at 15..60 m/s with speculative contacts against thin slabs")
+    ap.add_argument("--groups", type=int, default=4096, help="joints workload: jointed groups")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=None, help="CPU baseline sample (default 3 steps; joints 20)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lws", action="store_true", help="skip the learned-warm-start residual leg")
     ap.add_argument("--lws-settle", type=int, default=700)
@@ -152,8 +160,10 @@ def main() -> int:
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 600 if args.workload == "pyramid" else 60
-    if args.warmup is None:
-        args.warmup = 0 if args.workload == "pyramid" else 240
+    if args.warmup is None:  # bullets: time the impacts, not the pile they leave
+        args.warmup = {"pyramid": 0, "bullets": 10}.get(args.workload, 240)
+    if args.cpu_steps is None:
+        args.cpu_steps = {"joints": 20}.get(args.workload, 3)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md",
                           "n_gpus": args.gpus}))
@@ -174,8 +184,18 @@ def main() -> int:
         args.no_lws = True
         args.cpu_steps = args.steps  # the whole configuration is CPU-sized
         args.no_autotune = True  # the configuration's 600 steps start from the initial state
+    elif args.workload == "joints":
+        bodies, shapes, verts, cfg, dt, cons = scenes.joints(args.groups)
+        args.no_lws = True
+    elif args.workload == "bullets":
+        n = 32768 if args.bodies == 131072 else args.bodies
+        args.bodies = n
+        bodies, shapes, verts, cfg, dt = scenes.bullets(n, arena=30.0 * math.sqrt(n / 512.0))
+        args.no_lws = True
     else:
         bodies, shapes, verts, cfg, dt = scenes.heap(args.bodies)
+    if args.workload != "joints":
+        cons = None
     print(f"# scene built: {len(bodies)} bodies, {len(shapes)} shapes in {time.time() - t0:.1f}s", flush=True)
     n_dyn = int(np.sum(bodies["motion_type"] == 0))
     n_moving = int(np.sum(bodies["motion_type"] != 2))
@@ -184,6 +204,8 @@ def main() -> int:
     relax = int(cfg["relaxation_iterations"][0])
 
     gw = GpuWorld(bodies, shapes, verts, cfg)
+    if cons is not None:
+        gw.set_constraints(cons)
     t0 = time.time()
     for i in range(args.warmup):
         gw.step(dt)
@@ -201,6 +223,9 @@ def main() -> int:
 
     start_bodies = gw.bodies()
     start_arbiters = gw.arbiters()
+    start_cons = gw.constraints() if cons is not None else None
+    con_order = gw.constraint_order()[0] if cons is not None else None
+    n_cons = 0 if cons is None else int(np.sum(cons["enabled"] != 0))
 
     solve_ms, step_ms, bytes_per_launch, contacts = [], [], [], []
     stage_keys = ("ms_broadphase", "ms_narrowphase", "ms_epa", "ms_contact_cache", "ms_colouring", "ms_solve", "ms_update")
@@ -216,7 +241,7 @@ def main() -> int:
             stage_sum[key] += s[key]
         step_ms.append(s["ms_step"])
         contacts.append(s["contacts"])
-        bytes_per_launch.append(solver_bytes(s, n_dyn, n_moving, substeps, iters, relax))
+        bytes_per_launch.append(solver_bytes(s, n_dyn, n_moving, substeps, iters, relax, n_cons))
     t_end = time.perf_counter()
     marker.pop()
     wall = t_end - t_start
@@ -235,6 +260,9 @@ def main() -> int:
 
         ow = OracleWorld(start_bodies, shapes, verts, cfg)
         ow.set_arbiters(start_arbiters)
+        if start_cons is not None:
+            ow.set_constraints(start_cons)
+            ow.set_constraint_order(con_order)
         secs = ow.time_steps(dt, args.cpu_steps)
         cpu = {"value": n_dyn * args.cpu_steps / secs, "unit": "body-steps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body scene continued from the GPU state at "
@@ -268,7 +296,15 @@ def main() -> int:
                     f"{iters} iterations, {relax} relaxations")
                    if args.workload == "full" else
                    (f"20-row pyramid of {n_dyn} unit boxes on a static ground box, dt=0.01, {substeps} substeps, "
-                    f"{iters} iterations, {relax} relaxations"),
+                    f"{iters} iterations, {relax} relaxations")
+                   if args.workload == "pyramid" else
+                   (f"{args.groups} jointed groups, {n_dyn} dynamic bodies, {n_cons} constraints of 14 kinds over a "
+                    f"static floor, dt=1/60, {substeps} substeps, {iters} iterations, {relax} relaxations")
+                   if args.workload == "joints" else
+                   (f"{n_dyn} projectiles (spheres/boxes/capsules/16-point hulls) at 15..60 m/s with speculative "
+                    f"contacts against 0.2-thick static slabs, dt=1/60, {substeps} substeps, {iters} iterations, "
+                    f"{relax} relaxations"),
+                   "constraints": n_cons,
                    "autotune": tuning,
                    "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],
                    "contacts": final["contacts"], "colours": final["colours"], "max_degree": final["max_degree"],

@@ -462,6 +462,11 @@ jh_status colour_constraints(jh_world* w) {
     std::vector<int> fill(count.begin(), count.end() - 1);
     for (int i = 0; i < n; ++i)
         if (colour[i] >= 0) w->con_order_h[fill[colour[i]]++] = i;
+    // Within a batch the constraints touch disjoint dynamic bodies, so their order is free: group them
+    // by kind, so that a wavefront runs one kind's code instead of diverging over several.
+    for (int k = 0; k < ncol; ++k)
+        std::stable_sort(w->con_order_h.begin() + count[k], w->con_order_h.begin() + count[k + 1],
+                         [&](int a, int b) { return w->host_con[a].type < w->host_con[b].type; });
     HIPCHK(w->con_order.ensure(std::max<size_t>(w->con_order_h.size(), 1)));
     HIPCHK(w->con_offs.ensure(w->con_offs_h.size()));
     if (!w->con_order_h.empty())
@@ -1868,7 +1873,8 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
     std::vector<Cand> sc;
     if (w->kind == SOLVER_BARRIER || !w->host_con.empty()) {  // constraints run in the barrier solver
         if (!w->host_con.empty()) w->kind = SOLVER_BARRIER;
-        for (int n : uniq({cu / 2, cu, 2 * cu, 4 * cu, 8 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n});
+        // narrow grids too: each grid barrier's cache write-back and invalidate grow with the workgroups
+        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n});
     } else {
         for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, 3 * cu / 4, cu, 2 * cu}, w->flow_max))
             sc.push_back({SOLVER_FLOW, n});

@@ -520,10 +520,11 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
 
     rng = SplitMix64(seed)
     sb = SceneBuilder()
-    _static_box(sb, (0.0, -0.5, 0.0), (200.0, 1.0, 200.0))
+    side = int(math.ceil(math.sqrt(n_groups)))
+    floor = max(200.0, 8.0 * side + 40.0)
+    _static_box(sb, (0.0, -0.5, 0.0), (floor, 1.0, floor))
     recs: list = []
     plan = []  # deferred: constraints need the final body array
-    side = int(math.ceil(math.sqrt(n_groups)))
     for g in range(n_groups):
         gx, gz = (g % side - side / 2.0) * 8.0, (g // side - side / 2.0) * 8.0
         kind = g % 9
