@@ -2,8 +2,9 @@
 //   jh_cli <scene.bin> <warmup> <steps> [copy_mib]
 // copy_mib > 0 first copies that many MiB device to device (hipMemcpy), a transfer of known size that
 // calibrates HBM traffic counters against the kernels.
-// scene.bin (written by jitterphysics2_amd.scenes.save_scene): magic "JHSC", version 1, then
-// int32 counts (bodies, shapes, vertices), float dt, jh_config, jh_body[], jh_shape[], float[3*nv].
+// scene.bin (written by jitterphysics2_amd.scenes.save_scene): magic "JHSC", version 1 or 2, then
+// int32 counts (bodies, shapes, vertices), float dt, jh_config, jh_body[], jh_shape[], float[3*nv];
+// version 2 appends int32 nc and jh_constraint[nc].
 // After the warmup the launch shapes are autotuned as in the benchmark.  Prints the mean per-stage
 // timings (HIP events inside the library) and the wall time of the timed steps, which are bracketed
 // by the roctx range "timed_steps" for kernel-trace profiles.
@@ -37,7 +38,8 @@ int main(int argc, char** argv) {
     int32_t ver, nb, ns, nv;
     float dt;
     jh_config cfg;
-    if (fread(magic, 1, 4, f) != 4 || memcmp(magic, "JHSC", 4) != 0 || fread(&ver, 4, 1, f) != 1 || ver != 1 ||
+    if (fread(magic, 1, 4, f) != 4 || memcmp(magic, "JHSC", 4) != 0 || fread(&ver, 4, 1, f) != 1 ||
+        (ver != 1 && ver != 2) ||
         fread(&nb, 4, 1, f) != 1 || fread(&ns, 4, 1, f) != 1 || fread(&nv, 4, 1, f) != 1 || fread(&dt, 4, 1, f) != 1 ||
         fread(&cfg, sizeof(cfg), 1, f) != 1 || nb < 0 || ns < 0 || nv < 0) {
         fprintf(stderr, "bad scene file\n");
@@ -51,6 +53,19 @@ int main(int argc, char** argv) {
         fread(verts.data(), sizeof(float), verts.size(), f) != verts.size()) {
         fprintf(stderr, "truncated scene file\n");
         return 2;
+    }
+    int32_t nc = 0;
+    std::vector<jh_constraint> cons;
+    if (ver == 2) {
+        if (fread(&nc, 4, 1, f) != 1 || nc < 0) {
+            fprintf(stderr, "bad constraint count\n");
+            return 2;
+        }
+        cons.resize(nc);
+        if (fread(cons.data(), sizeof(jh_constraint), nc, f) != size_t(nc)) {
+            fprintf(stderr, "truncated constraints\n");
+            return 2;
+        }
     }
     fclose(f);
     int warmup = atoi(argv[2]), steps = atoi(argv[3]);
@@ -77,6 +92,7 @@ int main(int argc, char** argv) {
     if ((s = jh_world_set_bodies(w, bodies.data(), nb))) return die("jh_world_set_bodies", s);
     if ((s = jh_world_set_shapes(w, shapes.data(), ns, nv ? verts.data() : nullptr, nv)))
         return die("jh_world_set_shapes", s);
+    if (nc > 0 && (s = jh_world_set_constraints(w, cons.data(), nc))) return die("jh_world_set_constraints", s);
     for (int i = 0; i < warmup; ++i)
         if ((s = jh_world_step(w, dt))) return die("jh_world_step", s);
     printf("# warmup %d steps done\n", warmup);

@@ -15,8 +15,9 @@ import math
 
 import numpy as np
 
-from ._abi import (BODY_DTYPE, CONFIG_DTYPE, FILTER_TRIANGLE_EDGE, MOTION_DYNAMIC, MOTION_STATIC, SHAPE_BOX,
-                   SHAPE_CAPSULE, SHAPE_DTYPE, SHAPE_POINTCLOUD, SHAPE_SPHERE, SHAPE_TRIANGLE, default_config)
+from ._abi import (BODY_DTYPE, CONFIG_DTYPE, CONSTRAINT_DTYPE, FILTER_TRIANGLE_EDGE, MOTION_DYNAMIC, MOTION_STATIC,
+                   SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE, SHAPE_POINTCLOUD, SHAPE_SPHERE, SHAPE_TRIANGLE,
+                   default_config)
 
 f32 = np.float32
 
@@ -672,17 +673,22 @@ def two_bodies(shape_a, shape_b, pos_a, pos_b, q_a=(0, 0, 0, 1), q_b=(0, 0, 0, 1
 
 
 def save_scene(path: str, scene) -> None:
-    """Write (bodies, shapes, verts, config, dt) in the binary form read by the native driver
-    jh_cli: "JHSC", int32 version 1, int32 body/shape/vertex counts, float32 dt, the 64-byte
-    jh_config, then the jh_body, jh_shape and float32[3*nv] arrays."""
-    bodies, shapes, verts, cfg, dt = scene
+    """Write (bodies, shapes, verts, config, dt[, constraints]) in the binary form read by the native
+    driver jh_cli: "JHSC", int32 version (1, or 2 with constraints), int32 body/shape/vertex counts,
+    float32 dt, the 64-byte jh_config, then the jh_body, jh_shape and float32[3*nv] arrays; version 2
+    appends int32 nc and the jh_constraint records."""
+    bodies, shapes, verts, cfg, dt = scene[:5]
+    cons = scene[5] if len(scene) > 5 else None
     v = np.zeros((0, 3), np.float32) if verts is None else np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
     cfg = np.asarray(cfg, dtype=CONFIG_DTYPE).reshape(())
     with open(path, "wb") as f:
         f.write(b"JHSC")
-        f.write(np.array([1, len(bodies), len(shapes), len(v)], np.int32).tobytes())
+        f.write(np.array([1 if cons is None else 2, len(bodies), len(shapes), len(v)], np.int32).tobytes())
         f.write(np.float32(dt).tobytes())
         f.write(cfg.tobytes())
         f.write(np.ascontiguousarray(bodies, BODY_DTYPE).tobytes())
         f.write(np.ascontiguousarray(shapes, SHAPE_DTYPE).tobytes())
         f.write(v.tobytes())
+        if cons is not None:
+            f.write(np.int32(len(cons)).tobytes())
+            f.write(np.ascontiguousarray(cons, CONSTRAINT_DTYPE).tobytes())
