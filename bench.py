@@ -298,7 +298,7 @@ def main() -> int:
                    (f"20-row pyramid of {n_dyn} unit boxes on a static ground box, dt=0.01, {substeps} substeps, "
                     f"{iters} iterations, {relax} relaxations")
                    if args.workload == "pyramid" else
-                   (f"{args.groups} jointed groups, {n_dyn} dynamic bodies, {n_cons} constraints of 14 kinds over a "
+                   (f"{args.groups} jointed groups, {n_dyn} dynamic bodies, {n_cons} constraints of 15 kinds over a "
                     f"static floor, dt=1/60, {substeps} substeps, {iters} iterations, {relax} relaxations")
                    if args.workload == "joints" else
                    (f"{n_dyn} projectiles (spheres/boxes/capsules/16-point hulls) at 15..60 m/s with speculative "

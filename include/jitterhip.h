@@ -189,12 +189,16 @@ typedef struct jh_arbiter {
  *       LocalAnchor2[3], TargetPosition, MaximumForce, Softness, BiasFactor; state AccumulatedImpulse.
  *   TWIST_SERVO (TwistServo.cs): params LocalAxis2[3] (body-2 frame), ReferenceOrientation[4],
  *       TargetAngle, MaximumTorque, Softness, BiasFactor; state AccumulatedImpulse, Hemisphere.
+ *   ELLIPTICAL_CONE_LIMIT (EllipticalConeLimit.cs): params LocalAxis1[3], LocalAxis2[3], LocalReference1[3]
+ *       (body-1 frame, perpendicular to LocalAxis1), MinAngle1, MaxAngle1 (semiaxes along the reference),
+ *       MinAngle2, MaxAngle2 (perpendicular), Softness, BiasFactor; state AccumulatedImpulse, Jacobian[3],
+ *       Clamp (0 inside, 1 outer, 2 inner, 3 fixed).
  * A servo's MaximumForce / MaximumTorque may be +inf (unlimited); it must not be negative or NaN. */
 enum { JH_CONSTRAINT_BALL_SOCKET = 0, JH_CONSTRAINT_HINGE_ANGLE = 1, JH_CONSTRAINT_FIXED_ANGLE = 2,
        JH_CONSTRAINT_DISTANCE_LIMIT = 3, JH_CONSTRAINT_TWIST_ANGLE = 4, JH_CONSTRAINT_CONE_LIMIT = 5,
        JH_CONSTRAINT_ANGULAR_MOTOR = 6, JH_CONSTRAINT_LINEAR_MOTOR = 7, JH_CONSTRAINT_POINT_ON_LINE = 8,
        JH_CONSTRAINT_POINT_ON_PLANE = 9, JH_CONSTRAINT_ANGULAR_SERVO = 10, JH_CONSTRAINT_LINEAR_SERVO = 11,
-       JH_CONSTRAINT_LINEAR_AXIS_SERVO = 12, JH_CONSTRAINT_TWIST_SERVO = 13 };
+       JH_CONSTRAINT_LINEAR_AXIS_SERVO = 12, JH_CONSTRAINT_TWIST_SERVO = 13, JH_CONSTRAINT_ELLIPTICAL_CONE_LIMIT = 14 };
 
 typedef struct jh_constraint {
     int32_t type;           /* JH_CONSTRAINT_*                                                  */

@@ -150,6 +150,7 @@ CONSTRAINT_BALL_SOCKET, CONSTRAINT_HINGE_ANGLE, CONSTRAINT_FIXED_ANGLE, CONSTRAI
 CONSTRAINT_TWIST_ANGLE, CONSTRAINT_CONE_LIMIT, CONSTRAINT_ANGULAR_MOTOR, CONSTRAINT_LINEAR_MOTOR = 4, 5, 6, 7
 CONSTRAINT_POINT_ON_LINE, CONSTRAINT_POINT_ON_PLANE = 8, 9
 CONSTRAINT_ANGULAR_SERVO, CONSTRAINT_LINEAR_SERVO, CONSTRAINT_LINEAR_AXIS_SERVO, CONSTRAINT_TWIST_SERVO = 10, 11, 12, 13
+CONSTRAINT_ELLIPTICAL_CONE_LIMIT = 14
 
 CONSTRAINT_DTYPE = np.dtype([
     ("type", "<i4"), ("body1", "<i4"), ("body2", "<i4"), ("enabled", "<i4"),

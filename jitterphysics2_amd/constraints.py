@@ -12,7 +12,8 @@ import math
 import numpy as np
 
 from ._abi import (CONSTRAINT_ANGULAR_MOTOR, CONSTRAINT_ANGULAR_SERVO, CONSTRAINT_BALL_SOCKET, CONSTRAINT_CONE_LIMIT,
-                   CONSTRAINT_DISTANCE_LIMIT, CONSTRAINT_DTYPE, CONSTRAINT_FIXED_ANGLE, CONSTRAINT_HINGE_ANGLE,
+                   CONSTRAINT_DISTANCE_LIMIT, CONSTRAINT_DTYPE, CONSTRAINT_ELLIPTICAL_CONE_LIMIT,
+                   CONSTRAINT_FIXED_ANGLE, CONSTRAINT_HINGE_ANGLE,
                    CONSTRAINT_LINEAR_AXIS_SERVO, CONSTRAINT_LINEAR_MOTOR, CONSTRAINT_LINEAR_SERVO,
                    CONSTRAINT_POINT_ON_LINE, CONSTRAINT_POINT_ON_PLANE, CONSTRAINT_TWIST_ANGLE, CONSTRAINT_TWIST_SERVO)
 
@@ -206,6 +207,39 @@ def cone_limit(bodies, b1, b2, axis1, axis2=None, limit=(0.0, math.pi / 4)):
     p[6] = DEFAULT_ANGULAR_BIAS
     p[7] = DEFAULT_ANGULAR_SOFTNESS
     p[8], p[9] = f32(limit[0]), f32(limit[1])
+    return c
+
+
+def elliptical_cone_limit(bodies, b1, b2, axis1, axis2, reference, limit_reference, limit_perpendicular):
+    """EllipticalConeLimit.Initialize(axisBody1, axisBody2, referenceDirectionBody1, limitAlongReference,
+    limitAlongPerpendicular) (EllipticalConeLimit.cs:92-126): body 2's axis stays inside the elliptical
+    cone about body 1's axis whose semiaxes along the reference direction and perpendicular to it are the
+    limits' maxima (and, if both minima are positive, outside the inner ellipse of the minima).  Limits
+    are (min, max) radians with 0 <= min <= max < pi, max > 0, minima both zero or both positive."""
+    lims = [(f32(limit_reference[0]), f32(limit_reference[1])),
+            (f32(limit_perpendicular[0]), f32(limit_perpendicular[1]))]
+    for mn, mx in lims:
+        if not (math.isfinite(mn) and math.isfinite(mx) and mn >= 0 and mx > 0 and mx < f32(math.pi) and mn <= mx):
+            raise ValueError("elliptical cone limits must satisfy 0 <= min <= max < pi and max > 0")
+    if (lims[0][0] == 0) != (lims[1][0] == 0):
+        raise ValueError("the inner semiaxes must either both be zero or both be positive")
+    a1 = _normalize(axis1)
+    a2 = _normalize(axis2)
+    r = _normalize(reference)
+    d = f32(a1[0] * r[0] + a1[1] * r[1] + a1[2] * r[2])
+    r = np.array([r[0] - a1[0] * d, r[1] - a1[1] * d, r[2] - a1[2] * d], np.float32)
+    if float(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]) < 1e-12:
+        raise ValueError("the reference direction must not be parallel to the cone axis")
+    r = _normalize(r)
+    c = _record(CONSTRAINT_ELLIPTICAL_CONE_LIMIT, b1, b2)
+    p = c["params"]
+    p[0:3] = _rot_inv(a1, bodies["orientation"][b1])
+    p[3:6] = _rot_inv(a2, bodies["orientation"][b2])
+    p[6:9] = _rot_inv(r, bodies["orientation"][b1])
+    p[9], p[10] = lims[0]
+    p[11], p[12] = lims[1]
+    p[13] = DEFAULT_ANGULAR_SOFTNESS
+    p[14] = DEFAULT_ANGULAR_BIAS
     return c
 
 

@@ -1,6 +1,6 @@
 // Constraints on the device: BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit,
 // AngularMotor, LinearMotor, PointOnLine, PointOnPlane, AngularServo, LinearServo, LinearAxisServo,
-// TwistServo (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
+// TwistServo, EllipticalConeLimit (Dynamics/Constraints/*.cs) with the JMatrix / JSymmetricMatrix / MathHelper / AngularConstraintMath
 // pieces they use, FP32 in the reference's operation order.  One thread solves one constraint; the
 // solver visits constraints in batches with disjoint dynamic bodies.
 #pragma once
@@ -1116,6 +1116,110 @@ __device__ __forceinline__ void iterate_twist_servo(jh_constraint& c, Work& k, C
     twist_servo_apply(k, A, B, c.state[0] - previous);
 }
 
+// EllipticalConeLimit (EllipticalConeLimit.cs, layout as in jh_constraint): params LocalAxis1 (0-2), LocalAxis2 (3-5), LocalReference1
+// (6-8), MinAngle1 (9), MaxAngle1 (10), MinAngle2 (11), MaxAngle2 (12), Softness (13), BiasFactor (14);
+// state AccumulatedImpulse (0), Jacobian (1-3), Clamp (4) — the last two persist because the next
+// prepare compares against them.
+__device__ __forceinline__ float min_nan(float a, float b) {  // MathR.Min
+    if (a != a) return a;
+    if (b != b) return b;
+    return a < b ? a : b;
+}
+// EllipseRadius (:258-268): radius of the ellipse with semiaxes a, b at the azimuth (cosine, sine) and
+// the logarithmic azimuth derivative of that radius
+__device__ __forceinline__ float ellipse_radius(float a, float b, float cosine, float sine, float& derivative) {
+    float scale = min_nan(a, b);
+    float u = scale / a;
+    float v = scale / b;
+    float squared = cosine * cosine * u * u + sine * sine * v * v;
+    derivative = cosine * sine * (v * v - u * u) / squared;
+    return scale / sqrtf(squared);
+}
+// PrepareForIterationEllipticalConeLimit (:270-389)
+__device__ __forceinline__ void prepare_elliptical_cone(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt,
+                                                        float inv_sub) {
+    const float* p = c.params;
+    vec3 axis = rot(p3(p), A.q);
+    vec3 reference = rot(p3(p + 6), A.q);
+    vec3 other = cross(axis, reference);
+    vec3 target = rot(p3(p + 3), B.q);
+    float x = dot(target, reference);
+    float y = dot(target, other);
+    float z = dot(target, axis);
+    float sine = sqrtf(x * x + y * y);
+    float angle = stable::atan2(sine, z);
+    float ca, sa;
+    if (sine > 0.0f) {
+        ca = x / sine;
+        sa = y / sine;
+    } else {
+        sym3 inertia = sym_add(A.I, B.I);
+        bool use_reference = dot(sym_mul(other, inertia), other) >= dot(sym_mul(reference, inertia), reference);
+        ca = use_reference ? 1.0f : 0.0f;
+        sa = use_reference ? 0.0f : 1.0f;
+    }
+    float derivative;
+    float outer = ellipse_radius(p[10], p[12], ca, sa, derivative);
+    bool fixed_cone = p[9] == p[10] && p[11] == p[12];
+    float boundary = outer;
+    int clamp = 0;
+    if (fixed_cone) {
+        clamp = 3;
+    } else if (angle >= outer) {
+        clamp = 1;
+    } else if (p[9] > 0.0f) {
+        float inner_derivative;
+        float inner = ellipse_radius(p[9], p[11], ca, sa, inner_derivative);
+        if (angle <= inner) {
+            clamp = 2;
+            boundary = inner;
+            derivative = inner_derivative;
+        }
+    }
+    int previous = int(c.state[4]);
+    c.state[4] = float(clamp);
+    k.clamp = clamp;
+    if (clamp == 0) {
+        c.state[0] = 0.0f;
+        return;
+    }
+    vec3 tilt = reference * ca + other * sa;
+    vec3 azimuth_dir = cross(axis, tilt);
+    vec3 angle_dir = tilt * z - axis * sine;
+    float azimuth_factor = sine > 0.0f ? angle / sine * derivative : 0.0f;
+    float scale = max_nan(1.0f, fabsf(azimuth_factor));
+    vec3 jac = -azimuth_dir * (1.0f / scale) + angle_dir * (azimuth_factor / scale);
+    float inv_len = 1.0f / sqrtf(len_sq(jac));
+    jac = jac * inv_len;
+    inv_len /= scale;
+    if (previous != clamp || dot(p3(c.state + 1), jac) < 0.0f) c.state[0] = 0.0f;
+    set3(c.state + 1, jac);
+    if (clamp == 1) c.state[0] = min_nan(c.state[0], 0.0f);
+    else if (clamp == 2) c.state[0] = max_nan(c.state[0], 0.0f);
+    float mass = dot(sym_mul(jac, A.I), jac) + dot(sym_mul(jac, B.I), jac) + p[13] * inv_sub;
+    k.em_s = mass > 0.0f ? 1.0f / mass : 0.0f;
+    if (k.em_s == 0.0f) c.state[0] = 0.0f;
+    k.bias_s = (angle - boundary) * inv_len * p[14] * inv_dt;
+    float acc = c.state[0];
+    if (A.dyn) A.w = A.w + sym_mul(jac * acc, A.I);
+    if (B.dyn) B.w = B.w - sym_mul(jac * acc, B.I);
+}
+// IterateEllipticalConeLimit (:391-425)
+__device__ __forceinline__ void iterate_elliptical_cone(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
+    if (k.clamp == 0) return;
+    vec3 jac = p3(c.state + 1);
+    float jv = dot(A.w - B.w, jac);
+    float lambda = -k.em_s * (jv + k.bias_s + c.state[0] * c.params[13] * inv_sub);
+    float old = c.state[0];
+    float acc = old + lambda;
+    if (k.clamp == 1) acc = min_nan(acc, 0.0f);
+    else if (k.clamp == 2) acc = max_nan(acc, 0.0f);
+    c.state[0] = acc;
+    lambda = acc - old;
+    if (A.dyn) A.w = A.w + sym_mul(jac * lambda, A.I);
+    if (B.dyn) B.w = B.w - sym_mul(jac * lambda, B.I);
+}
+
 __device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_dt, float inv_sub,
                                         float sub_dt) {
     switch (c.type) {
@@ -1132,7 +1236,8 @@ __device__ __forceinline__ void prepare(jh_constraint& c, Work& k, CBody& A, CBo
         case JH_CONSTRAINT_ANGULAR_SERVO: prepare_angular_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
         case JH_CONSTRAINT_LINEAR_SERVO: prepare_linear_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
         case JH_CONSTRAINT_LINEAR_AXIS_SERVO: prepare_linear_axis_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
-        default: prepare_twist_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
+        case JH_CONSTRAINT_TWIST_SERVO: prepare_twist_servo(c, k, A, B, inv_dt, inv_sub, sub_dt); break;
+        default: prepare_elliptical_cone(c, k, A, B, inv_dt, inv_sub); break;
     }
 }
 __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBody& B, float inv_sub) {
@@ -1150,7 +1255,8 @@ __device__ __forceinline__ void iterate(jh_constraint& c, Work& k, CBody& A, CBo
         case JH_CONSTRAINT_ANGULAR_SERVO: iterate_angular_servo(c, k, A, B, inv_sub); break;
         case JH_CONSTRAINT_LINEAR_SERVO: iterate_linear_servo(c, k, A, B, inv_sub); break;
         case JH_CONSTRAINT_LINEAR_AXIS_SERVO: iterate_linear_axis_servo(c, k, A, B, inv_sub); break;
-        default: iterate_twist_servo(c, k, A, B, inv_sub); break;
+        case JH_CONSTRAINT_TWIST_SERVO: iterate_twist_servo(c, k, A, B, inv_sub); break;
+        default: iterate_elliptical_cone(c, k, A, B, inv_sub); break;
     }
 }
 

@@ -1329,7 +1329,7 @@ jh_status jh_world_set_constraints(jh_world* w, const jh_constraint* in, int32_t
     if (!w || n < 0 || (n > 0 && !in)) return fail(JH_ERR_INVALID_ARG, "bad arguments");
     for (int i = 0; i < n; ++i) {
         const jh_constraint& c = in[i];
-        if (c.type < JH_CONSTRAINT_BALL_SOCKET || c.type > JH_CONSTRAINT_TWIST_SERVO)
+        if (c.type < JH_CONSTRAINT_BALL_SOCKET || c.type > JH_CONSTRAINT_ELLIPTICAL_CONE_LIMIT)
             return fail(JH_ERR_INVALID_ARG, "constraint %d: unknown type %d", i, c.type);
         if (c.body1 < 0 || c.body1 >= w->nb || c.body2 < 0 || c.body2 >= w->nb)
             return fail(JH_ERR_INVALID_ARG, "constraint %d: body index out of range", i);
@@ -1345,6 +1345,19 @@ jh_status jh_world_set_constraints(jh_world* w, const jh_constraint* in, int32_t
                                                                : -1;
         if (imax >= 0 && c.params[imax] < 0.0f)
             return fail(JH_ERR_INVALID_ARG, "constraint %d: negative servo maximum", i);
+        // EllipticalConeLimit.ValidateLimits (EllipticalConeLimit.cs:168-191): 0 <= min <= max < pi, max > 0,
+        // inner semiaxes both zero or both positive
+        if (c.type == JH_CONSTRAINT_ELLIPTICAL_CONE_LIMIT) {
+            const float* p = c.params;
+            bool ok = true;
+            for (int r = 0; r < 2; ++r) {
+                float mn = p[9 + 2 * r], mx = p[10 + 2 * r];
+                ok = ok && std::isfinite(mn) && std::isfinite(mx) && mn >= 0.0f && mx > 0.0f && mx < 3.14159265f &&
+                     mn <= mx;
+            }
+            if (!ok || ((p[9] == 0.0f) != (p[11] == 0.0f)))
+                return fail(JH_ERR_INVALID_ARG, "constraint %d: elliptical cone limits out of range", i);
+        }
     }
     w->host_con.assign(in, in + n);
     size_t cap = std::max(n, 1);

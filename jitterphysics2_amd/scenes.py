@@ -515,7 +515,8 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
     TwistAngle), motor-driven wheels on hinges (AngularMotor) with a block pulled by a LinearMotor,
     prismatic sliders (PointOnLine + FixedAngle or HingeAngle, limits, motors), universal-joint chains
     with a body held on a plane (PointOnPlane), and servos (TwistServo on a hinged arm, AngularServo on a
-    ball-socketed box, LinearServo and LinearAxisServo on free blocks); free boxes fall between them.  Returns (bodies, shapes,
+    ball-socketed box, LinearServo and LinearAxisServo on free blocks, an EllipticalConeLimit on a swinging
+    capsule); free boxes fall between them.  Returns (bodies, shapes,
     verts, cfg, dt, constraints).  dt = 1/60, 4 substeps, 10 iterations."""
     from . import constraints as C
 
@@ -601,6 +602,9 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
             b = sb.body((gx, top - 3.0, gz - 2.0))
             sb.sphere(b, 0.3)
             plan.append(("axis_servo", post, b, (gx, top - 3.0, gz - 2.0)))
+            b = sb.body((gx + 2.0, top - 0.6, gz - 2.0), angular_velocity=(3.0 * rng.normal(), rng.normal(), 3.0))
+            sb.capsule(b, 0.2, 0.8)
+            plan.append(("ellipse", post, b, (gx + 2.0, top, gz - 2.0)))
         else:  # capsules on slack distance limits
             for k in range(links // 2):
                 b = sb.body((gx + rng.uniform(-2, 2), top - 2.0 - 1.5 * k, gz + rng.uniform(-2, 2)),
@@ -632,6 +636,10 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
         elif kind == "axis_servo":  # held 2 m below the post along y (the other directions are free)
             recs.append(C.linear_axis_servo(bodies, a, b, (0.0, 1.0, 0.0), bodies["position"][a], anchor,
                                             target=-4.0, max_force=math.inf))
+        elif kind == "ellipse":  # ball socket with an elliptical swing cone (0.8 rad along x, 0.3 along z)
+            recs.append(C.ball_socket(bodies, a, b, anchor))
+            recs.append(C.elliptical_cone_limit(bodies, a, b, (0.0, -1.0, 0.0), (0.0, -1.0, 0.0), (1.0, 0.0, 0.0),
+                                                (0.0, 0.8), (0.0, 0.3)))
         elif kind == "hinge":
             recs += C.hinge_joint(bodies, a, b, anchor, (1.0, 0.0, 0.0), (-math.pi / 3, math.pi / 3))
         elif kind == "ball":

@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only (see orc_math.hpp).  Constraints solved with the contacts —
 // BallSocket, HingeAngle, FixedAngle, DistanceLimit, TwistAngle, ConeLimit, AngularMotor, LinearMotor,
-// PointOnLine, PointOnPlane, AngularServo, LinearServo, LinearAxisServo, TwistServo (Dynamics/Constraints/*.cs) — and the JMatrix,
+// PointOnLine, PointOnPlane, AngularServo, LinearServo, LinearAxisServo, TwistServo, EllipticalConeLimit
+// (Dynamics/Constraints/*.cs) — and the JMatrix,
 // JSymmetricMatrix, MathHelper and AngularConstraintMath pieces they use, restated in the reference's
 // operation order and templated on the precision R.
 #pragma once
@@ -1143,6 +1144,111 @@ template <class R> void iterate_twist_servo(Constraint<R>& c, ConstraintBody<R>&
     twist_servo_apply(c, A, B, c.s[0] - previous);
 }
 
+// EllipticalConeLimit (EllipticalConeLimit.cs): params LocalAxis1 (0-2), LocalAxis2 (3-5), LocalReference1
+// (6-8), MinAngle1 (9), MaxAngle1 (10), MinAngle2 (11), MaxAngle2 (12), Softness (13), BiasFactor (14);
+// state AccumulatedImpulse (0), Jacobian (1-3), Clamp (4) — the last two persist because the next
+// prepare compares against them.
+template <class R> inline R min_nan(R a, R b) {  // MathR.Min
+    if (a != a) return a;
+    if (b != b) return b;
+    return a < b ? a : b;
+}
+// EllipseRadius (:258-268): radius of the ellipse with semiaxes a, b at the azimuth (cosine, sine) and
+// the logarithmic azimuth derivative of that radius
+template <class R> inline R ellipse_radius(R a, R b, R cosine, R sine, R& derivative) {
+    R scale = min_nan(a, b);
+    R u = scale / a;
+    R v = scale / b;
+    R squared = cosine * cosine * u * u + sine * sine * v * v;
+    derivative = cosine * sine * (v * v - u * u) / squared;
+    return scale / std::sqrt(squared);
+}
+// PrepareForIterationEllipticalConeLimit (:270-389)
+template <class R> void prepare_elliptical_cone(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                R inv_dt, R inv_sub) {
+    const R* p = c.p;
+    V3<R> axis = rotate(vecp(p), A.q);
+    V3<R> reference = rotate(vecp(p + 6), A.q);
+    V3<R> other = cross(axis, reference);
+    V3<R> target = rotate(vecp(p + 3), B.q);
+    R x = dot(target, reference);
+    R y = dot(target, other);
+    R z = dot(target, axis);
+    R sine = std::sqrt(x * x + y * y);
+    R angle = StableMath<R>::atan2(sine, z);
+    R ca, sa;
+    if (sine > R(0)) {
+        ca = x / sine;
+        sa = y / sine;
+    } else {
+        Sym<R> inertia = sym_add(A.iw, B.iw);
+        bool use_reference = dot(sym_mul(other, inertia), other) >= dot(sym_mul(reference, inertia), reference);
+        ca = use_reference ? R(1) : R(0);
+        sa = use_reference ? R(0) : R(1);
+    }
+    R derivative;
+    R outer = ellipse_radius(p[10], p[12], ca, sa, derivative);
+    bool fixed_cone = p[9] == p[10] && p[11] == p[12];
+    R boundary = outer;
+    int clamp = 0;
+    if (fixed_cone) {
+        clamp = 3;
+    } else if (angle >= outer) {
+        clamp = 1;
+    } else if (p[9] > R(0)) {
+        R inner_derivative;
+        R inner = ellipse_radius(p[9], p[11], ca, sa, inner_derivative);
+        if (angle <= inner) {
+            clamp = 2;
+            boundary = inner;
+            derivative = inner_derivative;
+        }
+    }
+    int previous = int(c.s[4]);
+    c.s[4] = R(clamp);
+    c.clamp = clamp;
+    if (clamp == 0) {
+        c.s[0] = R(0);
+        return;
+    }
+    V3<R> tilt = reference * ca + other * sa;
+    V3<R> azimuth_dir = cross(axis, tilt);
+    V3<R> angle_dir = tilt * z - axis * sine;
+    R azimuth_factor = sine > R(0) ? angle / sine * derivative : R(0);
+    R scale = max_nan(R(1), std::fabs(azimuth_factor));
+    V3<R> jac = -azimuth_dir * (R(1) / scale) + angle_dir * (azimuth_factor / scale);
+    R inv_len = R(1) / length(jac);
+    jac = jac * inv_len;
+    inv_len /= scale;
+    if (previous != clamp || dot(vecp(c.s + 1), jac) < R(0)) c.s[0] = R(0);
+    setv(c.s + 1, jac);
+    if (clamp == 1) c.s[0] = min_nan(c.s[0], R(0));
+    else if (clamp == 2) c.s[0] = max_nan(c.s[0], R(0));
+    R mass = dot(sym_mul(jac, A.iw), jac) + dot(sym_mul(jac, B.iw), jac) + p[13] * inv_sub;
+    c.em_s = mass > R(0) ? R(1) / mass : R(0);
+    if (c.em_s == R(0)) c.s[0] = R(0);
+    c.bias_s = (angle - boundary) * inv_len * p[14] * inv_dt;
+    R acc = c.s[0];
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(jac * acc, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(jac * acc, B.iw);
+}
+// IterateEllipticalConeLimit (:391-425)
+template <class R> void iterate_elliptical_cone(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B,
+                                                R inv_sub) {
+    if (c.clamp == 0) return;
+    V3<R> jac = vecp(c.s + 1);
+    R jv = dot(*A.angvel - *B.angvel, jac);
+    R lambda = -c.em_s * (jv + c.bias_s + c.s[0] * c.p[13] * inv_sub);
+    R old = c.s[0];
+    R acc = old + lambda;
+    if (c.clamp == 1) acc = min_nan(acc, R(0));
+    else if (c.clamp == 2) acc = max_nan(acc, R(0));
+    c.s[0] = acc;
+    lambda = acc - old;
+    if (A.dynamic) *A.angvel = *A.angvel + sym_mul(jac * lambda, A.iw);
+    if (B.dynamic) *B.angvel = *B.angvel - sym_mul(jac * lambda, B.iw);
+}
+
 template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                            R inv_sub, R sub_dt) {
     switch (c.type) {
@@ -1159,7 +1265,8 @@ template <class R> void constraint_prepare(Constraint<R>& c, ConstraintBody<R>& 
         case JH_CONSTRAINT_ANGULAR_SERVO: prepare_angular_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
         case JH_CONSTRAINT_LINEAR_SERVO: prepare_linear_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
         case JH_CONSTRAINT_LINEAR_AXIS_SERVO: prepare_linear_axis_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
-        default: prepare_twist_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
+        case JH_CONSTRAINT_TWIST_SERVO: prepare_twist_servo(c, A, B, inv_dt, inv_sub, sub_dt); break;
+        default: prepare_elliptical_cone(c, A, B, inv_dt, inv_sub); break;
     }
 }
 template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_sub) {
@@ -1177,7 +1284,8 @@ template <class R> void constraint_iterate(Constraint<R>& c, ConstraintBody<R>& 
         case JH_CONSTRAINT_ANGULAR_SERVO: iterate_angular_servo(c, A, B, inv_sub); break;
         case JH_CONSTRAINT_LINEAR_SERVO: iterate_linear_servo(c, A, B, inv_sub); break;
         case JH_CONSTRAINT_LINEAR_AXIS_SERVO: iterate_linear_axis_servo(c, A, B, inv_sub); break;
-        default: iterate_twist_servo(c, A, B, inv_sub); break;
+        case JH_CONSTRAINT_TWIST_SERVO: iterate_twist_servo(c, A, B, inv_sub); break;
+        default: iterate_elliptical_cone(c, A, B, inv_sub); break;
     }
 }
 

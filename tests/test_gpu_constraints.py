@@ -1,4 +1,4 @@
-"""GPU parity of the constraint solver (the fourteen constraint and servo kinds, solved in the barrier
+"""GPU parity of the constraint solver (the fifteen constraint and servo kinds, solved in the barrier
 solver's batches after the contacts of each pass) against the oracle replaying the GPU's contact and
 constraint orders: bodies, contacts and constraint impulses bit for bit in FP32."""
 import numpy as np
@@ -71,6 +71,10 @@ def test_constraint_input_errors():
     servo = C.as_array([C.linear_servo(bodies, 0, 1, bodies["position"][0], max_force=-1.0)])
     with pytest.raises(RuntimeError, match="invalid argument"):  # ServoMath.CheckMaximum
         gw.set_constraints(servo)
+    ell = C.as_array([C.elliptical_cone_limit(bodies, 0, 1, (0, 0, 1), (0, 0, 1), (1, 0, 0), (0, 1), (0, 0.5))])
+    ell["params"][0][10] = 3.5  # EllipticalConeLimit.ValidateRange: max < pi
+    with pytest.raises(RuntimeError, match="invalid argument"):
+        gw.set_constraints(ell)
     gw.set_constraints(cons)  # the valid set is accepted after the rejected ones
     assert len(gw.constraints()) == len(cons)
 
