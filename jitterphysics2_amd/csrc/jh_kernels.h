@@ -1315,12 +1315,70 @@ __device__ __forceinline__ void iterate_contact(const jh_arbiter& cd, jh_contact
     if (mode & JH_SOLVE_ANGULAR2) B2.w = B2.w + sym_mul(cross(rel2, lin), B2.I);
 }
 
+constexpr int kBufferDword3 = 0x00020000;  // raw buffer resource word 3 for gfx9-family targets
+constexpr int kCohSc1 = 16;                // cache-policy operand: sc1
+
+// Coherent body state for the barrier solver.  Within one launch, workgroups share the body velocities
+// and the poses that integrate_velocities writes; with every access to them a device-coherent (sc1)
+// 16-byte buffer access, served past the L2 of the issuing XCD, the grid barrier between batches only
+// has to see each workgroup's accesses completed — no L2 write-back on arrival, no L2 invalidate on
+// departure, which an agent-scope release/acquire barrier pays once per workgroup.  Coh = false keeps
+// plain accesses behind the fenced barrier.
+struct CohBodies {
+    __amdgpu_buffer_rsrc_t vel, angvel, pos, quat;
+};
+__device__ __forceinline__ CohBodies coh_bodies(const Bodies& B, int nb) {
+    return {__builtin_amdgcn_make_buffer_rsrc((void*)B.vel, 0, nb * 16, kBufferDword3),
+            __builtin_amdgcn_make_buffer_rsrc((void*)B.angvel, 0, nb * 16, kBufferDword3),
+            __builtin_amdgcn_make_buffer_rsrc((void*)B.pos, 0, nb * 16, kBufferDword3),
+            __builtin_amdgcn_make_buffer_rsrc((void*)B.quat, 0, nb * 16, kBufferDword3)};
+}
+__device__ __forceinline__ float4 coh_ld(__amdgpu_buffer_rsrc_t r, int i) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, i * 16, 0, kCohSc1));
+}
+__device__ __forceinline__ void coh_st(__amdgpu_buffer_rsrc_t r, int i, float4 v) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, i * 16, 0, kCohSc1);
+}
+template <bool Coh> __device__ __forceinline__ float4 bld(const float4* a, __amdgpu_buffer_rsrc_t r, int i) {
+    if constexpr (Coh) return coh_ld(r, i);
+    else return a[i];
+}
+template <bool Coh> __device__ __forceinline__ void bst(float4* a, __amdgpu_buffer_rsrc_t r, int i, float4 v) {
+    if constexpr (Coh) coh_st(r, i, v);
+    else a[i] = v;
+}
+// Grid barrier for the coherent accesses: each wave waits for its own loads and stores to complete
+// (vmcnt 0; the workgroup barrier alone does not wait for them), then one relaxed arrival per workgroup.
+__device__ __forceinline__ void grid_barrier_coh(unsigned* bar) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left at their maxima
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* count = bar;
+        unsigned* gen = bar + 32;
+        unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+template <bool Coh> __device__ __forceinline__ void sync_grid(unsigned* bar) {
+    if constexpr (Coh) grid_barrier_coh(bar);
+    else grid_barrier(bar);
+}
+
 // phase 0: prepare, 1: iterate with bias, 2: iterate without bias (relaxation)
-__device__ __forceinline__ void arbiter_pass(jh_arbiter& a, BodyReg& B1, BodyReg& B2, const Bodies& B, int phase,
-                                             const StepTime& ts) {
+template <bool Coh = false>
+__device__ __forceinline__ void arbiter_pass(jh_arbiter& a, BodyReg& B1, BodyReg& B2, const Bodies& B,
+                                             const CohBodies& cb, int phase, const StepTime& ts) {
     if (phase == 0) {
-        vec3 x1 = ld3(B.pos[a.body1]), x2 = ld3(B.pos[a.body2]);
-        quat q1 = ldq(B.quat[a.body1]), q2 = ldq(B.quat[a.body2]);
+        vec3 x1 = ld3(bld<Coh>(B.pos, cb.pos, a.body1)), x2 = ld3(bld<Coh>(B.pos, cb.pos, a.body2));
+        quat q1 = ldq(bld<Coh>(B.quat, cb.quat, a.body1)), q2 = ldq(bld<Coh>(B.quat, cb.quat, a.body2));
         for (int s = 0; s < 4; ++s)
             if (a.usage_mask & (1u << s)) prepare_contact(a, a.contacts[s], B1, B2, x1, q1, x2, q2, ts);
     } else {
@@ -1381,30 +1439,37 @@ struct SolveArgs {
     ConArgs con;             // barrier solver: constraints (nbatch 0: none)
 };
 
+
+template <bool Coh = false>
 __device__ __forceinline__ void integrate_forces(const SolveArgs& P, int t0, int stride) {
     const Bodies& B = P.B;
+    CohBodies cb = coh_bodies(B, P.nb);
     for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
         if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
-        B.angvel[b] = st3(ld3(B.angvel[b]) + ld3(B.dangvel[b]));
-        B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
+        bst<Coh>(B.angvel, cb.angvel, b, st3(ld3(bld<Coh>(B.angvel, cb.angvel, b)) + ld3(B.dangvel[b])));
+        bst<Coh>(B.vel, cb.vel, b, st3(ld3(bld<Coh>(B.vel, cb.vel, b)) + ld3(B.dvel[b])));
     }
 }
+template <bool Coh = false>
 __device__ __forceinline__ void integrate_velocities(const SolveArgs& P, int t0, int stride) {
     const Bodies& B = P.B;
+    CohBodies cb = coh_bodies(B, P.nb);
     for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
         if (B.motion[b] == JH_MOTION_STATIC) continue;
-        vec3 p = ld3(B.pos[b]) + ld3(B.vel[b]) * P.ts.substep_dt;
-        B.pos[b] = st3(p);
-        quat rq = rotation_quaternion(ld3(B.angvel[b]), P.ts.substep_dt);
-        B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+        vec3 p = ld3(bld<Coh>(B.pos, cb.pos, b)) + ld3(bld<Coh>(B.vel, cb.vel, b)) * P.ts.substep_dt;
+        bst<Coh>(B.pos, cb.pos, b, st3(p));
+        quat rq = rotation_quaternion(ld3(bld<Coh>(B.angvel, cb.angvel, b)), P.ts.substep_dt);
+        bst<Coh>(B.quat, cb.quat, b, stq(qnormalize(qmul(rq, ldq(bld<Coh>(B.quat, cb.quat, b))))));
     }
 }
 
 // Barrier solver: one grid barrier per colour batch per pass.
+template <bool Coh = false>
 __device__ __forceinline__ void solve_batches(const SolveArgs& P, int phase, int nc) {
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     const Bodies& B = P.B;
+    CohBodies cb = coh_bodies(B, P.nb);
     for (int c = 0; c <= nc; ++c) {
         int col = c < nc ? c : kMaxColours;
         int beg = P.offs[col], end = P.offs[col + 1];
@@ -1415,32 +1480,45 @@ __device__ __forceinline__ void solve_batches(const SolveArgs& P, int phase, int
             for (int k = beg + first; k < end; k += step) {
                 jh_arbiter& a = P.arb[uint32_t(P.order[k])];
                 BodyReg b1, b2;
-                b1.v = ld3(B.vel[a.body1]); b1.w = ld3(B.angvel[a.body1]); load_static_part(B, a.body1, b1);
-                b2.v = ld3(B.vel[a.body2]); b2.w = ld3(B.angvel[a.body2]); load_static_part(B, a.body2, b2);
-                arbiter_pass(a, b1, b2, B, phase, P.ts);
-                if (a.mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) { B.vel[a.body1] = st3(b1.v); B.angvel[a.body1] = st3(b1.w); }
-                if (a.mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) { B.vel[a.body2] = st3(b2.v); B.angvel[a.body2] = st3(b2.w); }
+                b1.v = ld3(bld<Coh>(B.vel, cb.vel, a.body1));
+                b1.w = ld3(bld<Coh>(B.angvel, cb.angvel, a.body1));
+                load_static_part(B, a.body1, b1);
+                b2.v = ld3(bld<Coh>(B.vel, cb.vel, a.body2));
+                b2.w = ld3(bld<Coh>(B.angvel, cb.angvel, a.body2));
+                load_static_part(B, a.body2, b2);
+                arbiter_pass<Coh>(a, b1, b2, B, cb, phase, P.ts);
+                if (a.mode & (JH_SOLVE_LINEAR1 | JH_SOLVE_ANGULAR1)) {
+                    bst<Coh>(B.vel, cb.vel, a.body1, st3(b1.v));
+                    bst<Coh>(B.angvel, cb.angvel, a.body1, st3(b1.w));
+                }
+                if (a.mode & (JH_SOLVE_LINEAR2 | JH_SOLVE_ANGULAR2)) {
+                    bst<Coh>(B.vel, cb.vel, a.body2, st3(b2.v));
+                    bst<Coh>(B.angvel, cb.angvel, a.body2, st3(b2.w));
+                }
             }
-        grid_barrier(P.bar);
+        sync_grid<Coh>(P.bar);
     }
 }
 
 // PrepareConstraints / IterateConstraints (World.Step.cs:621-716) after the contacts of the same pass:
 // phase 0 prepares, phase 1 iterates.  One grid barrier per batch.
-__device__ __forceinline__ void con_load(const Bodies& B, int b, con::CBody& r) {
-    r.v = ld3(B.vel[b]);
-    r.w = ld3(B.angvel[b]);
-    r.pos = ld3(B.pos[b]);
-    r.q = ldq(B.quat[b]);
+template <bool Coh>
+__device__ __forceinline__ void con_load(const Bodies& B, const CohBodies& cb, int b, con::CBody& r) {
+    r.v = ld3(bld<Coh>(B.vel, cb.vel, b));
+    r.w = ld3(bld<Coh>(B.angvel, cb.angvel, b));
+    r.pos = ld3(bld<Coh>(B.pos, cb.pos, b));
+    r.q = ldq(bld<Coh>(B.quat, cb.quat, b));
     r.imv = ld3(B.imv[b]);
     r.I = ld_iw(B, b);
     r.dyn = B.motion[b] == JH_MOTION_DYNAMIC;
 }
+template <bool Coh = false>
 __device__ __forceinline__ void solve_constraint_batches(const SolveArgs& P, int phase) {
     const ConArgs& C = P.con;
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     const Bodies& B = P.B;
+    CohBodies cb = coh_bodies(B, P.nb);
     for (int b = 0; b < C.nbatch; ++b) {
         int beg = C.offs[b], end = C.offs[b + 1];
         for (int k = beg + t0; k < end; k += stride) {
@@ -1448,16 +1526,22 @@ __device__ __forceinline__ void solve_constraint_batches(const SolveArgs& P, int
             jh_constraint c = C.c[i];
             con::Work w = C.work[i];
             con::CBody A, Bb;
-            con_load(B, c.body1, A);
-            con_load(B, c.body2, Bb);
+            con_load<Coh>(B, cb, c.body1, A);
+            con_load<Coh>(B, cb, c.body2, Bb);
             if (phase == 0) con::prepare(c, w, A, Bb, P.ts.inv_dt, P.ts.inv_substep_dt, P.ts.substep_dt);
             else con::iterate(c, w, A, Bb, P.ts.inv_substep_dt);
             for (int q = 0; q < 8; ++q) C.c[i].state[q] = c.state[q];
             if (phase == 0) C.work[i] = w;
-            if (A.dyn) { B.vel[c.body1] = st3(A.v); B.angvel[c.body1] = st3(A.w); }
-            if (Bb.dyn) { B.vel[c.body2] = st3(Bb.v); B.angvel[c.body2] = st3(Bb.w); }
+            if (A.dyn) {
+                bst<Coh>(B.vel, cb.vel, c.body1, st3(A.v));
+                bst<Coh>(B.angvel, cb.angvel, c.body1, st3(A.w));
+            }
+            if (Bb.dyn) {
+                bst<Coh>(B.vel, cb.vel, c.body2, st3(Bb.v));
+                bst<Coh>(B.angvel, cb.angvel, c.body2, st3(Bb.w));
+            }
         }
-        grid_barrier(P.bar);
+        sync_grid<Coh>(P.bar);
     }
 }
 
@@ -1466,17 +1550,17 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = *P.ncolours;
     for (int s = 0; s < P.ts.substeps; ++s) {
-        integrate_forces(P, t0, stride);
-        grid_barrier(P.bar);
-        solve_batches(P, 0, nc);
-        solve_constraint_batches(P, 0);
+        integrate_forces<true>(P, t0, stride);
+        grid_barrier_coh(P.bar);
+        solve_batches<true>(P, 0, nc);
+        solve_constraint_batches<true>(P, 0);
         for (int it = 0; it < P.iterations; ++it) {
-            solve_batches(P, 1, nc);
-            solve_constraint_batches(P, 1);
+            solve_batches<true>(P, 1, nc);
+            solve_constraint_batches<true>(P, 1);
         }
-        integrate_velocities(P, t0, stride);
-        grid_barrier(P.bar);
-        for (int it = 0; it < P.relaxations; ++it) solve_batches(P, 2, nc);
+        integrate_velocities<true>(P, t0, stride);
+        grid_barrier_coh(P.bar);
+        for (int it = 0; it < P.relaxations; ++it) solve_batches<true>(P, 2, nc);
     }
 }
 
@@ -1566,8 +1650,6 @@ __global__ void __launch_bounds__(256) k_soa_scatter(jh_arbiter* __restrict__ ar
 // (single-copy atomic within a cache line), so a half with the expected count holds that visit's
 // values.  The accesses carry the sc1 bit (device coherence: served past the L2 of the issuing
 // XCD), like the compiler's agent-scope atomics.
-constexpr int kBufferDword3 = 0x00020000;  // raw buffer resource word 3 for gfx9-family targets
-constexpr int kCohSc1 = 16;                // cache-policy operand: sc1
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t vw_rsrc(const SolveArgs& P) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)P.vw, 0, P.nb * 32, kBufferDword3);
 }
