@@ -57,6 +57,34 @@ __device__ __forceinline__ void grid_barrier(unsigned* bar) {
     __syncthreads();
 }
 
+// Device-coherent 4/8-byte accesses (agent-scope relaxed atomics: sc1 loads and stores, served past
+// the L2 of the issuing XCD), for data that workgroups share between relaxed grid barriers.
+template <class T> __device__ __forceinline__ T cld(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void cst(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Grid barrier for device-coherent accesses (cld / cst, coh_ld / coh_st, device-scope atomics): each wave
+// waits for its own loads and stores to complete (vmcnt 0; the workgroup barrier alone does not wait for
+// them), then one relaxed arrival per workgroup — no L2 write-back or invalidate.
+__device__ __forceinline__ void grid_barrier_coh(unsigned* bar) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left at their maxima
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned* count = bar;
+        unsigned* gen = bar + 32;
+        unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
 // Slot in a list appended to by many threads: one global atomic per wave (ballot + prefix count)
 // instead of one per element.  Call with the lanes that may append active; returns -1 for !pred.
 __device__ __forceinline__ int wave_append(int* counter, bool pred) {
@@ -1050,25 +1078,25 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
     for (int a = t0; a < A; a += stride) {
         const jh_arbiter& r = P.arb[a];
         bool active = (r.usage_mask & 15u) != 0;
-        P.colour[a] = active ? -1 : -2;
-        bmax[0][r.body1] = 0;
-        bmax[0][r.body2] = 0;
-        P.mask_lo[r.body1] = 0;
-        P.mask_lo[r.body2] = 0;
-        P.mask_hi[r.body1] = 0;
-        P.mask_hi[r.body2] = 0;
+        cst(&P.colour[a], active ? -1 : -2);
+        cst(&bmax[0][r.body1], 0ull);
+        cst(&bmax[0][r.body2], 0ull);
+        cst(&P.mask_lo[r.body1], 0ull);
+        cst(&P.mask_lo[r.body2], 0ull);
+        cst(&P.mask_hi[r.body1], 0ull);
+        cst(&P.mask_hi[r.body2], 0ull);
     }
-    if (t0 < 128) P.hist[t0] = 0;
-    if (t0 == 0) { P.wcount[0] = 0; P.wcount[1] = 0; *P.ncolours = 0; }
-    grid_barrier(P.bar);
+    if (t0 < 128) cst(&P.hist[t0], 0);
+    if (t0 == 0) { cst(&P.wcount[0], 0); cst(&P.wcount[1], 0); *P.ncolours = 0; }
+    grid_barrier_coh(P.bar);
     // persisting colours, and the initial worklist
     for (int a = t0; a < A; a += stride) {
         bool uncoloured = false;
-        if (P.colour[a] == -1) {
+        if (cld(&P.colour[a]) == -1) {
             const jh_arbiter& r = P.arb[a];
             uint32_t c = r.reserved;
             if (P.keep && c < uint32_t(kMaxColours)) {
-                P.colour[a] = int(c);
+                cst(&P.colour[a], int(c));
                 unsigned long long bit = 1ull << (c & 63);
                 unsigned long long* m1 = c < 64 ? P.mask_lo : P.mask_hi;
                 if (P.motion[r.body1] != JH_MOTION_STATIC) atomicOr(&m1[r.body1], bit);
@@ -1079,9 +1107,9 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
             }
         }
         int slot = wave_append(&P.wcount[0], uncoloured);
-        if (uncoloured) work[0][slot] = a;
+        if (uncoloured) cst(&work[0][slot], a);
     }
-    grid_barrier(P.bar);
+    grid_barrier_coh(P.bar);
     for (int round = 0;; ++round) {
         int cur = round & 1, nxt = cur ^ 1;
         int n = __hip_atomic_load(&P.wcount[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1089,58 +1117,58 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
         unsigned long long* bm = bmax[cur];
         unsigned long long* bn = bmax[nxt];
         for (int i = t0; i < n; i += stride) {
-            int a = work[cur][i];
+            int a = cld(&work[cur][i]);
             const jh_arbiter& r = P.arb[a];
             unsigned long long prio = colour_prio(r, a);
             if (P.motion[r.body1] != JH_MOTION_STATIC) atomicMax(&bm[r.body1], prio);
             if (P.motion[r.body2] != JH_MOTION_STATIC) atomicMax(&bm[r.body2], prio);
         }
-        if (t0 == 0) P.wcount[nxt] = 0;
-        grid_barrier(P.bar);
+        if (t0 == 0) cst(&P.wcount[nxt], 0);
+        grid_barrier_coh(P.bar);
         for (int i = t0; i < n; i += stride) {
-            int a = work[cur][i];
+            int a = cld(&work[cur][i]);
             const jh_arbiter& r = P.arb[a];
             unsigned long long prio = colour_prio(r, a);
             bool d1 = P.motion[r.body1] != JH_MOTION_STATIC, d2 = P.motion[r.body2] != JH_MOTION_STATIC;
-            bool win = (!d1 || bm[r.body1] == prio) && (!d2 || bm[r.body2] == prio);
-            bn[r.body1] = 0;
-            bn[r.body2] = 0;
+            bool win = (!d1 || cld(&bm[r.body1]) == prio) && (!d2 || cld(&bm[r.body2]) == prio);
+            cst(&bn[r.body1], 0ull);
+            cst(&bn[r.body2], 0ull);
             int slot = wave_append(&P.wcount[nxt], !win);
             if (!win) {
-                work[nxt][slot] = a;
+                cst(&work[nxt][slot], a);
                 continue;
             }
             unsigned long long lo = 0, hi = 0;
-            if (d1) { lo |= P.mask_lo[r.body1]; hi |= P.mask_hi[r.body1]; }
-            if (d2) { lo |= P.mask_lo[r.body2]; hi |= P.mask_hi[r.body2]; }
+            if (d1) { lo |= cld(&P.mask_lo[r.body1]); hi |= cld(&P.mask_hi[r.body1]); }
+            if (d2) { lo |= cld(&P.mask_lo[r.body2]); hi |= cld(&P.mask_hi[r.body2]); }
             int c;
             if (~lo) c = __ffsll((long long)~lo) - 1;
             else if (~hi) c = 64 + __ffsll((long long)~hi) - 1;
             else c = kMaxColours;
             if (c >= kMaxColours) c = kMaxColours;
-            else if (c < 64) {
-                if (d1) P.mask_lo[r.body1] |= (1ull << c);
-                if (d2) P.mask_lo[r.body2] |= (1ull << c);
+            else if (c < 64) {  // the round's only winner on these bodies: plain read-modify-write
+                if (d1) cst(&P.mask_lo[r.body1], cld(&P.mask_lo[r.body1]) | (1ull << c));
+                if (d2) cst(&P.mask_lo[r.body2], cld(&P.mask_lo[r.body2]) | (1ull << c));
             } else {
-                if (d1) P.mask_hi[r.body1] |= (1ull << (c - 64));
-                if (d2) P.mask_hi[r.body2] |= (1ull << (c - 64));
+                if (d1) cst(&P.mask_hi[r.body1], cld(&P.mask_hi[r.body1]) | (1ull << (c - 64)));
+                if (d2) cst(&P.mask_hi[r.body2], cld(&P.mask_hi[r.body2]) | (1ull << (c - 64)));
             }
-            P.colour[a] = c;
+            cst(&P.colour[a], c);  // read at the end by the workgroup that owns index a
             atomicAdd(&lhist[c], 1);
         }
-        grid_barrier(P.bar);
+        grid_barrier_coh(P.bar);
     }
     for (int a = t0; a < A; a += stride) {
-        int c = P.colour[a];
+        int c = cld(&P.colour[a]);
         P.arb[a].reserved = (c >= 0 && c < kMaxColours) ? uint32_t(c) : kNoColour;
     }
     __syncthreads();
     if (threadIdx.x < 128 && lhist[threadIdx.x]) atomicAdd(&P.hist[threadIdx.x], lhist[threadIdx.x]);
-    grid_barrier(P.bar);
+    grid_barrier_coh(P.bar);
     if (t0 == 0) {
         int nc = 0;
         for (int c = 0; c < kMaxColours; ++c)
-            if (P.hist[c] > 0) nc = c + 1;
+            if (cld(&P.hist[c]) > 0) nc = c + 1;
         *P.ncolours = nc;
     }
 }
@@ -1347,25 +1375,6 @@ template <bool Coh> __device__ __forceinline__ float4 bld(const float4* a, __amd
 template <bool Coh> __device__ __forceinline__ void bst(float4* a, __amdgpu_buffer_rsrc_t r, int i, float4 v) {
     if constexpr (Coh) coh_st(r, i, v);
     else a[i] = v;
-}
-// Grid barrier for the coherent accesses: each wave waits for its own loads and stores to complete
-// (vmcnt 0; the workgroup barrier alone does not wait for them), then one relaxed arrival per workgroup.
-__device__ __forceinline__ void grid_barrier_coh(unsigned* bar) {
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left at their maxima
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned* count = bar;
-        unsigned* gen = bar + 32;
-        unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __syncthreads();
 }
 template <bool Coh> __device__ __forceinline__ void sync_grid(unsigned* bar) {
     if constexpr (Coh) grid_barrier_coh(bar);
