@@ -93,6 +93,15 @@ def test_heap_bitexact_barrier_solver(monkeypatch):
     run_replay(scenes.heap(900, seed=5), steps=60, check_every=20)
 
 
+@pytest.mark.parametrize("blocks", ["1", "7", "64", "256"])
+def test_barrier_solver_launch_shapes(monkeypatch, blocks):
+    # the barrier solver's shared body state is device-coherent and its grid barrier relaxed: exact for
+    # any number of workgroups, including several per XCD and a single one
+    monkeypatch.setenv("JH_SOLVER", "barrier")
+    monkeypatch.setenv("JH_SOLVE_BLOCKS", blocks)
+    run_replay(scenes.heap(700, seed=12), steps=40, check_every=13)
+
+
 def test_dataflow_solver_launch_shapes(monkeypatch):
     # the dataflow solver is exact for any number of resident waves and any chunk ownership (taken
     # afresh from counters in every pass, taken in each phase's first pass and kept, round-robin)
@@ -104,23 +113,27 @@ def test_dataflow_solver_launch_shapes(monkeypatch):
             run_replay(scenes.heap(700, seed=11), steps=40, check_every=39)
 
 
-def test_solve_order_is_colouring():
+@pytest.mark.parametrize("colour_blocks", ["default", "1", "16", "128"])
+def test_solve_order_is_colouring(monkeypatch, colour_blocks):
+    # checked after every step: the colouring kernel's shared lists, masks and colours are
+    # device-coherent between its relaxed grid barriers, for any number of workgroups
+    if colour_blocks != "default":
+        monkeypatch.setenv("JH_COLOUR_BLOCKS", colour_blocks)
     bodies, shapes, verts, cfg, dt = scenes.heap(1200, seed=3)
     gw = GpuWorld(bodies, shapes, verts, cfg)
-    for _ in range(60):
-        gw.step(dt)
-    keys, offs = gw.solve_order()
-    arbs = gw.arbiters()
-    assert len(keys) >= len(arbs)  # arbiters were compacted after the solve
     body_of = shapes["body"]
     dyn = bodies["motion_type"] == 0
-    for b in range(len(offs) - 1):
-        k = keys[offs[b]:offs[b + 1]]
-        bl = np.concatenate([body_of[k[:, 0]], body_of[k[:, 1]]])
-        bl = bl[dyn[bl]]
-        serial = b == len(offs) - 2 and len(offs) - 1 > gw.stats()["colours"]
-        if not serial:
-            assert len(np.unique(bl)) == len(bl), f"batch {b} shares a dynamic body"
+    for step in range(60):
+        gw.step(dt)
+        keys, offs = gw.solve_order()
+        assert len(keys) >= len(gw.arbiters())  # arbiters were compacted after the solve
+        for b in range(len(offs) - 1):
+            k = keys[offs[b]:offs[b + 1]]
+            bl = np.concatenate([body_of[k[:, 0]], body_of[k[:, 1]]])
+            bl = bl[dyn[bl]]
+            serial = b == len(offs) - 2 and len(offs) - 1 > gw.stats()["colours"]
+            if not serial:
+                assert len(np.unique(bl)) == len(bl), f"step {step}: batch {b} shares a dynamic body"
 
 
 def test_step_argument_errors():
