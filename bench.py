@@ -16,6 +16,10 @@ Besides the JSON line's contract fields it reports:
   cpu_baseline — the oracle (CPU restatement, 1 thread) timed on a bounded sample: a few steps of the
                  same heap continued from the GPU state at the start of the timed region.
   learned_warm_start — residual probes of the settled heap (see learned_warm_start_leg).
+
+Other workloads (--workload): pyramid, batched, full (the other configurations), joints (4,096 jointed
+groups with constraints of every kind, solved by the barrier solver k_solve) and bullets (32,768
+projectiles with speculative contacts); their roofline object names the solver kernel that ran.
 """
 from __future__ import annotations
 
