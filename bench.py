@@ -3,8 +3,10 @@
 
 Workload: `scenes.heap(131072)` — dynamic spheres/boxes/capsules dropped into a static container,
 dt = 1/60, 4 substeps, 10 solver iterations, 4 relaxation iterations (the 32,768-body heap
-configuration scaled to 131,072 bodies, the body count the metric is quoted at).  W warmup steps let
-the heap form contacts; then K steps are timed with inputs resident in HBM.
+configuration scaled to 131,072 bodies, the body count the metric is quoted at).  The bodies start on a
+grid up to 182 m high, so the heap first settles for --settle untimed steps (600 by default: the top
+layers land after about 370), whatever --warmup is; then W warmup steps, the autotuner, and K timed
+steps with inputs resident in HBM.
 
 Besides the JSON line's contract fields it reports:
   roofline     — the solver kernel (k_solve_flow, the persistent dataflow solver; k_solve_tiles, one
@@ -39,8 +41,8 @@ HBM_PEAK_GBS = 8000.0
 
 
 class _Roctx:
-    """roctx range around the timed steps, so a kernel-trace profile of this command can be cut to
-    them (rocpd2summary --start-marker/--end-marker timed_steps).  No-op without the library."""
+    """roctx range around the timed steps, and roctxProfilerResume/Pause around them, so that
+    `rocprofv3 --selected-regions` collects exactly the timed steps.  No-op without the library."""
 
     def __init__(self):
         import ctypes
@@ -61,6 +63,14 @@ class _Roctx:
     def pop(self):
         if self.lib is not None:
             self.lib.roctxRangePop()
+
+    def resume(self):
+        if self.lib is not None and hasattr(self.lib, "roctxProfilerResume"):
+            self.lib.roctxProfilerResume(0)
+
+    def pause(self):
+        if self.lib is not None and hasattr(self.lib, "roctxProfilerPause"):
+            self.lib.roctxProfilerPause(0)
 
 
 def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: int, relax: int,
@@ -84,15 +94,98 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solve_flow_heap131072_traffi
 
 def measured_traffic(kernel: str, args) -> float | None:
     """HBM bytes per launch of the solver kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, each scaled by what it reads for a device-to-device copy of known
-    size).  That summary profiles the default run (131,072-body heap, 240 warmup steps, autotune,
-    60 timed steps, dataflow solver); any other run reports None."""
-    if (kernel != "k_solve_flow" or args.workload != "heap" or args.bodies != 131072 or args.warmup != 240
-            or args.steps != 60 or not os.path.exists(TRAFFIC_PROFILE)):
+    (FETCH_SIZE + WRITE_SIZE over the timed steps only, each scaled by what it reads for a
+    device-to-device copy of known size).  The summary records the workload it profiled (workload,
+    bodies, settle, warmup and timed steps); a run of any other workload reports None."""
+    if not os.path.exists(TRAFFIC_PROFILE):
         return None
     with open(TRAFFIC_PROFILE) as f:
         prof = json.load(f)
-    return float(prof["traffic_bytes_per_launch"]) if prof.get("kernel") == kernel else None
+    run = prof.get("run", {})
+    same = (prof.get("kernel") == kernel and run.get("workload") == args.workload and run.get("bodies") == args.bodies
+            and run.get("settle") == args.settle and run.get("warmup") == args.warmup and run.get("steps") == args.steps)
+    return float(prof["traffic_bytes_per_launch"]) if same else None
+
+
+def calibrate_copy(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB (hipMemcpy, the runtime's copy kernel) before the scene:
+    HBM traffic of known size, which scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes
+    hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+    hip.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
+    hip.hipFree.argtypes = [vp]
+    n = int(mib) << 20
+    src, dst = vp(), vp()
+    if hip.hipMalloc(ctypes.byref(src), n) or hip.hipMalloc(ctypes.byref(dst), n):
+        raise RuntimeError("calibration copy: hipMalloc failed")
+    if hip.hipMemset(src, 1, n) or hip.hipMemcpy(dst, src, n, 3) or hip.hipDeviceSynchronize():
+        raise RuntimeError("calibration copy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def _csv_rows(path: str) -> list[dict]:
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _kname(r: dict) -> str:
+    return r["Kernel_Name"].split("(")[0]
+
+
+def trace_summary(trace_csv: str, steps: int, top: int = 16) -> str:
+    """Per-kernel table of the timed steps of a `rocprofv3 --kernel-trace --output-format csv` run of
+    this script with --no-lws --no-cpu: the window starts after the solver launch that precedes the
+    last `steps` solver launches (settle, warmup and autotune steps fall before it)."""
+    rows = sorted(_csv_rows(trace_csv), key=lambda r: int(r["Start_Timestamp"]))
+    solves = [r for r in rows if _kname(r).startswith("jh::k_solve")]
+    if len(solves) <= steps:
+        raise SystemExit(f"{len(solves)} solver launches, need more than {steps}")
+    cut = int(solves[-steps - 1]["End_Timestamp"])
+    timed = [r for r in rows if int(r["Start_Timestamp"]) > cut]
+    agg: dict = {}
+    for r in timed:
+        d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        wg = int(r.get("Workgroup_Size_X") or r.get("Workgroup_Size") or 1)
+        grid = int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
+        a = agg.setdefault(_kname(r), [0, 0, grid // max(wg, 1), wg, r.get("VGPR_Count", ""),
+                                        r.get("Accum_VGPR_Count", ""), r.get("LDS_Block_Size", "")])
+        a[0] += 1
+        a[1] += d
+    busy = sum(a[1] for a in agg.values())
+    span = int(timed[-1]["End_Timestamp"]) - int(timed[0]["Start_Timestamp"])
+    out = [f"timed window: {len(timed)} dispatches, GPU busy {busy / 1e6:.2f} ms of {span / 1e6:.2f} ms wall "
+           f"({steps} steps)",
+           f"{'kernel':34s} {'calls':>6s} {'avg_us':>10s} {'per_step_ms':>12s} {'share':>7s} {'wgs':>6s} {'wg':>5s} "
+           f"{'vgpr':>5s} {'agpr':>5s} {'lds':>6s}"]
+    for n, a in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"{n:34s} {a[0]:6d} {a[1] / a[0] / 1e3:10.2f} {a[1] / steps / 1e6:12.3f} {100 * a[1] / busy:6.1f}% "
+                   f"{a[2]:6d} {a[3]:5d} {a[4]:>5s} {a[5]:>5s} {a[6]:>6s}")
+    return "\n".join(out)
+
+
+def counter_summary(counter_csv: str, kernel: str, steps: int, copy_mib: int) -> dict:
+    """Mean of each counter over the last `steps` launches of `kernel` (the timed steps), and the same
+    counter for the calibration copy of `copy_mib` MiB (runtime copy kernels before the first jh::
+    kernel): scale = bytes copied / counter bytes of the copy."""
+    rows = sorted(_csv_rows(counter_csv), key=lambda r: int(r["Dispatch_Id"]))
+    out: dict = {"kernel": kernel, "steps": steps, "copy_mib": copy_mib}
+    first_jh = min(int(r["Dispatch_Id"]) for r in rows if _kname(r).startswith("jh::"))
+    for name in sorted({r["Counter_Name"] for r in rows}):
+        rk = [r for r in rows if r["Counter_Name"] == name and _kname(r).endswith(kernel)]
+        per = {}
+        for r in rk:
+            per[int(r["Dispatch_Id"])] = per.get(int(r["Dispatch_Id"]), 0.0) + float(r["Counter_Value"])
+        last = [per[d] for d in sorted(per)][-steps:]
+        copy = sum(float(r["Counter_Value"]) for r in rows if r["Counter_Name"] == name and
+                   int(r["Dispatch_Id"]) < first_jh and "copyBuffer" in r["Kernel_Name"])
+        out[name] = {"kib_per_launch": float(np.mean(last)), "launches": len(last), "copy_kib": copy,
+                     "scale": (copy_mib * 1024.0 / copy) if copy else None}
+    return out
 
 
 def learned_warm_start_leg(gw, dt: float, cfg, settle_to: int, probe_iterations: int, timed: int) -> dict:
@@ -142,7 +235,10 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (heap default 60; pyramid 600)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (heap default 240; pyramid 0)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before the autotuner and the timed "
+                                                              "steps, after --settle (default 5; pyramid 0)")
+    ap.add_argument("--settle", type=int, default=None, help="untimed steps that let the scene form its contacts "
+                                                              "before everything else (heap default 600; others 0)")
     ap.add_argument("--bodies", type=int, default=131072)
     ap.add_argument("--workload", choices=("heap", "batched", "full", "pyramid", "joints", "bullets"), default="heap",
                     help="heap: one heap of --bodies bodies (default 131,072, the metric's configuration; 32,768 is "
@@ -161,11 +257,27 @@ def main() -> int:
     ap.add_argument("--no-lws", action="store_true", help="skip the learned-warm-start residual leg")
     ap.add_argument("--lws-settle", type=int, default=700)
     ap.add_argument("--lws-iterations", type=int, default=256)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (scales HBM traffic counters of a profiled run)")
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="print the timed steps' kernel table of a rocprofv3 kernel trace of this script, then exit")
+    ap.add_argument("--counter-summary", metavar="COUNTER_CSV",
+                    help="print the timed steps' counters of the solver kernel (rocprofv3 --pmc), then exit")
+    ap.add_argument("--kernel", default="k_solve_flow", help="kernel for --counter-summary")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary, args.steps or 60))
+        return 0
+    if args.counter_summary:
+        print(json.dumps(counter_summary(args.counter_summary, args.kernel, args.steps or 60, args.calibrate_copy),
+                         indent=1))
+        return 0
     if args.steps is None:
         args.steps = 600 if args.workload == "pyramid" else 60
+    if args.settle is None:  # the heap is timed once it has formed, not while it falls
+        args.settle = 600 if args.workload == "heap" else 0
     if args.warmup is None:  # bullets: time the impacts, not the pile they leave
-        args.warmup = {"pyramid": 0, "bullets": 10}.get(args.workload, 240)
+        args.warmup = {"pyramid": 0, "bullets": 10, "heap": 5}.get(args.workload, 240)
     if args.cpu_steps is None:
         args.cpu_steps = {"joints": 20}.get(args.workload, 3)
     if args.gpus != 1:
@@ -176,6 +288,8 @@ def main() -> int:
     from jitterphysics2_amd import scenes
     from jitterphysics2_amd.gpu import GpuWorld
 
+    if args.calibrate_copy > 0:
+        calibrate_copy(args.calibrate_copy)
     t0 = time.time()
     if args.workload == "batched":
         bodies, shapes, verts, cfg, dt = scenes.batched_worlds(args.worlds)
@@ -208,16 +322,19 @@ def main() -> int:
     relax = int(cfg["relaxation_iterations"][0])
 
     gw = GpuWorld(bodies, shapes, verts, cfg)
+    import ctypes
+    print(f"# native library {gw.lib._name} (jh_version at "
+          f"{ctypes.cast(gw.lib.jh_version, ctypes.c_void_p).value:#x})", flush=True)
     if cons is not None:
         gw.set_constraints(cons)
     t0 = time.time()
-    for i in range(args.warmup):
+    for i in range(args.settle + args.warmup):
         gw.step(dt)
-        if (i + 1) % 60 == 0:
+        if (i + 1) % 100 == 0:
             s = gw.stats()
-            print(f"# warmup {i + 1}/{args.warmup}: pairs {s['pairs']} arbiters {s['arbiters']} "
-                  f"colours {s['colours']} ms/step {s['ms_step']:.2f}", flush=True)
-    print(f"# warmup done in {time.time() - t0:.1f}s", flush=True)
+            print(f"# settle/warmup {i + 1}/{args.settle + args.warmup}: pairs {s['pairs']} arbiters {s['arbiters']} "
+                  f"contacts {s['contacts']} colours {s['colours']} ms/step {s['ms_step']:.2f}", flush=True)
+    print(f"# settle + warmup done in {time.time() - t0:.1f}s", flush=True)
     tuning = None
     if not args.no_autotune:
         tuning = gw.autotune(dt, rounds=2)
@@ -236,6 +353,8 @@ def main() -> int:
     stage_sum = dict.fromkeys(stage_keys, 0.0)
     marker = _Roctx()
     marker.push("timed_steps")
+    marker.resume()
+    first_step = gw.stats()["steps"]
     t_start = time.perf_counter()
     for _ in range(args.steps):
         gw.step(dt)  # returns after the step finished on the GPU
@@ -247,6 +366,7 @@ def main() -> int:
         contacts.append(s["contacts"])
         bytes_per_launch.append(solver_bytes(s, n_dyn, n_moving, substeps, iters, relax, n_cons))
     t_end = time.perf_counter()
+    marker.pause()
     marker.pop()
     wall = t_end - t_start
     value = n_dyn * args.steps / wall
@@ -254,7 +374,7 @@ def main() -> int:
     avg_solve_s = float(np.mean(solve_ms)) / 1000.0
     achieved = float(np.mean(bytes_per_launch)) / avg_solve_s / 1e9
     final = gw.stats()
-    lws = None if args.no_lws else learned_warm_start_leg(gw, dt, cfg, max(args.lws_settle, args.warmup + args.steps),
+    lws = None if args.no_lws else learned_warm_start_leg(gw, dt, cfg, max(args.lws_settle, gw.stats()["steps"]),
                                                           args.lws_iterations, 20)
     gw.close()
 
@@ -309,6 +429,9 @@ def main() -> int:
                     f"contacts against 0.2-thick static slabs, dt=1/60, {substeps} substeps, {iters} iterations, "
                     f"{relax} relaxations"),
                    "constraints": n_cons,
+                   "timed_steps": [first_step, first_step + args.steps],
+                   "settle": args.settle,
+                   "contacts_timed_mean": float(np.mean(contacts)),
                    "autotune": tuning,
                    "bodies": n_dyn, "pairs": final["pairs"], "arbiters": final["arbiters"],
                    "contacts": final["contacts"], "colours": final["colours"], "max_degree": final["max_degree"],

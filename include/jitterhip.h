@@ -39,7 +39,9 @@ enum {
     JH_ERR_HIP = -4,          /* a HIP runtime call failed                                            */
     JH_ERR_UNSUPPORTED = -5,  /* a feature outside the GPU path (motion locks, gyroscopic forces ...)  */
     JH_ERR_STATE = -6,        /* ObjectDisposedException / call order errors                          */
-    JH_ERR_NO_DEVICE = -7     /* no HIP device visible                                                */
+    JH_ERR_NO_DEVICE = -7,    /* no HIP device visible                                                */
+    JH_ERR_TIMEOUT = -8       /* a persistent kernel's bounded wait gave up, or jh_world_cancel: no
+                                 counterpart in the reference (its solver runs on host threads)       */
 };
 
 /* MotionType, Dynamics/RigidBody.cs:75-95 (bits 0-1 of RigidBodyData.Flags). */
@@ -267,8 +269,19 @@ jh_status jh_world_set_shapes(jh_world* world, const jh_shape* shapes, int32_t c
 jh_status jh_world_get_rigidbodydata(jh_world* world, void* records, int32_t count, int32_t stride);
 
 /* World.Step(dt): one full step (narrowphase over the pair set, contact cache, substepped
- * solve, contact refresh, body update, broadphase).  Returns after the step finished on the GPU. */
+ * solve, contact refresh, body update, broadphase).  Returns after the step finished on the GPU.
+ * JH_ERR_TIMEOUT: a grid barrier or dependency wait of a persistent kernel gave up (see
+ * jh_world_set_spin_limit) or the step was cancelled; every kernel of the step has returned and the
+ * world stays usable, but its device state is unspecified until bodies and arbiters are set again. */
 jh_status jh_world_step(jh_world* world, float dt);
+
+/* Bounded device waits (no counterpart in the reference, whose solver runs on host threads; for callers
+ * that must never hang the GPU).  Each wait of a persistent kernel gives up after `polls` polls
+ * (0: the default, 2^23, several seconds) and ends the launch; the step then returns JH_ERR_TIMEOUT.
+ * jh_world_cancel raises the device abort flag, so the next step's persistent kernels end at their
+ * first wait and it returns JH_ERR_TIMEOUT (the flag is cleared when the step reports it). */
+jh_status jh_world_set_spin_limit(jh_world* world, uint32_t polls);
+jh_status jh_world_cancel(jh_world* world);
 
 /* Broadphase only, on the current state: the pair set {(Key1, Key2)} that
  * DynamicTree.EnumerateOverlaps + World.Detect gating would visit, sorted by key.

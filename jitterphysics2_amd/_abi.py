@@ -24,7 +24,9 @@ ERRORS = {
     -5: "unsupported feature",
     -6: "invalid state",
     -7: "no HIP device",
+    -8: "device wait timed out or step cancelled",
 }
+JH_ERR_TIMEOUT = -8
 
 CONFIG_DTYPE = np.dtype([
     ("gravity", "<f4", 3),
@@ -220,6 +222,8 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_world_set_shapes": ([vp, vp, i32, vp, i32], i32),
         "jh_world_get_rigidbodydata": ([vp, vp, i32, i32], i32),
         "jh_world_step": ([vp, ctypes.c_float], i32),
+        "jh_world_set_spin_limit": ([vp, ctypes.c_uint32], i32),
+        "jh_world_cancel": ([vp], i32),
         "jh_world_compute_pairs": ([vp, vp, i32, pi32], i32),
         "jh_world_get_manifolds": ([vp, vp, i32, pi32], i32),
         "jh_world_get_arbiters": ([vp, vp, i32, pi32], i32),

@@ -39,22 +39,78 @@ struct StepTime {
     int substeps;
 };
 
+// ---- Bounded waits ---------------------------------------------------------------------------
+// Barrier state of a world (64 words, zeroed at creation): [0] arrivals, [32] generation, [33] abort.
+// Generation and abort flag are read together with one 64-bit load, so a waiting workgroup sees an
+// abort at its next poll.  Every wait of a persistent kernel (grid barrier, dataflow readiness poll)
+// gives up after `spin` polls: it raises the abort flag, and every wave that sees the flag returns
+// from its kernel, so a launch always drains.  The host reads the flag after the step and reports
+// JH_ERR_TIMEOUT; the barrier state is then reset.
+constexpr unsigned kSpinLimitDefault = 1u << 23;  // polls per wait (each >= 1 device-coherent round trip)
+constexpr int kBarGen = 32, kBarAbort = 33;
+
+__device__ __forceinline__ void raise_abort(unsigned* bar) {
+    __hip_atomic_store(bar + kBarAbort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool abort_raised(unsigned* bar) {
+    return __hip_atomic_load(bar + kBarAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+// Bookkeeping of one poll loop (n: polls so far, uniform across the wave): false when the wait must
+// end, either because this wait exceeded the limit (the abort is raised here) or because another
+// wave raised it.  The flag itself is read every 64 polls.
+__device__ __forceinline__ bool keep_polling(unsigned* bar, unsigned& n, unsigned spin) {
+    ++n;
+    if (n >= spin) {
+        raise_abort(bar);
+        return false;
+    }
+    if ((n & 63u) == 0u) return !abort_raised(bar);
+    return true;
+}
+__device__ __forceinline__ unsigned long long gen_and_abort(unsigned* bar, int order) {
+    return order == __ATOMIC_ACQUIRE
+               ? __hip_atomic_load((unsigned long long*)(bar + kBarGen), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+               : __hip_atomic_load((unsigned long long*)(bar + kBarGen), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Grid barrier for a cooperative launch: one arrival per workgroup, sense by generation counter.
-__device__ __forceinline__ void grid_barrier(unsigned* bar) {
-    __syncthreads();
+// Returns false (uniformly over the workgroup) when the launch is aborting: the abort flag is set, a
+// wave of this workgroup arrives with local_ok false (it gave up a wait, and the flag is raised), or
+// this wait gives up.  Every wave of the workgroup must call it, also one that gave up.  The
+// generation snapshot decides whether to arrive at all, so it is complete before the arrival is
+// issued; the last arriver resets the count before it publishes the next generation (release order).
+__device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned spin, bool local_ok = true) {
+    __shared__ unsigned bar_ok;
+    bool failed = __syncthreads_or(!local_ok);
     if (threadIdx.x == 0) {
         unsigned* count = bar;
-        unsigned* gen = bar + 32;
-        unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+        unsigned* gen = bar + kBarGen;
+        unsigned long long ga = gen_and_abort(bar, __ATOMIC_RELAXED);
+        bool ok = !failed && (ga >> 32) == 0ull;
+        if (ok) {
+            unsigned g = unsigned(ga);
+            unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (arrived == gridDim.x - 1) {
+                __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned n = 0;
+                while (true) {
+                    unsigned long long x = gen_and_abort(bar, __ATOMIC_ACQUIRE);
+                    if (unsigned(x) != g) break;
+                    if ((x >> 32) != 0ull || ++n >= spin) {
+                        if (n >= spin) raise_abort(bar);
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
         }
+        bar_ok = ok ? 1u : 0u;
     }
     __syncthreads();
+    return bar_ok != 0u;
 }
 
 // Device-coherent 4/8-byte accesses (agent-scope relaxed atomics: sc1 loads and stores, served past
@@ -68,22 +124,46 @@ template <class T> __device__ __forceinline__ void cst(T* p, T v) {
 // Grid barrier for device-coherent accesses (cld / cst, coh_ld / coh_st, device-scope atomics): each wave
 // waits for its own loads and stores to complete (vmcnt 0; the workgroup barrier alone does not wait for
 // them), then one relaxed arrival per workgroup — no L2 write-back or invalidate.
-__device__ __forceinline__ void grid_barrier_coh(unsigned* bar) {
+// The generation snapshot feeds the abort branch, so it has been performed before the arrival is
+// issued (a snapshot taken after another workgroup completed the barrier would never be left); the
+// last arriver waits for its count reset to complete (vmcnt 0) before it publishes the next
+// generation, so a released workgroup's next arrival cannot be wiped by a late reset.
+__device__ __forceinline__ bool grid_barrier_coh(unsigned* bar, unsigned spin, bool local_ok = true) {
+    __shared__ unsigned bar_ok;
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt left at their maxima
-    __syncthreads();
+    bool failed = __syncthreads_or(!local_ok);
     if (threadIdx.x == 0) {
         unsigned* count = bar;
-        unsigned* gen = bar + 32;
-        unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == gridDim.x - 1) {
-            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) __builtin_amdgcn_s_sleep(1);
+        unsigned* gen = bar + kBarGen;
+        unsigned long long ga = gen_and_abort(bar, __ATOMIC_RELAXED);
+        bool ok = !failed && (ga >> 32) == 0ull;
+        if (ok) {
+            unsigned g = unsigned(ga);
+            unsigned arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (arrived == gridDim.x - 1) {
+                __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(0x0F70);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __hip_atomic_store(gen, g + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned n = 0;
+                while (true) {
+                    unsigned long long x = gen_and_abort(bar, __ATOMIC_RELAXED);
+                    if (unsigned(x) != g) break;
+                    if ((x >> 32) != 0ull || ++n >= spin) {
+                        if (n >= spin) raise_abort(bar);
+                        ok = false;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
         }
+        bar_ok = ok ? 1u : 0u;
     }
     __syncthreads();
+    return bar_ok != 0u;
 }
 // Slot in a list appended to by many threads: one global atomic per wave (ballot + prefix count)
 // instead of one per element.  Call with the lanes that may append active; returns -1 for !pred.
@@ -1055,7 +1135,8 @@ struct ColourArgs {
     int* wcount;         // [2] worklist sizes
     int* ncolours;
     int* hist;           // [128]
-    unsigned* bar;       // grid barrier state
+    unsigned* bar;       // grid barrier state and abort flag
+    unsigned spin;       // polls before a wait gives up (raises the abort flag)
     int keep;            // reuse the colours of arbiters that persist from the previous step
 };
 
@@ -1088,7 +1169,7 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
     }
     if (t0 < 128) cst(&P.hist[t0], 0);
     if (t0 == 0) { cst(&P.wcount[0], 0); cst(&P.wcount[1], 0); *P.ncolours = 0; }
-    grid_barrier_coh(P.bar);
+    if (!grid_barrier_coh(P.bar, P.spin)) return;
     // persisting colours, and the initial worklist
     for (int a = t0; a < A; a += stride) {
         bool uncoloured = false;
@@ -1109,7 +1190,7 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
         int slot = wave_append(&P.wcount[0], uncoloured);
         if (uncoloured) cst(&work[0][slot], a);
     }
-    grid_barrier_coh(P.bar);
+    if (!grid_barrier_coh(P.bar, P.spin)) return;
     for (int round = 0;; ++round) {
         int cur = round & 1, nxt = cur ^ 1;
         int n = __hip_atomic_load(&P.wcount[cur], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1124,7 +1205,7 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
             if (P.motion[r.body2] != JH_MOTION_STATIC) atomicMax(&bm[r.body2], prio);
         }
         if (t0 == 0) cst(&P.wcount[nxt], 0);
-        grid_barrier_coh(P.bar);
+        if (!grid_barrier_coh(P.bar, P.spin)) return;
         for (int i = t0; i < n; i += stride) {
             int a = cld(&work[cur][i]);
             const jh_arbiter& r = P.arb[a];
@@ -1156,7 +1237,7 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
             cst(&P.colour[a], c);  // read at the end by the workgroup that owns index a
             atomicAdd(&lhist[c], 1);
         }
-        grid_barrier_coh(P.bar);
+        if (!grid_barrier_coh(P.bar, P.spin)) return;
     }
     for (int a = t0; a < A; a += stride) {
         int c = cld(&P.colour[a]);
@@ -1164,7 +1245,7 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
     }
     __syncthreads();
     if (threadIdx.x < 128 && lhist[threadIdx.x]) atomicAdd(&P.hist[threadIdx.x], lhist[threadIdx.x]);
-    grid_barrier_coh(P.bar);
+    if (!grid_barrier_coh(P.bar, P.spin)) return;
     if (t0 == 0) {
         int nc = 0;
         for (int c = 0; c < kMaxColours; ++c)
@@ -1376,9 +1457,9 @@ template <bool Coh> __device__ __forceinline__ void bst(float4* a, __amdgpu_buff
     if constexpr (Coh) coh_st(r, i, v);
     else a[i] = v;
 }
-template <bool Coh> __device__ __forceinline__ void sync_grid(unsigned* bar) {
-    if constexpr (Coh) grid_barrier_coh(bar);
-    else grid_barrier(bar);
+template <bool Coh> __device__ __forceinline__ bool sync_grid(unsigned* bar, unsigned spin) {
+    if constexpr (Coh) return grid_barrier_coh(bar, spin);
+    else return grid_barrier(bar, spin);
 }
 
 // phase 0: prepare, 1: iterate with bias, 2: iterate without bias (relaxation)
@@ -1438,7 +1519,8 @@ struct SolveArgs {
     int* queue;              // k_solve_flow: chunk counters, 2 per substep
     int qmode;               // k_solve_flow chunk ownership: 0 round-robin, 1 first pass of each phase,
                              // 2 taken afresh in every pass
-    unsigned* bar;           // grid barrier state
+    unsigned* bar;           // grid barrier state and abort flag
+    unsigned spin;           // polls before a wait gives up (raises the abort flag)
     Bodies B;
     int nb;
     StepTime ts;
@@ -1472,9 +1554,9 @@ __device__ __forceinline__ void integrate_velocities(const SolveArgs& P, int t0,
     }
 }
 
-// Barrier solver: one grid barrier per colour batch per pass.
+// Barrier solver: one grid barrier per colour batch per pass.  False when the launch is aborting.
 template <bool Coh = false>
-__device__ __forceinline__ void solve_batches(const SolveArgs& P, int phase, int nc) {
+__device__ __forceinline__ bool solve_batches(const SolveArgs& P, int phase, int nc) {
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     const Bodies& B = P.B;
@@ -1505,8 +1587,9 @@ __device__ __forceinline__ void solve_batches(const SolveArgs& P, int phase, int
                     bst<Coh>(B.angvel, cb.angvel, a.body2, st3(b2.w));
                 }
             }
-        sync_grid<Coh>(P.bar);
+        if (!sync_grid<Coh>(P.bar, P.spin)) return false;
     }
+    return true;
 }
 
 // PrepareConstraints / IterateConstraints (World.Step.cs:621-716) after the contacts of the same pass:
@@ -1522,7 +1605,7 @@ __device__ __forceinline__ void con_load(const Bodies& B, const CohBodies& cb, i
     r.dyn = B.motion[b] == JH_MOTION_DYNAMIC;
 }
 template <bool Coh = false>
-__device__ __forceinline__ void solve_constraint_batches(const SolveArgs& P, int phase) {
+__device__ __forceinline__ bool solve_constraint_batches(const SolveArgs& P, int phase) {
     const ConArgs& C = P.con;
     int stride = gridDim.x * blockDim.x;
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1550,8 +1633,9 @@ __device__ __forceinline__ void solve_constraint_batches(const SolveArgs& P, int
                 bst<Coh>(B.angvel, cb.angvel, c.body2, st3(Bb.w));
             }
         }
-        sync_grid<Coh>(P.bar);
+        if (!sync_grid<Coh>(P.bar, P.spin)) return false;
     }
+    return true;
 }
 
 __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
@@ -1560,16 +1644,17 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
     int nc = *P.ncolours;
     for (int s = 0; s < P.ts.substeps; ++s) {
         integrate_forces<true>(P, t0, stride);
-        grid_barrier_coh(P.bar);
-        solve_batches<true>(P, 0, nc);
-        solve_constraint_batches<true>(P, 0);
+        if (!grid_barrier_coh(P.bar, P.spin)) return;
+        if (!solve_batches<true>(P, 0, nc)) return;
+        if (!solve_constraint_batches<true>(P, 0)) return;
         for (int it = 0; it < P.iterations; ++it) {
-            solve_batches<true>(P, 1, nc);
-            solve_constraint_batches<true>(P, 1);
+            if (!solve_batches<true>(P, 1, nc)) return;
+            if (!solve_constraint_batches<true>(P, 1)) return;
         }
         integrate_velocities<true>(P, t0, stride);
-        grid_barrier_coh(P.bar);
-        for (int it = 0; it < P.relaxations; ++it) solve_batches<true>(P, 2, nc);
+        if (!grid_barrier_coh(P.bar, P.spin)) return;
+        for (int it = 0; it < P.relaxations; ++it)
+            if (!solve_batches<true>(P, 2, nc)) return;
     }
 }
 
@@ -1938,8 +2023,9 @@ __device__ __forceinline__ void load_slot_fixed(const ContactSoA& S, int s, int 
 // the dependency wait; the bodies' velocity records and the arbiter's versioned mutable fields are
 // polled until they show the awaited visit counts / passes, then the visit computes in registers and
 // publishes both.
+// False when the launch is aborting (a wait gave up here or elsewhere).
 template <bool PREPARE>
-__device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, bool apply_bias, bool active) {
+__device__ __forceinline__ bool flow_visit(const SolveArgs& P, int k, int g, bool apply_bias, bool active) {
     const Bodies& B = P.B;
     if (!active) k = 0;  // inactive lanes of the wave's last chunk mirror a valid arbiter but do nothing
     const ContactSoA& S = P.soa;
@@ -1981,6 +2067,7 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
     // showed the awaited version is not polled again (no other visit can change it before this one).
     bool done = !active;
     bool ok1 = !m1, ok2 = !m2, rows = false;
+    unsigned polls = 0;
     while (true) {
         asm volatile("" ::: "memory");
         if (!done) {
@@ -2007,7 +2094,8 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
                 done = true;
             }
         }
-        if (__all(done)) break;
+        if (__all(done)) return true;
+        if (!keep_polling(P.bar, polls, P.spin)) return false;
         __builtin_amdgcn_s_sleep(2);
     }
 }
@@ -2016,7 +2104,7 @@ __device__ __forceinline__ void flow_visit(const SolveArgs& P, int k, int g, boo
 // takes the oldest chunk nobody has taken, so a ready chunk never waits behind a slow chunk of the
 // same wave.  Deadlock-free: the oldest unfinished chunk has been taken and its predecessors are done.
 // Exact because an arbiter's mutable fields are versioned (load_mutable), whichever wave visits it.
-__device__ __forceinline__ void flow_passes_queue(const SolveArgs& P, int g_first, int npass, int phase_first,
+__device__ __forceinline__ bool flow_passes_queue(const SolveArgs& P, int g_first, int npass, int phase_first,
                                                   int phase_rest, int* queue) {
     int lane = threadIdx.x & 63;
     int J = *P.nchunks;
@@ -2025,15 +2113,15 @@ __device__ __forceinline__ void flow_passes_queue(const SolveArgs& P, int g_firs
         int t = 0;
         if (lane == 0) t = atomicAdd(queue, 1);
         t = __shfl(t, 0, 64);
-        if (t >= total) break;
+        if (t >= total) return true;
         int p = t / J, j = t - p * J;
         int g = g_first + p;
         int phase = p == 0 ? phase_first : phase_rest;
         int2 ch = P.chunks[j];
         int k = ch.x + lane;
         bool active = k < ch.y;
-        if (phase == 0) flow_visit<true>(P, k, g, false, active);
-        else flow_visit<false>(P, k, g, phase == 1, active);
+        bool ok = phase == 0 ? flow_visit<true>(P, k, g, false, active) : flow_visit<false>(P, k, g, phase == 1, active);
+        if (!ok) return false;
     }
 }
 
@@ -2043,16 +2131,13 @@ __device__ __forceinline__ void flow_passes_queue(const SolveArgs& P, int g_firs
 // takes at most kOwnMax chunks per phase; with more chunks than all waves can own, the phase uses the
 // round-robin ownership of flow_passes instead.
 constexpr int kOwnMax = 64;
-__device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int npass, int phase_first,
+__device__ __forceinline__ bool flow_passes(const SolveArgs& P, int g_first, int npass, int phase_first,
                                             int phase_rest);
-__device__ __forceinline__ void flow_passes_own(const SolveArgs& P, int g_first, int npass, int phase_first,
+__device__ __forceinline__ bool flow_passes_own(const SolveArgs& P, int g_first, int npass, int phase_first,
                                                 int phase_rest, int* queue, int* own) {
     int lane = threadIdx.x & 63;
     int J = *P.nchunks;
-    if (J > ((gridDim.x * blockDim.x) >> 6) * kOwnMax) {
-        flow_passes(P, g_first, npass, phase_first, phase_rest);
-        return;
-    }
+    if (J > ((gridDim.x * blockDim.x) >> 6) * kOwnMax) return flow_passes(P, g_first, npass, phase_first, phase_rest);
     int n = 0;
     for (int p = 0; p < npass; ++p) {
         int g = g_first + p;
@@ -2074,13 +2159,14 @@ __device__ __forceinline__ void flow_passes_own(const SolveArgs& P, int g_first,
             int2 ch = P.chunks[j];
             int k = ch.x + lane;
             bool active = k < ch.y;
-            if (phase == 0) flow_visit<true>(P, k, g, false, active);
-            else flow_visit<false>(P, k, g, phase == 1, active);
+            bool ok = phase == 0 ? flow_visit<true>(P, k, g, false, active) : flow_visit<false>(P, k, g, phase == 1, active);
+            if (!ok) return false;
         }
     }
+    return true;
 }
 
-__device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int npass, int phase_first,
+__device__ __forceinline__ bool flow_passes(const SolveArgs& P, int g_first, int npass, int phase_first,
                                             int phase_rest) {
     int lane = threadIdx.x & 63;
     int W = (gridDim.x * blockDim.x) >> 6;
@@ -2093,10 +2179,11 @@ __device__ __forceinline__ void flow_passes(const SolveArgs& P, int g_first, int
             int2 ch = P.chunks[j];
             int k = ch.x + lane;
             bool active = k < ch.y;
-            if (phase == 0) flow_visit<true>(P, k, g, false, active);
-            else flow_visit<false>(P, k, g, phase == 1, active);
+            bool ok = phase == 0 ? flow_visit<true>(P, k, g, false, active) : flow_visit<false>(P, k, g, phase == 1, active);
+            if (!ok) return false;
         }
     }
+    return true;
 }
 
 // One launch per step (World.Step.cs:234-253): per substep IntegrateForces | prepare + iterations
@@ -2120,11 +2207,13 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
             v = v + ld3(B.dvel[b]);
             st_vw(P, b, v, w, n);
         }
-        grid_barrier(P.bar);
-        if (P.qmode == 2) flow_passes_queue(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s);
-        else if (P.qmode == 1) flow_passes_own(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s, own);
-        else flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
-        grid_barrier(P.bar);
+        if (!grid_barrier(P.bar, P.spin)) return;
+        bool ok;
+        if (P.qmode == 2) ok = flow_passes_queue(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s);
+        else if (P.qmode == 1) ok = flow_passes_own(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s, own);
+        else ok = flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
+        // a wave that gave up still takes part in the barrier, which then fails for the workgroup
+        if (!grid_barrier(P.bar, P.spin, ok)) return;
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
             if (B.motion[b] == JH_MOTION_STATIC) continue;
             vec3 v, w;
@@ -2134,13 +2223,13 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
             quat rq = rotation_quaternion(w, P.ts.substep_dt);
             B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
         }
-        grid_barrier(P.bar);
+        if (!grid_barrier(P.bar, P.spin)) return;
         if (P.relaxations > 0) {
             int g0 = s * per_sub + 1 + P.iterations;
-            if (P.qmode == 2) flow_passes_queue(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1);
-            else if (P.qmode == 1) flow_passes_own(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1, own);
-            else flow_passes(P, g0, P.relaxations, 2, 2);
-            grid_barrier(P.bar);
+            if (P.qmode == 2) ok = flow_passes_queue(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1);
+            else if (P.qmode == 1) ok = flow_passes_own(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1, own);
+            else ok = flow_passes(P, g0, P.relaxations, 2, 2);
+            if (!grid_barrier(P.bar, P.spin, ok)) return;
         }
     }
     for (int b = t0; b < P.nb; b += stride) {
@@ -2247,8 +2336,9 @@ __device__ __forceinline__ void store_slot_rec(float4* q, const SlotReg& r, bool
 }
 
 // This lane's sequence over passes [g_first, g_first + npass) of one phase: the first pass prepares
-// when prep_first, the others iterate (with the penalty bias when `bias`).
-__device__ __forceinline__ void lane_passes(const SolveArgs& P, float4* __restrict__ rec, int A, int g_first,
+// when prep_first, the others iterate (with the penalty bias when `bias`).  False when the launch is
+// aborting.
+__device__ __forceinline__ bool lane_passes(const SolveArgs& P, float4* __restrict__ rec, int A, int g_first,
                                             int npass, bool prep_first, bool bias) {
     const Bodies& B = P.B;
     int NL = gridDim.x * blockDim.x;
@@ -2265,6 +2355,7 @@ __device__ __forceinline__ void lane_passes(const SolveArgs& P, float4* __restri
     vec3 x1{0, 0, 0}, x2{0, 0, 0};
     quat q1{0, 0, 0, 1}, q2{0, 0, 0, 1};
     __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
+    unsigned polls = 0;
     while (true) {
         asm volatile("" ::: "memory");
         if (!finished && !loaded) {
@@ -2326,8 +2417,13 @@ __device__ __forceinline__ void lane_passes(const SolveArgs& P, float4* __restri
                 }
             }
         }
-        if (__all(finished)) break;
-        if (!__any(progressed)) __builtin_amdgcn_s_sleep(2);
+        if (__all(finished)) return true;
+        if (__any(progressed)) {
+            polls = 0;
+        } else {
+            if (!keep_polling(P.bar, polls, P.spin)) return false;
+            __builtin_amdgcn_s_sleep(2);
+        }
     }
 }
 
@@ -2348,9 +2444,9 @@ __global__ void __launch_bounds__(256) k_solve_lanes(SolveArgs P, float4* __rest
             v = v + ld3(B.dvel[b]);
             st_vw(P, b, v, w, n);
         }
-        grid_barrier(P.bar);
-        lane_passes(P, rec, A, s * per_sub, 1 + P.iterations, true, true);
-        grid_barrier(P.bar);
+        if (!grid_barrier(P.bar, P.spin)) return;
+        bool ok = lane_passes(P, rec, A, s * per_sub, 1 + P.iterations, true, true);
+        if (!grid_barrier(P.bar, P.spin, ok)) return;
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
             if (B.motion[b] == JH_MOTION_STATIC) continue;
             vec3 v, w;
@@ -2359,10 +2455,10 @@ __global__ void __launch_bounds__(256) k_solve_lanes(SolveArgs P, float4* __rest
             quat rq = rotation_quaternion(w, P.ts.substep_dt);
             B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
         }
-        grid_barrier(P.bar);
+        if (!grid_barrier(P.bar, P.spin)) return;
         if (P.relaxations > 0) {
-            lane_passes(P, rec, A, s * per_sub + 1 + P.iterations, P.relaxations, false, false);
-            grid_barrier(P.bar);
+            ok = lane_passes(P, rec, A, s * per_sub + 1 + P.iterations, P.relaxations, false, false);
+            if (!grid_barrier(P.bar, P.spin, ok)) return;
         }
     }
     for (int b = t0; b < P.nb; b += stride) {
@@ -2589,10 +2685,8 @@ struct TileArgs {
     const uint32_t* shared; // per body: 1 if an arbiter of another tile touches it
     const uint32_t* toff;   // [T * 128 + 1]: start of (tile, colour) in tlist
     const uint64_t* tlist;  // (colour-order position << 32) | arbiter, tile-major, colour within tile
-    int* err;               // set when a dependency wait exceeds kTileSpinLimit polls (then all waits stop)
     int T;
 };
-constexpr int kTileSpinLimit = 1 << 20;
 __device__ __forceinline__ int tile_begin(int t, int M, int T) { return int((int64_t(t) * M + T - 1) / T); }
 
 // Body -> first position of one of its shapes in the sorted broadphase order.
@@ -2671,9 +2765,10 @@ __global__ void k_tile_serial(const uint64_t* __restrict__ order, const int* __r
 }
 
 // One visit of the arbiter at tile-list position p in global pass g.  Private bodies are read and
-// written in LDS; shared bodies wait for their counters and go through the coherent records.
+// written in LDS; shared bodies wait for their counters and go through the coherent records.  False
+// when the launch is aborting (then nothing is computed).
 template <bool PREPARE>
-__device__ __forceinline__ void tile_visit(const SolveArgs& P, const TileArgs& Tl, float4* lv, float2* lw, int p,
+__device__ __forceinline__ bool tile_visit(const SolveArgs& P, const TileArgs& Tl, float4* lv, float2* lw, int p,
                                            int g, bool apply_bias) {
     const Bodies& B = P.B;
     const ContactSoA& S = P.soa;
@@ -2711,15 +2806,11 @@ __device__ __forceinline__ void tile_visit(const SolveArgs& P, const TileArgs& T
     if (!m2) { R2.v = ld3(B.vel[b2]); R2.w = ld3(B.angvel[b2]); }
     else if (!sh2) { float4 a = lv[s2]; float2 c = lw[s2]; R2.v = {a.x, a.y, a.z}; R2.w = {a.w, c.x, c.y}; }
     __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
-    bool done = false, ok1 = !sh1, ok2 = !sh2;
-    int spins = 0;
+    bool done = false, ok1 = !sh1, ok2 = !sh2, gave_up = false;
+    unsigned polls = 0;
     while (true) {
         asm volatile("" ::: "memory");
-        if (!done && !(ok1 && ok2) && (++spins & 255) == 0) {
-            // bounded wait: a missing visit must end the launch with an error, not hang it
-            if (spins >= kTileSpinLimit) __hip_atomic_store(Tl.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__hip_atomic_load(Tl.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok1 = ok2 = true;
-        }
+        if (!done && !(ok1 && ok2) && !keep_polling(P.bar, polls, P.spin)) done = gave_up = true;
         if (!done) {
             if (!ok1) ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w);
             if (!ok2) ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w);
@@ -2742,33 +2833,34 @@ __device__ __forceinline__ void tile_visit(const SolveArgs& P, const TileArgs& T
                 done = true;
             }
         }
-        if (__all(done)) break;
+        if (__all(done)) return !gave_up;
         __builtin_amdgcn_s_sleep(2);
     }
 }
 
 // One pass of a tile: its arbiters colour by colour, a workgroup barrier after each non-empty
-// colour; the serial batch by one thread in solve order.
-__device__ __forceinline__ void tile_pass(const SolveArgs& P, const TileArgs& Tl, float4* lv, float2* lw,
+// colour; the serial batch by one thread in solve order.  False (for the whole workgroup) when the
+// launch is aborting.
+__device__ __forceinline__ bool tile_pass(const SolveArgs& P, const TileArgs& Tl, float4* lv, float2* lw,
                                           const uint32_t* segs, int nc, int g, int phase) {
+    bool ok = true;
     for (int c = 0; c < nc; ++c) {
         int beg = int(segs[c]), end = int(segs[c + 1]);
         if (beg == end) continue;
-        for (int p = beg + int(threadIdx.x); p < end; p += blockDim.x) {
-            if (phase == 0) tile_visit<true>(P, Tl, lv, lw, p, g, false);
-            else tile_visit<false>(P, Tl, lv, lw, p, g, phase == 1);
-        }
-        __syncthreads();
+        for (int p = beg + int(threadIdx.x); p < end && ok; p += blockDim.x)
+            ok = phase == 0 ? tile_visit<true>(P, Tl, lv, lw, p, g, false)
+                            : tile_visit<false>(P, Tl, lv, lw, p, g, phase == 1);
+        if (__syncthreads_or(!ok)) return false;
     }
     int beg = int(segs[kMaxColours]), end = int(segs[kMaxColours + 1]);
     if (beg < end) {
         if (threadIdx.x == 0)
-            for (int p = beg; p < end; ++p) {
-                if (phase == 0) tile_visit<true>(P, Tl, lv, lw, p, g, false);
-                else tile_visit<false>(P, Tl, lv, lw, p, g, phase == 1);
-            }
-        __syncthreads();
+            for (int p = beg; p < end && ok; ++p)
+                ok = phase == 0 ? tile_visit<true>(P, Tl, lv, lw, p, g, false)
+                                : tile_visit<false>(P, Tl, lv, lw, p, g, phase == 1);
+        if (__syncthreads_or(!ok)) return false;
     }
+    return true;
 }
 
 // One launch per step, grid = T tiles (cooperative); dynamic LDS holds 24 bytes per tile body.
@@ -2814,9 +2906,10 @@ __global__ void __launch_bounds__(256) k_solve_tiles(SolveArgs P, TileArgs Tl) {
             B.angvel[b] = st3(ld3(B.angvel[b]) + ld3(B.dangvel[b]));
             B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
         }
-        grid_barrier(P.bar);
-        for (int it = 0; it <= P.iterations; ++it) tile_pass(P, Tl, lv, lw, segs, nc, s * per_sub + it, it == 0 ? 0 : 1);
-        grid_barrier(P.bar);
+        if (!grid_barrier(P.bar, P.spin)) return;
+        for (int it = 0; it <= P.iterations; ++it)
+            if (!tile_pass(P, Tl, lv, lw, segs, nc, s * per_sub + it, it == 0 ? 0 : 1)) return;
+        if (!grid_barrier(P.bar, P.spin)) return;
         // IntegrateVelocities (World.Step.cs:1106-1133)
         for (int i = tb + int(threadIdx.x); i < te; i += blockDim.x) {
             int b = Tl.tbody[i];
@@ -2833,11 +2926,11 @@ __global__ void __launch_bounds__(256) k_solve_tiles(SolveArgs P, TileArgs Tl) {
             quat rq = rotation_quaternion(ld3(B.angvel[b]), P.ts.substep_dt);
             B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
         }
-        grid_barrier(P.bar);
+        if (!grid_barrier(P.bar, P.spin)) return;
         if (P.relaxations > 0) {
             for (int it = 0; it < P.relaxations; ++it)
-                tile_pass(P, Tl, lv, lw, segs, nc, s * per_sub + 1 + P.iterations + it, 2);
-            grid_barrier(P.bar);
+                if (!tile_pass(P, Tl, lv, lw, segs, nc, s * per_sub + 1 + P.iterations + it, 2)) return;
+            if (!grid_barrier(P.bar, P.spin)) return;
         }
     }
     for (int i = tb + int(threadIdx.x); i < te; i += blockDim.x) {

@@ -20,15 +20,15 @@ using namespace jh;
 
 namespace {
 
-thread_local std::string g_err = "";
+// Last error text of the calling thread (jh_last_error).  A plain array: nothing to destroy when a
+// thread or the process exits.
+thread_local char g_err[512] = "";
 
 jh_status fail(jh_status s, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
-    g_err = buf;
     return s;
 }
 
@@ -82,7 +82,6 @@ enum {
     C_SPEC = 15,      // speculative sweep queue
     C_NMOV = 16,      // tile solver: non-static bodies with shapes
     C_TCELLS = 17,    // tile solver: (tile, colour) cells + 1
-    C_SERR = 18,      // tile solver: a dependency wait gave up
     C_MAXDEG = 19,    // most solved arbiters on one non-static body
     C_ZOMB = 20,      // arbiters removed at the end of the last step (keys in zkey); kept across steps
     C_EVB = 21,       // BeginCollide events of this step
@@ -236,7 +235,9 @@ struct jh_world {
     DBuf<uint64_t> rk0, rk1;
     DBuf<int2> rank, chunks;
     DBuf<int> deg, cbeg;
-    DBuf<unsigned> bar;
+    DBuf<unsigned> bar;       // grid barrier state and abort flag of the persistent kernels (64 words)
+    unsigned spin_limit = 0;  // polls before a device wait gives up; 0: kSpinLimitDefault
+    unsigned* habort = nullptr;  // pinned host mirror of the abort flag
     bool recolour = true;  // next colouring ignores the colours stored in the arbiters
     int solve_count = 0, solve_batches = 0;
     std::vector<int> solve_offsets;
@@ -256,6 +257,7 @@ struct jh_world {
     bool have_step = false;
     int ev_begin = 0, ev_end = 0;  // collision events of the last step (in evb / eve)
 
+    unsigned spin() const { return spin_limit ? spin_limit : kSpinLimitDefault; }
     Bodies bodies() {
         Bodies B;
         B.pos = pos.p; B.quat = quat.p; B.vel = vel.p; B.angvel = angvel.p; B.dvel = dvel.p; B.dangvel = dangvel.p;
@@ -270,7 +272,7 @@ namespace {
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
     if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK ||
-        t == C_SERR || t == C_MAXDEG || t == C_SPEC || t == C_EVB || t == C_EVE)
+        t == C_MAXDEG || t == C_SPEC || t == C_EVB || t == C_EVE)
         cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
     else if (t < 6) bounds[t] = 0u;
@@ -752,6 +754,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         P.ncolours = cnt + C_NCOL;
         P.hist = w->chist.p;
         P.bar = w->bar.p;
+        P.spin = w->spin();
         // colours persist across steps; recolour everything after body changes or when the palette
         // has spread (first-fit over many steps can leave more colours than a fresh colouring)
         P.keep = (w->recolour || w->solve_batches > 48) ? 0 : 1;
@@ -801,6 +804,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.rank = w->rank.p;
         S.deg = w->deg.p;
         S.bar = w->bar.p;
+        S.spin = w->spin();
         S.B = B;
         S.nb = w->nb;
         S.ts = ts;
@@ -868,7 +872,6 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             Tl.shared = w->tshared.p;
             Tl.toff = w->toff.p;
             Tl.tlist = w->tlist.p;
-            Tl.err = cnt + C_SERR;
             Tl.T = w->tiles;
             void* targs[] = {&S, &Tl};
             HIPCHK(hipEventRecord(w->ev[8], st));
@@ -921,10 +924,22 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(w->ev[6], st));
     HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(w->habort, w->bar.p + kBarAbort, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     // DynamicTree.Update(multiThread, StepDt) (World.Step.cs:274-276)
     w->broadphase_dt = dt;
-    if (w->hcnt[C_SERR]) return fail(JH_ERR_HIP, "tile solver: a dependency wait did not finish");
+    if (*w->habort) {
+        // every persistent kernel of the step has drained; their barrier counts are left mid-way
+        HIPCHK(hipMemsetAsync(w->bar.p, 0, 64 * sizeof(unsigned), st));
+        HIPCHK(hipStreamSynchronize(st));
+        *w->habort = 0;
+        w->have_step = false;
+        w->recolour = true;
+        return fail(JH_ERR_TIMEOUT,
+                    "a persistent kernel's wait gave up after %u polls or the step was cancelled; the world's device "
+                    "state is unspecified until bodies and arbiters are set again",
+                    w->spin());
+    }
     // solve-order bookkeeping and stats
     w->a_host = w->hcnt[w->cur == 0 ? C_A0 : C_A1];
     w->solve_count = w->hcnt[cur == 0 ? C_A0 : C_A1];
@@ -969,7 +984,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
 
 extern "C" {
 
-const char* jh_last_error(void) { return g_err.c_str(); }
+const char* jh_last_error(void) { return g_err; }
 int32_t jh_version(void) { return 1; }
 
 // Sizes of the ABI structs (checked by bindings): 0 config, 1 body, 2 shape, 3 contact, 4 arbiter,
@@ -1023,7 +1038,9 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     w->cu_count = prop.multiProcessorCount;
     HIPCHK(hipStreamCreateWithFlags(&w->st, hipStreamNonBlocking));
     for (auto& e : w->ev) HIPCHK(hipEventCreate(&e));
-    HIPCHK(hipHostMalloc((void**)&w->hcnt, C_COUNT * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&w->hcnt, (C_COUNT + 1) * sizeof(int), hipHostMallocDefault));
+    w->habort = reinterpret_cast<unsigned*>(w->hcnt + C_COUNT);
+    *w->habort = 0;
     HIPCHK(w->cnt.ensure(C_COUNT));
     HIPCHK(hipMemset(w->cnt.p, 0, C_COUNT * sizeof(int)));
     HIPCHK(w->bounds.ensure(8));
@@ -1271,6 +1288,20 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_ZOMB, 0);
     w->a_host = 0;
     w->cur = 0;
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_set_spin_limit(jh_world* w, uint32_t polls) {
+    if (!w) return fail(JH_ERR_STATE, "world is NULL");
+    w->spin_limit = polls;
+    return JH_OK;
+}
+
+jh_status jh_world_cancel(jh_world* w) {
+    if (!w) return fail(JH_ERR_STATE, "world is NULL");
+    unsigned one = 1u;
+    HIPCHK(hipMemcpyAsync(w->bar.p + kBarAbort, &one, sizeof(unsigned), hipMemcpyHostToDevice, w->st));
     HIPCHK(hipStreamSynchronize(w->st));
     return JH_OK;
 }

@@ -580,16 +580,16 @@ __global__ void __launch_bounds__(256) k_probe(SolveArgs P, int K, double* __res
     int nc = *P.ncolours;
     int na = P.offs[kMaxColours + 1];
     integrate_forces(P, t0, stride);
-    grid_barrier(P.bar);
+    if (!grid_barrier(P.bar, P.spin)) return;
     for (int k = 0; k <= K; ++k) {
         if (k > 0) {
             for (int i = t0; i < na; i += stride) {
                 uint32_t ai = uint32_t(P.order[i]);
                 for (int s = 0; s < 4; ++s) lam_old[4 * size_t(ai) + s] = P.arb[ai].contacts[s].accumulated[0];
             }
-            grid_barrier(P.bar);
+            if (!grid_barrier(P.bar, P.spin)) return;
         }
-        solve_batches(P, k == 0 ? 0 : 1, nc);
+        if (!solve_batches(P, k == 0 ? 0 : 1, nc)) return;
         double s = 0.0;
         for (int i = t0; i < na; i += stride) {
             uint32_t ai = uint32_t(P.order[i]);
@@ -602,7 +602,7 @@ __global__ void __launch_bounds__(256) k_probe(SolveArgs P, int K, double* __res
             __syncthreads();
         }
         if (threadIdx.x == 0) part[size_t(k) * gridDim.x + blockIdx.x] = red[0];
-        grid_barrier(P.bar);
+        if (!grid_barrier(P.bar, P.spin)) return;
     }
 }
 

@@ -56,6 +56,16 @@ class GpuWorld:
     def step(self, dt: float) -> None:
         check(self.lib, self.lib.jh_world_step(self.handle, ctypes.c_float(dt)), "jh_world_step")
 
+    def step_status(self, dt: float) -> int:
+        """jh_world_step's status code, without raising."""
+        return int(self.lib.jh_world_step(self.handle, ctypes.c_float(dt)))
+
+    def set_spin_limit(self, polls: int) -> None:
+        check(self.lib, self.lib.jh_world_set_spin_limit(self.handle, ctypes.c_uint32(polls)), "jh_world_set_spin_limit")
+
+    def cancel(self) -> None:
+        check(self.lib, self.lib.jh_world_cancel(self.handle), "jh_world_cancel")
+
     def bodies(self) -> np.ndarray:
         out = np.zeros(self.n_bodies, BODY_DTYPE)
         check(self.lib, self.lib.jh_world_get_bodies(self.handle, ptr(out), self.n_bodies), "jh_world_get_bodies")

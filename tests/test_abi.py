@@ -68,3 +68,10 @@ def test_scene_builders_are_seeded_and_consistent():
     # unit box: mass 1, inverse inertia 6 (BoxShape.CalculateMassInertia)
     m, inertia = scenes.box_mass((1, 1, 1))
     assert m == 1.0 and abs(scenes._sym_inverse(*inertia)[0] - 6.0) < 1e-6
+
+
+def test_wait_control_rejects_null_world_without_a_device():
+    lib = _abi.load_library()
+    assert lib.jh_world_set_spin_limit(None, 1) == -6
+    assert lib.jh_world_cancel(None) == -6
+    assert _abi.ERRORS[_abi.JH_ERR_TIMEOUT].startswith("device wait timed out")
