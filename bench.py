@@ -40,39 +40,6 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-class _Roctx:
-    """roctx range around the timed steps, and roctxProfilerResume/Pause around them, so that
-    `rocprofv3 --selected-regions` collects exactly the timed steps.  No-op without the library."""
-
-    def __init__(self):
-        import ctypes
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so.1", "libroctx64.so.4"):
-            try:
-                self.lib = ctypes.CDLL(name)
-                break
-            except OSError:
-                continue
-        if self.lib is not None:
-            self.lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
-
-    def push(self, name: str):
-        if self.lib is not None:
-            self.lib.roctxRangePushA(name.encode())
-
-    def pop(self):
-        if self.lib is not None:
-            self.lib.roctxRangePop()
-
-    def resume(self):
-        if self.lib is not None and hasattr(self.lib, "roctxProfilerResume"):
-            self.lib.roctxProfilerResume(0)
-
-    def pause(self):
-        if self.lib is not None and hasattr(self.lib, "roctxProfilerPause"):
-            self.lib.roctxProfilerPause(0)
-
-
 def solver_bytes(stats: dict, n_dyn: int, n_moving: int, substeps: int, iters: int, relax: int,
                  n_cons: int = 0) -> float:
     """Algorithmic HBM bytes of one solver launch (one step, all substeps).
@@ -351,9 +318,6 @@ def main() -> int:
     solve_ms, step_ms, bytes_per_launch, contacts = [], [], [], []
     stage_keys = ("ms_broadphase", "ms_narrowphase", "ms_epa", "ms_contact_cache", "ms_colouring", "ms_solve", "ms_update")
     stage_sum = dict.fromkeys(stage_keys, 0.0)
-    marker = _Roctx()
-    marker.push("timed_steps")
-    marker.resume()
     first_step = gw.stats()["steps"]
     t_start = time.perf_counter()
     for _ in range(args.steps):
@@ -366,8 +330,6 @@ def main() -> int:
         contacts.append(s["contacts"])
         bytes_per_launch.append(solver_bytes(s, n_dyn, n_moving, substeps, iters, relax, n_cons))
     t_end = time.perf_counter()
-    marker.pause()
-    marker.pop()
     wall = t_end - t_start
     value = n_dyn * args.steps / wall
     ms_per_step = 1000.0 * wall / args.steps

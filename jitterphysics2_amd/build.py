@@ -28,7 +28,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd))
     # native driver (host C++ only, links the library next to it)
     cli = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-o", CLI + ".tmp",
-           os.path.join(CSRC, "jh_cli.cpp"), "-L" + PKG, "-ljitterhip", "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx",
+           os.path.join(CSRC, "jh_cli.cpp"), "-L" + PKG, "-ljitterhip", "-L/opt/rocm/lib",
            "-lamdhip64", "-Wl,-rpath,$ORIGIN:/opt/rocm/lib"]
     r = subprocess.run(cli, capture_output=True, text=True)
     if r.returncode != 0:

@@ -6,10 +6,9 @@
 // int32 counts (bodies, shapes, vertices), float dt, jh_config, jh_body[], jh_shape[], float[3*nv];
 // version 2 appends int32 nc and jh_constraint[nc].
 // After the warmup the launch shapes are autotuned as in the benchmark.  Prints the mean per-stage
-// timings (HIP events inside the library) and the wall time of the timed steps, which are bracketed
-// by the roctx range "timed_steps" for kernel-trace profiles.
+// timings (HIP events inside the library) and the wall time of the timed steps.  A kernel-trace
+// profile is cut to the timed steps by their solver launches (the last `steps` of them).
 #include <hip/hip_runtime_api.h>
-#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <chrono>
 #include <cstdio>
@@ -103,7 +102,6 @@ int main(int argc, char** argv) {
     fflush(stdout);
     double sum[9] = {0};
     jh_stats st;
-    roctxRangePush("timed_steps");
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < steps; ++i) {
         if ((s = jh_world_step(w, dt))) return die("jh_world_step", s);
@@ -113,7 +111,6 @@ int main(int argc, char** argv) {
         for (int k = 0; k < 9; ++k) sum[k] += v[k];
     }
     auto t1 = std::chrono::steady_clock::now();
-    roctxRangePop();
     double wall = std::chrono::duration<double>(t1 - t0).count();
     for (double& x : sum) x /= steps > 0 ? steps : 1;
     int ndyn = 0;
