@@ -15,8 +15,10 @@ Besides the JSON line's contract fields it reports:
                  its launch on the library's stream (jh_stats.ms_solve_kernel); algorithmic bytes per
                  launch from the per-step counts (see solver_bytes()); traffic from the committed
                  counter summary when this is the profiled run (see measured_traffic()).
-  cpu_baseline — the oracle (CPU restatement, 1 thread) timed on a bounded sample: a few steps of the
-                 same heap continued from the GPU state at the start of the timed region.
+  cpu_baseline — the oracle (CPU restatement) on the host's threads (OMP_NUM_THREADS), timed on a
+                 bounded sample: a few steps of the same heap continued from the GPU state at the
+                 start of the timed region (the narrowphase and per-body work in parallel, the
+                 Gauss-Seidel solve sequential).
   learned_warm_start — residual probes of the settled heap (see learned_warm_start_leg).
 
 Other workloads (--workload): pyramid, batched, full (the other configurations), joints (4,096 jointed
@@ -342,17 +344,23 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
+        from tests import orc
         from tests.orc import OracleWorld
 
+        threads = orc.set_threads(int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1))
         ow = OracleWorld(start_bodies, shapes, verts, cfg)
         ow.set_arbiters(start_arbiters)
         if start_cons is not None:
             ow.set_constraints(start_cons)
             ow.set_constraint_order(con_order)
         secs = ow.time_steps(dt, args.cpu_steps)
-        cpu = {"value": n_dyn * args.cpu_steps / secs, "unit": "body-steps/s", "cores": 1, "kind": "port",
+        orc.set_threads(1)
+        cpu = {"value": n_dyn * args.cpu_steps / secs, "unit": "body-steps/s", "cores": threads, "kind": "port",
+               "label": "restatement CPU (the reference, C#, cannot run here)",
                "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body scene continued from the GPU state at "
-                         f"the start of the timed region (oracle FP32, key-order solve, 1 thread)"}
+                         f"the start of the timed region: the oracle (C++ restatement, FP32) on {threads} host "
+                         f"threads, narrowphase, boxes, integration and body update in parallel, Gauss-Seidel "
+                         f"solve sequential in key order ({secs:.1f} s)"}
 
     kernel = {0: "k_solve", 1: "k_solve_flow", 2: "k_solve_worlds", 3: "k_solve_tiles",
               4: "k_solve_lanes"}[int(final["solver"])]

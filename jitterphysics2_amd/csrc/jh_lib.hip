@@ -209,7 +209,7 @@ struct jh_world {
     DBuf<float> soa_f;
     int soa_cap = 0;
     // world-local solver (many independent worlds)
-    int n_worlds = 0, world_blocks = 0;
+    int n_worlds = 0, world_blocks = 0, largest_world = 0;
     DBuf<int> wbody, wboff, wsched, wcells, wqueue;
     DBuf<uint32_t> whist, woff, wcur, wsums;
     DBuf<uint64_t> wlist;
@@ -1157,6 +1157,8 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
         std::stable_sort(sched.begin(), sched.begin() + W,
                          [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
         w->n_worlds = W;
+        w->largest_world = 0;
+        for (int k = 0; k < W; ++k) w->largest_world = std::max(w->largest_world, off[k + 1] - off[k]);
         const char* e = getenv("JH_SOLVER");
         if (e && strcmp(e, "barrier") == 0) w->kind = SOLVER_BARRIER;
         else if (e && strcmp(e, "world") == 0) w->kind = SOLVER_WORLDS;
@@ -1166,9 +1168,7 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
         else {
             // many worlds, or only small ones (one workgroup per world beats a device-wide launch
             // whose grid barriers and dependency round trips dominate a few hundred arbiters)
-            int largest = 0;
-            for (int k = 0; k < W; ++k) largest = std::max(largest, off[k + 1] - off[k]);
-            w->kind = (W >= 64 || largest <= kSmallWorld) ? SOLVER_WORLDS : SOLVER_FLOW;
+            w->kind = (W >= 64 || w->largest_world <= kSmallWorld) ? SOLVER_WORLDS : SOLVER_FLOW;
         }
         w->n_moving = 0;
         for (int i = 0; i < n; ++i) w->n_moving += bodies[i].motion_type != JH_MOTION_STATIC;
@@ -1925,7 +1925,9 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n});
         for (int T : {cu / 2, cu, 2 * cu})
             if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back({SOLVER_TILES, T});
-        if (w->n_worlds >= 1) sc.push_back({SOLVER_WORLDS, w->world_blocks});
+        // one workgroup per world: hopeless for a few large worlds (a 131k-body world takes ~0.5 s)
+        if (w->n_worlds >= 64 || (w->n_worlds >= 1 && w->largest_world <= 16 * kSmallWorld))
+            sc.push_back({SOLVER_WORLDS, w->world_blocks});
     }
     auto width = [&](int kind) -> int& {
         return kind == SOLVER_BARRIER ? w->coop_solve : kind == SOLVER_LANES ? w->lane_blocks

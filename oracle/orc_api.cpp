@@ -5,6 +5,9 @@
 #include <chrono>
 #include <cstdio>
 #include <memory>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "orc_world.hpp"
 
@@ -591,6 +594,15 @@ void orc_rotation_quaternion_f(const float* w, float dt, float* q) {
 // ExpansionArithmetic.Side on FP32 coordinates (exact orientation sign).
 int orc_exact_side(const float* a, const float* b, const float* c, const float* p) {
     return exact::side(vin<float>(a), vin<float>(b), vin<float>(c), vin<float>(p));
+}
+// Threads of the parallel parts of the step (OpenMP); results do not depend on it.
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 // Wall-clock time of k steps in key order (CPU baseline).
 double orc_time_steps(void* p, float dt, int k) {

@@ -4,8 +4,14 @@
 // restated from Jitter2.  The solve order is either given (a replay of an order produced
 // elsewhere) or the key order (ArbiterKey ascending), which is the deterministic order of
 // World.Deterministic.cs:142-203 for a world with a single island.
+//
+// Threads (OpenMP, orc_set_threads): the narrowphase of each pair, the shape boxes, the per-body
+// sweeps and the contact refresh run in parallel; each writes only its own pair's or body's or
+// arbiter's record, and contacts are registered into the cache afterwards in pair order, so results
+// are identical for any thread count.  The Gauss-Seidel solve itself is sequential.
 #pragma once
 #include <map>
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -124,10 +130,6 @@ template <class R> struct World {
         }
     }
     long epa_calls = 0;
-    Polytope<R>* poly = nullptr;
-
-    World() { poly = new Polytope<R>(); }
-    ~World() { delete poly; }
 
     // RigidBodyShape.UpdateWorldBoundingBox(BroadphaseStepDt) (RigidBodyShape.cs:37-58): bodies with
     // speculative contacts get their box grown by dt * |velocity| on every side.
@@ -148,7 +150,8 @@ template <class R> struct World {
     std::vector<std::pair<uint32_t, uint32_t>> compute_pairs(bool brute = false) const {
         size_t n = shapes.size();
         std::vector<V3<R>> mn(n), mx(n);
-        for (size_t i = 0; i < n; ++i) world_aabb(shapes[i], mn[i], mx[i]);
+#pragma omp parallel for schedule(static) if (n > 4096)
+        for (long i = 0; i < long(n); ++i) world_aabb(shapes[i], mn[i], mx[i]);
         std::vector<std::pair<uint32_t, uint32_t>> out;
         auto accept = [&](size_t i, size_t j) {
             const Shape<R>& a = shapes[i];
@@ -328,7 +331,8 @@ template <class R> struct World {
     }
 
     // Filter (:149-323); false = discard.  r may be replaced by the re-run with EPA threshold 0.
-    bool tef_filter(const Shape<R>& sA, const Shape<R>& sB, const Body<R>& b1, const Body<R>& b2, MprResult<R>& r) {
+    bool tef_filter(const Shape<R>& sA, const Shape<R>& sB, const Body<R>& b1, const Body<R>& b2, MprResult<R>& r,
+                    Polytope<R>* pt, long& epa) const {
         bool c1 = sA.type == JH_SHAPE_TRIANGLE, c2 = sB.type == JH_SHAPE_TRIANGLE;
         if (c1 == c2) return true;
         const Shape<R>& t = c1 ? sA : sB;
@@ -340,8 +344,8 @@ template <class R> struct World {
         if (e == TefKeep) return true;
         bool spec = r.penetration < R(0);
         if (!spec) {
-            MprResult<R> r2 = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, poly, R(0));
-            if (r2.epa) epa_calls++;
+            MprResult<R> r2 = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, pt, R(0));
+            if (r2.epa) epa++;
             if (!r2.hit) return false;
             r = r2;
             e = tef_evaluate(t, tb, c2, c1 ? r.pa : r.pb, r.normal, tn, nn);
@@ -390,10 +394,17 @@ template <class R> struct World {
         return it->second;
     }
 
+    // Narrowphase result of one pair (World.Detect.cs:125-218): what RegisterContact will be given.
+    struct Detected {
+        int kind = 0;  // 0 nothing, 1 contact manifold, 2 speculative contact
+        ManifoldOut<R> mo{};
+        long epa = 0;  // EPA runs (including the edge filter's re-run)
+    };
+
     // The speculative branch of World.Detect (World.Detect.cs:167-194) for a pair MPR found separated:
-    // a linear sweep; a time of impact inside this step registers one contact point whose solve mode
+    // a linear sweep; a time of impact inside this step gives one contact point whose solve mode
     // drops the angular terms (RegisterContact(..., SolveMode.Angular), World.Detect.cs:331-337, 232-250).
-    void detect_speculative(uint32_t k1, uint32_t k2, R step_dt) {
+    void detect_speculative(uint32_t k1, uint32_t k2, R step_dt, Polytope<R>* pt, Detected& d) const {
         const Shape<R>& sA = shapes[k1];
         const Shape<R>& sB = shapes[k2];
         const Body<R>& b1 = bodies[sA.body];
@@ -408,8 +419,8 @@ template <class R> struct World {
         r.hit = true;
         r.epa = false;
         r.penetration = dot(r.normal, r.pa - r.pb) * specrel;
-        if (triangle_edge_filter && !tef_filter(sA, sB, b1, b2, r)) return;
-        ManifoldOut<R> mo{};
+        if (triangle_edge_filter && !tef_filter(sA, sB, b1, b2, r, pt, d.epa)) return;
+        ManifoldOut<R>& mo = d.mo;
         mo.k1 = k1;
         mo.k2 = k2;
         mo.epa = false;
@@ -421,27 +432,25 @@ template <class R> struct World {
         mo.count = 1;
         mo.mA[0] = r.pa;
         mo.mB[0] = r.pb;
-        last_manifolds.push_back(mo);
-        Arbiter<R>& cd = arbiter_for(k1, k2);
-        if (!persistent) cd.usage = 0;
-        reset_mode(cd, JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2);
-        add_contact(cd, r.pa, r.pb, r.normal);
+        d.kind = 2;
     }
 
-    void detect(uint32_t k1, uint32_t k2, R step_dt) {
+    // World.Detect (World.Detect.cs:125-218) up to the RegisterContact call: reads the world only.
+    Detected detect_pair(uint32_t k1, uint32_t k2, R step_dt, Polytope<R>* pt) const {
+        Detected d;
         const Shape<R>& sA = shapes[k1];
         const Shape<R>& sB = shapes[k2];
         const Body<R>& b1 = bodies[sA.body];
         const Body<R>& b2 = bodies[sB.body];
-        if (b1.motion != JH_MOTION_DYNAMIC && b2.motion != JH_MOTION_DYNAMIC) return;
-        MprResult<R> r = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, poly);
-        if (r.epa) epa_calls++;
+        if (b1.motion != JH_MOTION_DYNAMIC && b2.motion != JH_MOTION_DYNAMIC) return d;
+        MprResult<R> r = mpr_epa(sA, sB, b1.q, b2.q, b1.pos, b2.pos, pt);
+        if (r.epa) d.epa++;
         if (!r.hit) {
-            detect_speculative(k1, k2, step_dt);
-            return;
+            detect_speculative(k1, k2, step_dt, pt, d);
+            return d;
         }
-        if (triangle_edge_filter && !tef_filter(sA, sB, b1, b2, r)) return;
-        ManifoldOut<R> mo{};
+        if (triangle_edge_filter && !tef_filter(sA, sB, b1, b2, r, pt, d.epa)) return d;
+        ManifoldOut<R>& mo = d.mo;
         mo.k1 = k1;
         mo.k2 = k2;
         mo.epa = r.epa;
@@ -459,12 +468,38 @@ template <class R> struct World {
         }
         mo.count = m.count;
         for (int i = 0; i < m.count; ++i) { mo.mA[i] = m.mA[i]; mo.mB[i] = m.mB[i]; }
+        d.kind = 1;
+        return d;
+    }
+
+    // RegisterContact (World.Detect.cs:294-324), in pair order.
+    void register_detected(const Detected& d) {
+        epa_calls += d.epa;
+        if (d.kind == 0) return;
+        const ManifoldOut<R>& mo = d.mo;
         last_manifolds.push_back(mo);
-        // RegisterContact (World.Detect.cs:294-324)
-        Arbiter<R>& cd = arbiter_for(k1, k2);
+        Arbiter<R>& cd = arbiter_for(mo.k1, mo.k2);
         if (!persistent) cd.usage = 0;
-        reset_mode(cd);
-        for (int e = 0; e < m.count; ++e) add_contact(cd, m.mA[e], m.mB[e], r.normal);
+        if (d.kind == 2) {
+            reset_mode(cd, JH_SOLVE_ANGULAR1 | JH_SOLVE_ANGULAR2);
+            add_contact(cd, mo.pa, mo.pb, mo.normal);
+        } else {
+            reset_mode(cd);
+            for (int e = 0; e < mo.count; ++e) add_contact(cd, mo.mA[e], mo.mB[e], mo.normal);
+        }
+    }
+
+    // Narrowphase over a pair list: parallel detection, registration in list order.
+    void detect_all(const std::vector<std::pair<uint32_t, uint32_t>>& pairs, R step_dt) {
+        std::vector<Detected> out(pairs.size());
+        long n = long(pairs.size());
+#pragma omp parallel if (n > 256)
+        {
+            std::unique_ptr<Polytope<R>> pt(new Polytope<R>());
+#pragma omp for schedule(dynamic, 64)
+            for (long i = 0; i < n; ++i) out[i] = detect_pair(pairs[i].first, pairs[i].second, step_dt, pt.get());
+        }
+        for (auto& d : out) register_detected(d);
     }
 
     static R sum3(const R* w, R a, R b, R c) { return w[0] * a + w[1] * b + w[2] * c; }
@@ -728,7 +763,7 @@ template <class R> struct World {
         last_manifolds.clear();
         epa_calls = 0;
         last_pairs = compute_pairs();
-        for (auto& p : last_pairs) detect(p.first, p.second, ts.dt);
+        detect_all(last_pairs, ts.dt);
         // Solve order: given, else key order.
         std::vector<Arbiter<R>*> list;
         if (order) {
@@ -742,7 +777,9 @@ template <class R> struct World {
         if (probe) run_probe(list, ts, *probe);
         for (int s = 0; s < ts.substeps; ++s) {
             // IntegrateForces (World.Step.cs:1060-1072)
-            for (auto& b : bodies) {
+#pragma omp parallel for schedule(static) if (bodies.size() > 4096)
+            for (long i = 0; i < long(bodies.size()); ++i) {
+                Body<R>& b = bodies[i];
                 if (b.motion != JH_MOTION_DYNAMIC) continue;
                 b.angvel = b.angvel + b.dangvel;
                 b.vel = b.vel + b.dvel;
@@ -765,7 +802,9 @@ template <class R> struct World {
                 });
             }
             // IntegrateVelocities (World.Step.cs:1106-1133), no gyroscopic term
-            for (auto& b : bodies) {
+#pragma omp parallel for schedule(static) if (bodies.size() > 4096)
+            for (long i = 0; i < long(bodies.size()); ++i) {
+                Body<R>& b = bodies[i];
                 if (b.motion == JH_MOTION_STATIC) continue;
                 b.pos = b.pos + b.vel * ts.substep_dt;
                 Q4<R> rq = rotation_quaternion(b.angvel, ts.substep_dt);
@@ -788,13 +827,20 @@ template <class R> struct World {
         broken.clear();
         // UpdateContacts (World.Step.cs:862-878): arbiters left without intact contacts are queued for
         // removal; until then they carry no solver work.
-        for (auto& kv : arbiters) {
-            arbiter_update_position(kv.second);
-            if ((kv.second.usage & 15u) == 0) broken.push_back(kv.first);
+        {
+            std::vector<Arbiter<R>*> all;
+            all.reserve(arbiters.size());
+            for (auto& kv : arbiters) all.push_back(&kv.second);
+#pragma omp parallel for schedule(static) if (all.size() > 4096)
+            for (long i = 0; i < long(all.size()); ++i) arbiter_update_position(*all[i]);
+            for (auto& kv : arbiters)
+                if ((kv.second.usage & 15u) == 0) broken.push_back(kv.first);
         }
         std::sort(ev_begin.begin(), ev_begin.end());
         // ForeachActiveBody -> RigidBody.Update (RigidBody.cs:716-764)
-        for (auto& b : bodies) {
+#pragma omp parallel for schedule(static) if (bodies.size() > 4096)
+        for (long i = 0; i < long(bodies.size()); ++i) {
+            Body<R>& b = bodies[i];
             if (b.motion != JH_MOTION_DYNAMIC) continue;
             b.update_world_inertia();
             b.angvel = b.angvel * b.ang_damp;

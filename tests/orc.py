@@ -50,6 +50,10 @@ def lib() -> ctypes.CDLL:
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32
+        L.orc_set_threads.argtypes = [i32]
+        L.orc_set_threads.restype = i32
+        # one thread unless asked: test workers run side by side; results do not depend on it
+        L.orc_set_threads(int(os.environ.get("ORC_THREADS", "1")))
         L.orc_epa_calls.argtypes = [vp]
         L.orc_epa_calls.restype = ctypes.c_long
         L.orc_sincos_f.argtypes = [f, ctypes.POINTER(f), ctypes.POINTER(f)]
@@ -167,6 +171,12 @@ class OracleWorld:
 
     def time_steps(self, dt: float, k: int) -> float:
         return float(self.L.orc_time_steps(self.h, dt, k))
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's parallel parts (narrowphase, boxes, per-body sweeps, contact refresh);
+    returns the count in effect.  Results are the same for any count."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def simplex_ab(V, A=None, B=None, fp64: bool = False):
