@@ -84,6 +84,12 @@ class OracleWorld:
         nv = 0 if self.verts is None else len(self.verts) // 3
         self.L.orc_set_shapes(self.h, ptr(np.ascontiguousarray(shapes)), len(shapes), ptr(self.verts), nv)
 
+    def set_bodies(self, bodies) -> None:
+        """Replace the body records (positions, velocities, forces ...): RigidBody property setters
+        and AddForce between steps."""
+        assert len(bodies) == self.n
+        self.L.orc_set_bodies(self.h, ptr(np.ascontiguousarray(bodies)), len(bodies))
+
     def __del__(self):
         try:
             self.L.orc_destroy(self.h)
