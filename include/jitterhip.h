@@ -384,7 +384,8 @@ jh_status jh_world_set_warm_start_model(jh_world* world, jh_lws* model);
  * for m = 0 previous-frame warm start, 1 learned, 2 cold (zero impulses); R_k = sqrt(residual[m][k] /
  * residual[2][0]).  Each probe runs on a scratch copy; the step itself then runs as jh_world_step.
  * With capacity > 0, the features of every contact point and its accumulated impulse after
- * `iterations` previous-frame-warm-started iterations are written (training traces). */
+ * `iterations` previous-frame-warm-started iterations are written (training traces); rows are the used
+ * contact slots in arbiter key order. */
 typedef struct jh_probe {
     int32_t iterations;      /* K, 1 .. JH_PROBE_MAX_ITERATIONS                             */
     int32_t modes;           /* bit m set: run probe m (m as above)                         */
@@ -392,6 +393,8 @@ typedef struct jh_probe {
     int32_t reserved;
     float* features;         /* [capacity][24]                                              */
     float* targets;          /* [capacity][3]                                               */
+    float* learned;          /* [capacity][3] or NULL: with probe 1, the impulse each row is
+                                warm-started with by the network (the in-step inference)      */
     int64_t points;          /* out: contact points                                          */
     int64_t rows;            /* out: rows written (min(points, capacity))                   */
     double residual[3][JH_PROBE_MAX_ITERATIONS + 1];  /* out                                 */

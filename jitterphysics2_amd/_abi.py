@@ -56,6 +56,7 @@ PROBE_DTYPE = np.dtype([
     ("reserved", "<i4"),
     ("features", "<u8"),
     ("targets", "<u8"),
+    ("learned", "<u8"),
     ("points", "<i8"),
     ("rows", "<i8"),
     ("residual", "<f8", (3, PROBE_MAX_ITERATIONS + 1)),

@@ -250,7 +250,7 @@ struct jh_world {
     DBuf<float4> pvel, pang;
     DBuf<jh_arbiter> parb;
     DBuf<double> ppart;
-    DBuf<float> pfeat, ptarg, plam;
+    DBuf<float> pfeat, ptarg, plam, plearn;
     DBuf<uint8_t> pvalid;
     int coop_probe = 0;
     jh_stats stats;
@@ -592,8 +592,14 @@ jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int ama
         P.B.vel = w->pvel.p;
         P.B.angvel = w->pang.p;
         if (m == 2) lws::k_zero_acc<<<ga, 256, 0, st>>>(w->parb.p, a_count);
-        if (m == 1) lws::k_infer<<<grid_for(std::max(4 * A, 1), 256, 2048), 256, 0, st>>>(w->parb.p, a_count, P.B,
-                                                                                        w->lws.model());
+        if (m == 1) {
+            lws::k_infer<<<grid_for(std::max(4 * A, 1), 256, 2048), 256, 0, st>>>(w->parb.p, a_count, P.B,
+                                                                                 w->lws.model());
+            if (trace && pr->learned) {
+                HIPCHK(w->plearn.ensure(size_t(std::max(4 * A, 1)) * 3));
+                lws::k_gather_acc<<<ga, 256, 0, st>>>(w->parb.p, a_count, w->plearn.p);
+            }
+        }
         HIPCHK(hipGetLastError());
         void* args[] = {&P, &K, &w->ppart.p, &w->plam.p};
         HIPCHK(hipLaunchCooperativeKernel((const void*)lws::k_probe, dim3(grid), dim3(256), args, 0, st));
@@ -614,6 +620,10 @@ jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int ama
         if (trace) {
             feat.resize(4 * size_t(A) * lws::kIn);
             targ.resize(4 * size_t(A) * 3);
+            bool learned = pr->learned && (pr->modes & 2);
+            std::vector<float> lrn(learned ? 4 * size_t(A) * 3 : 0);
+            if (learned)
+                HIPCHK(hipMemcpyAsync(lrn.data(), w->plearn.p, lrn.size() * sizeof(float), hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(valid.data(), w->pvalid.p, valid.size(), hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(feat.data(), w->pfeat.p, feat.size() * sizeof(float), hipMemcpyDeviceToHost, st));
             HIPCHK(hipMemcpyAsync(targ.data(), w->ptarg.p, targ.size() * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -624,6 +634,7 @@ jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int ama
                 if (rows < pr->capacity) {
                     memcpy(pr->features + rows * lws::kIn, &feat[i * lws::kIn], lws::kIn * sizeof(float));
                     memcpy(pr->targets + rows * 3, &targ[i * 3], 3 * sizeof(float));
+                    if (learned) memcpy(pr->learned + rows * 3, &lrn[i * 3], 3 * sizeof(float));
                 }
                 ++rows;
             }
@@ -1111,7 +1122,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
     w->cbeg.release(); w->bar.release();
     w->lws.release(); w->pvel.release(); w->pang.release(); w->parb.release(); w->ppart.release();
-    w->pfeat.release(); w->ptarg.release(); w->pvalid.release(); w->plam.release();
+    w->pfeat.release(); w->ptarg.release(); w->pvalid.release(); w->plam.release(); w->plearn.release();
     for (auto& e : w->ev) (void)hipEventDestroy(e);
     if (w->hcnt) (void)hipHostFree(w->hcnt);
     (void)hipStreamDestroy(w->st);

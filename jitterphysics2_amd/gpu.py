@@ -163,20 +163,23 @@ class GpuWorld:
     def step_probe(self, dt: float, iterations: int, modes: int, capacity: int = 0) -> dict:
         """One step with residual probes of its first substep (jh_world_step_probe).
         Returns residual (3, iterations + 1) sums of squares per mode (previous, learned, cold),
-        points, and with capacity > 0 the trace rows: features (rows, 24), targets (rows, 3)."""
+        points, and with capacity > 0 the trace rows: features (rows, 24), targets (rows, 3) and, with
+        the learned mode, the network's warm-start impulses (rows, 3)."""
         pr = np.zeros(1, PROBE_DTYPE)
         pr["iterations"] = iterations
         pr["modes"] = modes
         pr["capacity"] = capacity
         feats = np.zeros((max(capacity, 1), LWS_FEATURES), np.float32)
         targs = np.zeros((max(capacity, 1), 3), np.float32)
+        learned = np.zeros((max(capacity, 1), 3), np.float32)
         if capacity > 0:
             pr["features"] = feats.ctypes.data
             pr["targets"] = targs.ctypes.data
+            pr["learned"] = learned.ctypes.data
         check(self.lib, self.lib.jh_world_step_probe(self.handle, ctypes.c_float(dt), ptr(pr)), "jh_world_step_probe")
         rows = int(pr["rows"][0])
         return {"residual": pr["residual"][0][:, :iterations + 1].copy(), "points": int(pr["points"][0]),
-                "features": feats[:rows], "targets": targs[:rows]}
+                "features": feats[:rows], "targets": targs[:rows], "learned": learned[:rows]}
 
     def autotune(self, dt: float, rounds: int = 2) -> dict:
         """Pick the solver and colouring workgroup counts by timing candidates on real steps

@@ -6,7 +6,9 @@ three outputs.  With ``quantise=True`` the device's BF16 rounding points are rep
 nearest even from float32): inputs, weights, hidden activations H1 and H2, and the backward
 operands dY, dZ2, dZ1 that feed the BF16 matrix products; bias gradients are summed from the
 unrounded dY, dZ2, dZ1 as on the device.  With ``quantise=False`` it is the exact float64 network,
-pinned against central finite differences in tests/test_lws_oracle.py.
+pinned against central finite differences in tests/test_lws_oracle.py.  infer_warm_start restates
+what the in-step inference (lws::k_infer) does with the network output: trust clamp, impulse scale,
+normal impulse >= 0, non-finite fallback, SkipWarmStart rows untouched.
 """
 from __future__ import annotations
 
@@ -105,3 +107,36 @@ def init_params(seed: int) -> np.ndarray:
             u = (nxt() >> 11) * (1.0 / 9007199254740992.0)
             out.append(np.float32(np.float64(2.0 * u - 1.0) * np.float64(b)))
     return np.array(out, np.float32)
+
+
+TRUST = 3.0  # lws::kTrust
+
+
+def impulse_scale(features) -> np.ndarray:
+    """lws::impulse_scale (jh_lws.h) in float32, left to right:
+    |MassNormal| * (((|v.n| + |v.t1|) + |v.t2|) + |PenaltyBias|) + 0.05)."""
+    f = np.asarray(features, np.float32)
+    a = np.abs
+    return a(f[:, 13]) * ((((a(f[:, 9]) + a(f[:, 10])) + a(f[:, 11])) + a(f[:, 22])) + np.float32(0.05))
+
+
+def infer_warm_start(p, mean, inv_std, tscale, features, flags, quantise=True) -> np.ndarray:
+    """The warm start lws::k_infer (jh_lws.h) writes for contact rows with raw features [n][24] and
+    contact flags [n] (bit 2 SkipWarmStart): rows that skip the warm start keep the previous impulse
+    (features 16..18); the others take previous + clamp(y, -3, 3) * (tscale * impulse_scale) in float32,
+    with the clamp as fminf(fmaxf(y, -3), 3) (a NaN output clamps to -3), the normal impulse clamped
+    at 0, and the previous impulse kept when any component is non-finite."""
+    f = np.asarray(features, np.float32)
+    prev = f[:, 16:19].copy()
+    use = (np.asarray(flags, np.uint32) & 4) == 0
+    y = forward(p, normalise(f, mean, inv_std), quantise)["Y"].astype(np.float32)
+    y = np.fmin(np.fmax(y, np.float32(-TRUST)), np.float32(TRUST))
+    with np.errstate(over="ignore", invalid="ignore"):
+        sc = impulse_scale(f)
+        step = y * (np.asarray(tscale, np.float32)[None, :] * sc[:, None])
+        pred = prev + step
+    ok = use & np.all(np.isfinite(pred), axis=1)
+    out = prev.copy()
+    out[ok, 0] = np.maximum(pred[ok, 0], np.float32(0))
+    out[ok, 1:] = pred[ok, 1:]
+    return out

@@ -403,7 +403,8 @@ int orc_step(void* p, float dt, const uint32_t* order, int nkeys) {
 // cold); residual[3][iterations + 1], features [capacity][24], targets [capacity][3];
 // counts[0] = rows written, counts[1] = contact points.
 int orc_step_probe(void* p, float dt, const uint32_t* order, int nkeys, int iterations, int modes, double* residual,
-                   float* features, float* targets, long capacity, long* counts) {
+                   float* features, float* targets, long capacity, long* counts, const float* learned,
+                   uint32_t* flags) {
     Handle* h = (Handle*)p;
     std::vector<std::pair<uint32_t, uint32_t>> ord(nkeys);
     for (int i = 0; i < nkeys; ++i) ord[i] = {order[2 * i], order[2 * i + 1]};
@@ -416,6 +417,8 @@ int orc_step_probe(void* p, float dt, const uint32_t* order, int nkeys, int iter
         pr.residual = residual;
         pr.features = features;
         pr.targets = targets;
+        pr.learned = learned;
+        pr.flags = flags;
         W.step_impl(decltype(W.gravity.x)(dt), &ord, &pr);
         counts[0] = pr.rows;
         counts[1] = pr.points;

@@ -168,16 +168,20 @@ template <class R> struct World {
             for (size_t i = 0; i < n; ++i)
                 for (size_t j = i + 1; j < n; ++j) accept(i, j);
         } else {
+            // sort by (world, min x) and sweep within each world: pairs never cross worlds
             std::vector<uint32_t> order(n);
             for (size_t i = 0; i < n; ++i) order[i] = uint32_t(i);
+            auto wid = [&](uint32_t i) { return bodies[shapes[i].body].world; };
             std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+                int wa = wid(a), wb = wid(b);
+                if (wa != wb) return wa < wb;
                 return mn[a].x < mn[b].x || (mn[a].x == mn[b].x && a < b);
             });
             for (size_t oi = 0; oi < n; ++oi) {
                 uint32_t i = order[oi];
                 for (size_t oj = oi + 1; oj < n; ++oj) {
                     uint32_t j = order[oj];
-                    if (mn[j].x > mx[i].x) break;
+                    if (wid(j) != wid(i) || mn[j].x > mx[i].x) break;
                     accept(i, j);
                 }
             }
@@ -645,6 +649,9 @@ template <class R> struct World {
         double* residual = nullptr;  // [3][iterations + 1]
         float* features = nullptr;   // [capacity][24]
         float* targets = nullptr;    // [capacity][3]
+        const float* learned = nullptr;  // [capacity][3]: the warm start of probe 1 (an external
+                                         // network's impulses, replayed), rows in trace order
+        uint32_t* flags = nullptr;       // [capacity]: the contact flags of each row at the warm start
         long rows = 0, points = 0;
     };
 
@@ -702,12 +709,14 @@ template <class R> struct World {
             for (int s = 0; s < 4; ++s) {
                 if (!(kv.second.usage & (1u << s))) continue;
                 if (pr.features && pr.rows < pr.capacity) contact_features(kv.second, s, pr.features + 24 * pr.rows);
+                if (pr.flags && pr.rows < pr.capacity) pr.flags[pr.rows] = kv.second.c[s].flag;
                 if (pr.rows < pr.capacity) ++pr.rows;
                 ++pr.points;
             }
         int K = pr.iterations;
-        for (int m : {0, 2}) {
+        for (int m : {0, 1, 2}) {
             if (!(pr.modes & (1 << m))) continue;
+            if (m == 1 && !pr.learned) continue;
             std::vector<Body<R>> saved = bodies;
             std::vector<Arbiter<R>> pa;
             pa.reserve(list.size());
@@ -715,6 +724,19 @@ template <class R> struct World {
             if (m == 2)
                 for (auto& a : pa)
                     for (auto& c : a.c) c.acc[0] = c.acc[1] = c.acc[2] = R(0);
+            if (m == 1) {  // the given warm start, row by row (used slots in key order)
+                std::map<std::pair<uint32_t, uint32_t>, Arbiter<R>*> by_key;
+                for (auto& a : pa) by_key[{a.key1, a.key2}] = &a;
+                long row = 0;
+                for (auto& kv : arbiters)
+                    for (int s = 0; s < 4; ++s) {
+                        if (!(kv.second.usage & (1u << s))) continue;
+                        auto it = by_key.find(kv.first);
+                        if (row < pr.capacity && it != by_key.end())
+                            for (int r = 0; r < 3; ++r) it->second->c[s].acc[r] = R(pr.learned[3 * row + r]);
+                        ++row;
+                    }
+            }
             for (auto& b : bodies) {
                 if (b.motion != JH_MOTION_DYNAMIC) continue;
                 b.angvel = b.angvel + b.dangvel;

@@ -1,0 +1,102 @@
+"""GPU parity at the benchmark sizes.  The 32,768- and 131,072-body heaps are stepped on the GPU until
+they are in contact, the launch shapes are autotuned as the benchmark does, then the pair set is
+compared with the oracle's sort-and-sweep and two steps are replayed by the oracle in the GPU's
+solve order (bit-exact: manifolds, arbiters, bodies).  The 4,096-world batch and the full form
+(262,144 bodies over a 524,288-triangle heightfield) are checked on their first step (pairs,
+manifolds, arbiters, bodies); the 20-row pyramid over all 600 steps of its configuration, with the
+error of FP32 against the FP64 oracle replaying the same order reported at the end.  The oracle runs
+on the host's threads (results do not depend on the thread count)."""
+import os
+
+import numpy as np
+import pytest
+
+from jitterphysics2_amd import scenes
+from jitterphysics2_amd.gpu import GpuWorld
+from tests import orc
+from tests.orc import OracleWorld
+from tests.test_gpu_step import assert_arbiters_equal, assert_bodies_equal, assert_manifolds_equal
+
+pytestmark = pytest.mark.gpu
+
+THREADS = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+@pytest.fixture(autouse=True)
+def oracle_threads():
+    orc.set_threads(THREADS)
+    yield
+    orc.set_threads(1)
+
+
+def with_props(bodies, state):
+    out = bodies.copy()
+    for f in ("position", "orientation", "velocity", "angular_velocity", "delta_velocity", "delta_angular_velocity",
+              "force", "torque"):
+        out[f] = state[f]
+    return out
+
+
+def replay_steps(gw, ow, dt, steps, what):
+    for s in range(steps):
+        gw.step(dt)
+        ow.step(dt, gw.solve_order()[0])
+        assert_manifolds_equal(gw.manifolds(), ow.manifolds(), f"{what}, step {s}")
+        assert_bodies_equal(gw.bodies(), ow.bodies(), f"{what}, step {s}")
+        assert_arbiters_equal(gw.arbiters(), ow.arbiters(), f"{what}, step {s}")
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("n,settle", [(32768, 240), (131072, 300)])
+def test_heap_at_full_size(n, settle):
+    bodies, shapes, verts, cfg, dt = scenes.heap(n)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    for _ in range(settle):
+        gw.step(dt)
+    tune = gw.autotune(dt, rounds=1)  # the launch shape the benchmark would pick
+    st = gw.stats()
+    print(f"heap {n} after {st['steps']} steps: {st['arbiters']} arbiters, {st['contacts']} contacts, "
+          f"{st['colours']} colours; solver {tune['solver']} x {tune['solve_blocks']}", flush=True)
+    assert st["arbiters"] > n // 2 and st["colours"] >= 10
+    ow = OracleWorld(with_props(bodies, gw.bodies()), shapes, verts, cfg)
+    ow.set_arbiters(gw.arbiters())
+    pg, po = gw.compute_pairs(), ow.pairs()
+    assert len(pg) > n and np.array_equal(pg, po), f"pair set differs ({len(pg)} vs {len(po)})"
+    replay_steps(gw, ow, dt, 2, f"heap {n}")
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("scene", ["batched", "full"])
+def test_first_step_at_full_size(scene):
+    if scene == "batched":
+        bodies, shapes, verts, cfg, dt = scenes.batched_worlds(4096)
+    else:
+        bodies, shapes, verts, cfg, dt = scenes.full_form()
+    print(f"{scene}: {len(bodies)} bodies, {len(shapes)} shapes built", flush=True)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    ow = OracleWorld(bodies, shapes, verts, cfg)
+    pg, po = gw.compute_pairs(), ow.pairs()
+    assert len(pg) > 10000 and np.array_equal(pg, po), f"pair set differs ({len(pg)} vs {len(po)})"
+    replay_steps(gw, ow, dt, 1, scene)
+    assert len(gw.manifolds()) > 10
+
+
+@pytest.mark.timeout(600)
+def test_pyramid_configuration_600_steps():
+    bodies, shapes, verts, cfg, dt = scenes.pyramid(rows=20)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    o32 = OracleWorld(bodies, shapes, verts, cfg)
+    o64 = OracleWorld(bodies, shapes, verts, cfg, fp64=True)
+    for s in range(600):
+        gw.step(dt)
+        keys = gw.solve_order()[0]
+        o32.step(dt, keys)
+        o64.step(dt, keys)
+        if (s + 1) % 100 == 0:
+            assert_bodies_equal(gw.bodies(), o32.bodies(), f"pyramid step {s}")
+            assert_arbiters_equal(gw.arbiters(), o32.arbiters(), f"pyramid step {s}")
+    g, s64 = gw.bodies(), o64.state64()
+    dp = np.abs(g["position"].astype(np.float64) - s64[:, 0:3]).max()
+    dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10]).max()
+    print(f"pyramid, 600 steps: FP32 GPU vs FP64 oracle max |dp| {dp:.3e} m, max |dv| {dv:.3e} m/s", flush=True)
+    assert dp < 1e-4 and dv < 1e-3

@@ -138,3 +138,67 @@ def test_learned_without_model_is_an_error():
     gw.set_config(cfg)
     with pytest.raises(RuntimeError, match="invalid state"):
         gw.step_probe(dt, 4, PROBE_LEARNED)
+
+
+def _settled_heap(n=1500, steps=150, seed=7):
+    bodies, shapes, verts, cfg, dt = scenes.heap(n, seed=seed)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    for _ in range(steps):
+        gw.step(dt)
+    ow = OracleWorld(_with_props(bodies, gw.bodies()), shapes, verts, cfg)
+    ow.set_arbiters(gw.arbiters())
+    return gw, ow, dt
+
+
+def _with_props(bodies, state):
+    out = bodies.copy()
+    for f in ("position", "orientation", "velocity", "angular_velocity", "delta_velocity", "delta_angular_velocity",
+              "force", "torque"):
+        out[f] = state[f]
+    return out
+
+
+@pytest.mark.parametrize("shrink", ["committed", "unshrunk", "overflow"])
+def test_learned_warm_start_matches_oracle(shrink):
+    # the in-step inference (k_infer) with the committed model on a settled heap: features bit-exact,
+    # warm-start impulses within the BF16 bound of the network's outputs (|dy| <= 2e-2 max(1, |y|max),
+    # the forward test's bound, times the row's impulse scale and target scale), residuals of the
+    # oracle replaying those impulses equal to 1e-9; "unshrunk" undoes the committed 1/16 shrink of the
+    # correction, "overflow" (target scale 3e38) makes the step non-finite so rows keep their impulse
+    from jitterphysics2_amd.lws_train import DEFAULT_MODEL
+    model = LearnedWarmStart.load(DEFAULT_MODEL)
+    if shrink != "committed":
+        ts = model.target_scale * 16 if shrink == "unshrunk" else np.full(3, 3e38, np.float32)
+        model.set_normalisation(model.mean, model.inv_std, ts)
+    gw, ow, dt = _settled_heap()
+    gw.set_warm_start_model(model)
+    cap = 4 * len(gw.arbiters()) + 64
+    K = 16
+    modes = PROBE_PREVIOUS | PROBE_LEARNED | PROBE_COLD
+    print(f"learned warm start vs oracle: {shrink}, {len(gw.arbiters())} arbiters", flush=True)
+    r = gw.step_probe(dt, K, modes, capacity=cap)
+    keys, _ = gw.solve_order()
+    o = ow.step_probe(dt, keys, K, modes, capacity=cap, learned=r["learned"])
+    assert r["points"] == o["points"] > 1000
+    assert np.array_equal(r["features"], o["features"])
+    f, flags = o["features"], o["flags"]
+    want = mlp_ref.infer_warm_start(model.params(), model.mean, model.inv_std, model.target_scale, f, flags)
+    got = r["learned"]
+    skip = (flags & 4) != 0
+    assert np.array_equal(got[skip], f[skip, 16:19])
+    y = mlp_ref.forward(model.params(), mlp_ref.normalise(f, model.mean, model.inv_std))["Y"]
+    dy = 2e-2 * max(1.0, float(np.abs(y).max()))
+    with np.errstate(over="ignore", invalid="ignore"):
+        bound = dy * (model.target_scale[None, :].astype(np.float64) * mlp_ref.impulse_scale(f)[:, None]) + \
+            1e-6 * (1 + np.abs(want))
+    fin = np.isfinite(bound).all(1)
+    err = np.abs(got.astype(np.float64) - want)
+    assert np.all(err[fin] <= bound[fin]), float((err[fin] / bound[fin]).max())
+    if shrink == "overflow":
+        kept = np.all(want == f[:, 16:19], axis=1)
+        assert kept.mean() > 0.5 and np.array_equal(got[kept], f[kept, 16:19])
+    else:
+        changed = np.any(got != f[:, 16:19], axis=1)
+        assert changed.mean() > 0.5  # the network does act on most contacts
+    for m in (0, 1, 2):
+        assert np.allclose(r["residual"][m], o["residual"][m], rtol=1e-9, atol=0), m

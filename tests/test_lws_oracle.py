@@ -101,3 +101,53 @@ def test_probe_does_not_change_the_step():
     b.step_probe(dt, keys, 8, 5, capacity=100)
     for f in ("position", "orientation", "velocity", "angular_velocity"):
         assert np.array_equal(a.bodies()[f], b.bodies()[f])
+
+
+# ---- infer_warm_start: the in-step inference's post-processing (lws::k_infer) ------------------
+def _rows(n=200, seed=3):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=(n, 24)).astype(np.float32)
+    f[:, 13] = np.abs(f[:, 13]) + 0.1  # effective normal masses
+    f[:, 16] = np.abs(f[:, 16])        # previous normal impulses
+    return f, np.zeros(n, np.uint32)
+
+
+def _const_net(b3):
+    P = mlp_ref.unpack(mlp_ref.init_params(5))
+    P["W3"][:] = 0.0
+    P["b3"][:] = b3
+    return mlp_ref.pack(P).astype(np.float32)
+
+
+def test_infer_zero_output_keeps_previous_impulse():
+    f, flags = _rows()
+    out = mlp_ref.infer_warm_start(_const_net([0, 0, 0]), np.zeros(24), np.ones(24), np.ones(3), f, flags)
+    assert np.array_equal(out, f[:, 16:19])
+
+
+def test_infer_clamps_to_trust_region_and_normal_impulse():
+    # y = (10, -10, 0.5) clamps to (3, -3, 0.5); the normal impulse is not allowed below zero
+    f, flags = _rows()
+    ts = np.array([0.2, 0.1, 0.3], np.float32)
+    out = mlp_ref.infer_warm_start(_const_net([10, -10, 0.5]), np.zeros(24), np.ones(24), ts, f, flags)
+    a = np.abs(f.astype(np.float64))
+    sc = a[:, 13] * (a[:, 9] + a[:, 10] + a[:, 11] + a[:, 22] + 0.05)
+    np.testing.assert_allclose(out[:, 0], np.maximum(f[:, 16] + 3 * 0.2 * sc, 0), rtol=1e-5)
+    np.testing.assert_allclose(out[:, 1], f[:, 17] - 3 * 0.1 * sc, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(out[:, 2], f[:, 18] + 0.5 * 0.3 * sc, rtol=1e-5, atol=1e-6)
+    out = mlp_ref.infer_warm_start(_const_net([-10, 0, 0]), np.zeros(24), np.ones(24), ts, f, flags)
+    assert np.all(out[:, 0] >= 0) and np.any(out[:, 0] == 0)
+
+
+def test_infer_skip_rows_and_overflow_keep_previous():
+    f, flags = _rows()
+    flags[::3] = 4  # SkipWarmStart
+    out = mlp_ref.infer_warm_start(_const_net([1, 1, 1]), np.zeros(24), np.ones(24), np.ones(3), f, flags)
+    assert np.array_equal(out[::3], f[::3, 16:19]) and not np.array_equal(out[1::3], f[1::3, 16:19])
+    # target scale 3e38: the step overflows to inf wherever the impulse scale exceeds ~1.13
+    big = np.full(3, 3e38, np.float32)
+    out = mlp_ref.infer_warm_start(_const_net([1, 1, 1]), np.zeros(24), np.ones(24), big, f, np.zeros_like(flags))
+    sc = mlp_ref.impulse_scale(f)
+    over = sc * np.float32(3e38) > np.finfo(np.float32).max
+    assert over.any() and (~over).any()
+    assert np.array_equal(out[over], f[over, 16:19]) and np.all(np.isfinite(out))

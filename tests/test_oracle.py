@@ -520,6 +520,12 @@ def test_sweep_and_prune_equals_brute_force():
         w.step(dt)
     assert np.array_equal(w.pairs(brute=True), w.pairs())
     assert len(w.pairs()) > 100
+    # worlds sharing coordinates: the sweep runs world by world, brute force filters by world id
+    b, s, v, cfg, dt = scenes.batched_worlds(n_worlds=12, min_bodies=16, max_bodies=120, seed=2)
+    w = OracleWorld(b, s, v, cfg)
+    for _ in range(10):
+        w.step(dt)
+    assert np.array_equal(w.pairs(brute=True), w.pairs()) and len(w.pairs()) > 100
 
 
 def test_pyramid_stays_standing():  # Behavior/StackingTests.cs:36-50 (moves < 1 m)
