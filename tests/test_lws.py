@@ -201,5 +201,6 @@ def test_learned_warm_start_matches_oracle(shrink):
     else:
         changed = np.any(got != f[:, 16:19], axis=1)
         assert changed.mean() > 0.5  # the network does act on most contacts
+    # (with target scale 3e38 the finite predictions are huge and the first residuals overflow alike)
     for m in (0, 1, 2):
-        assert np.allclose(r["residual"][m], o["residual"][m], rtol=1e-9, atol=0), m
+        assert np.allclose(r["residual"][m], o["residual"][m], rtol=1e-9, atol=0, equal_nan=True), m
