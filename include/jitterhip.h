@@ -373,6 +373,16 @@ jh_status jh_lws_gradient(jh_lws* model, int64_t begin, int64_t count, float* gr
 /* Network outputs y [n][3] for raw features [n][24]. */
 jh_status jh_lws_predict(jh_lws* model, const float* features, int64_t count, float* outputs);
 
+/* jh_lws_predict without any allocation.  No counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  features [count][24] and outputs [count][3] are device buffers on the model's GPU; the
+ * normalised BF16 inputs are staged in one fixed device array per GPU of JH_LWS_PREDICT_MAX_ROWS rows
+ * (4 MiB, held until the process ends), which every call on that GPU shares: calls that overlap on one
+ * GPU race.  count > JH_LWS_PREDICT_MAX_ROWS returns JH_ERR_CAPACITY; a pointer that is not device
+ * memory of the model's GPU returns JH_ERR_INVALID_ARG.  Returns after the outputs are written. */
+#define JH_LWS_PREDICT_MAX_ROWS 65536
+jh_status jh_lws_predict_device(jh_lws* model, const float* features, int64_t count, float* outputs);
+
 /* Copies the model (weights and normalisation) into the world for JH_WARM_START_LEARNED. */
 jh_status jh_world_set_warm_start_model(jh_world* world, jh_lws* model);
 

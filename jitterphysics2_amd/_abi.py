@@ -47,6 +47,7 @@ FILTER_NONE, FILTER_TRIANGLE_EDGE = 0, 1
 
 WARM_START_PREVIOUS, WARM_START_LEARNED = 0, 1
 LWS_FEATURES, LWS_PARAMS, PROBE_MAX_ITERATIONS = 24, 20099, 256
+LWS_PREDICT_MAX_ROWS = 65536  # JH_LWS_PREDICT_MAX_ROWS
 PROBE_PREVIOUS, PROBE_LEARNED, PROBE_COLD = 1, 2, 4  # jh_probe.modes bits
 
 PROBE_DTYPE = np.dtype([
@@ -245,6 +246,7 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_lws_evaluate": ([vp, ctypes.c_int64, ctypes.c_int64, vp], i32),
         "jh_lws_gradient": ([vp, ctypes.c_int64, ctypes.c_int64, vp, vp], i32),
         "jh_lws_predict": ([vp, vp, ctypes.c_int64, vp], i32),
+        "jh_lws_predict_device": ([vp, vp, ctypes.c_int64, vp], i32),
         "jh_world_set_warm_start_model": ([vp, vp], i32),
         "jh_world_step_probe": ([vp, ctypes.c_float, vp], i32),
         "jh_world_autotune": ([vp, ctypes.c_float, i32, vp], i32),

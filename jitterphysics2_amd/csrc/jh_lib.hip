@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1569,6 +1570,8 @@ jh_status jh_world_get_stats(jh_world* w, jh_stats* s) {
 
 struct jh_lws {
     hipStream_t st = nullptr;
+    int device = 0;
+    uint16_t* scratch = nullptr;  // this GPU's staging array of jh_lws_predict_device (from lws_scratch_for)
     LwsWeights W;
     DBuf<float> params, grad, m, v;
     float hmean[lws::kIn], histd[lws::kIn], htscale[3];
@@ -1583,7 +1586,40 @@ struct jh_lws {
     int64_t adam_t = 0;
 };
 
+// ---- Staging without allocation (jh_lws_predict_device) ---------------------------------------
+// The normalised BF16 inputs of up to JH_LWS_PREDICT_MAX_ROWS rows, one copy per GPU for the life of
+// the process.  In host code `scratch` only names the symbol: each device's copy is found once, with
+// that device current, by hipGetSymbolAddress (lws_scratch_for), and only that address is used.
+static_assert(size_t(JH_LWS_PREDICT_MAX_ROWS) * lws::kInPad * sizeof(uint16_t) == (4u << 20), "4 MiB");
+static __device__ __align__(256) uint16_t scratch[size_t(JH_LWS_PREDICT_MAX_ROWS) * lws::kInPad];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint16_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's staging array (looked up once per device).
+jh_status lws_scratch_for(int dev, uint16_t** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(JH_ERR_UNSUPPORTED, "device %d: at most %d devices", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(JH_ERR_HIP, "hipGetSymbolAddress returned NULL");
+        g_scratch_ptr[dev] = static_cast<uint16_t*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return JH_OK;
+}
+
+// True when p is device memory of GPU dev (allocated memory, not host or registered memory).
+bool is_device_memory(const void* p, int dev) {
+    hipPointerAttribute_t a;
+    hipError_t e = hipPointerGetAttributes(&a, p);
+    (void)hipGetLastError();  // an unknown host pointer is reported as an error, which is not sticky
+    return e == hipSuccess && a.type == hipMemoryTypeDevice && a.device == dev;
+}
 
 uint64_t splitmix64(uint64_t& s) {
     uint64_t z = (s += 0x9E3779B97F4A7C15ull);
@@ -1689,6 +1725,14 @@ jh_status jh_lws_create(uint64_t seed, jh_lws** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(JH_ERR_NO_DEVICE, "no HIP device visible");
     jh_lws* L = new jh_lws();
+    HIPCHK(hipGetDevice(&L->device));
+    {
+        jh_status sc = lws_scratch_for(L->device, &L->scratch);
+        if (sc != JH_OK) {
+            delete L;
+            return sc;
+        }
+    }
     HIPCHK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
     HIPCHK(L->W.alloc());
     HIPCHK(L->params.ensure(lws::kParams));
@@ -1872,6 +1916,25 @@ jh_status jh_lws_predict(jh_lws* L, const float* features, int64_t count, float*
     if (e == hipSuccess) e = hipStreamSynchronize(L->st);
     xb.release();
     if (e != hipSuccess) return fail(JH_ERR_HIP, "jh_lws_predict: %s", hipGetErrorString(e));
+    return JH_OK;
+}
+
+jh_status jh_lws_predict_device(jh_lws* L, const float* features, int64_t count, float* outputs) {
+    if (!L || !features || !outputs || count < 1) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (count > JH_LWS_PREDICT_MAX_ROWS)
+        return fail(JH_ERR_CAPACITY, "jh_lws_predict_device: %lld rows exceed the staging array's %d rows",
+                    (long long)count, JH_LWS_PREDICT_MAX_ROWS);
+    if (!L->scratch) return fail(JH_ERR_STATE, "jh_lws_predict_device: no staging array on this device");
+    if (!is_device_memory(features, L->device) || !is_device_memory(outputs, L->device))
+        return fail(JH_ERR_INVALID_ARG, "jh_lws_predict_device: features and outputs must be device memory of GPU %d",
+                    L->device);
+    int n = int(count);
+    lws::k_prepare_data<<<(n + 255) / 256, 256, 0, L->st>>>(features, nullptr, n, L->W.mean.p, L->W.istd.p,
+                                                            L->W.tscale.p, L->scratch, nullptr);
+    int waves = (n + 15) / 16;
+    lws::k_predict<<<(waves + 3) / 4, 256, 0, L->st>>>(L->W.model(), L->scratch, n, outputs);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(L->st));
     return JH_OK;
 }
 
