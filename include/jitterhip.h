@@ -267,6 +267,14 @@ jh_status jh_world_set_shapes(jh_world* world, const jh_shape* shapes, int32_t c
 /* The reference's 128-byte RigidBodyData layout (FP32), for zero-copy exchange with World.RawData
  * (World.cs:95-141).  stride = bytes between records (>= 128). */
 jh_status jh_world_get_rigidbodydata(jh_world* world, void* records, int32_t count, int32_t stride);
+/* Writes of World.RawData back to the world: count = the body count, records in body order.  Takes
+ * Position, Velocity, AngularVelocity, DeltaVelocity, DeltaAngularVelocity, Orientation,
+ * InverseInertiaWorld and InverseMassVector as they are (the solver uses them until the end-of-step
+ * RigidBody.Update recomputes the world inertia, as in the reference); _index and _lockFlag are
+ * ignored.  Flags: the MotionType bits must equal the body's (JH_ERR_UNSUPPORTED otherwise: change
+ * it with jh_world_set_bodies), the gyroscopic and motion-lock bits (3..9) must be clear
+ * (JH_ERR_UNSUPPORTED); non-finite values give JH_ERR_NON_FINITE.  Nothing is written on an error. */
+jh_status jh_world_set_rigidbodydata(jh_world* world, const void* records, int32_t count, int32_t stride);
 
 /* World.Step(dt): one full step (narrowphase over the pair set, contact cache, substepped
  * solve, contact refresh, body update, broadphase).  Returns after the step finished on the GPU.
@@ -274,6 +282,15 @@ jh_status jh_world_get_rigidbodydata(jh_world* world, void* records, int32_t cou
  * jh_world_set_spin_limit) or the step was cancelled; every kernel of the step has returned and the
  * world stays usable, but its device state is unspecified until bodies and arbiters are set again. */
 jh_status jh_world_step(jh_world* world, float dt);
+
+/* World.Stabilize(dt, solverIterations, relaxationIterations) (World.Step.cs:318-392): per substep,
+ * SolveVelocities with `iterations` iterations and RelaxVelocities with `relaxations`, on the contacts
+ * and constraints the world holds (from the last step or jh_world_set_arbiters / set_constraints):
+ * no collision detection, no force or position integration, no body update; warm-starts from the
+ * accumulated impulses.  Argument checks and errors as StabilizeInternal; dt below float.Epsilon is a
+ * no-op.  Solved in colour batches (the colours of the current arbiters); jh_world_get_solve_order then
+ * returns this solve's order.  Sleeping (CheckDeactivation) is not part of the GPU path. */
+jh_status jh_world_stabilize(jh_world* world, float dt, int32_t iterations, int32_t relaxations);
 
 /* Bounded device waits (no counterpart in the reference, whose solver runs on host threads; for callers
  * that must never hang the GPU).  Each wait of a persistent kernel gives up after `polls` polls

@@ -223,8 +223,10 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_world_get_bodies": ([vp, vp, i32], i32),
         "jh_world_set_shapes": ([vp, vp, i32, vp, i32], i32),
         "jh_world_get_rigidbodydata": ([vp, vp, i32, i32], i32),
+        "jh_world_set_rigidbodydata": ([vp, vp, i32, i32], i32),
         "jh_world_step": ([vp, ctypes.c_float], i32),
         "jh_world_set_spin_limit": ([vp, ctypes.c_uint32], i32),
+        "jh_world_stabilize": ([vp, ctypes.c_float, i32, i32], i32),
         "jh_world_cancel": ([vp], i32),
         "jh_world_compute_pairs": ([vp, vp, i32, pi32], i32),
         "jh_world_get_manifolds": ([vp, vp, i32, pi32], i32),

@@ -272,6 +272,24 @@ __global__ void k_pack_rigidbodydata(Bodies B, int n, char* __restrict__ out, in
     ((int*)(rf + 28))[0] = (B.motion[b] & 3) | 4;  // MotionType | IsActive
 }
 
+// The body state of the reference's 128-byte FP32 RigidBodyData records (World.RawData): position,
+// velocities, delta velocities, orientation, InverseInertiaWorld and InverseMassVector; the flags
+// were checked on the host.
+__global__ void k_unpack_rigidbodydata(const char* __restrict__ in, int n, int stride, Bodies B) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const float* rf = (const float*)(in + size_t(b) * stride + 8);
+    B.pos[b] = make_float4(rf[0], rf[1], rf[2], 0.0f);
+    B.vel[b] = make_float4(rf[3], rf[4], rf[5], 0.0f);
+    B.angvel[b] = make_float4(rf[6], rf[7], rf[8], 0.0f);
+    B.dvel[b] = make_float4(rf[9], rf[10], rf[11], 0.0f);
+    B.dangvel[b] = make_float4(rf[12], rf[13], rf[14], 0.0f);
+    B.quat[b] = make_float4(rf[15], rf[16], rf[17], rf[18]);
+    B.iw0[b] = make_float4(rf[19], rf[20], rf[21], rf[22]);
+    B.iw1[b] = make_float2(rf[23], rf[24]);
+    B.imv[b] = make_float4(rf[25], rf[26], rf[27], 0.0f);
+}
+
 // ---- Broadphase -----------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t f2ord(float f) {
     uint32_t u = __float_as_uint(f);
@@ -1528,6 +1546,7 @@ struct SolveArgs {
     ContactSoA soa;          // dataflow solver: colour-ordered contact rows
     float4* vw;              // dataflow solvers: per body (v.x v.y v.z n)(w.x w.y w.z n), 32 bytes
     ConArgs con;             // barrier solver: constraints (nbatch 0: none)
+    int stabilize;           // barrier solver: 1 = World.Stabilize (no IntegrateForces / IntegrateVelocities)
 };
 
 
@@ -1643,16 +1662,20 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     int nc = *P.ncolours;
     for (int s = 0; s < P.ts.substeps; ++s) {
-        integrate_forces<true>(P, t0, stride);
-        if (!grid_barrier_coh(P.bar, P.spin)) return;
+        if (!P.stabilize) {
+            integrate_forces<true>(P, t0, stride);
+            if (!grid_barrier_coh(P.bar, P.spin)) return;
+        }
         if (!solve_batches<true>(P, 0, nc)) return;
         if (!solve_constraint_batches<true>(P, 0)) return;
         for (int it = 0; it < P.iterations; ++it) {
             if (!solve_batches<true>(P, 1, nc)) return;
             if (!solve_constraint_batches<true>(P, 1)) return;
         }
-        integrate_velocities<true>(P, t0, stride);
-        if (!grid_barrier_coh(P.bar, P.spin)) return;
+        if (!P.stabilize) {
+            integrate_velocities<true>(P, t0, stride);
+            if (!grid_barrier_coh(P.bar, P.spin)) return;
+        }
         for (int it = 0; it < P.relaxations; ++it)
             if (!solve_batches<true>(P, 2, nc)) return;
     }

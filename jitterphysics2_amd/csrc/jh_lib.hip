@@ -652,6 +652,48 @@ jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int ama
     return JH_OK;
 }
 
+// Graph colouring of the arbiters in arb[cur] (edges between non-static bodies) and the colour-sorted
+// solve order (*order_out, device keys (colour << 34 | contacts << 32) | arbiter); colour count, chunk
+// count and the contact points to solve land in the device counters.
+jh_status colour_arbiters(jh_world* w, int cur, int* a_cur, int amax_, const uint64_t** order_out) {
+    hipStream_t st = w->st;
+    int* cnt = w->cnt.p;
+    ColourArgs P;
+    P.arb = w->arb[cur].p;
+    P.a_count = a_cur;
+    P.motion = w->motion.p;
+    P.colour = w->colour.p;
+    P.body_max = w->bmax.p;
+    P.nb = int(w->bmax.cap / 2);
+    P.mask_lo = w->mlo.p;
+    P.mask_hi = w->mhi.p;
+    P.work = w->cwork.p;
+    P.wcount = cnt + C_REM;
+    P.ncolours = cnt + C_NCOL;
+    P.hist = w->chist.p;
+    P.bar = w->bar.p;
+    P.spin = w->spin();
+    // colours persist across steps; recolour everything after body changes or when the palette
+    // has spread (first-fit over many steps can leave more colours than a fresh colouring)
+    P.keep = (w->recolour || w->solve_batches > 48) ? 0 : 1;
+    w->recolour = false;
+    int grid = std::min(w->colour_blocks, grid_for(std::max(amax_, 128), 256, 1 << 20));
+    void* args[] = {&P};
+    jh_status s = launch_coop((const void*)k_colour, grid, 256, args, st);
+    if (s != JH_OK) return s;
+    k_colour_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(w->colour.p, w->arb[cur].p, a_cur, w->ok0.p,
+                                                                 cnt + C_CONTACTS);
+    int csh[2] = {32, 40};  // (colour, contacts) in bits 32..41
+    bool in1 = rsort::sort_pairs(w->ok0.p, w->ok1.p, nullptr, nullptr, a_cur, std::max(amax_, 1), csh, 2,
+                                 w->hist.p, st);
+    k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p, w->cbeg.p, cnt + C_NCHUNK);
+    const uint64_t* order = in1 ? w->ok1.p : w->ok0.p;
+    k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
+    HIPCHK(hipGetLastError());
+    *order_out = order;
+    return JH_OK;
+}
+
 jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     hipStream_t st = w->st;
     Bodies B = w->bodies();
@@ -752,37 +794,9 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     }
     bool use_tiles = kind == SOLVER_TILES;
     {
-        ColourArgs P;
-        P.arb = w->arb[cur].p;
-        P.a_count = a_cur;
-        P.motion = w->motion.p;
-        P.colour = w->colour.p;
-        P.body_max = w->bmax.p;
-        P.nb = int(w->bmax.cap / 2);
-        P.mask_lo = w->mlo.p;
-        P.mask_hi = w->mhi.p;
-        P.work = w->cwork.p;
-        P.wcount = cnt + C_REM;
-        P.ncolours = cnt + C_NCOL;
-        P.hist = w->chist.p;
-        P.bar = w->bar.p;
-        P.spin = w->spin();
-        // colours persist across steps; recolour everything after body changes or when the palette
-        // has spread (first-fit over many steps can leave more colours than a fresh colouring)
-        P.keep = (w->recolour || w->solve_batches > 48) ? 0 : 1;
-        w->recolour = false;
-        int grid = std::min(w->colour_blocks, grid_for(std::max(amax_, 128), 256, 1 << 20));
-        void* args[] = {&P};
-        s = launch_coop((const void*)k_colour, grid, 256, args, st);
+        const uint64_t* order = nullptr;
+        s = colour_arbiters(w, cur, a_cur, amax_, &order);
         if (s != JH_OK) return s;
-        k_colour_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(w->colour.p, w->arb[cur].p, a_cur, w->ok0.p,
-                                                                     cnt + C_CONTACTS);
-        int csh[2] = {32, 40};  // (colour, contacts) in bits 32..41
-        bool in1 = rsort::sort_pairs(w->ok0.p, w->ok1.p, nullptr, nullptr, a_cur, std::max(amax_, 1), csh, 2,
-                                     w->hist.p, st);
-        k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p, w->cbeg.p, cnt + C_NCHUNK);
-        const uint64_t* order = in1 ? w->ok1.p : w->ok0.p;
-        k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
         if (kind == SOLVER_FLOW || kind == SOLVER_TILES || kind == SOLVER_LANES) {
             // per-body update sequence: rank of each arbiter visit and body degree
             int ga = grid_for(std::max(amax_, 1));
@@ -838,6 +852,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.con.order = w->con_order.p;
         S.con.offs = w->con_offs.p;
         S.con.nbatch = kind == SOLVER_BARRIER ? constraint_batches(w) : 0;
+        S.stabilize = 0;
         if (probe) {
             s = run_probe(w, S, a_cur, amax_, probe);
             if (s != JH_OK) return s;
@@ -1238,6 +1253,32 @@ jh_status jh_world_get_bodies(jh_world* w, jh_body* bodies, int32_t n) {
     return JH_OK;
 }
 
+jh_status jh_world_set_rigidbodydata(jh_world* w, const void* records, int32_t n, int32_t stride) {
+    if (!w || (n > 0 && !records) || n < 0 || stride < 128) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (n != w->nb) return fail(JH_ERR_INVALID_ARG, "%d records for %d bodies: RawData covers every body", n, w->nb);
+    if (n == 0) return JH_OK;
+    const char* in = static_cast<const char*>(records);
+    for (int i = 0; i < n; ++i) {
+        const char* r = in + size_t(i) * stride;
+        int32_t flags;
+        memcpy(&flags, r + 120, 4);
+        if ((flags & 3) != w->host_bodies[i].motion_type)
+            return fail(JH_ERR_UNSUPPORTED, "body %d: the motion type differs from the body's; set it with jh_world_set_bodies", i);
+        if (flags & 0x3f8)
+            return fail(JH_ERR_UNSUPPORTED, "body %d: gyroscopic integration and motion locks are not on the GPU path", i);
+        float f[28];
+        memcpy(f, r + 8, sizeof(f));
+        for (int k = 0; k < 28; ++k)
+            if (!std::isfinite(f[k])) return fail(JH_ERR_NON_FINITE, "body %d: non-finite field at offset %d", i, 8 + 4 * k);
+    }
+    HIPCHK(w->rbd_io.ensure(size_t(n) * stride));
+    HIPCHK(hipMemcpyAsync(w->rbd_io.p, records, size_t(n) * stride, hipMemcpyHostToDevice, w->st));
+    k_unpack_rigidbodydata<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(w->rbd_io.p, n, stride, w->bodies());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
 jh_status jh_world_get_rigidbodydata(jh_world* w, void* records, int32_t n, int32_t stride) {
     if (!w || !records || n < 0 || n > w->nb || stride < 128) return fail(JH_ERR_INVALID_ARG, "bad arguments");
     if (n == 0) return JH_OK;
@@ -1301,6 +1342,79 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
     w->a_host = 0;
     w->cur = 0;
     HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_stabilize(jh_world* w, float dt, int32_t iterations, int32_t relaxations) {
+    if (!w) return fail(JH_ERR_STATE, "world is NULL");
+    // StabilizeInternal argument checks, in its order (World.Step.cs:332-350)
+    if (!std::isfinite(dt)) return fail(JH_ERR_NON_FINITE, "dt must be finite");
+    if (dt < 0.0f) return fail(JH_ERR_INVALID_ARG, "Time step cannot be negative.");
+    if (iterations < 1) return fail(JH_ERR_INVALID_ARG, "Solver iterations cannot be smaller than one.");
+    if (relaxations < 0) return fail(JH_ERR_INVALID_ARG, "Relaxation iterations cannot be smaller than zero.");
+    if (dt < 1.401298464e-45f || w->nb == 0) return JH_OK;  // nothing to do
+    hipStream_t st = w->st;
+    int* cnt = w->cnt.p;
+    jh_status s = ensure_step_buffers(w);
+    if (s != JH_OK) return s;
+    if (!w->host_con.empty() && w->con_dirty) {
+        s = colour_constraints(w);
+        if (s != JH_OK) return s;
+    }
+    const jh_config& cfg = w->cfg;
+    StepTime ts;  // TimeStep (TimeStep.cs:36-46)
+    ts.dt = dt;
+    ts.substeps = cfg.substep_count;
+    ts.substep_dt = dt / float(cfg.substep_count);
+    ts.inv_dt = 1.0f / dt;
+    ts.inv_substep_dt = ts.substep_dt > 0.0f ? 1.0f / ts.substep_dt : 0.0f;
+    int cur = w->cur;
+    int* a_cur = cnt + (cur == 0 ? C_A0 : C_A1);
+    int amax_ = w->a_host;
+    k_set_int<<<1, 1, 0, st>>>(cnt + C_CONTACTS, 0);
+    const uint64_t* order = nullptr;
+    s = colour_arbiters(w, cur, a_cur, amax_, &order);
+    if (s != JH_OK) return s;
+    // the colour-batch solver without IntegrateForces / IntegrateVelocities (SolveArgs.stabilize)
+    SolveArgs S{};
+    S.arb = w->arb[cur].p;
+    S.order = order;
+    S.offs = w->coffs.p;
+    S.ncolours = cnt + C_NCOL;
+    S.bar = w->bar.p;
+    S.spin = w->spin();
+    S.B = w->bodies();
+    S.nb = w->nb;
+    S.ts = ts;
+    S.iterations = iterations;
+    S.relaxations = relaxations;
+    S.con.c = w->con.p;
+    S.con.work = w->con_work.p;
+    S.con.order = w->con_order.p;
+    S.con.offs = w->con_offs.p;
+    S.con.nbatch = constraint_batches(w);
+    S.stabilize = 1;
+    int sgrid = std::min(w->coop_solve, grid_for(std::max(std::max(std::max(amax_, w->nb), 256),
+                                                          int(w->con_order_h.size())), 256, 1 << 20));
+    void* sargs[] = {&S};
+    s = launch_coop((const void*)k_solve, sgrid, 256, sargs, st);
+    if (s != JH_OK) return s;
+    HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(w->habort, w->bar.p + kBarAbort, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (*w->habort) {
+        HIPCHK(hipMemsetAsync(w->bar.p, 0, 64 * sizeof(unsigned), st));
+        HIPCHK(hipStreamSynchronize(st));
+        *w->habort = 0;
+        w->have_step = false;
+        w->recolour = true;
+        return fail(JH_ERR_TIMEOUT, "jh_world_stabilize: a wait of the solver gave up after %u polls or it was cancelled",
+                    w->spin());
+    }
+    // jh_world_get_solve_order now describes this solve
+    w->solve_count = w->hcnt[cur == 0 ? C_A0 : C_A1];
+    w->solve_batches = w->hcnt[C_NCOL];
+    w->have_step = true;
     return JH_OK;
 }
 

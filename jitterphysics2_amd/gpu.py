@@ -60,6 +60,12 @@ class GpuWorld:
         """jh_world_step's status code, without raising."""
         return int(self.lib.jh_world_step(self.handle, ctypes.c_float(dt)))
 
+    def stabilize(self, dt: float, iterations: int, relaxations: int = 0) -> None:
+        """World.Stabilize (jh_world_stabilize): solve the held contacts and constraints at the velocity
+        level, without detection or integration."""
+        check(self.lib, self.lib.jh_world_stabilize(self.handle, ctypes.c_float(dt), iterations, relaxations),
+              "jh_world_stabilize")
+
     def set_spin_limit(self, polls: int) -> None:
         check(self.lib, self.lib.jh_world_set_spin_limit(self.handle, ctypes.c_uint32(polls)), "jh_world_set_spin_limit")
 
@@ -76,6 +82,12 @@ class GpuWorld:
         check(self.lib, self.lib.jh_world_get_rigidbodydata(self.handle, ptr(out), self.n_bodies, stride),
               "jh_world_get_rigidbodydata")
         return out.reshape(self.n_bodies, stride)
+
+    def set_rigidbodydata(self, records: np.ndarray) -> None:
+        """Write RawData records (n, stride) uint8 back (jh_world_set_rigidbodydata)."""
+        r = np.ascontiguousarray(records, np.uint8)
+        check(self.lib, self.lib.jh_world_set_rigidbodydata(self.handle, ptr(r), r.shape[0], r.shape[1]),
+              "jh_world_set_rigidbodydata")
 
     def compute_pairs(self) -> np.ndarray:
         n = ctypes.c_int32(0)

@@ -399,6 +399,20 @@ int orc_step(void* p, float dt, const uint32_t* order, int nkeys) {
     DISPATCH(h, (W.step(decltype(W.gravity.x)(dt), order ? &ord : nullptr), 0));
     return 0;
 }
+// World.Stabilize(dt, iterations, relaxations); order = nkeys (key1, key2) pairs or NULL (key order).
+// Argument checks as StabilizeInternal (World.Step.cs:332-350): -1 bad argument, -2 non-finite dt.
+int orc_stabilize(void* p, float dt, int iters, int relax, const uint32_t* order, int nkeys) {
+    Handle* h = (Handle*)p;
+    if (!std::isfinite(dt)) return -2;
+    if (dt < 0.0f || iters < 1 || relax < 0) return -1;
+    std::vector<std::pair<uint32_t, uint32_t>> ord;
+    if (order) {
+        ord.resize(nkeys);
+        for (int i = 0; i < nkeys; ++i) ord[i] = {order[2 * i], order[2 * i + 1]};
+    }
+    DISPATCH(h, (W.stabilize(decltype(W.gravity.x)(dt), iters, relax, order ? &ord : nullptr), 0));
+    return 0;
+}
 // Step with residual probes of the first substep (modes: bit 0 previous-frame warm start, bit 2
 // cold); residual[3][iterations + 1], features [capacity][24], targets [capacity][3];
 // counts[0] = rows written, counts[1] = contact points.

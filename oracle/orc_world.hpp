@@ -775,6 +775,47 @@ template <class R> struct World {
 
     void step(R dt, const std::vector<std::pair<uint32_t, uint32_t>>* order) { step_impl(dt, order, nullptr); }
 
+    // World.StabilizeInternal (World.Step.cs:330-392), SolveMode.Regular, without CheckDeactivation
+    // (no sleeping here): per substep SolveVelocities with `iters` iterations (contacts, then
+    // constraints) and RelaxVelocities with `relax` iterations, on the existing contacts and
+    // constraints; no detection, no IntegrateForces / IntegrateVelocities, no body update.  The order
+    // is given (a replay) or the key order.
+    void stabilize(R dt, int iters, int relax, const std::vector<std::pair<uint32_t, uint32_t>>* order) {
+        if (dt < std::numeric_limits<R>::denorm_min()) return;
+        StepTiming<R> ts(dt, substeps);
+        std::vector<Arbiter<R>*> list;
+        if (order) {
+            for (auto& k : *order) {
+                auto it = arbiters.find(k);
+                if (it != arbiters.end()) list.push_back(&it->second);
+            }
+        } else {
+            for (auto& kv : arbiters) list.push_back(&kv.second);
+        }
+        for (int s = 0; s < ts.substeps; ++s) {
+            for (auto* a : list)
+                for (int i = 0; i < 4; ++i)
+                    if (a->usage & (1u << i)) prepare_contact(*a, a->c[i], ts);
+            for_constraints([&](Constraint<R>& c) {
+                ConstraintBody<R> A = cbody(c.b1), B = cbody(c.b2);
+                constraint_prepare(c, A, B, ts.inv_dt, ts.inv_substep_dt, ts.substep_dt);
+            });
+            for (int it = 0; it < iters; ++it) {
+                for (auto* a : list)
+                    for (int i = 0; i < 4; ++i)
+                        if (a->usage & (1u << i)) iterate_contact(*a, a->c[i], true);
+                for_constraints([&](Constraint<R>& c) {
+                    ConstraintBody<R> A = cbody(c.b1), B = cbody(c.b2);
+                    constraint_iterate(c, A, B, ts.inv_substep_dt);
+                });
+            }
+            for (int it = 0; it < relax; ++it)
+                for (auto* a : list)
+                    for (int i = 0; i < 4; ++i)
+                        if (a->usage & (1u << i)) iterate_contact(*a, a->c[i], false);
+        }
+    }
+
     void step_impl(R dt, const std::vector<std::pair<uint32_t, uint32_t>>* order, Probe* probe) {
         if (!(dt >= R(0)) || !std::isfinite(dt)) return;
         if (dt < std::numeric_limits<R>::denorm_min()) return;

@@ -47,6 +47,7 @@ def lib() -> ctypes.CDLL:
             "orc_get_events": [vp, vp, i32, vp, i32, vp],
             "orc_set_constraints": [vp, vp, i32], "orc_get_constraints": [vp, vp, i32],
             "orc_set_constraint_order": [vp, vp, i32],
+            "orc_stabilize": [vp, f, i32, i32, vp, i32],
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32
@@ -102,6 +103,13 @@ class OracleWorld:
         else:
             o = np.ascontiguousarray(order, np.uint32).reshape(-1, 2)
             self.L.orc_step(self.h, dt, ptr(o), len(o))
+
+    def stabilize(self, dt: float, iterations: int, relaxations: int = 0, order: np.ndarray | None = None) -> int:
+        """World.Stabilize: returns 0, -1 (bad argument) or -2 (non-finite dt)."""
+        if order is None:
+            return int(self.L.orc_stabilize(self.h, dt, iterations, relaxations, None, 0))
+        o = np.ascontiguousarray(order, np.uint32).reshape(-1, 2)
+        return int(self.L.orc_stabilize(self.h, dt, iterations, relaxations, ptr(o), len(o)))
 
     def step_probe(self, dt: float, order: np.ndarray, iterations: int, modes: int, capacity: int = 0,
                    learned: np.ndarray | None = None) -> dict:
