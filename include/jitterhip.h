@@ -76,7 +76,9 @@ typedef struct jh_config {
     int32_t warm_start;                  /* JH_WARM_START_*; no counterpart in the reference      */
     int32_t narrowphase_filter;          /* World.NarrowPhaseFilter: JH_FILTER_*, default none     */
     float speculative_velocity_threshold; /* World.SpeculativeVelocityThreshold, default 10       */
-    int32_t reserved[3];
+    int32_t allow_deactivation;          /* World.AllowDeactivation, default 1 (World.cs:333): islands
+                                            whose bodies rest for their DeactivationTime sleep       */
+    int32_t reserved[2];
 } jh_config;
 
 /* World.NarrowPhaseFilter (World.Detect.cs:197-203).  TRIANGLE_EDGE = TriangleEdgeCollisionFilter
@@ -116,6 +118,13 @@ typedef struct jh_body {
                                         the body's broadphase box is swept over dt * |velocity|
                                         and pairs MPR finds separated are swept for a time of
                                         impact (World.Detect.cs:162-194)                          */
+    /* RigidBody.DeactivationThreshold and DeactivationTime (RigidBody.cs:398-400, 476-507): the body
+     * rests while |v|^2 < linear_sq and |w|^2 < angular_sq (the squares, as the reference stores them;
+     * defaults 0.1, 0.1) and may sleep after resting for deactivation_time seconds (default 1). */
+    float deactivation_threshold_linear_sq;
+    float deactivation_threshold_angular_sq;
+    float deactivation_time;
+    int32_t reserved_body;
 } jh_body;
 
 /* One shape attached to a body.  shape_id is Shape.ShapeId (Collision/Shapes/Shape.cs:28); shapes
@@ -238,6 +247,7 @@ typedef struct jh_stats {
     int64_t max_degree;       /* most solved arbiters on one non-static body (a lower bound of the
                                  colour count); dataflow and tile solvers only, else 0 */
     double ms_solve_kernel;   /* part of ms_solve spent in the solver kernel itself */
+    int64_t bodies_asleep;    /* non-static bodies asleep after the last CheckDeactivation */
 } jh_stats;
 
 typedef struct jh_world jh_world;
@@ -258,11 +268,30 @@ jh_status jh_world_set_config(jh_world* world, const jh_config* config);
 jh_status jh_config_defaults(jh_config* out_config);
 
 /* Scene upload/download.  set_bodies replaces all bodies; set_shapes replaces all shapes and their
- * vertex data (vertices = 3 floats each) and clears the contact cache when the shape set changes. */
+ * vertex data (vertices = 3 floats each) and clears the contact cache when the shape set changes.
+ * Sleeping: writing a body's record is the reference's property setters at once, so each body is
+ * World.ActivateBodyNextStep(body, wakeUpStatic: true) as the Position setter does (RigidBody.cs:593:
+ * its sleep timer restarts, a sleeping body wakes at the next step, a static one wakes the bodies it
+ * touches); a new body count or a motion-type change rebuilds the islands with every body awake
+ * (World.CreateRigidBody).  set_shapes, set_arbiters and set_constraints also rebuild the islands. */
 jh_status jh_world_set_bodies(jh_world* world, const jh_body* bodies, int32_t count);
 jh_status jh_world_get_bodies(jh_world* world, jh_body* bodies, int32_t count);
 jh_status jh_world_set_shapes(jh_world* world, const jh_shape* shapes, int32_t count,
                               const float* vertices, int32_t vertex_count);
+/* The records of the listed bodies only (RigidBody property setters on those bodies), with
+ * set_bodies' rules for each listed body; indices distinct.  world_id and EnableSpeculativeContacts
+ * must stay (JH_ERR_UNSUPPORTED: use jh_world_set_bodies). */
+jh_status jh_world_update_bodies(jh_world* world, const int32_t* index, const jh_body* records, int32_t count);
+
+/* Sleeping (Collision/IslandHelper.cs, World.CheckDeactivation World.Step.cs:1289-1371, RigidBody.Update
+ * RigidBody.cs:716-743).  get_activation: RigidBody.IsActive (1 awake) and the sleep timer
+ * (InternalSleepTime, seconds; infinite after DeactivateBodyNextStep) of bodies 0..count-1.
+ * set_activation_state: RigidBody.SetActivationState (RigidBody.cs:1025-1041), effective at the next
+ * step: active != 0 restarts the timer and wakes a sleeping non-static body's island; 0 sets the
+ * timer to infinity, so the island sleeps once no body in it moves.  jh_stats.bodies_asleep counts
+ * the sleeping non-static bodies. */
+jh_status jh_world_get_activation(jh_world* world, uint8_t* active, float* sleep_time, int32_t count);
+jh_status jh_world_set_activation_state(jh_world* world, int32_t body, int32_t active);
 
 /* The reference's 128-byte RigidBodyData layout (FP32), for zero-copy exchange with World.RawData
  * (World.cs:95-141).  stride = bytes between records (>= 128). */
@@ -272,8 +301,10 @@ jh_status jh_world_get_rigidbodydata(jh_world* world, void* records, int32_t cou
  * InverseInertiaWorld and InverseMassVector as they are (the solver uses them until the end-of-step
  * RigidBody.Update recomputes the world inertia, as in the reference); _index and _lockFlag are
  * ignored.  Flags: the MotionType bits must equal the body's (JH_ERR_UNSUPPORTED otherwise: change
- * it with jh_world_set_bodies), the gyroscopic and motion-lock bits (3..9) must be clear
- * (JH_ERR_UNSUPPORTED); non-finite values give JH_ERR_NON_FINITE.  Nothing is written on an error. */
+ * it with jh_world_set_bodies), the IsActive bit the body's activation (JH_ERR_UNSUPPORTED: use
+ * jh_world_set_activation_state; a RawData write wakes nothing), the gyroscopic and motion-lock bits
+ * (3..9) must be clear (JH_ERR_UNSUPPORTED); non-finite values give JH_ERR_NON_FINITE.  Nothing is
+ * written on an error. */
 jh_status jh_world_set_rigidbodydata(jh_world* world, const void* records, int32_t count, int32_t stride);
 
 /* World.Step(dt): one full step (narrowphase over the pair set, contact cache, substepped
@@ -289,7 +320,8 @@ jh_status jh_world_step(jh_world* world, float dt);
  * no collision detection, no force or position integration, no body update; warm-starts from the
  * accumulated impulses.  Argument checks and errors as StabilizeInternal; dt below float.Epsilon is a
  * no-op.  Solved in colour batches (the colours of the current arbiters); jh_world_get_solve_order then
- * returns this solve's order.  Sleeping (CheckDeactivation) is not part of the GPU path. */
+ * returns this solve's order.  Starts with CheckDeactivation, as the reference does; only the contacts
+ * and constraints with an awake dynamic body are solved. */
 jh_status jh_world_stabilize(jh_world* world, float dt, int32_t iterations, int32_t relaxations);
 
 /* Bounded device waits (no counterpart in the reference, whose solver runs on host threads; for callers

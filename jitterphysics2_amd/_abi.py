@@ -40,7 +40,8 @@ CONFIG_DTYPE = np.dtype([
     ("warm_start", "<i4"),
     ("narrowphase_filter", "<i4"),
     ("speculative_velocity_threshold", "<f4"),
-    ("reserved", "<i4", 3),
+    ("allow_deactivation", "<i4"),
+    ("reserved", "<i4", 2),
 ])
 
 FILTER_NONE, FILTER_TRIANGLE_EDGE = 0, 1
@@ -82,6 +83,10 @@ BODY_DTYPE = np.dtype([
     ("affected_by_gravity", "<i4"),
     ("world_id", "<i4"),
     ("enable_speculative_contacts", "<i4"),
+    ("deactivation_threshold_linear_sq", "<f4"),
+    ("deactivation_threshold_angular_sq", "<f4"),
+    ("deactivation_time", "<f4"),
+    ("reserved_body", "<i4"),
 ])
 
 SHAPE_DTYPE = np.dtype([
@@ -141,7 +146,7 @@ STATS_DTYPE = np.dtype([
     ("steps", "<i8"), ("pair_overflow", "<i8"), ("arbiters_solved", "<i8"),
     ("ms_broadphase", "<f8"), ("ms_narrowphase", "<f8"), ("ms_contact_cache", "<f8"),
     ("ms_colouring", "<f8"), ("ms_solve", "<f8"), ("ms_update", "<f8"), ("ms_step", "<f8"), ("ms_epa", "<f8"),
-    ("solver", "<i8"), ("max_degree", "<i8"), ("ms_solve_kernel", "<f8"),
+    ("solver", "<i8"), ("max_degree", "<i8"), ("ms_solve_kernel", "<f8"), ("bodies_asleep", "<i8"),
 ])
 
 TUNING_DTYPE = np.dtype([
@@ -183,6 +188,7 @@ def default_config() -> np.ndarray:
     c["persistent_contact_manifold"] = 1
     c["pair_capacity"] = 0
     c["speculative_velocity_threshold"] = 10.0
+    c["allow_deactivation"] = 1
     return c
 
 
@@ -227,6 +233,9 @@ def load_library(path: str | None = None) -> ctypes.CDLL:
         "jh_world_step": ([vp, ctypes.c_float], i32),
         "jh_world_set_spin_limit": ([vp, ctypes.c_uint32], i32),
         "jh_world_stabilize": ([vp, ctypes.c_float, i32, i32], i32),
+        "jh_world_update_bodies": ([vp, vp, vp, i32], i32),
+        "jh_world_get_activation": ([vp, vp, vp, i32], i32),
+        "jh_world_set_activation_state": ([vp, i32, i32], i32),
         "jh_world_cancel": ([vp], i32),
         "jh_world_compute_pairs": ([vp, vp, i32, pi32], i32),
         "jh_world_get_manifolds": ([vp, vp, i32, pi32], i32),

@@ -32,6 +32,8 @@ struct Bodies {
     int* motion;          // MotionType
     int* gravity;         // AffectedByGravity
     int* world;           // sub-world id
+    float4* slp;          // DeactivationThreshold linear^2, angular^2, DeactivationTime, sleep timer
+    uint32_t* sflags;     // island flags (jh_islands.h); bit 0 = RigidBodyData.IsActive
 };
 
 struct StepTime {
@@ -204,10 +206,7 @@ __device__ __forceinline__ void update_world_inertia(const Bodies& B, int b) {
     }
 }
 
-__global__ void k_unpack_bodies(const jh_body* __restrict__ in, int n, Bodies B) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const jh_body& s = in[b];
+__device__ __forceinline__ void unpack_body(const jh_body& s, int b, const Bodies& B) {
     B.pos[b] = make_float4(s.position[0], s.position[1], s.position[2], 0);
     B.quat[b] = make_float4(s.orientation[0], s.orientation[1], s.orientation[2], s.orientation[3]);
     B.vel[b] = make_float4(s.velocity[0], s.velocity[1], s.velocity[2], 0);
@@ -223,7 +222,20 @@ __global__ void k_unpack_bodies(const jh_body* __restrict__ in, int n, Bodies B)
     B.motion[b] = s.motion_type;
     B.gravity[b] = s.affected_by_gravity;
     B.world[b] = s.world_id;
+    B.slp[b] = make_float4(s.deactivation_threshold_linear_sq, s.deactivation_threshold_angular_sq,
+                           s.deactivation_time, 0.0f);
     update_world_inertia(B, b);
+}
+
+__global__ void k_unpack_bodies(const jh_body* __restrict__ in, int n, Bodies B) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) unpack_body(in[b], b, B);
+}
+
+// Records in[k] for bodies idx[k] (distinct indices).
+__global__ void k_unpack_listed(const jh_body* __restrict__ in, const int* __restrict__ idx, int n, Bodies B) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) unpack_body(in[k], idx[k], B);
 }
 
 __global__ void k_pack_bodies(Bodies B, int n, jh_body* __restrict__ out) {
@@ -248,6 +260,11 @@ __global__ void k_pack_bodies(Bodies B, int n, jh_body* __restrict__ out) {
     o.affected_by_gravity = B.gravity[b];
     o.world_id = B.world[b];
     o.enable_speculative_contacts = m2.y != 0.0f;
+    v = B.slp[b];
+    o.deactivation_threshold_linear_sq = v.x;
+    o.deactivation_threshold_angular_sq = v.y;
+    o.deactivation_time = v.z;
+    o.reserved_body = 0;
 }
 
 // RigidBodyData in the reference's 128-byte FP32 layout (RigidBody.cs:107-160).
@@ -269,7 +286,7 @@ __global__ void k_pack_rigidbodydata(Bodies B, int n, char* __restrict__ out, in
     v = B.iw0[b]; rf[19] = v.x; rf[20] = v.y; rf[21] = v.z; rf[22] = v.w;
     float2 w = B.iw1[b]; rf[23] = w.x; rf[24] = w.y;
     v = B.imv[b]; rf[25] = v.x; rf[26] = v.y; rf[27] = v.z;
-    ((int*)(rf + 28))[0] = (B.motion[b] & 3) | 4;  // MotionType | IsActive
+    ((int*)(rf + 28))[0] = (B.motion[b] & 3) | ((B.sflags[b] & 1u) ? 4 : 0);  // MotionType | IsActive
 }
 
 // The body state of the reference's 128-byte FP32 RigidBodyData records (World.RawData): position,
@@ -513,7 +530,8 @@ __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restr
                                                   const int* __restrict__ left, const int* __restrict__ right,
                                                   const int2* __restrict__ range, const float4* __restrict__ nmin,
                                                   const float4* __restrict__ nmax, uint64_t* __restrict__ pairs,
-                                                  int* __restrict__ pair_count, int cap, int* __restrict__ err) {
+                                                  int* __restrict__ pair_count, int cap, int* __restrict__ err,
+                                                  const uint32_t* __restrict__ sflags) {
     __shared__ uint64_t sbuf[4][kPairBuf];
     __shared__ int scnt[4];
     int wv = threadIdx.x >> 6;
@@ -526,10 +544,12 @@ __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restr
     bool active = i < n && n >= 2;
     uint32_t si = 0;
     int bi = 0;
+    bool awake_i = true;
     if (active) {
         si = order[i];
         bi = shapes[si].body;
         active = motion[bi] == JH_MOTION_DYNAMIC;
+        if (sflags) awake_i = (sflags[bi] & 1u) != 0u;
     }
     if (active) {
         uint64_t w = keys[i] >> 32;
@@ -551,6 +571,7 @@ __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restr
                     int bj = shapes[sj].body;
                     if (bj == bi) continue;
                     if (motion[bj] == JH_MOTION_DYNAMIC && j < i) continue;
+                    if (!awake_i && !(sflags[bj] & 1u)) continue;  // both asleep (World.Detect.cs:158)
                     uint32_t a = min(si, sj), b = max(si, sj);
                     int slot = __hip_atomic_fetch_add(wcnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     wbuf[slot] = (uint64_t(a) << 32) | b;
@@ -1143,7 +1164,7 @@ __device__ __forceinline__ unsigned long long colour_prio(const jh_arbiter& r, i
 struct ColourArgs {
     jh_arbiter* arb;     // jh_arbiter.reserved carries the arbiter's colour from step to step
     const int* a_count;
-    const int* motion;
+    const int* motion;   // the solver's motion types: a sleeping body counts as static
     int* colour;
     unsigned long long* body_max;  // [2][nb]: Jones-Plassmann maxima, double-buffered by round parity
     int nb;
@@ -1176,7 +1197,8 @@ __global__ void __launch_bounds__(256) k_colour(ColourArgs P) {
     int* work[2] = {P.work, P.work + A};
     for (int a = t0; a < A; a += stride) {
         const jh_arbiter& r = P.arb[a];
-        bool active = (r.usage_mask & 15u) != 0;
+        bool active = (r.usage_mask & 15u) != 0 &&
+                      (P.motion[r.body1] != JH_MOTION_STATIC || P.motion[r.body2] != JH_MOTION_STATIC);
         cst(&P.colour[a], active ? -1 : -2);
         cst(&bmax[0][r.body1], 0ull);
         cst(&bmax[0][r.body2], 0ull);
@@ -1639,6 +1661,7 @@ __device__ __forceinline__ bool solve_constraint_batches(const SolveArgs& P, int
             con::CBody A, Bb;
             con_load<Coh>(B, cb, c.body1, A);
             con_load<Coh>(B, cb, c.body2, Bb);
+            if (!A.dyn && !Bb.dyn) continue;  // suspended (World.UpdateConstraintActivation, World.cs:741-760)
             if (phase == 0) con::prepare(c, w, A, Bb, P.ts.inv_dt, P.ts.inv_substep_dt, P.ts.substep_dt);
             else con::iterate(c, w, A, Bb, P.ts.inv_substep_dt);
             for (int q = 0; q < 8; ++q) C.c[i].state[q] = c.state[q];
@@ -2968,11 +2991,17 @@ __global__ void __launch_bounds__(256) k_solve_tiles(SolveArgs P, TileArgs Tl) {
 
 // ---- End of step ----------------------------------------------------------------------------
 // ContactData.UpdatePosition (Contact.cs:238-264, 762-797); flags arbiters that keep contacts.
+// Only the active contacts (memContacts.Active: a non-static body awake; smotion = the solver's
+// motion types, a sleeping body static); the others keep their data.
 __global__ void k_update_contacts(jh_arbiter* __restrict__ arb, const int* __restrict__ a_count, Bodies B,
-                                  uint32_t* __restrict__ keep) {
+                                  uint32_t* __restrict__ keep, const int* __restrict__ smotion) {
     int A = *a_count;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) {
         jh_arbiter& cd = arb[i];
+        if (smotion[cd.body1] == JH_MOTION_STATIC && smotion[cd.body2] == JH_MOTION_STATIC) {
+            keep[i] = 1u;
+            continue;
+        }
         uint32_t usage = cd.usage_mask & 15u;
         usage |= usage << 4;
         vec3 x1 = ld3(B.pos[cd.body1]), x2 = ld3(B.pos[cd.body2]);
@@ -3046,10 +3075,28 @@ __global__ void k_events(const uint64_t* __restrict__ nkey, const int* __restric
     }
 }
 
-// RigidBody.Update (RigidBody.cs:716-764) for dynamic bodies, no motion locks or sleeping.
+// RigidBody.Update (RigidBody.cs:716-764) for the awake bodies (ForeachActiveBody), no motion locks:
+// the sleep timer (a kinematic body rests only with both velocities exactly zero) and the island mark
+// for the next CheckDeactivation (bit 2 of sflags), then the dynamic bodies' damping, forces and
+// gravity.
 __global__ void k_update_bodies(Bodies B, int nb, StepTime ts, float gx, float gy, float gz) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb || B.motion[b] != JH_MOTION_DYNAMIC) return;
+    if (b >= nb) return;
+    uint32_t fl = B.sflags[b];
+    if (!(fl & 1u)) return;
+    int mt = B.motion[b];
+    {
+        float4 sl = B.slp[b];
+        vec3 v = ld3(B.vel[b]), w = ld3(B.angvel[b]);
+        bool kin = mt == JH_MOTION_KINEMATIC;
+        bool resting = kin ? (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f && v.x == 0.0f && v.y == 0.0f && v.z == 0.0f)
+                           : (len_sq(w) < sl.y && len_sq(v) < sl.x);
+        float t = resting ? sl.w + ts.dt : 0.0f;
+        B.slp[b].w = t;
+        bool mark = (kin && !resting) || t < sl.z;
+        B.sflags[b] = (fl & ~4u) | (mark ? 4u : 0u);
+    }
+    if (mt != JH_MOTION_DYNAMIC) return;
     update_world_inertia(B, b);
     float4 m = B.mat[b];
     B.angvel[b] = st3(ld3(B.angvel[b]) * B.mat2[b].x);

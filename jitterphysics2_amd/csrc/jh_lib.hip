@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "jh_kernels.h"
+#include "jh_islands.h"
 #include "jh_lws.h"
 #include "jh_scan.h"
 #include "jh_sort.h"
@@ -87,7 +88,8 @@ enum {
     C_ZOMB = 20,      // arbiters removed at the end of the last step (keys in zkey); kept across steps
     C_EVB = 21,       // BeginCollide events of this step
     C_EVE = 22,       // EndCollide events of this step
-    C_COUNT = 24
+    C_ASLEEP = 23,    // non-static bodies asleep after the last CheckDeactivation
+    C_COUNT = 25      // (the host mirror's last entry is scratch)
 };
 
 // Solver kernels (jh_stats.solver, jh_tuning.solver).
@@ -155,6 +157,13 @@ struct jh_world {
     DBuf<float2> iw1, il1;
     DBuf<int> motion, gravity, world;
     DBuf<jh_body> body_io;
+    // islands and sleeping (jh_islands.h): flags and timers per body, the solver's motion types, and
+    // the union-find of each CheckDeactivation
+    DBuf<float4> slp;
+    DBuf<uint32_t> sflags, hfl, gfl, anew;
+    DBuf<int> smotion, uf, hroot, groot, uidx;
+    int n_asleep = 0;        // non-static bodies asleep after the last check
+    bool isl_dirty = false;  // flags for the next check may be set
     DBuf<char> rbd_io;
     // shapes
     DBuf<ShapeRec> shapes;
@@ -264,6 +273,15 @@ struct jh_world {
         B.pos = pos.p; B.quat = quat.p; B.vel = vel.p; B.angvel = angvel.p; B.dvel = dvel.p; B.dangvel = dangvel.p;
         B.imv = imv.p; B.iw0 = iw0.p; B.iw1 = iw1.p; B.il0 = il0.p; B.il1 = il1.p; B.mat = mat.p; B.mat2 = mat2.p;
         B.force = force.p; B.torque = torque.p; B.motion = motion.p; B.gravity = gravity.p; B.world = world.p;
+        B.slp = slp.p; B.sflags = sflags.p;
+        return B;
+    }
+    // Islands are checked while deactivation is allowed or a body sleeps; otherwise every body is awake.
+    bool islands_on() const { return cfg.allow_deactivation != 0 || n_asleep > 0; }
+    // The bodies as the solver sees them: a sleeping body is static.
+    Bodies solver_bodies() {
+        Bodies B = bodies();
+        if (islands_on()) B.motion = smotion.p;
         return B;
     }
 };
@@ -279,6 +297,11 @@ __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     else if (t < 6) bounds[t] = 0u;
 }
 __global__ void k_set_int(int* p, int v) { *p = v; }
+__global__ void k_solver_motion(int nb, const int* __restrict__ motion, const uint32_t* __restrict__ sflags,
+                                int* __restrict__ smotion) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x)
+        smotion[b] = (motion[b] != JH_MOTION_STATIC && !(sflags[b] & isl::kAwake)) ? JH_MOTION_STATIC : motion[b];
+}
 __global__ void k_solve_keys(const uint64_t* __restrict__ order, const uint64_t* __restrict__ akey,
                              const int* __restrict__ a_count, uint64_t* __restrict__ out) {
     int A = *a_count;
@@ -298,6 +321,49 @@ jh_status check_config(const jh_config* c) {
     if (c->warm_start != JH_WARM_START_PREVIOUS && c->warm_start != JH_WARM_START_LEARNED)
         return fail(JH_ERR_INVALID_ARG, "warm_start must be JH_WARM_START_PREVIOUS or JH_WARM_START_LEARNED");
     return JH_OK;
+}
+
+// RigidBody.DeactivationTime / DeactivationThreshold setters (ArgumentCheck.NonNegative,
+// RigidBody.cs:476-507): finite and not negative.
+jh_status check_sleep_fields(const jh_body& b, int i) {
+    const float v[3] = {b.deactivation_threshold_linear_sq, b.deactivation_threshold_angular_sq, b.deactivation_time};
+    for (float x : v)
+        if (!std::isfinite(x) || x < 0.0f)
+            return fail(JH_ERR_INVALID_ARG, "body %d: deactivation thresholds and time must be finite and >= 0", i);
+    return JH_OK;
+}
+
+// The solver's motion types after a host write (a sleeping body stays static to the solver).
+void refresh_solver_motion(jh_world* w) {
+    if (w->nb == 0) return;
+    k_solver_motion<<<grid_for(w->nb, 256, 1 << 20), 256, 0, w->st>>>(w->nb, w->motion.p, w->sflags.p, w->smotion.p);
+}
+
+// ActivateBodyNextStep (active != 0; wake_static: wakeUpStatic, a static body wakes its partners) or
+// DeactivateBodyNextStep for the listed bodies (device indices).
+void wake_listed(jh_world* w, const int* idx, int n, int active, int wake_static) {
+    hipStream_t st = w->st;
+    isl::k_activate<<<grid_for(n, 256, 1 << 20), 256, 0, st>>>(idx, n, active, wake_static, w->motion.p, w->sflags.p,
+                                                               w->slp.p);
+    if (active && wake_static) {
+        int ncon = int(w->host_con.size());
+        int* cnt = w->cnt.p;
+        int total = w->a_host + int(w->zkey.cap) + ncon;
+        isl::k_wake_partners<<<grid_for(std::max(total, 1)), 256, 0, st>>>(
+            w->arb[w->cur].p, cnt + (w->cur == 0 ? C_A0 : C_A1), w->zkey.p, w->zkey.cap ? cnt + C_ZOMB : nullptr,
+            w->shapes.p, w->con.p, ncon, w->motion.p, w->sflags.p, w->slp.p);
+        isl::k_clear_bits<<<grid_for(std::max(w->nb, 1), 256, 1 << 20), 256, 0, st>>>(w->nb, w->sflags.p, isl::kSwake);
+    }
+    w->isl_dirty = true;
+}
+
+// Islands rebuilt with every body awake and on the active list (new contact cache or constraint set).
+void reset_islands(jh_world* w) {
+    if (w->nb == 0) return;
+    isl::k_upload<<<grid_for(w->nb, 256, 1 << 20), 256, 0, w->st>>>(w->nb, w->motion.p, w->sflags.p, w->slp.p, 1);
+    refresh_solver_motion(w);
+    w->n_asleep = 0;
+    w->isl_dirty = true;
 }
 
 jh_status ensure_pair_buffers(jh_world* w, int n) {
@@ -340,7 +406,7 @@ jh_status broadphase(jh_world* w) {
         k_set_int<<<1, 1, 0, st>>>(cnt + C_PAIRS, 0);
         k_traverse<<<g, 256, 0, st>>>(ns, keys, order, w->shapes.p, w->motion.p, w->lmin.p, w->lmax.p, w->left.p,
                                       w->right.p, w->range.p, w->nmin.p, w->nmax.p, w->pk0.p, cnt + C_PAIRS,
-                                      w->pair_cap, cnt + C_TERR);
+                                      w->pair_cap, cnt + C_TERR, w->islands_on() ? w->sflags.p : nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -394,6 +460,7 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->zkey.ensure(na, true));  // holds the last step's removed keys until k_events has read them
     HIPCHK(w->evb.ensure(np));
     HIPCHK(w->eve.ensure(w->zkey.cap));
+    HIPCHK(w->anew.ensure(na));
     if (w->kind != SOLVER_BARRIER && int(na) > w->soa_cap) {
         size_t cap = na + na / 4;
         HIPCHK(w->soa_bodies.ensure(cap));
@@ -576,7 +643,7 @@ jh_status run_probe(jh_world* w, const SolveArgs& S, const int* a_count, int ama
         HIPCHK(w->pfeat.ensure(size_t(std::max(4 * A, 1)) * lws::kIn));
         HIPCHK(w->ptarg.ensure(size_t(std::max(4 * A, 1)) * 3));
         HIPCHK(w->pvalid.ensure(std::max(4 * A, 1)));
-        lws::k_features<<<ga, 256, 0, st>>>(S.arb, a_count, B, w->pfeat.p, w->pvalid.p);
+        lws::k_features<<<ga, 256, 0, st>>>(S.arb, a_count, S.B, w->pfeat.p, w->pvalid.p);
     }
     std::vector<double> part(size_t(K + 1) * grid);
     for (int m = 0; m < 3; ++m) {
@@ -661,7 +728,7 @@ jh_status colour_arbiters(jh_world* w, int cur, int* a_cur, int amax_, const uin
     ColourArgs P;
     P.arb = w->arb[cur].p;
     P.a_count = a_cur;
-    P.motion = w->motion.p;
+    P.motion = w->islands_on() ? w->smotion.p : w->motion.p;
     P.colour = w->colour.p;
     P.body_max = w->bmax.p;
     P.nb = int(w->bmax.cap / 2);
@@ -691,6 +758,60 @@ jh_status colour_arbiters(jh_world* w, int cur, int* a_cur, int amax_, const uin
     k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
     HIPCHK(hipGetLastError());
     *order_out = order;
+    return JH_OK;
+}
+
+// World.CheckDeactivation on the components of the contact and constraint graph (jh_islands.h).
+// In a step (stabilize false): after the contact cache, arb[cur] holds this step's arbiters, evb the
+// keys of those Detect created, eve the broken ones RemoveBrokenArbiters frees in this step (edges of
+// this check's graph until then).  For World.Stabilize: arb[cur] and the broken arbiters waiting in
+// zkey, no new ones.
+jh_status check_islands(jh_world* w, int cur, int* a_cur, int amax_, bool stabilize) {
+    hipStream_t st = w->st;
+    int* cnt = w->cnt.p;
+    int nb = w->nb;
+    if (nb == 0) return JH_OK;
+    int gb = grid_for(nb, 256, 1 << 20);
+    int ga = grid_for(std::max(amax_, 1));
+    const int* motion = w->motion.p;
+    const jh_arbiter* arb = w->arb[cur].p;
+    int ncon = int(w->host_con.size());
+    isl::k_reset<<<gb, 256, 0, st>>>(nb, w->uf.p, w->hfl.p, w->gfl.p);
+    if (!stabilize) {
+        k_fill_u32<<<ga, 256, 0, st>>>(w->anew.p, size_t(std::max(amax_, 1)), 0u);
+        isl::k_mark_new<<<ga, 256, 0, st>>>(w->evb.p, cnt + C_EVB, w->akey[cur].p, a_cur, w->anew.p);
+    }
+    // H: the graph before this step's new arbiters
+    isl::k_union_arbiters<<<ga, 256, 0, st>>>(arb, a_cur, stabilize ? nullptr : w->anew.p, 0u, motion, w->uf.p);
+    const uint64_t* ekeys = stabilize ? w->zkey.p : w->eve.p;
+    const int* ecount = cnt + (stabilize ? C_ZOMB : C_EVE);
+    isl::k_union_keys<<<grid_for(std::max(int(w->zkey.cap), 1)), 256, 0, st>>>(ekeys, ecount, w->shapes.p, motion,
+                                                                              w->uf.p);
+    if (ncon > 0) isl::k_union_constraints<<<grid_for(ncon), 256, 0, st>>>(w->con.p, ncon, motion, w->uf.p);
+    isl::k_roots<<<gb, 256, 0, st>>>(nb, w->uf.p, w->hroot.p);
+    isl::k_hflags<<<gb, 256, 0, st>>>(nb, motion, w->sflags.p, w->hroot.p, w->hfl.p);
+    // G: with the new arbiters
+    const int* groot = w->hroot.p;
+    if (!stabilize) {
+        isl::k_union_arbiters<<<ga, 256, 0, st>>>(arb, a_cur, w->anew.p, 1u, motion, w->uf.p);
+        isl::k_roots<<<gb, 256, 0, st>>>(nb, w->uf.p, w->groot.p);
+        groot = w->groot.p;
+    }
+    isl::k_h_to_g<<<gb, 256, 0, st>>>(nb, motion, w->hroot.p, groot, w->hfl.p, w->gfl.p);
+    if (!stabilize)
+        isl::k_new_edges<<<ga, 256, 0, st>>>(arb, a_cur, w->anew.p, motion, w->sflags.p, w->hroot.p, w->hfl.p, groot,
+                                             w->gfl.p);
+    k_set_int<<<1, 1, 0, st>>>(cnt + C_ASLEEP, 0);
+    isl::k_decide<<<gb, 256, 0, st>>>(nb, motion, groot, w->gfl.p, w->sflags.p, w->slp.p,
+                                      w->cfg.allow_deactivation, w->smotion.p, cnt + C_ASLEEP);
+    if (!stabilize) {
+        // this step's RemoveBrokenArbiters, seen by the next check
+        isl::k_e_marks<<<grid_for(std::max(int(w->zkey.cap), 1)), 256, 0, st>>>(w->eve.p, cnt + C_EVE, w->shapes.p,
+                                                                               motion, groot, w->gfl.p, w->sflags.p);
+        isl::k_e_apply<<<gb, 256, 0, st>>>(nb, motion, groot, w->gfl.p, w->sflags.p);
+    }
+    HIPCHK(hipGetLastError());
+    w->isl_dirty = true;
     return JH_OK;
 }
 
@@ -780,6 +901,14 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     cur = nxt;
     w->cur = cur;
     a_cur = a_nxt;
+    if (w->islands_on()) {
+        s = check_islands(w, cur, a_cur, amax_, false);
+        if (s != JH_OK) return s;
+    } else if (w->isl_dirty) {
+        isl::k_clear_bits<<<grid_for(std::max(w->nb, 1), 256, 1 << 20), 256, 0, st>>>(
+            w->nb, w->sflags.p, isl::kTouch | isl::kM1 | isl::kN1 | isl::kBnew | isl::kSwake);
+        w->isl_dirty = false;
+    }
     HIPCHK(hipEventRecord(w->ev[3], st));
     // colouring of the arbiters (graph edges between dynamic bodies)
     // the tile solver needs its tiles to fit; otherwise this step runs the dataflow solver
@@ -831,7 +960,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.deg = w->deg.p;
         S.bar = w->bar.p;
         S.spin = w->spin();
-        S.B = B;
+        S.B = w->solver_bodies();
         S.nb = w->nb;
         S.ts = ts;
         S.iterations = cfg.solver_iterations;
@@ -858,7 +987,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             if (s != JH_OK) return s;
         }
         if (cfg.warm_start == JH_WARM_START_LEARNED) {
-            lws::k_infer<<<grid_for(std::max(4 * amax_, 1), 256, 2048), 256, 0, st>>>(S.arb, a_cur, B, w->lws.model());
+            lws::k_infer<<<grid_for(std::max(4 * amax_, 1), 256, 2048), 256, 0, st>>>(S.arb, a_cur, S.B,
+                                                                                     w->lws.model());
             HIPCHK(hipGetLastError());
         }
         void* sargs[] = {&S};
@@ -938,7 +1068,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     // UpdateContacts + removal of arbiters without contacts, then RigidBody.Update
     {
         int g = grid_for(std::max(amax_, 1));
-        k_update_contacts<<<g, 256, 0, st>>>(w->arb[cur].p, a_cur, B, w->flags.p);
+        k_update_contacts<<<g, 256, 0, st>>>(w->arb[cur].p, a_cur, B, w->flags.p, w->solver_bodies().motion);
         scan::exclusive_scan(w->flags.p, w->offs.p, a_cur, std::max(amax_, 1), w->sums.p,
                              cnt + (cur == 0 ? C_A1 : C_A0), st);
         k_compact_arbiters<<<g, 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->flags.p, w->offs.p,
@@ -968,6 +1098,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
                     w->spin());
     }
     // solve-order bookkeeping and stats
+    if (w->islands_on()) w->n_asleep = w->hcnt[C_ASLEEP];
     w->a_host = w->hcnt[w->cur == 0 ? C_A0 : C_A1];
     w->solve_count = w->hcnt[cur == 0 ? C_A0 : C_A1];
     w->solve_batches = w->hcnt[C_NCOL];
@@ -985,6 +1116,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
     w->ev_begin = w->hcnt[C_EVB];
     w->ev_end = w->hcnt[C_EVE];
     S.solver = kind;
+    S.bodies_asleep = w->n_asleep;
     S.steps++;
     float ms[7];
     for (int k = 0; k < 6; ++k) HIPCHK(hipEventElapsedTime(&ms[k], w->ev[k], w->ev[k + 1]));
@@ -1043,6 +1175,7 @@ jh_status jh_config_defaults(jh_config* c) {
     c->enable_auxiliary_contact_points = 1;
     c->persistent_contact_manifold = 1;
     c->speculative_velocity_threshold = 10.0f;
+    c->allow_deactivation = 1;
     return JH_OK;
 }
 
@@ -1150,7 +1283,16 @@ jh_status jh_world_set_config(jh_world* w, const jh_config* c) {
     if (!w) return fail(JH_ERR_INVALID_ARG, "world is NULL");
     jh_status s = check_config(c);
     if (s != JH_OK) return s;
+    // Deactivation switched on while every body is awake: the islands were not tracked, so every
+    // island is put on the active list for the next check (AddToActiveList: none sleeps before then).
+    bool touch_all = !w->cfg.allow_deactivation && c->allow_deactivation && w->n_asleep == 0 && w->nb > 0;
     w->cfg = *c;
+    if (touch_all) {
+        isl::k_set_bits<<<grid_for(w->nb, 256, 1 << 20), 256, 0, w->st>>>(w->nb, w->sflags.p, isl::kTouch);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(w->st));
+        w->isl_dirty = true;
+    }
     return JH_OK;
 }
 
@@ -1164,8 +1306,13 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
         const float* f = (const float*)&b;
         for (int k = 0; k < 36; ++k)
             if (!std::isfinite(f[k])) return fail(JH_ERR_NON_FINITE, "body %d: non-finite field", i);
+        jh_status sv = check_sleep_fields(b, i);
+        if (sv != JH_OK) return sv;
         maxw = std::max(maxw, b.world_id);
     }
+    // a new body count or a motion-type change rebuilds the islands with every body awake
+    bool rebuild = n != w->nb || int(w->host_bodies.size()) != n;
+    for (int i = 0; i < n && !rebuild; ++i) rebuild = w->host_bodies[i].motion_type != bodies[i].motion_type;
     w->nb = n;
     w->max_world = maxw;
     w->n_spec = 0;
@@ -1232,11 +1379,24 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     HIPCHK(w->mhi.ensure(nn));
     HIPCHK(w->deg.ensure(nn));
     HIPCHK(w->vwpack.ensure(2 * nn));
+    HIPCHK(w->slp.ensure(nn));
+    HIPCHK(w->sflags.ensure(nn));
+    HIPCHK(w->hfl.ensure(nn));
+    HIPCHK(w->gfl.ensure(nn));
+    HIPCHK(w->smotion.ensure(nn));
+    HIPCHK(w->uf.ensure(nn));
+    HIPCHK(w->hroot.ensure(nn));
+    HIPCHK(w->groot.ensure(nn));
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(w->body_io.p, bodies, n * sizeof(jh_body), hipMemcpyHostToDevice, w->st));
         k_unpack_bodies<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(w->body_io.p, n, w->bodies());
+        isl::k_upload<<<grid_for(n, 256, 1 << 20), 256, 0, w->st>>>(n, w->motion.p, w->sflags.p, w->slp.p,
+                                                                    rebuild ? 1 : 0);
         HIPCHK(hipGetLastError());
+        refresh_solver_motion(w);
+        w->isl_dirty = true;
     }
+    if (rebuild) w->n_asleep = 0;
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_NB, n);
     w->recolour = true;  // motion types may have changed
     HIPCHK(hipStreamSynchronize(w->st));
@@ -1253,17 +1413,93 @@ jh_status jh_world_get_bodies(jh_world* w, jh_body* bodies, int32_t n) {
     return JH_OK;
 }
 
+jh_status jh_world_update_bodies(jh_world* w, const int32_t* index, const jh_body* records, int32_t count) {
+    if (!w || count < 0 || (count > 0 && (!index || !records))) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (count == 0) return JH_OK;
+    std::vector<char> seen(w->nb, 0);
+    bool motion_change = false;
+    for (int k = 0; k < count; ++k) {
+        int i = index[k];
+        if (i < 0 || i >= w->nb) return fail(JH_ERR_INVALID_ARG, "record %d: body index %d out of range", k, i);
+        if (seen[i]) return fail(JH_ERR_INVALID_ARG, "record %d: body %d listed twice", k, i);
+        seen[i] = 1;
+        const jh_body& b = records[k];
+        if (b.motion_type < 0 || b.motion_type > 2) return fail(JH_ERR_INVALID_ARG, "record %d: bad motion_type", k);
+        if (b.world_id != w->host_bodies[i].world_id)
+            return fail(JH_ERR_UNSUPPORTED, "record %d: a world_id change needs jh_world_set_bodies", k);
+        if ((b.enable_speculative_contacts != 0) != (w->host_bodies[i].enable_speculative_contacts != 0))
+            return fail(JH_ERR_UNSUPPORTED, "record %d: EnableSpeculativeContacts changes need jh_world_set_bodies", k);
+        const float* f = (const float*)&b;
+        for (int q = 0; q < 36; ++q)
+            if (!std::isfinite(f[q])) return fail(JH_ERR_NON_FINITE, "record %d: non-finite field", k);
+        jh_status sv = check_sleep_fields(b, k);
+        if (sv != JH_OK) return sv;
+        motion_change = motion_change || b.motion_type != w->host_bodies[i].motion_type;
+    }
+    if (motion_change) {
+        // a motion-type change rebuilds the islands and the world's bookkeeping: the whole set
+        std::vector<jh_body> all(w->nb);
+        HIPCHK(w->body_io.ensure(std::max(w->nb, 1)));
+        jh_status s = jh_world_get_bodies(w, all.data(), w->nb);
+        if (s != JH_OK) return s;
+        for (int k = 0; k < count; ++k) all[index[k]] = records[k];
+        return jh_world_set_bodies(w, all.data(), w->nb);
+    }
+    HIPCHK(w->body_io.ensure(std::max<size_t>(size_t(count), 1)));
+    HIPCHK(w->uidx.ensure(std::max<size_t>(size_t(count), 1)));
+    HIPCHK(hipMemcpyAsync(w->body_io.p, records, size_t(count) * sizeof(jh_body), hipMemcpyHostToDevice, w->st));
+    HIPCHK(hipMemcpyAsync(w->uidx.p, index, size_t(count) * sizeof(int), hipMemcpyHostToDevice, w->st));
+    k_unpack_listed<<<grid_for(count, 256, 1 << 20), 256, 0, w->st>>>(w->body_io.p, w->uidx.p, count, w->bodies());
+    for (int k = 0; k < count; ++k) w->host_bodies[index[k]] = records[k];
+    // each listed body: ActivateBodyNextStep(body, wakeUpStatic = true)
+    wake_listed(w, w->uidx.p, count, 1, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_set_activation_state(jh_world* w, int32_t body, int32_t active) {
+    if (!w) return fail(JH_ERR_INVALID_ARG, "world is NULL");
+    if (body < 0 || body >= w->nb) return fail(JH_ERR_INVALID_ARG, "body index %d out of range", body);
+    HIPCHK(w->uidx.ensure(1));
+    HIPCHK(hipMemcpyAsync(w->uidx.p, &body, sizeof(int), hipMemcpyHostToDevice, w->st));
+    wake_listed(w, w->uidx.p, 1, active ? 1 : 0, 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(w->st));
+    return JH_OK;
+}
+
+jh_status jh_world_get_activation(jh_world* w, uint8_t* active, float* sleep_time, int32_t n) {
+    if (!w || n < 0 || n > w->nb) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    if (n == 0) return JH_OK;
+    std::vector<uint32_t> f(n);
+    std::vector<float4> t(n);
+    HIPCHK(hipMemcpyAsync(f.data(), w->sflags.p, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipMemcpyAsync(t.data(), w->slp.p, size_t(n) * sizeof(float4), hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
+    for (int i = 0; i < n; ++i) {
+        if (active) active[i] = (f[i] & isl::kAwake) ? 1 : 0;
+        if (sleep_time) sleep_time[i] = t[i].w;
+    }
+    return JH_OK;
+}
+
 jh_status jh_world_set_rigidbodydata(jh_world* w, const void* records, int32_t n, int32_t stride) {
     if (!w || (n > 0 && !records) || n < 0 || stride < 128) return fail(JH_ERR_INVALID_ARG, "bad arguments");
     if (n != w->nb) return fail(JH_ERR_INVALID_ARG, "%d records for %d bodies: RawData covers every body", n, w->nb);
     if (n == 0) return JH_OK;
     const char* in = static_cast<const char*>(records);
+    std::vector<uint32_t> sf(n);
+    HIPCHK(hipMemcpyAsync(sf.data(), w->sflags.p, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToHost, w->st));
+    HIPCHK(hipStreamSynchronize(w->st));
     for (int i = 0; i < n; ++i) {
         const char* r = in + size_t(i) * stride;
         int32_t flags;
         memcpy(&flags, r + 120, 4);
         if ((flags & 3) != w->host_bodies[i].motion_type)
             return fail(JH_ERR_UNSUPPORTED, "body %d: the motion type differs from the body's; set it with jh_world_set_bodies", i);
+        if (((flags & 4) != 0) != ((sf[i] & isl::kAwake) != 0))
+            return fail(JH_ERR_UNSUPPORTED, "body %d: IsActive differs from the body's; use jh_world_set_activation_state", i);
         if (flags & 0x3f8)
             return fail(JH_ERR_UNSUPPORTED, "body %d: gyroscopic integration and motion locks are not on the GPU path", i);
         float f[28];
@@ -1341,6 +1577,8 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_ZOMB, 0);
     w->a_host = 0;
     w->cur = 0;
+    reset_islands(w);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(w->st));
     return JH_OK;
 }
@@ -1372,6 +1610,10 @@ jh_status jh_world_stabilize(jh_world* w, float dt, int32_t iterations, int32_t 
     int* a_cur = cnt + (cur == 0 ? C_A0 : C_A1);
     int amax_ = w->a_host;
     k_set_int<<<1, 1, 0, st>>>(cnt + C_CONTACTS, 0);
+    if (w->islands_on()) {  // StabilizeInternal starts with CheckDeactivation (World.Step.cs:362)
+        s = check_islands(w, cur, a_cur, amax_, true);
+        if (s != JH_OK) return s;
+    }
     const uint64_t* order = nullptr;
     s = colour_arbiters(w, cur, a_cur, amax_, &order);
     if (s != JH_OK) return s;
@@ -1383,7 +1625,7 @@ jh_status jh_world_stabilize(jh_world* w, float dt, int32_t iterations, int32_t 
     S.ncolours = cnt + C_NCOL;
     S.bar = w->bar.p;
     S.spin = w->spin();
-    S.B = w->bodies();
+    S.B = w->solver_bodies();
     S.nb = w->nb;
     S.ts = ts;
     S.iterations = iterations;
@@ -1412,6 +1654,7 @@ jh_status jh_world_stabilize(jh_world* w, float dt, int32_t iterations, int32_t 
                     w->spin());
     }
     // jh_world_get_solve_order now describes this solve
+    if (w->islands_on()) w->n_asleep = w->hcnt[C_ASLEEP];
     w->solve_count = w->hcnt[cur == 0 ? C_A0 : C_A1];
     w->solve_batches = w->hcnt[C_NCOL];
     w->have_step = true;
@@ -1525,6 +1768,8 @@ jh_status jh_world_set_constraints(jh_world* w, const jh_constraint* in, int32_t
     w->con_dirty = true;
     jh_status s = colour_constraints(w);
     if (s != JH_OK) return s;
+    reset_islands(w);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(w->st));
     return JH_OK;
 }
@@ -1632,6 +1877,8 @@ jh_status jh_world_set_arbiters(jh_world* w, const jh_arbiter* in, int32_t n) {
     }
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + (w->cur == 0 ? C_A0 : C_A1), n);
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_ZOMB, 0);  // the restored cache has no pending removals
+    reset_islands(w);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(w->st));
     w->a_host = n;
     w->recolour = true;  // colours in uploaded arbiters are not trusted

@@ -169,8 +169,15 @@ __host__ __device__ __forceinline__ float impulse_scale(const float* f) {
     return fabsf(f[13]) * (fabsf(f[9]) + fabsf(f[10]) + fabsf(f[11]) + fabsf(f[22]) + 0.05f);
 }
 
-// Raw features of every slot (row = 4 * arbiter + slot); valid[row] = slot in use (other rows of
-// `out` are left untouched).
+// Whether the solver visits the arbiter: a body of it is non-static to the solver (B.motion holds
+// the solver's motion types, a sleeping body static).
+template <class BodiesT>
+__device__ __forceinline__ bool arbiter_solved(const jh_arbiter& a, const BodiesT& B) {
+    return B.motion[a.body1] != JH_MOTION_STATIC || B.motion[a.body2] != JH_MOTION_STATIC;
+}
+
+// Raw features of every solved slot (row = 4 * arbiter + slot); valid[row] = slot in use (other rows
+// of `out` are left untouched).
 template <class BodiesT>
 __global__ void __launch_bounds__(256) k_features(const jh_arbiter* __restrict__ arb, const int* __restrict__ a_count,
                                                   BodiesT B, float* __restrict__ out, uint8_t* __restrict__ valid) {
@@ -178,7 +185,7 @@ __global__ void __launch_bounds__(256) k_features(const jh_arbiter* __restrict__
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 4 * n; i += gridDim.x * blockDim.x) {
         const jh_arbiter& a = arb[i >> 2];
         int s = i & 3;
-        bool used = (a.usage_mask & (1u << s)) != 0;
+        bool used = (a.usage_mask & (1u << s)) != 0 && arbiter_solved(a, B);
         valid[i] = used ? 1 : 0;
         if (!used) continue;
         float f[kIn];
@@ -231,7 +238,7 @@ __global__ void __launch_bounds__(256) k_infer(jh_arbiter* __restrict__ arb, con
         if (i < n4) {
             const jh_arbiter& a = arb[i >> 2];
             int s = i & 3;
-            use = (a.usage_mask & (1u << s)) && !(a.contacts[s].flag & 4u);
+            use = (a.usage_mask & (1u << s)) && !(a.contacts[s].flag & 4u) && arbiter_solved(a, B);
             float f[kIn];
             if (use) {
                 contact_features(a, s, B, f);

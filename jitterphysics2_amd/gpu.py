@@ -47,6 +47,27 @@ class GpuWorld:
         check(self.lib, self.lib.jh_world_set_bodies(self.handle, ptr(b), len(b)), "jh_world_set_bodies")
         self.n_bodies = len(b)
 
+    def update_bodies(self, index, records) -> None:
+        """Replace the records of the listed bodies (jh_world_update_bodies): RigidBody property
+        setters on those bodies, each woken for the next step."""
+        idx = np.ascontiguousarray(index, np.int32).reshape(-1)
+        rec = np.ascontiguousarray(records, BODY_DTYPE).reshape(-1)
+        check(self.lib, self.lib.jh_world_update_bodies(self.handle, ptr(idx), ptr(rec), len(idx)),
+              "jh_world_update_bodies")
+
+    def activation(self) -> tuple[np.ndarray, np.ndarray]:
+        """(RigidBody.IsActive as bool, sleep timer as float32) per body (jh_world_get_activation)."""
+        act = np.zeros(self.n_bodies, np.uint8)
+        sl = np.zeros(self.n_bodies, np.float32)
+        check(self.lib, self.lib.jh_world_get_activation(self.handle, ptr(act), ptr(sl), self.n_bodies),
+              "jh_world_get_activation")
+        return act.astype(bool), sl
+
+    def set_activation_state(self, body: int, active: bool) -> None:
+        """RigidBody.SetActivationState (jh_world_set_activation_state), effective at the next step."""
+        check(self.lib, self.lib.jh_world_set_activation_state(self.handle, int(body), 1 if active else 0),
+              "jh_world_set_activation_state")
+
     def set_shapes(self, shapes: np.ndarray, verts: np.ndarray | None = None) -> None:
         s = np.ascontiguousarray(shapes)
         v = None if verts is None else np.ascontiguousarray(verts, np.float32).reshape(-1)

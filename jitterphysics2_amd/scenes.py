@@ -140,6 +140,9 @@ class SceneBuilder:
         b["motion_type"] = motion
         b["affected_by_gravity"] = 1
         b["world_id"] = world
+        b["deactivation_threshold_linear_sq"] = f32(0.1)  # RigidBody.cs:398-400 defaults
+        b["deactivation_threshold_angular_sq"] = f32(0.1)
+        b["deactivation_time"] = f32(1.0)
         self.bodies.append(b)
         return len(self.bodies) - 1
 
@@ -254,6 +257,7 @@ def pyramid(rows: int = 20, seed: int = 0x4A2D0001, jitter: float = 0.01):
             b = sb.body((x, 0.5 + i, z), friction=0.6)
             sb.box(b, (1.0, 1.0, 1.0))
     cfg = default_config()
+    cfg["allow_deactivation"] = 0  # the configurations run without sleeping
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 8
     cfg["relaxation_iterations"] = 4
@@ -313,6 +317,7 @@ def heap(n_bodies: int = 32768, seed: int = 0x4A2D0002, interior: float = 60.0, 
                 count += 1
         layer += 1
     cfg = default_config()
+    cfg["allow_deactivation"] = 0  # the configurations run without sleeping
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 10
     cfg["relaxation_iterations"] = 4
@@ -345,6 +350,7 @@ def batched_worlds(n_worlds: int = 4096, seed: int = 0x4A2D0003, min_bodies: int
             else:
                 sb.box(b, [2.0 * rng.uniform(0.2, 0.8) for _ in range(3)])
     cfg = default_config()
+    cfg["allow_deactivation"] = 0  # the configurations run without sleeping
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 10
     cfg["relaxation_iterations"] = 4
@@ -446,6 +452,7 @@ def full_form(n_bodies: int = 262144, cells: int = 512, worlds: int = 64, seed: 
             else:
                 _random_shape(sb, rng, b, (SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE)[k:k + 1])
     cfg = default_config()
+    cfg["allow_deactivation"] = 0  # the configurations run without sleeping
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 10
     cfg["relaxation_iterations"] = 4
@@ -499,6 +506,7 @@ def bullets(n_bodies: int = 512, seed: int = 0x4A2D0006, speed=(15.0, 60.0), spe
             _random_shape(sb, rng, b, (SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE)[k:k + 1])
         sb.bodies[b]["enable_speculative_contacts"] = 1 if rng.uniform() < speculative else 0
     cfg = default_config()
+    cfg["allow_deactivation"] = 0  # the configurations run without sleeping
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 10
     cfg["relaxation_iterations"] = 4
@@ -657,6 +665,7 @@ def joints(n_groups: int = 24, seed: int = 0x4A2D0007, links: int = 8):
             pb = bodies["position"][b].astype(np.float32)
             recs.append(C.distance_limit(bodies, a, b, pa, pb, (-1.0, 0.5)))
     cfg = default_config()
+    cfg["allow_deactivation"] = 0  # the configurations run without sleeping
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 10
     cfg["relaxation_iterations"] = 4

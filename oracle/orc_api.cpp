@@ -31,36 +31,62 @@ template <class R> void set_config(World<R>& w, const jh_config* c) {
     w.aux_contacts = c->enable_auxiliary_contact_points != 0;
     w.persistent = c->persistent_contact_manifold != 0;
     w.triangle_edge_filter = c->narrowphase_filter == JH_FILTER_TRIANGLE_EDGE;
+    w.allow_deactivation = c->allow_deactivation != 0;
 }
 
 template <class R> V3<R> vin(const float* f) { return {R(f[0]), R(f[1]), R(f[2])}; }
 template <class R> void vout(V3<R> v, float* f) { f[0] = float(v.x); f[1] = float(v.y); f[2] = float(v.z); }
 
+template <class R> void body_in(Body<R>& o, const jh_body& r) {
+    o.pos = vin<R>(r.position);
+    o.q = {R(r.orientation[0]), R(r.orientation[1]), R(r.orientation[2]), R(r.orientation[3])};
+    o.vel = vin<R>(r.velocity);
+    o.angvel = vin<R>(r.angular_velocity);
+    o.dvel = vin<R>(r.delta_velocity);
+    o.dangvel = vin<R>(r.delta_angular_velocity);
+    const float* I = r.inverse_inertia;
+    o.inv_inertia = {R(I[0]), R(I[1]), R(I[2]), R(I[3]), R(I[4]), R(I[5])};
+    o.inv_mass = R(r.inverse_mass);
+    o.friction = R(r.friction);
+    o.restitution = R(r.restitution);
+    o.lin_damp = R(r.linear_damping_multiplier);
+    o.ang_damp = R(r.angular_damping_multiplier);
+    o.force = vin<R>(r.force);
+    o.torque = vin<R>(r.torque);
+    o.motion = r.motion_type;
+    o.gravity = r.affected_by_gravity != 0;
+    o.world = r.world_id;
+    o.speculative = r.enable_speculative_contacts != 0;
+    o.sleep_lin_sq = R(r.deactivation_threshold_linear_sq);
+    o.sleep_ang_sq = R(r.deactivation_threshold_angular_sq);
+    o.deact_time = R(r.deactivation_time);
+    o.update_world_inertia();
+}
+// Replacing the records is every property setter at once: a new body count or a motion-type change
+// rebuilds the islands with every body awake (World.CreateRigidBody); otherwise each body is
+// ActivateBodyNextStep(body, wakeUpStatic = true) as the Position setter does (RigidBody.cs:593).
 template <class R> void set_bodies(World<R>& w, const jh_body* b, int n) {
+    bool rebuild = (int)w.bodies.size() != n;
+    for (int i = 0; i < n && !rebuild; ++i) rebuild = w.bodies[i].motion != b[i].motion_type;
     w.bodies.resize(n);
-    for (int i = 0; i < n; ++i) {
-        Body<R>& o = w.bodies[i];
-        o.pos = vin<R>(b[i].position);
-        o.q = {R(b[i].orientation[0]), R(b[i].orientation[1]), R(b[i].orientation[2]), R(b[i].orientation[3])};
-        o.vel = vin<R>(b[i].velocity);
-        o.angvel = vin<R>(b[i].angular_velocity);
-        o.dvel = vin<R>(b[i].delta_velocity);
-        o.dangvel = vin<R>(b[i].delta_angular_velocity);
-        const float* I = b[i].inverse_inertia;
-        o.inv_inertia = {R(I[0]), R(I[1]), R(I[2]), R(I[3]), R(I[4]), R(I[5])};
-        o.inv_mass = R(b[i].inverse_mass);
-        o.friction = R(b[i].friction);
-        o.restitution = R(b[i].restitution);
-        o.lin_damp = R(b[i].linear_damping_multiplier);
-        o.ang_damp = R(b[i].angular_damping_multiplier);
-        o.force = vin<R>(b[i].force);
-        o.torque = vin<R>(b[i].torque);
-        o.motion = b[i].motion_type;
-        o.gravity = b[i].affected_by_gravity != 0;
-        o.world = b[i].world_id;
-        o.speculative = b[i].enable_speculative_contacts != 0;
-        o.update_world_inertia();
+    for (int i = 0; i < n; ++i) body_in(w.bodies[i], b[i]);
+    if (rebuild) w.islands_reset();
+    else
+        for (int i = 0; i < n; ++i) w.activate_body_next_step(i, true);
+}
+// The listed bodies' records replaced, the same rules per listed body; -1 for an index out of range.
+template <class R> int update_bodies(World<R>& w, const int* idx, const jh_body* b, int count) {
+    for (int k = 0; k < count; ++k)
+        if (idx[k] < 0 || idx[k] >= (int)w.bodies.size()) return -1;
+    bool rebuild = false;
+    for (int k = 0; k < count; ++k) {
+        rebuild = rebuild || w.bodies[idx[k]].motion != b[k].motion_type;
+        body_in(w.bodies[idx[k]], b[k]);
     }
+    if (rebuild) w.islands_reset();
+    else
+        for (int k = 0; k < count; ++k) w.activate_body_next_step(idx[k], true);
+    return 0;
 }
 
 template <class R> void get_bodies(const World<R>& w, jh_body* b, int n) {
@@ -104,6 +130,7 @@ template <class R> void set_shapes(World<R>& w, const jh_shape* s, int n, const 
     }
     w.arbiters.clear();
     w.broken.clear();
+    w.islands_reset();
 }
 
 template <class R> void contact_out(const Contact<R>& c, jh_contact& o) {
@@ -174,6 +201,7 @@ template <class R> void set_arbiters(World<R>& w, const jh_arbiter* in, int n) {
         for (int k = 0; k < 4; ++k) contact_in(a.c[k], in[i].contacts[k]);
         w.arbiters[std::make_pair(a.key1, a.key2)] = a;
     }
+    w.islands_reset();
 }
 
 template <class R> int get_manifolds(const World<R>& w, jh_manifold* out, int cap) {
@@ -211,6 +239,7 @@ template <class R> void set_constraints(World<R>& w, const jh_constraint* in, in
         for (int k = 0; k < 8; ++k) c.s[k] = R(in[i].state[k]);
     }
     w.con_order.clear();
+    w.islands_reset();
 }
 template <class R> int get_constraints(const World<R>& w, jh_constraint* out, int cap) {
     int n = (int)w.constraints.size();
@@ -374,6 +403,11 @@ int orc_get_arbiters(void* p, jh_arbiter* a, int cap) {
     Handle* h = (Handle*)p;
     return DISPATCH(h, get_arbiters(W, a, cap));
 }
+// Every held arbiter, also those whose contacts all broke and that the next step removes.
+int orc_get_arbiters_all(void* p, jh_arbiter* a, int cap) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, get_arbiters(W, a, cap, true));
+}
 int orc_get_manifolds(void* p, jh_manifold* m, int cap) {
     Handle* h = (Handle*)p;
     return DISPATCH(h, get_manifolds(W, m, cap));
@@ -386,6 +420,44 @@ int orc_compute_pairs(void* p, uint32_t* out, int cap, int brute) {
         int n = (int)pairs.size();
         for (int i = 0; i < n && i < cap; ++i) { out[2 * i] = pairs[i].first; out[2 * i + 1] = pairs[i].second; }
         return n;
+    })());
+}
+// Activation state per body: active[i] = RigidBody.IsActive, sleep[i] = InternalSleepTime.
+int orc_get_activation(void* p, uint8_t* active, float* sleep, int n) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, ([&]() {
+        int m = std::min(n, (int)W.bodies.size());
+        for (int i = 0; i < m; ++i) {
+            if (active) active[i] = W.bodies[i].active ? 1 : 0;
+            if (sleep) sleep[i] = float(W.bodies[i].sleep_time);
+        }
+        return m;
+    })());
+}
+int orc_update_bodies(void* p, const int* idx, const jh_body* b, int count) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, update_bodies(W, idx, b, count));
+}
+// Island of each body: the index of its body with the smallest index (-1 never occurs).
+int orc_get_islands(void* p, int* out, int n) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, ([&]() {
+        int m = std::min(n, (int)W.bodies.size());
+        std::vector<int> lo(W.isl.size(), -1);
+        for (int i = (int)W.bodies.size() - 1; i >= 0; --i) lo[W.bodies[i].island] = i;
+        for (int i = 0; i < m; ++i) out[i] = lo[W.bodies[i].island];
+        return m;
+    })());
+}
+// RigidBody.SetActivationState (RigidBody.cs:1025-1038): active != 0 -> World.ActivateBodyNextStep,
+// else World.DeactivateBodyNextStep (the sleep timer set to infinity).
+int orc_set_activation_state(void* p, int body, int active) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, ([&]() {
+        if (body < 0 || body >= (int)W.bodies.size()) return -1;
+        if (active) W.activate_body_next_step(body);
+        else W.bodies[body].sleep_time = std::numeric_limits<decltype(W.gravity.x)>::infinity();
+        return 0;
     })());
 }
 // One World.Step; order = nkeys (key1, key2) pairs to replay, or NULL for key order.

@@ -36,6 +36,13 @@ template <class R> struct Body {
     bool gravity = true;
     bool speculative = false;  // RigidBody.EnableSpeculativeContacts (RigidBody.cs:877-900)
     int world = 0;
+    // sleeping (RigidBody.cs:315, 398-400; RigidBodyData.IsActive): island, connections to other
+    // non-static bodies (one entry per arbiter or constraint, InternalConnections)
+    bool active = true;
+    R sleep_time = R(0);
+    R sleep_lin_sq = R(0.1), sleep_ang_sq = R(0.1), deact_time = R(1.0);
+    int island = -1;
+    std::vector<int> conn;
 
     // RigidBody.UpdateWorldInertia (RigidBody.cs:903-923), no motion locks.
     void update_world_inertia() {
@@ -120,16 +127,219 @@ template <class R> struct World {
         return ConstraintBody<R>{&x.vel, &x.angvel, x.pos, x.inv_mass_vec, x.q, x.inv_inertia_world,
                                  x.motion == JH_MOTION_DYNAMIC};
     }
+    // the enabled constraints in the solver's active partition (UpdateConstraintActivation)
     template <class F> void for_constraints(F f) {
         if (con_order.empty()) {
             for (auto& c : constraints)
-                if (c.enabled) f(c);
+                if (c.enabled && constraint_active(c)) f(c);
         } else {
             for (int i : con_order)
-                if (constraints[i].enabled) f(constraints[i]);
+                if (constraints[i].enabled && constraint_active(constraints[i])) f(constraints[i]);
         }
     }
     long epa_calls = 0;
+
+    // ---- Islands and sleeping (Collision/IslandHelper.cs, World.cs:696-720, 900-910,
+    // World.Step.cs:1230-1371) ----------------------------------------------------------------
+    // Islands: connected components of non-static bodies through arbiters and constraints, kept
+    // incrementally (merge smaller into larger on a new connection, a two-sided search and split when
+    // one goes).  MarkedAsActive / NeedsUpdate are ORed on merges and copied on splits; the active
+    // set is the PartitionedSet's active partition.
+    struct Island {
+        std::vector<int> bodies;
+        bool marked = true, needs = false, active = true, alive = true;
+    };
+    std::vector<Island> isl;
+    std::vector<int> isl_free;
+    bool allow_deactivation = true;     // World.AllowDeactivation (World.cs:333)
+    std::vector<std::pair<uint32_t, uint32_t>> deferred;  // arbiters created by this step's Detect
+
+    int island_new() {
+        int i;
+        if (!isl_free.empty()) {
+            i = isl_free.back();
+            isl_free.pop_back();
+            isl[i] = Island{};
+        } else {
+            i = int(isl.size());
+            isl.emplace_back();
+        }
+        return i;  // GetFromPool: MarkedAsActive = true, NeedsUpdate = false
+    }
+    void island_free(int i) {
+        isl[i].alive = false;
+        isl[i].bodies.clear();
+        isl_free.push_back(i);
+    }
+    bool is_static(int b) const { return bodies[b].motion == JH_MOTION_STATIC; }
+    // World.AddToActiveList (World.cs:900-910)
+    void add_to_active(int i) {
+        bool was = isl[i].active;
+        isl[i].marked = true;
+        if (!was) isl[i].needs = true;
+        isl[i].active = true;
+    }
+    // IslandHelper.MergeIslands (IslandHelper.cs:236-277): both bodies non-static
+    void merge_islands(int b1, int b2) {
+        int i1 = bodies[b1].island, i2 = bodies[b2].island;
+        if (i1 == i2) return;
+        bool needs = isl[i1].needs || isl[i2].needs;
+        bool marked = isl[i1].marked || isl[i2].marked;
+        int small = isl[i1].bodies.size() > isl[i2].bodies.size() ? i2 : i1;
+        int large = small == i1 ? i2 : i1;
+        for (int b : isl[small].bodies) {
+            bodies[b].island = large;
+            isl[large].bodies.push_back(b);
+        }
+        isl[large].needs = isl[large].needs || needs;
+        isl[large].marked = isl[large].marked || marked;
+        island_free(small);
+    }
+    // IslandHelper.AddConnection (:103-121)
+    void add_connection(int b1, int b2) {
+        bool needs = !isl[bodies[b1].island].active || !isl[bodies[b2].island].active;
+        if (!is_static(b1) && !is_static(b2)) {
+            merge_islands(b1, b2);
+            bodies[b1].conn.push_back(b2);
+            bodies[b2].conn.push_back(b1);
+        }
+        if (needs) {
+            if (!is_static(b1)) isl[bodies[b1].island].needs = true;
+            if (!is_static(b2)) isl[bodies[b2].island].needs = true;
+        }
+    }
+    // IslandHelper.SplitIslands (:143-233): searches from both bodies alternately; if they meet, the
+    // island stays; otherwise the side that ran out first becomes a new island with the flags and the
+    // partition of the source island.
+    void split_islands(int b1, int b2) {
+        int src = bodies[b1].island;
+        std::vector<int> mark(bodies.size(), 0);  // InternalIslandMarker
+        std::vector<int> vl{b1}, vr{b2};
+        size_t hl = 0, hr = 0;  // queue heads (the queues are vl[hl..], vr[hr..])
+        mark[b1] = 1;
+        mark[b2] = 2;
+        while (hl < vl.size() && hr < vr.size()) {
+            int cur = vl[hl++];
+            if (!is_static(cur))
+                for (int c : bodies[cur].conn) {
+                    if (mark[c] == 0) { vl.push_back(c); mark[c] = 1; }
+                    else if (mark[c] == 2) return;
+                }
+            cur = vr[hr++];
+            if (!is_static(cur))
+                for (int c : bodies[cur].conn) {
+                    if (mark[c] == 0) { vr.push_back(c); mark[c] = 2; }
+                    else if (mark[c] == 1) return;
+                }
+        }
+        int ni = island_new();
+        isl[ni].needs = isl[src].needs;
+        isl[ni].marked = isl[src].marked;
+        isl[ni].active = isl[src].active;
+        const std::vector<int>& moved = hl == vl.size() ? vl : vr;
+        for (int b : moved) {
+            auto& sb = isl[src].bodies;
+            sb.erase(std::find(sb.begin(), sb.end(), b));
+            isl[ni].bodies.push_back(b);
+            bodies[b].island = ni;
+        }
+    }
+    // IslandHelper.RemoveConnection (:123-141)
+    void remove_connection(int b1, int b2) {
+        auto rm = [](std::vector<int>& l, int b) {
+            auto it = std::find(l.begin(), l.end(), b);
+            if (it == l.end()) return;
+            *it = l.back();
+            l.pop_back();
+        };
+        rm(bodies[b1].conn, b2);
+        rm(bodies[b2].conn, b1);
+        if (bodies[b1].island == bodies[b2].island) split_islands(b1, b2);
+    }
+    // World.ActivateBodyNextStep (World.cs:696-720); with wake_static a static body's island is put on
+    // the active list and the bodies it touches through arbiters and constraints are woken.
+    void activate_body_next_step(int b, bool wake_static = false) {
+        bodies[b].sleep_time = R(0);
+        if (bodies[b].active) return;
+        if (is_static(b) && !wake_static) return;
+        add_to_active(bodies[b].island);
+        if (is_static(b)) {
+            for (auto& c : constraints)
+                if (c.b1 == b || c.b2 == b) activate_body_next_step(c.b1 == b ? c.b2 : c.b1);
+            for (auto& kv : arbiters)
+                if (kv.second.b1 == b || kv.second.b2 == b)
+                    activate_body_next_step(kv.second.b1 == b ? kv.second.b2 : kv.second.b1);
+        }
+        isl[bodies[b].island].needs = true;
+    }
+    // Islands rebuilt from scratch: every body awake in its own island (IslandHelper.BodyAdded and
+    // AddToActiveList of World.CreateRigidBody), then every constraint and arbiter connected
+    // (ConstraintCreated / ArbiterCreated, each followed by AddToActiveList of both islands).
+    void islands_reset() {
+        isl.clear();
+        isl_free.clear();
+        for (size_t b = 0; b < bodies.size(); ++b) {
+            bodies[b].conn.clear();
+            bodies[b].active = true;
+            // World.CreateRigidBody leaves the timer at zero; MotionType = Static sets it to infinity
+            // (DeactivateBodyNextStep, World.cs:793-800)
+            bodies[b].sleep_time = is_static(int(b)) ? std::numeric_limits<R>::infinity() : R(0);
+            int i = island_new();
+            isl[i].bodies.push_back(int(b));
+            bodies[b].island = i;
+        }
+        for (auto& c : constraints) {
+            add_connection(c.b1, c.b2);
+            add_to_active(bodies[c.b1].island);
+            add_to_active(bodies[c.b2].island);
+        }
+        for (auto& kv : arbiters) {
+            add_connection(kv.second.b1, kv.second.b2);
+            add_to_active(bodies[kv.second.b1].island);
+            add_to_active(bodies[kv.second.b2].island);
+        }
+    }
+    // World.CheckDeactivation (World.Step.cs:1289-1371)
+    void check_deactivation() {
+        for (size_t i = 0; i < isl.size(); ++i) {
+            Island& I = isl[i];
+            if (!I.alive || !I.active) continue;
+            bool deactivate = !I.marked;
+            if (!allow_deactivation) deactivate = false;
+            I.marked = false;
+            bool needs = I.needs;
+            I.needs = false;
+            if (!deactivate && !needs) continue;
+            for (int b : I.bodies) {
+                Body<R>& x = bodies[b];
+                if (x.active != deactivate) {
+                    if (!needs) break;
+                    continue;
+                }
+                if (deactivate) {
+                    x.active = false;  // DeactivateBodyForSleep: velocities are kept
+                } else {
+                    if (x.motion == JH_MOTION_STATIC) continue;
+                    x.active = true;
+                    x.sleep_time = R(0);
+                }
+            }
+            if (deactivate) I.active = false;
+        }
+    }
+    // memContacts' active partition: a contact is active while a non-static body of it is awake
+    // (contacts move with their non-static bodies' activation, World.Step.cs:1245-1264, 1325-1328)
+    bool arbiter_active(const Arbiter<R>& a) const {
+        const Body<R>& x = bodies[a.b1];
+        const Body<R>& y = bodies[a.b2];
+        return (x.motion != JH_MOTION_STATIC && x.active) || (y.motion != JH_MOTION_STATIC && y.active);
+    }
+    // World.UpdateConstraintActivation (World.cs:741-760)
+    bool constraint_active(const Constraint<R>& c) const {
+        const Body<R>& x = bodies[c.b1];
+        const Body<R>& y = bodies[c.b2];
+        return (x.motion == JH_MOTION_DYNAMIC && x.active) || (y.motion == JH_MOTION_DYNAMIC && y.active);
+    }
 
     // RigidBodyShape.UpdateWorldBoundingBox(BroadphaseStepDt) (RigidBodyShape.cs:37-58): bodies with
     // speculative contacts get their box grown by dt * |velocity| on every side.
@@ -160,6 +370,7 @@ template <class R> struct World {
             const Body<R>& ba = bodies[a.body];
             const Body<R>& bb = bodies[b.body];
             if (ba.world != bb.world) return;
+            if (!ba.active && !bb.active) return;  // World.Detect.cs:158
             if (ba.motion != JH_MOTION_DYNAMIC && bb.motion != JH_MOTION_DYNAMIC) return;
             if (disjoint(mn[i], mx[i], mn[j], mx[j])) return;
             out.emplace_back(uint32_t(std::min(i, j)), uint32_t(std::max(i, j)));
@@ -394,6 +605,7 @@ template <class R> struct World {
             init_arbiter(a, k1, k2);
             it = arbiters.emplace(key, a).first;
             ev_begin.push_back(key);  // HandleDeferredArbiters raises BeginCollide (World.Step.cs:881-911)
+            deferred.push_back(key);
         }
         return it->second;
     }
@@ -707,7 +919,7 @@ template <class R> struct World {
         pr.rows = pr.points = 0;
         for (auto& kv : arbiters)
             for (int s = 0; s < 4; ++s) {
-                if (!(kv.second.usage & (1u << s))) continue;
+                if (!(kv.second.usage & (1u << s)) || !arbiter_active(kv.second)) continue;
                 if (pr.features && pr.rows < pr.capacity) contact_features(kv.second, s, pr.features + 24 * pr.rows);
                 if (pr.flags && pr.rows < pr.capacity) pr.flags[pr.rows] = kv.second.c[s].flag;
                 if (pr.rows < pr.capacity) ++pr.rows;
@@ -730,7 +942,7 @@ template <class R> struct World {
                 long row = 0;
                 for (auto& kv : arbiters)
                     for (int s = 0; s < 4; ++s) {
-                        if (!(kv.second.usage & (1u << s))) continue;
+                        if (!(kv.second.usage & (1u << s)) || !arbiter_active(kv.second)) continue;
                         auto it = by_key.find(kv.first);
                         if (row < pr.capacity && it != by_key.end())
                             for (int r = 0; r < 3; ++r) it->second->c[s].acc[r] = R(pr.learned[3 * row + r]);
@@ -738,7 +950,7 @@ template <class R> struct World {
                     }
             }
             for (auto& b : bodies) {
-                if (b.motion != JH_MOTION_DYNAMIC) continue;
+                if (b.motion != JH_MOTION_DYNAMIC || !b.active) continue;
                 b.angvel = b.angvel + b.dangvel;
                 b.vel = b.vel + b.dvel;
             }
@@ -761,7 +973,7 @@ template <class R> struct World {
                 long row = 0;
                 for (auto& kv : arbiters)
                     for (int s = 0; s < 4; ++s) {
-                        if (!(kv.second.usage & (1u << s))) continue;
+                        if (!(kv.second.usage & (1u << s)) || !arbiter_active(kv.second)) continue;
                         if (row < pr.capacity) {
                             const Contact<R>& c = by_key.at(kv.first)->c[s];
                             for (int r = 0; r < 3; ++r) pr.targets[3 * row + r] = float(c.acc[r]);
@@ -783,14 +995,16 @@ template <class R> struct World {
     void stabilize(R dt, int iters, int relax, const std::vector<std::pair<uint32_t, uint32_t>>* order) {
         if (dt < std::numeric_limits<R>::denorm_min()) return;
         StepTiming<R> ts(dt, substeps);
+        check_deactivation();
         std::vector<Arbiter<R>*> list;
         if (order) {
             for (auto& k : *order) {
                 auto it = arbiters.find(k);
-                if (it != arbiters.end()) list.push_back(&it->second);
+                if (it != arbiters.end() && arbiter_active(it->second)) list.push_back(&it->second);
             }
         } else {
-            for (auto& kv : arbiters) list.push_back(&kv.second);
+            for (auto& kv : arbiters)
+                if (arbiter_active(kv.second)) list.push_back(&kv.second);
         }
         for (int s = 0; s < ts.substeps; ++s) {
             for (auto* a : list)
@@ -826,16 +1040,27 @@ template <class R> struct World {
         last_manifolds.clear();
         epa_calls = 0;
         last_pairs = compute_pairs();
+        deferred.clear();
         detect_all(last_pairs, ts.dt);
+        // HandleDeferredArbiters (World.Step.cs:881-911): ArbiterCreated, AddToActiveList of both islands
+        for (auto& k : deferred) {
+            const Arbiter<R>& a = arbiters.at(k);
+            add_connection(a.b1, a.b2);
+            add_to_active(bodies[a.b1].island);
+            add_to_active(bodies[a.b2].island);
+        }
+        deferred.clear();
+        check_deactivation();
         // Solve order: given, else key order.
         std::vector<Arbiter<R>*> list;
         if (order) {
             for (auto& k : *order) {
                 auto it = arbiters.find(k);
-                if (it != arbiters.end()) list.push_back(&it->second);
+                if (it != arbiters.end() && arbiter_active(it->second)) list.push_back(&it->second);
             }
         } else {
-            for (auto& kv : arbiters) list.push_back(&kv.second);
+            for (auto& kv : arbiters)
+                if (arbiter_active(kv.second)) list.push_back(&kv.second);
         }
         if (probe) run_probe(list, ts, *probe);
         for (int s = 0; s < ts.substeps; ++s) {
@@ -843,7 +1068,7 @@ template <class R> struct World {
 #pragma omp parallel for schedule(static) if (bodies.size() > 4096)
             for (long i = 0; i < long(bodies.size()); ++i) {
                 Body<R>& b = bodies[i];
-                if (b.motion != JH_MOTION_DYNAMIC) continue;
+                if (b.motion != JH_MOTION_DYNAMIC || !b.active) continue;
                 b.angvel = b.angvel + b.dangvel;
                 b.vel = b.vel + b.dvel;
             }
@@ -868,7 +1093,7 @@ template <class R> struct World {
 #pragma omp parallel for schedule(static) if (bodies.size() > 4096)
             for (long i = 0; i < long(bodies.size()); ++i) {
                 Body<R>& b = bodies[i];
-                if (b.motion == JH_MOTION_STATIC) continue;
+                if (b.motion == JH_MOTION_STATIC || !b.active) continue;
                 b.pos = b.pos + b.vel * ts.substep_dt;
                 Q4<R> rq = rotation_quaternion(b.angvel, ts.substep_dt);
                 b.q = qnormalize(qmul(rq, b.q));
@@ -884,6 +1109,10 @@ template <class R> struct World {
         for (auto& k : broken) {
             auto it = arbiters.find(k);
             if (it == arbiters.end() || (it->second.usage & 15u) != 0) continue;
+            int b1 = it->second.b1, b2 = it->second.b2;
+            add_to_active(bodies[b1].island);
+            add_to_active(bodies[b2].island);
+            remove_connection(b1, b2);  // IslandHelper.ArbiterRemoved
             arbiters.erase(it);
             ev_end.push_back(k);
         }
@@ -891,19 +1120,30 @@ template <class R> struct World {
         // UpdateContacts (World.Step.cs:862-878): arbiters left without intact contacts are queued for
         // removal; until then they carry no solver work.
         {
-            std::vector<Arbiter<R>*> all;
+            std::vector<Arbiter<R>*> all;  // memContacts.Active
             all.reserve(arbiters.size());
-            for (auto& kv : arbiters) all.push_back(&kv.second);
+            for (auto& kv : arbiters)
+                if (arbiter_active(kv.second)) all.push_back(&kv.second);
 #pragma omp parallel for schedule(static) if (all.size() > 4096)
             for (long i = 0; i < long(all.size()); ++i) arbiter_update_position(*all[i]);
-            for (auto& kv : arbiters)
-                if ((kv.second.usage & 15u) == 0) broken.push_back(kv.first);
+            for (auto* a : all)
+                if ((a->usage & 15u) == 0) broken.push_back({a->key1, a->key2});
         }
         std::sort(ev_begin.begin(), ev_begin.end());
-        // ForeachActiveBody -> RigidBody.Update (RigidBody.cs:716-764)
+        // ForeachActiveBody -> RigidBody.Update (RigidBody.cs:716-764): the sleep timer of every active
+        // body (its island stays awake while the timer is short or a kinematic body moves), then the
+        // dynamic bodies' velocity update
+        std::vector<char> keep_awake(bodies.size(), 0);
 #pragma omp parallel for schedule(static) if (bodies.size() > 4096)
         for (long i = 0; i < long(bodies.size()); ++i) {
             Body<R>& b = bodies[i];
+            if (!b.active) continue;
+            bool kin = b.motion == JH_MOTION_KINEMATIC;
+            auto zero = [](V3<R> v) { return v.x == R(0) && v.y == R(0) && v.z == R(0); };  // JVector ==
+            bool resting = kin ? (zero(b.angvel) && zero(b.vel))
+                               : (len_sq(b.angvel) < b.sleep_ang_sq && len_sq(b.vel) < b.sleep_lin_sq);
+            b.sleep_time = resting ? b.sleep_time + ts.dt : R(0);
+            if ((kin && !resting) || b.sleep_time < b.deact_time) keep_awake[i] = 1;
             if (b.motion != JH_MOTION_DYNAMIC) continue;
             b.update_world_inertia();
             b.angvel = b.angvel * b.ang_damp;
@@ -914,6 +1154,8 @@ template <class R> struct World {
             b.force = {R(0), R(0), R(0)};
             b.torque = {R(0), R(0), R(0)};
         }
+        for (size_t i = 0; i < bodies.size(); ++i)
+            if (keep_awake[i]) isl[bodies[i].island].marked = true;  // InternalIsland.MarkedAsActive
         // DynamicTree.Update(multiThread, StepDt) (World.Step.cs:274-276): the next broadphase sweeps
         // speculative bodies' boxes over this step's length.
         broadphase_dt = ts.dt;

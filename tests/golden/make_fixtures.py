@@ -20,6 +20,7 @@ import os
 import sys
 
 import numpy as np
+import numpy.lib.recfunctions as rfn
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -29,11 +30,19 @@ if ROOT not in sys.path:
 from jitterphysics2_amd import scenes  # noqa: E402
 
 
+# Body fields outside the hash: the sleep parameters, which do not act in scenes that run without
+# deactivation (the config's allow_deactivation is hashed).
+SLEEP_FIELDS = ("deactivation_threshold_linear_sq", "deactivation_threshold_angular_sq", "deactivation_time",
+                "reserved_body")
+
+
 def scene_hash(scene) -> str:
     bodies, shapes, verts, cfg, dt = scene
     h = hashlib.sha256()
     v = np.zeros(0, np.float32) if verts is None else np.asarray(verts, np.float32)
-    for a in (bodies, shapes, v, cfg, np.float32(dt)):
+    b = np.ascontiguousarray(bodies)
+    b = rfn.repack_fields(b[[n for n in b.dtype.names if n not in SLEEP_FIELDS]])
+    for a in (b, shapes, v, cfg, np.float32(dt)):
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
 

@@ -48,6 +48,9 @@ def lib() -> ctypes.CDLL:
             "orc_set_constraints": [vp, vp, i32], "orc_get_constraints": [vp, vp, i32],
             "orc_set_constraint_order": [vp, vp, i32],
             "orc_stabilize": [vp, f, i32, i32, vp, i32],
+            "orc_get_activation": [vp, vp, vp, i32], "orc_set_activation_state": [vp, i32, i32],
+            "orc_update_bodies": [vp, vp, vp, i32], "orc_get_islands": [vp, vp, i32],
+            "orc_get_arbiters_all": [vp, vp, i32],
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32
@@ -87,9 +90,23 @@ class OracleWorld:
 
     def set_bodies(self, bodies) -> None:
         """Replace the body records (positions, velocities, forces ...): RigidBody property setters
-        and AddForce between steps."""
+        and AddForce between steps.  Each body is woken for the next step (a static one wakes the
+        bodies it touches); a motion-type change rebuilds the islands with every body awake."""
         assert len(bodies) == self.n
         self.L.orc_set_bodies(self.h, ptr(np.ascontiguousarray(bodies)), len(bodies))
+
+    def update_bodies(self, index, records) -> None:
+        """Replace the records of the listed bodies only, with set_bodies' rules for each of them."""
+        idx = np.ascontiguousarray(index, np.int32).reshape(-1)
+        rec = np.ascontiguousarray(records, BODY_DTYPE).reshape(-1)
+        assert len(idx) == len(rec)
+        assert self.L.orc_update_bodies(self.h, ptr(idx), ptr(rec), len(idx)) == 0
+
+    def islands(self) -> np.ndarray:
+        """Island of each body, named by its smallest body index."""
+        out = np.zeros(self.n, np.int32)
+        self.L.orc_get_islands(self.h, ptr(out), self.n)
+        return out
 
     def __del__(self):
         try:
@@ -132,6 +149,17 @@ class OracleWorld:
         return {"residual": res, "points": int(counts[1]), "features": feats[:rows], "targets": targs[:rows],
                 "flags": flags[:rows]}
 
+    def activation(self) -> tuple[np.ndarray, np.ndarray]:
+        """(RigidBody.IsActive as bool, InternalSleepTime as float32) per body."""
+        act = np.zeros(self.n, np.uint8)
+        sl = np.zeros(self.n, np.float32)
+        self.L.orc_get_activation(self.h, ptr(act), ptr(sl), self.n)
+        return act.astype(bool), sl
+
+    def set_activation_state(self, body: int, active: bool) -> None:
+        """RigidBody.SetActivationState: takes effect in the next step."""
+        assert self.L.orc_set_activation_state(self.h, int(body), 1 if active else 0) == 0
+
     def bodies(self) -> np.ndarray:
         out = np.zeros(self.n, BODY_DTYPE)
         self.L.orc_get_bodies(self.h, ptr(out), self.n)
@@ -152,6 +180,13 @@ class OracleWorld:
         n = self.L.orc_get_arbiters(self.h, None, 0)
         out = np.zeros(n, ARBITER_DTYPE)
         self.L.orc_get_arbiters(self.h, ptr(out), n)
+        return out
+
+    def all_arbiters(self) -> np.ndarray:
+        """Every held arbiter, also those whose contacts all broke (removed at the next step)."""
+        n = self.L.orc_get_arbiters_all(self.h, None, 0)
+        out = np.zeros(n, ARBITER_DTYPE)
+        self.L.orc_get_arbiters_all(self.h, ptr(out), n)
         return out
 
     def set_arbiters(self, arbs: np.ndarray) -> None:

@@ -67,6 +67,7 @@ def _resting_box(steps=150):
     cfg = scenes.default_config()
     cfg["substep_count"] = 4
     cfg["solver_iterations"] = 8
+    cfg["allow_deactivation"] = 0  # the box keeps being solved
     ow = OracleWorld(bodies, shapes, verts, cfg)
     for _ in range(steps):
         ow.step(0.01)
