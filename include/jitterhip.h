@@ -124,8 +124,20 @@ typedef struct jh_body {
     float deactivation_threshold_linear_sq;
     float deactivation_threshold_angular_sq;
     float deactivation_time;
-    int32_t reserved_body;
+    /* The complement of RigidBody.AllowedMotion (RigidBody.cs:571-594, RigidBodyData.Flags bits 4..9):
+     * JH_LOCK_* bits of the world axes the body may not move along or rotate about; 0 = free.
+     * Locked components of the velocities are cleared when the record is written; the solver sees a
+     * zero inverse mass along locked translations and the Schur complement of the inverse inertia for
+     * locked rotations (RigidBody.UpdateWorldInertia RigidBody.cs:903-923).  A change of a body's locks
+     * also clears its contact cache and the warm start of its constraints, as the setter does. */
+    int32_t locked_axes;
 } jh_body;
+
+/* MotionAxes (RigidBody.cs:26-53) as lock bits of jh_body.locked_axes. */
+enum {
+    JH_LOCK_LINEAR_X = 1, JH_LOCK_LINEAR_Y = 2, JH_LOCK_LINEAR_Z = 4,
+    JH_LOCK_ANGULAR_X = 8, JH_LOCK_ANGULAR_Y = 16, JH_LOCK_ANGULAR_Z = 32
+};
 
 /* One shape attached to a body.  shape_id is Shape.ShapeId (Collision/Shapes/Shape.cs:28); shapes
  * must be passed in strictly ascending shape_id order so that array order equals key order. */

@@ -10,6 +10,20 @@
 namespace jh {
 namespace con {
 
+// Leading entries of jh_constraint.state that hold the accumulated impulse, which ResetWarmStart of
+// each constraint class zeroes (the hemisphere and clamp records after it stay).
+__host__ __device__ __forceinline__ int impulse_count(int type) {
+    switch (type) {
+        case JH_CONSTRAINT_BALL_SOCKET:
+        case JH_CONSTRAINT_HINGE_ANGLE:
+        case JH_CONSTRAINT_FIXED_ANGLE:
+        case JH_CONSTRAINT_POINT_ON_LINE:
+        case JH_CONSTRAINT_ANGULAR_SERVO:
+        case JH_CONSTRAINT_LINEAR_SERVO: return 3;
+        default: return 1;
+    }
+}
+
 // ---- JMatrix (LinearMath/JMatrix.cs), row-major fields m11 .. m33 -----------------------------
 __device__ __forceinline__ mat3 mzero() { return {0, 0, 0, 0, 0, 0, 0, 0, 0}; }
 __device__ __forceinline__ mat3 mmul(const mat3& a, const mat3& b) {  // operator * (:620-633)

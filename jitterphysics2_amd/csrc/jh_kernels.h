@@ -21,7 +21,7 @@ constexpr uint32_t kNoColour = 0xffffffffu;  // arbiter without a colour from a 
 // ---- Device state ---------------------------------------------------------------------------
 struct Bodies {
     float4 *pos, *quat, *vel, *angvel, *dvel, *dangvel;
-    float4* imv;          // InverseMassVector (xyz)
+    float4* imv;          // InverseMassVector (xyz); w = 1 if the body has linear locks
     float4* iw0;          // InverseInertiaWorld M11 M12 M13 M22
     float2* iw1;          // InverseInertiaWorld M23 M33
     float4* il0;          // body-frame inverse inertia M11 M12 M13 M22
@@ -32,6 +32,7 @@ struct Bodies {
     int* motion;          // MotionType
     int* gravity;         // AffectedByGravity
     int* world;           // sub-world id
+    int* locks;           // locked MotionAxes (the complement of AllowedMotion), bits 0-5
     float4* slp;          // DeactivationThreshold linear^2, angular^2, DeactivationTime, sleep timer
     uint32_t* sflags;     // island flags (jh_islands.h); bit 0 = RigidBodyData.IsActive
 };
@@ -188,21 +189,72 @@ __device__ __forceinline__ sym3 ld_iw(const Bodies& B, int b) {
 }
 
 // ---- Bodies ---------------------------------------------------------------------------------
-// RigidBody.UpdateWorldInertia (RigidBody.cs:903-923), no motion locks.
+// RigidBody.RestrictLinearMotion / RestrictAngularMotion (RigidBody.cs:651-713) for locked axes
+// `locks` (bits 0-2 linear x, y, z; 3-5 angular x, y, z).
+__device__ __forceinline__ vec3 restrict_linear(vec3 v, int locks) {
+    if (locks & 1) v.x = 0.0f;
+    if (locks & 2) v.y = 0.0f;
+    if (locks & 4) v.z = 0.0f;
+    return v;
+}
+__device__ __forceinline__ vec3 restrict_angular(vec3 v, int locks) {
+    if (locks & 8) v.x = 0.0f;
+    if (locks & 16) v.y = 0.0f;
+    if (locks & 32) v.z = 0.0f;
+    return v;
+}
+// The inverse inertia with locked rotations eliminated by Schur complements, axis by axis.
+__device__ __forceinline__ sym3 restrict_inertia(sym3 I, int locks) {
+    if ((locks & 56) == 56) return sym3{0, 0, 0, 0, 0, 0};
+    if (locks & 8) {
+        if (I.m11 > 0.0f) {
+            float xy = I.m12 / I.m11, xz = I.m13 / I.m11;
+            I.m22 -= xy * I.m12;
+            I.m23 -= xy * I.m13;
+            I.m33 -= xz * I.m13;
+        }
+        I.m11 = I.m12 = I.m13 = 0.0f;
+    }
+    if (locks & 16) {
+        if (I.m22 > 0.0f) {
+            float xy = I.m12 / I.m22, yz = I.m23 / I.m22;
+            I.m11 -= xy * I.m12;
+            I.m13 -= xy * I.m23;
+            I.m33 -= yz * I.m23;
+        }
+        I.m12 = I.m22 = I.m23 = 0.0f;
+    }
+    if (locks & 32) {
+        if (I.m33 > 0.0f) {
+            float xz = I.m13 / I.m33, yz = I.m23 / I.m33;
+            I.m11 -= xz * I.m13;
+            I.m12 -= xz * I.m23;
+            I.m22 -= yz * I.m23;
+        }
+        I.m13 = I.m23 = I.m33 = 0.0f;
+    }
+    return I;
+}
+
+// RigidBody.UpdateWorldInertia (RigidBody.cs:903-923) with the AllowedMotion restrictions.
 __device__ __forceinline__ void update_world_inertia(const Bodies& B, int b) {
+    int locks = B.locks[b];
+    float ll = (locks & 7) ? 1.0f : 0.0f;
     if (B.motion[b] == JH_MOTION_DYNAMIC) {
         float4 l0 = B.il0[b];
         float2 l1 = B.il1[b];
         sym3 il{l0.x, l0.y, l0.z, l0.w, l1.x, l1.y};
         sym3 w = sym_rotate(il, mat_from_quat(ldq(B.quat[b])));
+        if (locks & 56) w = restrict_inertia(w, locks);
         B.iw0[b] = make_float4(w.m11, w.m12, w.m13, w.m22);
         B.iw1[b] = make_float2(w.m23, w.m33);
         float im = B.mat[b].x;
-        B.imv[b] = make_float4(im, im, im, 0.0f);
+        vec3 imv = restrict_linear(vec3{im, im, im}, locks);
+        B.imv[b] = make_float4(imv.x, imv.y, imv.z, ll);
     } else {
         B.iw0[b] = make_float4(0, 0, 0, 0);
         B.iw1[b] = make_float2(0, 0);
-        B.imv[b] = make_float4(0, 0, 0, 0);
+        B.imv[b] = make_float4(0, 0, 0, ll);
     }
 }
 
@@ -222,6 +274,14 @@ __device__ __forceinline__ void unpack_body(const jh_body& s, int b, const Bodie
     B.motion[b] = s.motion_type;
     B.gravity[b] = s.affected_by_gravity;
     B.world[b] = s.world_id;
+    int locks = s.locked_axes & 63;
+    B.locks[b] = locks;
+    if (locks) {  // the velocity and AllowedMotion setters clear locked components (RigidBody.cs:584-588)
+        B.vel[b] = st3(restrict_linear(ld3(B.vel[b]), locks));
+        B.dvel[b] = st3(restrict_linear(ld3(B.dvel[b]), locks));
+        B.angvel[b] = st3(restrict_angular(ld3(B.angvel[b]), locks));
+        B.dangvel[b] = st3(restrict_angular(ld3(B.dangvel[b]), locks));
+    }
     B.slp[b] = make_float4(s.deactivation_threshold_linear_sq, s.deactivation_threshold_angular_sq,
                            s.deactivation_time, 0.0f);
     update_world_inertia(B, b);
@@ -264,7 +324,7 @@ __global__ void k_pack_bodies(Bodies B, int n, jh_body* __restrict__ out) {
     o.deactivation_threshold_linear_sq = v.x;
     o.deactivation_threshold_angular_sq = v.y;
     o.deactivation_time = v.z;
-    o.reserved_body = 0;
+    o.locked_axes = B.locks[b];
 }
 
 // RigidBodyData in the reference's 128-byte FP32 layout (RigidBody.cs:107-160).
@@ -286,7 +346,8 @@ __global__ void k_pack_rigidbodydata(Bodies B, int n, char* __restrict__ out, in
     v = B.iw0[b]; rf[19] = v.x; rf[20] = v.y; rf[21] = v.z; rf[22] = v.w;
     float2 w = B.iw1[b]; rf[23] = w.x; rf[24] = w.y;
     v = B.imv[b]; rf[25] = v.x; rf[26] = v.y; rf[27] = v.z;
-    ((int*)(rf + 28))[0] = (B.motion[b] & 3) | ((B.sflags[b] & 1u) ? 4 : 0);  // MotionType | IsActive
+    // MotionType | IsActive | the locked axes (AllowedMotion's complement, bits 4..9)
+    ((int*)(rf + 28))[0] = (B.motion[b] & 3) | ((B.sflags[b] & 1u) ? 4 : 0) | (B.locks[b] << 4);
 }
 
 // The body state of the reference's 128-byte FP32 RigidBodyData records (World.RawData): position,
@@ -1337,10 +1398,13 @@ __global__ void k_colour_offsets(const int* __restrict__ hist, int* __restrict__
 struct BodyReg {
     vec3 v, w, imv;
     sym3 I;
+    bool ll;  // linear locks: the contact's linear mass is directional (Contact.cs:1055-1061)
 };
 
 __device__ __forceinline__ void load_static_part(const Bodies& B, int b, BodyReg& r) {
-    r.imv = ld3(B.imv[b]);
+    float4 iv = B.imv[b];
+    r.imv = vec3{iv.x, iv.y, iv.z};
+    r.ll = iv.w != 0.0f;
     r.I = ld_iw(B, b);
 }
 
@@ -1358,13 +1422,15 @@ __device__ __forceinline__ void prepare_contact(const jh_arbiter& cd, jh_contact
     vec3 imv{0.0f, 0.0f, 0.0f};
     if (mode & JH_SOLVE_LINEAR1) imv = B1.imv;
     if (mode & JH_SOLVE_LINEAR2) imv = imv + B2.imv;
-    float k[3] = {imv.x, imv.x, imv.x};
     if ((c.flag & 2u) == 0) c.bias = 0.0f;
     vec3 r1 = cr1 + x1, r2 = cr2 + x2;
     vec3 dist = r1 - r2;
     float NX[3] = {c.normal[0], c.tangent1[0], c.tangent2[0]};
     float NY[3] = {c.normal[1], c.tangent1[1], c.tangent2[1]};
     float NZ[3] = {c.normal[2], c.tangent1[2], c.tangent2[2]};
+    float k[3] = {imv.x, imv.x, imv.x};
+    if (B1.ll || B2.ll)
+        for (int l = 0; l < 3; ++l) k[l] = (NX[l] * imv.x) * NX[l] + (NY[l] * imv.y) * NY[l] + (NZ[l] * imv.z) * NZ[l];
     vec3 n{NX[0], NY[0], NZ[0]};
     float pen = dot(dist, n);
     if (pen < -0.02f) c.bias = pen * ts.inv_dt * cd.speculative_relaxation_factor;
@@ -1900,13 +1966,15 @@ __device__ __forceinline__ void prepare_reg(uint32_t mode, float specrel, SlotRe
     vec3 imv{0.0f, 0.0f, 0.0f};
     if (mode & JH_SOLVE_LINEAR1) imv = B1.imv;
     if (mode & JH_SOLVE_LINEAR2) imv = imv + B2.imv;
-    float k[3] = {imv.x, imv.x, imv.x};
     if ((c.flag & 2u) == 0) c.bias = 0.0f;
     vec3 r1 = cr1 + x1, r2 = cr2 + x2;
     vec3 dist = r1 - r2;
     const float* NX = c.nx;
     const float* NY = c.ny;
     const float* NZ = c.nz;
+    float k[3] = {imv.x, imv.x, imv.x};
+    if (B1.ll || B2.ll)
+        for (int l = 0; l < 3; ++l) k[l] = (NX[l] * imv.x) * NX[l] + (NY[l] * imv.y) * NY[l] + (NZ[l] * imv.z) * NZ[l];
     vec3 n{NX[0], NY[0], NZ[0]};
     float pen = dot(dist, n);
     if (pen < -0.02f) c.bias = pen * ts.inv_dt * specrel;
@@ -3085,6 +3153,11 @@ __global__ void k_update_bodies(Bodies B, int nb, StepTime ts, float gx, float g
     uint32_t fl = B.sflags[b];
     if (!(fl & 1u)) return;
     int mt = B.motion[b];
+    int locks = B.locks[b];
+    if (locks) {  // RigidBody.cs:719-722
+        B.vel[b] = st3(restrict_linear(ld3(B.vel[b]), locks));
+        B.angvel[b] = st3(restrict_angular(ld3(B.angvel[b]), locks));
+    }
     {
         float4 sl = B.slp[b];
         vec3 v = ld3(B.vel[b]), w = ld3(B.angvel[b]);
@@ -3103,7 +3176,7 @@ __global__ void k_update_bodies(Bodies B, int nb, StepTime ts, float gx, float g
     B.vel[b] = st3(ld3(B.vel[b]) * m.w);
     vec3 dv = cmul(ld3(B.force[b]), ld3(B.imv[b])) * ts.substep_dt;
     vec3 dw = sym_mul(ld3(B.torque[b]), ld_iw(B, b)) * ts.substep_dt;
-    if (B.gravity[b]) dv = dv + vec3{gx, gy, gz} * ts.substep_dt;
+    if (B.gravity[b]) dv = dv + restrict_linear(vec3{gx, gy, gz}, locks) * ts.substep_dt;
     B.dvel[b] = st3(dv);
     B.dangvel[b] = st3(dw);
     B.force[b] = make_float4(0, 0, 0, 0);

@@ -155,7 +155,7 @@ struct jh_world {
     // bodies
     DBuf<float4> pos, quat, vel, angvel, dvel, dangvel, imv, iw0, il0, mat, mat2, force, torque;
     DBuf<float2> iw1, il1;
-    DBuf<int> motion, gravity, world;
+    DBuf<int> motion, gravity, world, locks;
     DBuf<jh_body> body_io;
     // islands and sleeping (jh_islands.h): flags and timers per body, the solver's motion types, and
     // the union-find of each CheckDeactivation
@@ -273,7 +273,7 @@ struct jh_world {
         B.pos = pos.p; B.quat = quat.p; B.vel = vel.p; B.angvel = angvel.p; B.dvel = dvel.p; B.dangvel = dangvel.p;
         B.imv = imv.p; B.iw0 = iw0.p; B.iw1 = iw1.p; B.il0 = il0.p; B.il1 = il1.p; B.mat = mat.p; B.mat2 = mat2.p;
         B.force = force.p; B.torque = torque.p; B.motion = motion.p; B.gravity = gravity.p; B.world = world.p;
-        B.slp = slp.p; B.sflags = sflags.p;
+        B.slp = slp.p; B.sflags = sflags.p; B.locks = locks.p;
         return B;
     }
     // Islands are checked while deactivation is allowed or a body sleeps; otherwise every body is awake.
@@ -297,6 +297,23 @@ __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     else if (t < 6) bounds[t] = 0u;
 }
 __global__ void k_set_int(int* p, int v) { *p = v; }
+__global__ void k_mark_listed(const int* __restrict__ idx, int n, uint32_t* __restrict__ mark) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) mark[idx[i]] = 1u;
+}
+__global__ void k_locks_changed(jh_arbiter* __restrict__ arb, const int* __restrict__ a_count,
+                                jh_constraint* __restrict__ con, int ncon, const uint32_t* __restrict__ mark) {
+    int A = *a_count;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A + ncon; i += gridDim.x * blockDim.x) {
+        if (i < A) {
+            if (mark[arb[i].body1] || mark[arb[i].body2]) arb[i].usage_mask = 0u;
+        } else {
+            jh_constraint& c = con[i - A];
+            if (!mark[c.body1] && !mark[c.body2]) continue;
+            int n = con::impulse_count(c.type);
+            for (int k = 0; k < n; ++k) c.state[k] = 0.0f;
+        }
+    }
+}
 __global__ void k_solver_motion(int nb, const int* __restrict__ motion, const uint32_t* __restrict__ sflags,
                                 int* __restrict__ smotion) {
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x)
@@ -355,6 +372,25 @@ void wake_listed(jh_world* w, const int* idx, int n, int active, int wake_static
         isl::k_clear_bits<<<grid_for(std::max(w->nb, 1), 256, 1 << 20), 256, 0, st>>>(w->nb, w->sflags.p, isl::kSwake);
     }
     w->isl_dirty = true;
+}
+
+// The rest of the AllowedMotion setter for bodies whose locks changed (RigidBody.cs:589-592):
+// ClearContactCache (the usage masks of their arbiters to zero) and ResetWarmStart of their
+// constraints (the accumulated impulses).
+jh_status locks_changed(jh_world* w, const std::vector<int>& bodies) {
+    hipStream_t st = w->st;
+    HIPCHK(w->hfl.ensure(std::max(w->nb, 1)));
+    k_fill_u32<<<grid_for(std::max(w->nb, 1), 256, 1 << 20), 256, 0, st>>>(w->hfl.p, size_t(std::max(w->nb, 1)), 0u);
+    HIPCHK(w->uidx.ensure(bodies.size()));
+    HIPCHK(hipMemcpyAsync(w->uidx.p, bodies.data(), bodies.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    k_mark_listed<<<grid_for(int(bodies.size())), 256, 0, st>>>(w->uidx.p, int(bodies.size()), w->hfl.p);
+    int ncon = int(w->host_con.size());
+    int total = w->a_host + ncon;
+    k_locks_changed<<<grid_for(std::max(total, 1)), 256, 0, st>>>(
+        w->arb[w->cur].p, w->cnt.p + (w->cur == 0 ? C_A0 : C_A1), w->con.p, ncon, w->hfl.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return JH_OK;
 }
 
 // Islands rebuilt with every body awake and on the active list (new contact cache or constraint set).
@@ -1313,6 +1349,9 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     // a new body count or a motion-type change rebuilds the islands with every body awake
     bool rebuild = n != w->nb || int(w->host_bodies.size()) != n;
     for (int i = 0; i < n && !rebuild; ++i) rebuild = w->host_bodies[i].motion_type != bodies[i].motion_type;
+    std::vector<int> relocked;  // bodies whose motion locks change
+    for (int i = 0; i < n && i < int(w->host_bodies.size()); ++i)
+        if ((w->host_bodies[i].locked_axes & 63) != (bodies[i].locked_axes & 63)) relocked.push_back(i);
     w->nb = n;
     w->max_world = maxw;
     w->n_spec = 0;
@@ -1373,6 +1412,7 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     HIPCHK(w->motion.ensure(nn));
     HIPCHK(w->gravity.ensure(nn));
     HIPCHK(w->world.ensure(nn));
+    HIPCHK(w->locks.ensure(nn));
     HIPCHK(w->body_io.ensure(nn));
     HIPCHK(w->bmax.ensure(2 * nn));
     HIPCHK(w->mlo.ensure(nn));
@@ -1397,6 +1437,10 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
         w->isl_dirty = true;
     }
     if (rebuild) w->n_asleep = 0;
+    if (!relocked.empty()) {
+        jh_status s = locks_changed(w, relocked);
+        if (s != JH_OK) return s;
+    }
     k_set_int<<<1, 1, 0, w->st>>>(w->cnt.p + C_NB, n);
     w->recolour = true;  // motion types may have changed
     HIPCHK(hipStreamSynchronize(w->st));
@@ -1450,7 +1494,14 @@ jh_status jh_world_update_bodies(jh_world* w, const int32_t* index, const jh_bod
     HIPCHK(hipMemcpyAsync(w->body_io.p, records, size_t(count) * sizeof(jh_body), hipMemcpyHostToDevice, w->st));
     HIPCHK(hipMemcpyAsync(w->uidx.p, index, size_t(count) * sizeof(int), hipMemcpyHostToDevice, w->st));
     k_unpack_listed<<<grid_for(count, 256, 1 << 20), 256, 0, w->st>>>(w->body_io.p, w->uidx.p, count, w->bodies());
+    std::vector<int> relocked;
+    for (int k = 0; k < count; ++k)
+        if ((w->host_bodies[index[k]].locked_axes & 63) != (records[k].locked_axes & 63)) relocked.push_back(index[k]);
     for (int k = 0; k < count; ++k) w->host_bodies[index[k]] = records[k];
+    if (!relocked.empty()) {
+        jh_status s = locks_changed(w, relocked);
+        if (s != JH_OK) return s;
+    }
     // each listed body: ActivateBodyNextStep(body, wakeUpStatic = true)
     wake_listed(w, w->uidx.p, count, 1, 1);
     HIPCHK(hipGetLastError());
@@ -1500,8 +1551,10 @@ jh_status jh_world_set_rigidbodydata(jh_world* w, const void* records, int32_t n
             return fail(JH_ERR_UNSUPPORTED, "body %d: the motion type differs from the body's; set it with jh_world_set_bodies", i);
         if (((flags & 4) != 0) != ((sf[i] & isl::kAwake) != 0))
             return fail(JH_ERR_UNSUPPORTED, "body %d: IsActive differs from the body's; use jh_world_set_activation_state", i);
-        if (flags & 0x3f8)
-            return fail(JH_ERR_UNSUPPORTED, "body %d: gyroscopic integration and motion locks are not on the GPU path", i);
+        if (flags & 8) return fail(JH_ERR_UNSUPPORTED, "body %d: gyroscopic integration is not on the GPU path", i);
+        if (((flags >> 4) & 63) != (w->host_bodies[i].locked_axes & 63))
+            return fail(JH_ERR_UNSUPPORTED, "body %d: the motion locks differ from the body's; set them with jh_world_update_bodies", i);
+        if (flags & ~0x3ff) return fail(JH_ERR_UNSUPPORTED, "body %d: undefined flag bits", i);
         float f[28];
         memcpy(f, r + 8, sizeof(f));
         for (int k = 0; k < 28; ++k)

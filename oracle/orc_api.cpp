@@ -60,6 +60,12 @@ template <class R> void body_in(Body<R>& o, const jh_body& r) {
     o.sleep_lin_sq = R(r.deactivation_threshold_linear_sq);
     o.sleep_ang_sq = R(r.deactivation_threshold_angular_sq);
     o.deact_time = R(r.deactivation_time);
+    o.locks = r.locked_axes & 63;
+    // the velocity setters (and the AllowedMotion setter) clear locked components (RigidBody.cs:584-588)
+    restrict_linear(o.vel, o.locks);
+    restrict_linear(o.dvel, o.locks);
+    restrict_angular(o.angvel, o.locks);
+    restrict_angular(o.dangvel, o.locks);
     o.update_world_inertia();
 }
 // Replacing the records is every property setter at once: a new body count or a motion-type change
@@ -68,8 +74,12 @@ template <class R> void body_in(Body<R>& o, const jh_body& r) {
 template <class R> void set_bodies(World<R>& w, const jh_body* b, int n) {
     bool rebuild = (int)w.bodies.size() != n;
     for (int i = 0; i < n && !rebuild; ++i) rebuild = w.bodies[i].motion != b[i].motion_type;
+    std::vector<int> relocked;
+    for (int i = 0; i < n && i < (int)w.bodies.size(); ++i)
+        if (w.bodies[i].locks != (b[i].locked_axes & 63)) relocked.push_back(i);
     w.bodies.resize(n);
     for (int i = 0; i < n; ++i) body_in(w.bodies[i], b[i]);
+    for (int i : relocked) w.locks_changed(i);
     if (rebuild) w.islands_reset();
     else
         for (int i = 0; i < n; ++i) w.activate_body_next_step(i, true);
@@ -81,7 +91,9 @@ template <class R> int update_bodies(World<R>& w, const int* idx, const jh_body*
     bool rebuild = false;
     for (int k = 0; k < count; ++k) {
         rebuild = rebuild || w.bodies[idx[k]].motion != b[k].motion_type;
+        bool relock = w.bodies[idx[k]].locks != (b[k].locked_axes & 63);
         body_in(w.bodies[idx[k]], b[k]);
+        if (relock) w.locks_changed(idx[k]);
     }
     if (rebuild) w.islands_reset();
     else
@@ -420,6 +432,22 @@ int orc_compute_pairs(void* p, uint32_t* out, int cap, int brute) {
         int n = (int)pairs.size();
         for (int i = 0; i < n && i < cap; ++i) { out[2 * i] = pairs[i].first; out[2 * i + 1] = pairs[i].second; }
         return n;
+    })());
+}
+// Solver mass properties per body, 9 doubles: InverseMassVector, InverseInertiaWorld (M11 M12 M13 M22
+// M23 M33), as RigidBody.UpdateWorldInertia left them.
+int orc_get_mass(void* p, double* out, int n) {
+    Handle* h = (Handle*)p;
+    return DISPATCH(h, ([&]() {
+        int m = std::min(n, (int)W.bodies.size());
+        for (int i = 0; i < m; ++i) {
+            const auto& b = W.bodies[i];
+            double* d = out + 9 * i;
+            d[0] = b.inv_mass_vec.x; d[1] = b.inv_mass_vec.y; d[2] = b.inv_mass_vec.z;
+            d[3] = b.inv_inertia_world.m11; d[4] = b.inv_inertia_world.m12; d[5] = b.inv_inertia_world.m13;
+            d[6] = b.inv_inertia_world.m22; d[7] = b.inv_inertia_world.m23; d[8] = b.inv_inertia_world.m33;
+        }
+        return m;
     })());
 }
 // Activation state per body: active[i] = RigidBody.IsActive, sleep[i] = InternalSleepTime.

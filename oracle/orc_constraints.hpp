@@ -330,6 +330,20 @@ template <class R> inline void setv(R* p, V3<R> v) { p[0] = v.x; p[1] = v.y; p[2
 // BallSocket (BallSocket.cs): params LocalAnchor1 (0-2), LocalAnchor2 (3-5), BiasFactor (6), Softness (7);
 // state AccumulatedImpulse (0-2).
 // PrepareForIterationBallSocket (:140-183)
+// Leading entries of the constraint state that hold the accumulated impulse (ResetWarmStart of each
+// constraint class zeroes its AccumulatedImpulse; the hemisphere and clamp records after it stay).
+inline int constraint_impulse_count(int type) {
+    switch (type) {
+        case JH_CONSTRAINT_BALL_SOCKET:
+        case JH_CONSTRAINT_HINGE_ANGLE:
+        case JH_CONSTRAINT_FIXED_ANGLE:
+        case JH_CONSTRAINT_POINT_ON_LINE:
+        case JH_CONSTRAINT_ANGULAR_SERVO:
+        case JH_CONSTRAINT_LINEAR_SERVO: return 3;
+        default: return 1;
+    }
+}
+
 template <class R> void prepare_ball_socket(Constraint<R>& c, ConstraintBody<R>& A, ConstraintBody<R>& B, R inv_dt,
                                             R inv_sub) {
     c.r1 = rotate(vecp(c.p), A.q);

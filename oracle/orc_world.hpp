@@ -20,6 +20,55 @@
 
 namespace orc {
 
+// RigidBody.RestrictLinearMotion / RestrictAngularMotion (RigidBody.cs:651-713); `locks` holds the
+// locked axes (bits 0-2 linear x, y, z; 3-5 angular x, y, z).
+template <class R> inline void restrict_linear(V3<R>& v, int locks) {
+    if (!(locks & 7)) return;
+    if (locks & 1) v.x = R(0);
+    if (locks & 2) v.y = R(0);
+    if (locks & 4) v.z = R(0);
+}
+template <class R> inline void restrict_angular(V3<R>& v, int locks) {
+    if (!(locks & 56)) return;
+    if (locks & 8) v.x = R(0);
+    if (locks & 16) v.y = R(0);
+    if (locks & 32) v.z = R(0);
+}
+// The inverse inertia with locked rotations eliminated by Schur complements, axis by axis.
+template <class R> inline void restrict_inertia(Sym<R>& I, int locks) {
+    if ((locks & 56) == 56) {
+        I = Sym<R>{R(0), R(0), R(0), R(0), R(0), R(0)};
+        return;
+    }
+    if (locks & 8) {
+        if (I.m11 > R(0)) {
+            R xy = I.m12 / I.m11, xz = I.m13 / I.m11;
+            I.m22 -= xy * I.m12;
+            I.m23 -= xy * I.m13;
+            I.m33 -= xz * I.m13;
+        }
+        I.m11 = I.m12 = I.m13 = R(0);
+    }
+    if (locks & 16) {
+        if (I.m22 > R(0)) {
+            R xy = I.m12 / I.m22, yz = I.m23 / I.m22;
+            I.m11 -= xy * I.m12;
+            I.m13 -= xy * I.m23;
+            I.m33 -= yz * I.m23;
+        }
+        I.m12 = I.m22 = I.m23 = R(0);
+    }
+    if (locks & 32) {
+        if (I.m33 > R(0)) {
+            R xz = I.m13 / I.m33, yz = I.m23 / I.m33;
+            I.m11 -= xz * I.m13;
+            I.m12 -= xz * I.m23;
+            I.m22 -= yz * I.m23;
+        }
+        I.m13 = I.m23 = I.m33 = R(0);
+    }
+}
+
 template <class R> struct Body {
     // RigidBodyData (RigidBody.cs:107-250)
     V3<R> pos, vel, angvel, dvel, dangvel;
@@ -40,16 +89,19 @@ template <class R> struct Body {
     // non-static bodies (one entry per arbiter or constraint, InternalConnections)
     bool active = true;
     R sleep_time = R(0);
+    int locks = 0;  // locked MotionAxes (RigidBodyData.Flags bits 4..9 shifted down): 0 = AllowedMotion.All
     R sleep_lin_sq = R(0.1), sleep_ang_sq = R(0.1), deact_time = R(1.0);
     int island = -1;
     std::vector<int> conn;
 
-    // RigidBody.UpdateWorldInertia (RigidBody.cs:903-923), no motion locks.
+    // RigidBody.UpdateWorldInertia (RigidBody.cs:903-923) with the AllowedMotion restrictions.
     void update_world_inertia() {
         if (motion == JH_MOTION_DYNAMIC) {
             M3<R> m = mat_from_quat(q);
             inv_inertia_world = sym_transform(inv_inertia, m);
+            if (locks & 56) restrict_inertia(inv_inertia_world, locks);
             inv_mass_vec = {inv_mass, inv_mass, inv_mass};
+            restrict_linear(inv_mass_vec, locks);
         } else {
             inv_inertia_world = Sym<R>{R(0), R(0), R(0), R(0), R(0), R(0)};
             inv_mass_vec = {R(0), R(0), R(0)};
@@ -271,6 +323,15 @@ template <class R> struct World {
                     activate_body_next_step(kv.second.b1 == b ? kv.second.b2 : kv.second.b1);
         }
         isl[bodies[b].island].needs = true;
+    }
+    // The rest of the AllowedMotion setter (RigidBody.cs:589-592): ClearContactCache (usage masks of
+    // the body's arbiters to zero) and ResetWarmStart of its constraints (their accumulated impulses).
+    void locks_changed(int b) {
+        for (auto& kv : arbiters)
+            if (kv.second.b1 == b || kv.second.b2 == b) kv.second.usage = 0;
+        for (auto& c : constraints)
+            if (c.b1 == b || c.b2 == b)
+                for (int k = 0; k < constraint_impulse_count(c.type); ++k) c.s[k] = R(0);
     }
     // Islands rebuilt from scratch: every body awake in its own island (IslandHelper.BodyAdded and
     // AddToActiveList of World.CreateRigidBody), then every constraint and arbiter connected
@@ -732,15 +793,17 @@ template <class R> struct World {
         V3<R> imv{R(0), R(0), R(0)};
         if (cd.mode & JH_SOLVE_LINEAR1) imv = b1.inv_mass_vec;
         if (cd.mode & JH_SOLVE_LINEAR2) imv = imv + b2.inv_mass_vec;
+        // Rows: X = (n.x, t1.x, t2.x), Y, Z.
+        R NX[3] = {c.n.x, c.t1.x, c.t2.x}, NY[3] = {c.n.y, c.t1.y, c.t2.y}, NZ[3] = {c.n.z, c.t1.z, c.t2.z};
         R k[3] = {imv.x, imv.x, imv.x};
+        if ((b1.locks & 7) || (b2.locks & 7))  // directional linear mass with linear locks
+            for (int l = 0; l < 3; ++l) k[l] = (NX[l] * imv.x) * NX[l] + (NY[l] * imv.y) * NY[l] + (NZ[l] * imv.z) * NZ[l];
         if ((c.flag & Contact<R>::NewContact) == 0) c.bias = 0;
         V3<R> r1 = cr1 + b1.pos, r2 = cr2 + b2.pos;
         V3<R> dist = r1 - r2;
         R pen = dot(dist, c.n);
         if (pen < -Contact<R>::BreakThreshold) c.bias = pen * ts.inv_dt * cd.specrel;
         c.flag &= ~uint32_t(Contact<R>::NewContact);
-        // Rows: X = (n.x, t1.x, t2.x), Y, Z.
-        R NX[3] = {c.n.x, c.t1.x, c.t2.x}, NY[3] = {c.n.y, c.t1.y, c.t2.y}, NZ[3] = {c.n.z, c.t1.z, c.t2.z};
         R ktx[3] = {0, 0, 0}, kty[3] = {0, 0, 0}, ktz[3] = {0, 0, 0};
         V3<R> lin{sum3(ws, NX[0], NX[1], NX[2]), sum3(ws, NY[0], NY[1], NY[2]), sum3(ws, NZ[0], NZ[1], NZ[2])};
         if (cd.mode & JH_SOLVE_LINEAR1) b1.vel = b1.vel - cmul(lin, b1.inv_mass_vec);
@@ -1138,6 +1201,8 @@ template <class R> struct World {
         for (long i = 0; i < long(bodies.size()); ++i) {
             Body<R>& b = bodies[i];
             if (!b.active) continue;
+            restrict_linear(b.vel, b.locks);  // RigidBody.cs:719-722
+            restrict_angular(b.angvel, b.locks);
             bool kin = b.motion == JH_MOTION_KINEMATIC;
             auto zero = [](V3<R> v) { return v.x == R(0) && v.y == R(0) && v.z == R(0); };  // JVector ==
             bool resting = kin ? (zero(b.angvel) && zero(b.vel))
@@ -1150,7 +1215,11 @@ template <class R> struct World {
             b.vel = b.vel * b.lin_damp;
             b.dvel = cmul(b.force, b.inv_mass_vec) * ts.substep_dt;
             b.dangvel = sym_mul(b.torque, b.inv_inertia_world) * ts.substep_dt;
-            if (b.gravity) b.dvel = b.dvel + gravity * ts.substep_dt;
+            if (b.gravity) {
+                V3<R> g = gravity;
+                restrict_linear(g, b.locks);
+                b.dvel = b.dvel + g * ts.substep_dt;
+            }
             b.force = {R(0), R(0), R(0)};
             b.torque = {R(0), R(0), R(0)};
         }

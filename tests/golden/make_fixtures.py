@@ -31,9 +31,9 @@ from jitterphysics2_amd import scenes  # noqa: E402
 
 
 # Body fields outside the hash: the sleep parameters, which do not act in scenes that run without
-# deactivation (the config's allow_deactivation is hashed).
+# deactivation (the config's allow_deactivation is hashed), and the motion locks, which must be zero.
 SLEEP_FIELDS = ("deactivation_threshold_linear_sq", "deactivation_threshold_angular_sq", "deactivation_time",
-                "reserved_body")
+                "locked_axes")
 
 
 def scene_hash(scene) -> str:
@@ -41,6 +41,7 @@ def scene_hash(scene) -> str:
     h = hashlib.sha256()
     v = np.zeros(0, np.float32) if verts is None else np.asarray(verts, np.float32)
     b = np.ascontiguousarray(bodies)
+    assert not np.any(b["locked_axes"]), "fixture scenes have no motion locks"
     b = rfn.repack_fields(b[[n for n in b.dtype.names if n not in SLEEP_FIELDS]])
     for a in (b, shapes, v, cfg, np.float32(dt)):
         h.update(np.ascontiguousarray(a).tobytes())

@@ -51,6 +51,7 @@ def lib() -> ctypes.CDLL:
             "orc_get_activation": [vp, vp, vp, i32], "orc_set_activation_state": [vp, i32, i32],
             "orc_update_bodies": [vp, vp, vp, i32], "orc_get_islands": [vp, vp, i32],
             "orc_get_arbiters_all": [vp, vp, i32],
+            "orc_get_mass": [vp, vp, i32],
         }.items():
             getattr(L, n).argtypes = a
             getattr(L, n).restype = i32
@@ -148,6 +149,14 @@ class OracleWorld:
         rows = int(counts[0])
         return {"residual": res, "points": int(counts[1]), "features": feats[:rows], "targets": targs[:rows],
                 "flags": flags[:rows]}
+
+    def mass(self) -> tuple[np.ndarray, np.ndarray]:
+        """(InverseMassVector (n, 3), InverseInertiaWorld (n, 3, 3)) as the solver uses them."""
+        out = np.zeros((self.n, 9), np.float64)
+        self.L.orc_get_mass(self.h, ptr(out), self.n)
+        a, b, c, d, e, f = (out[:, 3 + k] for k in range(6))
+        iw = np.stack([np.stack([a, b, c], -1), np.stack([b, d, e], -1), np.stack([c, e, f], -1)], -2)
+        return out[:, 0:3], iw
 
     def activation(self) -> tuple[np.ndarray, np.ndarray]:
         """(RigidBody.IsActive as bool, InternalSleepTime as float32) per body."""

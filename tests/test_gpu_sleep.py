@@ -187,9 +187,12 @@ def test_slow_kinematic_platform_carries_its_rider():  # SleepTests.cs:402-423
     assert abs((st["position"][r][1] - st["position"][p][1]) - gap) <= 0.02
 
 
-def test_heap_settles_island_by_island():
+@pytest.mark.parametrize("solver", ["world", "flow", "barrier", "lanes", "tiles"])
+def test_heap_settles_island_by_island(monkeypatch, solver):
     """300 mixed shapes fall, spread and sleep island by island (rolling spheres stay awake); arbiters
-    appear and break throughout, so islands merge and split on both sides."""
+    appear and break throughout, so islands merge and split on both sides.  Every solver kernel (a
+    sleeping body is static to it)."""
+    monkeypatch.setenv("JH_SOLVER", solver)
     bodies, shapes, verts, cfg, dt = scenes.heap(300, seed=3)
     cfg["allow_deactivation"] = 1
     bodies["deactivation_time"] = 0.3
