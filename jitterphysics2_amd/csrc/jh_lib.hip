@@ -613,6 +613,8 @@ jh_status tile_setup(jh_world* w, const uint64_t* order, const int* a_cur, int a
     hipStream_t st = w->st;
     int* cnt = w->cnt.p;
     int ns = w->ns, nb = std::max(w->nb, 1), T = w->tiles;
+    // tiles hold the bodies the solver moves: sleeping bodies (static to the solver) stay out
+    const int* smotion = w->solver_bodies().motion;
     size_t cells = size_t(T) * 128 + 1;
     HIPCHK(w->tfirst.ensure(nb));
     HIPCHK(w->tile_of.ensure(nb));
@@ -636,7 +638,7 @@ jh_status tile_setup(jh_world* w, const uint64_t* order, const int* a_cur, int a
         const uint32_t* sorted = w->sorted_in1 ? w->mv1.p : w->mv0.p;
         int gs = grid_for(ns);
         k_body_first<<<gs, 256, 0, st>>>(sorted, cnt + C_NS, w->shapes.p, w->tfirst.p);
-        k_tile_flags<<<gs, 256, 0, st>>>(sorted, cnt + C_NS, w->shapes.p, w->motion.p, w->tfirst.p, w->tflags.p);
+        k_tile_flags<<<gs, 256, 0, st>>>(sorted, cnt + C_NS, w->shapes.p, smotion, w->tfirst.p, w->tflags.p);
         scan::exclusive_scan(w->tflags.p, w->trank.p, cnt + C_NS, ns, w->tsums.p, cnt + C_NMOV, st);
         k_tile_assign<<<gs, 256, 0, st>>>(sorted, cnt + C_NS, w->shapes.p, w->tflags.p, w->trank.p, cnt + C_NMOV, T,
                                           w->tbody.p, w->tile_of.p, w->tslot.p);
@@ -645,13 +647,13 @@ jh_status tile_setup(jh_world* w, const uint64_t* order, const int* a_cur, int a
     }
     int ga = grid_for(std::max(amax_, 1));
     jh_arbiter* arb = w->arb[w->cur].p;
-    k_tile_hist<<<ga, 256, 0, st>>>(order, w->coffs.p, arb, w->colour.p, w->motion.p, w->tile_of.p, w->thist.p,
+    k_tile_hist<<<ga, 256, 0, st>>>(order, w->coffs.p, arb, w->colour.p, smotion, w->tile_of.p, w->thist.p,
                                     w->tshared.p);
     k_set_int<<<1, 1, 0, st>>>(cnt + C_TCELLS, int(cells));
     scan::exclusive_scan(w->thist.p, w->toff.p, cnt + C_TCELLS, int(cells), w->tsums.p, nullptr, st);
-    k_tile_scatter<<<ga, 256, 0, st>>>(order, w->coffs.p, arb, w->colour.p, w->motion.p, w->tile_of.p, w->toff.p,
+    k_tile_scatter<<<ga, 256, 0, st>>>(order, w->coffs.p, arb, w->colour.p, smotion, w->tile_of.p, w->toff.p,
                                        w->tcur.p, w->tlist.p);
-    k_tile_serial<<<1, 64, 0, st>>>(order, w->coffs.p, arb, w->motion.p, w->tile_of.p, w->toff.p, w->tcur.p,
+    k_tile_serial<<<1, 64, 0, st>>>(order, w->coffs.p, arb, smotion, w->tile_of.p, w->toff.p, w->tcur.p,
                                     w->tlist.p);
     (void)a_cur;
     HIPCHK(hipGetLastError());
