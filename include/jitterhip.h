@@ -55,9 +55,17 @@ enum {
     JH_SHAPE_POINTCLOUD = 3, /* PointCloudShape.cs / VertexSupportMap.cs: params[0..2] = center (GetCenter),
                                 params[3..5] = shift, params[6..8] = cached local box min,
                                 params[9..11] = cached local box max; vertices[vertex_offset ..]           */
-    JH_SHAPE_TRIANGLE = 4    /* TriangleShape.cs (static meshes): vertices[vertex_offset .. +3] = A, B, C;
+    JH_SHAPE_TRIANGLE = 4,   /* TriangleShape.cs (static meshes): vertices[vertex_offset .. +3] = A, B, C;
                               * params: triangle normal(3), normals of NeighborA, B, C (3 each, zero =
                               * none), body frame (TriangleMesh.cs:56-63, 300-319)                     */
+    JH_SHAPE_CONVEXHULL = 5  /* ConvexHullShape.cs: params as POINTCLOUD (center = the hull's centre of
+                              * mass, shift, cached local box); vertex_count = V distinct hull vertices
+                              * (first-occurrence order of the triangle corners), vertices[vertex_offset
+                              * .. +V]; the vertex array continues with the adjacency as small integers
+                              * in floats: per vertex (first, end) into the neighbour list, then the
+                              * neighbour list (each vertex's neighbours ascending), padded to whole
+                              * vertices (3 floats).  SupportMap: the hill climb from vertex 0 with the
+                              * plateau pass (:378-450).                                               */
 };
 
 /* ContactData.SolveMode bits, Dynamics/Contact.cs:36-58. */

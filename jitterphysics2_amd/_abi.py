@@ -15,7 +15,7 @@ import numpy as np
 MOTION_DYNAMIC, MOTION_KINEMATIC, MOTION_STATIC = 0, 1, 2
 LOCK_LINEAR_X, LOCK_LINEAR_Y, LOCK_LINEAR_Z, LOCK_ANGULAR_X, LOCK_ANGULAR_Y, LOCK_ANGULAR_Z = 1, 2, 4, 8, 16, 32
 LOCK_LINEAR, LOCK_ANGULAR = 7, 56
-SHAPE_BOX, SHAPE_SPHERE, SHAPE_CAPSULE, SHAPE_POINTCLOUD, SHAPE_TRIANGLE = 0, 1, 2, 3, 4
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_CAPSULE, SHAPE_POINTCLOUD, SHAPE_TRIANGLE, SHAPE_CONVEXHULL = 0, 1, 2, 3, 4, 5
 
 JH_OK = 0
 ERRORS = {

@@ -16,6 +16,71 @@ struct ShapeRec {
     int vcnt;
 };
 
+// Hill climb over the hull's vertex graph from vertex 0 (ConvexHullShape.InternalSupportMap): move to
+// the first neighbour that is better by more than 1e-12; at a local maximum with a neighbour level
+// within 1e-12 (a plateau), climb along that neighbour's direction over equally good vertices.  The
+// dot products strictly increase (or, on one plateau, the secondary product does), so no vertex is
+// visited twice; the walk is capped at V + 1 moves, which a valid hull never reaches.
+__device__ __forceinline__ vec3 hull_support(const float* v, int V, vec3 d) {
+    const float* adj = v + 3 * V;
+    const float* nbl = adj + 2 * V;
+    auto vx = [&](int i) { return vec3{v[3 * i], v[3 * i + 1], v[3 * i + 2]}; };
+    const float eps = 1e-12f;
+    int cur = 0;
+    float dp = dot(vx(0), d);
+    for (int moves = 0; moves <= V;) {
+        bool verify = false, moved = false;
+        vec3 vdir{1.0f, 1.0f, 1.0f};  // JVector.Arbitrary
+        int lo = int(adj[2 * cur]), hi = int(adj[2 * cur + 1]);
+        for (int i = lo; i < hi; ++i) {
+            int nb = int(nbl[i]);
+            float np = dot(vx(nb), d);
+            if (fabsf(np - dp) < eps) {
+                vdir = vx(nb) - vx(cur);
+                verify = true;
+            }
+            if (np > dp + eps) {
+                dp = np;
+                cur = nb;
+                moved = true;
+                break;
+            }
+        }
+        if (moved) { ++moves; continue; }
+        if (!verify) break;
+        float d0 = dot(vdir, vx(cur));
+        bool back_to_main = false;
+        while (moves <= V) {
+            bool step = false;
+            lo = int(adj[2 * cur]);
+            hi = int(adj[2 * cur + 1]);
+            for (int i = lo; i < hi; ++i) {
+                int nb = int(nbl[i]);
+                float np = dot(vx(nb), d);
+                if (np > dp + eps) {
+                    dp = np;
+                    cur = nb;
+                    back_to_main = true;
+                    break;
+                }
+                if (fabsf(np - dp) < eps) {
+                    float d1 = dot(vdir, vx(nb));
+                    if (d1 > d0 + eps) {
+                        d0 = d1;
+                        cur = nb;
+                        step = true;
+                        break;
+                    }
+                }
+            }
+            ++moves;
+            if (back_to_main || !step) break;
+        }
+        if (!back_to_main) break;
+    }
+    return vx(cur);
+}
+
 struct ShapeView {
     const ShapeRec* s;
     const float* verts;  // all vertices (x, y, z) of the world
@@ -44,6 +109,8 @@ struct ShapeView {
                 }
                 return vec3{v[3 * idx], v[3 * idx + 1], v[3 * idx + 2]} + vec3{p[3], p[4], p[5]};
             }
+            case JH_SHAPE_CONVEXHULL:  // ConvexHullShape.InternalSupportMap (ConvexHullShape.cs:378-450) + shift
+                return hull_support(verts + 3 * s->voff, s->vcnt, d) + vec3{p[3], p[4], p[5]};
             default: {  // TriangleShape.cs:167-191
                 const float* v = verts + 3 * s->voff;
                 vec3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
@@ -59,7 +126,7 @@ struct ShapeView {
     }
     // ISupportMappable.GetCenter
     __device__ __forceinline__ vec3 center() const {
-        if (s->type == JH_SHAPE_POINTCLOUD) return {s->p[0], s->p[1], s->p[2]};
+        if (s->type == JH_SHAPE_POINTCLOUD || s->type == JH_SHAPE_CONVEXHULL) return {s->p[0], s->p[1], s->p[2]};
         if (s->type == JH_SHAPE_TRIANGLE) {
             const float* v = verts + 3 * s->voff;
             vec3 a{v[0], v[1], v[2]}, b{v[3], v[4], v[5]}, c{v[6], v[7], v[8]};
@@ -92,7 +159,8 @@ __device__ __forceinline__ void shape_aabb(const ShapeRec& s, const float* verts
             mx = mx + pos;
             return;
         }
-        case JH_SHAPE_POINTCLOUD: {  // PointCloudShape.cs:140-154
+        case JH_SHAPE_POINTCLOUD:    // PointCloudShape.cs:140-154
+        case JH_SHAPE_CONVEXHULL: {  // ConvexHullShape.cs:334-349: the same cached-box formula
             vec3 bmin{p[6], p[7], p[8]}, bmax{p[9], p[10], p[11]};
             vec3 half = 0.5f * (bmax - bmin);
             vec3 cen = 0.5f * (bmax + bmin);

@@ -402,6 +402,43 @@ void reset_islands(jh_world* w) {
     w->isl_dirty = true;
 }
 
+// A convex hull's vertex block (JH_SHAPE_CONVEXHULL): V vertices, then (first, end) per vertex into
+// the neighbour list, contiguous from 0, then the list of neighbour indices below V, all within the
+// vertex array.  Only a well-formed graph keeps every hill climb inside the block.
+jh_status check_hull(const jh_shape& s, const float* vertices, int nv, int i) {
+    long V = s.vertex_count, off = s.vertex_offset;
+    if (V < 1 || V > 65535 || off < 0 || off + V > nv)
+        return fail(JH_ERR_INVALID_ARG, "shape %d: bad convex hull vertex range", i);
+    const float* base = vertices + 3 * off;
+    size_t avail = 3 * size_t(nv - off);
+    if (5 * size_t(V) > avail) return fail(JH_ERR_INVALID_ARG, "shape %d: convex hull adjacency out of bounds", i);
+    const float* adj = base + 3 * V;
+    auto as_index = [](float f, long& out) {
+        if (!(f >= 0.0f) || f > 16777216.0f || float(long(f)) != f) return false;
+        out = long(f);
+        return true;
+    };
+    long prev_end = 0, n_list = 0;
+    for (long k = 0; k < V; ++k) {
+        long lo, hi;
+        if (!as_index(adj[2 * k], lo) || !as_index(adj[2 * k + 1], hi) || lo != prev_end || hi < lo)
+            return fail(JH_ERR_INVALID_ARG, "shape %d: convex hull adjacency ranges are not contiguous", i);
+        prev_end = hi;
+    }
+    n_list = prev_end;
+    if (5 * size_t(V) + size_t(n_list) > avail)
+        return fail(JH_ERR_INVALID_ARG, "shape %d: convex hull neighbour list out of bounds", i);
+    const float* nbl = adj + 2 * V;
+    for (long k = 0; k < n_list; ++k) {
+        long x;
+        if (!as_index(nbl[k], x) || x >= V)
+            return fail(JH_ERR_INVALID_ARG, "shape %d: convex hull neighbour index out of range", i);
+    }
+    for (long k = 0; k < 3 * V; ++k)
+        if (!std::isfinite(base[k])) return fail(JH_ERR_NON_FINITE, "shape %d: non-finite hull vertex", i);
+    return JH_OK;
+}
+
 jh_status ensure_pair_buffers(jh_world* w, int n) {
     HIPCHK(w->pk0.ensure(n));
     HIPCHK(w->pk1.ensure(n));
@@ -1591,8 +1628,12 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
         if (i > 0 && !(shapes[i - 1].shape_id < s.shape_id))
             return fail(JH_ERR_INVALID_ARG, "shape %d: shape ids must be strictly ascending", i);
         if (s.body < 0 || s.body >= w->nb) return fail(JH_ERR_INVALID_ARG, "shape %d: body index out of range", i);
-        if (s.type < JH_SHAPE_BOX || s.type > JH_SHAPE_TRIANGLE)
+        if (s.type < JH_SHAPE_BOX || s.type > JH_SHAPE_CONVEXHULL)
             return fail(JH_ERR_INVALID_ARG, "shape %d: unknown type", i);
+        if (s.type == JH_SHAPE_CONVEXHULL) {
+            jh_status hs = check_hull(s, vertices, nv, i);
+            if (hs != JH_OK) return hs;
+        }
         if ((s.type == JH_SHAPE_POINTCLOUD && (s.vertex_count < 1)) || (s.type == JH_SHAPE_TRIANGLE && s.vertex_count != 3))
             return fail(JH_ERR_INVALID_ARG, "shape %d: bad vertex count", i);
         if ((s.type == JH_SHAPE_POINTCLOUD || s.type == JH_SHAPE_TRIANGLE) &&

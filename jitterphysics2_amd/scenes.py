@@ -16,7 +16,8 @@ import math
 import numpy as np
 
 from ._abi import (BODY_DTYPE, CONFIG_DTYPE, CONSTRAINT_DTYPE, FILTER_TRIANGLE_EDGE, MOTION_DYNAMIC, MOTION_STATIC,
-                   SHAPE_BOX, SHAPE_CAPSULE, SHAPE_DTYPE, SHAPE_POINTCLOUD, SHAPE_SPHERE, SHAPE_TRIANGLE,
+                   SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CONVEXHULL, SHAPE_DTYPE, SHAPE_POINTCLOUD, SHAPE_SPHERE,
+                   SHAPE_TRIANGLE,
                    default_config)
 
 f32 = np.float32
@@ -94,6 +95,134 @@ def capsule_mass(r, length):
         f32(2.0 / 5.0) * r * r + f32(1.0 / 4.0) * length * length + f32(3.0 / 8.0) * length * r)
     i22 = f32(1.0 / 2.0) * mc * r * r + f32(2.0 / 5.0) * ms * r * r
     return f32(mc + ms), (i11, f32(0), f32(0), i22, f32(0), i11)
+
+
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+
+
+def hull_graph(triangles):
+    """ConvexHullShape's vertex graph (ConvexHullShape.cs:89-168): the distinct triangle corners in
+    first-occurrence order (exact FP32 equality, -0 = +0), the triangles as vertex indices, and each
+    vertex's neighbours ascending without repeats."""
+    tri = np.asarray(triangles, np.float32).reshape(-1, 3, 3)
+    if len(tri) == 0:
+        raise ValueError("Triangle set must contain at least one triangle.")
+    index, verts, faces = {}, [], []
+    for t in tri:
+        f = []
+        for corner in t:
+            key = tuple(float(x) + 0.0 for x in corner)
+            if key not in index:
+                if len(verts) >= 65535:
+                    raise ValueError("The convex hull consists of too many vertices (>65535).")
+                index[key] = len(verts)
+                verts.append(corner.copy())
+            f.append(index[key])
+        faces.append(f)
+    nbrs = [[] for _ in verts]
+    for a, b, c in faces:
+        nbrs[a] += [b, c]
+        nbrs[b] += [a, c]
+        nbrs[c] += [a, b]
+    return np.array(verts, np.float32), np.array(faces, np.int64), [sorted(set(n)) for n in nbrs]
+
+
+def hull_support(verts, nbrs, d, shift=(0.0, 0.0, 0.0)):
+    """ConvexHullShape.InternalSupportMap (ConvexHullShape.cs:378-450) in FP32: the hill climb from
+    vertex 0 with the plateau pass, plus the shift."""
+    d = np.asarray(d, np.float32)
+    eps = f32(1e-12)
+    v = verts
+    cur = 0
+    dp = _dot(v[0], d)
+    while True:
+        verify, moved = False, False
+        vdir = np.ones(3, np.float32)
+        for nb in nbrs[cur]:
+            npd = _dot(v[nb], d)
+            if abs(npd - dp) < eps:
+                vdir = v[nb] - v[cur]
+                verify = True
+            if npd > dp + eps:
+                dp, cur, moved = npd, nb, True
+                break
+        if moved:
+            continue
+        if not verify:
+            break
+        d0 = _dot(vdir, v[cur])
+        back = False
+        while True:
+            step = False
+            for nb in nbrs[cur]:
+                npd = _dot(v[nb], d)
+                if npd > dp + eps:
+                    dp, cur, back = npd, nb, True
+                    break
+                if abs(npd - dp) < eps:
+                    d1 = _dot(vdir, v[nb])
+                    if d1 > d0 + eps:
+                        d0, cur, step = d1, nb, True
+                        break
+            if back or not step:
+                break
+        if not back:
+            break
+    return v[cur] + np.asarray(shift, np.float32)
+
+
+def hull_mass(verts, faces, shift=(0.0, 0.0, 0.0)):
+    """ConvexHullShape.CalculateMassInertia (ConvexHullShape.cs:262-326) in FP32: signed tetrahedra from
+    the vertex mean to each face (winding fixed against that point), the canonical tetrahedron's second
+    moments mapped by each face's corner matrix; returns (mass, inertia about the origin (6), centre of
+    mass).  Density 1."""
+    sh = np.asarray(shift, np.float32)
+    a, b = f32(1.0 / 60.0), f32(1.0 / 120.0)
+    can = (a, b, b, a, b, a)
+    within = np.zeros(3, np.float32)
+    for p in verts:
+        within = within + p
+    within = within * (f32(1.0) / f32(len(verts))) + sh
+    inertia = [f32(0)] * 6
+    center = np.zeros(3, np.float32)
+    mass = f32(0)
+    for ia, ib, ic in faces:
+        c0, c1, c2 = verts[ia] + sh, verts[ib] + sh, verts[ic] + sh
+        n = np.cross(c1 - c0, c2 - c0).astype(np.float32)
+        if _dot(n, c0 - within) < f32(0):
+            c0, c1 = c1, c0
+        M = np.array([[c0[0], c1[0], c2[0]], [c0[1], c1[1], c2[1]], [c0[2], c1[2], c2[2]]], np.float32)
+        det = (M[0, 0] * M[1, 1] * M[2, 2] + M[0, 1] * M[1, 2] * M[2, 0] + M[0, 2] * M[1, 0] * M[2, 1]
+               - M[2, 0] * M[1, 1] * M[0, 2] - M[2, 1] * M[1, 2] * M[0, 0] - M[2, 2] * M[1, 0] * M[0, 1])
+        m11, m12, m13, m22, m23, m33 = can
+        prod = []
+        for r in M:  # JVector.Transform(row, canonical)
+            prod.append(np.array([m11 * r[0] + m12 * r[1] + m13 * r[2], m12 * r[0] + m22 * r[1] + m23 * r[2],
+                                  m13 * r[0] + m23 * r[1] + m33 * r[2]], np.float32))
+        t = [_dot(prod[0], M[0]), _dot(prod[0], M[1]), _dot(prod[0], M[2]), _dot(prod[1], M[1]),
+             _dot(prod[1], M[2]), _dot(prod[2], M[2])]
+        inertia = [inertia[k] + t[k] * det for k in range(6)]
+        tmass = f32(1.0 / 6.0) * det
+        center = center + tmass * (f32(0.25) * (c0 + c1 + c2))
+        mass = mass + tmass
+    if not np.isfinite(mass) or mass <= f32(1e-12):
+        raise ValueError("Convex hull must define a non-degenerate volume.")
+    tr = inertia[0] + inertia[3] + inertia[5]
+    inertia = [tr - inertia[0], -inertia[1], -inertia[2], tr - inertia[3], -inertia[4], tr - inertia[5]]
+    return mass, tuple(f32(x) for x in inertia), (center * (f32(1.0) / mass)).astype(np.float32)
+
+
+def hull_block(verts, nbrs):
+    """The vertex-array block of a JH_SHAPE_CONVEXHULL shape: vertices, (first, end) per vertex,
+    neighbour list, padded to whole vertices."""
+    ranges, flat = [], []
+    for n in nbrs:
+        ranges += [len(flat), len(flat) + len(n)]
+        flat += n
+    block = np.concatenate([np.asarray(verts, np.float32).ravel(), np.array(ranges + flat, np.float32)])
+    pad = (-len(block)) % 3
+    return np.concatenate([block, np.zeros(pad, np.float32)])
 
 
 class SceneBuilder:
@@ -187,6 +316,34 @@ class SceneBuilder:
         lo, hi = pts.min(axis=0), pts.max(axis=0)
         r = float(np.mean(np.linalg.norm(pts.astype(np.float64) - c, axis=1)))
         return self._shape(body, SHAPE_POINTCLOUD, [*c, 0.0, 0.0, 0.0, *lo, *hi], sphere_mass(r), verts=pts)
+
+    def convex_hull(self, body: int, triangles, shift=(0.0, 0.0, 0.0)) -> int:
+        """ConvexHullShape from surface triangles (n, 3, 3) (ConvexHullShape.cs:89-168): params = centre
+        of mass (GetCenter), shift, cached local box (CalcInitBox: the support along +-x/y/z,
+        :352-376); the vertex block holds the vertices and their adjacency.  Mass properties:
+        CalculateMassInertia, body inputs as SetMassInertia sets them (RigidBody.cs:1575-1608)."""
+        verts, faces, nbrs = hull_graph(triangles)
+        mass, inertia, com = hull_mass(verts, faces, shift)
+        e = np.eye(3, dtype=np.float32)
+        hi = [hull_support(verts, nbrs, e[k], shift)[k] for k in range(3)]
+        lo = [hull_support(verts, nbrs, -e[k], shift)[k] for k in range(3)]
+        block = hull_block(verts, nbrs)
+        s = np.zeros(1, SHAPE_DTYPE)[0]
+        s["shape_id"] = self._next_id
+        self._next_id += 1
+        s["body"] = body
+        s["type"] = SHAPE_CONVEXHULL
+        p = np.zeros(12, np.float32)
+        p[0:3], p[3:6], p[6:9], p[9:12] = com, shift, lo, hi
+        s["params"] = p
+        s["vertex_offset"] = self.n_vertices()
+        s["vertex_count"] = len(verts)
+        self.vertices.extend(block.tolist())
+        self.shapes.append(s)
+        b = self.bodies[body]
+        b["inverse_inertia"] = _sym_inverse(*inertia)
+        b["inverse_mass"] = f32(1.0) / f32(mass)
+        return self.n_shapes() - 1
 
     def triangle_mesh(self, body: int, vertices, triangles) -> np.ndarray:
         """TriangleShapes of a TriangleMesh (TriangleMesh.cs:117-319) on a static body: one shape per

@@ -18,6 +18,62 @@ template <class R> struct Shape {
     const float* verts = nullptr;  // point-cloud / triangle vertices (FP32 as uploaded)
     int nverts = 0;
 
+    // ConvexHullShape.InternalSupportMap: hill climb from vertex 0 over the hull's vertex graph
+    // (adjacency after the vertices: per vertex (first, end) into the neighbour list, then the list),
+    // with the plateau pass; capped at nverts + 1 moves, never reached by a valid hull.
+    V3<R> hull_support(V3<R> d) const {
+        const float* adj = verts + 3 * nverts;
+        const float* nbl = adj + 2 * nverts;
+        auto vx = [&](int i) { return V3<R>{R(verts[3 * i]), R(verts[3 * i + 1]), R(verts[3 * i + 2])}; };
+        const R eps = R(1e-12);
+        int cur = 0;
+        R dp = dot(vx(0), d);
+        int moves = 0;
+    main_pass:
+        if (moves > nverts) return vx(cur);
+        {
+            bool verify = false;
+            V3<R> vdir{R(1), R(1), R(1)};  // JVector.Arbitrary
+            for (int i = int(adj[2 * cur]); i < int(adj[2 * cur + 1]); ++i) {
+                int nb = int(nbl[i]);
+                R np = dot(vx(nb), d);
+                if (std::fabs(np - dp) < eps) {
+                    vdir = vx(nb) - vx(cur);
+                    verify = true;
+                }
+                if (np > dp + eps) {
+                    dp = np;
+                    cur = nb;
+                    ++moves;
+                    goto main_pass;
+                }
+            }
+            if (!verify) return vx(cur);
+            R d0 = dot(vdir, vx(cur));
+        secondary:
+            if (moves > nverts) return vx(cur);
+            ++moves;
+            for (int i = int(adj[2 * cur]); i < int(adj[2 * cur + 1]); ++i) {
+                int nb = int(nbl[i]);
+                R np = dot(vx(nb), d);
+                if (np > dp + eps) {
+                    dp = np;
+                    cur = nb;
+                    goto main_pass;
+                }
+                if (std::fabs(np - dp) < eps) {
+                    R d1 = dot(vdir, vx(nb));
+                    if (d1 > d0 + eps) {
+                        d0 = d1;
+                        cur = nb;
+                        goto secondary;
+                    }
+                }
+            }
+        }
+        return vx(cur);
+    }
+
     // ISupportMappable.SupportMap (local frame, direction not normalised).
     V3<R> support(V3<R> d) const {
         switch (type) {
@@ -42,6 +98,8 @@ template <class R> struct Shape {
                 V3<R> r{R(verts[3 * idx]), R(verts[3 * idx + 1]), R(verts[3 * idx + 2])};
                 return r + V3<R>{p[3], p[4], p[5]};
             }
+            case JH_SHAPE_CONVEXHULL:  // ConvexHullShape.InternalSupportMap (ConvexHullShape.cs:378-450) + shift
+                return hull_support(d) + V3<R>{p[3], p[4], p[5]};
             default: {  // TriangleShape.SupportMap (TriangleShape.cs:167-191)
                 V3<R> a{R(verts[0]), R(verts[1]), R(verts[2])};
                 V3<R> b{R(verts[3]), R(verts[4]), R(verts[5])};
@@ -58,7 +116,7 @@ template <class R> struct Shape {
     }
     // ISupportMappable.GetCenter.
     V3<R> center() const {
-        if (type == JH_SHAPE_POINTCLOUD) return {p[0], p[1], p[2]};
+        if (type == JH_SHAPE_POINTCLOUD || type == JH_SHAPE_CONVEXHULL) return {p[0], p[1], p[2]};
         if (type == JH_SHAPE_TRIANGLE) {
             V3<R> a{R(verts[0]), R(verts[1]), R(verts[2])};
             V3<R> b{R(verts[3]), R(verts[4]), R(verts[5])};
@@ -90,7 +148,8 @@ template <class R> struct Shape {
                 mx = mx + pos;
                 return;
             }
-            case JH_SHAPE_POINTCLOUD: {  // PointCloudShape.cs:140-154
+            case JH_SHAPE_POINTCLOUD:    // PointCloudShape.cs:140-154
+            case JH_SHAPE_CONVEXHULL: {  // ConvexHullShape.cs:334-349 (the same formula)
                 V3<R> bmin{p[6], p[7], p[8]}, bmax{p[9], p[10], p[11]};
                 V3<R> half = R(0.5) * (bmax - bmin);
                 V3<R> cen = R(0.5) * (bmax + bmin);
