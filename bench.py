@@ -18,7 +18,9 @@ Besides the JSON line's contract fields it reports:
   cpu_baseline — the oracle (CPU restatement) on the host's threads (OMP_NUM_THREADS), timed on a
                  bounded sample: a few steps of the same heap continued from the GPU state at the
                  start of the timed region (the narrowphase and per-body work in parallel, the
-                 Gauss-Seidel solve sequential).
+                 Gauss-Seidel solve sequential).  Its fp64_error: the FP32 error of one GPU step
+                 (taken just before the timed region) against the FP64 oracle stepping the same state
+                 in the GPU's colour order, over every dynamic body.
   learned_warm_start — residual probes of the settled heap (see learned_warm_start_leg).
 
 Other workloads (--workload): pyramid, batched, full (the other configurations), joints (4,096 jointed
@@ -311,6 +313,14 @@ def main() -> int:
               f"{tuning['ms_solve_default']:.2f} default), colour blocks {tuning['colour_blocks']} "
               f"({tuning['ms_colour_best']:.2f} ms vs {tuning['ms_colour_default']:.2f})", flush=True)
 
+    ref = None
+    if not args.no_cpu:
+        # one untimed step whose FP32 result the FP64 oracle checks (cpu_baseline.fp64_error)
+        ref = {"bodies": gw.bodies(), "arbiters": gw.arbiters(),
+               "cons": gw.constraints() if cons is not None else None}
+        gw.step(dt)
+        ref["keys"] = gw.solve_order()[0]
+        ref["after"] = gw.bodies()
     start_bodies = gw.bodies()
     start_arbiters = gw.arbiters()
     start_cons = gw.constraints() if cons is not None else None
@@ -354,13 +364,36 @@ def main() -> int:
             ow.set_constraints(start_cons)
             ow.set_constraint_order(con_order)
         secs = ow.time_steps(dt, args.cpu_steps)
+        del ow
+        # FP32 error of one GPU step against the FP64 oracle (Jitter2.Double) replaying it from the same
+        # state in the GPU's solve order
+        o64 = OracleWorld(ref["bodies"], shapes, verts, cfg, fp64=True)
+        o64.set_arbiters(ref["arbiters"])
+        if ref["cons"] is not None:
+            o64.set_constraints(ref["cons"])
+            o64.set_constraint_order(con_order)
+        o64.step(dt, ref["keys"])
+        s64 = o64.state64()
+        del o64
+        g = ref["after"]
+        dyn = bodies["motion_type"] == 0
+        dp = np.abs(g["position"].astype(np.float64) - s64[:, 0:3])[dyn]
+        dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10])[dyn]
+        dw = np.abs(g["angular_velocity"].astype(np.float64) - s64[:, 10:13])[dyn]
+        speed = np.linalg.norm(s64[dyn, 7:10], axis=1)
+        fp64 = {"steps": 1, "bodies": int(dyn.sum()), "max_abs_dp_m": float(dp.max(initial=0.0)),
+                "max_abs_dv_mps": float(dv.max(initial=0.0)), "max_abs_dw_radps": float(dw.max(initial=0.0)),
+                "rms_dv_mps": float(np.sqrt(np.mean(dv ** 2))) if dv.size else 0.0,
+                "max_speed_mps": float(speed.max(initial=0.0)),
+                "what": "FP32 GPU step vs the FP64 oracle stepping the same state in the GPU's colour order"}
         orc.set_threads(1)
         cpu = {"value": n_dyn * args.cpu_steps / secs, "unit": "body-steps/s", "cores": threads, "kind": "port",
                "label": "restatement CPU (the reference, C#, cannot run here)",
                "sample": f"{args.cpu_steps} steps of the same {n_dyn}-body scene continued from the GPU state at "
                          f"the start of the timed region: the oracle (C++ restatement, FP32) on {threads} host "
                          f"threads, narrowphase, boxes, integration and body update in parallel, Gauss-Seidel "
-                         f"solve sequential in key order ({secs:.1f} s)"}
+                         f"solve sequential in key order ({secs:.1f} s)",
+               "fp64_error": fp64}
 
     kernel = {0: "k_solve", 1: "k_solve_flow", 2: "k_solve_worlds", 3: "k_solve_tiles",
               4: "k_solve_lanes"}[int(final["solver"])]

@@ -97,8 +97,12 @@ int main(int argc, char** argv) {
     printf("# warmup %d steps done\n", warmup);
     // launch shapes as the benchmark picks them (2 autotune rounds, which advance the simulation)
     jh_tuning tune;
-    if ((s = jh_world_autotune(w, dt, 2, &tune))) return die("jh_world_autotune", s);
-    printf("# autotune: solver %d x %d, colour blocks %d\n", tune.solver, tune.solve_blocks, tune.colour_blocks);
+    if (getenv("JH_NO_AUTOTUNE")) {
+        printf("# autotune skipped (JH_NO_AUTOTUNE)\n");
+    } else {
+        if ((s = jh_world_autotune(w, dt, 2, &tune))) return die("jh_world_autotune", s);
+        printf("# autotune: solver %d x %d, colour blocks %d\n", tune.solver, tune.solve_blocks, tune.colour_blocks);
+    }
     fflush(stdout);
     double sum[9] = {0};
     jh_stats st;

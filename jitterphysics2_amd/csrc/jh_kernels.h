@@ -1955,18 +1955,21 @@ __device__ __forceinline__ void store_slot(jh_contact& c, const SlotReg& r, bool
     for (int l = 0; l < 3; ++l) c.accumulated[l] = r.acc[l];
 }
 
-// Same arithmetic as prepare_contact, on register copies.
-__device__ __forceinline__ void prepare_reg(uint32_t mode, float specrel, SlotReg& c, BodyReg& B1, BodyReg& B2,
-                                            vec3 x1, quat q1, vec3 x2, quat q2, const StepTime& ts) {
-    bool skip = (c.flag & 4u) != 0;
-    c.flag &= ~4u;
-    float ws[3] = {skip ? 0.0f : c.acc[0], skip ? 0.0f : c.acc[1], skip ? 0.0f : c.acc[2]};
+// Same arithmetic as prepare_contact, on register copies, in two parts.  prepare_fixed computes what
+// depends only on the poses, the inertia and the contact frame (penetration, effective masses,
+// penalty bias), which no other visit of the solve changes; prepare_warm applies the part that
+// depends on the mutable fields (flags, bias, accumulated impulses) and on the velocities.  The
+// dataflow solver runs the first before it waits for its inputs.
+struct SlotPrep {
+    float pen, penalty, mass[3];
+};
+__device__ __forceinline__ void prepare_fixed(uint32_t mode, const SlotReg& c, const BodyReg& B1, const BodyReg& B2,
+                                              vec3 x1, quat q1, vec3 x2, quat q2, const StepTime& ts, SlotPrep& sp) {
     vec3 cr1 = rot(c.pos1, q1);
     vec3 cr2 = rot(c.pos2, q2);
     vec3 imv{0.0f, 0.0f, 0.0f};
     if (mode & JH_SOLVE_LINEAR1) imv = B1.imv;
     if (mode & JH_SOLVE_LINEAR2) imv = imv + B2.imv;
-    if ((c.flag & 2u) == 0) c.bias = 0.0f;
     vec3 r1 = cr1 + x1, r2 = cr2 + x2;
     vec3 dist = r1 - r2;
     const float* NX = c.nx;
@@ -1977,67 +1980,101 @@ __device__ __forceinline__ void prepare_reg(uint32_t mode, float specrel, SlotRe
         for (int l = 0; l < 3; ++l) k[l] = (NX[l] * imv.x) * NX[l] + (NY[l] * imv.y) * NY[l] + (NZ[l] * imv.z) * NZ[l];
     vec3 n{NX[0], NY[0], NZ[0]};
     float pen = dot(dist, n);
-    if (pen < -0.02f) c.bias = pen * ts.inv_dt * specrel;
-    c.flag &= ~2u;
+    sp.pen = pen;
     float ktx[3] = {0, 0, 0}, kty[3] = {0, 0, 0}, ktz[3] = {0, 0, 0};
-    vec3 lin{ws[0] * NX[0] + ws[1] * NX[1] + ws[2] * NX[2], ws[0] * NY[0] + ws[1] * NY[1] + ws[2] * NY[2],
-             ws[0] * NZ[0] + ws[1] * NZ[1] + ws[2] * NZ[2]};
-    if (mode & JH_SOLVE_LINEAR1) B1.v = B1.v - cmul(lin, B1.imv);
     if (mode & JH_SOLVE_ANGULAR1) {
         vec3 rp = c.rel1;
         const sym3& I = B1.I;
-        float e1[3], e2[3], e3[3];
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
             float rrx = rp.y * NZ[l] - rp.z * NY[l];
             float rry = rp.z * NX[l] - rp.x * NZ[l];
             float rrz = rp.x * NY[l] - rp.y * NX[l];
-            e1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
-            e2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
-            e3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
-        }
-        vec3 ai{ws[0] * e1[0] + ws[1] * e1[1] + ws[2] * e1[2], ws[0] * e2[0] + ws[1] * e2[1] + ws[2] * e2[2],
-                ws[0] * e3[0] + ws[1] * e3[1] + ws[2] * e3[2]};
-        B1.w = B1.w - ai;
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-            ktx[l] = e2[l] * rp.z - e3[l] * rp.y;
-            kty[l] = e3[l] * rp.x - e1[l] * rp.z;
-            ktz[l] = e1[l] * rp.y - e2[l] * rp.x;
+            float e1 = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+            float e2 = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+            float e3 = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+            ktx[l] = e2 * rp.z - e3 * rp.y;
+            kty[l] = e3 * rp.x - e1 * rp.z;
+            ktz[l] = e1 * rp.y - e2 * rp.x;
         }
     }
-    if (mode & JH_SOLVE_LINEAR2) B2.v = B2.v + cmul(lin, B2.imv);
     if (mode & JH_SOLVE_ANGULAR2) {
         vec3 rp = c.rel2;
         const sym3& I = B2.I;
-        float f1[3], f2[3], f3[3];
 #pragma unroll
         for (int l = 0; l < 3; ++l) {
             float rrx = rp.y * NZ[l] - rp.z * NY[l];
             float rry = rp.z * NX[l] - rp.x * NZ[l];
             float rrz = rp.x * NY[l] - rp.y * NX[l];
-            f1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
-            f2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
-            f3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
-        }
-        vec3 ai{ws[0] * f1[0] + ws[1] * f1[1] + ws[2] * f1[2], ws[0] * f2[0] + ws[1] * f2[1] + ws[2] * f2[2],
-                ws[0] * f3[0] + ws[1] * f3[1] + ws[2] * f3[2]};
-        B2.w = B2.w + ai;
-#pragma unroll
-        for (int l = 0; l < 3; ++l) {
-            ktx[l] = ktx[l] + (f2[l] * rp.z - f3[l] * rp.y);
-            kty[l] = kty[l] + (f3[l] * rp.x - f1[l] * rp.z);
-            ktz[l] = ktz[l] + (f1[l] * rp.y - f2[l] * rp.x);
+            float f1 = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+            float f2 = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+            float f3 = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+            ktx[l] = ktx[l] + (f2 * rp.z - f3 * rp.y);
+            kty[l] = kty[l] + (f3 * rp.x - f1 * rp.z);
+            ktz[l] = ktz[l] + (f1 * rp.y - f2 * rp.x);
         }
     }
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
         float kres = NX[l] * ktx[l] + NY[l] * kty[l] + NZ[l] * ktz[l];
         float kk = k[l] + kres;
-        c.mass[l] = kk > 0.0f ? 1.0f / kk : 0.0f;
+        sp.mass[l] = kk > 0.0f ? 1.0f / kk : 0.0f;
     }
     float pb = 0.2f * ts.inv_dt * fmaxr(0.0f, pen - 0.01f);
-    c.penalty = fminr(pb, 100.0f);
+    sp.penalty = fminr(pb, 100.0f);
+}
+// I (rp x n_l) for the three lanes; the inputs pass through an empty asm so that a visit recomputes
+// these after its wait instead of keeping the copies prepare_fixed made alive across it.
+__device__ __forceinline__ void lever_inertia(vec3 rp, sym3 I, const float* NX, const float* NY, const float* NZ,
+                                              float e1[3], float e2[3], float e3[3]) {
+    asm volatile("" : "+v"(rp.x), "+v"(rp.y), "+v"(rp.z));
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        float rrx = rp.y * NZ[l] - rp.z * NY[l];
+        float rry = rp.z * NX[l] - rp.x * NZ[l];
+        float rrz = rp.x * NY[l] - rp.y * NX[l];
+        e1[l] = I.m11 * rrx + I.m12 * rry + I.m13 * rrz;
+        e2[l] = I.m12 * rrx + I.m22 * rry + I.m23 * rrz;
+        e3[l] = I.m13 * rrx + I.m23 * rry + I.m33 * rrz;
+    }
+}
+__device__ __forceinline__ void prepare_warm(uint32_t mode, float specrel, SlotReg& c, BodyReg& B1, BodyReg& B2,
+                                             const SlotPrep& sp, const StepTime& ts) {
+    bool skip = (c.flag & 4u) != 0;
+    c.flag &= ~4u;
+    float ws[3] = {skip ? 0.0f : c.acc[0], skip ? 0.0f : c.acc[1], skip ? 0.0f : c.acc[2]};
+    if ((c.flag & 2u) == 0) c.bias = 0.0f;
+    if (sp.pen < -0.02f) c.bias = sp.pen * ts.inv_dt * specrel;
+    c.flag &= ~2u;
+    c.penalty = sp.penalty;
+    for (int l = 0; l < 3; ++l) c.mass[l] = sp.mass[l];
+    const float* NX = c.nx;
+    const float* NY = c.ny;
+    const float* NZ = c.nz;
+    vec3 lin{ws[0] * NX[0] + ws[1] * NX[1] + ws[2] * NX[2], ws[0] * NY[0] + ws[1] * NY[1] + ws[2] * NY[2],
+             ws[0] * NZ[0] + ws[1] * NZ[1] + ws[2] * NZ[2]};
+    if (mode & JH_SOLVE_LINEAR1) B1.v = B1.v - cmul(lin, B1.imv);
+    if (mode & JH_SOLVE_ANGULAR1) {
+        float e1[3], e2[3], e3[3];
+        lever_inertia(c.rel1, B1.I, NX, NY, NZ, e1, e2, e3);
+        vec3 ai{ws[0] * e1[0] + ws[1] * e1[1] + ws[2] * e1[2], ws[0] * e2[0] + ws[1] * e2[1] + ws[2] * e2[2],
+                ws[0] * e3[0] + ws[1] * e3[1] + ws[2] * e3[2]};
+        B1.w = B1.w - ai;
+    }
+    if (mode & JH_SOLVE_LINEAR2) B2.v = B2.v + cmul(lin, B2.imv);
+    if (mode & JH_SOLVE_ANGULAR2) {
+        float f1[3], f2[3], f3[3];
+        lever_inertia(c.rel2, B2.I, NX, NY, NZ, f1, f2, f3);
+        vec3 ai{ws[0] * f1[0] + ws[1] * f1[1] + ws[2] * f1[2], ws[0] * f2[0] + ws[1] * f2[1] + ws[2] * f2[2],
+                ws[0] * f3[0] + ws[1] * f3[1] + ws[2] * f3[2]};
+        B2.w = B2.w + ai;
+    }
+}
+__device__ __forceinline__ void prepare_reg(uint32_t mode, float specrel, SlotReg& c, BodyReg& B1, BodyReg& B2,
+                                            vec3 x1, quat q1, vec3 x2, quat q2, const StepTime& ts) {
+    SlotPrep sp;
+    prepare_fixed(mode, c, B1, B2, x1, q1, x2, q2, ts, sp);
+    prepare_warm(mode, specrel, c, B1, B2, sp, ts);
 }
 
 // Same arithmetic as iterate_contact, on register copies.
@@ -2089,32 +2126,34 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t mv_rsrc(const ContactSoA& S, i
     return __builtin_amdgcn_make_buffer_rsrc((void*)(S.mv + size_t(s) * 3 * S.cap), 0, 3 * 16 * S.cap, kBufferDword3);
 }
 __device__ __forceinline__ bool load_mutable(const ContactSoA& S, int s, int k, SlotReg& c, int want_visit,
-                                             int want_prepare) {
+                                             int want_prepare, bool prepare) {
     __amdgpu_buffer_rsrc_t r = mv_rsrc(S, s);
     int o = k * 16;
     float4 m0 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o, 0, kCohSc1));
     float4 m1 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + S.cap * 16, 0, kCohSc1));
     float4 m2 = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, o + 2 * S.cap * 16, 0, kCohSc1));
     c.acc[0] = m0.x; c.acc[1] = m0.y; c.acc[2] = m0.z;
-    c.bias = m1.x; c.penalty = m1.y; c.mass[0] = m1.z;
-    c.mass[1] = m2.x; c.mass[2] = m2.y; c.flag = __float_as_uint(m2.z);
+    c.bias = m1.x; c.flag = __float_as_uint(m2.z);
+    if (!prepare) { c.penalty = m1.y; c.mass[0] = m1.z; c.mass[1] = m2.x; c.mass[2] = m2.y; }  // a prepare recomputes them
     return __float_as_int(m0.w) == want_visit && __float_as_int(m1.w) == want_prepare &&
            __float_as_int(m2.w) == want_prepare;
 }
-__device__ __forceinline__ void store_mutable(const ContactSoA& S, int s, int k, const SlotReg& c, int g,
-                                              bool prepare) {
+// The fields travel by value: through a reference the compiler widens the three accumulated-impulse
+// loads into one 16-byte load that reaches into the next field, which keeps the caller's register copy
+// of the contact in scratch memory.
+__device__ __forceinline__ void store_mutable(const ContactSoA& S, int s, int k, int g, bool prepare, float a0, float a1,
+                                              float a2, float bias, float penalty, float m0, float m1, float m2,
+                                              uint32_t flag) {
     typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
     __amdgpu_buffer_rsrc_t r = mv_rsrc(S, s);
     int o = k * 16;
     float vg = __int_as_float(g);
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(c.acc[0], c.acc[1], c.acc[2], vg)), r,
-                                           o, 0, kCohSc1);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(a0, a1, a2, vg)), r, o, 0, kCohSc1);
     if (prepare) {
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(c.bias, c.penalty, c.mass[0], vg)),
-                                               r, o + S.cap * 16, 0, kCohSc1);
-        __builtin_amdgcn_raw_buffer_store_b128(
-            __builtin_bit_cast(u32x4, make_float4(c.mass[1], c.mass[2], __uint_as_float(c.flag), vg)), r,
-            o + 2 * S.cap * 16, 0, kCohSc1);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(bias, penalty, m0, vg)), r,
+                                               o + S.cap * 16, 0, kCohSc1);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(m1, m2, __uint_as_float(flag), vg)),
+                                               r, o + 2 * S.cap * 16, 0, kCohSc1);
     }
 }
 // Only the fields no visit changes (contact frame, anchors).
@@ -2173,6 +2212,12 @@ __device__ __forceinline__ bool flow_visit(const SolveArgs& P, int k, int g, boo
         q1 = ldq(B.quat[b1]);
         q2 = ldq(B.quat[b2]);
     }
+    SlotPrep sp[4];
+    if (PREPARE) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            if (usage & (1u << s)) prepare_fixed(mode, sr[s], R1, R2, x1, q1, x2, q2, P.ts, sp[s]);
+    }
     if (!m1) ld_vw(P, b1, R1.v, R1.w);
     if (!m2) ld_vw(P, b2, R2.v, R2.w);
     __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
@@ -2189,7 +2234,7 @@ __device__ __forceinline__ bool flow_visit(const SolveArgs& P, int k, int g, boo
                 rows = true;
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
-                    if (usage & (1u << s)) rows = load_mutable(S, s, k, sr[s], want_visit, want_prep) && rows;
+                    if (usage & (1u << s)) rows = load_mutable(S, s, k, sr[s], want_visit, want_prep, PREPARE) && rows;
             }
             if (!ok1) ok1 = coh_poll_vw(vr, b1, need1, R1.v, R1.w);
             if (!ok2) ok2 = coh_poll_vw(vr, b2, need2, R2.v, R2.w);
@@ -2197,14 +2242,16 @@ __device__ __forceinline__ bool flow_visit(const SolveArgs& P, int k, int g, boo
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     if (!(usage & (1u << s))) continue;
-                    if (PREPARE) prepare_reg(mode, specrel, sr[s], R1, R2, x1, q1, x2, q2, P.ts);
+                    if (PREPARE) prepare_warm(mode, specrel, sr[s], R1, R2, sp[s], P.ts);
                     else iterate_reg(mode, friction, sr[s], R1, R2, apply_bias);
                 }
                 if (m1) coh_publish_vw(vr, b1, R1.v, R1.w, need1 + 1);
                 if (m2) coh_publish_vw(vr, b2, R2.v, R2.w, need2 + 1);
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
-                    if (usage & (1u << s)) store_mutable(S, s, k, sr[s], g, PREPARE);
+                    if (usage & (1u << s))
+                        store_mutable(S, s, k, g, PREPARE, sr[s].acc[0], sr[s].acc[1], sr[s].acc[2], sr[s].bias,
+                                      sr[s].penalty, sr[s].mass[0], sr[s].mass[1], sr[s].mass[2], sr[s].flag);
                 done = true;
             }
         }

@@ -1295,6 +1295,12 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     w->coop_probe = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_flow, 256, 0));
     w->flow_max = std::max(1, occ) * w->cu_count;
+    if (getenv("JH_AUTOTUNE_VERBOSE")) {
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, (const void*)k_solve_flow) == hipSuccess)
+            fprintf(stderr, "k_solve_flow: %d registers, %zu B static LDS, %zu B scratch, %d max threads; %d blocks per CU\n",
+                    fa.numRegs, fa.sharedSizeBytes, fa.localSizeBytes, fa.maxThreadsPerBlock, occ);
+    }
     w->flow_blocks = std::min(w->flow_max, w->cu_count);
     HIPCHK(w->bar.ensure(64));
     HIPCHK(hipMemset(w->bar.p, 0, 64 * sizeof(unsigned)));
@@ -2451,7 +2457,7 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         // narrow grids too: each grid barrier's cache write-back and invalidate grow with the workgroups
         for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n});
     } else {
-        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, 3 * cu / 4, cu, 2 * cu}, w->flow_max))
+        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, 3 * cu / 4, cu, 3 * cu / 2, 2 * cu}, w->flow_max))
             sc.push_back({SOLVER_FLOW, n});
         for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n});
         for (int T : {cu / 2, cu, 2 * cu})
@@ -2484,6 +2490,9 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         size_t bi = 0;
         for (size_t i = 1; i < sc.size(); ++i)
             if (tot[i] < tot[bi]) bi = i;
+        if (getenv("JH_AUTOTUNE_VERBOSE"))
+            for (size_t i = 0; i < sc.size(); ++i)
+                fprintf(stderr, "autotune: solver %d x %d: %.3f ms solve\n", sc[i].kind, sc[i].n, tot[i] / rounds);
         for (int k = 0; k < 5; ++k) width(k) = keep[k];
         apply(sc[bi]);
         out->ms_solve_best = tot[bi] / rounds;
