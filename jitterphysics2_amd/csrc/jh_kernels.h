@@ -1375,20 +1375,15 @@ __global__ void k_colour_keys(const int* __restrict__ colour, const jh_arbiter* 
     if (local) atomicAdd(contacts, local);
 }
 
-// Colour offsets, and chunk bases for the dataflow solver (64-arbiter chunks, serial colour: 1 each).
-__global__ void k_colour_offsets(const int* __restrict__ hist, int* __restrict__ offs, int* __restrict__ cbeg,
-                                 int* __restrict__ nchunks) {
+// Colour offsets of the colour-sorted order.
+__global__ void k_colour_offsets(const int* __restrict__ hist, int* __restrict__ offs) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        int s = 0, cs = 0;
+        int s = 0;
         for (int c = 0; c < 128; ++c) {
             offs[c] = s;
-            cbeg[c] = cs;
             s += hist[c];
-            cs += c == kMaxColours ? hist[c] : (hist[c] + 63) / 64;
         }
         offs[128] = s;
-        cbeg[128] = cs;
-        *nchunks = cs;
     }
 }
 
@@ -1598,6 +1593,8 @@ struct ContactSoA {
     uint32_t* flag;  // [4][cap]
     float* f;        // [4][kSoaFields][cap]
     float4* mv;      // [4][3][cap] versioned mutable fields (k_solve_flow), see load_mutable
+    int4* nd;        // [cap] k_solve_flow: update counts of body1 and body2 (-1: static to the solver) and the
+                     // arbiter's rank in each body's update sequence
     int cap;
     __device__ __forceinline__ float& at(int s, int fld, int k) const {
         return f[(size_t(s) * kSoaFields + fld) * cap + k];
@@ -1618,8 +1615,6 @@ struct SolveArgs {
     const uint64_t* order;   // (colour << 32) | arbiter, sorted
     const int* offs;         // 129 colour offsets
     const int* ncolours;
-    const int2* chunks;      // [begin, end) of the colour-sorted order; one colour per chunk
-    const int* nchunks;
     const int2* rank;        // per order position: rank of the arbiter in each body's update sequence
     const int* deg;          // per body: arbiters touching it
     int* queue;              // k_solve_flow: chunk counters, 2 per substep
@@ -1777,16 +1772,25 @@ __global__ void __launch_bounds__(256) k_solve(SolveArgs P) {
 // visits that precede it in the colour-ordered Gauss-Seidel sequence.  Each body therefore sees its
 // updates in the same order as the barrier solver (bit-identical results) without grid barriers
 // between colours.  The count travels inside the body's packed velocity record (see coh_poll_vw).
-// Waves own chunks statically (chunk j -> wave j mod W, for every pass), processed in global order,
-// so the earliest unfinished chunk always has its dependencies met: the loop cannot deadlock on a
-// cooperative (fully resident) grid.  Arbiter contact data stays with its owning wave.
+// Waves visit 64-position chunks of the colour order (handed out by flow_passes_queue,
+// flow_passes_own or flow_passes), each wave its chunks in increasing order, so the earliest
+// unfinished chunk always has its dependencies met: the loop cannot deadlock on a cooperative (fully
+// resident) grid.  k_soa_gather lays the arbiters out by colour-order position.
 __global__ void __launch_bounds__(256) k_soa_gather(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
-                                                    const int* __restrict__ offs, ContactSoA S) {
+                                                    const int* __restrict__ offs, ContactSoA S,
+                                                    const int2* __restrict__ rank = nullptr,
+                                                    const int* __restrict__ deg = nullptr,
+                                                    const int* __restrict__ motion = nullptr) {
     int A = offs[128];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
         const jh_arbiter& a = arb[uint32_t(order[k])];
         int n = __popc(a.usage_mask & 15u);
         S.bodies[k] = make_int2(a.body1, a.body2);
+        if (S.nd) {
+            int2 r = rank[k];
+            S.nd[k] = make_int4(motion[a.body1] != JH_MOTION_STATIC ? deg[a.body1] : -1,
+                                motion[a.body2] != JH_MOTION_STATIC ? deg[a.body2] : -1, r.x, r.y);
+        }
         S.um[k] = make_uint2((1u << n) - 1u, a.mode);  // used slots compacted to rows 0 .. n-1, in slot order
         S.fs[k] = make_float2(a.friction, a.speculative_relaxation_factor);
         for (int j = 0, s = 0; s < 4; ++s) {
@@ -2177,21 +2181,30 @@ __device__ __forceinline__ void load_slot_fixed(const ContactSoA& S, int s, int 
 // polled until they show the awaited visit counts / passes, then the visit computes in registers and
 // publishes both.
 // False when the launch is aborting (a wait gave up here or elsewhere).
+// What a visit needs first: the bodies, slots, mode, friction and update counts of the arbiter at
+// colour-order position k (inactive lanes of a wave's last chunk read position 0 and do nothing).
+struct FlowHead {
+    int2 bb;
+    uint2 um;
+    float2 fs;
+    int4 nd;
+};
+__device__ __forceinline__ int flow_pos(int k, bool active) { return active ? k : 0; }
+__device__ __forceinline__ FlowHead load_head(const ContactSoA& S, int k) {
+    return FlowHead{S.bodies[k], S.um[k], S.fs[k], S.nd[k]};
+}
 template <bool PREPARE>
-__device__ __forceinline__ bool flow_visit(const SolveArgs& P, int k, int g, bool apply_bias, bool active) {
+__device__ __forceinline__ bool flow_visit(const SolveArgs& P, const FlowHead& h, int k, int g, bool apply_bias,
+                                           bool active) {
     const Bodies& B = P.B;
-    if (!active) k = 0;  // inactive lanes of the wave's last chunk mirror a valid arbiter but do nothing
+    k = flow_pos(k, active);
     const ContactSoA& S = P.soa;
-    int2 bb = S.bodies[k];
-    uint2 um = S.um[k];
-    float2 fs = S.fs[k];
-    int b1 = bb.x, b2 = bb.y;
-    uint32_t usage = um.x, mode = um.y;
-    float friction = fs.x, specrel = fs.y;
-    bool m1 = B.motion[b1] != JH_MOTION_STATIC, m2 = B.motion[b2] != JH_MOTION_STATIC;
-    int2 r = P.rank[k];
-    int need1 = m1 ? g * P.deg[b1] + r.x : 0;
-    int need2 = m2 ? g * P.deg[b2] + r.y : 0;
+    int b1 = h.bb.x, b2 = h.bb.y;
+    uint32_t usage = h.um.x, mode = h.um.y;
+    float friction = h.fs.x, specrel = h.fs.y;
+    bool m1 = h.nd.x >= 0, m2 = h.nd.y >= 0;
+    int need1 = m1 ? g * h.nd.x + h.nd.z : 0;
+    int need2 = m2 ? g * h.nd.y + h.nd.w : 0;
     // passes the mutable fields must come from: the arbiter's previous visit (-1: the gather) and the
     // last prepare (for a prepare: the previous substep's, or the gather)
     int per_sub = 1 + P.iterations + P.relaxations;
@@ -2263,12 +2276,17 @@ __device__ __forceinline__ bool flow_visit(const SolveArgs& P, int k, int g, boo
 
 // Chunks of a phase handed out in global order from a counter, in every pass: a wave that frees up
 // takes the oldest chunk nobody has taken, so a ready chunk never waits behind a slow chunk of the
-// same wave.  Deadlock-free: the oldest unfinished chunk has been taken and its predecessors are done.
-// Exact because an arbiter's mutable fields are versioned (load_mutable), whichever wave visits it.
+// same wave.  Chunks are the 64-position blocks of the colour order, so a chunk's bounds follow from
+// the counter without a table load; a block that straddles two colours may hold an arbiter that
+// waits for another lane of the same wave, which the lane-wise wait loop of flow_visit serves.
+// Deadlock-free: the oldest unfinished chunk has been taken, and its predecessors (lower positions,
+// possibly in the same chunk) are done or progress.  Exact because an arbiter's mutable fields are
+// versioned (load_mutable), whichever wave visits it.
 __device__ __forceinline__ bool flow_passes_queue(const SolveArgs& P, int g_first, int npass, int phase_first,
                                                   int phase_rest, int* queue) {
     int lane = threadIdx.x & 63;
-    int J = *P.nchunks;
+    int A = P.offs[128];
+    int J = (A + 63) >> 6;
     int total = J * npass;
     while (true) {
         int t = 0;
@@ -2278,10 +2296,11 @@ __device__ __forceinline__ bool flow_passes_queue(const SolveArgs& P, int g_firs
         int p = t / J, j = t - p * J;
         int g = g_first + p;
         int phase = p == 0 ? phase_first : phase_rest;
-        int2 ch = P.chunks[j];
-        int k = ch.x + lane;
-        bool active = k < ch.y;
-        bool ok = phase == 0 ? flow_visit<true>(P, k, g, false, active) : flow_visit<false>(P, k, g, phase == 1, active);
+        int k = (j << 6) + lane;
+        bool active = k < A;
+        FlowHead h = load_head(P.soa, flow_pos(k, active));
+        bool ok = phase == 0 ? flow_visit<true>(P, h, k, g, false, active)
+                             : flow_visit<false>(P, h, k, g, phase == 1, active);
         if (!ok) return false;
     }
 }
@@ -2297,7 +2316,8 @@ __device__ __forceinline__ bool flow_passes(const SolveArgs& P, int g_first, int
 __device__ __forceinline__ bool flow_passes_own(const SolveArgs& P, int g_first, int npass, int phase_first,
                                                 int phase_rest, int* queue, int* own) {
     int lane = threadIdx.x & 63;
-    int J = *P.nchunks;
+    int A = P.offs[128];
+    int J = (A + 63) >> 6;  // 64-position chunks of the colour order
     if (J > ((gridDim.x * blockDim.x) >> 6) * kOwnMax) return flow_passes(P, g_first, npass, phase_first, phase_rest);
     int n = 0;
     for (int p = 0; p < npass; ++p) {
@@ -2317,10 +2337,11 @@ __device__ __forceinline__ bool flow_passes_own(const SolveArgs& P, int g_first,
                 if (i == n) break;
                 j = own[i];
             }
-            int2 ch = P.chunks[j];
-            int k = ch.x + lane;
-            bool active = k < ch.y;
-            bool ok = phase == 0 ? flow_visit<true>(P, k, g, false, active) : flow_visit<false>(P, k, g, phase == 1, active);
+            int k = (j << 6) + lane;
+            bool active = k < A;
+            FlowHead h = load_head(P.soa, flow_pos(k, active));
+            bool ok = phase == 0 ? flow_visit<true>(P, h, k, g, false, active)
+                                 : flow_visit<false>(P, h, k, g, phase == 1, active);
             if (!ok) return false;
         }
     }
@@ -2332,15 +2353,17 @@ __device__ __forceinline__ bool flow_passes(const SolveArgs& P, int g_first, int
     int lane = threadIdx.x & 63;
     int W = (gridDim.x * blockDim.x) >> 6;
     int gw = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    int J = *P.nchunks;
+    int A = P.offs[128];
+    int J = (A + 63) >> 6;  // 64-position chunks of the colour order
     for (int p = 0; p < npass; ++p) {
         int g = g_first + p;
         int phase = p == 0 ? phase_first : phase_rest;
         for (int j = gw; j < J; j += W) {
-            int2 ch = P.chunks[j];
-            int k = ch.x + lane;
-            bool active = k < ch.y;
-            bool ok = phase == 0 ? flow_visit<true>(P, k, g, false, active) : flow_visit<false>(P, k, g, phase == 1, active);
+            int k = (j << 6) + lane;
+            bool active = k < A;
+            FlowHead h = load_head(P.soa, flow_pos(k, active));
+            bool ok = phase == 0 ? flow_visit<true>(P, h, k, g, false, active)
+                                 : flow_visit<false>(P, h, k, g, phase == 1, active);
             if (!ok) return false;
         }
     }
@@ -2816,18 +2839,6 @@ __global__ void k_rank_assign(const uint64_t* __restrict__ keys, const int* __re
         }
     }
 }
-// Chunks of at most 64 arbiters of one colour; the serial overflow colour gets one arbiter per chunk.
-__global__ void k_chunks(const uint64_t* __restrict__ order, const int* __restrict__ offs, const int* __restrict__ cbeg,
-                         int2* __restrict__ chunks) {
-    int A = offs[128];
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
-        int c = int(order[k] >> 34);
-        int rel = k - offs[c];
-        if (c == kMaxColours) chunks[cbeg[c] + rel] = make_int2(k, k + 1);
-        else if ((rel & 63) == 0) chunks[cbeg[c] + (rel >> 6)] = make_int2(k, min(k + 64, offs[c + 1]));
-    }
-}
-
 // ---- Tile solver ----------------------------------------------------------------------------
 // One workgroup per tile: a contiguous range of the non-static bodies in broadphase (Morton) order.
 // Each arbiter belongs to the tile of its first non-static body.  A body is *private* when every

@@ -79,7 +79,7 @@ enum {
     C_NS = 10,   // shape count
     C_NB = 11,   // body count
     C_CONTACTS = 12,  // contact points solved
-    C_NCHUNK = 13,    // dataflow solver chunks
+    // 13: unused
     C_NRANK = 14,     // rank keys
     C_SPEC = 15,      // speculative sweep queue
     C_NMOV = 16,      // tile solver: non-static bodies with shapes
@@ -215,6 +215,7 @@ struct jh_world {
     DBuf<int2> soa_bodies;
     DBuf<uint2> soa_um;
     DBuf<float2> soa_fs;
+    DBuf<int4> soa_nd;
     DBuf<uint32_t> soa_flag;
     DBuf<float> soa_f;
     int soa_cap = 0;
@@ -243,8 +244,8 @@ struct jh_world {
     DBuf<uint64_t> ok0, ok1, solve_keys;
     // dataflow solver
     DBuf<uint64_t> rk0, rk1;
-    DBuf<int2> rank, chunks;
-    DBuf<int> deg, cbeg;
+    DBuf<int2> rank;
+    DBuf<int> deg;
     DBuf<unsigned> bar;       // grid barrier state and abort flag of the persistent kernels (64 words)
     unsigned spin_limit = 0;  // polls before a device wait gives up; 0: kSpinLimitDefault
     unsigned* habort = nullptr;  // pinned host mirror of the abort flag
@@ -539,6 +540,7 @@ jh_status ensure_step_buffers(jh_world* w) {
         HIPCHK(w->soa_bodies.ensure(cap));
         HIPCHK(w->soa_um.ensure(cap));
         HIPCHK(w->soa_fs.ensure(cap));
+        HIPCHK(w->soa_nd.ensure(cap));
         HIPCHK(w->soa_flag.ensure(4 * cap));
         HIPCHK(w->soa_f.ensure(4 * kSoaFields * cap));
         HIPCHK(w->soa_mv.ensure(4 * 3 * cap));
@@ -550,7 +552,6 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->rk0.ensure(2 * na));
     HIPCHK(w->rk1.ensure(2 * na));
     HIPCHK(w->rank.ensure(na));
-    HIPCHK(w->chunks.ensure(na));
     HIPCHK(w->flags.ensure(na));
     HIPCHK(w->offs.ensure(na));
     HIPCHK(w->sums.ensure(na / 256 + 2));
@@ -828,7 +829,7 @@ jh_status colour_arbiters(jh_world* w, int cur, int* a_cur, int amax_, const uin
     int csh[2] = {32, 40};  // (colour, contacts) in bits 32..41
     bool in1 = rsort::sort_pairs(w->ok0.p, w->ok1.p, nullptr, nullptr, a_cur, std::max(amax_, 1), csh, 2,
                                  w->hist.p, st);
-    k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p, w->cbeg.p, cnt + C_NCHUNK);
+    k_colour_offsets<<<1, 1, 0, st>>>(w->chist.p, w->coffs.p);
     const uint64_t* order = in1 ? w->ok1.p : w->ok0.p;
     k_solve_keys<<<grid_for(std::max(amax_, 1)), 256, 0, st>>>(order, w->akey[cur].p, a_cur, w->solve_keys.p);
     HIPCHK(hipGetLastError());
@@ -1016,7 +1017,6 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
                                           rsh, nrs, w->hist.p, st);
             k_rank_assign<<<grid_for(2 * std::max(amax_, 1)), 256, 0, st>>>(rin1 ? w->rk1.p : w->rk0.p, cnt + C_NRANK,
                                                                             w->rank.p, w->deg.p, cnt + C_MAXDEG);
-            k_chunks<<<ga, 256, 0, st>>>(order, w->coffs.p, w->cbeg.p, w->chunks.p);
             if (use_tiles) {
                 s = tile_setup(w, order, a_cur, amax_);
                 if (s != JH_OK) return s;
@@ -1029,8 +1029,6 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.order = order;
         S.offs = w->coffs.p;
         S.ncolours = cnt + C_NCOL;
-        S.chunks = w->chunks.p;
-        S.nchunks = cnt + C_NCHUNK;
         S.rank = w->rank.p;
         S.deg = w->deg.p;
         S.bar = w->bar.p;
@@ -1051,6 +1049,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         S.queue = w->fqueue.p;
         S.qmode = w->flow_qmode;
         S.soa.mv = kind == SOLVER_FLOW ? w->soa_mv.p : nullptr;
+        S.soa.nd = kind == SOLVER_FLOW ? w->soa_nd.p : nullptr;
         S.con.c = w->con.p;
         S.con.work = w->con_work.p;
         S.con.order = w->con_order.p;
@@ -1123,7 +1122,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             k_rec_scatter<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, w->rec.p);
         } else if (kind == SOLVER_FLOW) {
             int gsoa = grid_for(std::max(amax_, 1));
-            k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa);
+            k_soa_gather<<<gsoa, 256, 0, st>>>(S.arb, order, w->coffs.p, S.soa, S.rank, S.deg, S.B.motion);
             HIPCHK(hipEventRecord(w->ev[8], st));
             s = launch_coop((const void*)k_solve_flow, w->flow_blocks, 256, sargs, st);
             if (s != JH_OK) return s;
@@ -1304,7 +1303,6 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     w->flow_blocks = std::min(w->flow_max, w->cu_count);
     HIPCHK(w->bar.ensure(64));
     HIPCHK(hipMemset(w->bar.p, 0, 64 * sizeof(unsigned)));
-    HIPCHK(w->cbeg.ensure(130));
     // launch-shape overrides for tuning sweeps (clamped to the co-resident maximum)
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_lanes, 256, 0));
     w->lane_max = std::max(1, occ) * w->cu_count;
@@ -1341,7 +1339,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->ckey.release(); w->nkey.release(); w->cpair.release(); w->cfound.release();
     for (int k = 0; k < 2; ++k) { w->arb[k].release(); w->akey[k].release(); }
     w->colour.release(); w->cwork.release(); w->soa_bodies.release(); w->soa_um.release();
-    w->soa_fs.release(); w->soa_flag.release(); w->soa_f.release();
+    w->soa_fs.release(); w->soa_nd.release(); w->soa_flag.release(); w->soa_f.release();
     w->wbody.release(); w->wboff.release(); w->wsched.release(); w->wcells.release(); w->wqueue.release();
     w->whist.release(); w->woff.release(); w->wcur.release(); w->wsums.release(); w->wlist.release();
     w->tfirst.release(); w->tbody.release(); w->tile_of.release(); w->tslot.release(); w->tflags.release();
@@ -1349,8 +1347,8 @@ jh_status jh_world_destroy(jh_world* w) {
     w->tsums.release(); w->tlist.release(); w->rec.release();
     w->vwpack.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
-    w->rk0.release(); w->rk1.release(); w->rank.release(); w->chunks.release(); w->deg.release();
-    w->cbeg.release(); w->bar.release();
+    w->rk0.release(); w->rk1.release(); w->rank.release(); w->deg.release();
+    w->bar.release();
     w->lws.release(); w->pvel.release(); w->pang.release(); w->parb.release(); w->ppart.release();
     w->pfeat.release(); w->ptarg.release(); w->pvalid.release(); w->plam.release(); w->plearn.release();
     for (auto& e : w->ev) (void)hipEventDestroy(e);
