@@ -229,7 +229,7 @@ struct jh_world {
     int kind = SOLVER_FLOW;
     // lane solver (k_solve_lanes): workgroups and arbiter records
     int lane_blocks = 0, lane_max = 0;
-    int flow_qmode = 2;                // k_solve_flow chunk ownership (SolveArgs.qmode)
+    int flow_qmode = 1;                // k_solve_flow chunk ownership (SolveArgs.qmode)
     DBuf<float4> soa_mv;               // versioned mutable contact fields of k_solve_flow
     DBuf<int> fqueue;
     DBuf<float4> rec;

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from jitterphysics2_amd import scenes
+from jitterphysics2_amd._abi import MOTION_STATIC, default_config
 from jitterphysics2_amd.gpu import GpuWorld
 from tests.orc import OracleWorld
 
@@ -111,6 +112,35 @@ def test_dataflow_solver_launch_shapes(monkeypatch):
         for blocks in ("1", "7", "64"):
             monkeypatch.setenv("JH_SOLVE_BLOCKS", blocks)
             run_replay(scenes.heap(700, seed=11), steps=40, check_every=39)
+
+
+def hub_scene(n_side=18):
+    """A dynamic platform carrying n_side^2 spheres: the platform takes part in more arbiters than
+    there are colours, so the last batch is the serial overflow colour, whose arbiters all share the
+    platform and follow each other in the dataflow solver's chunks."""
+    sb = scenes.SceneBuilder()
+    g = sb.body((0.0, -0.5, 0.0), motion=MOTION_STATIC)
+    sb.box(g, (60.0, 1.0, 60.0))
+    p = sb.body((0.0, 0.25, 0.0))
+    sb.box(p, (30.0, 0.5, 30.0))
+    for i in range(n_side):
+        for j in range(n_side):
+            b = sb.body((1.5 * (i - n_side / 2), 0.9, 1.5 * (j - n_side / 2)))
+            sb.sphere(b, 0.4)
+    bodies, shapes, verts = sb.arrays()
+    cfg = default_config()
+    cfg["allow_deactivation"] = 0
+    return bodies, shapes, verts, cfg, 0.01
+
+
+@pytest.mark.parametrize("queue", ["0", "1", "2"])
+def test_dataflow_solver_serial_colour(monkeypatch, queue):
+    monkeypatch.setenv("JH_SOLVER", "flow")
+    monkeypatch.setenv("JH_FLOW_QUEUE", queue)
+    gw, ow = run_replay(hub_scene(), steps=12, check_every=3)
+    keys, offs = gw.solve_order()
+    assert len(offs) - 1 > gw.stats()["colours"], "the scene should overflow into the serial colour"
+    assert offs[-1] - offs[-2] > 64  # serial arbiters fill whole chunks
 
 
 @pytest.mark.parametrize("colour_blocks", ["default", "1", "16", "128"])
