@@ -483,20 +483,21 @@ jh_status jh_world_step_probe(jh_world* world, float dt, jh_probe* probe);
 
 /* ---- Launch-shape autotuner (no counterpart in the reference) --------------------------------
  * Times candidate workgroup counts of the persistent solver (1/16 .. 2x the CU count, clamped to what
- * can be co-resident), with several worlds also the world-local solver (one workgroup per world), and
- * the workgroup count of the colouring kernel (16 .. 512) on real steps, round-robin over the
- * candidates for `rounds` rounds each, and keeps the fastest.  The world is stepped
- * rounds * candidates times (the simulation advances); results are deterministic whatever the
- * launch shape. */
+ * can be co-resident), with several worlds also the world-local solver (one workgroup per world), the
+ * workgroup count of the colouring kernel (16 .. 512) and the pair traversal (wavefront or per-thread)
+ * on real steps, round-robin over the candidates for `rounds` rounds each, and keeps the fastest.  The
+ * world is stepped rounds * candidates times (the simulation advances); results are deterministic
+ * whatever the launch shape. */
 typedef struct jh_tuning {
     int32_t solver;            /* chosen solver kernel (as jh_stats.solver; 4 = lanes)           */
     int32_t solve_blocks;      /* its workgroups (tiles for the tile solver)                      */
     int32_t colour_blocks;     /* chosen colouring workgroups          */
     int32_t candidates;
     int32_t steps;             /* steps taken                          */
-    int32_t reserved;
+    int32_t traversal;         /* chosen pair traversal: 1 wavefront (k_traverse_wave), 0 per thread */
     double ms_solve_best, ms_solve_default;    /* mean solver time: chosen / default shape      */
     double ms_colour_best, ms_colour_default;  /* mean colouring time: chosen / default shape   */
+    double ms_broadphase_best, ms_broadphase_default;  /* mean broadphase time: chosen / wavefront */
 } jh_tuning;
 
 jh_status jh_world_autotune(jh_world* world, float dt, int32_t rounds, jh_tuning* out);

@@ -153,8 +153,9 @@ STATS_DTYPE = np.dtype([
 
 TUNING_DTYPE = np.dtype([
     ("solver", "<i4"), ("solve_blocks", "<i4"), ("colour_blocks", "<i4"), ("candidates", "<i4"), ("steps", "<i4"),
-    ("reserved", "<i4"),
+    ("traversal", "<i4"),
     ("ms_solve_best", "<f8"), ("ms_solve_default", "<f8"), ("ms_colour_best", "<f8"), ("ms_colour_default", "<f8"),
+    ("ms_broadphase_best", "<f8"), ("ms_broadphase_default", "<f8"),
 ])
 
 CONSTRAINT_BALL_SOCKET, CONSTRAINT_HINGE_ANGLE, CONSTRAINT_FIXED_ANGLE, CONSTRAINT_DISTANCE_LIMIT = 0, 1, 2, 3

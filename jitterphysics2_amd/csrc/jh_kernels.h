@@ -508,16 +508,37 @@ __device__ __forceinline__ float4 coh_ld4(const float4* p) {
 // Leaf boxes in sorted order, then bottom-up union of child boxes (second arrival continues).  A
 // thread publishes its box (coherent stores, waited for with s_waitcnt) before it bumps the node's
 // arrival counter; the second arrival reads both children with coherent loads.
+// Leaf j's boxes carry the leaf's identity in their w components (box tests read x, y, z only):
+// lmin[j].w = the shape index, lmax[j].w = its body | dynamic << 30 | awake << 31, so a traversal
+// tests a leaf with the two loads of its box and no dependent loads.
+//
+// Child records for k_traverse_wave: internal node c's two children in one 64-byte record plus 16
+// bytes, so a traversal step is one round of loads:
+//   nbox[4c + 0] = (left min xyz, left code)   nbox[4c + 1] = (left max xyz, 0)
+//   nbox[4c + 2] = (right min xyz, right code) nbox[4c + 3] = (right max xyz, 0)
+//   nrng[c] = (left a, left b, right a, right b)
+// code: internal node index, or ~j for leaf j; (a, b): an internal child's leaf range, a leaf
+// child's (shape index, packed body/dynamic/awake word).
+constexpr uint32_t kLeafBody = 0x3fffffffu, kLeafDyn = 1u << 30, kLeafAwake = 1u << 31;
+
 __global__ void k_lbvh_refit(int n, const uint32_t* __restrict__ order, const float4* __restrict__ amin,
                              const float4* __restrict__ amax, float4* __restrict__ lmin, float4* __restrict__ lmax,
                              const int* __restrict__ left, const int* __restrict__ right, const int* __restrict__ parent,
                              const int* __restrict__ leaf_parent, float4* __restrict__ nmin, float4* __restrict__ nmax,
-                             int* __restrict__ flags) {
+                             int* __restrict__ flags, const ShapeRec* __restrict__ shapes,
+                             const int* __restrict__ motion, const uint32_t* __restrict__ sflags,
+                             const int2* __restrict__ range, float4* __restrict__ nbox, int4* __restrict__ nrng) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     uint32_t s = order[j];
-    coh_st4(&lmin[j], amin[s]);
-    coh_st4(&lmax[j], amax[s]);
+    int b = shapes[s].body;
+    uint32_t info = uint32_t(b) | (motion[b] == JH_MOTION_DYNAMIC ? kLeafDyn : 0u) |
+                    ((sflags == nullptr || (sflags[b] & 1u) != 0u) ? kLeafAwake : 0u);
+    float4 mn = amin[s], mx = amax[s];
+    mn.w = __uint_as_float(s);
+    mx.w = __uint_as_float(info);
+    coh_st4(&lmin[j], mn);
+    coh_st4(&lmax[j], mx);
     if (n == 1) return;
     int node = leaf_parent[j];
     while (node >= 0) {
@@ -534,6 +555,16 @@ __global__ void k_lbvh_refit(int n, const uint32_t* __restrict__ order, const fl
         float4 b1 = coh_ld4(r < 0 ? &lmax[~r] : &nmax[r]);
         coh_st4(&nmin[node], make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0));
         coh_st4(&nmax[node], make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0));
+        if (nbox) {  // child record (read only by a later launch)
+            int2 gl = l < 0 ? make_int2(__float_as_int(a0.w), __float_as_int(a1.w)) : range[l];
+            int2 gr = r < 0 ? make_int2(__float_as_int(b0.w), __float_as_int(b1.w)) : range[r];
+            float4* rec = nbox + 4 * size_t(node);
+            rec[0] = make_float4(a0.x, a0.y, a0.z, __int_as_float(l));
+            rec[1] = make_float4(a1.x, a1.y, a1.z, 0.0f);
+            rec[2] = make_float4(b0.x, b0.y, b0.z, __int_as_float(r));
+            rec[3] = make_float4(b1.x, b1.y, b1.z, 0.0f);
+            nrng[node] = make_int4(gl.x, gl.y, gr.x, gr.y);
+        }
         node = parent[node];
     }
 }
@@ -627,12 +658,12 @@ __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restr
                 if (c < 0) {
                     int j = ~c;
                     if (j < lo || j >= hi || j == i) continue;
-                    if (box_disjoint(qmn, qmx, lmin[j], lmax[j])) continue;
-                    uint32_t sj = order[j];
-                    int bj = shapes[sj].body;
-                    if (bj == bi) continue;
-                    if (motion[bj] == JH_MOTION_DYNAMIC && j < i) continue;
-                    if (!awake_i && !(sflags[bj] & 1u)) continue;  // both asleep (World.Detect.cs:158)
+                    float4 mn = lmin[j], mx = lmax[j];
+                    if (box_disjoint(qmn, qmx, mn, mx)) continue;
+                    uint32_t sj = __float_as_uint(mn.w), inf = __float_as_uint(mx.w);  // packed leaf record
+                    if (int(inf & kLeafBody) == bi) continue;
+                    if ((inf & kLeafDyn) != 0u && j < i) continue;
+                    if (!awake_i && !(inf & kLeafAwake)) continue;  // both asleep (World.Detect.cs:158)
                     uint32_t a = min(si, sj), b = max(si, sj);
                     int slot = __hip_atomic_fetch_add(wcnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                     wbuf[slot] = (uint64_t(a) << 32) | b;
@@ -652,6 +683,102 @@ __global__ void __launch_bounds__(256) k_traverse(int n, const uint64_t* __restr
         }
     }
     // all lanes of the wave are back together here
+    flush_pairs(wbuf, wcnt, pairs, pair_count, cap);
+}
+
+// Wavefront traversal: the 64 lanes of a wave hold 64 consecutive leaves of the Morton order
+// (neighbouring boxes) and walk the LBVH together.  At each internal node the wave loads the node's
+// child record (scalar loads: the node is the same for the whole wave) and every lane tests its query
+// against both children; a leaf child is settled at once by each lane's pair rules, an internal
+// child is entered when any lane's query overlaps it (a ballot).  When both children are entered the
+// right one waits on a wave-shared stack in LDS.  No lane diverges and no lane keeps a stack in
+// private memory.  Same pair rules as k_traverse (the list is sorted afterwards).
+constexpr int kWaveStack = 128;
+
+__global__ void __launch_bounds__(256) k_traverse_wave(int n, const uint64_t* __restrict__ keys,
+                                                       const float4* __restrict__ lmin,
+                                                       const float4* __restrict__ lmax,
+                                                       const float4* __restrict__ nbox,
+                                                       const int4* __restrict__ nrng, uint64_t* __restrict__ pairs,
+                                                       int* __restrict__ pair_count, int cap, int* __restrict__ err) {
+    __shared__ uint64_t sbuf[4][kPairBuf];
+    __shared__ int scnt[4];
+    __shared__ int sstk[4][kWaveStack];
+    int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) scnt[wv] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    uint64_t* wbuf = sbuf[wv];
+    int* wcnt = &scnt[wv];
+    int* stk = sstk[wv];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = i < n && n >= 2;
+    float4 qmn = make_float4(0, 0, 0, 0), qmx = qmn;
+    uint32_t si = 0, bi = 0;
+    bool awake_i = true;
+    int lo = 0, hi = 0;
+    if (active) {
+        qmn = lmin[i];
+        qmx = lmax[i];
+        uint32_t inf = __float_as_uint(qmx.w);
+        si = __float_as_uint(qmn.w);
+        bi = inf & kLeafBody;
+        awake_i = (inf & kLeafAwake) != 0u;
+        active = (inf & kLeafDyn) != 0u;
+    }
+    if (active) {
+        uint64_t w = keys[i] >> 32;
+        lo = lower_bound_u64(keys, n, w << 32);
+        hi = lower_bound_u64(keys, n, (w + 1) << 32);
+    }
+    if (__ballot(active) == 0ull) return;  // wave-uniform
+    // one child: a leaf is settled here (per lane), an internal node returns whether the wave enters it
+    auto child = [&](int code, float4 bmn, float4 bmx, int a, int b) -> bool {
+        if (code < 0) {
+            int j = ~code;
+            bool hit = active && j >= lo && j < hi && j != i && !box_disjoint(qmn, qmx, bmn, bmx);
+            uint32_t inf = uint32_t(b);
+            // different bodies; two dynamic shapes once (from the lower leaf); not both asleep
+            if (hit && (inf & kLeafBody) != bi && !((inf & kLeafDyn) != 0u && j < i) &&
+                (awake_i || (inf & kLeafAwake) != 0u)) {
+                uint32_t sj = uint32_t(a);
+                uint32_t x = min(si, sj), y = max(si, sj);
+                int slot = __hip_atomic_fetch_add(wcnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                wbuf[slot] = (uint64_t(x) << 32) | y;
+            }
+            return false;
+        }
+        bool hit = active && b >= lo && a < hi && !box_disjoint(qmn, qmx, bmn, bmx);
+        return __ballot(hit) != 0ull;
+    };
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const float4* rec = nbox + 4 * size_t(node);
+        float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+        int4 g = nrng[node];
+        int cl = __float_as_int(r0.w), cr = __float_as_int(r2.w);
+        bool dl = child(cl, r0, r1, g.x, g.y);
+        bool dr = child(cr, r2, r3, g.z, g.w);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        // a step adds at most two pairs per lane: 127 + 128 < kPairBuf
+        if (__hip_atomic_load(wcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >= kPairFlush)
+            flush_pairs(wbuf, wcnt, pairs, pair_count, cap);
+        int next;
+        if (dl) {
+            if (dr) {
+                if (sp < kWaveStack) stk[sp++] = cr;
+                else if ((threadIdx.x & 63) == 0) atomicOr(err, 1);
+            }
+            next = cl;
+        } else if (dr) {
+            next = cr;
+        } else {
+            if (sp == 0) break;
+            next = stk[--sp];
+        }
+        node = __builtin_amdgcn_readfirstlane(next);
+    }
     flush_pairs(wbuf, wcnt, pairs, pair_count, cap);
 }
 
@@ -2806,39 +2933,96 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
 }
 
 // Per-step set-up of the dataflow solver.
-// Rank keys: (body << 32) | (order position << 1 | which) for each non-static body of each solved arbiter.
-__global__ void k_rank_keys(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
-                            const int* __restrict__ offs, const int* __restrict__ motion, uint64_t* __restrict__ keys,
-                            int* __restrict__ nkeys) {
+// Ranks and degrees from the colouring's per-body colour masks, without sorting.  After k_colour,
+// mask_lo/mask_hi of every solver-non-static body that an active arbiter touches hold exactly the
+// colours 0..126 of its arbiters, and no two of them share a colour, so in the colour order
+// (colours ascending, the serial batch 127 last) the arbiters of body b before one of colour c are
+// those of b's colours below c: rank = popcount of b's mask below c.  The serial batch may hold
+// several arbiters of one body; their ranks continue after all of b's coloured arbiters, in
+// position order (k_rank_serial).  deg = b's coloured arbiters + its serial ones.  Same values as
+// sorting the (body, position) keys.
+__device__ __forceinline__ int colour_of_position(const uint64_t* order, int k) {
+    return int((order[k] >> 34) & 0xffu);  // solve-order key: (colour << 2 | contacts - 1) << 32 | arbiter
+}
+__device__ __forceinline__ int mask_rank(const unsigned long long* lo, const unsigned long long* hi, int b, int c) {
+    unsigned long long l = lo[b], h = hi[b];
+    if (c >= kMaxColours) return __popcll(l) + __popcll(h);
+    if (c < 64) return __popcll(l & ((1ull << c) - 1ull));
+    return __popcll(l) + __popcll(h & ((1ull << (c - 64)) - 1ull));
+}
+// Serial counts of the bodies of the solve order cleared.
+__global__ void k_rank_clear(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
+                             const int* __restrict__ offs, int* __restrict__ scnt) {
     int A = offs[128];
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
         const jh_arbiter& a = arb[uint32_t(order[k])];
-        bool d1 = motion[a.body1] != JH_MOTION_STATIC, d2 = motion[a.body2] != JH_MOTION_STATIC;
-        int s1 = wave_append(nkeys, d1);
-        if (d1) keys[s1] = (uint64_t(uint32_t(a.body1)) << 32) | (uint64_t(k) << 1);
-        int s2 = wave_append(nkeys, d2);
-        if (d2) keys[s2] = (uint64_t(uint32_t(a.body2)) << 32) | (uint64_t(k) << 1) | 1u;
+        scnt[a.body1] = 0;
+        scnt[a.body2] = 0;
     }
 }
-// After sorting the rank keys: rank = position within the body's segment; deg = segment length.
-__global__ void k_rank_assign(const uint64_t* __restrict__ keys, const int* __restrict__ nkeys,
-                              int2* __restrict__ rank, int* __restrict__ deg, int* __restrict__ max_deg) {
-    int n = *nkeys;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint64_t key = keys[i];
-        uint64_t b = key >> 32;
-        int lo = lower_bound_u64(keys, n, b << 32);
-        uint32_t pos = uint32_t(key & 0xffffffffu);
-        int k = int(pos >> 1);
-        if (pos & 1u) rank[k].y = i - lo;
-        else rank[k].x = i - lo;
-        if (i == lo) {
-            int d = lower_bound_u64(keys, n, (b + 1) << 32) - lo;
-            deg[b] = d;
-            atomicMax(max_deg, d);
+// One workgroup walks the serial batch [offs[127], offs[128]) in 256-position chunks: a position's
+// rank among its body's serial arbiters = the body's count from earlier chunks + the earlier
+// positions of this chunk touching the body.  Writes that rank to rank[k] (completed by
+// k_rank_masks) and leaves each body's serial count in scnt.
+__global__ void __launch_bounds__(256) k_rank_serial(const jh_arbiter* __restrict__ arb,
+                                                     const uint64_t* __restrict__ order, const int* __restrict__ offs,
+                                                     const int* __restrict__ motion, int2* __restrict__ rank,
+                                                     int* __restrict__ scnt) {
+    __shared__ int sb[2][256];
+    int s0 = offs[kMaxColours], s1 = offs[128];
+    int t = threadIdx.x;
+    for (int base = s0; base < s1; base += 256) {
+        int k = base + t;
+        int b1 = -1, b2 = -1;
+        if (k < s1) {
+            const jh_arbiter& a = arb[uint32_t(order[k])];
+            if (motion[a.body1] != JH_MOTION_STATIC) b1 = a.body1;
+            if (motion[a.body2] != JH_MOTION_STATIC) b2 = a.body2;
         }
+        sb[0][t] = b1;
+        sb[1][t] = b2;
+        __syncthreads();
+        int r1 = b1 >= 0 ? __hip_atomic_load(&scnt[b1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        int r2 = b2 >= 0 ? __hip_atomic_load(&scnt[b2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        for (int u = 0; u < t; ++u) {
+            int c1 = sb[0][u], c2 = sb[1][u];
+            if (b1 >= 0) r1 += int(c1 == b1) + int(c2 == b1);
+            if (b2 >= 0) r2 += int(c1 == b2) + int(c2 == b2);
+        }
+        if (k < s1) rank[k] = make_int2(r1, r2);
+        __syncthreads();  // every count of this chunk is read before any is raised
+        if (b1 >= 0) __hip_atomic_fetch_add(&scnt[b1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (b2 >= 0) __hip_atomic_fetch_add(&scnt[b2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);  // the additions are performed before the next chunk reads
+        __syncthreads();
     }
 }
+__global__ void k_rank_masks(const jh_arbiter* __restrict__ arb, const uint64_t* __restrict__ order,
+                             const int* __restrict__ offs, const int* __restrict__ motion,
+                             const unsigned long long* __restrict__ mask_lo,
+                             const unsigned long long* __restrict__ mask_hi, const int* __restrict__ scnt,
+                             int2* __restrict__ rank, int* __restrict__ deg, int* __restrict__ max_deg) {
+    int A = offs[128];
+    int m = 0;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A; k += gridDim.x * blockDim.x) {
+        const jh_arbiter& a = arb[uint32_t(order[k])];
+        int c = colour_of_position(order, k);
+        int2 r = c >= kMaxColours ? rank[k] : make_int2(0, 0);
+        int b[2] = {a.body1, a.body2};
+        int rr[2] = {r.x, r.y};
+        for (int e = 0; e < 2; ++e) {
+            if (motion[b[e]] == JH_MOTION_STATIC) continue;
+            rr[e] += mask_rank(mask_lo, mask_hi, b[e], c);
+            int d = mask_rank(mask_lo, mask_hi, b[e], kMaxColours) + scnt[b[e]];
+            deg[b[e]] = d;  // the same value from each arbiter of the body
+            m = max(m, d);
+        }
+        rank[k] = make_int2(rr[0], rr[1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(max_deg, m);
+}
+
 // ---- Tile solver ----------------------------------------------------------------------------
 // One workgroup per tile: a contiguous range of the non-static bodies in broadphase (Morton) order.
 // Each arbiter belongs to the tile of its first non-static body.  A body is *private* when every

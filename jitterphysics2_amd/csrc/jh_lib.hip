@@ -173,6 +173,9 @@ struct jh_world {
     DBuf<uint64_t> mk0, mk1;
     DBuf<uint32_t> mv0, mv1;
     DBuf<int> left, right, parent, leaf_parent, nflags;
+    DBuf<float4> nbox;           // LBVH child records (k_lbvh_refit, k_traverse_wave): 4 per node
+    DBuf<int4> nrng;
+    int traverse_wave = 1;       // pair traversal: 1 k_traverse_wave, 0 k_traverse (JH_TRAVERSE=thread)
     DBuf<int2> range;
     DBuf<uint32_t> bounds, hist;
     DBuf<uint64_t> pk0, pk1;
@@ -243,9 +246,9 @@ struct jh_world {
     DBuf<uint64_t> tlist;
     DBuf<uint64_t> ok0, ok1, solve_keys;
     // dataflow solver
-    DBuf<uint64_t> rk0, rk1;
     DBuf<int2> rank;
     DBuf<int> deg;
+    DBuf<int> scnt;  // per body: arbiters in the serial colour batch (k_rank_serial)
     DBuf<unsigned> bar;       // grid barrier state and abort flag of the persistent kernels (64 words)
     unsigned spin_limit = 0;  // polls before a device wait gives up; 0: kSpinLimitDefault
     unsigned* habort = nullptr;  // pinned host mirror of the abort flag
@@ -472,15 +475,23 @@ jh_status broadphase(jh_world* w) {
     uint64_t* keys = in1 ? w->mk1.p : w->mk0.p;
     uint32_t* order = in1 ? w->mv1.p : w->mv0.p;
     w->sorted_in1 = in1;
-    k_lbvh_build<<<grid_for(ns - 1, 256, 1 << 20), 256, 0, st>>>(keys, ns, w->left.p, w->right.p, w->range.p,
-                                                                  w->parent.p, w->leaf_parent.p, w->nflags.p);
+    int gi = grid_for(ns - 1, 256, 1 << 20);
+    k_lbvh_build<<<gi, 256, 0, st>>>(keys, ns, w->left.p, w->right.p, w->range.p, w->parent.p, w->leaf_parent.p,
+                                     w->nflags.p);
+    const uint32_t* sfl = w->islands_on() ? w->sflags.p : nullptr;
     k_lbvh_refit<<<g, 256, 0, st>>>(ns, order, w->amin.p, w->amax.p, w->lmin.p, w->lmax.p, w->left.p, w->right.p,
-                                    w->parent.p, w->leaf_parent.p, w->nmin.p, w->nmax.p, w->nflags.p);
+                                    w->parent.p, w->leaf_parent.p, w->nmin.p, w->nmax.p, w->nflags.p, w->shapes.p,
+                                    w->motion.p, sfl, w->range.p, w->traverse_wave ? w->nbox.p : nullptr,
+                                    w->nrng.p);
     for (int attempt = 0; attempt < 3; ++attempt) {
         k_set_int<<<1, 1, 0, st>>>(cnt + C_PAIRS, 0);
-        k_traverse<<<g, 256, 0, st>>>(ns, keys, order, w->shapes.p, w->motion.p, w->lmin.p, w->lmax.p, w->left.p,
-                                      w->right.p, w->range.p, w->nmin.p, w->nmax.p, w->pk0.p, cnt + C_PAIRS,
-                                      w->pair_cap, cnt + C_TERR, w->islands_on() ? w->sflags.p : nullptr);
+        if (w->traverse_wave)
+            k_traverse_wave<<<g, 256, 0, st>>>(ns, keys, w->lmin.p, w->lmax.p, w->nbox.p, w->nrng.p, w->pk0.p,
+                                               cnt + C_PAIRS, w->pair_cap, cnt + C_TERR);
+        else
+            k_traverse<<<g, 256, 0, st>>>(ns, keys, order, w->shapes.p, w->motion.p, w->lmin.p, w->lmax.p, w->left.p,
+                                          w->right.p, w->range.p, w->nmin.p, w->nmax.p, w->pk0.p, cnt + C_PAIRS,
+                                          w->pair_cap, cnt + C_TERR, sfl);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -549,13 +560,11 @@ jh_status ensure_step_buffers(jh_world* w) {
     HIPCHK(w->ok0.ensure(na));
     HIPCHK(w->ok1.ensure(na));
     HIPCHK(w->solve_keys.ensure(na));
-    HIPCHK(w->rk0.ensure(2 * na));
-    HIPCHK(w->rk1.ensure(2 * na));
     HIPCHK(w->rank.ensure(na));
     HIPCHK(w->flags.ensure(na));
     HIPCHK(w->offs.ensure(na));
     HIPCHK(w->sums.ensure(na / 256 + 2));
-    HIPCHK(w->hist.ensure(rsort::hist_entries(2 * na)));  // the rank-key sort holds up to 2 keys per arbiter
+    HIPCHK(w->hist.ensure(rsort::hist_entries(na)));
     if (w->kind == SOLVER_TILES) HIPCHK(w->tlist.ensure(na));
     if (w->kind == SOLVER_LANES) HIPCHK(w->rec.ensure(na * kRecVec));
     return JH_OK;
@@ -1003,20 +1012,14 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         s = colour_arbiters(w, cur, a_cur, amax_, &order);
         if (s != JH_OK) return s;
         if (kind == SOLVER_FLOW || kind == SOLVER_TILES || kind == SOLVER_LANES) {
-            // per-body update sequence: rank of each arbiter visit and body degree
+            // per-body update sequence: rank of each arbiter visit and body degree, from the
+            // colouring's per-body colour masks (the serial batch ranked by one workgroup)
             int ga = grid_for(std::max(amax_, 1));
-            k_set_int<<<1, 1, 0, st>>>(cnt + C_NRANK, 0);
-            k_rank_keys<<<ga, 256, 0, st>>>(w->arb[cur].p, order, w->coffs.p, w->motion.p, w->rk0.p, cnt + C_NRANK);
-            int rsh[8];
-            int nrs = 0;
-            int kb = bits_for(uint32_t(2 * std::max(amax_, 1)));
-            for (int b = 0; b < kb; b += 8) rsh[nrs++] = b;
-            int bb = bits_for(uint32_t(std::max(w->nb - 1, 1)));
-            for (int b = 32; b < 32 + bb; b += 8) rsh[nrs++] = b;
-            bool rin1 = rsort::sort_pairs(w->rk0.p, w->rk1.p, nullptr, nullptr, cnt + C_NRANK, 2 * std::max(amax_, 1),
-                                          rsh, nrs, w->hist.p, st);
-            k_rank_assign<<<grid_for(2 * std::max(amax_, 1)), 256, 0, st>>>(rin1 ? w->rk1.p : w->rk0.p, cnt + C_NRANK,
-                                                                            w->rank.p, w->deg.p, cnt + C_MAXDEG);
+            const int* smo = w->solver_bodies().motion;
+            k_rank_clear<<<ga, 256, 0, st>>>(w->arb[cur].p, order, w->coffs.p, w->scnt.p);
+            k_rank_serial<<<1, 256, 0, st>>>(w->arb[cur].p, order, w->coffs.p, smo, w->rank.p, w->scnt.p);
+            k_rank_masks<<<ga, 256, 0, st>>>(w->arb[cur].p, order, w->coffs.p, smo, w->mlo.p, w->mhi.p, w->scnt.p,
+                                             w->rank.p, w->deg.p, cnt + C_MAXDEG);
             if (use_tiles) {
                 s = tile_setup(w, order, a_cur, amax_);
                 if (s != JH_OK) return s;
@@ -1313,6 +1316,7 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
         w->lane_blocks = std::max(1, std::min(atoi(e), w->lane_max));
     }
     if (const char* e = getenv("JH_FLOW_QUEUE")) w->flow_qmode = std::max(0, std::min(atoi(e), 2));
+    if (const char* e = getenv("JH_TRAVERSE")) w->traverse_wave = strcmp(e, "thread") != 0;
     w->tiles = w->cu_count;
     if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
     w->colour_blocks = std::min(64, w->coop_colour);
@@ -1333,6 +1337,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->body_io.release(); w->rbd_io.release(); w->shapes.release(); w->verts.release();
     w->mk0.release(); w->mk1.release(); w->mv0.release(); w->mv1.release();
     w->left.release(); w->right.release(); w->parent.release(); w->leaf_parent.release(); w->nflags.release();
+    w->nbox.release(); w->nrng.release();
     w->range.release(); w->bounds.release(); w->hist.release(); w->pk0.release(); w->pk1.release();
     w->cnt.release(); w->man.release(); w->deep.release(); w->epa_work.release(); w->epa_side.release(); w->epa_res.release(); w->fqueue.release(); w->soa_mv.release();
     w->flags.release(); w->offs.release(); w->sums.release(); w->cnew.release(); w->noffs.release();
@@ -1347,7 +1352,7 @@ jh_status jh_world_destroy(jh_world* w) {
     w->tsums.release(); w->tlist.release(); w->rec.release();
     w->vwpack.release(); w->chist.release(); w->coffs.release(); w->bmax.release(); w->mlo.release(); w->mhi.release();
     w->ok0.release(); w->ok1.release(); w->solve_keys.release();
-    w->rk0.release(); w->rk1.release(); w->rank.release(); w->deg.release();
+    w->rank.release(); w->deg.release(); w->scnt.release();
     w->bar.release();
     w->lws.release(); w->pvel.release(); w->pang.release(); w->parb.release(); w->ppart.release();
     w->pfeat.release(); w->ptarg.release(); w->pvalid.release(); w->plam.release(); w->plearn.release();
@@ -1377,6 +1382,8 @@ jh_status jh_world_set_config(jh_world* w, const jh_config* c) {
 
 jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     if (!w || (n > 0 && !bodies) || n < 0) return fail(JH_ERR_INVALID_ARG, "bad arguments");
+    // the broadphase packs a body index into 30 bits of its leaf records (kLeafBody)
+    if (n > int(kLeafBody)) return fail(JH_ERR_CAPACITY, "at most %u bodies per world", kLeafBody);
     int maxw = 0;
     for (int i = 0; i < n; ++i) {
         const jh_body& b = bodies[i];
@@ -1461,6 +1468,7 @@ jh_status jh_world_set_bodies(jh_world* w, const jh_body* bodies, int32_t n) {
     HIPCHK(w->mlo.ensure(nn));
     HIPCHK(w->mhi.ensure(nn));
     HIPCHK(w->deg.ensure(nn));
+    HIPCHK(w->scnt.ensure(nn));
     HIPCHK(w->vwpack.ensure(2 * nn));
     HIPCHK(w->slp.ensure(nn));
     HIPCHK(w->sflags.ensure(nn));
@@ -1662,6 +1670,7 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
     HIPCHK(w->mk0.ensure(nn)); HIPCHK(w->mk1.ensure(nn)); HIPCHK(w->mv0.ensure(nn)); HIPCHK(w->mv1.ensure(nn));
     HIPCHK(w->left.ensure(nn)); HIPCHK(w->right.ensure(nn)); HIPCHK(w->parent.ensure(nn));
     HIPCHK(w->leaf_parent.ensure(nn)); HIPCHK(w->nflags.ensure(nn)); HIPCHK(w->range.ensure(nn));
+    HIPCHK(w->nbox.ensure(4 * nn)); HIPCHK(w->nrng.ensure(nn));
     HIPCHK(w->hist.ensure(rsort::hist_entries(size_t(nn))));
     int want = w->cfg.pair_capacity > 0 ? w->cfg.pair_capacity : 24 * n + 1024;
     if (w->pair_cap < want) {
@@ -2518,10 +2527,27 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         for (size_t i = 0; i < cc.size(); ++i)
             if (cc[i] == colour_default) out->ms_colour_default = tot[i] / rounds;
     }
+    // pair traversal: wavefront (default) or one thread per leaf; same pair set either way
+    int tv_default = w->traverse_wave;
+    {
+        double tot[2] = {0.0, 0.0};
+        for (int r = 0; r < rounds; ++r)
+            for (int m = 0; m < 2; ++m) {
+                w->traverse_wave = 1 - m;
+                jh_status st = do_step(w, dt, nullptr);
+                if (st != JH_OK) return st;
+                tot[m] += w->stats.ms_broadphase;
+            }
+        int best = tot[1] < tot[0] ? 1 : 0;
+        w->traverse_wave = 1 - best;
+        out->ms_broadphase_best = tot[best] / rounds;
+        out->ms_broadphase_default = tot[tv_default ? 0 : 1] / rounds;
+    }
+    out->traversal = w->traverse_wave;
     out->solver = w->kind;
     out->solve_blocks = width(w->kind);
     out->colour_blocks = w->colour_blocks;
-    out->candidates = int(sc.size() + cc.size());
+    out->candidates = int(sc.size() + cc.size() + 2);
     out->steps = rounds * out->candidates;
     return JH_OK;
 }
