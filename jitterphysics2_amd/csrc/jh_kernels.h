@@ -1173,15 +1173,33 @@ __global__ void k_new_compact(const uint64_t* __restrict__ ckey, const int* __re
 }
 
 // Merge the key-sorted cache with the key-sorted new keys into the other buffer.
+// Arbiter records are moved whole by groups of 16 lanes, as 8-byte words (a record is 65 of them),
+// so a wave's loads and stores are contiguous.  Unused slots travel too: like the reference's
+// contact memory they keep what their last contact left (a slot reused by AddContact starts from
+// it), and the learned warm start's features read some of those fields before PrepareForIteration.
+static_assert(sizeof(jh_arbiter) == 520 && sizeof(jh_contact) == 120, "arbiter record layout");
+constexpr int kArbWords = 65;
+__device__ __forceinline__ void copy_arbiter16(const jh_arbiter* src, jh_arbiter* dst, int sub) {
+    const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+    uint64_t* d = reinterpret_cast<uint64_t*>(dst);
+    for (int q = sub; q < kArbWords; q += 16) d[q] = s[q];
+}
+
 __global__ void k_merge_existing(const jh_arbiter* __restrict__ ain, const uint64_t* __restrict__ akey,
                                  const int* __restrict__ a_count, const uint64_t* __restrict__ nkey,
                                  const int* __restrict__ n_new, jh_arbiter* __restrict__ aout,
                                  uint64_t* __restrict__ okey) {
     int A = *a_count, N = *n_new;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) {
-        int pos = i + lower_bound_u64(nkey, N, akey[i]);
-        aout[pos] = ain[i];
-        okey[pos] = akey[i];
+    int sub = threadIdx.x & 15;
+    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < A; i += (gridDim.x * blockDim.x) >> 4) {
+        int pos = 0;
+        if (sub == 0) {
+            uint64_t k = akey[i];
+            pos = i + lower_bound_u64(nkey, N, k);
+            okey[pos] = k;
+        }
+        pos = __shfl(pos, 0, 16);
+        copy_arbiter16(&ain[i], &aout[pos], sub);
     }
 }
 
@@ -1194,18 +1212,20 @@ __global__ void k_merge_new(const uint64_t* __restrict__ akey, const int* __rest
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
         uint64_t k = nkey[j];
         int pos = j + lower_bound_u64(akey, A, k);
-        jh_arbiter a;
-        memset(&a, 0, sizeof(a));
+        jh_arbiter& a = aout[pos];
         a.shape1 = uint32_t(k >> 32);
         a.shape2 = uint32_t(k & 0xffffffffu);
         a.body1 = shapes[a.shape1].body;
         a.body2 = shapes[a.shape2].body;
         float4 m1 = B.mat[a.body1], m2 = B.mat[a.body2];
+        a.usage_mask = 0u;
+        a.mode = 0u;
         a.friction = fmaxr(m1.y, m2.y);
         a.restitution = fmaxr(m1.z, m2.z);
         a.speculative_relaxation_factor = specrel;
         a.reserved = kNoColour;
-        aout[pos] = a;
+        uint64_t* cw = reinterpret_cast<uint64_t*>(&a.contacts[0]);
+        for (int q = 0; q < int(sizeof(a.contacts) / 8); ++q) cw[q] = 0ull;
         okey[pos] = k;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *out_count = A + N;
@@ -1316,7 +1336,7 @@ __global__ void k_register(const uint64_t* __restrict__ ckey, const int* __restr
         uint64_t k = ckey[c];
         int f = cfound[c];
         int pos = f >= 0 ? f + lower_bound_u64(nkey, N, k) : int(noffs[c]) + lower_bound_u64(akey_old, A, k);
-        jh_arbiter cd = arb[pos];
+        jh_arbiter& cd = arb[pos];  // in place: one colliding pair per arbiter
         const jh_manifold& m = man[cpair[c]];
         if (!persistent) cd.usage_mask = 0;
         // ContactData.ResetMode (Contact.cs:291-296); speculative contacts remove SolveMode.Angular
@@ -1330,7 +1350,6 @@ __global__ void k_register(const uint64_t* __restrict__ ckey, const int* __restr
             vec3 b{m.manifold_b[e][0], m.manifold_b[e][1], m.manifold_b[e][2]};
             add_contact(cd, B, a, b, n3);
         }
-        arb[pos] = cd;
     }
 }
 
@@ -2796,13 +2815,22 @@ struct WorldArgs {
     int* queue;           // next world to take
 };
 
+// What a world-local visit reads first (bodies, used slots and mode, friction); the next batch's is
+// fetched before the batch barrier, so a visit starts with one round of loads, not two.
+struct WorldHead {
+    int2 bb;
+    uint2 um;
+    float2 fs;
+};
+__device__ __forceinline__ WorldHead world_head(const ContactSoA& S, int p) { return {S.bodies[p], S.um[p], S.fs[p]}; }
+
 template <bool PREPARE>
-__device__ __forceinline__ void world_visit(const SolveArgs& P, int p, bool apply_bias) {
+__device__ __forceinline__ void world_visit(const SolveArgs& P, int p, bool apply_bias, const WorldHead& h) {
     const Bodies& B = P.B;
     const ContactSoA& S = P.soa;
-    int2 bb = S.bodies[p];
-    uint2 um = S.um[p];
-    float2 fs = S.fs[p];
+    int2 bb = h.bb;
+    uint2 um = h.um;
+    float2 fs = h.fs;
     int b1 = bb.x, b2 = bb.y;
     uint32_t usage = um.x, mode = um.y;
     SlotReg sr[4];
@@ -2835,30 +2863,40 @@ __device__ __forceinline__ void world_visit(const SolveArgs& P, int p, bool appl
         if (usage & (1u << s)) store_slot_soa(S, s, p, sr[s], PREPARE);
 }
 
-__device__ __forceinline__ void world_pass(const SolveArgs& P, const uint32_t* segs, int phase) {
-    for (int c = 0; c < kMaxColours; ++c) {
-        int beg = int(segs[c]), end = int(segs[c + 1]);
-        if (beg == end) continue;
-        for (int p = beg + int(threadIdx.x); p < end; p += blockDim.x) {
-            if (phase == 0) world_visit<true>(P, p, false);
-            else world_visit<false>(P, p, phase == 1);
-        }
-        __syncthreads();
-    }
-    int beg = int(segs[kMaxColours]), end = int(segs[kMaxColours + 1]);
-    if (beg < end) {  // the serial batch, in solve order
-        if (threadIdx.x == 0)
-            for (int p = beg; p < end; ++p) {
-                if (phase == 0) world_visit<true>(P, p, false);
-                else world_visit<false>(P, p, phase == 1);
+// One pass over a world's colour batches.  segs lists the non-empty batches' bounds (nseg batches,
+// the serial batch last when present); h holds this thread's first visit's head for batch 0 and
+// returns the one for batch 0 of the next pass.
+__device__ __forceinline__ void world_pass(const SolveArgs& P, const uint32_t* segs, int nseg, bool serial,
+                                           int phase, WorldHead& h) {
+    const ContactSoA& S = P.soa;
+    int t = int(threadIdx.x);
+    for (int j = 0; j < nseg; ++j) {
+        int beg = int(segs[j]), end = int(segs[j + 1]);
+        bool ser = serial && j == nseg - 1;
+        if (!ser) {
+            for (int p = beg + t; p < end; p += blockDim.x) {
+                WorldHead hp = p == beg + t ? h : world_head(S, p);
+                if (phase == 0) world_visit<true>(P, p, false, hp);
+                else world_visit<false>(P, p, phase == 1, hp);
             }
+        } else if (t == 0) {  // the serial batch, in solve order
+            for (int p = beg; p < end; ++p) {
+                WorldHead hp = world_head(S, p);
+                if (phase == 0) world_visit<true>(P, p, false, hp);
+                else world_visit<false>(P, p, phase == 1, hp);
+            }
+        }
+        // the next batch's first head (batch 0 again after the last one): no visit writes heads
+        int jn = j + 1 < nseg ? j + 1 : 0;
+        int pn = int(segs[jn]) + t;
+        if (pn < int(segs[jn + 1])) h = world_head(S, pn);
         __syncthreads();
     }
 }
 
 __global__ void __launch_bounds__(512) k_solve_worlds(SolveArgs P, WorldArgs Wd) {
     __shared__ uint32_t segs[kMaxColours + 2];
-    __shared__ int take;
+    __shared__ int take, nseg_s, serial_s;
     const Bodies& B = P.B;
     while (true) {
         if (threadIdx.x == 0) take = atomicAdd(Wd.queue, 1);
@@ -2866,9 +2904,25 @@ __global__ void __launch_bounds__(512) k_solve_worlds(SolveArgs P, WorldArgs Wd)
         int q = take;
         if (q >= Wd.W) break;
         int w = Wd.wsched[q];
-        for (int c = threadIdx.x; c <= kMaxColours + 1; c += blockDim.x) segs[c] = Wd.woff[size_t(w) * 128 + c];
         int bo = Wd.wboff[w], be = Wd.wboff[w + 1];
+        if (threadIdx.x == 0) {  // bounds of the world's non-empty batches, serial batch last
+            const uint32_t* wo = Wd.woff + size_t(w) * 128;
+            int n = 0;
+            uint32_t prev = wo[0];
+            for (int c = 0; c <= kMaxColours; ++c) {
+                uint32_t nx = wo[c + 1];
+                if (nx != prev) segs[n++] = prev;
+                prev = nx;
+            }
+            segs[n] = prev;
+            nseg_s = n;
+            serial_s = wo[kMaxColours + 1] != wo[kMaxColours];
+        }
         __syncthreads();
+        int nseg = nseg_s;
+        bool serial = serial_s != 0;
+        WorldHead h{};
+        if (nseg > 0 && int(segs[0]) + int(threadIdx.x) < int(segs[1])) h = world_head(P.soa, int(segs[0]) + threadIdx.x);
         for (int s = 0; s < P.ts.substeps; ++s) {
             for (int i = bo + int(threadIdx.x); i < be; i += blockDim.x) {  // IntegrateForces
                 int b = Wd.wbody[i];
@@ -2877,7 +2931,7 @@ __global__ void __launch_bounds__(512) k_solve_worlds(SolveArgs P, WorldArgs Wd)
                 B.vel[b] = st3(ld3(B.vel[b]) + ld3(B.dvel[b]));
             }
             __syncthreads();
-            for (int it = 0; it <= P.iterations; ++it) world_pass(P, segs, it == 0 ? 0 : 1);
+            for (int it = 0; it <= P.iterations; ++it) world_pass(P, segs, nseg, serial, it == 0 ? 0 : 1, h);
             for (int i = bo + int(threadIdx.x); i < be; i += blockDim.x) {  // IntegrateVelocities
                 int b = Wd.wbody[i];
                 if (B.motion[b] == JH_MOTION_STATIC) continue;
@@ -2887,7 +2941,7 @@ __global__ void __launch_bounds__(512) k_solve_worlds(SolveArgs P, WorldArgs Wd)
                 B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
             }
             __syncthreads();
-            for (int it = 0; it < P.relaxations; ++it) world_pass(P, segs, 2);
+            for (int it = 0; it < P.relaxations; ++it) world_pass(P, segs, nseg, serial, 2, h);
         }
         __syncthreads();
     }
@@ -3350,14 +3404,22 @@ __global__ void k_compact_arbiters(const jh_arbiter* __restrict__ ain, const uin
                                    int* __restrict__ zcount) {
     int A = *a_count;
     if (A == 0 && blockIdx.x == 0 && threadIdx.x == 0) *zcount = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A; i += gridDim.x * blockDim.x) {
-        if (i == A - 1) *zcount = A - int(offs[i] + keep[i]);
+    int sub = threadIdx.x & 15;
+    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < A; i += (gridDim.x * blockDim.x) >> 4) {
+        // the 16 lanes of a group share i (and so keep[i])
+        uint32_t o = offs[i];
         if (!keep[i]) {
-            zkey[i - offs[i]] = akey[i];
+            if (sub == 0) {
+                zkey[i - o] = akey[i];
+                if (i == A - 1) *zcount = A - int(o);
+            }
             continue;
         }
-        aout[offs[i]] = ain[i];
-        okey[offs[i]] = akey[i];
+        if (sub == 0) {
+            okey[o] = akey[i];
+            if (i == A - 1) *zcount = A - int(o + 1);
+        }
+        copy_arbiter16(&ain[i], &aout[o], sub);
     }
 }
 

@@ -224,7 +224,9 @@ struct jh_world {
     int soa_cap = 0;
     // world-local solver (many independent worlds)
     int n_worlds = 0, world_blocks = 0, largest_world = 0;
+    int world_occ[4] = {0, 0, 0, 0};  // resident k_solve_worlds workgroups per CU at 64, 128, 256, 512 threads
     DBuf<int> wbody, wboff, wsched, wcells, wqueue;
+    int world_threads = 512;     // k_solve_worlds workgroup size (autotuned: 64 .. 512)
     DBuf<uint32_t> whist, woff, wcur, wsums;
     DBuf<uint64_t> wlist;
     DBuf<float4> vwpack;  // dataflow solver: packed velocities, 2 per body
@@ -973,7 +975,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         w->nkey.p, cnt + C_NEW, w->zkey.p, cnt + C_ZOMB, w->ckey.p, cnt + C_COLL, w->evb.p, cnt + C_EVB, w->eve.p,
         cnt + C_EVE);
     int amax_ = w->a_host + np;
-    k_merge_existing<<<grid_for(std::max(w->a_host, 1)), 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->nkey.p,
+    k_merge_existing<<<grid_for(16 * std::max(w->a_host, 1)), 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->nkey.p,
                                                                        cnt + C_NEW, w->arb[nxt].p, w->akey[nxt].p);
     k_merge_new<<<grid_for(std::max(np, 1)), 256, 0, st>>>(w->akey[cur].p, a_cur, w->nkey.p, cnt + C_NEW, w->shapes.p, B,
                                                            cfg.speculative_relaxation_factor, w->arb[nxt].p,
@@ -1091,7 +1093,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             Wd.W = w->n_worlds;
             Wd.queue = w->wqueue.p;
             HIPCHK(hipEventRecord(w->ev[8], st));
-            k_solve_worlds<<<std::max(1, std::min(w->world_blocks, w->n_worlds)), 512, 0, st>>>(S, Wd);
+            k_solve_worlds<<<std::max(1, std::min(w->world_blocks, w->n_worlds)), w->world_threads, 0, st>>>(S, Wd);
             HIPCHK(hipEventRecord(w->ev[9], st));
             k_soa_scatter<<<gsoa, 256, 0, st>>>(S.arb, w->wlist.p, w->coffs.p, S.soa);
             HIPCHK(hipGetLastError());
@@ -1148,7 +1150,8 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         k_update_contacts<<<g, 256, 0, st>>>(w->arb[cur].p, a_cur, B, w->flags.p, w->solver_bodies().motion);
         scan::exclusive_scan(w->flags.p, w->offs.p, a_cur, std::max(amax_, 1), w->sums.p,
                              cnt + (cur == 0 ? C_A1 : C_A0), st);
-        k_compact_arbiters<<<g, 256, 0, st>>>(w->arb[cur].p, w->akey[cur].p, a_cur, w->flags.p, w->offs.p,
+        k_compact_arbiters<<<grid_for(16 * std::max(amax_, 1)), 256, 0, st>>>(
+            w->arb[cur].p, w->akey[cur].p, a_cur, w->flags.p, w->offs.p,
                                               w->arb[1 - cur].p, w->akey[1 - cur].p, w->zkey.p, cnt + C_ZOMB);
         w->cur = 1 - cur;
         if (w->nb > 0)
@@ -1290,6 +1293,18 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     w->coop_solve = std::max(1, occ) * w->cu_count;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_worlds, 512, 0));
     w->world_blocks = std::max(1, occ) * w->cu_count;
+    for (int k = 0; k < 4; ++k) {
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_solve_worlds, 64 << k, 0));
+        w->world_occ[k] = std::max(1, occ);
+    }
+    if (const char* e = getenv("JH_WORLD_THREADS")) {
+        int t = atoi(e);
+        for (int k = 0; k < 4; ++k)
+            if (t == (64 << k)) {
+                w->world_threads = t;
+                w->world_blocks = w->world_occ[k] * w->cu_count;
+            }
+    }
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_epa_wave, 64, 0));
     w->epa_wave_blocks = std::max(1, occ) * w->cu_count;
     if (const char* e = getenv("JH_EPA")) w->epa_wave = strcmp(e, "thread") != 0;
@@ -2457,32 +2472,34 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         return v;
     };
     // solver candidates (kind, workgroups); the barrier solver only tunes its own grid
-    struct Cand { int kind, n; };
+    struct Cand { int kind, n, threads; };
     std::vector<Cand> sc;
     if (w->kind == SOLVER_BARRIER || !w->host_con.empty()) {  // constraints run in the barrier solver
         if (!w->host_con.empty()) w->kind = SOLVER_BARRIER;
         // narrow grids too: each grid barrier's cache write-back and invalidate grow with the workgroups
-        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n});
+        for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, cu, 2 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n, 0});
     } else {
         for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, 3 * cu / 4, cu, 3 * cu / 2, 2 * cu}, w->flow_max))
-            sc.push_back({SOLVER_FLOW, n});
-        for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n});
+            sc.push_back({SOLVER_FLOW, n, 0});
+        for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n, 0});
         for (int T : {cu / 2, cu, 2 * cu})
-            if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back({SOLVER_TILES, T});
+            if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back({SOLVER_TILES, T, 0});
         // one workgroup per world: hopeless for a few large worlds (a 131k-body world takes ~0.5 s)
         if (w->n_worlds >= 64 || (w->n_worlds >= 1 && w->largest_world <= 16 * kSmallWorld))
-            sc.push_back({SOLVER_WORLDS, w->world_blocks});
+            for (int k = 0; k < 4; ++k) sc.push_back({SOLVER_WORLDS, w->world_occ[k] * cu, 64 << k});
     }
     auto width = [&](int kind) -> int& {
         return kind == SOLVER_BARRIER ? w->coop_solve : kind == SOLVER_LANES ? w->lane_blocks
              : kind == SOLVER_TILES ? w->tiles : kind == SOLVER_WORLDS ? w->world_blocks : w->flow_blocks;
     };
-    Cand def{w->kind, width(w->kind)};
+    Cand def{w->kind, width(w->kind), w->world_threads};
     int keep[5];
     for (int k = 0; k < 5; ++k) keep[k] = width(k);
+    int keep_threads = w->world_threads;
     auto apply = [&](Cand c) {
         w->kind = c.kind;
         width(c.kind) = c.n;
+        if (c.kind == SOLVER_WORLDS) w->world_threads = c.threads;
     };
     // round-robin over the candidates so that the evolving scene weighs them equally
     {
@@ -2499,13 +2516,16 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
             if (tot[i] < tot[bi]) bi = i;
         if (getenv("JH_AUTOTUNE_VERBOSE"))
             for (size_t i = 0; i < sc.size(); ++i)
-                fprintf(stderr, "autotune: solver %d x %d: %.3f ms solve\n", sc[i].kind, sc[i].n, tot[i] / rounds);
+                fprintf(stderr, "autotune: solver %d x %d (%d threads): %.3f ms solve\n", sc[i].kind, sc[i].n,
+                        sc[i].threads, tot[i] / rounds);
         for (int k = 0; k < 5; ++k) width(k) = keep[k];
+        w->world_threads = keep_threads;
         apply(sc[bi]);
         out->ms_solve_best = tot[bi] / rounds;
         out->ms_solve_default = out->ms_solve_best;
         for (size_t i = 0; i < sc.size(); ++i)
-            if (sc[i].kind == def.kind && sc[i].n == def.n) out->ms_solve_default = tot[i] / rounds;
+            if (sc[i].kind == def.kind && sc[i].n == def.n && (def.kind != SOLVER_WORLDS || sc[i].threads == def.threads))
+                out->ms_solve_default = tot[i] / rounds;
     }
     std::vector<int> cc = uniq({16, 32, 64, 128, 256, 512}, w->coop_colour);
     int colour_default = w->colour_blocks;

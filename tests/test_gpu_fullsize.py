@@ -100,3 +100,66 @@ def test_pyramid_configuration_600_steps():
     dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10]).max()
     print(f"pyramid, 600 steps: FP32 GPU vs FP64 oracle max |dp| {dp:.3e} m, max |dv| {dv:.3e} m/s", flush=True)
     assert dp < 1e-4 and dv < 1e-3
+
+
+def extract_world(bodies, shapes, verts, w):
+    """Sub-world w of a world batch as a scene of its own: its bodies and shapes in their original
+    relative order (indices remapped monotonically, so key order is kept), the vertices its shapes
+    use, and the maps from the batch's body and shape indices to the sub-world's (-1 elsewhere)."""
+    bsel = np.nonzero(bodies["world_id"] == w)[0]
+    bmap = np.full(len(bodies), -1, np.int64)
+    bmap[bsel] = np.arange(len(bsel))
+    ssel = np.nonzero(bmap[shapes["body"]] >= 0)[0]
+    smap = np.full(len(shapes), -1, np.int64)
+    smap[ssel] = np.arange(len(ssel))
+    sb = bodies[bsel].copy()
+    ss = shapes[ssel].copy()
+    ss["body"] = bmap[ss["body"]]
+    v = np.asarray(verts, np.float32).reshape(-1, 3)
+    parts, off = [], 0
+    for k in range(len(ss)):
+        n = int(ss["vertex_count"][k])
+        if n > 0:
+            o = int(ss["vertex_offset"][k])
+            parts.append(v[o:o + n])
+            ss["vertex_offset"][k] = off
+            off += n
+    sv = np.concatenate(parts).reshape(-1) if parts else None
+    return sb, ss, sv, bsel, smap
+
+
+@pytest.mark.timeout(900)
+def test_full_form_subworld_fp64():
+    """The full form (262,144 bodies, 64 sub-worlds over the heightfield) stepped on the GPU; the
+    sub-world whose body count is closest to 1,024 is replayed by the oracle on its own in FP32 and
+    FP64, in the GPU's solve order restricted to that sub-world (worlds never interact, so the
+    restricted order is its Gauss-Seidel order).  FP32: bit-exact bodies every 30 steps; FP64
+    (Jitter2.Double): the FP32 error after 120 steps, landing on the heightfield included."""
+    bodies, shapes, verts, cfg, dt = scenes.full_form()
+    counts = np.bincount(bodies["world_id"][bodies["motion_type"] == 0])
+    w = int(np.argmin(np.abs(counts - 1024)))
+    sb, ss, sv, bsel, smap = extract_world(bodies, shapes, verts, w)
+    print(f"full form: sub-world {w} with {counts[w]} dynamic bodies, {len(ss)} shapes", flush=True)
+    gw = GpuWorld(bodies, shapes, verts, cfg)
+    o32 = OracleWorld(sb, ss, sv, cfg)
+    o64 = OracleWorld(sb, ss, sv, cfg, fp64=True)
+    contacts = 0
+    for s in range(120):
+        gw.step(dt)
+        keys = gw.solve_order()[0].astype(np.int64)
+        mine = smap[keys[:, 0]] >= 0
+        assert np.array_equal(mine, smap[keys[:, 1]] >= 0), "a solved pair crosses worlds"
+        sub = smap[keys[mine]].astype(np.uint32)
+        contacts = max(contacts, len(sub))
+        o32.step(dt, sub)
+        o64.step(dt, sub)
+        if (s + 1) % 30 == 0:
+            assert_bodies_equal(gw.bodies()[bsel], o32.bodies(), f"full-form sub-world {w}, step {s}")
+    assert contacts > counts[w] // 2, f"only {contacts} arbiters solved in the sub-world"
+    g, s64 = gw.bodies()[bsel], o64.state64()
+    dyn = sb["motion_type"] == 0
+    dp = np.abs(g["position"].astype(np.float64) - s64[:, 0:3])[dyn].max()
+    dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10])[dyn].max()
+    print(f"full-form sub-world {w} ({counts[w]} bodies), 120 steps: FP32 GPU vs FP64 oracle max |dp| {dp:.3e} m, "
+          f"max |dv| {dv:.3e} m/s; {contacts} arbiters solved at most", flush=True)
+    assert dp < 1e-2 and dv < 1e-1
