@@ -2482,6 +2482,8 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         for (int n : uniq({cu / 16, cu / 8, cu / 4, cu / 2, 3 * cu / 4, cu, 3 * cu / 2, 2 * cu}, w->flow_max))
             sc.push_back({SOLVER_FLOW, n, 0});
         for (int n : uniq({cu / 4, cu / 2, cu, 2 * cu}, w->lane_max)) sc.push_back({SOLVER_LANES, n, 0});
+        // colour batches with a relaxed grid barrier each: wins when the batches are large (many worlds)
+        for (int n : uniq({cu / 2, cu, 2 * cu}, w->coop_solve)) sc.push_back({SOLVER_BARRIER, n, 0});
         for (int T : {cu / 2, cu, 2 * cu})
             if (T >= 1 && tile_lds_bytes(w, T) > 0) sc.push_back({SOLVER_TILES, T, 0});
         // one workgroup per world: hopeless for a few large worlds (a 131k-body world takes ~0.5 s)

@@ -125,26 +125,42 @@ def extract_world(bodies, shapes, verts, w):
             ss["vertex_offset"][k] = off
             off += n
     sv = np.concatenate(parts).reshape(-1) if parts else None
-    return sb, ss, sv, bsel, smap
+    return sb, ss, sv, bsel, bmap, smap
+
+
+def subworld_arbiters(arb, bmap, smap):
+    """The batch's arbiters of one sub-world, with shape and body indices remapped (key order kept)."""
+    mine = smap[arb["shape1"].astype(np.int64)] >= 0
+    out = arb[mine].copy()
+    for f, m in (("shape1", smap), ("shape2", smap), ("body1", bmap), ("body2", bmap)):
+        out[f] = m[out[f].astype(np.int64)]
+    return out
 
 
 @pytest.mark.timeout(900)
 def test_full_form_subworld_fp64():
     """The full form (262,144 bodies, 64 sub-worlds over the heightfield) stepped on the GPU; the
-    sub-world whose body count is closest to 1,024 is replayed by the oracle on its own in FP32 and
-    FP64, in the GPU's solve order restricted to that sub-world (worlds never interact, so the
-    restricted order is its Gauss-Seidel order).  FP32: bit-exact bodies every 30 steps; FP64
-    (Jitter2.Double): the FP32 error after 120 steps, landing on the heightfield included."""
+    sub-world whose body count is closest to 1,024 is replayed by the oracle on its own in FP32, in
+    the GPU's solve order restricted to that sub-world (worlds never interact, so the restricted order
+    is its Gauss-Seidel order): bit-exact bodies every 30 steps.  FP64 (Jitter2.Double): at steps 30,
+    60, 90 and 120 the FP64 oracle takes the GPU's state (bodies, contact cache) and steps it once in
+    the GPU's order; the FP32 step's error against it is bounded.  The landing heap is chaotic, so the
+    two trajectories drift apart over many steps; that divergence is reported, not bounded."""
     bodies, shapes, verts, cfg, dt = scenes.full_form()
     counts = np.bincount(bodies["world_id"][bodies["motion_type"] == 0])
     w = int(np.argmin(np.abs(counts - 1024)))
-    sb, ss, sv, bsel, smap = extract_world(bodies, shapes, verts, w)
+    sb, ss, sv, bsel, bmap, smap = extract_world(bodies, shapes, verts, w)
     print(f"full form: sub-world {w} with {counts[w]} dynamic bodies, {len(ss)} shapes", flush=True)
     gw = GpuWorld(bodies, shapes, verts, cfg)
     o32 = OracleWorld(sb, ss, sv, cfg)
-    o64 = OracleWorld(sb, ss, sv, cfg, fp64=True)
-    contacts = 0
+    o64_run = OracleWorld(sb, ss, sv, cfg, fp64=True)
+    dyn = sb["motion_type"] == 0
+    contacts, worst = 0, (0.0, 0.0)
     for s in range(120):
+        check = (s + 1) % 30 == 0
+        if check:  # one FP64 step from the GPU's state
+            o64 = OracleWorld(with_props(sb, gw.bodies()[bsel]), ss, sv, cfg, fp64=True)
+            o64.set_arbiters(subworld_arbiters(gw.arbiters(), bmap, smap))
         gw.step(dt)
         keys = gw.solve_order()[0].astype(np.int64)
         mine = smap[keys[:, 0]] >= 0
@@ -152,14 +168,20 @@ def test_full_form_subworld_fp64():
         sub = smap[keys[mine]].astype(np.uint32)
         contacts = max(contacts, len(sub))
         o32.step(dt, sub)
-        o64.step(dt, sub)
-        if (s + 1) % 30 == 0:
-            assert_bodies_equal(gw.bodies()[bsel], o32.bodies(), f"full-form sub-world {w}, step {s}")
+        o64_run.step(dt, sub)
+        if check:
+            g = gw.bodies()[bsel]
+            assert_bodies_equal(g, o32.bodies(), f"full-form sub-world {w}, step {s}")
+            o64.step(dt, sub)
+            s64 = o64.state64()
+            dp = np.abs(g["position"].astype(np.float64) - s64[:, 0:3])[dyn].max()
+            dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10])[dyn].max()
+            worst = (max(worst[0], dp), max(worst[1], dv))
+            print(f"  step {s}: one FP32 step vs FP64 max |dp| {dp:.3e} m, max |dv| {dv:.3e} m/s", flush=True)
     assert contacts > counts[w] // 2, f"only {contacts} arbiters solved in the sub-world"
-    g, s64 = gw.bodies()[bsel], o64.state64()
-    dyn = sb["motion_type"] == 0
-    dp = np.abs(g["position"].astype(np.float64) - s64[:, 0:3])[dyn].max()
-    dv = np.abs(g["velocity"].astype(np.float64) - s64[:, 7:10])[dyn].max()
-    print(f"full-form sub-world {w} ({counts[w]} bodies), 120 steps: FP32 GPU vs FP64 oracle max |dp| {dp:.3e} m, "
-          f"max |dv| {dv:.3e} m/s; {contacts} arbiters solved at most", flush=True)
-    assert dp < 1e-2 and dv < 1e-1
+    g, s64 = gw.bodies()[bsel], o64_run.state64()
+    ddp = np.abs(g["position"].astype(np.float64) - s64[:, 0:3])[dyn].max()
+    print(f"full-form sub-world {w} ({counts[w]} bodies): one-step FP32 error vs FP64 at most |dp| {worst[0]:.3e} m, "
+          f"|dv| {worst[1]:.3e} m/s; 120-step trajectories apart by up to {ddp:.3e} m; {contacts} arbiters solved "
+          f"at most", flush=True)
+    assert worst[0] < 1e-3 and worst[1] < 5e-2
