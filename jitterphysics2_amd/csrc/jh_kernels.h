@@ -2031,6 +2031,15 @@ __device__ __forceinline__ void coh_publish_vw(__amdgpu_buffer_rsrc_t r, int b, 
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, make_float4(w.x, w.y, w.z, fn)), r, b * 32 + 16,
                                            0, kCohSc1);
 }
+// Device-coherent read of a whole record (the integration loops of k_solve_flow, which sits between
+// relaxed grid barriers): returns the count, v and w.
+__device__ __forceinline__ int coh_ld_vw(__amdgpu_buffer_rsrc_t r, int b, vec3& v, vec3& w) {
+    float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32, 0, kCohSc1));
+    float4 c = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, b * 32 + 16, 0, kCohSc1));
+    v = {a.x, a.y, a.z};
+    w = {c.x, c.y, c.z};
+    return __float_as_int(a.w);
+}
 // Plain accesses, for phases separated from the dataflow passes by grid barriers.
 __device__ __forceinline__ int ld_vw(const SolveArgs& P, int b, vec3& v, vec3& w) {
     float4 a = P.vw[2 * b], c = P.vw[2 * b + 1];
@@ -2365,11 +2374,12 @@ __device__ __forceinline__ bool flow_visit(const SolveArgs& P, const FlowHead& h
     load_static_part(B, b2, R2);
     vec3 x1, x2;
     quat q1, q2;
-    if (PREPARE) {
-        x1 = ld3(B.pos[b1]);
-        x2 = ld3(B.pos[b2]);
-        q1 = ldq(B.quat[b1]);
-        q2 = ldq(B.quat[b2]);
+    if (PREPARE) {  // poses written by the previous substep's IntegrateVelocities, in this launch
+        CohBodies cb = coh_bodies(B, P.nb);
+        x1 = ld3(coh_ld(cb.pos, b1));
+        x2 = ld3(coh_ld(cb.pos, b2));
+        q1 = ldq(coh_ld(cb.quat, b1));
+        q2 = ldq(coh_ld(cb.quat, b2));
     }
     SlotPrep sp[4];
     if (PREPARE) {
@@ -2377,9 +2387,9 @@ __device__ __forceinline__ bool flow_visit(const SolveArgs& P, const FlowHead& h
         for (int s = 0; s < 4; ++s)
             if (usage & (1u << s)) prepare_fixed(mode, sr[s], R1, R2, x1, q1, x2, q2, P.ts, sp[s]);
     }
-    if (!m1) ld_vw(P, b1, R1.v, R1.w);
-    if (!m2) ld_vw(P, b2, R2.v, R2.w);
     __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
+    if (!m1) coh_ld_vw(vr, b1, R1.v, R1.w);
+    if (!m2) coh_ld_vw(vr, b2, R2.v, R2.w);
     // Each lane computes and publishes as soon as its own inputs are ready; the wave leaves when all
     // of its lanes have (a lane never waits for an unrelated lane's dependencies).  An input that
     // showed the awaited version is not polled again (no other visit can change it before this one).
@@ -2518,6 +2528,10 @@ __device__ __forceinline__ bool flow_passes(const SolveArgs& P, int g_first, int
 
 // One launch per step (World.Step.cs:234-253): per substep IntegrateForces | prepare + iterations
 // (dataflow) | IntegrateVelocities | relaxations (dataflow), grid barriers only between these phases.
+// Everything workgroups share within the launch — the velocity records, the poses IntegrateVelocities
+// writes for the next substep's prepare, the chunk counters, the versioned contact fields — goes
+// through device-coherent (sc1) accesses, so the phase barriers are relaxed ones (each wave's accesses
+// complete, one relaxed arrival per workgroup; no L2 write-back or invalidate).
 __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
     __shared__ int own_all[4][kOwnMax];  // chunks owned by each wave in the current phase
     int* own = own_all[threadIdx.x >> 6];
@@ -2525,46 +2539,49 @@ __global__ void __launch_bounds__(256) k_solve_flow(SolveArgs P) {
     int t0 = blockIdx.x * blockDim.x + threadIdx.x;
     int per_sub = 1 + P.iterations + P.relaxations;
     const Bodies& B = P.B;
-    for (int b = t0; b < P.nb; b += stride) st_vw(P, b, ld3(B.vel[b]), ld3(B.angvel[b]), 0);
+    __amdgpu_buffer_rsrc_t vr = vw_rsrc(P);
+    CohBodies cb = coh_bodies(B, P.nb);
+    for (int b = t0; b < P.nb; b += stride) coh_publish_vw(vr, b, ld3(B.vel[b]), ld3(B.angvel[b]), 0);
     if (P.qmode)  // visible after the first grid barrier
-        for (int i = t0; i < 2 * P.ts.substeps; i += stride) P.queue[i] = 0;
+        for (int i = t0; i < 2 * P.ts.substeps; i += stride)
+            __hip_atomic_store(&P.queue[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int s = 0; s < P.ts.substeps; ++s) {
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateForces (World.Step.cs:1060-1072)
             if (B.motion[b] != JH_MOTION_DYNAMIC) continue;
             vec3 v, w;
-            int n = ld_vw(P, b, v, w);
+            int n = coh_ld_vw(vr, b, v, w);
             w = w + ld3(B.dangvel[b]);
             v = v + ld3(B.dvel[b]);
-            st_vw(P, b, v, w, n);
+            coh_publish_vw(vr, b, v, w, n);
         }
-        if (!grid_barrier(P.bar, P.spin)) return;
+        if (!grid_barrier_coh(P.bar, P.spin)) return;
         bool ok;
         if (P.qmode == 2) ok = flow_passes_queue(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s);
         else if (P.qmode == 1) ok = flow_passes_own(P, s * per_sub, 1 + P.iterations, 0, 1, P.queue + 2 * s, own);
         else ok = flow_passes(P, s * per_sub, 1 + P.iterations, 0, 1);
         // a wave that gave up still takes part in the barrier, which then fails for the workgroup
-        if (!grid_barrier(P.bar, P.spin, ok)) return;
+        if (!grid_barrier_coh(P.bar, P.spin, ok)) return;
         for (int b = t0; b < P.nb; b += stride) {  // IntegrateVelocities (World.Step.cs:1106-1133)
             if (B.motion[b] == JH_MOTION_STATIC) continue;
             vec3 v, w;
-            ld_vw(P, b, v, w);
-            vec3 p = ld3(B.pos[b]) + v * P.ts.substep_dt;
-            B.pos[b] = st3(p);
+            coh_ld_vw(vr, b, v, w);
+            vec3 p = ld3(coh_ld(cb.pos, b)) + v * P.ts.substep_dt;
+            coh_st(cb.pos, b, st3(p));
             quat rq = rotation_quaternion(w, P.ts.substep_dt);
-            B.quat[b] = stq(qnormalize(qmul(rq, ldq(B.quat[b]))));
+            coh_st(cb.quat, b, stq(qnormalize(qmul(rq, ldq(coh_ld(cb.quat, b))))));
         }
-        if (!grid_barrier(P.bar, P.spin)) return;
+        if (!grid_barrier_coh(P.bar, P.spin)) return;
         if (P.relaxations > 0) {
             int g0 = s * per_sub + 1 + P.iterations;
             if (P.qmode == 2) ok = flow_passes_queue(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1);
             else if (P.qmode == 1) ok = flow_passes_own(P, g0, P.relaxations, 2, 2, P.queue + 2 * s + 1, own);
             else ok = flow_passes(P, g0, P.relaxations, 2, 2);
-            if (!grid_barrier(P.bar, P.spin, ok)) return;
+            if (!grid_barrier_coh(P.bar, P.spin, ok)) return;
         }
     }
     for (int b = t0; b < P.nb; b += stride) {
         vec3 v, w;
-        ld_vw(P, b, v, w);
+        coh_ld_vw(vr, b, v, w);
         B.vel[b] = st3(v);
         B.angvel[b] = st3(w);
     }
