@@ -83,7 +83,8 @@ __device__ __forceinline__ vec3 hull_support(const float* v, int V, vec3 d) {
 
 struct ShapeView {
     const ShapeRec* s;
-    const float* verts;  // all vertices (x, y, z) of the world
+    const float* verts;           // all vertices (x, y, z) of the world
+    const float* lv = nullptr;    // a point cloud's own vertices staged in LDS (k_narrow_hulls), else null
 
     // ISupportMappable.SupportMap, local frame, direction not normalised.
     __device__ __forceinline__ vec3 support(vec3 d) const {
@@ -99,7 +100,7 @@ struct ShapeView {
                 return r;
             }
             case JH_SHAPE_POINTCLOUD: {  // VertexSupportMap.cs:64-109 (last maximum wins) + shift
-                const float* v = verts + 3 * s->voff;
+                const float* v = lv ? lv : verts + 3 * s->voff;
                 float best = -3.402823466e+38f;
                 int idx = 0;
                 for (int i = 0; i < s->vcnt; ++i) {

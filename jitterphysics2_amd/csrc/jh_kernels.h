@@ -908,75 +908,141 @@ enum { kDeepPlain = 0, kDeepFilter = 1, kDeepRerun = 2 };
 // Separated pairs with a body that has speculative contacts and a relative speed of at least the
 // speculative velocity threshold (spec_thr_sq = its square) go to the sweep queue (World.Detect.cs:
 // 167-176); spec_any = 0 when no body has speculative contacts.
+// Narrowphase of one pair (World.Detect.cs:125-218): MPR, then the manifold, the EPA queue, the
+// triangle edge filter or the speculative sweep queue.  mk may carry LDS copies of point clouds.
+__device__ __forceinline__ void narrow_pair(int p, uint64_t key, MinkowskiPair& mk, quat qa, quat qb, vec3 pa, vec3 pb,
+                                            const ShapeRec* __restrict__ shapes, const float* __restrict__ verts,
+                                            const Bodies& B, int aux, int filter, jh_manifold* __restrict__ out,
+                                            DeepRec* __restrict__ deep, int* __restrict__ deep_count, int spec_any,
+                                            float spec_thr_sq, int* __restrict__ spec_queue,
+                                            int* __restrict__ spec_count) {
+    int side = filter ? tef::applies(*mk.a.s, *mk.b.s, B.motion) : 0;
+    MprOut r;
+    solve_mpr(mk, side ? 0.0f : 0.02f, r);
+    if (!r.hit) {
+        write_manifold(out[p], key, r, qa, pa, nullptr, 0);
+        if (spec_any) {
+            int b1 = mk.a.s->body, b2 = mk.b.s->body;
+            if (B.mat2[b1].y != 0.0f || B.mat2[b2].y != 0.0f) {
+                vec3 dv = ld3(B.vel[b2]) - ld3(B.vel[b1]);
+                if (!(len_sq(dv) < spec_thr_sq)) spec_queue[atomicAdd(spec_count, 1)] = p;
+            }
+        }
+        return;
+    }
+    if (!side) {
+        if (r.deep) {
+            int slot = atomicAdd(deep_count, 1);
+            DeepRec& d = deep[slot];
+            d.pair = p;
+            d.kind = kDeepPlain;
+            for (int k = 0; k < 4; ++k) d.portal[k] = r.portal[k];
+        }
+        ManifoldBuf m;
+        manifold_for(mk, qa, qb, pa, pb, r, aux, m);
+        write_manifold(out[p], key, r, qa, pa, &m, 0);
+        return;
+    }
+    tef::Ctx c = filter_ctx(side, shapes, key, verts, qa, qb, pa, pb);
+    tef::WRes w = to_world(r, qa, pa);
+    bool keep;
+    int kind = -1;
+    if (r.pen > 0.02f) {  // MprEpa ran EPA already: the re-run repeats the same answer
+        kind = kDeepFilter;
+        keep = tef::filter_same(c, w);
+    } else {
+        vec3 tn, nn;
+        int e = tef::eval_res(c, w, tn, nn);
+        if (e == tef::Discard) {
+            keep = false;
+        } else if (e == tef::Keep) {
+            keep = true;
+        } else if (w.pen < 0.0f) {
+            keep = tef::adjust(w, tn, nn);
+        } else {
+            if (r.deep) kind = kDeepRerun;  // the re-run refines with EPA (penetration > 0)
+            keep = tef::after_rerun(c, w);  // answer if EPA fails: the MPR one again
+        }
+    }
+    if (kind >= 0) {
+        int slot = atomicAdd(deep_count, 1);
+        DeepRec& d = deep[slot];
+        d.pair = p;
+        d.kind = kind;
+        for (int k = 0; k < 4; ++k) d.portal[k] = r.portal[k];
+    }
+    write_filtered(out[p], key, w, keep, mk, qa, qb, pa, pb, aux, 0);
+}
+
+// Optionally (JH_NARROW_LDS=1), point clouds of up to kLdsCloud vertices are narrowphased by
+// k_narrow_hulls: k_narrow leaves their pairs to it (hull_list), and it stages each pair's clouds in
+// LDS once, so the support scans of MPR and the manifold read LDS instead of the L2 (a 32-point cloud
+// is 384 bytes, scanned whole by every support call).  Bit-exact, but slower on the full form
+// (narrowphase 8.4 against 6.0 ms): 768 bytes of LDS per lane leave 3 waves per CU, and the L2
+// already serves the clouds, so it is off by default.
+constexpr int kLdsCloud = 32;
+__host__ __device__ __forceinline__ bool lds_cloud(const ShapeRec& s) {
+    return s.type == JH_SHAPE_POINTCLOUD && s.vcnt <= kLdsCloud;
+}
+
 __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
                                                 const ShapeRec* __restrict__ shapes, const float* __restrict__ verts,
                                                 Bodies B, int aux, int filter, jh_manifold* __restrict__ out,
                                                 DeepRec* __restrict__ deep, int* __restrict__ deep_count,
                                                 int spec_any, float spec_thr_sq, int* __restrict__ spec_queue,
-                                                int* __restrict__ spec_count) {
+                                                int* __restrict__ spec_count, int* __restrict__ hull_list,
+                                                int* __restrict__ hull_count) {
     int n = *n_ptr;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint64_t key = pairs[p];
+        if (hull_list) {
+            uint32_t s1 = uint32_t(key >> 32), s2 = uint32_t(key & 0xffffffffu);
+            if (lds_cloud(shapes[s1]) || lds_cloud(shapes[s2])) {
+                hull_list[atomicAdd(hull_count, 1)] = p;
+                continue;
+            }
+        }
+        MinkowskiPair mk;
+        quat qa, qb;
+        vec3 pa, pb;
+        pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
+        narrow_pair(p, key, mk, qa, qb, pa, pb, shapes, verts, B, aux, filter, out, deep, deep_count, spec_any,
+                    spec_thr_sq, spec_queue, spec_count);
+    }
+}
+
+// The pairs k_narrow left: one wavefront per workgroup, each lane's pair's point clouds copied to its
+// own 768 bytes of LDS (two clouds of up to kLdsCloud vertices), then the same narrowphase.
+__global__ void __launch_bounds__(64) k_narrow_hulls(const uint64_t* __restrict__ pairs,
+                                                     const int* __restrict__ hull_list,
+                                                     const int* __restrict__ hull_count,
+                                                     const ShapeRec* __restrict__ shapes,
+                                                     const float* __restrict__ verts, Bodies B, int aux, int filter,
+                                                     jh_manifold* __restrict__ out, DeepRec* __restrict__ deep,
+                                                     int* __restrict__ deep_count, int spec_any, float spec_thr_sq,
+                                                     int* __restrict__ spec_queue, int* __restrict__ spec_count) {
+    __shared__ float cloud[64][2][3 * kLdsCloud];
+    int n = *hull_count;
+    int t = threadIdx.x;
+    for (int i = blockIdx.x * blockDim.x + t; i < n; i += gridDim.x * blockDim.x) {
+        int p = hull_list[i];
         uint64_t key = pairs[p];
         MinkowskiPair mk;
         quat qa, qb;
         vec3 pa, pb;
         pair_frame(shapes, verts, B, key, mk, qa, qb, pa, pb);
-        int side = filter ? tef::applies(*mk.a.s, *mk.b.s, B.motion) : 0;
-        MprOut r;
-        solve_mpr(mk, side ? 0.0f : 0.02f, r);
-        if (!r.hit) {
-            write_manifold(out[p], key, r, qa, pa, nullptr, 0);
-            if (spec_any) {
-                int b1 = mk.a.s->body, b2 = mk.b.s->body;
-                if (B.mat2[b1].y != 0.0f || B.mat2[b2].y != 0.0f) {
-                    vec3 dv = ld3(B.vel[b2]) - ld3(B.vel[b1]);
-                    if (!(len_sq(dv) < spec_thr_sq)) spec_queue[atomicAdd(spec_count, 1)] = p;
-                }
-            }
-            continue;
+        if (lds_cloud(*mk.a.s)) {
+            const float* src = verts + 3 * size_t(mk.a.s->voff);
+            for (int k = 0; k < 3 * mk.a.s->vcnt; ++k) cloud[t][0][k] = src[k];
+            mk.a.lv = cloud[t][0];
         }
-        if (!side) {
-            if (r.deep) {
-                int slot = atomicAdd(deep_count, 1);
-                DeepRec& d = deep[slot];
-                d.pair = p;
-                d.kind = kDeepPlain;
-                for (int k = 0; k < 4; ++k) d.portal[k] = r.portal[k];
-            }
-            ManifoldBuf m;
-            manifold_for(mk, qa, qb, pa, pb, r, aux, m);
-            write_manifold(out[p], key, r, qa, pa, &m, 0);
-            continue;
+        if (lds_cloud(*mk.b.s)) {
+            const float* src = verts + 3 * size_t(mk.b.s->voff);
+            for (int k = 0; k < 3 * mk.b.s->vcnt; ++k) cloud[t][1][k] = src[k];
+            mk.b.lv = cloud[t][1];
         }
-        tef::Ctx c = filter_ctx(side, shapes, key, verts, qa, qb, pa, pb);
-        tef::WRes w = to_world(r, qa, pa);
-        bool keep;
-        int kind = -1;
-        if (r.pen > 0.02f) {  // MprEpa ran EPA already: the re-run repeats the same answer
-            kind = kDeepFilter;
-            keep = tef::filter_same(c, w);
-        } else {
-            vec3 tn, nn;
-            int e = tef::eval_res(c, w, tn, nn);
-            if (e == tef::Discard) {
-                keep = false;
-            } else if (e == tef::Keep) {
-                keep = true;
-            } else if (w.pen < 0.0f) {
-                keep = tef::adjust(w, tn, nn);
-            } else {
-                if (r.deep) kind = kDeepRerun;  // the re-run refines with EPA (penetration > 0)
-                keep = tef::after_rerun(c, w);  // answer if EPA fails: the MPR one again
-            }
-        }
-        if (kind >= 0) {
-            int slot = atomicAdd(deep_count, 1);
-            DeepRec& d = deep[slot];
-            d.pair = p;
-            d.kind = kind;
-            for (int k = 0; k < 4; ++k) d.portal[k] = r.portal[k];
-        }
-        write_filtered(out[p], key, w, keep, mk, qa, qb, pa, pb, aux, 0);
+        narrow_pair(p, key, mk, qa, qb, pa, pb, shapes, verts, B, aux, filter, out, deep, deep_count, spec_any,
+                    spec_thr_sq, spec_queue, spec_count);
     }
 }
 

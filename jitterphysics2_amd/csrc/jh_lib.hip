@@ -40,9 +40,14 @@ jh_status fail(jh_status s, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(JH_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));         \
     } while (0)
 
+// Device buffer owned by a world: freed by release() or, at the latest, with the world.
 template <class T> struct DBuf {
     T* p = nullptr;
     size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
     hipError_t ensure(size_t n, bool keep = false) {
         if (n <= cap && p) return hipSuccess;
         size_t nc = std::max<size_t>(n + n / 4, 64);
@@ -80,7 +85,7 @@ enum {
     C_NB = 11,   // body count
     C_CONTACTS = 12,  // contact points solved
     // 13: unused
-    C_NRANK = 14,     // rank keys
+    C_HULL = 14,      // pairs left to k_narrow_hulls
     C_SPEC = 15,      // speculative sweep queue
     C_NMOV = 16,      // tile solver: non-static bodies with shapes
     C_TCELLS = 17,    // tile solver: (tile, colour) cells + 1
@@ -191,6 +196,9 @@ struct jh_world {
     DBuf<char> epa_work, epa_side;
     DBuf<EpaRes> epa_res;
     DBuf<int> specq;  // pairs queued for the speculative sweep
+    DBuf<int> hullq;  // pairs with point clouds, narrowphased by k_narrow_hulls
+    bool has_clouds = false;  // some shape is a point cloud of at most kLdsCloud vertices
+    int narrow_lds = 0;       // stage those clouds in LDS (JH_NARROW_LDS=1; measured slower, off by default)
     DBuf<uint64_t> zkey, evb, eve;  // keys of arbiters removed by the last step; collision events
     // constraints (solved by the barrier solver after the contacts of each pass)
     std::vector<jh_constraint> host_con;
@@ -296,7 +304,7 @@ namespace {
 
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
-    if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_NRANK ||
+    if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_HULL ||
         t == C_MAXDEG || t == C_SPEC || t == C_EVB || t == C_EVE)
         cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
@@ -929,10 +937,18 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         int spec = w->n_spec > 0;
         if (spec) HIPCHK(w->specq.ensure(size_t(np)));
         float thr = cfg.speculative_velocity_threshold;
+        bool hulls = w->has_clouds && w->narrow_lds;
+        if (hulls) HIPCHK(w->hullq.ensure(size_t(np)));
         k_narrow<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->verts.p, B, aux,
                                                              cfg.narrowphase_filter, w->man.p, w->deep.p,
                                                              cnt + C_DEEP, spec, thr * thr, w->specq.p,
-                                                             cnt + C_SPEC);
+                                                             cnt + C_SPEC, hulls ? w->hullq.p : nullptr,
+                                                             cnt + C_HULL);
+        if (hulls)  // the pairs with point clouds, their vertices staged in LDS
+            k_narrow_hulls<<<grid_for(np, 64, 1 << 20), 64, 0, st>>>(pairs, w->hullq.p, cnt + C_HULL, w->shapes.p,
+                                                                      w->verts.p, B, aux, cfg.narrowphase_filter,
+                                                                      w->man.p, w->deep.p, cnt + C_DEEP, spec,
+                                                                      thr * thr, w->specq.p, cnt + C_SPEC);
         if (spec)
             k_sweep<<<grid_for(np, 64, 4096), 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, cfg.narrowphase_filter,
                                                            dt, cfg.speculative_relaxation_factor, w->specq.p,
@@ -1332,6 +1348,7 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     }
     if (const char* e = getenv("JH_FLOW_QUEUE")) w->flow_qmode = std::max(0, std::min(atoi(e), 2));
     if (const char* e = getenv("JH_TRAVERSE")) w->traverse_wave = strcmp(e, "thread") != 0;
+    if (const char* e = getenv("JH_NARROW_LDS")) w->narrow_lds = atoi(e) != 0;
     w->tiles = w->cu_count;
     if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
     w->colour_blocks = std::min(64, w->coop_colour);
@@ -1677,6 +1694,8 @@ jh_status jh_world_set_shapes(jh_world* w, const jh_shape* shapes, int32_t n, co
     }
     w->ns = n;
     w->nv = nv;
+    w->has_clouds = false;
+    for (int i = 0; i < n; ++i) w->has_clouds = w->has_clouds || lds_cloud(recs[i]);
     size_t nn = std::max(n, 2);
     HIPCHK(w->shapes.ensure(nn));
     HIPCHK(w->verts.ensure(std::max<size_t>(3 * size_t(nv), 3)));
