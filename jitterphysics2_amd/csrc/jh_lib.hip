@@ -178,6 +178,14 @@ struct jh_world {
     DBuf<uint64_t> mk0, mk1;
     DBuf<uint32_t> mv0, mv1;
     DBuf<int> left, right, parent, leaf_parent, nflags;
+    // broadphase launch sequence captured as a HIP graph (broadphase_signature), replayed every step
+    int use_graphs = 1;
+    hipGraph_t bp_graph = nullptr;
+    hipGraphExec_t bp_exec = nullptr;
+    std::vector<uint64_t> bp_sig;
+    uint64_t* bp_keys = nullptr;
+    uint32_t* bp_order = nullptr;
+    long stats_graph_captures = 0;
     DBuf<float4> nbox;           // LBVH child records (k_lbvh_refit, k_traverse_wave): 4 per node
     DBuf<int4> nrng;
     int traverse_wave = 1;       // pair traversal: 1 k_traverse_wave, 0 k_traverse (JH_TRAVERSE=thread)
@@ -462,17 +470,17 @@ jh_status ensure_pair_buffers(jh_world* w, int n) {
 
 // Broadphase: AABBs, Morton keys, sort, LBVH, refit, traversal, pair sort.  Syncs once to read the
 // pair count (growing the pair buffer and re-traversing on overflow).
-jh_status broadphase(jh_world* w) {
+void traverse_launch(jh_world* w, const uint64_t* keys, const uint32_t* order);
+
+// The broadphase launches up to the first traversal: the same sequence every step while the shapes,
+// the buffers and the broadphase step length stay, so it is captured once into a HIP graph and
+// replayed (JH_GRAPH=0: launched directly).  Returns where the sorted keys and shape order ended.
+void broadphase_launches(jh_world* w, uint64_t*& keys, uint32_t*& order) {
     hipStream_t st = w->st;
     Bodies B = w->bodies();
     int ns = w->ns;
     int* cnt = w->cnt.p;
     k_reset_counters<<<1, 64, 0, st>>>(cnt, w->bounds.p);
-    w->n_pairs = 0;
-    if (ns < 2) {
-        HIPCHK(hipGetLastError());
-        return JH_OK;
-    }
     int g = grid_for(ns, 256, 1 << 20);
     k_aabb<<<g, 256, 0, st>>>(w->shapes.p, ns, w->verts.p, B, w->broadphase_dt, w->amin.p, w->amax.p, w->bounds.p);
     k_morton<<<g, 256, 0, st>>>(w->shapes.p, ns, w->world.p, w->amin.p, w->amax.p, w->bounds.p, w->mk0.p, w->mv0.p);
@@ -482,9 +490,8 @@ jh_status broadphase(jh_world* w) {
     int wb = bits_for(uint32_t(w->max_world));
     for (int s = 32; s < 32 + wb; s += 8) shifts[nsh++] = s;
     bool in1 = rsort::sort_pairs(w->mk0.p, w->mk1.p, w->mv0.p, w->mv1.p, cnt + C_NS, ns, shifts, nsh, w->hist.p, st);
-    uint64_t* keys = in1 ? w->mk1.p : w->mk0.p;
-    uint32_t* order = in1 ? w->mv1.p : w->mv0.p;
-    w->sorted_in1 = in1;
+    keys = in1 ? w->mk1.p : w->mk0.p;
+    order = in1 ? w->mv1.p : w->mv0.p;
     int gi = grid_for(ns - 1, 256, 1 << 20);
     k_lbvh_build<<<gi, 256, 0, st>>>(keys, ns, w->left.p, w->right.p, w->range.p, w->parent.p, w->leaf_parent.p,
                                      w->nflags.p);
@@ -493,15 +500,84 @@ jh_status broadphase(jh_world* w) {
                                     w->parent.p, w->leaf_parent.p, w->nmin.p, w->nmax.p, w->nflags.p, w->shapes.p,
                                     w->motion.p, sfl, w->range.p, w->traverse_wave ? w->nbox.p : nullptr,
                                     w->nrng.p);
+    traverse_launch(w, keys, order);
+}
+
+void traverse_launch(jh_world* w, const uint64_t* keys, const uint32_t* order) {
+    hipStream_t st = w->st;
+    int ns = w->ns;
+    int* cnt = w->cnt.p;
+    int g = grid_for(ns, 256, 1 << 20);
+    const uint32_t* sfl = w->islands_on() ? w->sflags.p : nullptr;
+    k_set_int<<<1, 1, 0, st>>>(cnt + C_PAIRS, 0);
+    if (w->traverse_wave)
+        k_traverse_wave<<<g, 256, 0, st>>>(ns, keys, w->lmin.p, w->lmax.p, w->nbox.p, w->nrng.p, w->pk0.p,
+                                           cnt + C_PAIRS, w->pair_cap, cnt + C_TERR);
+    else
+        k_traverse<<<g, 256, 0, st>>>(ns, keys, order, w->shapes.p, w->motion.p, w->lmin.p, w->lmax.p, w->left.p,
+                                      w->right.p, w->range.p, w->nmin.p, w->nmax.p, w->pk0.p, cnt + C_PAIRS,
+                                      w->pair_cap, cnt + C_TERR, sfl);
+}
+
+// What the broadphase graph's launches depend on: a graph is replayed only while this is unchanged.
+std::vector<uint64_t> broadphase_signature(jh_world* w) {
+    std::vector<uint64_t> v;
+    Bodies B = w->bodies();
+    const uint64_t* bw = reinterpret_cast<const uint64_t*>(&B);
+    v.assign(bw, bw + (sizeof(Bodies) + 7) / 8);
+    uint32_t dtb;
+    memcpy(&dtb, &w->broadphase_dt, 4);
+    const void* ptrs[] = {w->cnt.p, w->bounds.p, w->shapes.p, w->verts.p, w->world.p, w->amin.p, w->amax.p,
+                          w->mk0.p, w->mk1.p, w->mv0.p, w->mv1.p, w->hist.p, w->left.p, w->right.p, w->range.p,
+                          w->parent.p, w->leaf_parent.p, w->nflags.p, w->lmin.p, w->lmax.p, w->nmin.p, w->nmax.p,
+                          w->nbox.p, w->nrng.p, w->motion.p, w->sflags.p, w->pk0.p};
+    for (const void* p : ptrs) v.push_back(uint64_t(uintptr_t(p)));
+    v.push_back(uint64_t(uint32_t(w->ns)) | uint64_t(uint32_t(w->max_world)) << 32);
+    v.push_back(uint64_t(uint32_t(w->pair_cap)) | uint64_t(dtb) << 32);
+    v.push_back(uint64_t(w->traverse_wave) | uint64_t(w->islands_on() ? 1 : 0) << 8);
+    return v;
+}
+
+jh_status broadphase(jh_world* w) {
+    hipStream_t st = w->st;
+    int ns = w->ns;
+    int* cnt = w->cnt.p;
+    w->n_pairs = 0;
+    if (ns < 2) {
+        k_reset_counters<<<1, 64, 0, st>>>(cnt, w->bounds.p);
+        HIPCHK(hipGetLastError());
+        return JH_OK;
+    }
+    uint64_t* keys = nullptr;
+    uint32_t* order = nullptr;
+    if (w->use_graphs) {
+        std::vector<uint64_t> sig = broadphase_signature(w);
+        if (!w->bp_exec || sig != w->bp_sig) {  // capture the launches once for this configuration
+            if (w->bp_exec) HIPCHK(hipGraphExecDestroy(w->bp_exec));
+            if (w->bp_graph) HIPCHK(hipGraphDestroy(w->bp_graph));
+            w->bp_exec = nullptr;
+            w->bp_graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            broadphase_launches(w, w->bp_keys, w->bp_order);
+            hipError_t e = hipGetLastError();
+            hipGraph_t gr = nullptr;
+            hipError_t e2 = hipStreamEndCapture(st, &gr);
+            if (e != hipSuccess) return fail(JH_ERR_HIP, "broadphase launch: %s", hipGetErrorString(e));
+            if (e2 != hipSuccess) return fail(JH_ERR_HIP, "broadphase capture: %s", hipGetErrorString(e2));
+            w->bp_graph = gr;
+            HIPCHK(hipGraphInstantiate(&w->bp_exec, gr, nullptr, nullptr, 0));
+            w->bp_sig = std::move(sig);
+            w->stats_graph_captures++;
+        }
+        HIPCHK(hipGraphLaunch(w->bp_exec, st));
+        keys = w->bp_keys;
+        order = w->bp_order;
+    } else {
+        broadphase_launches(w, keys, order);
+    }
+    w->sorted_in1 = keys == w->mk1.p;
     for (int attempt = 0; attempt < 3; ++attempt) {
-        k_set_int<<<1, 1, 0, st>>>(cnt + C_PAIRS, 0);
-        if (w->traverse_wave)
-            k_traverse_wave<<<g, 256, 0, st>>>(ns, keys, w->lmin.p, w->lmax.p, w->nbox.p, w->nrng.p, w->pk0.p,
-                                               cnt + C_PAIRS, w->pair_cap, cnt + C_TERR);
-        else
-            k_traverse<<<g, 256, 0, st>>>(ns, keys, order, w->shapes.p, w->motion.p, w->lmin.p, w->lmax.p, w->left.p,
-                                          w->right.p, w->range.p, w->nmin.p, w->nmax.p, w->pk0.p, cnt + C_PAIRS,
-                                          w->pair_cap, cnt + C_TERR, sfl);
+        if (attempt > 0) traverse_launch(w, keys, order);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(w->hcnt, cnt, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -1349,6 +1425,7 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     if (const char* e = getenv("JH_FLOW_QUEUE")) w->flow_qmode = std::max(0, std::min(atoi(e), 2));
     if (const char* e = getenv("JH_TRAVERSE")) w->traverse_wave = strcmp(e, "thread") != 0;
     if (const char* e = getenv("JH_NARROW_LDS")) w->narrow_lds = atoi(e) != 0;
+    if (const char* e = getenv("JH_GRAPH")) w->use_graphs = atoi(e) != 0;
     w->tiles = w->cu_count;
     if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
     w->colour_blocks = std::min(64, w->coop_colour);
@@ -1388,6 +1465,8 @@ jh_status jh_world_destroy(jh_world* w) {
     w->bar.release();
     w->lws.release(); w->pvel.release(); w->pang.release(); w->parb.release(); w->ppart.release();
     w->pfeat.release(); w->ptarg.release(); w->pvalid.release(); w->plam.release(); w->plearn.release();
+    if (w->bp_exec) (void)hipGraphExecDestroy(w->bp_exec);
+    if (w->bp_graph) (void)hipGraphDestroy(w->bp_graph);
     for (auto& e : w->ev) (void)hipEventDestroy(e);
     if (w->hcnt) (void)hipHostFree(w->hcnt);
     (void)hipStreamDestroy(w->st);
