@@ -514,11 +514,11 @@ int orc_stabilize(void* p, float dt, int iters, int relax, const uint32_t* order
     return 0;
 }
 // Step with residual probes of the first substep (modes: bit 0 previous-frame warm start, bit 2
-// cold); residual[3][iterations + 1], features [capacity][24], targets [capacity][3];
-// counts[0] = rows written, counts[1] = contact points.
+// cold); residual[3][iterations + 1], features [capacity][24], targets [capacity][3], flags and the
+// two bodies of each row (optional); counts[0] = rows written, counts[1] = contact points.
 int orc_step_probe(void* p, float dt, const uint32_t* order, int nkeys, int iterations, int modes, double* residual,
                    float* features, float* targets, long capacity, long* counts, const float* learned,
-                   uint32_t* flags) {
+                   uint32_t* flags, int32_t* row_bodies) {
     Handle* h = (Handle*)p;
     std::vector<std::pair<uint32_t, uint32_t>> ord(nkeys);
     for (int i = 0; i < nkeys; ++i) ord[i] = {order[2 * i], order[2 * i + 1]};
@@ -533,6 +533,7 @@ int orc_step_probe(void* p, float dt, const uint32_t* order, int nkeys, int iter
         pr.targets = targets;
         pr.learned = learned;
         pr.flags = flags;
+        pr.row_bodies = row_bodies;
         W.step_impl(decltype(W.gravity.x)(dt), &ord, &pr);
         counts[0] = pr.rows;
         counts[1] = pr.points;

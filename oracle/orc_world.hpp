@@ -927,6 +927,7 @@ template <class R> struct World {
         const float* learned = nullptr;  // [capacity][3]: the warm start of probe 1 (an external
                                          // network's impulses, replayed), rows in trace order
         uint32_t* flags = nullptr;       // [capacity]: the contact flags of each row at the warm start
+        int32_t* row_bodies = nullptr;   // [capacity][2]: body1, body2 of each row
         long rows = 0, points = 0;
     };
 
@@ -985,6 +986,10 @@ template <class R> struct World {
                 if (!(kv.second.usage & (1u << s)) || !arbiter_active(kv.second)) continue;
                 if (pr.features && pr.rows < pr.capacity) contact_features(kv.second, s, pr.features + 24 * pr.rows);
                 if (pr.flags && pr.rows < pr.capacity) pr.flags[pr.rows] = kv.second.c[s].flag;
+                if (pr.row_bodies && pr.rows < pr.capacity) {
+                    pr.row_bodies[2 * pr.rows] = int32_t(kv.second.b1);
+                    pr.row_bodies[2 * pr.rows + 1] = int32_t(kv.second.b2);
+                }
                 if (pr.rows < pr.capacity) ++pr.rows;
                 ++pr.points;
             }

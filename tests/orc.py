@@ -69,7 +69,7 @@ def lib() -> ctypes.CDLL:
         L.orc_atan2_d.restype = ctypes.c_double
         L.orc_rotation_quaternion_f.argtypes = [vp, f, vp]
         L.orc_time_steps.argtypes = [vp, f, i32]
-        L.orc_step_probe.argtypes = [vp, f, vp, i32, i32, i32, vp, vp, vp, ctypes.c_long, vp, vp, vp]
+        L.orc_step_probe.argtypes = [vp, f, vp, i32, i32, i32, vp, vp, vp, ctypes.c_long, vp, vp, vp, vp]
         L.orc_step_probe.restype = i32
         L.orc_time_steps.restype = ctypes.c_double
         _lib = L
@@ -133,22 +133,23 @@ class OracleWorld:
                    learned: np.ndarray | None = None) -> dict:
         """Step with residual probes of the first substep (modes: 1 previous-frame warm start,
         2 the warm start `learned` [rows][3] given in trace-row order, 4 cold), replaying `order`
-        like step().  Returns also each trace row's contact flags at the warm start."""
+        like step().  Returns also each trace row's contact flags at the warm start and its two bodies."""
         o = np.ascontiguousarray(order, np.uint32).reshape(-1, 2)
         res = np.zeros((3, iterations + 1), np.float64)
         feats = np.zeros((max(capacity, 1), 24), np.float32)
         targs = np.zeros((max(capacity, 1), 3), np.float32)
         flags = np.zeros(max(capacity, 1), np.uint32)
+        rb = np.zeros((max(capacity, 1), 2), np.int32)
         counts = np.zeros(2, np.int64)
         lr = None if learned is None else np.ascontiguousarray(learned, np.float32)
         if lr is not None:
             assert len(lr) <= capacity
         self.L.orc_step_probe(self.h, dt, ptr(o), len(o), iterations, modes, ptr(res),
                               ptr(feats) if capacity else None, ptr(targs) if capacity else None, capacity, ptr(counts),
-                              ptr(lr), ptr(flags) if capacity else None)
+                              ptr(lr), ptr(flags) if capacity else None, ptr(rb) if capacity else None)
         rows = int(counts[0])
         return {"residual": res, "points": int(counts[1]), "features": feats[:rows], "targets": targs[:rows],
-                "flags": flags[:rows]}
+                "flags": flags[:rows], "bodies": rb[:rows]}
 
     def mass(self) -> tuple[np.ndarray, np.ndarray]:
         """(InverseMassVector (n, 3), InverseInertiaWorld (n, 3, 3)) as the solver uses them."""
