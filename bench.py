@@ -316,7 +316,7 @@ def main() -> int:
         print(f"# autotune: solver {tuning['solver']} x {tuning['solve_blocks']} ({tuning['ms_solve_best']:.2f} ms vs "
               f"{tuning['ms_solve_default']:.2f} default), colour blocks {tuning['colour_blocks']} "
               f"({tuning['ms_colour_best']:.2f} ms vs {tuning['ms_colour_default']:.2f}), traversal "
-              f"{'wavefront' if tuning['traversal'] else 'per-thread'} (broadphase {tuning['ms_broadphase_best']:.2f} ms "
+              f"{'wavefront, %d leaves per wave' % tuning['traversal'] if tuning['traversal'] else 'per-thread'} (broadphase {tuning['ms_broadphase_best']:.2f} ms "
               f"vs {tuning['ms_broadphase_default']:.2f})", flush=True)
 
     ref = None

@@ -484,7 +484,8 @@ jh_status jh_world_step_probe(jh_world* world, float dt, jh_probe* probe);
 /* ---- Launch-shape autotuner (no counterpart in the reference) --------------------------------
  * Times candidate workgroup counts of the persistent solver (1/16 .. 2x the CU count, clamped to what
  * can be co-resident), with several worlds also the world-local solver (one workgroup per world), the
- * workgroup count of the colouring kernel (16 .. 512) and the pair traversal (wavefront or per-thread)
+ * workgroup count of the colouring kernel (16 .. 512) and the pair traversal (wavefront at 64, 32 or
+ * 16 leaves per wave, or per-thread)
  * on real steps, round-robin over the candidates for `rounds` rounds each, and keeps the fastest.  The
  * world is stepped rounds * candidates times (the simulation advances); results are deterministic
  * whatever the launch shape. */
@@ -494,7 +495,8 @@ typedef struct jh_tuning {
     int32_t colour_blocks;     /* chosen colouring workgroups          */
     int32_t candidates;
     int32_t steps;             /* steps taken                          */
-    int32_t traversal;         /* chosen pair traversal: 1 wavefront (k_traverse_wave), 0 per thread */
+    int32_t traversal;         /* chosen pair traversal: 0 per thread, else the wavefront traversal
+                                  (k_traverse_wave) with this many leaves per wave (64, 32 or 16) */
     double ms_solve_best, ms_solve_default;    /* mean solver time: chosen / default shape      */
     double ms_colour_best, ms_colour_default;  /* mean colouring time: chosen / default shape   */
     double ms_broadphase_best, ms_broadphase_default;  /* mean broadphase time: chosen / wavefront */

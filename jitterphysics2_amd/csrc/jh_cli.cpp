@@ -101,8 +101,8 @@ int main(int argc, char** argv) {
         printf("# autotune skipped (JH_NO_AUTOTUNE)\n");
     } else {
         if ((s = jh_world_autotune(w, dt, 2, &tune))) return die("jh_world_autotune", s);
-        printf("# autotune: solver %d x %d, colour blocks %d, traversal %s\n", tune.solver, tune.solve_blocks,
-               tune.colour_blocks, tune.traversal ? "wavefront" : "per-thread");
+        printf("# autotune: solver %d x %d, colour blocks %d, traversal %s (%d leaves per wave)\n", tune.solver,
+               tune.solve_blocks, tune.colour_blocks, tune.traversal ? "wavefront" : "per-thread", tune.traversal);
     }
     fflush(stdout);
     double sum[9] = {0};

@@ -700,7 +700,8 @@ __global__ void __launch_bounds__(256) k_traverse_wave(int n, const uint64_t* __
                                                        const float4* __restrict__ lmax,
                                                        const float4* __restrict__ nbox,
                                                        const int4* __restrict__ nrng, uint64_t* __restrict__ pairs,
-                                                       int* __restrict__ pair_count, int cap, int* __restrict__ err) {
+                                                       int* __restrict__ pair_count, int cap, int* __restrict__ err,
+                                                       int qpw) {
     __shared__ uint64_t sbuf[4][kPairBuf];
     __shared__ int scnt[4];
     __shared__ int sstk[4][kWaveStack];
@@ -711,8 +712,11 @@ __global__ void __launch_bounds__(256) k_traverse_wave(int n, const uint64_t* __
     uint64_t* wbuf = sbuf[wv];
     int* wcnt = &scnt[wv];
     int* stk = sstk[wv];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool active = i < n && n >= 2;
+    // qpw consecutive leaves per wave (64, 32 or 16): fewer leaves per wave give shorter walks and more
+    // waves to hide the node loads' latency
+    int lane = threadIdx.x & 63;
+    int i = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * qpw + lane;
+    bool active = lane < qpw && i < n && n >= 2;
     float4 qmn = make_float4(0, 0, 0, 0), qmx = qmn;
     uint32_t si = 0, bi = 0;
     bool awake_i = true;

@@ -189,6 +189,7 @@ struct jh_world {
     DBuf<float4> nbox;           // LBVH child records (k_lbvh_refit, k_traverse_wave): 4 per node
     DBuf<int4> nrng;
     int traverse_wave = 1;       // pair traversal: 1 k_traverse_wave, 0 k_traverse (JH_TRAVERSE=thread)
+    int traverse_qpw = 32;       // k_traverse_wave: leaves per wave (JH_TRAVERSE_QPW=64|32|16)
     DBuf<int2> range;
     DBuf<uint32_t> bounds, hist;
     DBuf<uint64_t> pk0, pk1;
@@ -510,10 +511,14 @@ void traverse_launch(jh_world* w, const uint64_t* keys, const uint32_t* order) {
     int g = grid_for(ns, 256, 1 << 20);
     const uint32_t* sfl = w->islands_on() ? w->sflags.p : nullptr;
     k_set_int<<<1, 1, 0, st>>>(cnt + C_PAIRS, 0);
-    if (w->traverse_wave)
-        k_traverse_wave<<<g, 256, 0, st>>>(ns, keys, w->lmin.p, w->lmax.p, w->nbox.p, w->nrng.p, w->pk0.p,
-                                           cnt + C_PAIRS, w->pair_cap, cnt + C_TERR);
-    else
+    if (w->traverse_wave) {
+        // one thread per leaf at 64 leaves per wave; fewer leaves per wave only while the grid stays
+        // within the launch cap (4 threads per leaf at 16)
+        int q = int64_t(ns) * 4 <= (int64_t(1) << 28) ? w->traverse_qpw : 64;
+        int gw = grid_for(int((int64_t(ns) + q - 1) / q * 64), 256, 1 << 20);
+        k_traverse_wave<<<gw, 256, 0, st>>>(ns, keys, w->lmin.p, w->lmax.p, w->nbox.p, w->nrng.p, w->pk0.p,
+                                            cnt + C_PAIRS, w->pair_cap, cnt + C_TERR, q);
+    } else
         k_traverse<<<g, 256, 0, st>>>(ns, keys, order, w->shapes.p, w->motion.p, w->lmin.p, w->lmax.p, w->left.p,
                                       w->right.p, w->range.p, w->nmin.p, w->nmax.p, w->pk0.p, cnt + C_PAIRS,
                                       w->pair_cap, cnt + C_TERR, sfl);
@@ -534,7 +539,7 @@ std::vector<uint64_t> broadphase_signature(jh_world* w) {
     for (const void* p : ptrs) v.push_back(uint64_t(uintptr_t(p)));
     v.push_back(uint64_t(uint32_t(w->ns)) | uint64_t(uint32_t(w->max_world)) << 32);
     v.push_back(uint64_t(uint32_t(w->pair_cap)) | uint64_t(dtb) << 32);
-    v.push_back(uint64_t(w->traverse_wave) | uint64_t(w->islands_on() ? 1 : 0) << 8);
+    v.push_back(uint64_t(w->traverse_wave) | uint64_t(w->islands_on() ? 1 : 0) << 8 | uint64_t(w->traverse_qpw) << 16);
     return v;
 }
 
@@ -1426,6 +1431,10 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     if (const char* e = getenv("JH_TRAVERSE")) w->traverse_wave = strcmp(e, "thread") != 0;
     if (const char* e = getenv("JH_NARROW_LDS")) w->narrow_lds = atoi(e) != 0;
     if (const char* e = getenv("JH_GRAPH")) w->use_graphs = atoi(e) != 0;
+    if (const char* e = getenv("JH_TRAVERSE_QPW")) {
+        int q = atoi(e);
+        if (q == 16 || q == 32 || q == 64) w->traverse_qpw = q;
+    }
     w->tiles = w->cu_count;
     if (const char* e = getenv("JH_TILES")) w->tiles = std::max(1, std::min(atoi(e), 8 * w->cu_count));
     w->colour_blocks = std::min(64, w->coop_colour);
@@ -2647,27 +2656,34 @@ jh_status jh_world_autotune(jh_world* w, float dt, int32_t rounds, jh_tuning* ou
         for (size_t i = 0; i < cc.size(); ++i)
             if (cc[i] == colour_default) out->ms_colour_default = tot[i] / rounds;
     }
-    // pair traversal: wavefront (default) or one thread per leaf; same pair set either way
-    int tv_default = w->traverse_wave;
+    // pair traversal: wavefront at 32, 64 or 16 leaves per wave, or one thread per leaf; same pair set
+    // either way (candidate 0 is the setting the world had)
     {
-        double tot[2] = {0.0, 0.0};
+        const int tw[4] = {w->traverse_wave, 1, 1, 1 - w->traverse_wave};
+        int q0 = w->traverse_qpw;
+        const int tq[4] = {q0, q0 == 64 ? 32 : 64, q0 == 16 ? 32 : 16, q0};
+        double tot[4] = {0.0, 0.0, 0.0, 0.0};
         for (int r = 0; r < rounds; ++r)
-            for (int m = 0; m < 2; ++m) {
-                w->traverse_wave = 1 - m;
+            for (int m = 0; m < 4; ++m) {
+                w->traverse_wave = tw[m];
+                w->traverse_qpw = tq[m];
                 jh_status st = do_step(w, dt, nullptr);
                 if (st != JH_OK) return st;
                 tot[m] += w->stats.ms_broadphase;
             }
-        int best = tot[1] < tot[0] ? 1 : 0;
-        w->traverse_wave = 1 - best;
+        int best = 0;
+        for (int m = 1; m < 4; ++m)
+            if (tot[m] < tot[best]) best = m;
+        w->traverse_wave = tw[best];
+        w->traverse_qpw = tq[best];
         out->ms_broadphase_best = tot[best] / rounds;
-        out->ms_broadphase_default = tot[tv_default ? 0 : 1] / rounds;
+        out->ms_broadphase_default = tot[0] / rounds;
     }
-    out->traversal = w->traverse_wave;
+    out->traversal = w->traverse_wave ? w->traverse_qpw : 0;
     out->solver = w->kind;
     out->solve_blocks = width(w->kind);
     out->colour_blocks = w->colour_blocks;
-    out->candidates = int(sc.size() + cc.size() + 2);
+    out->candidates = int(sc.size() + cc.size() + 4);
     out->steps = rounds * out->candidates;
     return JH_OK;
 }
