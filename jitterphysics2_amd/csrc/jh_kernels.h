@@ -989,15 +989,66 @@ __host__ __device__ __forceinline__ bool lds_cloud(const ShapeRec& s) {
     return s.type == JH_SHAPE_POINTCLOUD && s.vcnt <= kLdsCloud;
 }
 
+// Pairs grouped by the types of their two shapes, so a wave of the narrowphase runs one pair of
+// support maps instead of every type's branch: class = 8 * type1 + type2; counts per class (LDS
+// histogram per workgroup), an exclusive scan, then each pair's position in its class's range.  The
+// order within a class is arbitrary: every result is written at the pair's own index.
+constexpr int kPairClasses = 64;
+__device__ __forceinline__ int pair_class(const ShapeRec* shapes, uint64_t key) {
+    return (shapes[uint32_t(key >> 32)].type & 7) * 8 + (shapes[uint32_t(key & 0xffffffffu)].type & 7);
+}
+__global__ void __launch_bounds__(256) k_class_count(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
+                                                     const ShapeRec* __restrict__ shapes, int* __restrict__ counts) {
+    __shared__ int h[kPairClasses];
+    if (threadIdx.x < kPairClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    int n = *n_ptr;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) atomicAdd(&h[pair_class(shapes, pairs[p])], 1);
+    __syncthreads();
+    if (threadIdx.x < kPairClasses && h[threadIdx.x] > 0) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
+}
+// One wave: base = exclusive scan of the counts; the counts are cleared for the next step.
+__global__ void __launch_bounds__(64) k_class_scan(int* __restrict__ counts, int* __restrict__ base) {
+    int l = threadIdx.x;
+    int v = counts[l];
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (l >= o) x += y;
+    }
+    base[l] = x - v;
+    counts[l] = 0;
+}
+__global__ void __launch_bounds__(256) k_class_scatter(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
+                                                       const ShapeRec* __restrict__ shapes, int* __restrict__ base,
+                                                       int* __restrict__ perm) {
+    __shared__ int h[kPairClasses], off[kPairClasses];
+    if (threadIdx.x < kPairClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    int n = *n_ptr;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0, r = 0;
+    if (p < n) {
+        c = pair_class(shapes, pairs[p]);
+        r = atomicAdd(&h[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kPairClasses && h[threadIdx.x] > 0) off[threadIdx.x] = atomicAdd(&base[threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+    if (p < n) perm[off[c] + r] = p;
+}
+
 __global__ void __launch_bounds__(128) k_narrow(const uint64_t* __restrict__ pairs, const int* __restrict__ n_ptr,
                                                 const ShapeRec* __restrict__ shapes, const float* __restrict__ verts,
                                                 Bodies B, int aux, int filter, jh_manifold* __restrict__ out,
                                                 DeepRec* __restrict__ deep, int* __restrict__ deep_count,
                                                 int spec_any, float spec_thr_sq, int* __restrict__ spec_queue,
                                                 int* __restrict__ spec_count, int* __restrict__ hull_list,
-                                                int* __restrict__ hull_count) {
+                                                int* __restrict__ hull_count, const int* __restrict__ perm) {
     int n = *n_ptr;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int p = perm ? perm[t] : t;  // pairs grouped by shape types, or in key order
         uint64_t key = pairs[p];
         if (hull_list) {
             uint32_t s1 = uint32_t(key >> 32), s2 = uint32_t(key & 0xffffffffu);

@@ -208,6 +208,8 @@ struct jh_world {
     DBuf<int> hullq;  // pairs with point clouds, narrowphased by k_narrow_hulls
     bool has_clouds = false;  // some shape is a point cloud of at most kLdsCloud vertices
     int narrow_lds = 0;       // stage those clouds in LDS (JH_NARROW_LDS=1; measured slower, off by default)
+    int narrow_sort = 1;      // narrowphase over the pairs grouped by shape types (JH_NARROW_SORT=0: key order)
+    DBuf<int> pperm, pcls;    // that grouping: pair permutation; class counts and bases
     DBuf<uint64_t> zkey, evb, eve;  // keys of arbiters removed by the last step; collision events
     // constraints (solved by the barrier solver after the contacts of each pass)
     std::vector<jh_constraint> host_con;
@@ -1020,11 +1022,24 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
         float thr = cfg.speculative_velocity_threshold;
         bool hulls = w->has_clouds && w->narrow_lds;
         if (hulls) HIPCHK(w->hullq.ensure(size_t(np)));
+        const int* perm = nullptr;
+        if (w->narrow_sort) {  // pairs grouped by their shapes' types (one launch per pair over np)
+            HIPCHK(w->pperm.ensure(size_t(np)));
+            if (!w->pcls.p) {
+                HIPCHK(w->pcls.ensure(2 * kPairClasses));
+                HIPCHK(hipMemsetAsync(w->pcls.p, 0, 2 * kPairClasses * sizeof(int), st));
+            }
+            int gp = (np + 255) / 256;
+            k_class_count<<<gp, 256, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->pcls.p);
+            k_class_scan<<<1, 64, 0, st>>>(w->pcls.p, w->pcls.p + kPairClasses);
+            k_class_scatter<<<gp, 256, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->pcls.p + kPairClasses, w->pperm.p);
+            perm = w->pperm.p;
+        }
         k_narrow<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, cnt + C_PAIRS, w->shapes.p, w->verts.p, B, aux,
                                                              cfg.narrowphase_filter, w->man.p, w->deep.p,
                                                              cnt + C_DEEP, spec, thr * thr, w->specq.p,
                                                              cnt + C_SPEC, hulls ? w->hullq.p : nullptr,
-                                                             cnt + C_HULL);
+                                                             cnt + C_HULL, perm);
         if (hulls)  // the pairs with point clouds, their vertices staged in LDS
             k_narrow_hulls<<<grid_for(np, 64, 1 << 20), 64, 0, st>>>(pairs, w->hullq.p, cnt + C_HULL, w->shapes.p,
                                                                       w->verts.p, B, aux, cfg.narrowphase_filter,
@@ -1430,6 +1445,7 @@ jh_status jh_world_create(const jh_config* config, jh_world** out) {
     if (const char* e = getenv("JH_FLOW_QUEUE")) w->flow_qmode = std::max(0, std::min(atoi(e), 2));
     if (const char* e = getenv("JH_TRAVERSE")) w->traverse_wave = strcmp(e, "thread") != 0;
     if (const char* e = getenv("JH_NARROW_LDS")) w->narrow_lds = atoi(e) != 0;
+    if (const char* e = getenv("JH_NARROW_SORT")) w->narrow_sort = atoi(e) != 0;
     if (const char* e = getenv("JH_GRAPH")) w->use_graphs = atoi(e) != 0;
     if (const char* e = getenv("JH_TRAVERSE_QPW")) {
         int q = atoi(e);
