@@ -1157,14 +1157,17 @@ struct EpaRes {
 __global__ void __launch_bounds__(64, 6) k_epa_wave(const uint64_t* __restrict__ pairs, const ShapeRec* __restrict__ shapes,
                                                  const float* __restrict__ verts, Bodies B,
                                                  const DeepRec* __restrict__ deep, const int* __restrict__ deep_count,
-                                                 char* __restrict__ side, EpaRes* __restrict__ res) {
+                                                 char* __restrict__ side, EpaRes* __restrict__ res,
+                                                 int* __restrict__ next) {
     __shared__ epw::Lds L;
     int lane = threadIdx.x;
     epw::Poly w;
     w.L = &L;
     w.bind(side + size_t(blockIdx.x) * epw::kSideBytes);
     int n = *deep_count;
-    for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    // the first pair of each wave by its index, then the next unclaimed pair from a counter: pairs take
+    // 1 to 85 iterations, so a fixed round-robin leaves a long tail behind the unlucky waves
+    for (int q = blockIdx.x; q < n;) {
         const DeepRec& d = deep[q];
         uint64_t key = pairs[d.pair];
         MinkowskiPair mk;
@@ -1176,6 +1179,9 @@ __global__ void __launch_bounds__(64, 6) k_epa_wave(const uint64_t* __restrict__
         r.ok = epw::solve(w, mk, lane, r.pa, r.pb, r.normal, r.pen) ? 1 : 0;
         if (lane == 0) res[q] = r;
         epw::wave_sync();
+        int t = 0;
+        if (lane == 0) t = atomicAdd(next, 1);
+        q = __builtin_amdgcn_readfirstlane(t) + int(gridDim.x);
     }
 }
 

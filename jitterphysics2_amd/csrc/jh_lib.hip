@@ -84,7 +84,7 @@ enum {
     C_NS = 10,   // shape count
     C_NB = 11,   // body count
     C_CONTACTS = 12,  // contact points solved
-    // 13: unused
+    C_EPAQ = 13,      // k_epa_wave: next deep pair to hand out
     C_HULL = 14,      // pairs left to k_narrow_hulls
     C_SPEC = 15,      // speculative sweep queue
     C_NMOV = 16,      // tile solver: non-static bodies with shapes
@@ -316,7 +316,7 @@ namespace {
 __global__ void k_reset_counters(int* cnt, uint32_t* bounds) {
     int t = threadIdx.x;
     if (t == C_PAIRS || t == C_DEEP || t == C_TERR || t == C_COLL || t == C_NEW || t == C_CONTACTS || t == C_HULL ||
-        t == C_MAXDEG || t == C_SPEC || t == C_EVB || t == C_EVE)
+        t == C_MAXDEG || t == C_SPEC || t == C_EVB || t == C_EVE || t == C_EPAQ)
         cnt[t] = 0;
     if (t < 3) bounds[t] = 0xffffffffu;
     else if (t < 6) bounds[t] = 0u;
@@ -1055,7 +1055,7 @@ jh_status do_step(jh_world* w, float dt, jh_probe* probe) {
             HIPCHK(w->epa_side.ensure(size_t(w->epa_wave_blocks) * epw::kSideBytes));
             HIPCHK(w->epa_res.ensure(std::max(np, 1)));
             k_epa_wave<<<blocks, 64, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, w->deep.p, cnt + C_DEEP,
-                                              w->epa_side.p, w->epa_res.p);
+                                              w->epa_side.p, w->epa_res.p, cnt + C_EPAQ);
             k_epa_finish<<<grid_for(np, 128, 1 << 20), 128, 0, st>>>(pairs, w->shapes.p, w->verts.p, B, aux, w->man.p,
                                                                   w->deep.p, cnt + C_DEEP, w->epa_res.p);
         } else {
